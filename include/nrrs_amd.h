@@ -1,0 +1,210 @@
+/*
+ * nrrs_amd.h - C ABI of the MI355X (gfx950) neural Russian-roulette-and-splitting stage.
+ *
+ * One context owns everything the reference's NRRSNetwork owns
+ * (src/render/nrrs/net/network.h:22-214): the StatNet and RRSNet parameters
+ * (hash grid + MLP, FP32 master copies and FP16/BF16 shadows), their Adam state,
+ * and the scratch buffers of the RRS decision. The caller owns the queues and the
+ * film and passes device pointers, as the integrator does
+ * (src/render/nrrs/integrator.cpp:1057-1483).
+ *
+ * Conventions
+ *  - Every function returns an nrrs_status (0 = ok) and never throws.
+ *  - Pointers named *_dev are device pointers; everything else is host memory.
+ *  - Work is enqueued on the hipStream_t passed as `stream` (NULL = default stream)
+ *    and is asynchronous unless the comment says otherwise.
+ *  - Per-vertex arrays are component-major: field c of vertex i is at [c * stride + i].
+ */
+#ifndef NRRS_AMD_H
+#define NRRS_AMD_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef enum {
+    NRRS_OK = 0,
+    NRRS_ERR_ARG = 1,         /* null pointer / invalid value */
+    NRRS_ERR_SHAPE = 2,       /* size beyond the capacity fixed at create time */
+    NRRS_ERR_OOM = 3,         /* device allocation failed */
+    NRRS_ERR_OVERFLOW = 4,    /* survivor queue capacity exceeded (queue truncated) */
+    NRRS_ERR_NONFINITE = 5,   /* NaN / Inf met where a finite value is required */
+    NRRS_ERR_HIP = 6,         /* HIP runtime error */
+    NRRS_ERR_UNSUPPORTED = 7, /* configuration this build does not implement */
+    NRRS_ERR_NO_DEVICE = 8    /* no usable gfx950 device */
+} nrrs_status;
+
+typedef enum { NRRS_FP32 = 0, NRRS_FP16 = 1, NRRS_BF16 = 2 } nrrs_precision;
+
+/* Opaque context: one per device, used from one host thread at a time. */
+typedef struct nrrs_ctx nrrs_ctx;
+
+/* Network ids, as NRRSNetwork::LL2_NET / RRS_NET (src/render/nrrs/net/network.h:211-212). */
+typedef enum { NRRS_STATNET = 0, NRRS_RRSNET = 1 } nrrs_net;
+
+/* Where the RRS factor q comes from (src/render/nrrs/integrator.cpp:1184-1256). */
+typedef enum {
+    NRRS_Q_RRSNET = 0,         /* q = activationSigma(RRSNet(encode(v)))                 */
+    NRRS_Q_RRSNET_CHAINED = 1, /* StatNet stats feed the RRSNet input, then as above     */
+    NRRS_Q_STATNET_ADRRS = 2   /* q = mean(clamp(L,1e-4,1e4) * thp / (I_px + 0.01))      */
+} nrrs_q_mode;
+
+/* Multiresolution hash grid (tcnn "HashGrid", 2net-simple.json:46-51). */
+typedef struct {
+    uint32_t n_levels;        /* 1..16                                   */
+    uint32_t n_features;      /* must be 2                               */
+    uint32_t log2_table_size; /* T = 2^log2_table_size entries per level  */
+    uint32_t base_resolution;
+    float per_level_scale;
+} nrrs_grid_config;
+
+typedef struct {
+    uint32_t n_hidden_layers; /* hidden layers of width 64 (1..6)      */
+    uint32_t n_outputs;       /* StatNet 6, RRSNet 1                      */
+} nrrs_mlp_config;
+
+typedef struct {
+    nrrs_grid_config grid;   /* shared by both nets (each net has its own table) */
+    nrrs_mlp_config statnet; /* 4 hidden, 6 outputs                              */
+    nrrs_mlp_config rrsnet;  /* 3 hidden, 1 output                               */
+    int32_t precision;       /* nrrs_precision of the MLP contractions           */
+
+    /* RRS decision: integrator.h:265, render/test.json:63-67 */
+    float rrs_clamp_min;        /* 0.05 */
+    float rrs_clamp_max;        /* 20   */
+    float rrs_normalize_rate;   /* 0.85, used when !rrs_use_ceil */
+    int32_t rrs_use_ceil;       /* 1 */
+    int32_t rrs_no_upscale;     /* 1 = donnot_normalize_when_less_than_one */
+    int32_t rrs_weight_window;  /* 0 = off (RRSWeightWindow, commoncudautils.h:14-28) */
+    float weight_window_size;   /* 5 */
+    uint32_t split_cap;         /* largest split count, 8 */
+
+    /* Losses (2net-simple.json:3-4, 145-148) */
+    int32_t loss_clamp_on;  /* 1  */
+    float loss_clamp_max;   /* 10 */
+    int32_t train_sigma;    /* must be 0 (channels 3-5 model E[L^2]) */
+    uint32_t ll2_step;      /* 0: L only, >=1: L and L^2 */
+    uint32_t rrs_step;      /* 3: on-policy RRSNet loss; <3: RRSNet gradient is zero */
+    float gamma[4];         /* 0.01, 0.05, 0.0, 0.01 */
+
+    /* Adam (2net-simple.json:16-28), FP32 master weights */
+    float lr_statnet, lr_rrsnet;
+    float adam_beta1, adam_beta2, adam_eps, adam_l2_reg;
+    float loss_scale_init;          /* dynamic loss scaling start (FP16/BF16), 2^15 */
+    uint32_t loss_scale_growth_interval;
+
+    /* Scene and frame */
+    float aabb_min[3], aabb_max[3];
+    uint32_t n_pixels;   /* pixels of the film (resolution) */
+    uint32_t max_queue;  /* capacity of one inference batch (vertices) */
+    uint32_t max_train;  /* capacity of one training batch */
+    uint64_t seed;       /* parameter initialisation seed */
+} nrrs_config;
+
+/* SoA path vertices of one bounce (replaces NRRSInferenceWorkItem, workitem.h:9-15). */
+typedef struct {
+    const float* pos;       /* [3][stride] world-space position          */
+    const float* dir;       /* [3][stride] unit direction                 */
+    const float* normal;    /* [3][stride] unit shading normal            */
+    const float* rough;     /* [stride] roughness                         */
+    const float* albedo_d;  /* [3][stride]                                */
+    const float* albedo_s;  /* [3][stride]                                */
+    const float* thp;       /* [3][stride] throughput before RRS          */
+    const uint32_t* pixel;  /* [stride] film pixel of the path            */
+    const uint32_t* stid;   /* [stride] scatter-queue index; NULL = i     */
+    uint32_t stride;
+} nrrs_vertices;
+
+/* Arguments of the per-bounce RRS stage. */
+typedef struct {
+    uint32_t n;                   /* inference queue size                               */
+    nrrs_vertices v;              /* device SoA                                         */
+    const float* film_dev;        /* [3][n_pixels] per-pixel estimate I (RGB)           */
+    int32_t q_mode;               /* nrrs_q_mode                                        */
+    const uint32_t* spec_stid_dev;/* [n_spec] specular stids, already in the queue       */
+    const uint32_t* n_spec_dev;   /* device count of specular stids (may be NULL = 0)    */
+    uint64_t pcg_seed;            /* PCG32 (seed, stream); vertex i draws output number i */
+    uint64_t pcg_stream;
+    /* outputs (device) */
+    float* q_dev;                 /* [n] clamped q before normalisation (mRRSArray)       */
+    float* qn_dev;                /* [n] normalised q' (mScatterRayQueue->mRRS), or NULL  */
+    uint8_t* split_dev;           /* [n] split count s, or NULL                           */
+    uint32_t* survivors_dev;      /* [capacity] compacted survivor stids                  */
+    uint32_t capacity;
+    uint32_t* n_survivors_dev;    /* device: number of survivors (incl. specular)         */
+    uint32_t* overflow_dev;       /* device: survivors dropped for lack of capacity, or NULL */
+} nrrs_decide_args;
+
+/* ---------------- configuration and lifetime ---------------- */
+
+/* Fills the synthetic-benchmark defaults (16-level grid, 64-wide MLPs, render/test.json RRS knobs). */
+nrrs_status nrrs_default_config(nrrs_config* cfg);
+
+/* sizeof of the ABI structs, so a binding can check its own layout (no reference counterpart). */
+size_t nrrs_struct_size(int32_t which); /* 0 config, 1 vertices, 2 decide_args, 3 vertices_out */
+
+/* Replaces NRRSNetwork(json&) + resize/reset (network.cpp:28-35, 351-576). Allocates every
+ * device buffer and initialises parameters (grid U[-1e-4,1e-4], MLP Xavier-uniform) from cfg->seed.
+ * Synchronous. */
+nrrs_status nrrs_create(const nrrs_config* cfg, nrrs_ctx** out);
+nrrs_status nrrs_destroy(nrrs_ctx* ctx);
+const char* nrrs_status_string(nrrs_status s);
+
+/* Hash-grid level table: res/offset/size in entries, dense flag, FP32 scale per level.
+ * Host-only. Returns the total number of grid parameters in *n_params. */
+nrrs_status nrrs_grid_layout(const nrrs_grid_config* g, uint32_t* res, uint32_t* offset,
+                             uint32_t* size, uint32_t* dense, float* scale, uint64_t* n_params);
+
+/* ---------------- parameters (fixtures / finite differences) ---------------- */
+
+/* Number of FP32 parameters of one net: grid entries * 2, then MLP weights. */
+nrrs_status nrrs_param_count(nrrs_ctx* ctx, int32_t net, uint64_t* n_grid, uint64_t* n_mlp);
+/* Host copy of [grid | W0 | W1 | ... ] (row-major W_l[out][in]). Synchronous. */
+nrrs_status nrrs_params_get(nrrs_ctx* ctx, int32_t net, float* host_dst);
+/* Upload, refresh low-precision shadows. Synchronous. */
+nrrs_status nrrs_params_set(nrrs_ctx* ctx, int32_t net, const float* host_src);
+
+/* ---------------- inference (network.cpp:297-349) ---------------- */
+
+/* Encoded network input [n][80] of `net` (debug / parity). stats_dev: [6][n] StatNet outputs
+ * feeding a chained RRSNet, or NULL. */
+nrrs_status nrrs_encode(nrrs_ctx* ctx, int32_t net, uint32_t n, const nrrs_vertices* v,
+                        const float* film_dev, const float* stats_dev, float* enc_dev, void* stream);
+
+/* Hash-grid table indices [n][levels][8] (corner bit0=x, bit1=y, bit2=z), within the level. */
+nrrs_status nrrs_grid_indices(nrrs_ctx* ctx, uint32_t n, const nrrs_vertices* v,
+                              uint32_t* idx_dev, void* stream);
+
+/* StatNet inference: stats_dev [6][n] = (L_rgb, L2_rgb). */
+nrrs_status nrrs_statnet_infer(nrrs_ctx* ctx, uint32_t n, const nrrs_vertices* v,
+                               float* stats_dev, void* stream);
+
+/* RRSNet inference: raw_dev [n] = network output before activationSigma. */
+nrrs_status nrrs_rrsnet_infer(nrrs_ctx* ctx, uint32_t n, const nrrs_vertices* v,
+                              const float* film_dev, const float* stats_dev, float* raw_dev,
+                              void* stream);
+
+/* ---------------- the RRS stage (integrator.cpp:1057-1483) ---------------- */
+
+/* Inference + q + clamp/ceil + exact global sum + normalise + split + stable compaction. */
+nrrs_status nrrs_rrs_infer_decide(nrrs_ctx* ctx, const nrrs_decide_args* a, void* stream);
+
+/* Decision only, from a given clamped q (a->q_dev is an input here). */
+nrrs_status nrrs_rrs_decide(nrrs_ctx* ctx, const nrrs_decide_args* a, void* stream);
+
+/* Synthetic vertices of the benchmark workloads, generated on the device from PCG32
+ * (seed, field stream). out: buffers of a nrrs_vertices with stride >= n (written). */
+typedef struct {
+    float* pos; float* dir; float* normal; float* rough; float* albedo_d; float* albedo_s;
+    float* thp; uint32_t* pixel; uint32_t stride;
+} nrrs_vertices_out;
+nrrs_status nrrs_synth_vertices(uint32_t n, uint64_t seed, uint32_t n_pixels,
+                                const nrrs_vertices_out* out, void* stream);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* NRRS_AMD_H */

@@ -1,0 +1,453 @@
+// C ABI of the neural RRS stage: context lifetime, parameters, inference and the
+// per-bounce RRS decision. Mirrors the integrator-facing surface of NRRSNetwork
+// (src/render/nrrs/net/network.h:22-126) with explicit device pointers instead of
+// json-carried raw addresses, and status codes instead of Log(Fatal).
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstring>
+#include <new>
+#include <vector>
+
+#include "host_common.hpp"
+
+using namespace nrrs;
+
+namespace {
+
+nrrs_status hip_status(hipError_t e) {
+    if (e == hipSuccess) return NRRS_OK;
+    if (e == hipErrorOutOfMemory) return NRRS_ERR_OOM;
+    return NRRS_ERR_HIP;
+}
+
+#define NRRS_TRY(expr)                         \
+    do {                                       \
+        nrrs_status st_ = (expr);              \
+        if (st_ != NRRS_OK) return st_;        \
+    } while (0)
+#define NRRS_HIP(expr) NRRS_TRY(hip_status(expr))
+
+struct Pcg32Host {
+    uint64_t state, inc;
+    Pcg32Host(uint64_t seed, uint64_t stream) {
+        inc = (stream << 1u) | 1u;
+        state = 0;
+        next();
+        state += seed;
+        next();
+    }
+    uint32_t next() {
+        uint64_t old = state;
+        state = old * kPcgMult + inc;
+        uint32_t xs = (uint32_t)(((old >> 18u) ^ old) >> 27u);
+        uint32_t rot = (uint32_t)(old >> 59u);
+        return (xs >> rot) | (xs << ((32u - rot) & 31u));
+    }
+    double uniform() { return (double)(next() >> 8) * 0x1p-24; }
+};
+
+// Level table of the hash grid. scale_l = base * per_level_scale^l (rounded to
+// FP32 once, here); res_l = ceil(scale_l) + 2 so that every corner of a cell of
+// a position in [0,1] is inside the level; a level is dense when res^3 <= T.
+// Level starts are padded to multiples of 8 entries.
+bool make_grid(const nrrs_grid_config& g, GridParams* gp, uint32_t* res, uint32_t* offset,
+               uint32_t* size, uint32_t* dense, float* scale, uint64_t* n_params) {
+    if (g.n_levels < 1 || g.n_levels > (uint32_t)kMaxLevels || g.n_features != 2 ||
+        g.log2_table_size < 4 || g.log2_table_size > 24 || g.base_resolution < 1 ||
+        !(g.per_level_scale >= 1.0f))
+        return false;
+    const uint64_t T = 1ull << g.log2_table_size;
+    uint64_t off = 0;
+    GridParams p{};
+    for (uint32_t l = 0; l < g.n_levels; ++l) {
+        double s = (double)g.base_resolution * std::pow((double)g.per_level_scale, (double)l);
+        float sf = (float)s;
+        if (!(sf < 1.0e6f)) return false;
+        uint32_t r = (uint32_t)std::ceil((double)sf) + 2u;
+        uint64_t r3 = (uint64_t)r * r * r;
+        uint32_t d = r3 <= T ? 1u : 0u;
+        uint32_t sz = (uint32_t)(d ? r3 : T);
+        p.scale[l] = sf;
+        p.res[l] = r;
+        p.offset[l] = (uint32_t)off;
+        if (d) p.dense_mask |= 1u << l;
+        if (res) res[l] = r;
+        if (offset) offset[l] = (uint32_t)off;
+        if (size) size[l] = sz;
+        if (dense) dense[l] = d;
+        if (scale) scale[l] = sf;
+        off += (sz + 7ull) / 8ull * 8ull;
+    }
+    if (off > (1ull << 31)) return false;
+    p.n_levels = g.n_levels;
+    p.hash_mask = (uint32_t)(T - 1);
+    p.n_entries = (uint32_t)off;
+    if (gp) *gp = p;
+    if (n_params) *n_params = off * 2;
+    return true;
+}
+
+// normalizeSpatialCoord frame (src/render/nrc/nrctrain.h:111-115), computed once
+// in FP32 with no contraction (the library is built with -ffp-contract=off on the host).
+Frame make_frame(const float* mn, const float* mx) {
+    Frame f;
+    float dx = mx[0] - mn[0], dy = mx[1] - mn[1], dz = mx[2] - mn[2];
+    float len = std::sqrt(dx * dx + dy * dy + dz * dz);
+    float infl = len * 0.005f;
+    for (int k = 0; k < 3; ++k) {
+        float lo = mn[k] - infl, hi = mx[k] + infl;
+        f.c[k] = (lo + hi) * 0.5f;
+        f.d[k] = hi - lo;
+    }
+    return f;
+}
+
+uint64_t mlp_count(const nrrs_mlp_config& m) {
+    return (uint64_t)kHidden * kEnc + (uint64_t)(m.n_hidden_layers - 1) * kHidden * kHidden +
+           (uint64_t)m.n_outputs * kHidden;
+}
+
+bool valid_shapes(const nrrs_config& c) {
+    // Shapes the fused kernels are compiled for.
+    return c.statnet.n_hidden_layers == 4 && c.statnet.n_outputs == 6 &&
+           c.rrsnet.n_hidden_layers == 3 && c.rrsnet.n_outputs == 1;
+}
+
+template <typename T>
+nrrs_status dmalloc(T** p, size_t count) {
+    *p = nullptr;
+    if (!count) return NRRS_OK;
+    return hip_status(hipMalloc(reinterpret_cast<void**>(p), count * sizeof(T)));
+}
+
+// Parameters of one net: grid U[-1e-4, 1e-4] (the tcnn grid initialisation), then
+// every MLP layer Xavier-uniform, from PCG32(seed, 100 + net).
+void init_params(const nrrs_config& c, int net, uint64_t n_grid, std::vector<float>& out) {
+    const nrrs_mlp_config& m = net == NRRS_STATNET ? c.statnet : c.rrsnet;
+    out.resize(n_grid + mlp_count(m));
+    Pcg32Host r(c.seed, 100 + (uint64_t)net);
+    for (uint64_t k = 0; k < n_grid; ++k) out[k] = (float)(1e-4 * (2.0 * r.uniform() - 1.0));
+    size_t pos = n_grid;
+    int in = kEnc;
+    const int nl = (int)m.n_hidden_layers + 1;
+    for (int layer = 0; layer < nl; ++layer) {
+        int o = layer == nl - 1 ? (int)m.n_outputs : kHidden;
+        double lim = std::sqrt(6.0 / (double)(in + o));
+        for (int k = 0; k < o * in; ++k) out[pos++] = (float)(lim * (2.0 * r.uniform() - 1.0));
+        in = o;
+    }
+}
+
+EncodeArgs encode_args(nrrs_ctx* ctx, const nrrs_vertices& v, const float* film, const float* stats,
+                       uint32_t stats_stride) {
+    EncodeArgs a;
+    a.v = v;
+    a.film = film;
+    a.n_pixels = ctx->cfg.n_pixels;
+    a.stats = stats;
+    a.stats_stride = stats_stride;
+    a.grid = ctx->grid;
+    a.frame = ctx->frame;
+    return a;
+}
+
+bool vertices_ok(const nrrs_vertices* v, uint32_t n) {
+    return v && v->pos && v->dir && v->normal && v->rough && v->albedo_d && v->albedo_s && v->thp &&
+           v->pixel && v->stride >= n;
+}
+
+void free_ctx(nrrs_ctx* c) {
+    for (auto& ns : c->net) {
+        hipFree(ns.grid_f32);
+        hipFree(ns.mlp_f32);
+        hipFree(ns.grid_lp);
+        hipFree(ns.mlp_lp);
+    }
+    hipFree(c->sum_dev);
+    hipFree(c->tile_sums);
+    hipFree(c->tile_offsets);
+    hipFree(c->stats_scratch);
+    hipFree(c->q_scratch);
+    hipFree(c->s_scratch);
+    delete c;
+}
+
+}  // namespace
+
+extern "C" {
+
+const char* nrrs_status_string(nrrs_status s) {
+    switch (s) {
+        case NRRS_OK: return "ok";
+        case NRRS_ERR_ARG: return "invalid argument";
+        case NRRS_ERR_SHAPE: return "size exceeds the capacity of the context";
+        case NRRS_ERR_OOM: return "device out of memory";
+        case NRRS_ERR_OVERFLOW: return "survivor queue overflow";
+        case NRRS_ERR_NONFINITE: return "non-finite value";
+        case NRRS_ERR_HIP: return "HIP runtime error";
+        case NRRS_ERR_UNSUPPORTED: return "unsupported configuration";
+        case NRRS_ERR_NO_DEVICE: return "no gfx950 device";
+    }
+    return "unknown status";
+}
+
+size_t nrrs_struct_size(int32_t which) {
+    switch (which) {
+        case 0: return sizeof(nrrs_config);
+        case 1: return sizeof(nrrs_vertices);
+        case 2: return sizeof(nrrs_decide_args);
+        case 3: return sizeof(nrrs_vertices_out);
+    }
+    return 0;
+}
+
+nrrs_status nrrs_default_config(nrrs_config* c) {
+    if (!c) return NRRS_ERR_ARG;
+    std::memset(c, 0, sizeof(*c));
+    c->grid.n_levels = 16;
+    c->grid.n_features = 2;
+    c->grid.log2_table_size = 19;
+    c->grid.base_resolution = 16;
+    c->grid.per_level_scale = 1.5f;
+    c->statnet.n_hidden_layers = 4;
+    c->statnet.n_outputs = 6;
+    c->rrsnet.n_hidden_layers = 3;
+    c->rrsnet.n_outputs = 1;
+    c->precision = NRRS_FP32;
+    c->rrs_clamp_min = 0.05f;
+    c->rrs_clamp_max = 20.0f;
+    c->rrs_normalize_rate = 0.85f;
+    c->rrs_use_ceil = 1;
+    c->rrs_no_upscale = 1;
+    c->rrs_weight_window = 0;
+    c->weight_window_size = 5.0f;
+    c->split_cap = 8;
+    c->loss_clamp_on = 1;
+    c->loss_clamp_max = 10.0f;
+    c->train_sigma = 0;
+    c->ll2_step = 1;
+    c->rrs_step = 3;
+    c->gamma[0] = 0.01f;
+    c->gamma[1] = 0.05f;
+    c->gamma[2] = 0.0f;
+    c->gamma[3] = 0.01f;
+    c->lr_statnet = 1e-2f;
+    c->lr_rrsnet = 1e-2f;
+    c->adam_beta1 = 0.9f;
+    c->adam_beta2 = 0.99f;
+    c->adam_eps = 1e-15f;
+    c->adam_l2_reg = 0.0f;
+    c->loss_scale_init = 32768.0f;
+    c->loss_scale_growth_interval = 1000;
+    for (int k = 0; k < 3; ++k) {
+        c->aabb_min[k] = 0.0f;
+        c->aabb_max[k] = 1.0f;
+    }
+    c->n_pixels = 128 * 128;
+    c->max_queue = 1u << 14;
+    c->max_train = 1u << 14;
+    c->seed = 0x4E525253ull;
+    return NRRS_OK;
+}
+
+nrrs_status nrrs_grid_layout(const nrrs_grid_config* g, uint32_t* res, uint32_t* offset,
+                             uint32_t* size, uint32_t* dense, float* scale, uint64_t* n_params) {
+    if (!g) return NRRS_ERR_ARG;
+    return make_grid(*g, nullptr, res, offset, size, dense, scale, n_params) ? NRRS_OK : NRRS_ERR_ARG;
+}
+
+nrrs_status nrrs_create(const nrrs_config* cfg, nrrs_ctx** out) {
+    if (!cfg || !out) return NRRS_ERR_ARG;
+    *out = nullptr;
+    if (!valid_shapes(*cfg) || cfg->train_sigma) return NRRS_ERR_UNSUPPORTED;
+    if (cfg->precision != NRRS_FP32) return NRRS_ERR_UNSUPPORTED;
+    if (cfg->max_queue == 0 || cfg->n_pixels == 0 || cfg->split_cap == 0 || cfg->split_cap > 255 ||
+        !(cfg->rrs_clamp_min > 0.0f) || !(cfg->rrs_clamp_max >= cfg->rrs_clamp_min))
+        return NRRS_ERR_ARG;
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess) return NRRS_ERR_NO_DEVICE;
+    hipDeviceProp_t prop;
+    if (hipGetDeviceProperties(&prop, dev) != hipSuccess) return NRRS_ERR_NO_DEVICE;
+    if (std::strncmp(prop.gcnArchName, "gfx950", 6) != 0) return NRRS_ERR_NO_DEVICE;
+
+    nrrs_ctx* c = new (std::nothrow) nrrs_ctx();
+    if (!c) return NRRS_ERR_OOM;
+    c->cfg = *cfg;
+    c->device = dev;
+    uint64_t n_grid = 0;
+    if (!make_grid(cfg->grid, &c->grid, nullptr, nullptr, nullptr, nullptr, nullptr, &n_grid)) {
+        delete c;
+        return NRRS_ERR_ARG;
+    }
+    c->frame = make_frame(cfg->aabb_min, cfg->aabb_max);
+    c->dec.clamp_min = cfg->rrs_clamp_min;
+    c->dec.clamp_max = cfg->rrs_clamp_max;
+    c->dec.rate = cfg->rrs_normalize_rate;
+    c->dec.window_size = cfg->weight_window_size;
+    c->dec.use_ceil = cfg->rrs_use_ceil;
+    c->dec.no_upscale = cfg->rrs_no_upscale;
+    c->dec.weight_window = cfg->rrs_weight_window;
+    c->dec.split_cap = cfg->split_cap;
+    c->dec.n_pixels = cfg->n_pixels;
+
+    nrrs_status st = NRRS_OK;
+    std::vector<float> host;
+    for (int net = 0; net < 2 && st == NRRS_OK; ++net) {
+        const nrrs_mlp_config& m = net == NRRS_STATNET ? cfg->statnet : cfg->rrsnet;
+        NetState& ns = c->net[net];
+        ns.n_layers = (int)m.n_hidden_layers + 1;
+        ns.n_out = (int)m.n_outputs;
+        ns.n_grid = n_grid;
+        ns.n_mlp = mlp_count(m);
+        if ((st = dmalloc(&ns.grid_f32, ns.n_grid)) != NRRS_OK) break;
+        if ((st = dmalloc(&ns.mlp_f32, ns.n_mlp)) != NRRS_OK) break;
+        init_params(*cfg, net, n_grid, host);
+        st = hip_status(hipMemcpy(ns.grid_f32, host.data(), ns.n_grid * 4, hipMemcpyHostToDevice));
+        if (st == NRRS_OK)
+            st = hip_status(hipMemcpy(ns.mlp_f32, host.data() + n_grid, ns.n_mlp * 4, hipMemcpyHostToDevice));
+    }
+    c->max_tiles = (cfg->max_queue + 255) / 256;
+    if (st == NRRS_OK) st = dmalloc(&c->sum_dev, 1);
+    if (st == NRRS_OK) st = dmalloc(&c->tile_sums, c->max_tiles);
+    if (st == NRRS_OK) st = dmalloc(&c->tile_offsets, c->max_tiles);
+    if (st == NRRS_OK) st = dmalloc(&c->stats_scratch, (size_t)6 * cfg->max_queue);
+    if (st == NRRS_OK) st = dmalloc(&c->q_scratch, cfg->max_queue);
+    if (st == NRRS_OK) st = dmalloc(&c->s_scratch, cfg->max_queue);
+    if (st != NRRS_OK) {
+        free_ctx(c);
+        return st;
+    }
+    *out = c;
+    return NRRS_OK;
+}
+
+nrrs_status nrrs_destroy(nrrs_ctx* ctx) {
+    if (!ctx) return NRRS_ERR_ARG;
+    hipDeviceSynchronize();
+    free_ctx(ctx);
+    return NRRS_OK;
+}
+
+nrrs_status nrrs_param_count(nrrs_ctx* ctx, int32_t net, uint64_t* n_grid, uint64_t* n_mlp) {
+    if (!ctx || (net != 0 && net != 1)) return NRRS_ERR_ARG;
+    if (n_grid) *n_grid = ctx->net[net].n_grid;
+    if (n_mlp) *n_mlp = ctx->net[net].n_mlp;
+    return NRRS_OK;
+}
+
+nrrs_status nrrs_params_get(nrrs_ctx* ctx, int32_t net, float* host_dst) {
+    if (!ctx || !host_dst || (net != 0 && net != 1)) return NRRS_ERR_ARG;
+    const NetState& ns = ctx->net[net];
+    NRRS_HIP(hipDeviceSynchronize());
+    NRRS_HIP(hipMemcpy(host_dst, ns.grid_f32, ns.n_grid * 4, hipMemcpyDeviceToHost));
+    NRRS_HIP(hipMemcpy(host_dst + ns.n_grid, ns.mlp_f32, ns.n_mlp * 4, hipMemcpyDeviceToHost));
+    return NRRS_OK;
+}
+
+nrrs_status nrrs_params_set(nrrs_ctx* ctx, int32_t net, const float* host_src) {
+    if (!ctx || !host_src || (net != 0 && net != 1)) return NRRS_ERR_ARG;
+    NetState& ns = ctx->net[net];
+    NRRS_HIP(hipDeviceSynchronize());
+    NRRS_HIP(hipMemcpy(ns.grid_f32, host_src, ns.n_grid * 4, hipMemcpyHostToDevice));
+    NRRS_HIP(hipMemcpy(ns.mlp_f32, host_src + ns.n_grid, ns.n_mlp * 4, hipMemcpyHostToDevice));
+    return NRRS_OK;
+}
+
+nrrs_status nrrs_encode(nrrs_ctx* ctx, int32_t net, uint32_t n, const nrrs_vertices* v,
+                        const float* film_dev, const float* stats_dev, float* enc_dev, void* stream) {
+    if (!ctx || (net != 0 && net != 1) || !vertices_ok(v, n) || !enc_dev) return NRRS_ERR_ARG;
+    if (net == NRRS_RRSNET && !film_dev) return NRRS_ERR_ARG;
+    EncodeArgs a = encode_args(ctx, *v, film_dev, net == NRRS_RRSNET ? stats_dev : nullptr, n);
+    return hip_status(launch_encode(a, net, n, ctx->net[net].grid_f32, enc_dev, (hipStream_t)stream));
+}
+
+nrrs_status nrrs_grid_indices(nrrs_ctx* ctx, uint32_t n, const nrrs_vertices* v, uint32_t* idx_dev,
+                              void* stream) {
+    if (!ctx || !v || !v->pos || v->stride < n || !idx_dev) return NRRS_ERR_ARG;
+    EncodeArgs a = encode_args(ctx, *v, nullptr, nullptr, 0);
+    return hip_status(launch_grid_indices(a, n, idx_dev, (hipStream_t)stream));
+}
+
+nrrs_status nrrs_statnet_infer(nrrs_ctx* ctx, uint32_t n, const nrrs_vertices* v, float* stats_dev,
+                               void* stream) {
+    if (!ctx || !vertices_ok(v, n) || !stats_dev) return NRRS_ERR_ARG;
+    const NetState& ns = ctx->net[NRRS_STATNET];
+    EncodeArgs a = encode_args(ctx, *v, nullptr, nullptr, 0);
+    EpiArgs ep{};
+    ep.out = stats_dev;
+    ep.out_stride = n;
+    return hip_status(launch_net_fp32(a, NRRS_STATNET, ns.n_layers, ns.n_out, EPI_RAW, n, ns.grid_f32,
+                                      ns.mlp_f32, ep, (hipStream_t)stream));
+}
+
+nrrs_status nrrs_rrsnet_infer(nrrs_ctx* ctx, uint32_t n, const nrrs_vertices* v, const float* film_dev,
+                              const float* stats_dev, float* raw_dev, void* stream) {
+    if (!ctx || !vertices_ok(v, n) || !film_dev || !raw_dev) return NRRS_ERR_ARG;
+    const NetState& ns = ctx->net[NRRS_RRSNET];
+    EncodeArgs a = encode_args(ctx, *v, film_dev, stats_dev, n);
+    EpiArgs ep{};
+    ep.out = raw_dev;
+    ep.out_stride = n;
+    return hip_status(launch_net_fp32(a, NRRS_RRSNET, ns.n_layers, ns.n_out, EPI_RAW, n, ns.grid_f32,
+                                      ns.mlp_f32, ep, (hipStream_t)stream));
+}
+
+static nrrs_status check_decide_args(nrrs_ctx* ctx, const nrrs_decide_args* a) {
+    if (!ctx || !a || !a->q_dev || !a->survivors_dev || !a->n_survivors_dev) return NRRS_ERR_ARG;
+    if (a->n > ctx->cfg.max_queue) return NRRS_ERR_SHAPE;
+    if (a->n_spec_dev && !a->spec_stid_dev) return NRRS_ERR_ARG;
+    return NRRS_OK;
+}
+
+nrrs_status nrrs_rrs_infer_decide(nrrs_ctx* ctx, const nrrs_decide_args* a, void* stream) {
+    NRRS_TRY(check_decide_args(ctx, a));
+    if (!vertices_ok(&a->v, a->n) || !a->film_dev) return NRRS_ERR_ARG;
+    hipStream_t s = (hipStream_t)stream;
+    NRRS_HIP(launch_zero_u64(ctx->sum_dev, s));
+    EpiArgs ep{};
+    ep.q = a->q_dev;
+    ep.sum = ctx->sum_dev;
+    ep.d = ctx->dec;
+    const NetState& sn = ctx->net[NRRS_STATNET];
+    const NetState& rn = ctx->net[NRRS_RRSNET];
+    if (a->q_mode == NRRS_Q_STATNET_ADRRS) {
+        EncodeArgs ea = encode_args(ctx, a->v, a->film_dev, nullptr, 0);
+        NRRS_HIP(launch_net_fp32(ea, NRRS_STATNET, sn.n_layers, sn.n_out, EPI_ADRRS, a->n, sn.grid_f32,
+                                 sn.mlp_f32, ep, s));
+    } else if (a->q_mode == NRRS_Q_RRSNET || a->q_mode == NRRS_Q_RRSNET_CHAINED) {
+        const float* stats = nullptr;
+        if (a->q_mode == NRRS_Q_RRSNET_CHAINED) {
+            EncodeArgs ea = encode_args(ctx, a->v, nullptr, nullptr, 0);
+            EpiArgs es{};
+            es.out = ctx->stats_scratch;
+            es.out_stride = ctx->cfg.max_queue;
+            NRRS_HIP(launch_net_fp32(ea, NRRS_STATNET, sn.n_layers, sn.n_out, EPI_RAW, a->n, sn.grid_f32,
+                                     sn.mlp_f32, es, s));
+            stats = ctx->stats_scratch;
+        }
+        EncodeArgs ea = encode_args(ctx, a->v, a->film_dev, stats, ctx->cfg.max_queue);
+        NRRS_HIP(launch_net_fp32(ea, NRRS_RRSNET, rn.n_layers, rn.n_out, EPI_RRS, a->n, rn.grid_f32,
+                                 rn.mlp_f32, ep, s));
+    } else {
+        return NRRS_ERR_ARG;
+    }
+    return hip_status(launch_decide(ctx, *a, s));
+}
+
+nrrs_status nrrs_rrs_decide(nrrs_ctx* ctx, const nrrs_decide_args* a, void* stream) {
+    NRRS_TRY(check_decide_args(ctx, a));
+    hipStream_t s = (hipStream_t)stream;
+    NRRS_HIP(launch_qsum(ctx, a->q_dev, a->n, s));
+    return hip_status(launch_decide(ctx, *a, s));
+}
+
+nrrs_status nrrs_synth_vertices(uint32_t n, uint64_t seed, uint32_t n_pixels, const nrrs_vertices_out* o,
+                                void* stream) {
+    if (!o || !o->pos || !o->dir || !o->normal || !o->rough || !o->albedo_d || !o->albedo_s || !o->thp ||
+        !o->pixel || o->stride < n || n_pixels == 0)
+        return NRRS_ERR_ARG;
+    return hip_status(launch_synth(n, seed, n_pixels, *o, (hipStream_t)stream));
+}
+
+}  // extern "C"

@@ -1,0 +1,189 @@
+// RRS normalisation, split decision and stable survivor compaction
+// (src/render/nrrs/integrator.cpp:1301-1460, src/render/common/commonworkqueue.h:19-25).
+//
+// The reference appends survivors with one global fetch_add per vertex, so its
+// queue order is nondeterministic. Here the queue is a stable compaction:
+// specular stids first (in the order given), then for each inference index i in
+// ascending order, s_i copies of stid_i. It is built by a tile reduce, a scan of
+// the tile totals and a scatter that re-scans each tile with wavefront prefix sums.
+#include "device_common.hpp"
+
+namespace nrrs {
+namespace {
+
+constexpr int kTile = 256;
+constexpr int kScanThreads = 1024;
+
+__global__ void k_zero_u64(unsigned long long* p) { *p = 0ull; }
+
+// Inclusive prefix sum over the 64 lanes of a wavefront.
+__device__ __forceinline__ uint32_t wave_inclusive_scan(uint32_t v) {
+    const int lane = threadIdx.x & 63;
+#pragma unroll
+    for (int off = 1; off < 64; off <<= 1) {
+        uint32_t y = __shfl_up(v, off, 64);
+        if (lane >= off) v += y;
+    }
+    return v;
+}
+
+// Exclusive prefix over the block; *total receives the block sum.
+template <int BLOCK>
+__device__ __forceinline__ uint32_t block_exclusive_scan(uint32_t v, uint32_t* total) {
+    __shared__ uint32_t wsum[BLOCK / 64];
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    uint32_t inc = wave_inclusive_scan(v);
+    if (lane == 63) wsum[wave] = inc;
+    __syncthreads();
+    uint32_t base = 0, tot = 0;
+#pragma unroll
+    for (int w = 0; w < BLOCK / 64; ++w) {
+        uint32_t x = wsum[w];
+        base += (w < wave) ? x : 0u;
+        tot += x;
+    }
+    *total = tot;
+    return base + inc - v;
+}
+
+struct SplitArgs {
+    const float* q;
+    uint32_t n;
+    const unsigned long long* sum;
+    const uint32_t* n_spec;
+    uint64_t seed, stream;
+    float* qn;
+    uint8_t* s;
+    uint32_t* tile_sums;
+    DecideParams d;
+};
+
+__global__ __launch_bounds__(kTile) void k_split(SplitArgs a) {
+    const uint32_t i = blockIdx.x * kTile + threadIdx.x;
+    uint32_t si = 0;
+    if (i < a.n) {
+        // normalizeFactor (integrator.cpp:1376-1377) from the exact sum
+        double sum = (double)(*a.sum) * 0x1p-32;
+        float sum_f = (float)sum;
+        float rcp = sum_f > 0.0f ? 1.0f / sum_f : 0.0f;
+        uint32_t nspec = a.n_spec ? *a.n_spec : 0u;
+        float nf = (float)(a.d.n_pixels - nspec) * rcp;
+        nf = nf * (a.d.use_ceil ? 1.0f : a.d.rate);
+        // integrator.cpp:1391-1397
+        float r = a.q[i];
+        r = a.d.no_upscale ? r * (nf < 1.0f ? nf : 1.0f) : r * nf;
+        if (a.qn) a.qn[i] = r;
+        float u = u24(pcg_output(pcg_state_at(a.seed, a.stream, i)));
+        // integrator.cpp:1415-1416, 1454
+        int s = (int)r;
+        float left = r - (float)s;
+        s += (u <= left) ? 1 : 0;
+        si = (uint32_t)min(s, (int)a.d.split_cap);
+        a.s[i] = (uint8_t)si;
+    }
+    uint32_t total;
+    block_exclusive_scan<kTile>(si, &total);
+    if (threadIdx.x == 0) a.tile_sums[blockIdx.x] = total;
+}
+
+__global__ __launch_bounds__(kScanThreads) void k_scan_tiles(const uint32_t* __restrict__ tile_sums,
+                                                             uint32_t n_tiles,
+                                                             const uint32_t* __restrict__ n_spec,
+                                                             uint32_t capacity,
+                                                             uint32_t* __restrict__ tile_offsets,
+                                                             uint32_t* __restrict__ n_survivors,
+                                                             uint32_t* __restrict__ overflow) {
+    const uint32_t per = (n_tiles + kScanThreads - 1) / kScanThreads;
+    const uint32_t t0 = threadIdx.x * per;
+    uint32_t local = 0;
+    for (uint32_t t = t0; t < t0 + per && t < n_tiles; ++t) local += tile_sums[t];
+    uint32_t total;
+    uint32_t run = block_exclusive_scan<kScanThreads>(local, &total);
+    const uint32_t nspec = n_spec ? *n_spec : 0u;
+    run += nspec;
+    for (uint32_t t = t0; t < t0 + per && t < n_tiles; ++t) {
+        tile_offsets[t] = run;
+        run += tile_sums[t];
+    }
+    if (threadIdx.x == 0) {
+        uint64_t all = (uint64_t)total + nspec;
+        uint32_t kept = all < capacity ? (uint32_t)all : capacity;
+        *n_survivors = kept;
+        if (overflow) *overflow = (uint32_t)(all - kept);
+    }
+}
+
+__global__ __launch_bounds__(kTile) void k_scatter(const uint8_t* __restrict__ s, uint32_t n,
+                                                   const uint32_t* __restrict__ stid,
+                                                   const uint32_t* __restrict__ tile_offsets,
+                                                   uint32_t capacity, uint32_t* __restrict__ out) {
+    const uint32_t i = blockIdx.x * kTile + threadIdx.x;
+    const uint32_t si = i < n ? s[i] : 0u;
+    uint32_t total;
+    uint32_t pos = tile_offsets[blockIdx.x] + block_exclusive_scan<kTile>(si, &total);
+    if (si) {
+        const uint32_t id = stid ? stid[i] : i;
+        for (uint32_t c = 0; c < si; ++c)
+            if (pos + c < capacity) out[pos + c] = id;
+    }
+}
+
+__global__ void k_spec_copy(const uint32_t* __restrict__ spec, const uint32_t* __restrict__ n_spec,
+                            uint32_t capacity, uint32_t* __restrict__ out) {
+    const uint32_t ns = *n_spec;
+    const uint32_t m = ns < capacity ? ns : capacity;
+    for (uint32_t k = blockIdx.x * blockDim.x + threadIdx.x; k < m; k += gridDim.x * blockDim.x)
+        out[k] = spec[k];
+}
+
+// Normalisation sum of a given clamped q (decision-only entry point).
+__global__ __launch_bounds__(kTile) void k_qsum(const float* __restrict__ q, uint32_t n, DecideParams d,
+                                                unsigned long long* sum) {
+    const uint32_t i = blockIdx.x * kTile + threadIdx.x;
+    uint64_t term = i < n ? rrs_sum_term(d, q[i]) : 0ull;
+    block_atomic_add_u64(term, sum);
+}
+
+}  // namespace
+
+hipError_t launch_qsum(nrrs_ctx* ctx, const float* q, uint32_t n, hipStream_t s) {
+    hipLaunchKernelGGL(k_zero_u64, dim3(1), dim3(1), 0, s, ctx->sum_dev);
+    if (n) hipLaunchKernelGGL(k_qsum, dim3((n + kTile - 1) / kTile), dim3(kTile), 0, s, q, n, ctx->dec, ctx->sum_dev);
+    return hipGetLastError();
+}
+
+hipError_t launch_zero_u64(unsigned long long* p, hipStream_t s) {
+    hipLaunchKernelGGL(k_zero_u64, dim3(1), dim3(1), 0, s, p);
+    return hipGetLastError();
+}
+
+hipError_t launch_decide(nrrs_ctx* ctx, const nrrs_decide_args& a, hipStream_t s) {
+    const uint32_t tiles = (a.n + kTile - 1) / kTile;
+    if (tiles > ctx->max_tiles) return hipErrorInvalidValue;
+    if (tiles) {
+        SplitArgs sa;
+        sa.q = a.q_dev;
+        sa.n = a.n;
+        sa.sum = ctx->sum_dev;
+        sa.n_spec = a.n_spec_dev;
+        sa.seed = a.pcg_seed;
+        sa.stream = a.pcg_stream;
+        sa.qn = a.qn_dev;
+        sa.s = a.split_dev ? a.split_dev : ctx->s_scratch;
+        sa.tile_sums = ctx->tile_sums;
+        sa.d = ctx->dec;
+        hipLaunchKernelGGL(k_split, dim3(tiles), dim3(kTile), 0, s, sa);
+    }
+    hipLaunchKernelGGL(k_scan_tiles, dim3(1), dim3(kScanThreads), 0, s, ctx->tile_sums, tiles,
+                       a.n_spec_dev, a.capacity, ctx->tile_offsets, a.n_survivors_dev, a.overflow_dev);
+    if (a.n_spec_dev && a.spec_stid_dev)
+        hipLaunchKernelGGL(k_spec_copy, dim3(64), dim3(256), 0, s, a.spec_stid_dev, a.n_spec_dev,
+                           a.capacity, a.survivors_dev);
+    if (tiles)
+        hipLaunchKernelGGL(k_scatter, dim3(tiles), dim3(kTile), 0, s,
+                           a.split_dev ? a.split_dev : ctx->s_scratch, a.n, a.v.stid, ctx->tile_offsets,
+                           a.capacity, a.survivors_dev);
+    return hipGetLastError();
+}
+
+}  // namespace nrrs

@@ -1,0 +1,280 @@
+// Device-side building blocks of the neural RRS stage (gfx950).
+#pragma once
+
+#include <hip/hip_bf16.h>
+#include <hip/hip_fp16.h>
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "host_common.hpp"
+
+namespace nrrs {
+
+// ---------------------------------------------------------------- PCG32
+// O'Neill (2014) pcg32, XSH-RR output, settable stream. Vertex i of a stream
+// draws output number i, found by LCG jump-ahead (Brown 1994) so that every
+// thread is independent of every other.
+__device__ __forceinline__ uint32_t pcg_output(uint64_t old) {
+    uint32_t xs = (uint32_t)(((old >> 18u) ^ old) >> 27u);
+    uint32_t rot = (uint32_t)(old >> 59u);
+    return (xs >> rot) | (xs << ((32u - rot) & 31u));
+}
+
+// State after seeding (pcg32_srandom_r), jumped `delta` steps ahead.
+__device__ __forceinline__ uint64_t pcg_state_at(uint64_t seed, uint64_t stream, uint64_t delta) {
+    const uint64_t inc = (stream << 1u) | 1u;
+    uint64_t state = inc * kPcgMult + inc;  // state = 0; next()
+    state = (state + seed) * kPcgMult + inc;  // state += seed; next()
+    uint64_t acc_mult = 1, acc_plus = 0, cur_mult = kPcgMult, cur_plus = inc;
+    while (delta) {
+        if (delta & 1u) {
+            acc_mult *= cur_mult;
+            acc_plus = acc_plus * cur_mult + cur_plus;
+        }
+        cur_plus = (cur_mult + 1) * cur_plus;
+        cur_mult *= cur_mult;
+        delta >>= 1u;
+    }
+    return acc_mult * state + acc_plus;
+}
+
+__device__ __forceinline__ uint32_t pcg_next(uint64_t& state, uint64_t stream) {
+    uint64_t old = state;
+    state = old * kPcgMult + ((stream << 1u) | 1u);
+    return pcg_output(old);
+}
+
+// 24-bit dyadic uniform in [0,1)
+__device__ __forceinline__ float u24(uint32_t x) { return (float)(x >> 8) * 0x1p-24f; }
+
+// ---------------------------------------------------------------- scalar maps
+// activationSigma (src/render/nrrs/net/networkcommon.h:61-68): softplus for x<0,
+// 0.5x + ln 2 above; C1 at 0.
+__device__ __forceinline__ float activation_sigma(float x) {
+    return (x < 0.0f) ? logf(1.0f + expf(x)) : (0.5f * x + 0.6931471805599453f);
+}
+__device__ __forceinline__ float activation_sigma_grad(float x) {
+    if (x < 0.0f) {
+        float e = expf(x);
+        return e / (1.0f + e);
+    }
+    return 0.5f;
+}
+
+// box_cox_transform with lambda = 0.5 (src/render/nrrs/train.cu:19-39)
+__device__ __forceinline__ float box_cox(float x) { return 2.0f * (sqrtf(x) - 1.0f); }
+
+// warp_roughness (src/render/nrrs/train.h:95)
+__device__ __forceinline__ float warp_roughness(float r) { return 1.0f - expf(-r); }
+
+// normalizeSpatialCoord (src/render/nrc/nrctrain.h:111-115), clamped to [0,1]
+__device__ __forceinline__ float norm_coord(float p, float c, float d) {
+    float x = 0.5f + (p - c) / d;
+    return fminf(fmaxf(x, 0.0f), 1.0f);
+}
+
+// ---------------------------------------------------------------- SH degree 4
+// Real spherical harmonics, bands 0..3 (Sloan 2008), written to o[0..15].
+__device__ __forceinline__ void sh4(float x, float y, float z, float* o) {
+    float xx = x * x, yy = y * y, zz = z * z;
+    o[0] = 0.28209479177387814f;
+    o[1] = -0.48860251190291987f * y;
+    o[2] = 0.48860251190291987f * z;
+    o[3] = -0.48860251190291987f * x;
+    o[4] = 1.0925484305920792f * x * y;
+    o[5] = -1.0925484305920792f * y * z;
+    o[6] = 0.94617469575755997f * zz - 0.31539156525251999f;
+    o[7] = -1.0925484305920792f * x * z;
+    o[8] = 0.54627421529603959f * (xx - yy);
+    o[9] = 0.59004358992664352f * y * (-3.0f * xx + yy);
+    o[10] = 2.8906114426405538f * x * y * z;
+    o[11] = 0.45704579946446572f * y * (1.0f - 5.0f * zz);
+    o[12] = 0.3731763325901154f * z * (5.0f * zz - 3.0f);
+    o[13] = 0.45704579946446572f * x * (1.0f - 5.0f * zz);
+    o[14] = 1.4453057213202769f * z * (xx - yy);
+    o[15] = 0.59004358992664352f * x * (-xx + 3.0f * yy);
+}
+
+// ---------------------------------------------------------------- hash grid
+// Cell position of level l: fma(x, scale_l, 0.5), corner c = floor + bit.
+struct LevelCell {
+    uint32_t c[3];
+    float f[3];
+};
+
+__device__ __forceinline__ LevelCell level_cell(const GridParams& g, int l, const float* x) {
+    LevelCell lc;
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+        float p = __fmaf_rn(x[k], g.scale[l], 0.5f);
+        float fl = floorf(p);
+        lc.c[k] = (uint32_t)fl;
+        lc.f[k] = p - fl;
+    }
+    return lc;
+}
+
+// Index of corner `cn` (bit0 = x, bit1 = y, bit2 = z) within level l.
+__device__ __forceinline__ uint32_t corner_index(const GridParams& g, int l, const LevelCell& lc,
+                                                 int cn) {
+    uint32_t cx = lc.c[0] + (cn & 1), cy = lc.c[1] + ((cn >> 1) & 1), cz = lc.c[2] + ((cn >> 2) & 1);
+    if ((g.dense_mask >> l) & 1u) return cx + g.res[l] * (cy + g.res[l] * cz);
+    // instant-NGP spatial hash (Mueller et al. 2022, eq. 4)
+    return (cx ^ (cy * 2654435761u) ^ (cz * 805459861u)) & g.hash_mask;
+}
+
+__device__ __forceinline__ float corner_weight(const LevelCell& lc, int cn) {
+    float wx = (cn & 1) ? lc.f[0] : 1.0f - lc.f[0];
+    float wy = (cn & 2) ? lc.f[1] : 1.0f - lc.f[1];
+    float wz = (cn & 4) ? lc.f[2] : 1.0f - lc.f[2];
+    return (wx * wy) * wz;
+}
+
+// Two features of one table entry, as FP32.
+template <typename TT>
+struct TableLoad;
+template <>
+struct TableLoad<float> {
+    __device__ __forceinline__ static float2 get(const float* t, uint32_t e) {
+        return reinterpret_cast<const float2*>(t)[e];
+    }
+};
+template <>
+struct TableLoad<__half> {
+    __device__ __forceinline__ static float2 get(const __half* t, uint32_t e) {
+        __half2 h = reinterpret_cast<const __half2*>(t)[e];
+        return __half22float2(h);
+    }
+};
+template <>
+struct TableLoad<__hip_bfloat16> {
+    __device__ __forceinline__ static float2 get(const __hip_bfloat16* t, uint32_t e) {
+        uint32_t u = reinterpret_cast<const uint32_t*>(t)[e];
+        return make_float2(__uint_as_float(u << 16), __uint_as_float(u & 0xffff0000u));
+    }
+};
+
+// Trilinear interpolation of the two features of level l.
+template <typename TT>
+__device__ __forceinline__ float2 grid_level(const TT* __restrict__ table, const GridParams& g, int l,
+                                             const float* x) {
+    LevelCell lc = level_cell(g, l, x);
+    float2 acc = make_float2(0.0f, 0.0f);
+#pragma unroll
+    for (int cn = 0; cn < 8; ++cn) {
+        uint32_t e = g.offset[l] + corner_index(g, l, lc, cn);
+        float w = corner_weight(lc, cn);
+        float2 v = TableLoad<TT>::get(table, e);
+        acc.x = __fmaf_rn(w, v.x, acc.x);
+        acc.y = __fmaf_rn(w, v.y, acc.y);
+    }
+    return acc;
+}
+
+// ---------------------------------------------------------------- encoding
+// Network input layout (80 channels):
+//   [0,32)  hash grid, feature f of level l at 2l+f
+//   [32,48) SH4(direction)      [48,64) SH4(normal)
+//   StatNet [64] warp(rough) [65,68) albedo_d [68,71) albedo_s [71] 1 [72,80) 0
+//   RRSNet  [64] warp(rough) [65,68) BC(thp) [68] BC(ref mean)
+//           [69,72) BC(max(L,0)) or albedo_d   [72,75) BC(max(L2,0)) or albedo_s
+//           [75] 1 [76,80) 0
+// The constant channel stands in for the bias the fused MLP does not have
+// (the reference injects 1 as its padding channel, train.cu:94-96).
+__device__ __forceinline__ void load_xn(const EncodeArgs& a, uint32_t i, float* x) {
+    const uint32_t S = a.v.stride;
+#pragma unroll
+    for (int k = 0; k < 3; ++k) x[k] = norm_coord(a.v.pos[k * S + i], a.frame.c[k], a.frame.d[k]);
+}
+
+// Non-grid channels [32,80).
+__device__ __forceinline__ void encode_tail(const EncodeArgs& a, int net, uint32_t i, float* e) {
+    const uint32_t S = a.v.stride;
+    sh4(a.v.dir[i], a.v.dir[S + i], a.v.dir[2 * S + i], e + 32);
+    sh4(a.v.normal[i], a.v.normal[S + i], a.v.normal[2 * S + i], e + 48);
+#pragma unroll
+    for (int k = 64; k < kEnc; ++k) e[k] = 0.0f;
+    e[64] = warp_roughness(a.v.rough[i]);
+    if (net == NRRS_STATNET) {
+#pragma unroll
+        for (int k = 0; k < 3; ++k) {
+            e[65 + k] = a.v.albedo_d[k * S + i];
+            e[68 + k] = a.v.albedo_s[k * S + i];
+        }
+        e[71] = 1.0f;
+    } else {
+        const uint32_t px = a.v.pixel[i];
+        const uint32_t P = a.n_pixels;
+        float ref = (a.film[px] + a.film[P + px] + a.film[2 * P + px]) * (1.0f / 3.0f);
+#pragma unroll
+        for (int k = 0; k < 3; ++k) e[65 + k] = box_cox(a.v.thp[k * S + i]);
+        e[68] = box_cox(ref);
+        if (a.stats) {
+#pragma unroll
+            for (int k = 0; k < 3; ++k) {
+                e[69 + k] = box_cox(fmaxf(a.stats[(size_t)k * a.stats_stride + i], 0.0f));
+                e[72 + k] = box_cox(fmaxf(a.stats[(size_t)(3 + k) * a.stats_stride + i], 0.0f));
+            }
+        } else {
+#pragma unroll
+            for (int k = 0; k < 3; ++k) {
+                e[69 + k] = a.v.albedo_d[k * S + i];
+                e[72 + k] = a.v.albedo_s[k * S + i];
+            }
+        }
+        e[75] = 1.0f;
+    }
+}
+
+template <typename TT>
+__device__ __forceinline__ void encode_full(const EncodeArgs& a, int net, const TT* __restrict__ table,
+                                            uint32_t i, float* e) {
+    float x[3];
+    load_xn(a, i, x);
+#pragma unroll
+    for (int l = 0; l < kMaxLevels; ++l) {
+        if (l < (int)a.grid.n_levels) {
+            float2 f = grid_level<TT>(table, a.grid, l, x);
+            e[2 * l] = f.x;
+            e[2 * l + 1] = f.y;
+        } else {
+            e[2 * l] = 0.0f;
+            e[2 * l + 1] = 0.0f;
+        }
+    }
+    encode_tail(a, net, i, e);
+}
+
+// ---------------------------------------------------------------- RRS epilogue
+// q from the RRSNet raw output (integrator.cpp:1203-1219) or from StatNet L
+// (integrator.cpp:1246-1255); optional weight window; clamp (integrator.cpp:1277).
+__device__ __forceinline__ float rrs_window_clamp(const DecideParams& d, float r) {
+    if (d.weight_window) {
+        float dm = 2.0f / (1.0f + d.window_size), dp = dm * d.window_size;
+        r = r < dm ? r / dm : (r > dp ? r / dp : 1.0f);
+    }
+    return fminf(fmaxf(r, d.clamp_min), d.clamp_max);
+}
+
+// Exact fixed-point term (2^-32 resolution) of the normalisation sum.
+__device__ __forceinline__ uint64_t rrs_sum_term(const DecideParams& d, float q) {
+    float t = d.use_ceil ? ceilf(q) : q;
+    return (uint64_t)((double)t * 4294967296.0);
+}
+
+// Block-wide sum of a u64 (blockDim.x multiple of 64), one atomic per block.
+__device__ __forceinline__ void block_atomic_add_u64(uint64_t v, unsigned long long* dst) {
+    __shared__ unsigned long long part[16];
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    if (lane == 0) part[wave] = v;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        unsigned long long s = 0;
+        for (int w = 0; w < (int)(blockDim.x >> 6); ++w) s += part[w];
+        if (s) atomicAdd(dst, s);
+    }
+}
+
+}  // namespace nrrs

@@ -1,0 +1,106 @@
+// Shared host/device definitions of the neural RRS stage.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "../../include/nrrs_amd.h"
+
+namespace nrrs {
+
+constexpr int kEnc = 80;       // network input width (64 encoded + 16 per-vertex channels)
+constexpr int kHidden = 64;    // MLP width
+constexpr int kMaxLevels = 16;
+constexpr int kMaxLayers = 7;  // n_hidden_layers <= 6
+constexpr uint64_t kPcgMult = 6364136223846793005ull;
+
+// Per-level hash-grid constants, passed to kernels by value.
+struct GridParams {
+    float scale[kMaxLevels];
+    uint32_t res[kMaxLevels];
+    uint32_t offset[kMaxLevels];  // first entry of the level
+    uint32_t dense_mask;          // bit l set: level l is a dense (res^3 <= T) grid
+    uint32_t n_levels;
+    uint32_t hash_mask;           // T - 1
+    uint32_t n_entries;
+};
+
+// normalizeSpatialCoord frame: x = 0.5 + (p - c) / d.
+struct Frame {
+    float c[3];
+    float d[3];
+};
+
+struct EncodeArgs {
+    nrrs_vertices v;
+    const float* film;        // [3][n_pixels]
+    uint32_t n_pixels;
+    const float* stats;       // [6][stats_stride] StatNet outputs, or null
+    uint32_t stats_stride;
+    GridParams grid;
+    Frame frame;
+};
+
+struct DecideParams {
+    float clamp_min, clamp_max, rate, window_size;
+    int32_t use_ceil, no_upscale, weight_window;
+    uint32_t split_cap;
+    uint32_t n_pixels;
+};
+
+// Outputs of the fused network kernels.
+struct EpiArgs {
+    float* out;                  // EPI_RAW: [n_out][out_stride]
+    uint32_t out_stride;
+    float* q;                    // EPI_RRS/EPI_ADRRS: [n]
+    unsigned long long* sum;     // fixed-point normalisation sum
+    DecideParams d;
+};
+
+enum Epilogue { EPI_RAW = 0, EPI_RRS = 1, EPI_ADRRS = 2 };
+
+struct NetState {
+    int n_layers = 0, n_out = 0;
+    uint64_t n_grid = 0, n_mlp = 0;  // FP32 parameter counts
+    float* grid_f32 = nullptr;       // master grid parameters [entries][2]
+    float* mlp_f32 = nullptr;        // master MLP weights, row-major per layer
+    uint16_t* grid_lp = nullptr;     // FP16/BF16 shadow of the grid
+    uint16_t* mlp_lp = nullptr;      // FP16/BF16 shadow of the MLP (MFMA tile order)
+};
+
+}  // namespace nrrs
+
+struct nrrs_ctx {
+    nrrs_config cfg;
+    int device = 0;
+    nrrs::GridParams grid;
+    nrrs::Frame frame;
+    nrrs::DecideParams dec;
+    nrrs::NetState net[2];
+    // RRS-stage scratch
+    unsigned long long* sum_dev = nullptr;  // [1]
+    uint32_t* tile_sums = nullptr;          // [max tiles]
+    uint32_t* tile_offsets = nullptr;       // [max tiles]
+    float* stats_scratch = nullptr;         // [6][max_queue]
+    float* q_scratch = nullptr;             // [max_queue]
+    uint8_t* s_scratch = nullptr;           // [max_queue]
+    uint32_t max_tiles = 0;
+};
+
+namespace nrrs {
+
+// ------------------------------------------------------------ launchers
+hipError_t launch_encode(const EncodeArgs& a, int net, uint32_t n, const float* table, float* enc,
+                         hipStream_t s);
+hipError_t launch_grid_indices(const EncodeArgs& a, uint32_t n, uint32_t* idx, hipStream_t s);
+hipError_t launch_net_fp32(const EncodeArgs& a, int net, int n_layers, int n_out, int epi, uint32_t n,
+                           const float* table, const float* W, const EpiArgs& ep, hipStream_t s);
+
+hipError_t launch_zero_u64(unsigned long long* p, hipStream_t s);
+hipError_t launch_qsum(nrrs_ctx* ctx, const float* q, uint32_t n, hipStream_t s);
+hipError_t launch_decide(nrrs_ctx* ctx, const nrrs_decide_args& a, hipStream_t s);
+
+hipError_t launch_synth(uint32_t n, uint64_t seed, uint32_t n_pixels, const nrrs_vertices_out& o,
+                        hipStream_t s);
+
+}  // namespace nrrs

@@ -1,0 +1,376 @@
+"""ctypes binding of libnrrs_amd.so (the C ABI declared in include/nrrs_amd.h).
+
+This is the harness-side view of the library: torch tensors provide device
+memory and streams, every computation runs in the HIP library. There is no
+CPU fallback: loading fails loudly when the shared library is missing.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(os.path.dirname(_HERE), "lib", "libnrrs_amd.so")
+
+OK, ERR_ARG, ERR_SHAPE, ERR_OOM, ERR_OVERFLOW, ERR_NONFINITE, ERR_HIP, ERR_UNSUPPORTED, ERR_NO_DEVICE = range(9)
+FP32, FP16, BF16 = 0, 1, 2
+STATNET, RRSNET = 0, 1
+Q_RRSNET, Q_RRSNET_CHAINED, Q_STATNET_ADRRS = 0, 1, 2
+ENC_WIDTH = 80
+
+
+class GridConfig(C.Structure):
+    _fields_ = [
+        ("n_levels", C.c_uint32),
+        ("n_features", C.c_uint32),
+        ("log2_table_size", C.c_uint32),
+        ("base_resolution", C.c_uint32),
+        ("per_level_scale", C.c_float),
+    ]
+
+
+class MlpConfig(C.Structure):
+    _fields_ = [("n_hidden_layers", C.c_uint32), ("n_outputs", C.c_uint32)]
+
+
+class Config(C.Structure):
+    _fields_ = [
+        ("grid", GridConfig),
+        ("statnet", MlpConfig),
+        ("rrsnet", MlpConfig),
+        ("precision", C.c_int32),
+        ("rrs_clamp_min", C.c_float),
+        ("rrs_clamp_max", C.c_float),
+        ("rrs_normalize_rate", C.c_float),
+        ("rrs_use_ceil", C.c_int32),
+        ("rrs_no_upscale", C.c_int32),
+        ("rrs_weight_window", C.c_int32),
+        ("weight_window_size", C.c_float),
+        ("split_cap", C.c_uint32),
+        ("loss_clamp_on", C.c_int32),
+        ("loss_clamp_max", C.c_float),
+        ("train_sigma", C.c_int32),
+        ("ll2_step", C.c_uint32),
+        ("rrs_step", C.c_uint32),
+        ("gamma", C.c_float * 4),
+        ("lr_statnet", C.c_float),
+        ("lr_rrsnet", C.c_float),
+        ("adam_beta1", C.c_float),
+        ("adam_beta2", C.c_float),
+        ("adam_eps", C.c_float),
+        ("adam_l2_reg", C.c_float),
+        ("loss_scale_init", C.c_float),
+        ("loss_scale_growth_interval", C.c_uint32),
+        ("aabb_min", C.c_float * 3),
+        ("aabb_max", C.c_float * 3),
+        ("n_pixels", C.c_uint32),
+        ("max_queue", C.c_uint32),
+        ("max_train", C.c_uint32),
+        ("seed", C.c_uint64),
+    ]
+
+
+class Vertices(C.Structure):
+    _fields_ = [
+        ("pos", C.c_void_p),
+        ("dir", C.c_void_p),
+        ("normal", C.c_void_p),
+        ("rough", C.c_void_p),
+        ("albedo_d", C.c_void_p),
+        ("albedo_s", C.c_void_p),
+        ("thp", C.c_void_p),
+        ("pixel", C.c_void_p),
+        ("stid", C.c_void_p),
+        ("stride", C.c_uint32),
+    ]
+
+
+class VerticesOut(C.Structure):
+    _fields_ = [
+        ("pos", C.c_void_p),
+        ("dir", C.c_void_p),
+        ("normal", C.c_void_p),
+        ("rough", C.c_void_p),
+        ("albedo_d", C.c_void_p),
+        ("albedo_s", C.c_void_p),
+        ("thp", C.c_void_p),
+        ("pixel", C.c_void_p),
+        ("stride", C.c_uint32),
+    ]
+
+
+class DecideArgs(C.Structure):
+    _fields_ = [
+        ("n", C.c_uint32),
+        ("v", Vertices),
+        ("film_dev", C.c_void_p),
+        ("q_mode", C.c_int32),
+        ("spec_stid_dev", C.c_void_p),
+        ("n_spec_dev", C.c_void_p),
+        ("pcg_seed", C.c_uint64),
+        ("pcg_stream", C.c_uint64),
+        ("q_dev", C.c_void_p),
+        ("qn_dev", C.c_void_p),
+        ("split_dev", C.c_void_p),
+        ("survivors_dev", C.c_void_p),
+        ("capacity", C.c_uint32),
+        ("n_survivors_dev", C.c_void_p),
+        ("overflow_dev", C.c_void_p),
+    ]
+
+
+# name -> (restype, argtypes)
+_P = C.c_void_p
+_SIGS = {
+    "nrrs_status_string": (C.c_char_p, [C.c_int]),
+    "nrrs_struct_size": (C.c_size_t, [C.c_int32]),
+    "nrrs_default_config": (C.c_int, [C.POINTER(Config)]),
+    "nrrs_create": (C.c_int, [C.POINTER(Config), C.POINTER(_P)]),
+    "nrrs_destroy": (C.c_int, [_P]),
+    "nrrs_grid_layout": (C.c_int, [C.POINTER(GridConfig), _P, _P, _P, _P, _P, C.POINTER(C.c_uint64)]),
+    "nrrs_param_count": (C.c_int, [_P, C.c_int32, C.POINTER(C.c_uint64), C.POINTER(C.c_uint64)]),
+    "nrrs_params_get": (C.c_int, [_P, C.c_int32, _P]),
+    "nrrs_params_set": (C.c_int, [_P, C.c_int32, _P]),
+    "nrrs_encode": (C.c_int, [_P, C.c_int32, C.c_uint32, C.POINTER(Vertices), _P, _P, _P, _P]),
+    "nrrs_grid_indices": (C.c_int, [_P, C.c_uint32, C.POINTER(Vertices), _P, _P]),
+    "nrrs_statnet_infer": (C.c_int, [_P, C.c_uint32, C.POINTER(Vertices), _P, _P]),
+    "nrrs_rrsnet_infer": (C.c_int, [_P, C.c_uint32, C.POINTER(Vertices), _P, _P, _P, _P]),
+    "nrrs_rrs_infer_decide": (C.c_int, [_P, C.POINTER(DecideArgs), _P]),
+    "nrrs_rrs_decide": (C.c_int, [_P, C.POINTER(DecideArgs), _P]),
+    "nrrs_synth_vertices": (C.c_int, [C.c_uint32, C.c_uint64, C.c_uint32, C.POINTER(VerticesOut), _P]),
+}
+
+_lib = None
+
+
+def exported_names():
+    return list(_SIGS)
+
+
+def load(path=LIB_PATH):
+    """Loads the HIP library (raises if it has not been built)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(path):
+        raise RuntimeError(f"libnrrs_amd.so not built ({path}); run nrrs-code_amd/build.py")
+    lib = C.CDLL(path)
+    for name, (res, args) in _SIGS.items():
+        fn = getattr(lib, name)
+        fn.restype = res
+        fn.argtypes = args
+    for k, cls in enumerate((Config, Vertices, DecideArgs, VerticesOut)):
+        if lib.nrrs_struct_size(k) != C.sizeof(cls):
+            raise RuntimeError(f"ABI mismatch for {cls.__name__}: {lib.nrrs_struct_size(k)} != {C.sizeof(cls)}")
+    _lib = lib
+    return lib
+
+
+class NrrsError(RuntimeError):
+    def __init__(self, code, what):
+        self.code = code
+        super().__init__(f"{what}: {load().nrrs_status_string(code).decode()} ({code})")
+
+
+def check(code, what=""):
+    if code != OK:
+        raise NrrsError(code, what)
+    return code
+
+
+def default_config(**overrides):
+    cfg = Config()
+    check(load().nrrs_default_config(C.byref(cfg)), "default_config")
+    for k, v in overrides.items():
+        if k in ("gamma", "aabb_min", "aabb_max"):
+            arr = getattr(cfg, k)
+            for i, x in enumerate(v):
+                arr[i] = x
+        elif k in ("grid", "statnet", "rrsnet"):
+            sub = getattr(cfg, k)
+            for kk, vv in v.items():
+                setattr(sub, kk, vv)
+        else:
+            setattr(cfg, k, v)
+    return cfg
+
+
+def grid_layout(grid_cfg):
+    L = grid_cfg.n_levels
+    res, off, size, dense = (np.zeros(L, np.uint32) for _ in range(4))
+    scale = np.zeros(L, np.float32)
+    n = C.c_uint64()
+    check(load().nrrs_grid_layout(C.byref(grid_cfg), res.ctypes.data, off.ctypes.data, size.ctypes.data,
+                                  dense.ctypes.data, scale.ctypes.data, C.byref(n)), "grid_layout")
+    return dict(res=res, offset=off, size=size, dense=dense, scale=scale, n_params=int(n.value))
+
+
+def _ptr(t):
+    return None if t is None else C.c_void_p(t.data_ptr())
+
+
+def _stream(stream=None):
+    import torch
+
+    s = stream if stream is not None else torch.cuda.current_stream()
+    return C.c_void_p(s.cuda_stream)
+
+
+class VertexBatch:
+    """Device SoA of path vertices (component-major fields of length `stride`)."""
+
+    FIELDS3 = ("pos", "dir", "normal", "albedo_d", "albedo_s", "thp")
+
+    def __init__(self, n, device="cuda", stride=None):
+        import torch
+
+        self.n = n
+        self.stride = stride or max(n, 1)
+        f = dict(dtype=torch.float32, device=device)
+        for k in self.FIELDS3:
+            setattr(self, k, torch.zeros(3, self.stride, **f))
+        self.rough = torch.zeros(self.stride, **f)
+        self.pixel = torch.zeros(self.stride, dtype=torch.int32, device=device)
+        self.stid = None
+
+    @classmethod
+    def from_numpy(cls, d, device="cuda"):
+        import torch
+
+        n = d["rough"].shape[0]
+        vb = cls(n, device)
+        for k in cls.FIELDS3:
+            getattr(vb, k).copy_(torch.from_numpy(np.ascontiguousarray(d[k], np.float32)))
+        vb.rough.copy_(torch.from_numpy(np.ascontiguousarray(d["rough"], np.float32)))
+        vb.pixel.copy_(torch.from_numpy(np.ascontiguousarray(d["pixel"]).astype(np.int32)))
+        if d.get("stid") is not None:
+            vb.stid = torch.from_numpy(np.ascontiguousarray(d["stid"]).astype(np.int32)).to(device)
+        return vb
+
+    def to_numpy(self):
+        out = {k: getattr(self, k)[:, : self.n].cpu().numpy() for k in self.FIELDS3}
+        out["rough"] = self.rough[: self.n].cpu().numpy()
+        out["pixel"] = self.pixel[: self.n].cpu().numpy().astype(np.uint32)
+        out["stid"] = None if self.stid is None else self.stid[: self.n].cpu().numpy().astype(np.uint32)
+        return out
+
+    def c_struct(self):
+        v = Vertices()
+        for k in self.FIELDS3 + ("rough", "pixel"):
+            setattr(v, k, getattr(self, k).data_ptr())
+        v.stid = None if self.stid is None else self.stid.data_ptr()
+        v.stride = self.stride
+        return v
+
+    def c_out(self):
+        o = VerticesOut()
+        for k in self.FIELDS3 + ("rough", "pixel"):
+            setattr(o, k, getattr(self, k).data_ptr())
+        o.stride = self.stride
+        return o
+
+
+def synth_vertices(n, seed, n_pixels, device="cuda", stream=None):
+    vb = VertexBatch(n, device)
+    o = vb.c_out()
+    check(load().nrrs_synth_vertices(n, seed, n_pixels, C.byref(o), _stream(stream)), "synth_vertices")
+    return vb
+
+
+class Context:
+    """One nrrs_ctx (the NRRSNetwork equivalent)."""
+
+    def __init__(self, cfg=None, **overrides):
+        self.lib = load()
+        self.cfg = cfg if cfg is not None else default_config(**overrides)
+        h = C.c_void_p()
+        check(self.lib.nrrs_create(C.byref(self.cfg), C.byref(h)), "nrrs_create")
+        self.h = h
+
+    def close(self):
+        if self.h:
+            check(self.lib.nrrs_destroy(self.h), "nrrs_destroy")
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def param_count(self, net):
+        g, m = C.c_uint64(), C.c_uint64()
+        check(self.lib.nrrs_param_count(self.h, net, C.byref(g), C.byref(m)), "param_count")
+        return int(g.value), int(m.value)
+
+    def params_get(self, net):
+        g, m = self.param_count(net)
+        out = np.zeros(g + m, np.float32)
+        check(self.lib.nrrs_params_get(self.h, net, out.ctypes.data), "params_get")
+        return out
+
+    def params_set(self, net, arr):
+        g, m = self.param_count(net)
+        a = np.ascontiguousarray(arr, np.float32)
+        assert a.shape == (g + m,)
+        check(self.lib.nrrs_params_set(self.h, net, a.ctypes.data), "params_set")
+
+    def encode(self, net, vb, film=None, stats=None, stream=None):
+        import torch
+
+        enc = torch.empty(vb.n, ENC_WIDTH, dtype=torch.float32, device=vb.pos.device)
+        v = vb.c_struct()
+        check(self.lib.nrrs_encode(self.h, net, vb.n, C.byref(v), _ptr(film), _ptr(stats), _ptr(enc),
+                                   _stream(stream)), "encode")
+        return enc
+
+    def grid_indices(self, vb, stream=None):
+        import torch
+
+        idx = torch.empty(vb.n, self.cfg.grid.n_levels, 8, dtype=torch.int32, device=vb.pos.device)
+        v = vb.c_struct()
+        check(self.lib.nrrs_grid_indices(self.h, vb.n, C.byref(v), _ptr(idx), _stream(stream)), "grid_indices")
+        return idx
+
+    def statnet_infer(self, vb, stream=None):
+        import torch
+
+        out = torch.empty(6, vb.n, dtype=torch.float32, device=vb.pos.device)
+        v = vb.c_struct()
+        check(self.lib.nrrs_statnet_infer(self.h, vb.n, C.byref(v), _ptr(out), _stream(stream)), "statnet_infer")
+        return out
+
+    def rrsnet_infer(self, vb, film, stats=None, stream=None):
+        import torch
+
+        out = torch.empty(vb.n, dtype=torch.float32, device=vb.pos.device)
+        v = vb.c_struct()
+        check(self.lib.nrrs_rrsnet_infer(self.h, vb.n, C.byref(v), _ptr(film), _ptr(stats), _ptr(out),
+                                         _stream(stream)), "rrsnet_infer")
+        return out
+
+    def decide_args(self, vb, film, q, survivors, n_surv, overflow=None, q_mode=Q_RRSNET, spec=None,
+                    n_spec=None, seed=0, stream_id=0, qn=None, split=None):
+        a = DecideArgs()
+        a.n = vb.n
+        a.v = vb.c_struct()
+        a.film_dev = None if film is None else film.data_ptr()
+        a.q_mode = q_mode
+        a.spec_stid_dev = None if spec is None else spec.data_ptr()
+        a.n_spec_dev = None if n_spec is None else n_spec.data_ptr()
+        a.pcg_seed = seed
+        a.pcg_stream = stream_id
+        a.q_dev = q.data_ptr()
+        a.qn_dev = None if qn is None else qn.data_ptr()
+        a.split_dev = None if split is None else split.data_ptr()
+        a.survivors_dev = survivors.data_ptr()
+        a.capacity = survivors.numel()
+        a.n_survivors_dev = n_surv.data_ptr()
+        a.overflow_dev = None if overflow is None else overflow.data_ptr()
+        return a
+
+    def rrs_infer_decide(self, args, stream=None):
+        return self.lib.nrrs_rrs_infer_decide(self.h, C.byref(args), _stream(stream))
+
+    def rrs_decide(self, args, stream=None):
+        return self.lib.nrrs_rrs_decide(self.h, C.byref(args), _stream(stream))

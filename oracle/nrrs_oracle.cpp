@@ -1,0 +1,474 @@
+// CPU oracle of the neural RRS stage - TEST INFRASTRUCTURE ONLY (the checker).
+// FP64 restatement of the reference algorithms; see nrrs_oracle.h for what pins
+// each function. Built with -ffp-contract=off so FP32 expressions round exactly
+// as written.
+#include "nrrs_oracle.h"
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <vector>
+
+namespace {
+
+constexpr uint64_t kPcgMult = 6364136223846793005ull;
+
+struct Pcg32 {
+    uint64_t state, inc;
+    // pcg32_srandom_r: O'Neill, "PCG: A Family of Simple Fast Space-Efficient
+    // Statistically Good Algorithms for Random Number Generation" (2014).
+    Pcg32(uint64_t seed, uint64_t stream) {
+        inc = (stream << 1u) | 1u;
+        state = 0;
+        next();
+        state += seed;
+        next();
+    }
+    uint32_t next() {
+        uint64_t old = state;
+        state = old * kPcgMult + inc;
+        uint32_t xs = (uint32_t)(((old >> 18u) ^ old) >> 27u);
+        uint32_t rot = (uint32_t)(old >> 59u);
+        return (xs >> rot) | (xs << ((32u - rot) & 31u));
+    }
+    // LCG jump-ahead (F. Brown, "Random number generation with arbitrary
+    // strides", 1994): state <- A^delta state + C(delta).
+    void advance(uint64_t delta) {
+        uint64_t acc_mult = 1, acc_plus = 0, cur_mult = kPcgMult, cur_plus = inc;
+        while (delta) {
+            if (delta & 1u) {
+                acc_mult *= cur_mult;
+                acc_plus = acc_plus * cur_mult + cur_plus;
+            }
+            cur_plus = (cur_mult + 1) * cur_plus;
+            cur_mult *= cur_mult;
+            delta >>= 1u;
+        }
+        state = acc_mult * state + acc_plus;
+    }
+};
+
+uint32_t pcg_at(uint64_t seed, uint64_t stream, uint64_t pos) {
+    Pcg32 r(seed, stream);
+    r.advance(pos);
+    return r.next();
+}
+
+float u24(uint32_t x) { return (float)(x >> 8) * 0x1p-24f; }
+
+struct Level {
+    uint32_t res, offset, size, dense;
+    float scale;
+};
+
+std::vector<Level> levels_of(const nrrs_grid_config& g, uint64_t* n_entries) {
+    std::vector<Level> L(g.n_levels);
+    const uint64_t T = 1ull << g.log2_table_size;
+    uint64_t off = 0;
+    for (uint32_t l = 0; l < g.n_levels; ++l) {
+        double s = (double)g.base_resolution * std::pow((double)g.per_level_scale, (double)l);
+        L[l].scale = (float)s;
+        L[l].res = (uint32_t)std::ceil((double)L[l].scale) + 2u;
+        uint64_t r3 = (uint64_t)L[l].res * L[l].res * L[l].res;
+        L[l].dense = r3 <= T ? 1u : 0u;
+        L[l].size = (uint32_t)(L[l].dense ? r3 : T);
+        L[l].offset = (uint32_t)off;
+        off += (L[l].size + 7ull) / 8ull * 8ull;
+    }
+    if (n_entries) *n_entries = off;
+    return L;
+}
+
+// normalizeSpatialCoord (src/render/nrc/nrctrain.h:111-115): the AABB is
+// inflated by 0.5% of its diagonal length, then p -> 0.5 + (p - c) / d.
+void aabb_frame(const float* mn, const float* mx, float* c, float* d) {
+    float dx = mx[0] - mn[0], dy = mx[1] - mn[1], dz = mx[2] - mn[2];
+    float len = std::sqrt(dx * dx + dy * dy + dz * dz);
+    float infl = len * 0.005f;
+    for (int k = 0; k < 3; ++k) {
+        float lo = mn[k] - infl, hi = mx[k] + infl;
+        c[k] = (lo + hi) * 0.5f;
+        d[k] = hi - lo;
+    }
+}
+
+// fmin/fmax semantics: a NaN coordinate maps to 0.
+float norm_coord(float p, float c, float d) {
+    float x = 0.5f + (p - c) / d;
+    return std::fmin(std::fmax(x, 0.0f), 1.0f);
+}
+
+uint32_t corner_index(const Level& L, uint32_t log2T, uint32_t cx, uint32_t cy, uint32_t cz) {
+    if (L.dense) return cx + L.res * (cy + L.res * cz);
+    // instant-NGP spatial hash (Mueller et al. 2022, eq. 4)
+    uint32_t h = cx ^ (cy * 2654435761u) ^ (cz * 805459861u);
+    return h & ((1u << log2T) - 1u);
+}
+
+double sigma_d(double x) { return x < 0 ? std::log1p(std::exp(x)) : 0.5 * x + 0.6931471805599453; }
+
+float sigma_f(float x) {
+    // activationSigma (src/render/nrrs/net/networkcommon.h:61-68) in FP32
+    return (x < 0) ? std::log(1.0f + std::exp(x)) : (0.5f * x + 0.6931471805599453f);
+}
+
+double bc_d(double x) { return 2.0 * (std::sqrt(x) - 1.0); }
+
+
+void mlp_dims(const nrrs_config* cfg, int net, int* n_layers, int* n_out) {
+    const nrrs_mlp_config& m = net == NRRS_STATNET ? cfg->statnet : cfg->rrsnet;
+    *n_layers = (int)m.n_hidden_layers + 1;
+    *n_out = (int)m.n_outputs;
+}
+
+}  // namespace
+
+extern "C" {
+
+void ora_pcg32(uint64_t seed, uint64_t stream, uint64_t skip, uint32_t n, uint32_t* out) {
+    Pcg32 r(seed, stream);
+    r.advance(skip);
+    for (uint32_t i = 0; i < n; ++i) out[i] = r.next();
+}
+
+uint64_t ora_grid_layout(const nrrs_grid_config* g, uint32_t* res, uint32_t* offset,
+                         uint32_t* size, uint32_t* dense, float* scale) {
+    uint64_t n = 0;
+    std::vector<Level> L = levels_of(*g, &n);
+    for (uint32_t l = 0; l < g->n_levels; ++l) {
+        if (res) res[l] = L[l].res;
+        if (offset) offset[l] = L[l].offset;
+        if (size) size[l] = L[l].size;
+        if (dense) dense[l] = L[l].dense;
+        if (scale) scale[l] = L[l].scale;
+    }
+    return n * g->n_features;
+}
+
+void ora_normalize_pos(const float* aabb_min, const float* aabb_max, uint32_t n,
+                       const float* pos, uint32_t stride, float* xn) {
+    float c[3], d[3];
+    aabb_frame(aabb_min, aabb_max, c, d);
+    for (uint32_t i = 0; i < n; ++i)
+        for (int k = 0; k < 3; ++k) xn[i * 3 + k] = norm_coord(pos[k * stride + i], c[k], d[k]);
+}
+
+void ora_grid_indices(const nrrs_config* cfg, uint32_t n, const nrrs_vertices* v, uint32_t* idx) {
+    const nrrs_grid_config& g = cfg->grid;
+    std::vector<Level> L = levels_of(g, nullptr);
+    float c[3], d[3];
+    aabb_frame(cfg->aabb_min, cfg->aabb_max, c, d);
+    for (uint32_t i = 0; i < n; ++i) {
+        float x[3];
+        for (int k = 0; k < 3; ++k) x[k] = norm_coord(v->pos[k * v->stride + i], c[k], d[k]);
+        for (uint32_t l = 0; l < g.n_levels; ++l) {
+            uint32_t cell[3];
+            for (int k = 0; k < 3; ++k) {
+                float p = std::fma(x[k], L[l].scale, 0.5f);
+                cell[k] = (uint32_t)std::floor(p);
+            }
+            for (uint32_t cn = 0; cn < 8; ++cn)
+                idx[((size_t)i * g.n_levels + l) * 8 + cn] =
+                    corner_index(L[l], g.log2_table_size, cell[0] + (cn & 1u),
+                                 cell[1] + ((cn >> 1) & 1u), cell[2] + ((cn >> 2) & 1u));
+        }
+    }
+}
+
+double ora_activation_sigma(double x) { return sigma_d(x); }
+double ora_box_cox(double x) { return bc_d(x); }
+
+double ora_weight_window(double q, double size) {
+    // RRSWeightWindow (src/render/common/commoncudautils.h:14-28)
+    double dminus = 2.0 / (1.0 + size), dplus = dminus * size;
+    if (q < dminus) return q / dminus;
+    if (q > dplus) return q / dplus;
+    return 1.0;
+}
+
+void ora_sh4(double x, double y, double z, double* o) {
+    // Real spherical harmonics, bands 0..3 (Sloan, "Stupid Spherical Harmonics Tricks", 2008)
+    double xx = x * x, yy = y * y, zz = z * z;
+    o[0] = 0.28209479177387814;
+    o[1] = -0.48860251190291987 * y;
+    o[2] = 0.48860251190291987 * z;
+    o[3] = -0.48860251190291987 * x;
+    o[4] = 1.0925484305920792 * x * y;
+    o[5] = -1.0925484305920792 * y * z;
+    o[6] = 0.94617469575755997 * zz - 0.31539156525251999;
+    o[7] = -1.0925484305920792 * x * z;
+    o[8] = 0.54627421529603959 * (xx - yy);
+    o[9] = 0.59004358992664352 * y * (-3.0 * xx + yy);
+    o[10] = 2.8906114426405538 * x * y * z;
+    o[11] = 0.45704579946446572 * y * (1.0 - 5.0 * zz);
+    o[12] = 0.3731763325901154 * z * (5.0 * zz - 3.0);
+    o[13] = 0.45704579946446572 * x * (1.0 - 5.0 * zz);
+    o[14] = 1.4453057213202769 * z * (xx - yy);
+    o[15] = 0.59004358992664352 * x * (-xx + 3.0 * yy);
+}
+
+uint64_t ora_param_count(const nrrs_config* cfg, int32_t net, uint64_t* n_grid, uint64_t* n_mlp) {
+    uint64_t ng = ora_grid_layout(&cfg->grid, nullptr, nullptr, nullptr, nullptr, nullptr);
+    int nl, no;
+    mlp_dims(cfg, net, &nl, &no);
+    uint64_t nm = (uint64_t)ORA_HIDDEN * ORA_ENC_WIDTH +
+                  (uint64_t)(nl - 2) * ORA_HIDDEN * ORA_HIDDEN + (uint64_t)no * ORA_HIDDEN;
+    if (n_grid) *n_grid = ng;
+    if (n_mlp) *n_mlp = nm;
+    return ng + nm;
+}
+
+void ora_encode(const nrrs_config* cfg, int32_t net, uint32_t n, const nrrs_vertices* v,
+                const float* film, const double* stats, const double* params, double* enc) {
+    const nrrs_grid_config& g = cfg->grid;
+    std::vector<Level> L = levels_of(g, nullptr);
+    float c[3], d[3];
+    aabb_frame(cfg->aabb_min, cfg->aabb_max, c, d);
+    const uint32_t S = v->stride;
+    for (uint32_t i = 0; i < n; ++i) {
+        double* e = enc + (size_t)i * ORA_ENC_WIDTH;
+        std::fill(e, e + ORA_ENC_WIDTH, 0.0);
+        float x[3];
+        for (int k = 0; k < 3; ++k) x[k] = norm_coord(v->pos[k * S + i], c[k], d[k]);
+        for (uint32_t l = 0; l < g.n_levels; ++l) {
+            uint32_t cell[3];
+            double fr[3];
+            for (int k = 0; k < 3; ++k) {
+                float p = std::fma(x[k], L[l].scale, 0.5f);
+                float fl = std::floor(p);
+                cell[k] = (uint32_t)fl;
+                fr[k] = (double)(p - fl);
+            }
+            double f0 = 0, f1 = 0;
+            for (uint32_t cn = 0; cn < 8; ++cn) {
+                double w = 1.0;
+                for (int k = 0; k < 3; ++k) w *= ((cn >> k) & 1u) ? fr[k] : 1.0 - fr[k];
+                uint32_t id = corner_index(L[l], g.log2_table_size, cell[0] + (cn & 1u),
+                                           cell[1] + ((cn >> 1) & 1u), cell[2] + ((cn >> 2) & 1u));
+                const double* t = params + ((size_t)L[l].offset + id) * 2;
+                f0 += w * t[0];
+                f1 += w * t[1];
+            }
+            e[2 * l] = f0;
+            e[2 * l + 1] = f1;
+        }
+        ora_sh4(v->dir[i], v->dir[S + i], v->dir[2 * S + i], e + 32);
+        ora_sh4(v->normal[i], v->normal[S + i], v->normal[2 * S + i], e + 48);
+        // warp_roughness (src/render/nrrs/train.h:95)
+        e[64] = 1.0 - std::exp(-(double)v->rough[i]);
+        if (net == NRRS_STATNET) {
+            for (int k = 0; k < 3; ++k) {
+                e[65 + k] = v->albedo_d[k * S + i];
+                e[68 + k] = v->albedo_s[k * S + i];
+            }
+            e[71] = 1.0;
+        } else {
+            // RRSNet input: Box-Cox of thp, ref mean, StatNet stats
+            // (nrrs_generate_inference_data_from_0to1, src/render/nrrs/train.cu:41-112)
+            uint32_t px = v->pixel[i];
+            double ref = ((double)film[px] + (double)film[cfg->n_pixels + px] +
+                          (double)film[2 * (size_t)cfg->n_pixels + px]) / 3.0;
+            for (int k = 0; k < 3; ++k) e[65 + k] = bc_d((double)v->thp[k * S + i]);
+            e[68] = bc_d(ref);
+            for (int k = 0; k < 3; ++k) {
+                if (stats) {
+                    e[69 + k] = bc_d(std::max(stats[(size_t)k * n + i], 0.0));
+                    e[72 + k] = bc_d(std::max(stats[(size_t)(3 + k) * n + i], 0.0));
+                } else {
+                    e[69 + k] = v->albedo_d[k * S + i];
+                    e[72 + k] = v->albedo_s[k * S + i];
+                }
+            }
+            e[75] = 1.0;
+        }
+    }
+}
+
+void ora_net_forward(const nrrs_config* cfg, int32_t net, uint32_t n, const nrrs_vertices* v,
+                     const float* film, const double* stats, const double* params, double* out) {
+    uint64_t ng = 0;
+    ora_param_count(cfg, net, &ng, nullptr);
+    int nl, no;
+    mlp_dims(cfg, net, &nl, &no);
+    std::vector<double> enc((size_t)n * ORA_ENC_WIDTH);
+    ora_encode(cfg, net, n, v, film, stats, params, enc.data());
+    const double* W = params + ng;
+    std::vector<double> h(ORA_ENC_WIDTH), t(ORA_HIDDEN);
+    for (uint32_t i = 0; i < n; ++i) {
+        int in = ORA_ENC_WIDTH;
+        h.assign(enc.begin() + (size_t)i * ORA_ENC_WIDTH, enc.begin() + (size_t)(i + 1) * ORA_ENC_WIDTH);
+        const double* w = W;
+        for (int layer = 0; layer < nl; ++layer) {
+            int o = layer == nl - 1 ? no : ORA_HIDDEN;
+            t.assign(o, 0.0);
+            for (int r = 0; r < o; ++r) {
+                double acc = 0;
+                for (int k = 0; k < in; ++k) acc += w[(size_t)r * in + k] * h[k];
+                t[r] = layer == nl - 1 ? acc : std::max(acc, 0.0);
+            }
+            w += (size_t)o * in;
+            h.assign(t.begin(), t.begin() + o);
+            in = o;
+        }
+        for (int r = 0; r < no; ++r) out[(size_t)i * no + r] = h[r];
+    }
+}
+
+void ora_q(const nrrs_config* cfg, int32_t mode, uint32_t n, const double* src,
+           const nrrs_vertices* v, const float* film, double* q) {
+    const uint32_t S = v->stride, P = cfg->n_pixels;
+    for (uint32_t i = 0; i < n; ++i) {
+        double r;
+        if (mode == NRRS_Q_STATNET_ADRRS) {
+            // integrator.cpp:1246-1255
+            uint32_t px = v->pixel[i];
+            r = 0;
+            for (int k = 0; k < 3; ++k) {
+                double L = std::fmin(std::fmax(src[(size_t)k * n + i], 1e-4), 1e4);
+                r += L * (double)v->thp[k * S + i] / ((double)film[(size_t)k * P + px] + 0.01);
+            }
+            r /= 3.0;
+        } else {
+            r = sigma_d(src[i]);  // integrator.cpp:1203-1219
+        }
+        if (cfg->rrs_weight_window) r = ora_weight_window(r, cfg->weight_window_size);
+        // NaN q becomes the clamp minimum (fmin/fmax semantics)
+        q[i] = std::fmin(std::fmax(r, (double)cfg->rrs_clamp_min), (double)cfg->rrs_clamp_max);
+    }
+}
+
+void ora_q_f32(const nrrs_config* cfg, int32_t mode, uint32_t n, const float* src,
+               const nrrs_vertices* v, const float* film, float* q) {
+    const uint32_t S = v->stride, P = cfg->n_pixels;
+    for (uint32_t i = 0; i < n; ++i) {
+        float r;
+        if (mode == NRRS_Q_STATNET_ADRRS) {
+            uint32_t px = v->pixel[i];
+            float acc = 0.f;
+            for (int k = 0; k < 3; ++k) {
+                float L = std::fmin(std::fmax(src[(size_t)k * n + i], 1e-4f), 1e4f);
+                acc += L * v->thp[k * S + i] / (film[(size_t)k * P + px] + 0.01f);
+            }
+            r = acc / 3.0f;
+        } else {
+            r = sigma_f(src[i]);
+        }
+        if (cfg->rrs_weight_window) {
+            float dm = 2.0f / (1.0f + cfg->weight_window_size), dp = dm * cfg->weight_window_size;
+            r = r < dm ? r / dm : (r > dp ? r / dp : 1.0f);
+        }
+        q[i] = std::fmin(std::fmax(r, cfg->rrs_clamp_min), cfg->rrs_clamp_max);
+    }
+}
+
+int32_t ora_rrs_decide(const nrrs_config* cfg, uint32_t n, const float* q, const uint32_t* stid,
+                       uint32_t n_spec, const uint32_t* spec_stid, uint64_t seed, uint64_t stream,
+                       float* qn, uint8_t* s, uint32_t* survivors, uint32_t capacity,
+                       uint32_t* n_survivors, uint32_t* overflow, double* sum_out) {
+    // Global sum of q (or ceil q) - calcSum2PassAsync (commoncudautilshost.cpp:6-22) -
+    // restated as an exact fixed-point sum with 2^-32 resolution, so the result is
+    // independent of summation order.
+    uint64_t acc = 0;
+    for (uint32_t i = 0; i < n; ++i) {
+        float t = cfg->rrs_use_ceil ? std::ceil(q[i]) : q[i];
+        acc += (uint64_t)((double)t * 4294967296.0);
+    }
+    double sum = (double)acc * 0x1p-32;
+    if (sum_out) *sum_out = sum;
+    float sum_f = (float)sum;
+    float rcp = sum_f > 0.0f ? 1.0f / sum_f : 0.0f;
+    // normalizeFactor (integrator.cpp:1376-1377)
+    float nf = (float)(cfg->n_pixels - n_spec) * rcp;
+    nf = nf * (cfg->rrs_use_ceil ? 1.0f : cfg->rrs_normalize_rate);
+    uint32_t written = 0;
+    uint64_t total = 0;
+    for (uint32_t k = 0; k < n_spec; ++k) {
+        if (written < capacity) survivors[written++] = spec_stid[k];
+        ++total;
+    }
+    Pcg32 rng(seed, stream);
+    for (uint32_t i = 0; i < n; ++i) {
+        // integrator.cpp:1391-1397
+        float r = q[i];
+        r = cfg->rrs_no_upscale ? r * (nf < 1.0f ? nf : 1.0f) : r * nf;
+        if (qn) qn[i] = r;
+        float u = u24(rng.next());  // vertex i draws output number i of the stream
+        // integrator.cpp:1415-1416, 1454: s = floor(q') + [u <= frac(q')]
+        int si = (int)r;
+        float left = r - (float)si;
+        si += (u <= left) ? 1 : 0;
+        if ((uint32_t)si > cfg->split_cap) si = (int)cfg->split_cap;
+        if (s) s[i] = (uint8_t)si;
+        uint32_t id = stid ? stid[i] : i;
+        for (int c = 0; c < si; ++c) {
+            if (written < capacity) survivors[written++] = id;
+            ++total;
+        }
+    }
+    if (n_survivors) *n_survivors = written;
+    uint32_t dropped = (uint32_t)(total - written);
+    if (overflow) *overflow = dropped;
+    return dropped ? NRRS_ERR_OVERFLOW : 0;
+}
+
+void ora_synth_vertices(uint32_t n, uint64_t seed, uint32_t n_pixels, const nrrs_vertices_out* o) {
+    // 64 triangles, corners uniform in [0,1]^3 (stream 0, outputs 0..575)
+    float tri[64 * 9];
+    {
+        Pcg32 r(seed, 0);
+        for (int k = 0; k < 64 * 9; ++k) tri[k] = u24(r.next());
+    }
+    const uint32_t S = o->stride;
+    const float two_pi = 6.283185307179586f;
+    for (uint32_t i = 0; i < n; ++i) {
+        // position: stream 1, 3 outputs per vertex
+        uint32_t a = pcg_at(seed, 1, 3ull * i), b = pcg_at(seed, 1, 3ull * i + 1),
+                 cc = pcg_at(seed, 1, 3ull * i + 2);
+        const float* T = tri + (a & 63u) * 9;
+        float su = std::sqrt(u24(b));
+        float b0 = 1.0f - su, b1 = u24(cc) * su, b2 = 1.0f - b0 - b1;
+        for (int k = 0; k < 3; ++k) o->pos[k * S + i] = b0 * T[k] + b1 * T[3 + k] + b2 * T[6 + k];
+        // direction (stream 2) and normal (stream 3): uniform on the sphere
+        for (int f = 0; f < 2; ++f) {
+            float* dst = f ? o->normal : o->dir;
+            float u1 = u24(pcg_at(seed, 2 + f, 2ull * i)), u2 = u24(pcg_at(seed, 2 + f, 2ull * i + 1));
+            float z = 1.0f - 2.0f * u1;
+            float rr = std::sqrt(std::max(0.0f, 1.0f - z * z));
+            float ph = two_pi * u2;
+            dst[i] = rr * std::cos(ph);
+            dst[S + i] = rr * std::sin(ph);
+            dst[2 * S + i] = z;
+        }
+        o->rough[i] = 0.05f + 0.95f * u24(pcg_at(seed, 4, i));
+        for (int k = 0; k < 3; ++k) {
+            o->albedo_d[k * S + i] = u24(pcg_at(seed, 5, 3ull * i + k));
+            o->albedo_s[k * S + i] = u24(pcg_at(seed, 6, 3ull * i + k));
+            // log-normal(mu=-1, sigma=1): Box-Muller, stream 7, 2 outputs per channel
+            float ua = u24(pcg_at(seed, 7, 6ull * i + 2 * k)), ub = u24(pcg_at(seed, 7, 6ull * i + 2 * k + 1));
+            float nrm = std::sqrt(-2.0f * std::log(1.0f - ua)) * std::cos(two_pi * ub);
+            o->thp[k * S + i] = std::exp(-1.0f + nrm);
+        }
+        o->pixel[i] = i % n_pixels;
+    }
+}
+
+void ora_init_params(const nrrs_config* cfg, int32_t net, uint64_t seed, double grid_amp,
+                     double* params) {
+    uint64_t ng = 0, nm = 0;
+    ora_param_count(cfg, net, &ng, &nm);
+    Pcg32 r(seed, 100 + (uint64_t)net);
+    for (uint64_t k = 0; k < ng; ++k) params[k] = grid_amp * (2.0 * u24(r.next()) - 1.0);
+    int nl, no;
+    mlp_dims(cfg, net, &nl, &no);
+    double* w = params + ng;
+    int in = ORA_ENC_WIDTH;
+    for (int layer = 0; layer < nl; ++layer) {
+        int o = layer == nl - 1 ? no : ORA_HIDDEN;
+        double lim = std::sqrt(6.0 / (double)(in + o));
+        for (int k = 0; k < o * in; ++k) w[k] = lim * (2.0 * u24(r.next()) - 1.0);
+        w += (size_t)o * in;
+        in = o;
+    }
+}
+
+}  // extern "C"

@@ -1,0 +1,95 @@
+/*
+ * CPU oracle of the neural RRS stage - TEST INFRASTRUCTURE ONLY.
+ *
+ * This is the checker, never the product: only tests/, the smoke check and the
+ * CPU-baseline leg of bench.py may load it. It restates the reference algorithms
+ * in plain C++ with FP64 arithmetic (FP32 only where the reference's integer
+ * results depend on FP32 rounding: the hash-grid cell position and the RRS
+ * decision, which consumes the FP32 q it is given).
+ *
+ * Pinning status of each part (how a plausible mistake would be caught):
+ *  - PCG32 + advance: known-answer vectors from the published implementation
+ *    (tests/golden/pcg32_vectors.json).
+ *  - activationSigma, Box-Cox, warp_roughness, normalizeSpatialCoord,
+ *    RRSWeightWindow, normalizeFactor, split: closed forms / known answers
+ *    (sigma(0)=ln 2, BC(1)=0, BC(0)=-2, window(5): d-=1/3, d+=5/3) and
+ *    brute-force expectation E[s] = q' over the 2^24 grid of u.
+ *  - Hash grid: dense levels are injective, hash indices < T, trilinear weights
+ *    sum to 1, a grid holding a linear function reproduces it exactly, and the
+ *    backward is pinned to the forward by an adjoint test.
+ *  - SH degree 4: orthonormality under a Lebedev-free product quadrature.
+ *  - MLP forward/backward: finite differences on FP64.
+ *  - Losses: the NRRS_LL2 / NRRS_RRS loss sources are not part of the reference
+ *    checkout (tcnn submodule absent): these are the build's documented
+ *    restatement, pinned only against their own finite differences
+ *    ("parity unpinned" w.r.t. the reference loss code).
+ *  - Adam: hand-computed 3-step trajectory.
+ */
+#ifndef NRRS_ORACLE_H
+#define NRRS_ORACLE_H
+
+#include <stdint.h>
+
+#include "../include/nrrs_amd.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define ORA_ENC_WIDTH 80
+#define ORA_HIDDEN 64
+
+void ora_pcg32(uint64_t seed, uint64_t stream, uint64_t skip, uint32_t n, uint32_t* out);
+
+uint64_t ora_grid_layout(const nrrs_grid_config* g, uint32_t* res, uint32_t* offset,
+                         uint32_t* size, uint32_t* dense, float* scale);
+
+/* normalizeSpatialCoord + clamp to [0,1], FP32 (xn: [n][3]). */
+void ora_normalize_pos(const float* aabb_min, const float* aabb_max, uint32_t n,
+                       const float* pos, uint32_t stride, float* xn);
+
+/* idx: [n][levels][8] */
+void ora_grid_indices(const nrrs_config* cfg, uint32_t n, const nrrs_vertices* v, uint32_t* idx);
+
+/* Scalar closed forms (exported for known-answer tests). */
+double ora_activation_sigma(double x);
+double ora_box_cox(double x);
+double ora_weight_window(double q, double size);
+void ora_sh4(double x, double y, double z, double* out16);
+
+/* enc: [n][80]; params: [grid | mlp] of that net (FP64); stats: [6][n] or NULL. */
+void ora_encode(const nrrs_config* cfg, int32_t net, uint32_t n, const nrrs_vertices* v,
+                const float* film, const double* stats, const double* params, double* enc);
+
+/* out: [n][n_outputs] */
+void ora_net_forward(const nrrs_config* cfg, int32_t net, uint32_t n, const nrrs_vertices* v,
+                     const float* film, const double* stats, const double* params, double* out);
+
+uint64_t ora_param_count(const nrrs_config* cfg, int32_t net, uint64_t* n_grid, uint64_t* n_mlp);
+
+/* q before normalisation (after activation / ADRRS form, weight window, clamp) in FP64.
+ * src: raw RRSNet output [n] (modes 0/1) or StatNet stats [6][n] (mode 2). */
+void ora_q(const nrrs_config* cfg, int32_t mode, uint32_t n, const double* src,
+           const nrrs_vertices* v, const float* film, double* q);
+
+/* Same, FP32 semantics applied to a FP32 raw/stat input (used to check the GPU epilogue). */
+void ora_q_f32(const nrrs_config* cfg, int32_t mode, uint32_t n, const float* src,
+               const nrrs_vertices* v, const float* film, float* q);
+
+/* The decision from a FP32 clamped q. Returns 0, or NRRS_ERR_OVERFLOW when truncated. */
+int32_t ora_rrs_decide(const nrrs_config* cfg, uint32_t n, const float* q, const uint32_t* stid,
+                       uint32_t n_spec, const uint32_t* spec_stid, uint64_t seed, uint64_t stream,
+                       float* qn, uint8_t* s, uint32_t* survivors, uint32_t capacity,
+                       uint32_t* n_survivors, uint32_t* overflow, double* sum_out);
+
+/* Synthetic vertices (host), same recipe as the device generator. */
+void ora_synth_vertices(uint32_t n, uint64_t seed, uint32_t n_pixels, const nrrs_vertices_out* out);
+
+/* Parameter init of the oracle's own CPU runs (grid U[-a,a], MLP Xavier-uniform). */
+void ora_init_params(const nrrs_config* cfg, int32_t net, uint64_t seed, double grid_amp,
+                     double* params);
+
+#ifdef __cplusplus
+}
+#endif
+#endif
