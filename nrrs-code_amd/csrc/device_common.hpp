@@ -23,7 +23,7 @@ __device__ __forceinline__ uint32_t pcg_output(uint64_t old) {
 // State after seeding (pcg32_srandom_r), jumped `delta` steps ahead.
 __device__ __forceinline__ uint64_t pcg_state_at(uint64_t seed, uint64_t stream, uint64_t delta) {
     const uint64_t inc = (stream << 1u) | 1u;
-    uint64_t state = inc * kPcgMult + inc;  // state = 0; next()
+    uint64_t state = inc;                     // state = 0; next()
     state = (state + seed) * kPcgMult + inc;  // state += seed; next()
     uint64_t acc_mult = 1, acc_plus = 0, cur_mult = kPcgMult, cur_plus = inc;
     while (delta) {
