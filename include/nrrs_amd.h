@@ -204,6 +204,50 @@ typedef struct {
 nrrs_status nrrs_synth_vertices(uint32_t n, uint64_t seed, uint32_t n_pixels,
                                 const nrrs_vertices_out* out, void* stream);
 
+/* ---------------- training (network.cpp:196-295, myNetworkLL2.cpp:41-89) ---------------- */
+
+/* Per-sample side inputs of the RRSNet loss (the arrays NRRSNetwork::getLossHelper hands
+ * out, network.h:49-61), device pointers of length n. */
+typedef struct {
+    const float* ref_mean;      /* mean of the sample's pixel estimate; NULL = from the film */
+    const float* error;         /* relative error of the sample's pixel                     */
+    const float* sample_weight; /* 1 / (training samples of the pixel)                      */
+    const float* q_target;      /* q used when the path was traced; -1 = duplicate node      */
+    float error_sum;            /* sum of the per-pixel error over the film                 */
+} nrrs_rrs_side;
+
+/* One training step: StatNet (net_mask bit 0) on the NRRS_LL2 loss against target_L
+ * [3][n], then RRSNet (bit 1) on the NRRS_RRS loss, its StatNet-statistics inputs taken
+ * from the StatNet forward of the same batch; backward; Adam on each trained net.
+ * loss_dev (optional, device, 2 floats) receives the two batch losses. */
+nrrs_status nrrs_train_step(nrrs_ctx* ctx, uint32_t net_mask, uint32_t n, const nrrs_vertices* v,
+                            const float* film_dev, const float* target_L_dev, const nrrs_rrs_side* side,
+                            float* loss_dev, void* stream);
+
+/* Loss kind of nrrs_backward: dL/dy given, or one of the two losses. */
+typedef enum { NRRS_LOSS_EXTERNAL = 0, NRRS_LOSS_LL2 = 1, NRRS_LOSS_RRS = 2 } nrrs_loss_kind;
+
+/* Forward + backward of one net, accumulating dL/dparams (scaled by loss_scale) into the
+ * context's gradient buffer. kind EXTERNAL: dy_dev [n_out][n] is dL/dy. LL2: target_L_dev.
+ * RRS: side + stats_dev (StatNet statistics [6][n], also the RRSNet input). loss_dev: 1 float. */
+nrrs_status nrrs_backward(nrrs_ctx* ctx, int32_t net, uint32_t n, const nrrs_vertices* v,
+                          const float* film_dev, const float* stats_dev, int32_t kind,
+                          const float* dy_dev, const float* target_L_dev, const nrrs_rrs_side* side,
+                          float loss_scale, float* loss_dev, void* stream);
+
+/* Gradient buffer of one net: zero it / copy it to the host ([grid | mlp], synchronous). */
+nrrs_status nrrs_grads_zero(nrrs_ctx* ctx, int32_t net, void* stream);
+nrrs_status nrrs_grads_get(nrrs_ctx* ctx, int32_t net, float* host_dst);
+
+/* Adam on the FP32 masters from the gradient buffer (divided by the current loss scale);
+ * refreshes the FP16/BF16 shadows and zeroes the gradient buffer. With dynamic loss scaling
+ * (FP16/BF16 contexts) a non-finite gradient skips the update and halves the scale. */
+nrrs_status nrrs_adam_step(nrrs_ctx* ctx, int32_t net, void* stream);
+
+/* Training state of one net: Adam step count, current loss scale, skipped steps. Synchronous. */
+nrrs_status nrrs_train_state(nrrs_ctx* ctx, int32_t net, uint32_t* step, float* loss_scale,
+                             uint32_t* skipped);
+
 #ifdef __cplusplus
 }
 #endif

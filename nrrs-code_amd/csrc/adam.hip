@@ -1,0 +1,78 @@
+// Adam on the FP32 master parameters (Kingma & Ba 2015; L2 regularisation added
+// to the gradient), with dynamic loss scaling for FP16/BF16 contexts: a gradient
+// with a NaN/Inf skips the step and halves the scale; `growth_interval` finite
+// steps in a row double it. Everything stays on the device (no host sync).
+#include "device_common.hpp"
+#include "train_common.hpp"
+
+namespace nrrs {
+namespace {
+
+__global__ void k_check_finite(const float* __restrict__ g, uint64_t n, uint32_t* flag) {
+    bool bad = false;
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x)
+        bad |= !isfinite(g[i]);
+    if (__any(bad) && (threadIdx.x & 63) == 0) atomicOr(flag, 1u);
+}
+
+__global__ void k_adam(AdamArgs a) {
+    const uint32_t skip = a.state[3];
+    const float inv_s = 1.0f / *a.loss_scale;
+    const uint32_t t = a.state[0] + 1;
+    const float c1 = 1.0f - powf(a.beta1, (float)t), c2 = 1.0f - powf(a.beta2, (float)t);
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < a.n; i += (uint64_t)gridDim.x * blockDim.x) {
+        float g = a.grad[i];
+        a.grad[i] = 0.0f;
+        if (skip) continue;
+        float p = a.params[i];
+        g = g * inv_s + a.l2 * p;
+        float m = a.beta1 * a.m[i] + (1.0f - a.beta1) * g;
+        float v = a.beta2 * a.v[i] + (1.0f - a.beta2) * g * g;
+        a.m[i] = m;
+        a.v[i] = v;
+        p -= a.lr * (m / c1) / (sqrtf(v / c2) + a.eps);
+        a.params[i] = p;
+        if (a.grid_lp && i < a.n_grid) {
+            a.grid_lp[i] = a.lp_precision == NRRS_BF16 ? __builtin_bit_cast(uint16_t, (__bf16)p)
+                                                        : __builtin_bit_cast(uint16_t, (_Float16)p);
+        }
+    }
+}
+
+__global__ void k_adam_finish(AdamArgs a) {
+    uint32_t* st = a.state;
+    if (st[3]) {
+        st[2] += 1;
+        st[1] = 0;
+        if (a.dynamic_scale) *a.loss_scale = fmaxf(*a.loss_scale * 0.5f, 1.0f);
+    } else {
+        st[0] += 1;
+        st[1] += 1;
+        if (a.dynamic_scale && st[1] >= a.growth_interval) {
+            *a.loss_scale = fminf(*a.loss_scale * 2.0f, 65536.0f * 65536.0f);
+            st[1] = 0;
+        }
+    }
+    st[3] = 0;
+}
+
+__global__ void k_fill(float* p, float v, uint64_t n) {
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x)
+        p[i] = v;
+}
+
+}  // namespace
+
+hipError_t launch_adam(const AdamArgs& a, hipStream_t s) {
+    hipLaunchKernelGGL(k_check_finite, dim3(1024), dim3(256), 0, s, a.grad, a.n, a.state + 3);
+    hipLaunchKernelGGL(k_adam, dim3(2048), dim3(256), 0, s, a);
+    hipLaunchKernelGGL(k_adam_finish, dim3(1), dim3(1), 0, s, a);
+    return hipGetLastError();
+}
+
+hipError_t launch_fill_f32(float* p, float v, uint64_t n, hipStream_t s) {
+    if (n) hipLaunchKernelGGL(k_fill, dim3(1024), dim3(256), 0, s, p, v, n);
+    return hipGetLastError();
+}
+
+}  // namespace nrrs

@@ -10,6 +10,7 @@
 #include <vector>
 
 #include "host_common.hpp"
+#include "train_common.hpp"
 
 using namespace nrrs;
 
@@ -157,13 +158,152 @@ bool vertices_ok(const nrrs_vertices* v, uint32_t n) {
            v->pixel && v->stride >= n;
 }
 
+// Low-precision shadows (grid table and packed MFMA tiles) from the FP32 masters.
+nrrs_status refresh_lp(nrrs_ctx* c, int net, hipStream_t s) {
+    if (c->cfg.precision == NRRS_FP32) return NRRS_OK;
+    NetState& ns = c->net[net];
+    return hip_status(launch_pack_lp(c->cfg.precision, ns.grid_f32, ns.n_grid, ns.mlp_f32, ns.n_layers, ns.n_out,
+                                     ns.grid_lp, ns.mlp_lp, s));
+}
+
+// One fused encode + MLP launch of `net` in the context's precision.
+hipError_t run_net(nrrs_ctx* c, int net, const EncodeArgs& a, int epi, uint32_t n, const EpiArgs& ep,
+                   hipStream_t s) {
+    const NetState& ns = c->net[net];
+    if (c->cfg.precision == NRRS_FP32)
+        return launch_net_fp32(a, net, ns.n_layers, ns.n_out, epi, n, ns.grid_f32, ns.mlp_f32, ep, s);
+    return launch_net_mfma(c->cfg.precision, a, net, ns.n_layers, ns.n_out, epi, n, ns.grid_lp, ns.mlp_lp, ep, s);
+}
+
+// Packed training tiles from the FP32 masters (after every parameter change).
+nrrs_status refresh_train(nrrs_ctx* c, int net, hipStream_t s) {
+    NetState& ns = c->net[net];
+    if (!ns.train_ready) return NRRS_OK;
+    return hip_status(launch_pack_train(c->cfg.precision, ns.mlp_f32, ns.n_layers, ns.n_out, ns.w_fwd, ns.w_bwd, s));
+}
+
+// Training buffers of one net, allocated on first use.
+nrrs_status ensure_train(nrrs_ctx* c, int net) {
+    NetState& ns = c->net[net];
+    if (ns.train_ready) return NRRS_OK;
+    const uint64_t np = ns.n_grid + ns.n_mlp;
+    const size_t ob = train_op_bytes(c->cfg.precision);
+    NRRS_TRY(dmalloc(&ns.grad, np));
+    NRRS_TRY(dmalloc(&ns.adam_m, np));
+    NRRS_TRY(dmalloc(&ns.adam_v, np));
+    NRRS_TRY(dmalloc(reinterpret_cast<uint8_t**>(&ns.w_fwd), (size_t)train_tiles_fwd(ns.n_layers) * 64 * ob));
+    NRRS_TRY(dmalloc(reinterpret_cast<uint8_t**>(&ns.w_bwd), (size_t)train_tiles_bwd(ns.n_layers) * 64 * ob));
+    NRRS_TRY(dmalloc(&ns.dw_partial, (size_t)c->max_train_blocks * ns.n_mlp));
+    NRRS_TRY(dmalloc(&ns.loss_scale, 1));
+    NRRS_TRY(dmalloc(&ns.state, 4));
+    NRRS_HIP(hipMemset(ns.grad, 0, np * 4));
+    NRRS_HIP(hipMemset(ns.adam_m, 0, np * 4));
+    NRRS_HIP(hipMemset(ns.adam_v, 0, np * 4));
+    NRRS_HIP(hipMemset(ns.state, 0, 16));
+    const float s0 = c->cfg.precision == NRRS_FP32 ? 1.0f : c->cfg.loss_scale_init;
+    NRRS_HIP(hipMemcpy(ns.loss_scale, &s0, 4, hipMemcpyHostToDevice));
+    if (!c->train_stats) NRRS_TRY(dmalloc(&c->train_stats, (size_t)6 * c->cfg.max_train));
+    if (!c->scale_tmp) NRRS_TRY(dmalloc(&c->scale_tmp, 1));
+    ns.train_ready = true;
+    NRRS_TRY(refresh_train(c, net, nullptr));
+    return hip_status(hipDeviceSynchronize());
+}
+
+// Forward + loss + backward of one net into its gradient buffer.
+nrrs_status run_backward(nrrs_ctx* c, int net, uint32_t n, const nrrs_vertices& v, const float* film,
+                         const float* stats, uint32_t stats_stride, int kind, const float* dy,
+                         const float* target_L, const nrrs_rrs_side* side, const float* scale, float* y_out,
+                         uint32_t out_stride, float* loss_out, hipStream_t s) {
+    NetState& ns = c->net[net];
+    const nrrs_config& cfg = c->cfg;
+    TrainArgs t{};
+    t.a = encode_args(c, v, film, stats, stats_stride);
+    t.net = net;
+    t.n = n;
+    t.table_f32 = cfg.precision == NRRS_FP32 ? ns.grid_f32 : nullptr;
+    t.table_lp = reinterpret_cast<const uint32_t*>(ns.grid_lp);
+    t.w_fwd = ns.w_fwd;
+    t.w_bwd = ns.w_bwd;
+    t.bwd_hidden_base = 4;
+    t.bwd_last_base = 4 + (ns.n_layers - 2) * 8;
+    t.loss_scale = scale;
+    t.dy_ext = dy;
+    t.target_L = target_L;
+    if (side) {
+        t.ref_mean = side->ref_mean;
+        t.error = side->error;
+        t.sample_weight = side->sample_weight;
+        t.q_target = side->q_target;
+        t.ebar = std::fmax(side->error_sum / (float)cfg.n_pixels, 1e-4f);
+    }
+    for (int k = 0; k < 4; ++k) t.gamma[k] = cfg.gamma[k];
+    t.clamp_on = cfg.loss_clamp_on;
+    t.clamp_max = cfg.loss_clamp_max;
+    t.ll2_step = cfg.ll2_step;
+    t.rrs_step = cfg.rrs_step;
+    const int active = kind == NRRS_LOSS_LL2 ? (cfg.ll2_step == 0 ? 3 : 6) : 1;
+    t.inv_norm = n ? 1.0f / ((float)n * (float)active) : 0.0f;
+    t.y_out = y_out;
+    t.out_stride = out_stride;
+    t.grad_grid = ns.grad;
+    t.dw_partial = ns.dw_partial;
+    t.n_mlp = (uint32_t)ns.n_mlp;
+    t.loss_out = loss_out;
+    uint32_t iters = (n + 127) / 128;
+    uint32_t blocks = iters < c->max_train_blocks ? iters : c->max_train_blocks;
+    if (!blocks) return NRRS_OK;
+    NRRS_HIP(launch_train(cfg.precision, t, ns.n_layers, ns.n_out, kind, blocks, s));
+    return hip_status(launch_reduce_partials(ns.dw_partial, blocks, (uint32_t)ns.n_mlp, ns.grad + ns.n_grid, s));
+}
+
+nrrs_status run_adam(nrrs_ctx* c, int net, hipStream_t s) {
+    NetState& ns = c->net[net];
+    const nrrs_config& cfg = c->cfg;
+    AdamArgs a{};
+    a.params = ns.params;
+    a.grad = ns.grad;
+    a.m = ns.adam_m;
+    a.v = ns.adam_v;
+    a.n = ns.n_grid + ns.n_mlp;
+    a.n_grid = ns.n_grid;
+    a.grid_lp = ns.grid_lp;
+    a.lp_precision = cfg.precision;
+    a.lr = net == NRRS_STATNET ? cfg.lr_statnet : cfg.lr_rrsnet;
+    a.beta1 = cfg.adam_beta1;
+    a.beta2 = cfg.adam_beta2;
+    a.eps = cfg.adam_eps;
+    a.l2 = cfg.adam_l2_reg;
+    a.loss_scale = ns.loss_scale;
+    a.state = ns.state;
+    a.dynamic_scale = cfg.precision != NRRS_FP32;
+    a.growth_interval = cfg.loss_scale_growth_interval ? cfg.loss_scale_growth_interval : 1000;
+    NRRS_HIP(launch_adam(a, s));
+    // MLP shadows (the grid shadow is refreshed inside the Adam kernel)
+    if (cfg.precision != NRRS_FP32)
+        NRRS_HIP(launch_pack_lp(cfg.precision, nullptr, 0, ns.mlp_f32, ns.n_layers, ns.n_out, nullptr, ns.mlp_lp, s));
+    return refresh_train(c, net, s);
+}
+
+bool side_ok(const nrrs_rrs_side* side) {
+    return side && side->error && side->sample_weight && side->q_target;
+}
+
 void free_ctx(nrrs_ctx* c) {
     for (auto& ns : c->net) {
-        hipFree(ns.grid_f32);
-        hipFree(ns.mlp_f32);
+        hipFree(ns.params);
         hipFree(ns.grid_lp);
         hipFree(ns.mlp_lp);
+        hipFree(ns.grad);
+        hipFree(ns.adam_m);
+        hipFree(ns.adam_v);
+        hipFree(ns.w_fwd);
+        hipFree(ns.w_bwd);
+        hipFree(ns.dw_partial);
+        hipFree(ns.loss_scale);
+        hipFree(ns.state);
     }
+    hipFree(c->train_stats);
+    hipFree(c->scale_tmp);
     hipFree(c->sum_dev);
     hipFree(c->tile_sums);
     hipFree(c->tile_offsets);
@@ -261,7 +401,8 @@ nrrs_status nrrs_create(const nrrs_config* cfg, nrrs_ctx** out) {
     if (!cfg || !out) return NRRS_ERR_ARG;
     *out = nullptr;
     if (!valid_shapes(*cfg) || cfg->train_sigma) return NRRS_ERR_UNSUPPORTED;
-    if (cfg->precision != NRRS_FP32) return NRRS_ERR_UNSUPPORTED;
+    if (cfg->precision != NRRS_FP32 && cfg->precision != NRRS_FP16 && cfg->precision != NRRS_BF16)
+        return NRRS_ERR_UNSUPPORTED;
     if (cfg->max_queue == 0 || cfg->n_pixels == 0 || cfg->split_cap == 0 || cfg->split_cap > 255 ||
         !(cfg->rrs_clamp_min > 0.0f) || !(cfg->rrs_clamp_max >= cfg->rrs_clamp_min))
         return NRRS_ERR_ARG;
@@ -300,12 +441,19 @@ nrrs_status nrrs_create(const nrrs_config* cfg, nrrs_ctx** out) {
         ns.n_out = (int)m.n_outputs;
         ns.n_grid = n_grid;
         ns.n_mlp = mlp_count(m);
-        if ((st = dmalloc(&ns.grid_f32, ns.n_grid)) != NRRS_OK) break;
-        if ((st = dmalloc(&ns.mlp_f32, ns.n_mlp)) != NRRS_OK) break;
+        if ((st = dmalloc(&ns.params, ns.n_grid + ns.n_mlp)) != NRRS_OK) break;
+        ns.grid_f32 = ns.params;
+        ns.mlp_f32 = ns.params + ns.n_grid;
+        if (cfg->precision != NRRS_FP32) {
+            if ((st = dmalloc(&ns.grid_lp, ns.n_grid)) != NRRS_OK) break;
+            if ((st = dmalloc(&ns.mlp_lp, mfma_packed_halves(ns.n_layers))) != NRRS_OK) break;
+        }
         init_params(*cfg, net, n_grid, host);
         st = hip_status(hipMemcpy(ns.grid_f32, host.data(), ns.n_grid * 4, hipMemcpyHostToDevice));
         if (st == NRRS_OK)
             st = hip_status(hipMemcpy(ns.mlp_f32, host.data() + n_grid, ns.n_mlp * 4, hipMemcpyHostToDevice));
+        if (st == NRRS_OK) st = refresh_lp(c, net, nullptr);
+        if (st == NRRS_OK) st = hip_status(hipDeviceSynchronize());
     }
     c->max_tiles = (cfg->max_queue + 255) / 256;
     if (st == NRRS_OK) st = dmalloc(&c->sum_dev, 1);
@@ -351,7 +499,9 @@ nrrs_status nrrs_params_set(nrrs_ctx* ctx, int32_t net, const float* host_src) {
     NRRS_HIP(hipDeviceSynchronize());
     NRRS_HIP(hipMemcpy(ns.grid_f32, host_src, ns.n_grid * 4, hipMemcpyHostToDevice));
     NRRS_HIP(hipMemcpy(ns.mlp_f32, host_src + ns.n_grid, ns.n_mlp * 4, hipMemcpyHostToDevice));
-    return NRRS_OK;
+    NRRS_TRY(refresh_lp(ctx, net, nullptr));
+    NRRS_TRY(refresh_train(ctx, net, nullptr));
+    return hip_status(hipDeviceSynchronize());
 }
 
 nrrs_status nrrs_encode(nrrs_ctx* ctx, int32_t net, uint32_t n, const nrrs_vertices* v,
@@ -372,25 +522,21 @@ nrrs_status nrrs_grid_indices(nrrs_ctx* ctx, uint32_t n, const nrrs_vertices* v,
 nrrs_status nrrs_statnet_infer(nrrs_ctx* ctx, uint32_t n, const nrrs_vertices* v, float* stats_dev,
                                void* stream) {
     if (!ctx || !vertices_ok(v, n) || !stats_dev) return NRRS_ERR_ARG;
-    const NetState& ns = ctx->net[NRRS_STATNET];
     EncodeArgs a = encode_args(ctx, *v, nullptr, nullptr, 0);
     EpiArgs ep{};
     ep.out = stats_dev;
     ep.out_stride = n;
-    return hip_status(launch_net_fp32(a, NRRS_STATNET, ns.n_layers, ns.n_out, EPI_RAW, n, ns.grid_f32,
-                                      ns.mlp_f32, ep, (hipStream_t)stream));
+    return hip_status(run_net(ctx, NRRS_STATNET, a, EPI_RAW, n, ep, (hipStream_t)stream));
 }
 
 nrrs_status nrrs_rrsnet_infer(nrrs_ctx* ctx, uint32_t n, const nrrs_vertices* v, const float* film_dev,
                               const float* stats_dev, float* raw_dev, void* stream) {
     if (!ctx || !vertices_ok(v, n) || !film_dev || !raw_dev) return NRRS_ERR_ARG;
-    const NetState& ns = ctx->net[NRRS_RRSNET];
     EncodeArgs a = encode_args(ctx, *v, film_dev, stats_dev, n);
     EpiArgs ep{};
     ep.out = raw_dev;
     ep.out_stride = n;
-    return hip_status(launch_net_fp32(a, NRRS_RRSNET, ns.n_layers, ns.n_out, EPI_RAW, n, ns.grid_f32,
-                                      ns.mlp_f32, ep, (hipStream_t)stream));
+    return hip_status(run_net(ctx, NRRS_RRSNET, a, EPI_RAW, n, ep, (hipStream_t)stream));
 }
 
 static nrrs_status check_decide_args(nrrs_ctx* ctx, const nrrs_decide_args* a) {
@@ -409,12 +555,9 @@ nrrs_status nrrs_rrs_infer_decide(nrrs_ctx* ctx, const nrrs_decide_args* a, void
     ep.q = a->q_dev;
     ep.sum = ctx->sum_dev;
     ep.d = ctx->dec;
-    const NetState& sn = ctx->net[NRRS_STATNET];
-    const NetState& rn = ctx->net[NRRS_RRSNET];
     if (a->q_mode == NRRS_Q_STATNET_ADRRS) {
         EncodeArgs ea = encode_args(ctx, a->v, a->film_dev, nullptr, 0);
-        NRRS_HIP(launch_net_fp32(ea, NRRS_STATNET, sn.n_layers, sn.n_out, EPI_ADRRS, a->n, sn.grid_f32,
-                                 sn.mlp_f32, ep, s));
+        NRRS_HIP(run_net(ctx, NRRS_STATNET, ea, EPI_ADRRS, a->n, ep, s));
     } else if (a->q_mode == NRRS_Q_RRSNET || a->q_mode == NRRS_Q_RRSNET_CHAINED) {
         const float* stats = nullptr;
         if (a->q_mode == NRRS_Q_RRSNET_CHAINED) {
@@ -422,13 +565,11 @@ nrrs_status nrrs_rrs_infer_decide(nrrs_ctx* ctx, const nrrs_decide_args* a, void
             EpiArgs es{};
             es.out = ctx->stats_scratch;
             es.out_stride = ctx->cfg.max_queue;
-            NRRS_HIP(launch_net_fp32(ea, NRRS_STATNET, sn.n_layers, sn.n_out, EPI_RAW, a->n, sn.grid_f32,
-                                     sn.mlp_f32, es, s));
+            NRRS_HIP(run_net(ctx, NRRS_STATNET, ea, EPI_RAW, a->n, es, s));
             stats = ctx->stats_scratch;
         }
         EncodeArgs ea = encode_args(ctx, a->v, a->film_dev, stats, ctx->cfg.max_queue);
-        NRRS_HIP(launch_net_fp32(ea, NRRS_RRSNET, rn.n_layers, rn.n_out, EPI_RRS, a->n, rn.grid_f32,
-                                 rn.mlp_f32, ep, s));
+        NRRS_HIP(run_net(ctx, NRRS_RRSNET, ea, EPI_RRS, a->n, ep, s));
     } else {
         return NRRS_ERR_ARG;
     }
@@ -448,6 +589,101 @@ nrrs_status nrrs_synth_vertices(uint32_t n, uint64_t seed, uint32_t n_pixels, co
         !o->pixel || o->stride < n || n_pixels == 0)
         return NRRS_ERR_ARG;
     return hip_status(launch_synth(n, seed, n_pixels, *o, (hipStream_t)stream));
+}
+
+nrrs_status nrrs_backward(nrrs_ctx* ctx, int32_t net, uint32_t n, const nrrs_vertices* v, const float* film_dev,
+                          const float* stats_dev, int32_t kind, const float* dy_dev, const float* target_L_dev,
+                          const nrrs_rrs_side* side, float loss_scale, float* loss_dev, void* stream) {
+    if (!ctx || (net != 0 && net != 1) || !vertices_ok(v, n)) return NRRS_ERR_ARG;
+    if (n > ctx->cfg.max_train) return NRRS_ERR_SHAPE;
+    if (kind == NRRS_LOSS_EXTERNAL) {
+        if (!dy_dev) return NRRS_ERR_ARG;
+    } else if (kind == NRRS_LOSS_LL2) {
+        if (net != NRRS_STATNET || !target_L_dev) return NRRS_ERR_ARG;
+    } else if (kind == NRRS_LOSS_RRS) {
+        if (net != NRRS_RRSNET || !side_ok(side) || !stats_dev) return NRRS_ERR_ARG;
+    } else {
+        return NRRS_ERR_ARG;
+    }
+    if (net == NRRS_RRSNET && !film_dev) return NRRS_ERR_ARG;
+    NRRS_TRY(ensure_train(ctx, net));
+    hipStream_t s = (hipStream_t)stream;
+    NRRS_HIP(launch_fill_f32(ctx->scale_tmp, loss_scale, 1, s));
+    if (loss_dev) NRRS_HIP(launch_fill_f32(loss_dev, 0.0f, 1, s));
+    return run_backward(ctx, net, n, *v, film_dev, net == NRRS_RRSNET ? stats_dev : nullptr, n, kind, dy_dev,
+                        target_L_dev, side, ctx->scale_tmp, nullptr, 0, loss_dev, s);
+}
+
+nrrs_status nrrs_grads_zero(nrrs_ctx* ctx, int32_t net, void* stream) {
+    if (!ctx || (net != 0 && net != 1)) return NRRS_ERR_ARG;
+    NRRS_TRY(ensure_train(ctx, net));
+    NetState& ns = ctx->net[net];
+    return hip_status(launch_fill_f32(ns.grad, 0.0f, ns.n_grid + ns.n_mlp, (hipStream_t)stream));
+}
+
+nrrs_status nrrs_grads_get(nrrs_ctx* ctx, int32_t net, float* host_dst) {
+    if (!ctx || (net != 0 && net != 1) || !host_dst) return NRRS_ERR_ARG;
+    NRRS_TRY(ensure_train(ctx, net));
+    NetState& ns = ctx->net[net];
+    NRRS_HIP(hipDeviceSynchronize());
+    return hip_status(hipMemcpy(host_dst, ns.grad, (ns.n_grid + ns.n_mlp) * 4, hipMemcpyDeviceToHost));
+}
+
+nrrs_status nrrs_adam_step(nrrs_ctx* ctx, int32_t net, void* stream) {
+    if (!ctx || (net != 0 && net != 1)) return NRRS_ERR_ARG;
+    NRRS_TRY(ensure_train(ctx, net));
+    return run_adam(ctx, net, (hipStream_t)stream);
+}
+
+nrrs_status nrrs_train_state(nrrs_ctx* ctx, int32_t net, uint32_t* step, float* loss_scale, uint32_t* skipped) {
+    if (!ctx || (net != 0 && net != 1)) return NRRS_ERR_ARG;
+    NRRS_TRY(ensure_train(ctx, net));
+    NetState& ns = ctx->net[net];
+    uint32_t st[4];
+    float sc;
+    NRRS_HIP(hipDeviceSynchronize());
+    NRRS_HIP(hipMemcpy(st, ns.state, 16, hipMemcpyDeviceToHost));
+    NRRS_HIP(hipMemcpy(&sc, ns.loss_scale, 4, hipMemcpyDeviceToHost));
+    if (step) *step = st[0];
+    if (loss_scale) *loss_scale = sc;
+    if (skipped) *skipped = st[2];
+    return NRRS_OK;
+}
+
+nrrs_status nrrs_train_step(nrrs_ctx* ctx, uint32_t net_mask, uint32_t n, const nrrs_vertices* v,
+                            const float* film_dev, const float* target_L_dev, const nrrs_rrs_side* side,
+                            float* loss_dev, void* stream) {
+    if (!ctx || !vertices_ok(v, n) || !(net_mask & 3u)) return NRRS_ERR_ARG;
+    if (n > ctx->cfg.max_train) return NRRS_ERR_SHAPE;
+    if ((net_mask & 1u) && !target_L_dev) return NRRS_ERR_ARG;
+    if ((net_mask & 2u) && (!side_ok(side) || !film_dev)) return NRRS_ERR_ARG;
+    hipStream_t s = (hipStream_t)stream;
+    if (loss_dev) NRRS_HIP(launch_fill_f32(loss_dev, 0.0f, 2, s));
+    const uint32_t ms = ctx->cfg.max_train;
+    if (net_mask & 1u) {
+        NRRS_TRY(ensure_train(ctx, NRRS_STATNET));
+        NetState& ns = ctx->net[NRRS_STATNET];
+        // StatNet step (myNetworkLL2.cpp:41-89); its forward outputs feed RRSNet
+        NRRS_TRY(run_backward(ctx, NRRS_STATNET, n, *v, film_dev, nullptr, 0, NRRS_LOSS_LL2, nullptr, target_L_dev,
+                              nullptr, ns.loss_scale, ctx->train_stats, ms, loss_dev, s));
+        NRRS_TRY(run_adam(ctx, NRRS_STATNET, s));
+    }
+    if (net_mask & 2u) {
+        NRRS_TRY(ensure_train(ctx, NRRS_RRSNET));
+        if (!(net_mask & 1u)) {
+            if (!ctx->train_stats) NRRS_TRY(dmalloc(&ctx->train_stats, (size_t)6 * ms));
+            EncodeArgs ea = encode_args(ctx, *v, nullptr, nullptr, 0);
+            EpiArgs es{};
+            es.out = ctx->train_stats;
+            es.out_stride = ms;
+            NRRS_HIP(run_net(ctx, NRRS_STATNET, ea, EPI_RAW, n, es, s));
+        }
+        NetState& ns = ctx->net[NRRS_RRSNET];
+        NRRS_TRY(run_backward(ctx, NRRS_RRSNET, n, *v, film_dev, ctx->train_stats, ms, NRRS_LOSS_RRS, nullptr,
+                              nullptr, side, ns.loss_scale, nullptr, 0, loss_dev ? loss_dev + 1 : nullptr, s));
+        NRRS_TRY(run_adam(ctx, NRRS_RRSNET, s));
+    }
+    return NRRS_OK;
 }
 
 }  // extern "C"

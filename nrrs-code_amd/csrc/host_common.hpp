@@ -62,10 +62,21 @@ enum Epilogue { EPI_RAW = 0, EPI_RRS = 1, EPI_ADRRS = 2 };
 struct NetState {
     int n_layers = 0, n_out = 0;
     uint64_t n_grid = 0, n_mlp = 0;  // FP32 parameter counts
-    float* grid_f32 = nullptr;       // master grid parameters [entries][2]
-    float* mlp_f32 = nullptr;        // master MLP weights, row-major per layer
+    float* params = nullptr;         // FP32 masters [grid | mlp]
+    float* grid_f32 = nullptr;       // = params: grid [entries][2]
+    float* mlp_f32 = nullptr;        // = params + n_grid: MLP weights, row-major W_l[out][in]
     uint16_t* grid_lp = nullptr;     // FP16/BF16 shadow of the grid
     uint16_t* mlp_lp = nullptr;      // FP16/BF16 shadow of the MLP (MFMA tile order)
+    // training state, allocated on first use
+    bool train_ready = false;
+    float* grad = nullptr;           // [grid | mlp]
+    float* adam_m = nullptr;
+    float* adam_v = nullptr;
+    void* w_fwd = nullptr;           // packed training tiles (forward / transposed)
+    void* w_bwd = nullptr;
+    float* dw_partial = nullptr;     // [max_train_blocks][n_mlp]
+    float* loss_scale = nullptr;     // device scalar
+    uint32_t* state = nullptr;       // device: step, good steps, skipped, non-finite flag
 };
 
 }  // namespace nrrs
@@ -85,6 +96,10 @@ struct nrrs_ctx {
     float* q_scratch = nullptr;             // [max_queue]
     uint8_t* s_scratch = nullptr;           // [max_queue]
     uint32_t max_tiles = 0;
+    // training scratch
+    float* train_stats = nullptr;           // [6][max_train] StatNet outputs of the batch
+    float* scale_tmp = nullptr;             // device scalar for nrrs_backward's loss scale
+    uint32_t max_train_blocks = 512;
 };
 
 namespace nrrs {
@@ -95,6 +110,13 @@ hipError_t launch_encode(const EncodeArgs& a, int net, uint32_t n, const float* 
 hipError_t launch_grid_indices(const EncodeArgs& a, uint32_t n, uint32_t* idx, hipStream_t s);
 hipError_t launch_net_fp32(const EncodeArgs& a, int net, int n_layers, int n_out, int epi, uint32_t n,
                            const float* table, const float* W, const EpiArgs& ep, hipStream_t s);
+
+size_t mfma_packed_halves(int n_layers);
+hipError_t launch_pack_lp(int precision, const float* grid, uint64_t n_grid, const float* W, int n_layers,
+                          int n_out, uint16_t* grid_lp, uint16_t* W_lp, hipStream_t s);
+hipError_t launch_net_mfma(int precision, const EncodeArgs& a, int net, int n_layers, int n_out, int epi,
+                           uint32_t n, const uint16_t* table, const uint16_t* Wp, const EpiArgs& ep,
+                           hipStream_t s);
 
 hipError_t launch_zero_u64(unsigned long long* p, hipStream_t s);
 hipError_t launch_qsum(nrrs_ctx* ctx, const float* q, uint32_t n, hipStream_t s);

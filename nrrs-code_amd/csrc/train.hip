@@ -1,0 +1,626 @@
+// Training: fused forward, loss, backward and weight-gradient kernel on MFMA, for
+// FP32 (v_mfma_f32_32x32x2_f32), FP16 and BF16 (v_mfma_f32_32x32x16_*) operands
+// with FP32 accumulation.
+//
+// One workgroup = 4 wavefronts = 128 vertices per iteration (persistent loop).
+// Operand layout is the one of the inference kernel: lane l owns vertex l % 32
+// and element j of every 16-wide K chunk with k = 8(l/32) + j; layer outputs are
+// chained without shuffles through the permuted column order
+// p(kb,h,j) = 16kb + 8(j/4) + 4h + j%4.
+//
+// Backward sweep, layer l = L-1 .. 0:
+//   1. recompute the layer input h_l from the encoding (only the encoding and one
+//      layer of activations are ever live: activations are recomputed, not stored);
+//   2. stage delta_l and h_l of the 128 vertices in LDS as [neuron][vertex];
+//   3. dW_l += delta_l h_l^T on MFMA with the vertices as the contraction index,
+//      each wave owning a fixed subset of the 32x32 dW tiles in its registers for
+//      the whole kernel (written once per workgroup, reduced deterministically);
+//   4. delta_{l-1} = (W_l^T delta_l) * [h_l > 0] on MFMA with pre-packed W^T tiles;
+//      at l = 0 the 32 grid channels of dL/denc are scatter-added into the table.
+#include "device_common.hpp"
+#include "train_common.hpp"
+
+namespace nrrs {
+namespace {
+
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef _Float16 f16x8 __attribute__((ext_vector_type(8)));
+typedef float f32x8 __attribute__((ext_vector_type(8)));
+typedef float f32x16 __attribute__((ext_vector_type(16)));
+typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
+
+// Operand policies: 8 values per lane per 16-deep K chunk.
+struct TBF16 {
+    static constexpr bool kF32Table = false;
+    typedef bf16x8 op;
+    typedef __bf16 elem;
+    __device__ static op pack(const float* x) {
+        op v;
+#pragma unroll
+        for (int j = 0; j < 8; ++j) v[j] = (__bf16)x[j];
+        return v;
+    }
+    __device__ static float get(const op& v, int j) { return (float)v[j]; }
+    __device__ static f32x16 mma16(op a, op b, f32x16 c) {
+        return __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, b, c, 0, 0, 0);
+    }
+    __device__ static float2 pair(uint32_t u) {
+        return make_float2(__uint_as_float(u << 16), __uint_as_float(u & 0xffff0000u));
+    }
+};
+struct TF16 {
+    static constexpr bool kF32Table = false;
+    typedef f16x8 op;
+    typedef _Float16 elem;
+    __device__ static op pack(const float* x) {
+        op v;
+#pragma unroll
+        for (int j = 0; j < 8; ++j) v[j] = (_Float16)x[j];
+        return v;
+    }
+    __device__ static float get(const op& v, int j) { return (float)v[j]; }
+    __device__ static f32x16 mma16(op a, op b, f32x16 c) {
+        return __builtin_amdgcn_mfma_f32_32x32x16_f16(a, b, c, 0, 0, 0);
+    }
+    __device__ static float2 pair(uint32_t u) {
+        _Float16 lo = __builtin_bit_cast(_Float16, (uint16_t)(u & 0xffffu));
+        _Float16 hi = __builtin_bit_cast(_Float16, (uint16_t)(u >> 16));
+        return make_float2((float)lo, (float)hi);
+    }
+};
+struct TF32 {
+    static constexpr bool kF32Table = true;
+    typedef f32x8 op;
+    typedef float elem;
+    __device__ static op pack(const float* x) {
+        op v;
+#pragma unroll
+        for (int j = 0; j < 8; ++j) v[j] = x[j];
+        return v;
+    }
+    __device__ static float get(const op& v, int j) { return v[j]; }
+    // A 16-deep chunk as 8 K=2 MFMAs: step s pairs k = s (lanes 0-31) with k = 8 + s
+    // (lanes 32-63), so every k of the chunk is contracted once.
+    __device__ static f32x16 mma16(op a, op b, f32x16 c) {
+#pragma unroll
+        for (int s = 0; s < 8; ++s) c = __builtin_amdgcn_mfma_f32_32x32x2f32(a[s], b[s], c, 0, 0, 0);
+        return c;
+    }
+};
+
+constexpr int kWaves = 4;
+constexpr int kThreads = 64 * kWaves;
+constexpr int kVert = 32 * kWaves;     // vertices per workgroup iteration
+constexpr int kStageLd = kVert + 8;    // LDS row stride (elements), breaks bank aliasing
+constexpr int kStageRowsD = 64;        // delta rows
+constexpr int kStageRowsH = 96;        // input rows (80 padded to 3 x 32)
+
+template <typename P>
+struct Stage {
+    typename P::elem d[kStageRowsD * kStageLd];
+    typename P::elem h[kStageRowsH * kStageLd];
+};
+
+// Number of 32x32 dW tiles of layer l (out rows x in cols).
+__host__ __device__ constexpr int dw_tiles(int l, int nl) {
+    return (l == 0 ? 2 * 3 : (l == nl - 1 ? 1 * 2 : 2 * 2));
+}
+// Accumulator slots a wave needs: ceil(tiles / waves) summed over layers.
+__host__ __device__ constexpr int dw_slots(int nl) {
+    int s = 0;
+    for (int l = 0; l < nl; ++l) s += (dw_tiles(l, nl) + kWaves - 1) / kWaves;
+    return s;
+}
+__host__ __device__ constexpr int dw_slot_base(int l, int nl) {
+    int s = 0;
+    for (int k = 0; k < l; ++k) s += (dw_tiles(k, nl) + kWaves - 1) / kWaves;
+    return s;
+}
+
+template <typename P>
+__device__ __forceinline__ typename P::op load_op(const typename P::op* __restrict__ W, int tile, int lane) {
+    return W[tile * 64 + lane];
+}
+
+// LDS operand: 8 consecutive vertices of row `row`, starting at vertex v0.
+template <typename P>
+__device__ __forceinline__ typename P::op stage_op(const typename P::elem* s, int row, int v0) {
+    const typename P::elem* p = s + row * kStageLd + v0;
+    typename P::op v;
+#pragma unroll
+    for (int j = 0; j < 8; ++j) v[j] = p[j];
+    return v;
+}
+
+// Forward of hidden layer `layer` (1..NL-2) or layer 0 from operands X (5 chunks
+// for layer 0, 4 otherwise) into fp32 accumulators D[2].
+template <typename P, int NL>
+__device__ __forceinline__ void fwd_layer(const typename P::op* __restrict__ Wf, int layer, const typename P::op* X,
+                                          f32x16* D, int lane) {
+#pragma unroll
+    for (int ot = 0; ot < 2; ++ot) {
+        D[ot] = f32x16{};
+        if (layer == 0) {
+#pragma unroll
+            for (int kb = 0; kb < 5; ++kb) D[ot] = P::mma16(load_op<P>(Wf, ot * 5 + kb, lane), X[kb], D[ot]);
+        } else {
+            const int base = 10 + (layer - 1) * 8;
+#pragma unroll
+            for (int kb = 0; kb < 4; ++kb) D[ot] = P::mma16(load_op<P>(Wf, base + ot * 4 + kb, lane), X[kb], D[ot]);
+        }
+    }
+}
+
+// ReLU of accumulators into operands; mask bit 16*ot + r set when D[ot][r] > 0.
+template <typename P>
+__device__ __forceinline__ uint32_t relu_ops(const f32x16* D, typename P::op* H) {
+    uint32_t mask = 0;
+#pragma unroll
+    for (int ot = 0; ot < 2; ++ot) {
+#pragma unroll
+        for (int g = 0; g < 2; ++g) {
+            float x[8];
+#pragma unroll
+            for (int j = 0; j < 8; ++j) {
+                float v = D[ot][8 * g + j];
+                mask |= (v > 0.0f ? 1u : 0u) << (16 * ot + 8 * g + j);
+                x[j] = fmaxf(v, 0.0f);
+            }
+            H[2 * ot + g] = P::pack(x);
+        }
+    }
+    return mask;
+}
+
+// Input of layer `upto` (>= 1): forward through layers 0..upto-1 from the encoding.
+template <typename P, int NL>
+__device__ __forceinline__ uint32_t recompute(const typename P::op* __restrict__ Wf, int upto, const typename P::op* E,
+                                              typename P::op* H, int lane) {
+    f32x16 D[2];
+    fwd_layer<P, NL>(Wf, 0, E, D, lane);
+    uint32_t mask = relu_ops<P>(D, H);
+    for (int layer = 1; layer < upto; ++layer) {
+        fwd_layer<P, NL>(Wf, layer, H, D, lane);
+        mask = relu_ops<P>(D, H);
+    }
+    return mask;
+}
+
+template <typename P, int NL, int NOUT, int LOSS>
+__global__ __launch_bounds__(kThreads) void k_train(TrainArgs t) {
+    typedef typename P::op op;
+    extern __shared__ __align__(16) unsigned char smem_raw[];
+    Stage<P>& st = *reinterpret_cast<Stage<P>*>(smem_raw);
+    __shared__ float loss_part[kWaves];
+
+    const int lane = threadIdx.x & 63, h = lane >> 5, col = lane & 31, wave = threadIdx.x >> 6;
+    const op* __restrict__ Wf = reinterpret_cast<const op*>(t.w_fwd);
+    const op* __restrict__ Wb = reinterpret_cast<const op*>(t.w_bwd);
+    const float S = *t.loss_scale;
+    constexpr int kSlots = dw_slots(NL);
+    f32x16 acc[kSlots];
+#pragma unroll
+    for (int s = 0; s < kSlots; ++s) acc[s] = f32x16{};
+    float loss_acc = 0.0f;
+
+    // zero the padded input rows once (layer 0 has 80 of 96 rows)
+    for (int k = threadIdx.x; k < 16 * kStageLd; k += kThreads) st.h[80 * kStageLd + k] = (typename P::elem)0.0f;
+
+    const uint32_t n_iter = (t.n + kVert - 1) / kVert;
+    for (uint32_t it = blockIdx.x; it < n_iter; it += gridDim.x) {
+        const uint32_t i = it * kVert + wave * 32 + col;
+        const bool live = i < t.n;
+        const int vcol = wave * 32 + col;  // vertex column in the stage
+
+        // ---- encoding (B layout, natural channel order)
+        op E[5];
+        {
+            float e[kEnc];
+            if (live) {
+                if constexpr (P::kF32Table) {
+                    encode_full<float>(t.a, t.net, t.table_f32, i, e);
+                } else {
+                    float x[3];
+                    load_xn(t.a, i, x);
+#pragma unroll
+                    for (int l = 0; l < kMaxLevels; ++l) {
+                        float2 f = make_float2(0.0f, 0.0f);
+                        if (l < (int)t.a.grid.n_levels) {
+                            LevelCell lc = level_cell(t.a.grid, l, x);
+#pragma unroll
+                            for (int cn = 0; cn < 8; ++cn) {
+                                uint32_t ent = t.a.grid.offset[l] + corner_index(t.a.grid, l, lc, cn);
+                                float w = corner_weight(lc, cn);
+                                float2 v = P::pair(t.table_lp[ent]);
+                                f.x = __fmaf_rn(w, v.x, f.x);
+                                f.y = __fmaf_rn(w, v.y, f.y);
+                            }
+                        }
+                        e[2 * l] = f.x;
+                        e[2 * l + 1] = f.y;
+                    }
+                    encode_tail(t.a, t.net, i, e);
+                }
+            } else {
+#pragma unroll
+                for (int k = 0; k < kEnc; ++k) e[k] = 0.0f;
+            }
+#pragma unroll
+            for (int kb = 0; kb < 5; ++kb) {
+                float x8[8];
+#pragma unroll
+                for (int j = 0; j < 8; ++j) x8[j] = h ? e[kb * 16 + 8 + j] : e[kb * 16 + j];
+                E[kb] = P::pack(x8);
+            }
+        }
+
+        // ---- forward to the output, loss, dL/dy
+        op H[4];
+        uint32_t mask = recompute<P, NL>(Wf, NL - 1, E, H, lane);
+        f32x16 Y = f32x16{};
+        {
+            const int base = 10 + (NL - 2) * 8;
+#pragma unroll
+            for (int kb = 0; kb < 4; ++kb) Y = P::mma16(load_op<P>(Wf, base + kb, lane), H[kb], Y);
+        }
+        // output c sits in lane half c/4, register c%4
+        float dyr[4] = {0.0f, 0.0f, 0.0f, 0.0f};
+        if (live) {
+            if (LOSS == NRRS_LOSS_EXTERNAL) {
+#pragma unroll
+                for (int r = 0; r < 4; ++r) {
+                    const int c = 4 * h + r;
+                    if (c < NOUT) dyr[r] = t.dy_ext[(size_t)c * t.n + i];
+                }
+            } else if (LOSS == NRRS_LOSS_LL2) {
+                const int active = t.ll2_step == 0 ? 3 : 6;
+#pragma unroll
+                for (int r = 0; r < 4; ++r) {
+                    const int c = 4 * h + r;
+                    if (c < NOUT && c < active) {
+                        float tg = fmaxf(t.target_L[(size_t)(c % 3) * t.n + i], 0.0f);
+                        if (t.clamp_on) tg = fminf(tg, t.clamp_max);
+                        if (c >= 3) tg = tg * tg;
+                        const float y = Y[r];
+                        const float den = y * y + 0.01f;
+                        const float d = y - tg;
+                        loss_acc += d * d / den * t.inv_norm;
+                        dyr[r] = 2.0f * d / den * t.inv_norm;
+                    }
+                }
+            } else if (LOSS == NRRS_LOSS_RRS) {
+                if (h == 0 && t.rrs_step >= 3) {
+                    const float qt = t.q_target[i];
+                    if (qt >= 0.0f) {
+                        const uint32_t S3 = t.a.v.stride;
+                        float ref;
+                        if (t.ref_mean) {
+                            ref = t.ref_mean[i];
+                        } else {
+                            const uint32_t px = t.a.v.pixel[i], Pn = t.a.n_pixels;
+                            ref = (t.a.film[px] + t.a.film[Pn + px] + t.a.film[2 * Pn + px]) * (1.0f / 3.0f);
+                        }
+                        float m2 = 0.0f;
+#pragma unroll
+                        for (int c = 0; c < 3; ++c) {
+                            const float th = t.a.v.thp[c * S3 + i];
+                            m2 += th * th * fmaxf(t.a.stats[(size_t)(3 + c) * t.a.stats_stride + i], 0.0f);
+                        }
+                        float vv = m2 * (1.0f / 3.0f) / (ref * ref + 0.01f);
+                        if (t.clamp_on) vv = fminf(vv, t.clamp_max);
+                        const float e = fmaxf(t.error[i], 1e-4f);
+                        const float y = Y[0];
+                        const float q = activation_sigma(y);
+                        const float qe = q + 0.01f;
+                        const float w = t.sample_weight[i];
+                        const float f = t.gamma[0] * (vv / qe + q * t.ebar) + t.gamma[1] * (vv / qe + q * e) +
+                                        t.gamma[2] * q + t.gamma[3] * (q - qt) * (q - qt);
+                        const float fp = -(t.gamma[0] + t.gamma[1]) * vv / (qe * qe) + t.gamma[0] * t.ebar +
+                                         t.gamma[1] * e + t.gamma[2] + 2.0f * t.gamma[3] * (q - qt);
+                        loss_acc += w * f * t.inv_norm;
+                        dyr[0] = w * fp * activation_sigma_grad(y) * t.inv_norm;
+                    }
+                }
+            }
+            if (t.y_out) {
+#pragma unroll
+                for (int r = 0; r < 4; ++r) {
+                    const int c = 4 * h + r;
+                    if (c < NOUT) t.y_out[(size_t)c * t.out_stride + i] = Y[r];
+                }
+            }
+        }
+        // dy operand (k = c = 8h + j): half 0 holds outputs 0..7, fetching 4..7 from half 1
+        op DY;
+        {
+            float x8[8];
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                const float other = __shfl_xor(dyr[r], 32, 64);
+                x8[r] = h ? 0.0f : dyr[r] * S;
+                x8[4 + r] = h ? 0.0f : other * S;
+            }
+            DY = P::pack(x8);
+        }
+
+        // ---- backward sweep
+        op Dl[4];  // delta of the current layer as operands (k-chunks of its output neurons)
+#pragma unroll
+        for (int kb = 0; kb < 4; ++kb) Dl[kb] = DY;  // only Dl[0] is used for the output layer
+#pragma unroll 1
+        for (int layer = NL - 1; layer >= 0; --layer) {
+            const bool last = layer == NL - 1;
+            // (1) layer input h_l and its ReLU mask
+            if (!last && layer > 0) mask = recompute<P, NL>(Wf, layer, E, H, lane);
+            // (2) stage delta_l rows [out neuron][vertex] and h_l rows [in neuron][vertex]
+            __syncthreads();  // previous layer's dW reads are done
+            if (last) {
+#pragma unroll
+                for (int j = 0; j < 8; ++j)
+                    if (h == 0 && j < NOUT) st.d[j * kStageLd + vcol] = (typename P::elem)P::get(Dl[0], j);
+            } else {
+#pragma unroll
+                for (int kb = 0; kb < 4; ++kb)
+#pragma unroll
+                    for (int j = 0; j < 8; ++j)
+                        st.d[(16 * kb + 8 * (j >> 2) + 4 * h + (j & 3)) * kStageLd + vcol] =
+                            (typename P::elem)P::get(Dl[kb], j);
+            }
+            if (layer == 0) {
+#pragma unroll
+                for (int kb = 0; kb < 5; ++kb)
+#pragma unroll
+                    for (int j = 0; j < 8; ++j)
+                        st.h[(kb * 16 + 8 * h + j) * kStageLd + vcol] = (typename P::elem)P::get(E[kb], j);
+            } else {
+#pragma unroll
+                for (int kb = 0; kb < 4; ++kb)
+#pragma unroll
+                    for (int j = 0; j < 8; ++j)
+                        st.h[(16 * kb + 8 * (j >> 2) + 4 * h + (j & 3)) * kStageLd + vcol] =
+                            (typename P::elem)P::get(H[kb], j);
+            }
+            __syncthreads();
+            // (3) dW tiles of this layer owned by this wave
+            {
+                const int n_ti = layer == 0 ? 3 : 2;
+                const int n_tiles = dw_tiles(layer, NL);
+                int slot = dw_slot_base(layer, NL);
+                for (int tt = wave; tt < n_tiles; tt += kWaves, ++slot) {
+                    const int to = tt / n_ti, ti = tt % n_ti;
+                    f32x16 a = f32x16{};
+#pragma unroll
+                    for (int s = 0; s < kVert / 16; ++s) {
+                        op A = stage_op<P>(st.d, to * 32 + col, 16 * s + 8 * h);
+                        op B = stage_op<P>(st.h, ti * 32 + col, 16 * s + 8 * h);
+                        a = P::mma16(A, B, a);
+                    }
+                    // registers are indexed statically: select the slot by a switch
+#pragma unroll
+                    for (int s2 = 0; s2 < kSlots; ++s2)
+                        if (s2 == slot) acc[s2] += a;
+                }
+            }
+            // (4) propagate delta to the layer input
+            if (layer > 0) {
+                f32x16 D[2];
+#pragma unroll
+                for (int ot = 0; ot < 2; ++ot) {
+                    D[ot] = f32x16{};
+                    if (last) {
+                        D[ot] = P::mma16(load_op<P>(Wb, t.bwd_last_base + ot, lane), Dl[0], D[ot]);
+                    } else {
+                        const int base = t.bwd_hidden_base + (layer - 1) * 8;
+#pragma unroll
+                        for (int kb = 0; kb < 4; ++kb)
+                            D[ot] = P::mma16(load_op<P>(Wb, base + ot * 4 + kb, lane), Dl[kb], D[ot]);
+                    }
+                }
+                // delta_{l-1} = D * [h_l > 0]
+#pragma unroll
+                for (int ot = 0; ot < 2; ++ot)
+#pragma unroll
+                    for (int g = 0; g < 2; ++g) {
+                        float x8[8];
+#pragma unroll
+                        for (int j = 0; j < 8; ++j)
+                            x8[j] = ((mask >> (16 * ot + 8 * g + j)) & 1u) ? D[ot][8 * g + j] : 0.0f;
+                        Dl[2 * ot + g] = P::pack(x8);
+                    }
+            } else {
+                // dL/denc for the 32 grid channels: rows c = 8(r/4) + 4h + r%4
+                f32x16 G = f32x16{};
+#pragma unroll
+                for (int kb = 0; kb < 4; ++kb) G = P::mma16(load_op<P>(Wb, kb, lane), Dl[kb], G);
+                if (live) {
+                    float x[3];
+                    load_xn(t.a, i, x);
+#pragma unroll
+                    for (int q4 = 0; q4 < 4; ++q4)
+#pragma unroll
+                        for (int pr = 0; pr < 2; ++pr) {
+                            // registers 4*q4 + 2*pr + {0,1} hold features 0/1 of level 4*q4 + 2h + pr
+                            const int l = 4 * q4 + 2 * h + pr;
+                            if (l < (int)t.a.grid.n_levels) {
+                                const float g0 = G[4 * q4 + 2 * pr], g1 = G[4 * q4 + 2 * pr + 1];
+                                LevelCell lc = level_cell(t.a.grid, l, x);
+#pragma unroll
+                                for (int cn = 0; cn < 8; ++cn) {
+                                    const uint32_t ent = t.a.grid.offset[l] + corner_index(t.a.grid, l, lc, cn);
+                                    const float w = corner_weight(lc, cn);
+                                    atomicAdd(t.grad_grid + 2 * (size_t)ent, w * g0);
+                                    atomicAdd(t.grad_grid + 2 * (size_t)ent + 1, w * g1);
+                                }
+                            }
+                        }
+                }
+            }
+        }
+    }
+
+    // ---- per-workgroup dW partials (rows < out, cols < in) and the loss
+    float* part = t.dw_partial + (size_t)blockIdx.x * t.n_mlp;
+#pragma unroll
+    for (int layer = 0; layer < NL; ++layer) {
+        const int n_ti = layer == 0 ? 3 : 2;
+        const int n_tiles = dw_tiles(layer, NL);
+        const int in = layer == 0 ? kEnc : kHidden;
+        const int out = layer == NL - 1 ? NOUT : kHidden;
+        const int woff = layer == 0 ? 0 : kHidden * kEnc + (layer - 1) * kHidden * kHidden;
+        int slot = dw_slot_base(layer, NL);
+        for (int tt = wave; tt < n_tiles; tt += kWaves, ++slot) {
+            const int to = tt / n_ti, ti = tt % n_ti;
+            f32x16 a = f32x16{};
+#pragma unroll
+            for (int s2 = 0; s2 < kSlots; ++s2)
+                if (s2 == slot) a = acc[s2];
+#pragma unroll
+            for (int r = 0; r < 16; ++r) {
+                const int row = to * 32 + 8 * (r >> 2) + 4 * h + (r & 3);
+                const int cc = ti * 32 + col;
+                if (row < out && cc < in) part[woff + row * in + cc] = a[r];
+            }
+        }
+    }
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) loss_acc += __shfl_xor(loss_acc, o, 64);
+    if (lane == 0) loss_part[wave] = loss_acc;
+    __syncthreads();
+    if (threadIdx.x == 0 && t.loss_out) {
+        float s = 0.0f;
+        for (int w = 0; w < kWaves; ++w) s += loss_part[w];
+        atomicAdd(t.loss_out, s);
+    }
+}
+
+// ---------------------------------------------------------------- packing
+// Forward tiles: identical to the inference packing (A operand of W_l).
+// Backward tiles (A operand of W_l^T): [0,4) layer 0 -> 32 grid channels (rows)
+// x 64 outputs (K, permuted); then hidden layers 1..L-2: 2 row tiles x 4 K chunks
+// (rows = input neuron, K = output neuron permuted); then the output layer:
+// 2 row tiles x 1 K chunk (K = output c = 8h + j, natural).
+__device__ __forceinline__ int pcol(int kb, int h, int j) { return 16 * kb + 8 * (j >> 2) + 4 * h + (j & 3); }
+
+template <typename E>
+__global__ void k_pack_train(const float* __restrict__ W, int nl, int n_out, E* __restrict__ fwd, E* __restrict__ bwd,
+                             int n_fwd, int n_bwd) {
+    const int tile = blockIdx.x, lane = threadIdx.x, h = lane >> 5, r32 = lane & 31;
+    auto woff = [&](int layer) { return layer == 0 ? 0 : kHidden * kEnc + (layer - 1) * kHidden * kHidden; };
+    if (tile < n_fwd) {
+        int layer, ot, kb;
+        if (tile < 10) {
+            layer = 0;
+            ot = tile / 5;
+            kb = tile % 5;
+        } else {
+            const int u = tile - 10;
+            layer = 1 + u / 8;
+            if (layer <= nl - 2) {
+                ot = (u % 8) / 4;
+                kb = u % 4;
+            } else {
+                layer = nl - 1;
+                ot = 0;
+                kb = u - (nl - 2) * 8;
+            }
+        }
+        const int in = layer == 0 ? kEnc : kHidden;
+        const int rows = layer == nl - 1 ? n_out : kHidden;
+        const int row = ot * 32 + r32;
+#pragma unroll
+        for (int j = 0; j < 8; ++j) {
+            const int c = layer == 0 ? kb * 16 + 8 * h + j : pcol(kb, h, j);
+            const float w = row < rows ? W[woff(layer) + row * in + c] : 0.0f;
+            fwd[((size_t)tile * 64 + lane) * 8 + j] = (E)w;
+        }
+        return;
+    }
+    const int bt = tile - n_fwd;
+    if (bt >= n_bwd) return;
+#pragma unroll
+    for (int j = 0; j < 8; ++j) {
+        float w = 0.0f;
+        if (bt < 4) {  // layer 0: row = grid channel r32, K = output neuron pcol(bt, h, j)
+            w = W[woff(0) + pcol(bt, h, j) * kEnc + r32];
+        } else if (bt < 4 + (nl - 2) * 8) {
+            const int u = bt - 4, layer = 1 + u / 8, ot = (u % 8) / 4, kb = u % 4;
+            w = W[woff(layer) + pcol(kb, h, j) * kHidden + ot * 32 + r32];
+        } else {
+            const int ot = bt - 4 - (nl - 2) * 8;
+            const int c = 8 * h + j;
+            w = c < n_out ? W[woff(nl - 1) + c * kHidden + ot * 32 + r32] : 0.0f;
+        }
+        bwd[((size_t)bt * 64 + lane) * 8 + j] = (E)w;
+    }
+}
+
+template <typename P, int NL, int NOUT, int LOSS>
+hipError_t launch_train_one(const TrainArgs& t, uint32_t blocks, hipStream_t s) {
+    constexpr size_t smem = sizeof(Stage<P>);
+    static bool attr = false;
+    if (!attr) {
+        hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(&k_train<P, NL, NOUT, LOSS>),
+                                           hipFuncAttributeMaxDynamicSharedMemorySize, (int)smem);
+        if (e != hipSuccess) return e;
+        attr = true;
+    }
+    hipLaunchKernelGGL((k_train<P, NL, NOUT, LOSS>), dim3(blocks), dim3(kThreads), smem, s, t);
+    return hipGetLastError();
+}
+
+template <typename P>
+hipError_t dispatch_train(const TrainArgs& t, int nl, int n_out, int loss, uint32_t blocks, hipStream_t s) {
+    if (nl == 5 && n_out == 6) {
+        if (loss == NRRS_LOSS_EXTERNAL) return launch_train_one<P, 5, 6, NRRS_LOSS_EXTERNAL>(t, blocks, s);
+        if (loss == NRRS_LOSS_LL2) return launch_train_one<P, 5, 6, NRRS_LOSS_LL2>(t, blocks, s);
+    }
+    if (nl == 4 && n_out == 1) {
+        if (loss == NRRS_LOSS_EXTERNAL) return launch_train_one<P, 4, 1, NRRS_LOSS_EXTERNAL>(t, blocks, s);
+        if (loss == NRRS_LOSS_RRS) return launch_train_one<P, 4, 1, NRRS_LOSS_RRS>(t, blocks, s);
+    }
+    return hipErrorInvalidValue;
+}
+
+__global__ void k_reduce_partials(const float* __restrict__ part, uint32_t n_blocks, uint32_t n_mlp,
+                                  float* __restrict__ grad) {
+    const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= n_mlp) return;
+    float s = 0.0f;
+    for (uint32_t b = 0; b < n_blocks; ++b) s += part[(size_t)b * n_mlp + k];
+    grad[k] += s;
+}
+
+}  // namespace
+
+int train_tiles_fwd(int nl) { return 10 + (nl - 2) * 8 + 4; }
+int train_tiles_bwd(int nl) { return 4 + (nl - 2) * 8 + 2; }
+size_t train_op_bytes(int precision) { return precision == NRRS_FP32 ? 32 : 16; }
+
+hipError_t launch_pack_train(int precision, const float* W, int nl, int n_out, void* fwd, void* bwd,
+                             hipStream_t s) {
+    const int nf = train_tiles_fwd(nl), nb = train_tiles_bwd(nl);
+    if (precision == NRRS_FP32)
+        hipLaunchKernelGGL(k_pack_train<float>, dim3(nf + nb), dim3(64), 0, s, W, nl, n_out,
+                           static_cast<float*>(fwd), static_cast<float*>(bwd), nf, nb);
+    else if (precision == NRRS_BF16)
+        hipLaunchKernelGGL(k_pack_train<__bf16>, dim3(nf + nb), dim3(64), 0, s, W, nl, n_out,
+                           static_cast<__bf16*>(fwd), static_cast<__bf16*>(bwd), nf, nb);
+    else
+        hipLaunchKernelGGL(k_pack_train<_Float16>, dim3(nf + nb), dim3(64), 0, s, W, nl, n_out,
+                           static_cast<_Float16*>(fwd), static_cast<_Float16*>(bwd), nf, nb);
+    return hipGetLastError();
+}
+
+hipError_t launch_train(int precision, const TrainArgs& t, int nl, int n_out, int loss, uint32_t blocks,
+                        hipStream_t s) {
+    if (precision == NRRS_FP32) return dispatch_train<TF32>(t, nl, n_out, loss, blocks, s);
+    if (precision == NRRS_BF16) return dispatch_train<TBF16>(t, nl, n_out, loss, blocks, s);
+    return dispatch_train<TF16>(t, nl, n_out, loss, blocks, s);
+}
+
+hipError_t launch_reduce_partials(const float* part, uint32_t n_blocks, uint32_t n_mlp, float* grad, hipStream_t s) {
+    hipLaunchKernelGGL(k_reduce_partials, dim3((n_mlp + 255) / 256), dim3(256), 0, s, part, n_blocks, n_mlp, grad);
+    return hipGetLastError();
+}
+
+}  // namespace nrrs

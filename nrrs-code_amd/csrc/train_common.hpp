@@ -1,0 +1,66 @@
+// Arguments and launchers of the training kernels.
+#pragma once
+
+#include "host_common.hpp"
+
+namespace nrrs {
+
+struct TrainArgs {
+    EncodeArgs a;                  // vertices, film, RRSNet statistics input, grid
+    int net;
+    uint32_t n;
+    const float* table_f32;        // FP32 grid (FP32 contexts) or null
+    const uint32_t* table_lp;      // FP16/BF16 grid (two features per 32-bit word)
+    const void* w_fwd;             // packed forward A tiles
+    const void* w_bwd;             // packed W^T A tiles
+    int bwd_hidden_base, bwd_last_base;
+    const float* loss_scale;       // device scalar
+    // loss inputs
+    const float* dy_ext;           // [n_out][n] (external loss)
+    const float* target_L;         // [3][n] (StatNet)
+    const float* ref_mean;         // RRS side arrays [n]
+    const float* error;
+    const float* sample_weight;
+    const float* q_target;
+    float ebar;                    // max(error_sum / pixels, 1e-4)
+    float gamma[4];
+    int clamp_on;
+    float clamp_max;
+    uint32_t ll2_step, rrs_step;
+    float inv_norm;                // loss normalisation
+    // outputs
+    float* y_out;                  // [n_out][out_stride] forward outputs, or null
+    uint32_t out_stride;
+    float* grad_grid;              // [entries][2], atomically accumulated
+    float* dw_partial;             // [gridDim.x][n_mlp]
+    uint32_t n_mlp;
+    float* loss_out;               // device scalar (atomic add), or null
+};
+
+int train_tiles_fwd(int nl);
+int train_tiles_bwd(int nl);
+size_t train_op_bytes(int precision);
+hipError_t launch_pack_train(int precision, const float* W, int nl, int n_out, void* fwd, void* bwd, hipStream_t s);
+hipError_t launch_train(int precision, const TrainArgs& t, int nl, int n_out, int loss, uint32_t blocks,
+                        hipStream_t s);
+hipError_t launch_reduce_partials(const float* part, uint32_t n_blocks, uint32_t n_mlp, float* grad, hipStream_t s);
+
+struct AdamArgs {
+    float* params;        // [n] FP32 masters
+    float* grad;          // [n], zeroed after use
+    float* m;
+    float* v;
+    uint64_t n;
+    uint64_t n_grid;      // leading grid parameters (their low-precision shadow is refreshed)
+    uint16_t* grid_lp;    // or null
+    int lp_precision;
+    float lr, beta1, beta2, eps, l2;
+    float* loss_scale;    // device
+    uint32_t* state;      // device: [0] step, [1] good steps, [2] skipped, [3] non-finite flag
+    int dynamic_scale;
+    uint32_t growth_interval;
+};
+hipError_t launch_adam(const AdamArgs& a, hipStream_t s);
+hipError_t launch_fill_f32(float* p, float v, uint64_t n, hipStream_t s);
+
+}  // namespace nrrs

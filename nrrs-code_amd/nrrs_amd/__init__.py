@@ -119,6 +119,18 @@ class DecideArgs(C.Structure):
     ]
 
 
+class RrsSide(C.Structure):
+    _fields_ = [
+        ("ref_mean", C.c_void_p),
+        ("error", C.c_void_p),
+        ("sample_weight", C.c_void_p),
+        ("q_target", C.c_void_p),
+        ("error_sum", C.c_float),
+    ]
+
+
+LOSS_EXTERNAL, LOSS_LL2, LOSS_RRS = 0, 1, 2
+
 # name -> (restype, argtypes)
 _P = C.c_void_p
 _SIGS = {
@@ -138,6 +150,15 @@ _SIGS = {
     "nrrs_rrs_infer_decide": (C.c_int, [_P, C.POINTER(DecideArgs), _P]),
     "nrrs_rrs_decide": (C.c_int, [_P, C.POINTER(DecideArgs), _P]),
     "nrrs_synth_vertices": (C.c_int, [C.c_uint32, C.c_uint64, C.c_uint32, C.POINTER(VerticesOut), _P]),
+    "nrrs_train_step": (C.c_int, [_P, C.c_uint32, C.c_uint32, C.POINTER(Vertices), _P, _P, C.POINTER(RrsSide),
+                                  _P, _P]),
+    "nrrs_backward": (C.c_int, [_P, C.c_int32, C.c_uint32, C.POINTER(Vertices), _P, _P, C.c_int32, _P, _P,
+                                C.POINTER(RrsSide), C.c_float, _P, _P]),
+    "nrrs_grads_zero": (C.c_int, [_P, C.c_int32, _P]),
+    "nrrs_grads_get": (C.c_int, [_P, C.c_int32, _P]),
+    "nrrs_adam_step": (C.c_int, [_P, C.c_int32, _P]),
+    "nrrs_train_state": (C.c_int, [_P, C.c_int32, C.POINTER(C.c_uint32), C.POINTER(C.c_float),
+                                   C.POINTER(C.c_uint32)]),
 }
 
 _lib = None
@@ -374,3 +395,44 @@ class Context:
 
     def rrs_decide(self, args, stream=None):
         return self.lib.nrrs_rrs_decide(self.h, C.byref(args), _stream(stream))
+
+    # ---- training
+    @staticmethod
+    def side(ref_mean=None, error=None, sample_weight=None, q_target=None, error_sum=0.0):
+        """nrrs_rrs_side from device tensors (keeps no reference: the caller owns them)."""
+        sd = RrsSide()
+        sd.ref_mean = None if ref_mean is None else ref_mean.data_ptr()
+        sd.error = error.data_ptr()
+        sd.sample_weight = sample_weight.data_ptr()
+        sd.q_target = q_target.data_ptr()
+        sd.error_sum = float(error_sum)
+        return sd
+
+    def backward(self, net, vb, kind, film=None, stats=None, dy=None, target_L=None, side=None, loss_scale=1.0,
+                 loss=None, stream=None):
+        v = vb.c_struct()
+        return self.lib.nrrs_backward(self.h, net, vb.n, C.byref(v), _ptr(film), _ptr(stats), kind, _ptr(dy),
+                                      _ptr(target_L), None if side is None else C.byref(side), loss_scale,
+                                      _ptr(loss), _stream(stream))
+
+    def grads_zero(self, net, stream=None):
+        check(self.lib.nrrs_grads_zero(self.h, net, _stream(stream)), "grads_zero")
+
+    def grads_get(self, net):
+        g, m = self.param_count(net)
+        out = np.zeros(g + m, np.float32)
+        check(self.lib.nrrs_grads_get(self.h, net, out.ctypes.data), "grads_get")
+        return out
+
+    def adam_step(self, net, stream=None):
+        check(self.lib.nrrs_adam_step(self.h, net, _stream(stream)), "adam_step")
+
+    def train_state(self, net):
+        step, sc, sk = C.c_uint32(), C.c_float(), C.c_uint32()
+        check(self.lib.nrrs_train_state(self.h, net, C.byref(step), C.byref(sc), C.byref(sk)), "train_state")
+        return dict(step=step.value, loss_scale=sc.value, skipped=sk.value)
+
+    def train_step(self, vb, net_mask, film=None, target_L=None, side=None, loss=None, stream=None):
+        v = vb.c_struct()
+        return self.lib.nrrs_train_step(self.h, net_mask, vb.n, C.byref(v), _ptr(film), _ptr(target_L),
+                                        None if side is None else C.byref(side), _ptr(loss), _stream(stream))

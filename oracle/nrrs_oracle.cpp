@@ -472,3 +472,234 @@ void ora_init_params(const nrrs_config* cfg, int32_t net, uint64_t seed, double 
 }
 
 }  // extern "C"
+
+// ---------------------------------------------------------------- training
+namespace {
+
+// NRRS_LL2 restated (the loss source, nrrs_ll2.h, is not in the reference checkout):
+// relative L2 of the 3 (step 0) or 6 (step >= 1) outputs against t = (L, L^2), L
+// clamped to loss_clamp_max when loss_clamp_on; the denominator y^2 + 0.01 is a
+// constant of the gradient (relative-L2 convention). Loss = mean over samples and
+// active channels.
+double loss_ll2(const nrrs_config* cfg, uint32_t n, const double* y, const float* tL, const double* y_den,
+                double* dy) {
+    const int active = cfg->ll2_step == 0 ? 3 : 6;
+    const double norm = 1.0 / ((double)n * active);
+    double loss = 0;
+    for (uint32_t i = 0; i < n; ++i) {
+        for (int c = 0; c < 6; ++c) {
+            double t = std::fmax((double)tL[(size_t)(c % 3) * n + i], 0.0);
+            if (cfg->loss_clamp_on) t = std::fmin(t, (double)cfg->loss_clamp_max);
+            if (c >= 3) t = t * t;
+            double yy = y[(size_t)c * n + i];
+            double yd = y_den ? y_den[(size_t)c * n + i] : yy;
+            double den = yd * yd + 0.01;
+            double d = yy - t;
+            if (c < active) {
+                loss += d * d / den * norm;
+                if (dy) dy[(size_t)c * n + i] = 2.0 * d / den * norm;
+            } else if (dy) {
+                dy[(size_t)c * n + i] = 0.0;
+            }
+        }
+    }
+    return loss;
+}
+
+double sigma_grad_d(double x) { return x < 0 ? std::exp(x) / (1.0 + std::exp(x)) : 0.5; }
+
+// NRRS_RRS restated (nrrs_rrs.h is not in the reference checkout). With q = sigma(y),
+// v = mean_c(thp_c^2 max(L2_c,0)) / (ref^2 + 0.01) (clamped at loss_clamp_max when
+// loss_clamp_on), E = max(error_sum / pixels, 1e-4), e = max(error, 1e-4):
+//   f(q) = g1 (v/(q+0.01) + q E) + g2 (v/(q+0.01) + q e) + g3 q + g4 (q - q_t)^2
+// loss = (1/n) sum over samples with q_t >= 0 of sample_weight * f(q); zero for step < 3.
+double loss_rrs(const nrrs_config* cfg, uint32_t n, const double* y, const nrrs_vertices* v,
+                const float* film, const double* stats, const float* ref_mean, const float* err,
+                const float* sw, const float* qt, double error_sum, double* dy) {
+    double loss = 0;
+    const double Ebar = std::fmax(error_sum / (double)cfg->n_pixels, 1e-4);
+    const double* g = nullptr;
+    double gam[4] = {cfg->gamma[0], cfg->gamma[1], cfg->gamma[2], cfg->gamma[3]};
+    (void)g;
+    const uint32_t S = v->stride, P = cfg->n_pixels;
+    for (uint32_t i = 0; i < n; ++i) {
+        if (dy) dy[i] = 0.0;
+        if (cfg->rrs_step < 3 || qt[i] < 0.0f) continue;
+        double ref;
+        if (ref_mean) {
+            ref = ref_mean[i];
+        } else {
+            uint32_t px = v->pixel[i];
+            ref = ((double)film[px] + (double)film[P + px] + (double)film[2 * (size_t)P + px]) / 3.0;
+        }
+        double m2 = 0;
+        for (int c = 0; c < 3; ++c) {
+            double t = v->thp[c * S + i];
+            m2 += t * t * std::fmax(stats[(size_t)(3 + c) * n + i], 0.0);
+        }
+        double vv = m2 / 3.0 / (ref * ref + 0.01);
+        if (cfg->loss_clamp_on) vv = std::fmin(vv, (double)cfg->loss_clamp_max);
+        double e = std::fmax((double)err[i], 1e-4);
+        double q = sigma_d(y[i]);
+        double qe = q + 0.01;
+        double f = gam[0] * (vv / qe + q * Ebar) + gam[1] * (vv / qe + q * e) + gam[2] * q +
+                   gam[3] * (q - qt[i]) * (q - qt[i]);
+        double fp = -(gam[0] + gam[1]) * vv / (qe * qe) + gam[0] * Ebar + gam[1] * e + gam[2] +
+                    2.0 * gam[3] * (q - qt[i]);
+        loss += sw[i] * f / n;
+        if (dy) dy[i] = sw[i] * fp * sigma_grad_d(y[i]) / n;
+    }
+    return loss;
+}
+
+}  // namespace
+
+extern "C" {
+
+double ora_loss(const nrrs_config* cfg, int32_t kind, uint32_t n, const double* y, const float* target_L,
+                const nrrs_vertices* v, const float* film, const double* stats, const float* ref_mean,
+                const float* err, const float* sw, const float* qt, double error_sum, const double* y_den,
+                double* dy) {
+    if (kind == NRRS_LOSS_LL2) return loss_ll2(cfg, n, y, target_L, y_den, dy);
+    if (kind == NRRS_LOSS_RRS)
+        return loss_rrs(cfg, n, y, v, film, stats, ref_mean, err, sw, qt, error_sum, dy);
+    return 0.0;
+}
+
+// Forward (activations kept), loss, backward of one net in FP64. grad: [grid | mlp],
+// overwritten with d(loss_scale * loss)/dparams. dy_ext: [n_out][n] for kind EXTERNAL.
+double ora_train_grads(const nrrs_config* cfg, int32_t net, uint32_t n, const nrrs_vertices* v,
+                       const float* film, const double* stats, const double* params, int32_t kind,
+                       const double* dy_ext, const float* target_L, const float* ref_mean, const float* err,
+                       const float* sw, const float* qt, double error_sum, double loss_scale, double* grad,
+                       double* dy_out) {
+    uint64_t ng = 0, nm = 0;
+    ora_param_count(cfg, net, &ng, &nm);
+    int nl, no;
+    mlp_dims(cfg, net, &nl, &no);
+    std::fill(grad, grad + ng + nm, 0.0);
+    std::vector<double> enc((size_t)n * ORA_ENC_WIDTH);
+    ora_encode(cfg, net, n, v, film, stats, params, enc.data());
+    const double* W = params + ng;
+    double* dW = grad + ng;
+    // forward, keeping every layer input
+    std::vector<std::vector<double>> act(nl);  // act[l] = input of layer l, [n][in_l]
+    act[0] = enc;
+    std::vector<double> y((size_t)no * n);
+    {
+        size_t woff = 0;
+        int in = ORA_ENC_WIDTH;
+        for (int layer = 0; layer < nl; ++layer) {
+            int o = layer == nl - 1 ? no : ORA_HIDDEN;
+            const double* w = W + woff;
+            std::vector<double> out((size_t)n * o);
+            for (uint32_t i = 0; i < n; ++i)
+                for (int r = 0; r < o; ++r) {
+                    double acc = 0;
+                    for (int k = 0; k < in; ++k) acc += w[(size_t)r * in + k] * act[layer][(size_t)i * in + k];
+                    out[(size_t)i * o + r] = layer == nl - 1 ? acc : std::max(acc, 0.0);
+                }
+            if (layer < nl - 1) {
+                act[layer + 1] = out;
+            } else {
+                for (uint32_t i = 0; i < n; ++i)
+                    for (int r = 0; r < o; ++r) y[(size_t)r * n + i] = out[(size_t)i * o + r];
+            }
+            woff += (size_t)o * in;
+            in = o;
+        }
+    }
+    std::vector<double> dy((size_t)no * n, 0.0);
+    double loss = 0;
+    if (kind == NRRS_LOSS_EXTERNAL) {
+        std::copy(dy_ext, dy_ext + (size_t)no * n, dy.begin());
+    } else {
+        loss = ora_loss(cfg, kind, n, y.data(), target_L, v, film, stats, ref_mean, err, sw, qt, error_sum,
+                        nullptr, dy.data());
+    }
+    for (auto& d : dy) d *= loss_scale;
+    if (dy_out) std::copy(dy.begin(), dy.end(), dy_out);
+    // backward
+    std::vector<size_t> woffs(nl);
+    {
+        size_t woff = 0;
+        int in = ORA_ENC_WIDTH;
+        for (int layer = 0; layer < nl; ++layer) {
+            int o = layer == nl - 1 ? no : ORA_HIDDEN;
+            woffs[layer] = woff;
+            woff += (size_t)o * in;
+            in = o;
+        }
+    }
+    std::vector<double> delta((size_t)n * no), prev;
+    for (uint32_t i = 0; i < n; ++i)
+        for (int r = 0; r < no; ++r) delta[(size_t)i * no + r] = dy[(size_t)r * n + i];
+    int o = no;
+    for (int layer = nl - 1; layer >= 0; --layer) {
+        int in = layer == 0 ? ORA_ENC_WIDTH : ORA_HIDDEN;
+        const double* w = W + woffs[layer];
+        double* dw = dW + woffs[layer];
+        const std::vector<double>& a = act[layer];
+        prev.assign((size_t)n * in, 0.0);
+        for (uint32_t i = 0; i < n; ++i) {
+            for (int r = 0; r < o; ++r) {
+                double dr = delta[(size_t)i * o + r];
+                if (dr == 0.0) continue;
+                for (int k = 0; k < in; ++k) {
+                    dw[(size_t)r * in + k] += dr * a[(size_t)i * in + k];
+                    prev[(size_t)i * in + k] += w[(size_t)r * in + k] * dr;
+                }
+            }
+            if (layer > 0)
+                for (int k = 0; k < in; ++k)
+                    if (!(a[(size_t)i * in + k] > 0.0)) prev[(size_t)i * in + k] = 0.0;
+        }
+        delta.swap(prev);
+        o = in;
+    }
+    // delta = dL/denc; scatter the grid channels into the table gradient
+    {
+        const nrrs_grid_config& g = cfg->grid;
+        std::vector<Level> L = levels_of(g, nullptr);
+        float c[3], d[3];
+        aabb_frame(cfg->aabb_min, cfg->aabb_max, c, d);
+        for (uint32_t i = 0; i < n; ++i) {
+            float x[3];
+            for (int k = 0; k < 3; ++k) x[k] = norm_coord(v->pos[k * v->stride + i], c[k], d[k]);
+            for (uint32_t l = 0; l < g.n_levels; ++l) {
+                uint32_t cell[3];
+                double fr[3];
+                for (int k = 0; k < 3; ++k) {
+                    float p = std::fma(x[k], L[l].scale, 0.5f);
+                    float fl = std::floor(p);
+                    cell[k] = (uint32_t)fl;
+                    fr[k] = (double)(p - fl);
+                }
+                for (uint32_t cn = 0; cn < 8; ++cn) {
+                    double w = 1.0;
+                    for (int k = 0; k < 3; ++k) w *= ((cn >> k) & 1u) ? fr[k] : 1.0 - fr[k];
+                    uint32_t id = corner_index(L[l], g.log2_table_size, cell[0] + (cn & 1u),
+                                               cell[1] + ((cn >> 1) & 1u), cell[2] + ((cn >> 2) & 1u));
+                    double* t = grad + ((size_t)L[l].offset + id) * 2;
+                    t[0] += w * delta[(size_t)i * ORA_ENC_WIDTH + 2 * l];
+                    t[1] += w * delta[(size_t)i * ORA_ENC_WIDTH + 2 * l + 1];
+                }
+            }
+        }
+    }
+    return loss;
+}
+
+// Adam (Kingma & Ba 2015) with L2 regularisation added to the gradient; step t >= 1.
+void ora_adam(uint64_t n, double* p, double* m, double* v, const double* g, double lr, double b1, double b2,
+              double eps, double l2, uint32_t t) {
+    double c1 = 1.0 - std::pow(b1, (double)t), c2 = 1.0 - std::pow(b2, (double)t);
+    for (uint64_t i = 0; i < n; ++i) {
+        double gg = g[i] + l2 * p[i];
+        m[i] = b1 * m[i] + (1.0 - b1) * gg;
+        v[i] = b2 * v[i] + (1.0 - b2) * gg * gg;
+        p[i] -= lr * (m[i] / c1) / (std::sqrt(v[i] / c2) + eps);
+    }
+}
+
+}  // extern "C"

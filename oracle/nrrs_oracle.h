@@ -89,6 +89,24 @@ void ora_synth_vertices(uint32_t n, uint64_t seed, uint32_t n_pixels, const nrrs
 void ora_init_params(const nrrs_config* cfg, int32_t net, uint64_t seed, double grid_amp,
                      double* params);
 
+/* Loss of kind NRRS_LOSS_LL2 / NRRS_LOSS_RRS and dL/dy. y, dy: [n_out][n]; stats: [6][n]
+ * StatNet statistics (RRS loss); y_den: LL2 denominators taken from these outputs
+ * instead of y (frozen-denominator surrogate for finite differences), or NULL. */
+double ora_loss(const nrrs_config* cfg, int32_t kind, uint32_t n, const double* y, const float* target_L,
+                const nrrs_vertices* v, const float* film, const double* stats, const float* ref_mean,
+                const float* err, const float* sw, const float* qt, double error_sum, const double* y_den,
+                double* dy);
+
+/* FP64 forward + loss + backward of one net; grad = d(loss_scale * loss)/dparams. Returns the loss. */
+double ora_train_grads(const nrrs_config* cfg, int32_t net, uint32_t n, const nrrs_vertices* v,
+                       const float* film, const double* stats, const double* params, int32_t kind,
+                       const double* dy_ext, const float* target_L, const float* ref_mean, const float* err,
+                       const float* sw, const float* qt, double error_sum, double loss_scale, double* grad,
+                       double* dy_out);
+
+void ora_adam(uint64_t n, double* p, double* m, double* v, const double* g, double lr, double b1, double b2,
+              double eps, double l2, uint32_t t);
+
 #ifdef __cplusplus
 }
 #endif

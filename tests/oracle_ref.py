@@ -178,3 +178,89 @@ def rrs_decide(cfg, q, stid=None, spec=None, seed=0, stream=0, capacity=None):
                                C.byref(ovf), C.byref(total))
     return dict(status=st, qn=qn, s=s, survivors=surv[: n_surv.value], n_survivors=n_surv.value,
                 overflow=ovf.value, sum=total.value)
+
+
+_SIGS_TRAIN = {
+    "ora_loss": (C.c_double, [C.POINTER(na.Config), C.c_int32, C.c_uint32, _P, _P, C.POINTER(na.Vertices), _P, _P,
+                              _P, _P, _P, _P, C.c_double, _P, _P]),
+    "ora_train_grads": (C.c_double, [C.POINTER(na.Config), C.c_int32, C.c_uint32, C.POINTER(na.Vertices), _P, _P,
+                                     _P, C.c_int32, _P, _P, _P, _P, _P, _P, C.c_double, C.c_double, _P, _P]),
+    "ora_adam": (None, [C.c_uint64, _P, _P, _P, _P, C.c_double, C.c_double, C.c_double, C.c_double,
+                        C.c_double, C.c_uint32]),
+}
+
+
+def _load_train():
+    lib = load()
+    if not getattr(lib, "_train_sigs", False):
+        for name, (res, args) in _SIGS_TRAIN.items():
+            fn = getattr(lib, name)
+            fn.restype = res
+            fn.argtypes = args
+        lib._train_sigs = True
+    return lib
+
+
+class RrsSide:
+    """Host side arrays of the RRS loss (the nrrs_rrs_side fields)."""
+
+    def __init__(self, ref_mean, error, sample_weight, q_target, error_sum):
+        self.ref_mean = None if ref_mean is None else np.ascontiguousarray(ref_mean, np.float32)
+        self.error = np.ascontiguousarray(error, np.float32)
+        self.sample_weight = np.ascontiguousarray(sample_weight, np.float32)
+        self.q_target = np.ascontiguousarray(q_target, np.float32)
+        self.error_sum = float(error_sum)
+
+
+def _side_ptrs(side):
+    if side is None:
+        return None, None, None, None, 0.0
+    return _p(side.ref_mean), _p(side.error), _p(side.sample_weight), _p(side.q_target), side.error_sum
+
+
+def loss(cfg, kind, hv, y, film=None, target_L=None, stats=None, side=None, y_den=None):
+    """y: [n_out][n] (FP64). Returns (loss, dy)."""
+    y = np.ascontiguousarray(y, np.float64)
+    dy = np.zeros_like(y)
+    rm, er, sw, qt, es = _side_ptrs(side)
+    film = None if film is None else np.ascontiguousarray(film, np.float32)
+    target_L = None if target_L is None else np.ascontiguousarray(target_L, np.float32)
+    stats = None if stats is None else np.ascontiguousarray(stats, np.float64)
+    y_den = None if y_den is None else np.ascontiguousarray(y_den, np.float64)
+    val = _load_train().ora_loss(C.byref(cfg), kind, hv.n, y.ctypes.data, _p(target_L), C.byref(hv.v), _p(film),
+                                 _p(stats), rm, er, sw, qt, es, _p(y_den), dy.ctypes.data)
+    return val, dy
+
+
+def train_grads(cfg, net, hv, params, kind, film=None, stats=None, dy_ext=None, target_L=None, side=None,
+                loss_scale=1.0):
+    """FP64 (loss, grad [n_params], dy [n_out][n])."""
+    g, m = param_count(cfg, net)
+    n_out = cfg.statnet.n_outputs if net == na.STATNET else cfg.rrsnet.n_outputs
+    grad = np.zeros(g + m, np.float64)
+    dy = np.zeros((n_out, hv.n), np.float64)
+    params = np.ascontiguousarray(params, np.float64)
+    film = None if film is None else np.ascontiguousarray(film, np.float32)
+    stats = None if stats is None else np.ascontiguousarray(stats, np.float64)
+    dy_ext = None if dy_ext is None else np.ascontiguousarray(dy_ext, np.float64)
+    target_L = None if target_L is None else np.ascontiguousarray(target_L, np.float32)
+    rm, er, sw, qt, es = _side_ptrs(side)
+    val = _load_train().ora_train_grads(C.byref(cfg), net, hv.n, C.byref(hv.v), _p(film), _p(stats),
+                                        params.ctypes.data, kind, _p(dy_ext), _p(target_L), rm, er, sw, qt, es,
+                                        loss_scale, grad.ctypes.data, dy.ctypes.data)
+    return val, grad, dy
+
+
+def adam(p, m, v, g, lr, b1, b2, eps, l2, t):
+    for a in (p, m, v, g):
+        assert a.dtype == np.float64 and a.flags.c_contiguous
+    _load_train().ora_adam(p.size, p.ctypes.data, m.ctypes.data, v.ctypes.data, g.ctypes.data, lr, b1, b2, eps,
+                           l2, t)
+
+
+def synth_targets(n, seed):
+    """Radiance targets: log-normal(-1, 1.5) per channel, 5% of samples x100."""
+    rng = np.random.default_rng(seed)
+    L = rng.lognormal(-1.0, 1.5, (3, n))
+    L[:, rng.uniform(size=n) < 0.05] *= 100.0
+    return L.astype(np.float32)

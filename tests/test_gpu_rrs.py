@@ -80,8 +80,17 @@ def _rel_err(a, b):
     return np.max(np.abs(a - b)) / max(np.max(np.abs(b)), 1e-30)
 
 
-def test_statnet_rrsnet_forward_match_fp64(torch_cuda, ora):
-    ctx = na.Context()
+# Output error relative to the output range, per precision of the MLP contractions:
+# FP32 vector path, FP16 and BF16 MFMA with FP32 accumulation (8 / 11-bit mantissas).
+OUT_TOL = {na.FP32: 1e-5, na.FP16: 5e-3, na.BF16: 4e-2}
+# Relative error of q = activationSigma(raw) against FP64.
+Q_TOL = {na.FP32: 1e-4, na.FP16: 5e-3, na.BF16: 3e-2}
+PRECISIONS = [na.FP32, na.FP16, na.BF16]
+
+
+@pytest.mark.parametrize("prec", PRECISIONS)
+def test_statnet_rrsnet_forward_match_fp64(torch_cuda, ora, prec):
+    ctx = na.Context(precision=prec)
     ps = random_params(ctx, na.STATNET, 11)
     pr = random_params(ctx, na.RRSNET, 12)
     vb = na.synth_vertices(N_SMALL, 31, N_SMALL)
@@ -96,10 +105,10 @@ def test_statnet_rrsnet_forward_match_fp64(torch_cuda, ora):
     gstats = stats.cpu().numpy()
     ref_ch = ora.net_forward(ctx.cfg, na.RRSNET, hv, film_np, gstats.astype(np.float64),
                              pr.astype(np.float64))[:, 0]
-    # FP32 path vs FP64: relative to the output range, 1e-5
-    assert _rel_err(gstats, ref_stats) < 1e-5
-    assert _rel_err(raw.cpu().numpy(), ref_raw) < 1e-5
-    assert _rel_err(raw_chained.cpu().numpy(), ref_ch) < 1e-5
+    errs = (_rel_err(gstats, ref_stats), _rel_err(raw.cpu().numpy(), ref_raw),
+            _rel_err(raw_chained.cpu().numpy(), ref_ch))
+    print(f"precision {prec}: statnet {errs[0]:.2e} rrsnet {errs[1]:.2e} chained {errs[2]:.2e}")
+    assert max(errs) < OUT_TOL[prec]
 
 
 def _decide_buffers(torch, n, capacity):
@@ -151,12 +160,13 @@ def test_decide_bit_exact(torch_cuda, ora, n, n_spec, capacity_frac):
         assert np.array_equal(b["qn"][:n].cpu().numpy(), ref["qn"])
 
 
+@pytest.mark.parametrize("prec", PRECISIONS)
 @pytest.mark.parametrize("mode", [na.Q_RRSNET, na.Q_RRSNET_CHAINED, na.Q_STATNET_ADRRS])
-def test_infer_decide_pipeline(torch_cuda, ora, mode):
+def test_infer_decide_pipeline(torch_cuda, ora, mode, prec):
     """Full stage: q within tolerance of FP64, decision bit-exact given the device q."""
     torch = torch_cuda
     n = N_SMALL
-    ctx = na.Context()
+    ctx = na.Context(precision=prec)
     ps = random_params(ctx, na.STATNET, 41)
     pr = random_params(ctx, na.RRSNET, 42)
     vb = na.synth_vertices(n, 77, ctx.cfg.n_pixels)
@@ -177,7 +187,14 @@ def test_infer_decide_pipeline(torch_cuda, ora, mode):
             stats = ora.net_forward(ctx.cfg, na.STATNET, hv, film_np, None, ps.astype(np.float64)).T
         src = ora.net_forward(ctx.cfg, na.RRSNET, hv, film_np, stats, pr.astype(np.float64))[:, 0]
     q64 = ora.q_f64(ctx.cfg, mode, hv, src, film_np)
-    assert np.max(np.abs(q - q64) / q64) < 1e-4
+    qerr = np.abs(q - q64) / q64
+    print(f"precision {prec} mode {mode}: q rel err max {qerr.max():.2e} mean {qerr.mean():.2e}")
+    if mode == na.Q_STATNET_ADRRS:
+        # q = mean(clamp(L) thp / I): relative error of L near 0 is unbounded in low
+        # precision, so the bound is on the error against the q range, and the mean
+        assert np.max(np.abs(q - q64)) / q64.max() < Q_TOL[prec] and qerr.mean() < Q_TOL[prec]
+    else:
+        assert qerr.max() < Q_TOL[prec]
     # the decision from the device's own FP32 q is bit-exact
     ref = ora.rrs_decide(ctx.cfg, q, stid=None, seed=5, stream=1)
     ns = int(b["n_surv"].item())
@@ -189,4 +206,4 @@ def test_infer_decide_pipeline(torch_cuda, ora, mode):
     ref64 = ora.rrs_decide(ctx.cfg, q64.astype(np.float32), seed=5, stream=1)
     flips = int(np.sum(ref64["s"] != ref["s"]))
     print(f"mode {mode}: {flips} of {n} split counts differ under the FP64 q")
-    assert flips <= n // 1000 + 2
+    assert flips <= n * Q_TOL[prec] + 2

@@ -1,0 +1,219 @@
+"""Training parity: hand-written backward kernels (encoding, MLP, both losses) and
+Adam against the FP64 oracle, central finite differences of the device forward,
+loss-scale handling, and that training reduces the loss."""
+import numpy as np
+import pytest
+
+import nrrs_amd as na
+
+pytestmark = pytest.mark.gpu
+
+N = 4096
+# Gradient error relative to the largest gradient entry, per operand precision
+# (FP32 MFMA, FP16 / BF16 MFMA with FP32 accumulation and FP32 gradient buffers).
+GRAD_TOL = {na.FP32: 2e-4, na.FP16: 1e-2, na.BF16: 5e-2}
+PRECISIONS = [na.FP32, na.FP16, na.BF16]
+
+
+@pytest.fixture(scope="module")
+def torch_cuda():
+    import torch
+
+    assert torch.cuda.is_available()
+    return torch
+
+
+def _setup(torch, prec, n=N, seed=1, **kw):
+    cfg = na.default_config(precision=prec, n_pixels=n, max_queue=n, max_train=n, **kw)
+    ctx = na.Context(cfg)
+    rng = np.random.default_rng(seed)
+    for net in (na.STATNET, na.RRSNET):
+        p = ctx.params_get(net)
+        g, _ = ctx.param_count(net)
+        p[:g] = rng.uniform(-0.3, 0.3, g).astype(np.float32)
+        ctx.params_set(net, p)
+    vb = na.synth_vertices(n, 100 + seed, n)
+    film_np = rng.lognormal(-1, 1, (3, n)).astype(np.float32)
+    return ctx, vb, film_np, torch.from_numpy(film_np).cuda()
+
+
+def _rel(a, b):
+    return float(np.max(np.abs(a - b)) / max(np.max(np.abs(b)), 1e-30))
+
+
+def _rrs_side_np(n, seed):
+    rng = np.random.default_rng(seed)
+    qt = rng.uniform(0.05, 5.0, n).astype(np.float32)
+    qt[rng.uniform(size=n) < 0.1] = -1.0
+    return dict(error=rng.uniform(0, 2, n).astype(np.float32),
+                sample_weight=(1.0 / rng.integers(1, 4, n)).astype(np.float32), q_target=qt,
+                error_sum=float(n) * 0.7)
+
+
+@pytest.mark.parametrize("prec", PRECISIONS)
+@pytest.mark.parametrize("net", [na.STATNET, na.RRSNET])
+def test_backward_external_matches_fp64(torch_cuda, ora, prec, net):
+    torch = torch_cuda
+    ctx, vb, film_np, film = _setup(torch, prec)
+    n_out = 6 if net == na.STATNET else 1
+    rng = np.random.default_rng(3)
+    stats_np = np.abs(rng.normal(0.5, 0.5, (6, N))).astype(np.float32) if net == na.RRSNET else None
+    stats = None if stats_np is None else torch.from_numpy(stats_np).cuda()
+    hv = ora.HostVertices(vb.to_numpy())
+    if prec == na.FP32:
+        # zero-mean noise: the weight gradients are pure cancellation sums
+        dy = rng.normal(0, 1e-3, (n_out, N)).astype(np.float32)
+    else:
+        # dL/dy of 1e-3 * y^2 / 2 plus noise: with zero-mean noise alone, ReLU-mask flips
+        # of the low-precision forward (O(1) errors on ~1% of the terms) dominate the
+        # cancellation sums; a coherent dL/dy measures the rounding of the backward itself
+        y64 = ora.net_forward(ctx.cfg, net, hv, film_np, None if stats_np is None else stats_np.astype(np.float64),
+                              ctx.params_get(net).astype(np.float64)).T
+        dy = (1e-3 * y64 + rng.normal(0, 1e-5, (n_out, N))).astype(np.float32)
+    ctx.grads_zero(net)
+    na.check(ctx.backward(net, vb, na.LOSS_EXTERNAL, film=film, stats=stats, dy=torch.from_numpy(dy).cuda()),
+             "backward")
+    g_dev = ctx.grads_get(net)
+    p = ctx.params_get(net).astype(np.float64)
+    _, g_ref, _ = ora.train_grads(ctx.cfg, net, hv, p, 0, film=film_np,
+                                  stats=None if stats_np is None else stats_np.astype(np.float64),
+                                  dy_ext=dy.astype(np.float64))
+    g, _m = ctx.param_count(net)
+    e_grid, e_mlp = _rel(g_dev[:g], g_ref[:g]), _rel(g_dev[g:], g_ref[g:])
+    print(f"prec {prec} net {net}: grid grad err {e_grid:.2e}, mlp grad err {e_mlp:.2e}")
+    assert e_grid < GRAD_TOL[prec] and e_mlp < GRAD_TOL[prec]
+
+
+@pytest.mark.parametrize("prec", PRECISIONS)
+def test_ll2_and_rrs_loss_backward(torch_cuda, ora, prec):
+    torch = torch_cuda
+    ctx, vb, film_np, film = _setup(torch, prec, seed=2)
+    hv = ora.HostVertices(vb.to_numpy())
+    tL = ora.synth_targets(N, 4)
+    loss = torch.zeros(1, device="cuda")
+    ctx.grads_zero(na.STATNET)
+    na.check(ctx.backward(na.STATNET, vb, na.LOSS_LL2, target_L=torch.from_numpy(tL).cuda(), loss=loss), "ll2")
+    p = ctx.params_get(na.STATNET).astype(np.float64)
+    l_ref, g_ref, _ = ora.train_grads(ctx.cfg, na.STATNET, hv, p, 1, film=film_np, target_L=tL)
+    g_dev = ctx.grads_get(na.STATNET)
+    print(f"prec {prec}: LL2 loss {loss.item():.6g} vs {l_ref:.6g}, grad err {_rel(g_dev, g_ref):.2e}")
+    assert abs(loss.item() - l_ref) / l_ref < GRAD_TOL[prec]
+    assert _rel(g_dev, g_ref) < GRAD_TOL[prec]
+
+    sd = _rrs_side_np(N, 5)
+    stats_np = np.abs(np.random.default_rng(6).normal(0.5, 0.5, (6, N))).astype(np.float32)
+    keep = {k: torch.from_numpy(v).cuda() for k, v in sd.items() if k != "error_sum"}
+    side = ctx.side(error=keep["error"], sample_weight=keep["sample_weight"], q_target=keep["q_target"],
+                    error_sum=sd["error_sum"])
+    ctx.grads_zero(na.RRSNET)
+    na.check(ctx.backward(na.RRSNET, vb, na.LOSS_RRS, film=film, stats=torch.from_numpy(stats_np).cuda(),
+                          side=side, loss=loss), "rrs")
+    p = ctx.params_get(na.RRSNET).astype(np.float64)
+    ref_side = ora.RrsSide(None, sd["error"], sd["sample_weight"], sd["q_target"], sd["error_sum"])
+    l_ref, g_ref, _ = ora.train_grads(ctx.cfg, na.RRSNET, hv, p, 2, film=film_np, stats=stats_np.astype(np.float64),
+                                      side=ref_side)
+    g_dev = ctx.grads_get(na.RRSNET)
+    print(f"prec {prec}: RRS loss {loss.item():.6g} vs {l_ref:.6g}, grad err {_rel(g_dev, g_ref):.2e}")
+    assert abs(loss.item() - l_ref) / l_ref < GRAD_TOL[prec]
+    assert _rel(g_dev, g_ref) < GRAD_TOL[prec]
+
+
+def test_finite_differences_of_device_forward(torch_cuda, ora):
+    """Central differences (eps 1e-3) of the FP32 device forward through the
+    frozen-denominator LL2 loss, on grid entries and MLP weights."""
+    torch = torch_cuda
+    ctx, vb, film_np, film = _setup(torch, na.FP32, seed=3)
+    tL = ora.synth_targets(N, 7)
+    hv = ora.HostVertices(vb.to_numpy())
+    y0 = ctx.statnet_infer(vb).cpu().numpy().astype(np.float64)
+    ctx.grads_zero(na.STATNET)
+    na.check(ctx.backward(na.STATNET, vb, na.LOSS_LL2, target_L=torch.from_numpy(tL).cuda()), "ll2")
+    grad = ctx.grads_get(na.STATNET).astype(np.float64)
+    p0 = ctx.params_get(na.STATNET)
+    g, m = ctx.param_count(na.STATNET)
+    rng = np.random.default_rng(0)
+    touched = np.nonzero(np.abs(grad[:g]) > 1e-3 * np.abs(grad[:g]).max())[0]
+    idx = np.concatenate([rng.choice(touched, 16, replace=False), g + rng.choice(m, 16, replace=False)])
+
+    def loss_at(p):
+        ctx.params_set(na.STATNET, p)
+        y = ctx.statnet_infer(vb).cpu().numpy().astype(np.float64)
+        return ora.loss(ctx.cfg, 1, hv, y, target_L=tL, y_den=y0)[0]
+
+    errs = []
+    for k in idx:
+        p1 = p0.copy()
+        p1[k] += 1e-3
+        p2 = p0.copy()
+        p2[k] -= 1e-3
+        fd = (loss_at(p1) - loss_at(p2)) / 2e-3
+        errs.append(abs(fd - grad[k]) / np.abs(grad[idx]).max())
+    ctx.params_set(na.STATNET, p0)
+    errs = np.array(errs)
+    print(f"FD errors: median {np.median(errs):.2e} max {errs.max():.2e}")
+    # FP32 forward noise / (2 eps) and ReLU kinks: a few may be off, the bulk is tight
+    assert np.median(errs) < 1e-3 and np.mean(errs < 1e-2) >= 0.9
+
+
+@pytest.mark.parametrize("prec", [na.FP32, na.BF16])
+def test_adam_matches_oracle(torch_cuda, ora, prec):
+    torch = torch_cuda
+    ctx, vb, film_np, film = _setup(torch, prec, seed=4, adam_l2_reg=1e-4)
+    tL = torch.from_numpy(ora.synth_targets(N, 8)).cuda()
+    p = ctx.params_get(na.STATNET).astype(np.float64)
+    mo, vo = np.zeros_like(p), np.zeros_like(p)
+    for step in range(1, 4):
+        ctx.grads_zero(na.STATNET)
+        na.check(ctx.backward(na.STATNET, vb, na.LOSS_LL2, target_L=tL,
+                              loss_scale=ctx.train_state(na.STATNET)["loss_scale"]), "ll2")
+        gd = ctx.grads_get(na.STATNET).astype(np.float64) / ctx.train_state(na.STATNET)["loss_scale"]
+        ctx.adam_step(na.STATNET)
+        ora.adam(p, mo, vo, gd, ctx.cfg.lr_statnet, ctx.cfg.adam_beta1, ctx.cfg.adam_beta2, ctx.cfg.adam_eps,
+                 ctx.cfg.adam_l2_reg, step)
+        pd = ctx.params_get(na.STATNET).astype(np.float64)
+        err = np.max(np.abs(pd - p))
+        print(f"prec {prec} step {step}: max |p_dev - p_ref| = {err:.2e}")
+        assert err < 1e-5 * max(1.0, step)
+        p = pd  # follow the device trajectory (FP32 rounding)
+    assert ctx.train_state(na.STATNET)["step"] == 3
+
+
+def test_loss_scale_overflow_skips_step(torch_cuda, ora):
+    """FP16: an initial scale of 2^40 overflows the FP16 dy operands; the step is
+    skipped (parameters unchanged, gradients cleared) and the scale halves."""
+    torch = torch_cuda
+    ctx, vb, _, _ = _setup(torch, na.FP16, seed=5, loss_scale_init=float(2 ** 40))
+    tL = torch.from_numpy(ora.synth_targets(N, 9)).cuda()
+    p0 = ctx.params_get(na.STATNET)
+    na.check(ctx.train_step(vb, 1, target_L=tL), "train_step")
+    st = ctx.train_state(na.STATNET)
+    assert st["skipped"] == 1 and st["step"] == 0 and st["loss_scale"] == float(2 ** 39)
+    assert np.array_equal(ctx.params_get(na.STATNET), p0)
+    assert not ctx.grads_get(na.STATNET).any()
+
+
+@pytest.mark.parametrize("prec", PRECISIONS)
+def test_training_reduces_losses(torch_cuda, ora, prec):
+    """StatNet alone, then RRSNet alone on the frozen StatNet (whose statistics are its
+    inputs, so its loss is stationary only then), then both jointly."""
+    torch = torch_cuda
+    n = 1 << 14
+    ctx, vb, film_np, film = _setup(torch, prec, n=n, seed=6, lr_rrsnet=1e-3)
+    tL = torch.from_numpy(ora.synth_targets(n, 10)).cuda()
+    sd = _rrs_side_np(n, 11)
+    keep = {k: torch.from_numpy(v).cuda() for k, v in sd.items() if k != "error_sum"}
+    side = ctx.side(error=keep["error"], sample_weight=keep["sample_weight"], q_target=keep["q_target"],
+                    error_sum=sd["error_sum"])
+    loss = torch.zeros(2, device="cuda")
+    hist = {}
+    for mask, steps in ((1, 40), (2, 40), (3, 10)):
+        h = []
+        for _ in range(steps):
+            na.check(ctx.train_step(vb, mask, film=film, target_L=tL, side=side, loss=loss), "train_step")
+            h.append(loss.cpu().numpy().copy())
+        hist[mask] = np.array(h)
+    print(f"prec {prec}: StatNet loss {hist[1][0, 0]:.4g} -> {hist[1][-1, 0]:.4g}, RRSNet {hist[2][0, 1]:.4g} -> "
+          f"{hist[2][-1, 1]:.4g}")
+    assert all(np.all(np.isfinite(h)) for h in hist.values())
+    assert hist[1][-5:, 0].mean() < 0.7 * hist[1][:3, 0].mean()
+    assert hist[2][-5:, 1].mean() < 0.9 * hist[2][:3, 1].mean()
