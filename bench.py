@@ -1,15 +1,21 @@
 """Benchmark of the neural RRS stage on one MI355X.
 
-    python bench.py [--gpus 1] [--steps K] [--warmup W] [--precision fp32] [--no-cpu]
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu]
 
-A step is one bounce of the RRS stage over 2^20 synthetic path vertices that
-are already resident in HBM: StatNet inference, RRSNet inference on the StatNet
-statistics, q, the exact normalisation sum, the split decision and the stable
-survivor compaction. `value` is RRS queries per second of the whole step.
+Workload: 2^20 synthetic path vertices resident in HBM (16-level hash grid with
+T = 2^19, SH-4 direction/normal encodings, 64-wide StatNet 4x64->6 and RRSNet
+3x64->1). One step is one online-training iteration of the stage:
+  1. the RRS stage of one bounce: StatNet -> RRSNet chained inference, q, the exact
+     normalisation sum, the split decision and the stable survivor compaction;
+  2. one joint training step on the same 2^20 vertices: StatNet (NRRS_LL2) and
+     RRSNet (NRRS_RRS) forward, backward and Adam.
+`value` is RRS queries per second of that whole step in BF16 (MFMA, FP32
+accumulation, FP32 master weights, dynamic loss scaling); the per-phase rates
+and the same step in FP32 and FP16 are reported beside it.
 
-The roofline entry covers the dominant kernel (the fused StatNet encode+MLP
-kernel), timed with HIP events on the stream it runs on. The CPU baseline is the
-FP64 oracle run single-threaded on a bounded sample of the same workload.
+The roofline entry covers the dominant kernel of the BF16 step, timed with HIP
+events on the stream it runs on. The CPU baseline is the FP64 oracle, single
+threaded, on a bounded sample of the same step.
 """
 import argparse
 import json
@@ -21,51 +27,141 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, os.path.join(ROOT, "nrrs-code_amd"))
 
 N_VERTICES = 1 << 20
-N_PIXELS = 1024 * 1024
+N_PIXELS = 1 << 20
+SEED = 0x4E525253 ^ 2
 HBM_PEAK_GBS = 8000.0
-FP32_PEAK_TFLOPS = 157.3
-MLP_MACS = {"statnet": 80 * 64 + 3 * 64 * 64 + 64 * 6, "rrsnet": 80 * 64 + 2 * 64 * 64 + 64 * 1}
-
-
-def statnet_bytes_per_query(table_bytes):
-    # input record (pos, dir, normal, rough, albedo_d, albedo_s) + 16 levels x 8 corners
-    # x 2 features + 6 FP32 outputs
-    return 64 + 16 * 8 * 2 * table_bytes + 24
+BF16_PEAK_TFLOPS = 2500.0
+MACS = {"statnet": 80 * 64 + 3 * 64 * 64 + 64 * 6, "rrsnet": 80 * 64 + 2 * 64 * 64 + 64 * 1}
 
 
 def parse():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=20)
-    ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--precision", default="fp32", choices=["fp32"])
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU-baseline leg")
-    ap.add_argument("--cpu-sample", type=int, default=1 << 15)
+    ap.add_argument("--cpu-sample", type=int, default=1 << 16)
+    ap.add_argument("--precisions", default="bf16,fp16,fp32")
     return ap.parse_args()
 
 
 def cpu_baseline(sample):
-    """FP64 oracle, one thread, on `sample` vertices of the same recipe."""
+    """FP64 oracle, one thread: the same step (chained inference + decision, then
+    StatNet and RRSNet forward/backward and Adam) on `sample` vertices."""
     import numpy as np
 
     sys.path.insert(0, os.path.join(ROOT, "tests"))
-    import oracle_ref as ora
     import nrrs_amd as na
+    import oracle_ref as ora
 
-    cfg = na.default_config(n_pixels=N_PIXELS, max_queue=sample)
-    d = ora.synth_vertices(sample, 0x4E525253 ^ 2, N_PIXELS)
+    cfg = na.default_config(n_pixels=N_PIXELS, max_queue=sample, max_train=sample)
+    d = ora.synth_vertices(sample, SEED, N_PIXELS)
     hv = ora.HostVertices(d)
-    film = np.random.default_rng(1).lognormal(-1, 1, (3, N_PIXELS)).astype(np.float32)
+    rng = np.random.default_rng(1)
+    film = rng.lognormal(-1, 1, (3, N_PIXELS)).astype(np.float32)
     ps = ora.init_params(cfg, na.STATNET, 1, 0.5)
     pr = ora.init_params(cfg, na.RRSNET, 2, 0.5)
+    tL = ora.synth_targets(sample, 3)
+    side = ora.RrsSide(None, rng.uniform(0, 2, sample), np.ones(sample), rng.uniform(0.05, 5, sample), sample * 0.7)
     t0 = time.perf_counter()
-    stats = ora.net_forward(cfg, na.STATNET, hv, film, None, ps).T
-    raw = ora.net_forward(cfg, na.RRSNET, hv, film, np.ascontiguousarray(stats), pr)[:, 0]
+    stats = np.ascontiguousarray(ora.net_forward(cfg, na.STATNET, hv, film, None, ps).T)
+    raw = ora.net_forward(cfg, na.RRSNET, hv, film, stats, pr)[:, 0]
     q = ora.q_f64(cfg, na.Q_RRSNET_CHAINED, hv, raw, film)
     ora.rrs_decide(cfg, q.astype(np.float32), seed=1, stream=0)
+    t1 = time.perf_counter()
+    for net, p, kind in ((na.STATNET, ps, na.LOSS_LL2), (na.RRSNET, pr, na.LOSS_RRS)):
+        _, g, _ = ora.train_grads(cfg, net, hv, p, kind, film=film, stats=stats if net == na.RRSNET else None,
+                                  target_L=tL, side=side)
+        m, v = np.zeros_like(p), np.zeros_like(p)
+        ora.adam(p, m, v, g, 1e-2, 0.9, 0.99, 1e-15, 0.0, 1)
     dt = time.perf_counter() - t0
-    return {"value": sample / dt, "unit": "queries/s", "cores": 1, "kind": "port",
-            "sample": f"{sample} vertices of the same workload, FP64 oracle, 1 thread, {dt:.2f} s"}
+    return {"value": round(sample / dt, 1), "unit": "queries/s", "cores": 1, "kind": "port",
+            "sample": f"{sample} vertices of the same step (inference + decision {t1 - t0:.2f} s, "
+                      f"training {dt - (t1 - t0):.2f} s), FP64 oracle, 1 thread, {dt:.2f} s total"}
+
+
+class Path:
+    """One precision: context, inputs and the timed step."""
+
+    def __init__(self, precision, torch, na, np):
+        self.torch, self.na = torch, na
+        n = N_VERTICES
+        self.n = n
+        self.ctx = na.Context(precision=precision, n_pixels=N_PIXELS, max_queue=n, max_train=n)
+        rng = np.random.default_rng(0)
+        for net in (na.STATNET, na.RRSNET):
+            p = self.ctx.params_get(net)
+            g, _ = self.ctx.param_count(net)
+            p[:g] = rng.uniform(-0.3, 0.3, g).astype(np.float32)
+            self.ctx.params_set(net, p)
+        self.vb = na.synth_vertices(n, SEED, N_PIXELS)
+        gen = torch.Generator(device="cuda")
+        gen.manual_seed(SEED)
+        dev = dict(device="cuda", dtype=torch.float32)
+        self.film = torch.exp(torch.randn(3, N_PIXELS, generator=gen, **dev) - 1.0)
+        L = torch.exp(torch.randn(3, n, generator=gen, **dev) * 1.5 - 1.0)
+        fire = torch.rand(n, generator=gen, **dev) < 0.05
+        self.target_L = torch.where(fire, L * 100.0, L).contiguous()
+        self.err = torch.rand(n, generator=gen, **dev) * 2.0
+        self.sw = torch.ones(n, **dev)
+        self.qt = torch.rand(n, generator=gen, **dev) * 5.0 + 0.05
+        self.side = self.ctx.side(error=self.err, sample_weight=self.sw, q_target=self.qt, error_sum=0.7 * N_PIXELS)
+        self.q = torch.empty(n, **dev)
+        self.qn = torch.empty(n, **dev)
+        self.split = torch.empty(n, dtype=torch.uint8, device="cuda")
+        self.surv = torch.empty(8 * n, dtype=torch.int32, device="cuda")
+        self.n_surv = torch.zeros(1, dtype=torch.int32, device="cuda")
+        self.ovf = torch.zeros(1, dtype=torch.int32, device="cuda")
+        self.loss = torch.zeros(2, **dev)
+        self.args = self.ctx.decide_args(self.vb, self.film, self.q, self.surv, self.n_surv, self.ovf,
+                                         q_mode=na.Q_RRSNET_CHAINED, qn=self.qn, split=self.split)
+
+    def infer(self, k):
+        self.args.pcg_stream = k
+        self.na.check(self.ctx.rrs_infer_decide(self.args), "rrs_infer_decide")
+
+    def train(self):
+        self.na.check(self.ctx.train_step(self.vb, 3, film=self.film, target_L=self.target_L, side=self.side,
+                                          loss=self.loss), "train_step")
+
+    def run(self, steps, warmup):
+        torch = self.torch
+        stream = torch.cuda.current_stream()
+        for k in range(warmup):
+            self.infer(k)
+            self.train()
+        torch.cuda.synchronize()
+        ev = [[torch.cuda.Event(enable_timing=True) for _ in range(3)] for _ in range(steps)]
+        t0 = time.perf_counter()
+        for k in range(steps):
+            ev[k][0].record(stream)
+            self.infer(warmup + k)
+            ev[k][1].record(stream)
+            self.train()
+            ev[k][2].record(stream)
+        torch.cuda.synchronize()
+        wall = time.perf_counter() - t0
+        inf = sum(e[0].elapsed_time(e[1]) for e in ev) / steps
+        trn = sum(e[1].elapsed_time(e[2]) for e in ev) / steps
+        return {"ms_per_step": 1e3 * wall / steps, "infer_ms": inf, "train_ms": trn,
+                "rrs_queries_per_s": self.n / (inf * 1e-3), "train_samples_per_s": self.n / (trn * 1e-3),
+                "step_queries_per_s": self.n * steps / wall, "loss": self.loss.cpu().tolist(),
+                "survivors": int(self.n_surv.item())}
+
+    def time_statnet_kernel(self, reps):
+        """Average duration of the fused StatNet encode+MLP inference kernel (one
+        launch per call), HIP events on the stream it is launched on."""
+        torch = self.torch
+        stream = torch.cuda.current_stream()
+        out = self.ctx.statnet_infer(self.vb)
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record(stream)
+        for _ in range(reps):
+            self.ctx.statnet_infer(self.vb)
+        e1.record(stream)
+        torch.cuda.synchronize()
+        del out
+        return e0.elapsed_time(e1) / 1e3 / reps
 
 
 def main():
@@ -79,82 +175,74 @@ def main():
     import nrrs_amd as na
 
     torch.cuda.set_device(0)
+    precs = {"bf16": na.BF16, "fp16": na.FP16, "fp32": na.FP32}
+    results, q_by = {}, {}
+    for name in args.precisions.split(","):
+        path = Path(precs[name], torch, na, np)
+        r = path.run(args.steps if name == "bf16" else max(2, args.steps // 2), args.warmup)
+        if name == "bf16":
+            kern_s = path.time_statnet_kernel(max(5, args.steps))
+            r["statnet_kernel_us"] = kern_s * 1e6
+        # q of the same (pre-training) parameters for the matched-error report
+        fresh = Path(precs[name], torch, na, np)
+        fresh.infer(0)
+        q_by[name] = fresh.q.cpu().numpy()
+        del fresh
+        results[name] = r
+        del path
+        torch.cuda.empty_cache()
+        print(f"[bench] {name}: {json.dumps({k: (round(v, 4) if isinstance(v, float) else v) for k, v in r.items()})}",
+              file=sys.stderr, flush=True)
+
+    bf = results["bf16"]
     n = N_VERTICES
-    ctx = na.Context(n_pixels=N_PIXELS, max_queue=n)
-    rng = np.random.default_rng(0)
-    for net in (na.STATNET, na.RRSNET):
-        p = ctx.params_get(net)
-        g, _ = ctx.param_count(net)
-        p[:g] = rng.uniform(-0.5, 0.5, g).astype(np.float32)
-        ctx.params_set(net, p)
-    vb = na.synth_vertices(n, 0x4E525253 ^ 2, N_PIXELS)
-    film = torch.from_numpy(rng.lognormal(-1, 1, (3, N_PIXELS)).astype(np.float32)).cuda()
-    q = torch.empty(n, device="cuda")
-    qn = torch.empty(n, device="cuda")
-    split = torch.empty(n, dtype=torch.uint8, device="cuda")
-    surv = torch.empty(8 * n, dtype=torch.int32, device="cuda")
-    n_surv = torch.zeros(1, dtype=torch.int32, device="cuda")
-    ovf = torch.zeros(1, dtype=torch.int32, device="cuda")
-    stream = torch.cuda.current_stream()
-
-    args_chain = ctx.decide_args(vb, film, q, surv, n_surv, ovf, q_mode=na.Q_RRSNET_CHAINED, qn=qn, split=split)
-
-    def step(k):
-        args_chain.pcg_stream = k
-        na.check(ctx.rrs_infer_decide(args_chain), "rrs_infer_decide")
-
-    for k in range(args.warmup):
-        step(k)
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    for k in range(args.steps):
-        step(args.warmup + k)
-    torch.cuda.synchronize()
-    dt = time.perf_counter() - t0
-    ms_per_step = 1e3 * dt / args.steps
-    qps = n * args.steps / dt
-
-    # dominant kernel: fused StatNet encode + MLP, HIP events on its stream
-    reps = max(args.steps, 5)
-    e0 = torch.cuda.Event(enable_timing=True)
-    e1 = torch.cuda.Event(enable_timing=True)
-    stats = ctx.statnet_infer(vb)
-    e0.record(stream)
-    for _ in range(reps):
-        ctx.statnet_infer(vb)
-    e1.record(stream)
-    torch.cuda.synchronize()
-    kern_s = e0.elapsed_time(e1) / 1e3 / reps
-    del stats
-    bytes_q = statnet_bytes_per_query(4)
-    flops_q = 2 * MLP_MACS["statnet"]
+    # algorithmic traffic and work of the fused StatNet inference kernel per query:
+    # input record 64 B + 16 levels x 8 corners x 2 BF16 features + 6 FP32 outputs
+    bytes_q = 64 + 16 * 8 * 2 * 2 + 24
+    flops_q = 2 * MACS["statnet"]
+    kern_s = bf["statnet_kernel_us"] * 1e-6
     gbs = bytes_q * n / kern_s / 1e9
     tflops = flops_q * n / kern_s / 1e12
-    # the FP32 path is VALU-bound: report against the FP32 peak
-    roofline = {"bound": "mfma", "achieved": round(tflops, 3), "peak": FP32_PEAK_TFLOPS, "unit": "TFLOP/s",
-                "frac": round(tflops / FP32_PEAK_TFLOPS, 4), "traffic": None,
-                "kernel": "k_net_fp32<5,6,EPI_RAW> (StatNet encode+MLP)",
-                "kernel_us": round(kern_s * 1e6, 2), "hbm_gbs_algorithmic": round(gbs, 1)}
-
+    roofline = {"bound": "hbm", "achieved": round(gbs, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                "frac": round(gbs / HBM_PEAK_GBS, 4), "traffic": None,
+                "kernel": "k_net_mfma<BF16,5,6,EPI_RAW> (StatNet encode + MLP)",
+                "kernel_us": round(bf["statnet_kernel_us"], 2),
+                "mfma_tflops": round(tflops, 2), "mfma_frac": round(tflops / BF16_PEAK_TFLOPS, 4)}
+    matched = {}
+    if "fp32" in q_by:
+        for name in q_by:
+            if name != "fp32":
+                rel = np.abs(q_by[name] - q_by["fp32"]) / q_by["fp32"]
+                matched[name] = {"q_rel_err_vs_fp32_max": float(rel.max()), "q_rel_err_vs_fp32_mean": float(rel.mean())}
     out = {
         "metric": "RRS queries/s + train samples/s at 2^20 vertices, BF16 vs FP32 at matched error",
-        "value": round(qps, 1),
+        "value": round(bf["step_queries_per_s"], 1),
         "unit": "queries/s",
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": round(ms_per_step, 4),
+        "ms_per_step": round(bf["ms_per_step"], 4),
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": args.precision,
-        "data": "synthetic (PCG32 seeded vertices, random-init grid and Xavier MLP weights)",
-        "config": {"workload": "RRS stage, StatNet->RRSNet chained inference + normalise + split + "
-                               "compaction, 2^20 vertices, 16-level hash grid T=2^19, 64-wide MLPs",
-                   "vertices": n, "pixels": N_PIXELS, "precision": args.precision},
+        "dtype": "bf16",
+        "data": "synthetic (PCG32-seeded vertices, log-normal radiance targets, random-init grid, Xavier MLP)",
+        "config": {"workload": "online RRS-stage step at 2^20 vertices: StatNet->RRSNet chained inference + "
+                               "normalise + split + compaction, then one joint StatNet+RRSNet training step",
+                   "vertices": n, "pixels": N_PIXELS, "grid": "16 levels x 2 features, T=2^19, base 16, scale 1.5",
+                   "mlp": "StatNet 80-64x4-6, RRSNet 80-64x3-1"},
+        "per_precision": {k: {kk: (round(vv, 2) if isinstance(vv, float) else vv) for kk, vv in v.items()}
+                          for k, v in results.items()},
+        "rrs_queries_per_s": round(bf["rrs_queries_per_s"], 1),
+        "train_samples_per_s": round(bf["train_samples_per_s"], 1),
+        "matched_error": matched,
         "roofline": roofline,
-        "survivors_last_step": int(n_surv.item()),
     }
+    if "fp32" in results:
+        out["bf16_over_fp32"] = {
+            "rrs_queries": round(bf["rrs_queries_per_s"] / results["fp32"]["rrs_queries_per_s"], 3),
+            "train_samples": round(bf["train_samples_per_s"] / results["fp32"]["train_samples_per_s"], 3),
+            "step": round(bf["step_queries_per_s"] / results["fp32"]["step_queries_per_s"], 3)}
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(args.cpu_sample)
     print(json.dumps(out))

@@ -8,10 +8,13 @@
 namespace nrrs {
 namespace {
 
-__global__ void k_check_finite(const float* __restrict__ g, uint64_t n, uint32_t* flag) {
+__global__ void k_check_finite(const float* __restrict__ g, uint64_t n, const uint16_t* __restrict__ glp,
+                               uint64_t n_lp, uint32_t* flag) {
     bool bad = false;
-    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x)
-        bad |= !isfinite(g[i]);
+    const uint64_t i0 = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x, step = (uint64_t)gridDim.x * blockDim.x;
+    for (uint64_t i = i0; i < n; i += step) bad |= !isfinite(g[i]);
+    // FP16 infinity / NaN: all exponent bits set
+    for (uint64_t i = i0; glp && i < n_lp; i += step) bad |= (glp[i] & 0x7c00u) == 0x7c00u;
     if (__any(bad) && (threadIdx.x & 63) == 0) atomicOr(flag, 1u);
 }
 
@@ -21,8 +24,14 @@ __global__ void k_adam(AdamArgs a) {
     const uint32_t t = a.state[0] + 1;
     const float c1 = 1.0f - powf(a.beta1, (float)t), c2 = 1.0f - powf(a.beta2, (float)t);
     for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < a.n; i += (uint64_t)gridDim.x * blockDim.x) {
-        float g = a.grad[i];
-        a.grad[i] = 0.0f;
+        float g;
+        if (a.grad_lp && i < a.n_grid) {
+            g = (float)__builtin_bit_cast(_Float16, a.grad_lp[i]);
+            a.grad_lp[i] = 0;
+        } else {
+            g = a.grad[i];
+            a.grad[i] = 0.0f;
+        }
         if (skip) continue;
         float p = a.params[i];
         g = g * inv_s + a.l2 * p;
@@ -64,7 +73,10 @@ __global__ void k_fill(float* p, float v, uint64_t n) {
 }  // namespace
 
 hipError_t launch_adam(const AdamArgs& a, hipStream_t s) {
-    hipLaunchKernelGGL(k_check_finite, dim3(1024), dim3(256), 0, s, a.grad, a.n, a.state + 3);
+    // with an FP16 grid gradient only the MLP part of `grad` is live
+    const uint64_t off = a.grad_lp ? a.n_grid : 0;
+    hipLaunchKernelGGL(k_check_finite, dim3(1024), dim3(256), 0, s, a.grad + off, a.n - off, a.grad_lp,
+                       a.grad_lp ? a.n_grid : 0, a.state + 3);
     hipLaunchKernelGGL(k_adam, dim3(2048), dim3(256), 0, s, a);
     hipLaunchKernelGGL(k_adam_finish, dim3(1), dim3(1), 0, s, a);
     return hipGetLastError();

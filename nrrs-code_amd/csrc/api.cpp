@@ -189,6 +189,10 @@ nrrs_status ensure_train(nrrs_ctx* c, int net) {
     const uint64_t np = ns.n_grid + ns.n_mlp;
     const size_t ob = train_op_bytes(c->cfg.precision);
     NRRS_TRY(dmalloc(&ns.grad, np));
+    if (c->cfg.precision != NRRS_FP32) {
+        NRRS_TRY(dmalloc(&ns.grad_lp, ns.n_grid));
+        NRRS_HIP(hipMemset(ns.grad_lp, 0, ns.n_grid * 2));
+    }
     NRRS_TRY(dmalloc(&ns.adam_m, np));
     NRRS_TRY(dmalloc(&ns.adam_v, np));
     NRRS_TRY(dmalloc(reinterpret_cast<uint8_t**>(&ns.w_fwd), (size_t)train_tiles_fwd(ns.n_layers) * 64 * ob));
@@ -246,6 +250,7 @@ nrrs_status run_backward(nrrs_ctx* c, int net, uint32_t n, const nrrs_vertices& 
     t.y_out = y_out;
     t.out_stride = out_stride;
     t.grad_grid = ns.grad;
+    t.grad_grid_h2 = reinterpret_cast<_Float16 __attribute__((ext_vector_type(2)))*>(ns.grad_lp);
     t.dw_partial = ns.dw_partial;
     t.n_mlp = (uint32_t)ns.n_mlp;
     t.loss_out = loss_out;
@@ -262,6 +267,7 @@ nrrs_status run_adam(nrrs_ctx* c, int net, hipStream_t s) {
     AdamArgs a{};
     a.params = ns.params;
     a.grad = ns.grad;
+    a.grad_lp = ns.grad_lp;
     a.m = ns.adam_m;
     a.v = ns.adam_v;
     a.n = ns.n_grid + ns.n_mlp;
@@ -294,6 +300,7 @@ void free_ctx(nrrs_ctx* c) {
         hipFree(ns.grid_lp);
         hipFree(ns.mlp_lp);
         hipFree(ns.grad);
+        hipFree(ns.grad_lp);
         hipFree(ns.adam_m);
         hipFree(ns.adam_v);
         hipFree(ns.w_fwd);
@@ -618,6 +625,8 @@ nrrs_status nrrs_grads_zero(nrrs_ctx* ctx, int32_t net, void* stream) {
     if (!ctx || (net != 0 && net != 1)) return NRRS_ERR_ARG;
     NRRS_TRY(ensure_train(ctx, net));
     NetState& ns = ctx->net[net];
+    if (ns.grad_lp)
+        NRRS_HIP(launch_fill_f32(reinterpret_cast<float*>(ns.grad_lp), 0.0f, ns.n_grid / 2, (hipStream_t)stream));
     return hip_status(launch_fill_f32(ns.grad, 0.0f, ns.n_grid + ns.n_mlp, (hipStream_t)stream));
 }
 
@@ -626,7 +635,13 @@ nrrs_status nrrs_grads_get(nrrs_ctx* ctx, int32_t net, float* host_dst) {
     NRRS_TRY(ensure_train(ctx, net));
     NetState& ns = ctx->net[net];
     NRRS_HIP(hipDeviceSynchronize());
-    return hip_status(hipMemcpy(host_dst, ns.grad, (ns.n_grid + ns.n_mlp) * 4, hipMemcpyDeviceToHost));
+    NRRS_HIP(hipMemcpy(host_dst, ns.grad, (ns.n_grid + ns.n_mlp) * 4, hipMemcpyDeviceToHost));
+    if (ns.grad_lp) {  // FP16 grid gradient of FP16/BF16 contexts
+        std::vector<_Float16> h(ns.n_grid);
+        NRRS_HIP(hipMemcpy(h.data(), ns.grad_lp, ns.n_grid * 2, hipMemcpyDeviceToHost));
+        for (uint64_t k = 0; k < ns.n_grid; ++k) host_dst[k] = (float)h[k];
+    }
+    return NRRS_OK;
 }
 
 nrrs_status nrrs_adam_step(nrrs_ctx* ctx, int32_t net, void* stream) {

@@ -70,6 +70,7 @@ struct NetState {
     // training state, allocated on first use
     bool train_ready = false;
     float* grad = nullptr;           // [grid | mlp]
+    uint16_t* grad_lp = nullptr;     // FP16 grid gradient of FP16/BF16 contexts [n_grid]
     float* adam_m = nullptr;
     float* adam_v = nullptr;
     void* w_fwd = nullptr;           // packed training tiles (forward / transposed)

@@ -28,6 +28,7 @@ typedef _Float16 f16x8 __attribute__((ext_vector_type(8)));
 typedef float f32x8 __attribute__((ext_vector_type(8)));
 typedef float f32x16 __attribute__((ext_vector_type(16)));
 typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
+typedef _Float16 h2v __attribute__((ext_vector_type(2)));
 
 // Operand policies: 8 values per lane per 16-deep K chunk.
 struct TBF16 {
@@ -448,8 +449,14 @@ __global__ __launch_bounds__(kThreads) void k_train(TrainArgs t) {
                                 for (int cn = 0; cn < 8; ++cn) {
                                     const uint32_t ent = t.a.grid.offset[l] + corner_index(t.a.grid, l, lc, cn);
                                     const float w = corner_weight(lc, cn);
-                                    atomicAdd(t.grad_grid + 2 * (size_t)ent, w * g0);
-                                    atomicAdd(t.grad_grid + 2 * (size_t)ent + 1, w * g1);
+                                    if (t.grad_grid_h2) {
+                                        // both features in one packed FP16 atomic
+                                        h2v gv = {(_Float16)(w * g0), (_Float16)(w * g1)};
+                                        __builtin_amdgcn_global_atomic_fadd_v2f16(t.grad_grid_h2 + ent, gv);
+                                    } else {
+                                        atomicAdd(t.grad_grid + 2 * (size_t)ent, w * g0);
+                                        atomicAdd(t.grad_grid + 2 * (size_t)ent + 1, w * g1);
+                                    }
                                 }
                             }
                         }

@@ -31,7 +31,8 @@ struct TrainArgs {
     // outputs
     float* y_out;                  // [n_out][out_stride] forward outputs, or null
     uint32_t out_stride;
-    float* grad_grid;              // [entries][2], atomically accumulated
+    float* grad_grid;              // [entries][2], atomically accumulated (FP32 contexts)
+    _Float16 __attribute__((ext_vector_type(2)))* grad_grid_h2;  // [entries] FP16 pairs (FP16/BF16)
     float* dw_partial;             // [gridDim.x][n_mlp]
     uint32_t n_mlp;
     float* loss_out;               // device scalar (atomic add), or null
@@ -47,7 +48,8 @@ hipError_t launch_reduce_partials(const float* part, uint32_t n_blocks, uint32_t
 
 struct AdamArgs {
     float* params;        // [n] FP32 masters
-    float* grad;          // [n], zeroed after use
+    float* grad;          // [n], zeroed after use (its grid part is unused with grad_lp)
+    uint16_t* grad_lp;    // [n_grid] FP16 grid gradient (FP16/BF16 contexts) or null
     float* m;
     float* v;
     uint64_t n;

@@ -12,6 +12,11 @@ N = 4096
 # Gradient error relative to the largest gradient entry, per operand precision
 # (FP32 MFMA, FP16 / BF16 MFMA with FP32 accumulation and FP32 gradient buffers).
 GRAD_TOL = {na.FP32: 2e-4, na.FP16: 1e-2, na.BF16: 5e-2}
+# Norm-wise error of the table gradient in low precision (see the external-dy test).
+GRID_NORM_TOL = {na.FP16: 5e-2, na.BF16: 1.5e-1}
+# FP16/BF16 contexts accumulate the table gradient in FP16 (packed atomics): the tests
+# scale the loss as training does, so that it is neither subnormal nor overflowing.
+SCALE = {na.FP32: 1.0, na.FP16: 4096.0, na.BF16: 4096.0}
 PRECISIONS = [na.FP32, na.FP16, na.BF16]
 
 
@@ -71,17 +76,26 @@ def test_backward_external_matches_fp64(torch_cuda, ora, prec, net):
                               ctx.params_get(net).astype(np.float64)).T
         dy = (1e-3 * y64 + rng.normal(0, 1e-5, (n_out, N))).astype(np.float32)
     ctx.grads_zero(net)
-    na.check(ctx.backward(net, vb, na.LOSS_EXTERNAL, film=film, stats=stats, dy=torch.from_numpy(dy).cuda()),
-             "backward")
-    g_dev = ctx.grads_get(net)
+    na.check(ctx.backward(net, vb, na.LOSS_EXTERNAL, film=film, stats=stats, dy=torch.from_numpy(dy).cuda(),
+                          loss_scale=SCALE[prec]), "backward")
+    g_dev = ctx.grads_get(net) / SCALE[prec]
     p = ctx.params_get(net).astype(np.float64)
     _, g_ref, _ = ora.train_grads(ctx.cfg, net, hv, p, 0, film=film_np,
                                   stats=None if stats_np is None else stats_np.astype(np.float64),
                                   dy_ext=dy.astype(np.float64))
     g, _m = ctx.param_count(net)
     e_grid, e_mlp = _rel(g_dev[:g], g_ref[:g]), _rel(g_dev[g:], g_ref[g:])
-    print(f"prec {prec} net {net}: grid grad err {e_grid:.2e}, mlp grad err {e_mlp:.2e}")
-    assert e_grid < GRAD_TOL[prec] and e_mlp < GRAD_TOL[prec]
+    n_grid = float(np.linalg.norm(g_dev[:g] - g_ref[:g]) / np.linalg.norm(g_ref[:g]))
+    print(f"prec {prec} net {net}: grid grad err max {e_grid:.2e} norm {n_grid:.2e}, mlp grad err {e_mlp:.2e}")
+    assert e_mlp < GRAD_TOL[prec]
+    if prec == na.FP32:
+        assert e_grid < GRAD_TOL[prec]
+    else:
+        # A fine-level table entry is touched by one or two vertices, so its gradient is
+        # one term w * dL/denc: a ReLU-mask flip of the low-precision forward in that
+        # vertex moves it by O(1). Entry-wise maxima are therefore not a precision
+        # measure there; the norm-wise error over the table is.
+        assert n_grid < GRID_NORM_TOL[prec]
 
 
 @pytest.mark.parametrize("prec", PRECISIONS)
@@ -92,10 +106,11 @@ def test_ll2_and_rrs_loss_backward(torch_cuda, ora, prec):
     tL = ora.synth_targets(N, 4)
     loss = torch.zeros(1, device="cuda")
     ctx.grads_zero(na.STATNET)
-    na.check(ctx.backward(na.STATNET, vb, na.LOSS_LL2, target_L=torch.from_numpy(tL).cuda(), loss=loss), "ll2")
+    na.check(ctx.backward(na.STATNET, vb, na.LOSS_LL2, target_L=torch.from_numpy(tL).cuda(), loss=loss,
+                          loss_scale=SCALE[prec]), "ll2")
     p = ctx.params_get(na.STATNET).astype(np.float64)
     l_ref, g_ref, _ = ora.train_grads(ctx.cfg, na.STATNET, hv, p, 1, film=film_np, target_L=tL)
-    g_dev = ctx.grads_get(na.STATNET)
+    g_dev = ctx.grads_get(na.STATNET) / SCALE[prec]
     print(f"prec {prec}: LL2 loss {loss.item():.6g} vs {l_ref:.6g}, grad err {_rel(g_dev, g_ref):.2e}")
     assert abs(loss.item() - l_ref) / l_ref < GRAD_TOL[prec]
     assert _rel(g_dev, g_ref) < GRAD_TOL[prec]
@@ -107,12 +122,12 @@ def test_ll2_and_rrs_loss_backward(torch_cuda, ora, prec):
                     error_sum=sd["error_sum"])
     ctx.grads_zero(na.RRSNET)
     na.check(ctx.backward(na.RRSNET, vb, na.LOSS_RRS, film=film, stats=torch.from_numpy(stats_np).cuda(),
-                          side=side, loss=loss), "rrs")
+                          side=side, loss=loss, loss_scale=SCALE[prec]), "rrs")
     p = ctx.params_get(na.RRSNET).astype(np.float64)
     ref_side = ora.RrsSide(None, sd["error"], sd["sample_weight"], sd["q_target"], sd["error_sum"])
     l_ref, g_ref, _ = ora.train_grads(ctx.cfg, na.RRSNET, hv, p, 2, film=film_np, stats=stats_np.astype(np.float64),
                                       side=ref_side)
-    g_dev = ctx.grads_get(na.RRSNET)
+    g_dev = ctx.grads_get(na.RRSNET) / SCALE[prec]
     print(f"prec {prec}: RRS loss {loss.item():.6g} vs {l_ref:.6g}, grad err {_rel(g_dev, g_ref):.2e}")
     assert abs(loss.item() - l_ref) / l_ref < GRAD_TOL[prec]
     assert _rel(g_dev, g_ref) < GRAD_TOL[prec]
