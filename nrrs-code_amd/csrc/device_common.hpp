@@ -130,43 +130,75 @@ __device__ __forceinline__ float corner_weight(const LevelCell& lc, int cn) {
     return (wx * wy) * wz;
 }
 
-// Two features of one table entry, as FP32.
+// Corner values of a level fetched as x-adjacent pairs. In a dense level the
+// corners (x, y, z) and (x+1, y, z) are consecutive entries; in a hashed level
+// with even x they are the aligned pair {h & ~1, h | 1} (the x prime of the
+// spatial hash is 1). One 2-entry load then replaces two, which cuts the cache
+// lines a vertex touches from 8 to 4..8 per level. Values are summed in corner
+// order 0..7, exactly as the per-corner form.
 template <typename TT>
-struct TableLoad;
+struct PairLoad;
 template <>
-struct TableLoad<float> {
-    __device__ __forceinline__ static float2 get(const float* t, uint32_t e) {
+struct PairLoad<float> {  // FP32 table: entry = 2 floats
+    typedef float T;
+    __device__ __forceinline__ static float2 one(const float* t, uint32_t e) {
         return reinterpret_cast<const float2*>(t)[e];
     }
-};
-template <>
-struct TableLoad<__half> {
-    __device__ __forceinline__ static float2 get(const __half* t, uint32_t e) {
-        __half2 h = reinterpret_cast<const __half2*>(t)[e];
-        return __half22float2(h);
+    __device__ __forceinline__ static void two(const float* t, uint32_t e, float2& a, float2& b) {
+        float4 q;
+        __builtin_memcpy(&q, t + 2 * (size_t)e, sizeof(q));
+        a = make_float2(q.x, q.y);
+        b = make_float2(q.z, q.w);
     }
 };
-template <>
-struct TableLoad<__hip_bfloat16> {
-    __device__ __forceinline__ static float2 get(const __hip_bfloat16* t, uint32_t e) {
-        uint32_t u = reinterpret_cast<const uint32_t*>(t)[e];
-        return make_float2(__uint_as_float(u << 16), __uint_as_float(u & 0xffff0000u));
+template <typename Dec>
+struct PairLoadLp {  // FP16/BF16 table: entry = one 32-bit word, decoded by Dec::pair
+    typedef uint32_t T;
+    __device__ __forceinline__ static float2 one(const uint32_t* t, uint32_t e) { return Dec::pair(t[e]); }
+    __device__ __forceinline__ static void two(const uint32_t* t, uint32_t e, float2& a, float2& b) {
+        uint2 q;
+        __builtin_memcpy(&q, t + e, sizeof(q));
+        a = Dec::pair(q.x);
+        b = Dec::pair(q.y);
     }
 };
 
-// Trilinear interpolation of the two features of level l.
-template <typename TT>
-__device__ __forceinline__ float2 grid_level(const TT* __restrict__ table, const GridParams& g, int l,
-                                             const float* x) {
+template <typename L>
+__device__ __forceinline__ float2 grid_level_pairs(const typename L::T* __restrict__ table, const GridParams& g,
+                                                   int l, const float* x) {
     LevelCell lc = level_cell(g, l, x);
+    const uint32_t off = g.offset[l];
+    float2 v[8];
+    if ((g.dense_mask >> l) & 1u) {
+        const uint32_t r = g.res[l];
+#pragma unroll
+        for (int yz = 0; yz < 4; ++yz) {
+            const uint32_t cy = lc.c[1] + (yz & 1), cz = lc.c[2] + (yz >> 1);
+            L::two(table, off + lc.c[0] + r * (cy + r * cz), v[2 * yz], v[2 * yz + 1]);
+        }
+    } else {
+        const bool odd = lc.c[0] & 1u;
+#pragma unroll
+        for (int yz = 0; yz < 4; ++yz) {
+            const uint32_t cy = lc.c[1] + (yz & 1), cz = lc.c[2] + (yz >> 1);
+            const uint32_t yzh = (cy * 2654435761u) ^ (cz * 805459861u);
+            const uint32_t h0 = (lc.c[0] ^ yzh) & g.hash_mask;
+            float2 lo, hi;
+            L::two(table, off + (h0 & ~1u), lo, hi);
+            v[2 * yz] = (h0 & 1u) ? hi : lo;
+            if (odd) {
+                v[2 * yz + 1] = L::one(table, off + (((lc.c[0] + 1) ^ yzh) & g.hash_mask));
+            } else {
+                v[2 * yz + 1] = (h0 & 1u) ? lo : hi;
+            }
+        }
+    }
     float2 acc = make_float2(0.0f, 0.0f);
 #pragma unroll
     for (int cn = 0; cn < 8; ++cn) {
-        uint32_t e = g.offset[l] + corner_index(g, l, lc, cn);
-        float w = corner_weight(lc, cn);
-        float2 v = TableLoad<TT>::get(table, e);
-        acc.x = __fmaf_rn(w, v.x, acc.x);
-        acc.y = __fmaf_rn(w, v.y, acc.y);
+        const float w = corner_weight(lc, cn);
+        acc.x = __fmaf_rn(w, v[cn].x, acc.x);
+        acc.y = __fmaf_rn(w, v[cn].y, acc.y);
     }
     return acc;
 }
@@ -234,7 +266,7 @@ __device__ __forceinline__ void encode_full(const EncodeArgs& a, int net, const 
 #pragma unroll
     for (int l = 0; l < kMaxLevels; ++l) {
         if (l < (int)a.grid.n_levels) {
-            float2 f = grid_level<TT>(table, a.grid, l, x);
+            float2 f = grid_level_pairs<PairLoad<TT>>(table, a.grid, l, x);
             e[2 * l] = f.x;
             e[2 * l + 1] = f.y;
         } else {

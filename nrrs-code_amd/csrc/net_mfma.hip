@@ -117,23 +117,6 @@ __global__ void k_to_lp(const float* __restrict__ src, uint16_t* __restrict__ ds
         dst[i] = lp_bits<P>(src[i]);
 }
 
-// Trilinear interpolation of level l from a low-precision table.
-template <typename P>
-__device__ __forceinline__ float2 grid_level_lp(const uint32_t* __restrict__ table, const GridParams& g, int l,
-                                                const float* x) {
-    LevelCell lc = level_cell(g, l, x);
-    float2 acc = make_float2(0.0f, 0.0f);
-#pragma unroll
-    for (int cn = 0; cn < 8; ++cn) {
-        uint32_t e = g.offset[l] + corner_index(g, l, lc, cn);
-        float w = corner_weight(lc, cn);
-        float2 v = P::pair(table[e]);
-        acc.x = __fmaf_rn(w, v.x, acc.x);
-        acc.y = __fmaf_rn(w, v.y, acc.y);
-    }
-    return acc;
-}
-
 // The 40 channels of vertex i that half h of the wavefront feeds to layer 0:
 // channel kb*16 + 8h + j goes to element j of K block kb.
 template <typename P>
@@ -148,7 +131,7 @@ __device__ __forceinline__ void encode_half(const EncodeArgs& a, int net, const 
         for (int ll = 0; ll < 4; ++ll) {
             const int l = 8 * kb + 4 * h + ll;
             float2 f = make_float2(0.0f, 0.0f);
-            if (l < (int)a.grid.n_levels) f = grid_level_lp<P>(table, a.grid, l, x);
+            if (l < (int)a.grid.n_levels) f = grid_level_pairs<PairLoadLp<P>>(table, a.grid, l, x);
             e[2 * ll] = f.x;
             e[2 * ll + 1] = f.y;
         }

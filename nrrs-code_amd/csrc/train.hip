@@ -226,17 +226,8 @@ __global__ __launch_bounds__(kThreads) void k_train(TrainArgs t) {
 #pragma unroll
                     for (int l = 0; l < kMaxLevels; ++l) {
                         float2 f = make_float2(0.0f, 0.0f);
-                        if (l < (int)t.a.grid.n_levels) {
-                            LevelCell lc = level_cell(t.a.grid, l, x);
-#pragma unroll
-                            for (int cn = 0; cn < 8; ++cn) {
-                                uint32_t ent = t.a.grid.offset[l] + corner_index(t.a.grid, l, lc, cn);
-                                float w = corner_weight(lc, cn);
-                                float2 v = P::pair(t.table_lp[ent]);
-                                f.x = __fmaf_rn(w, v.x, f.x);
-                                f.y = __fmaf_rn(w, v.y, f.y);
-                            }
-                        }
+                        if (l < (int)t.a.grid.n_levels)
+                            f = grid_level_pairs<PairLoadLp<P>>(t.table_lp, t.a.grid, l, x);
                         e[2 * l] = f.x;
                         e[2 * l + 1] = f.y;
                     }
