@@ -248,6 +248,48 @@ nrrs_status nrrs_adam_step(nrrs_ctx* ctx, int32_t net, void* stream);
 nrrs_status nrrs_train_state(nrrs_ctx* ctx, int32_t net, uint32_t* step, float* loss_scale,
                              uint32_t* skipped);
 
+/* ---------------- batched uneven wavefront: one persistent launch per frame ---------------- */
+
+/* One frame of `n_bounces` RRS stages. Queue 0 holds the primary paths [0, n_primary)
+ * (path id = pixel). Bounce b draws vertex k of its queue from the synthetic recipe with
+ * seed + b * 0x9E3779B97F4A7C15 (standing in for the closest-hit program that fills the
+ * inference queue, device.cu:97-108), runs StatNet -> RRSNet -> q -> normalise -> split
+ * (PCG32 stream (seed, 1000 + b), vertex k draws output k) and compacts the survivors'
+ * path ids into the next queue. Sizes stay on the device. FP16/BF16 contexts only. */
+typedef struct {
+    uint32_t n_primary;
+    uint32_t n_bounces;       /* 1..16 */
+    uint64_t seed;
+    uint64_t* seed_dev;       /* optional device seed (read instead of `seed`), for graph replays */
+    int32_t advance_seed;     /* with seed_dev: advance it at the end of the frame */
+    const float* film_dev;    /* [3][n_pixels] */
+    uint32_t* queue_dev;      /* [2][capacity] ping-pong path-id queues */
+    uint32_t capacity;        /* <= max_queue of the context */
+    uint32_t* sizes_dev;      /* [n_bounces + 1] size of the queue entering each bounce, then the final one */
+    float* q_dev;             /* optional [n_bounces][capacity] clamped q per bounce, or NULL */
+    uint32_t* trace_dev;      /* optional [n_bounces + 1][capacity] every queue, or NULL */
+    uint32_t* status_dev;     /* [1] bit 0 barrier timeout, bit 1 queue overflow (must start at 0) */
+} nrrs_frame_args;
+
+/* Tile shape of the frame kernel: wavefronts per workgroup (1, 2, 4) and vertices a
+ * workgroup takes per step M (64, 128, 256; a multiple of 32 * waves). */
+typedef struct {
+    uint32_t waves;
+    uint32_t m;
+} nrrs_tile_shape;
+
+nrrs_status nrrs_frame_run(nrrs_ctx* ctx, const nrrs_frame_args* a, void* stream);
+nrrs_status nrrs_frame_set_shape(nrrs_ctx* ctx, const nrrs_tile_shape* shape);
+/* Times every valid shape on this frame (synchronous) and keeps the fastest. */
+nrrs_status nrrs_frame_autotune(nrrs_ctx* ctx, const nrrs_frame_args* a, nrrs_tile_shape* best, float* best_ms,
+                                void* stream);
+/* Captures one frame (with the current shape) into a hipGraph owned by the context. */
+nrrs_status nrrs_frame_graph_capture(nrrs_ctx* ctx, const nrrs_frame_args* a, void* stream);
+nrrs_status nrrs_frame_graph_launch(nrrs_ctx* ctx, void* stream);
+/* The vertices bounce `bounce` of a frame draws for a queue (for tests and tools). */
+nrrs_status nrrs_frame_vertices(uint64_t seed, uint32_t bounce, const uint32_t* queue_dev, uint32_t n,
+                                const nrrs_vertices_out* out, void* stream);
+
 #ifdef __cplusplus
 }
 #endif

@@ -11,6 +11,7 @@
 
 #include "host_common.hpp"
 #include "train_common.hpp"
+#include "frame_common.hpp"
 
 using namespace nrrs;
 
@@ -290,6 +291,66 @@ nrrs_status run_adam(nrrs_ctx* c, int net, hipStream_t s) {
     return refresh_train(c, net, s);
 }
 
+// Frame scratch for queues of up to `cap` vertices (allocated outside any capture).
+nrrs_status ensure_frame(nrrs_ctx* c, uint32_t cap) {
+    if (c->frame_cap >= cap) return NRRS_OK;
+    hipFree(c->frame_q);
+    hipFree(c->frame_s);
+    hipFree(c->frame_tiles);
+    c->frame_q = nullptr;
+    c->frame_s = nullptr;
+    c->frame_tiles = nullptr;
+    c->frame_cap = 0;
+    NRRS_TRY(dmalloc(&c->frame_q, cap));
+    NRRS_TRY(dmalloc(&c->frame_s, cap));
+    NRRS_TRY(dmalloc(&c->frame_tiles, 2 * ((size_t)cap / 64 + 1)));
+    if (!c->frame_sums) NRRS_TRY(dmalloc(&c->frame_sums, 16));
+    if (!c->frame_bar) {
+        NRRS_TRY(dmalloc(&c->frame_bar, 2));
+        NRRS_HIP(hipMemset(c->frame_bar, 0, 8));
+        NRRS_HIP(hipDeviceSynchronize());
+    }
+    c->frame_cap = cap;
+    return NRRS_OK;
+}
+
+nrrs_status frame_params(nrrs_ctx* c, const nrrs_frame_args* a, FrameParams* f) {
+    if (!a || !a->film_dev || !a->queue_dev || !a->sizes_dev || !a->status_dev) return NRRS_ERR_ARG;
+    if (a->n_bounces < 1 || a->n_bounces > 16 || a->capacity == 0) return NRRS_ERR_ARG;
+    if (a->capacity > c->cfg.max_queue) return NRRS_ERR_SHAPE;
+    if (c->cfg.precision == NRRS_FP32) return NRRS_ERR_UNSUPPORTED;
+    NRRS_TRY(ensure_frame(c, a->capacity));
+    *f = FrameParams{};
+    f->grid = c->grid;
+    f->frame = c->frame;
+    f->d = c->dec;
+    f->table_stat = reinterpret_cast<const uint32_t*>(c->net[NRRS_STATNET].grid_lp);
+    f->table_rrs = reinterpret_cast<const uint32_t*>(c->net[NRRS_RRSNET].grid_lp);
+    f->w_stat = reinterpret_cast<const u32x4_t*>(c->net[NRRS_STATNET].mlp_lp);
+    f->w_rrs = reinterpret_cast<const u32x4_t*>(c->net[NRRS_RRSNET].mlp_lp);
+    f->film = a->film_dev;
+    f->n_pixels = c->cfg.n_pixels;
+    f->n_primary = a->n_primary;
+    f->n_bounces = a->n_bounces;
+    f->capacity = a->capacity;
+    f->cpw = c->frame_m / (32 * c->frame_waves);
+    f->seed = a->seed;
+    f->seed_dev = a->seed_dev;
+    f->advance_seed = a->advance_seed;
+    f->queue = a->queue_dev;
+    f->sizes = a->sizes_dev;
+    f->q_out = a->q_dev;
+    f->trace = a->trace_dev;
+    f->status = a->status_dev;
+    f->q_scr = c->frame_q;
+    f->s_scr = c->frame_s;
+    f->tile_sums = c->frame_tiles;
+    f->tile_offsets = c->frame_tiles + (a->capacity / 64 + 1);
+    f->sums = c->frame_sums;
+    f->bar = c->frame_bar;
+    return NRRS_OK;
+}
+
 bool side_ok(const nrrs_rrs_side* side) {
     return side && side->error && side->sample_weight && side->q_target;
 }
@@ -311,6 +372,13 @@ void free_ctx(nrrs_ctx* c) {
     }
     hipFree(c->train_stats);
     hipFree(c->scale_tmp);
+    if (c->frame_exec) hipGraphExecDestroy(c->frame_exec);
+    if (c->frame_graph) hipGraphDestroy(c->frame_graph);
+    hipFree(c->frame_q);
+    hipFree(c->frame_s);
+    hipFree(c->frame_tiles);
+    hipFree(c->frame_sums);
+    hipFree(c->frame_bar);
     hipFree(c->sum_dev);
     hipFree(c->tile_sums);
     hipFree(c->tile_offsets);
@@ -699,6 +767,105 @@ nrrs_status nrrs_train_step(nrrs_ctx* ctx, uint32_t net_mask, uint32_t n, const 
         NRRS_TRY(run_adam(ctx, NRRS_RRSNET, s));
     }
     return NRRS_OK;
+}
+
+nrrs_status nrrs_frame_set_shape(nrrs_ctx* ctx, const nrrs_tile_shape* shape) {
+    if (!ctx || !shape) return NRRS_ERR_ARG;
+    const uint32_t w = shape->waves, m = shape->m;
+    if ((w != 1 && w != 2 && w != 4) || (m != 64 && m != 128 && m != 256) || m % (32 * w)) return NRRS_ERR_ARG;
+    ctx->frame_waves = w;
+    ctx->frame_m = m;
+    return NRRS_OK;
+}
+
+nrrs_status nrrs_frame_run(nrrs_ctx* ctx, const nrrs_frame_args* a, void* stream) {
+    if (!ctx) return NRRS_ERR_ARG;
+    FrameParams f;
+    NRRS_TRY(frame_params(ctx, a, &f));
+    return hip_status(launch_frame(ctx->cfg.precision, f, (int)ctx->frame_waves, 0, (hipStream_t)stream));
+}
+
+nrrs_status nrrs_frame_autotune(nrrs_ctx* ctx, const nrrs_frame_args* a, nrrs_tile_shape* best, float* best_ms,
+                                void* stream) {
+    if (!ctx) return NRRS_ERR_ARG;
+    hipStream_t s = (hipStream_t)stream;
+    const nrrs_tile_shape shapes[] = {{1, 64}, {1, 128}, {1, 256}, {2, 64}, {2, 128}, {2, 256}, {4, 128}, {4, 256}};
+    hipEvent_t e0, e1;
+    NRRS_HIP(hipEventCreate(&e0));
+    NRRS_HIP(hipEventCreate(&e1));
+    float best_t = 1e30f;
+    nrrs_tile_shape best_s = {ctx->frame_waves, ctx->frame_m};
+    nrrs_status st = NRRS_OK;
+    for (const nrrs_tile_shape& sh : shapes) {
+        if ((st = nrrs_frame_set_shape(ctx, &sh)) != NRRS_OK) break;
+        FrameParams f;
+        if ((st = frame_params(ctx, a, &f)) != NRRS_OK) break;
+        float t = 0.0f;
+        for (int rep = 0; rep < 2 && st == NRRS_OK; ++rep) {  // first launch warms up
+            st = hip_status(hipEventRecord(e0, s));
+            if (st == NRRS_OK) st = hip_status(launch_frame(ctx->cfg.precision, f, (int)sh.waves, 0, s));
+            if (st == NRRS_OK) st = hip_status(hipEventRecord(e1, s));
+            if (st == NRRS_OK) st = hip_status(hipEventSynchronize(e1));
+            if (st == NRRS_OK) st = hip_status(hipEventElapsedTime(&t, e0, e1));
+        }
+        if (st != NRRS_OK) break;
+        uint32_t status = 0;
+        if ((st = hip_status(hipMemcpy(&status, a->status_dev, 4, hipMemcpyDeviceToHost))) != NRRS_OK) break;
+        if (status & 1u) {
+            st = NRRS_ERR_HIP;
+            break;
+        }
+        if (t < best_t) {
+            best_t = t;
+            best_s = sh;
+        }
+    }
+    hipEventDestroy(e0);
+    hipEventDestroy(e1);
+    NRRS_TRY(st);
+    NRRS_TRY(nrrs_frame_set_shape(ctx, &best_s));
+    if (best) *best = best_s;
+    if (best_ms) *best_ms = best_t;
+    return NRRS_OK;
+}
+
+nrrs_status nrrs_frame_graph_capture(nrrs_ctx* ctx, const nrrs_frame_args* a, void* stream) {
+    if (!ctx) return NRRS_ERR_ARG;
+    hipStream_t s = (hipStream_t)stream;
+    if (!s) return NRRS_ERR_ARG;  // the legacy default stream cannot be captured
+    FrameParams f;
+    NRRS_TRY(frame_params(ctx, a, &f));  // allocates outside the capture
+    if (ctx->frame_exec) {
+        hipGraphExecDestroy(ctx->frame_exec);
+        ctx->frame_exec = nullptr;
+    }
+    if (ctx->frame_graph) {
+        hipGraphDestroy(ctx->frame_graph);
+        ctx->frame_graph = nullptr;
+    }
+    NRRS_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+    hipError_t e = launch_frame(ctx->cfg.precision, f, (int)ctx->frame_waves, 0, s);
+    hipGraph_t g = nullptr;
+    hipError_t e2 = hipStreamEndCapture(s, &g);
+    if (e != hipSuccess || e2 != hipSuccess) {
+        if (g) hipGraphDestroy(g);
+        return NRRS_ERR_HIP;
+    }
+    ctx->frame_graph = g;
+    return hip_status(hipGraphInstantiate(&ctx->frame_exec, g, nullptr, nullptr, 0));
+}
+
+nrrs_status nrrs_frame_graph_launch(nrrs_ctx* ctx, void* stream) {
+    if (!ctx || !ctx->frame_exec) return NRRS_ERR_ARG;
+    return hip_status(hipGraphLaunch(ctx->frame_exec, (hipStream_t)stream));
+}
+
+nrrs_status nrrs_frame_vertices(uint64_t seed, uint32_t bounce, const uint32_t* queue_dev, uint32_t n,
+                                const nrrs_vertices_out* o, void* stream) {
+    if (!o || !queue_dev || !o->pos || !o->dir || !o->normal || !o->rough || !o->albedo_d || !o->albedo_s ||
+        !o->thp || !o->pixel || o->stride < n)
+        return NRRS_ERR_ARG;
+    return hip_status(launch_bounce_vertices(seed, bounce, queue_dev, n, *o, (hipStream_t)stream));
 }
 
 }  // extern "C"

@@ -8,6 +8,13 @@
 
 #include "host_common.hpp"
 
+// No implicit FMA contraction in the device code that includes this header: the
+// encoding and the epilogues then round identically in every kernel that inlines
+// them (the per-net kernels, the persistent frame kernel, the training kernel), so
+// their results agree bit for bit. Fused multiply-adds that are wanted are written
+// out (__fmaf_rn); the MLPs run on MFMA.
+#pragma clang fp contract(off)
+
 namespace nrrs {
 
 // ---------------------------------------------------------------- PCG32
@@ -46,6 +53,60 @@ __device__ __forceinline__ uint32_t pcg_next(uint64_t& state, uint64_t stream) {
 
 // 24-bit dyadic uniform in [0,1)
 __device__ __forceinline__ float u24(uint32_t x) { return (float)(x >> 8) * 0x1p-24f; }
+
+// ---------------------------------------------------------------- synthetic vertices
+// Recipe of the benchmark workloads: fields on separate PCG32 streams of one seed;
+// vertex i draws fixed output numbers of each stream, so vertices are independent.
+//   stream 0: 64 triangles, 9 corner coordinates each, uniform in [0,1]
+//   stream 1: (triangle = x0 & 63, barycentrics from u1, u2) at outputs 3i..3i+2
+//   stream 2/3: direction / normal uniform on the sphere, outputs 2i, 2i+1
+//   stream 4: roughness U[0.05, 1]
+//   stream 5/6: diffuse / specular albedo U[0,1]^3, outputs 3i+k
+//   stream 7: throughput log-normal(-1, 1) per channel, Box-Muller from 6i+2k, 6i+2k+1
+struct SynthVertex {
+    float pos[3], dir[3], nrm[3], rough, ad[3], as[3], thp[3];
+};
+
+__device__ __forceinline__ SynthVertex synth_vertex(uint64_t seed, uint64_t i) {
+    SynthVertex v;
+    const float two_pi = 6.283185307179586f;
+    {
+        uint64_t st = pcg_state_at(seed, 1, 3ull * i);
+        uint32_t a = pcg_next(st, 1), b = pcg_next(st, 1), c = pcg_next(st, 1);
+        uint64_t tst = pcg_state_at(seed, 0, 9ull * (a & 63u));
+        float T[9];
+#pragma unroll
+        for (int k = 0; k < 9; ++k) T[k] = u24(pcg_next(tst, 0));
+        float su = sqrtf(u24(b));
+        float b0 = 1.0f - su, b1 = u24(c) * su, b2 = 1.0f - b0 - b1;
+#pragma unroll
+        for (int k = 0; k < 3; ++k) v.pos[k] = b0 * T[k] + b1 * T[3 + k] + b2 * T[6 + k];
+    }
+#pragma unroll
+    for (int f = 0; f < 2; ++f) {
+        float* dst = f ? v.nrm : v.dir;
+        uint64_t st = pcg_state_at(seed, 2 + f, 2ull * i);
+        float u1 = u24(pcg_next(st, 2 + f)), u2 = u24(pcg_next(st, 2 + f));
+        float z = 1.0f - 2.0f * u1;
+        float rr = sqrtf(fmaxf(0.0f, 1.0f - z * z));
+        float ph = two_pi * u2;
+        dst[0] = rr * cosf(ph);
+        dst[1] = rr * sinf(ph);
+        dst[2] = z;
+    }
+    v.rough = 0.05f + 0.95f * u24(pcg_output(pcg_state_at(seed, 4, i)));
+    uint64_t sa = pcg_state_at(seed, 5, 3ull * i), ss = pcg_state_at(seed, 6, 3ull * i),
+             st = pcg_state_at(seed, 7, 6ull * i);
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+        v.ad[k] = u24(pcg_next(sa, 5));
+        v.as[k] = u24(pcg_next(ss, 6));
+        float ua = u24(pcg_next(st, 7)), ub = u24(pcg_next(st, 7));
+        float nrm = sqrtf(-2.0f * logf(1.0f - ua)) * cosf(two_pi * ub);
+        v.thp[k] = expf(-1.0f + nrm);
+    }
+    return v;
+}
 
 // ---------------------------------------------------------------- scalar maps
 // activationSigma (src/render/nrrs/net/networkcommon.h:61-68): softplus for x<0,
@@ -219,43 +280,81 @@ __device__ __forceinline__ void load_xn(const EncodeArgs& a, uint32_t i, float* 
     for (int k = 0; k < 3; ++k) x[k] = norm_coord(a.v.pos[k * S + i], a.frame.c[k], a.frame.d[k]);
 }
 
-// Non-grid channels [32,80).
-__device__ __forceinline__ void encode_tail(const EncodeArgs& a, int net, uint32_t i, float* e) {
+// One path vertex in registers (position already in the normalised frame).
+struct VertexRec {
+    float x[3], dir[3], nrm[3], rough, ad[3], as[3], thp[3];
+    uint32_t pixel;
+};
+
+__device__ __forceinline__ VertexRec vertex_from_soa(const EncodeArgs& a, uint32_t i) {
+    VertexRec v;
     const uint32_t S = a.v.stride;
-    sh4(a.v.dir[i], a.v.dir[S + i], a.v.dir[2 * S + i], e + 32);
-    sh4(a.v.normal[i], a.v.normal[S + i], a.v.normal[2 * S + i], e + 48);
+    load_xn(a, i, v.x);
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+        v.dir[k] = a.v.dir[k * S + i];
+        v.nrm[k] = a.v.normal[k * S + i];
+        v.ad[k] = a.v.albedo_d[k * S + i];
+        v.as[k] = a.v.albedo_s[k * S + i];
+        v.thp[k] = a.v.thp[k * S + i];
+    }
+    v.rough = a.v.rough[i];
+    v.pixel = a.v.pixel[i];
+    return v;
+}
+
+// Mean of the pixel estimate (film) of a pixel.
+__device__ __forceinline__ float film_mean(const float* film, uint32_t n_pixels, uint32_t px) {
+    return (film[px] + film[n_pixels + px] + film[2 * n_pixels + px]) * (1.0f / 3.0f);
+}
+
+// Non-grid channels [32,80) of a vertex. RRSNet: ref = pixel mean, st = StatNet
+// statistics (L_rgb, L2_rgb) or null.
+__device__ __forceinline__ void encode_tail_rec(const VertexRec& v, int net, float ref, const float* st, float* e) {
+    sh4(v.dir[0], v.dir[1], v.dir[2], e + 32);
+    sh4(v.nrm[0], v.nrm[1], v.nrm[2], e + 48);
 #pragma unroll
     for (int k = 64; k < kEnc; ++k) e[k] = 0.0f;
-    e[64] = warp_roughness(a.v.rough[i]);
+    e[64] = warp_roughness(v.rough);
     if (net == NRRS_STATNET) {
 #pragma unroll
         for (int k = 0; k < 3; ++k) {
-            e[65 + k] = a.v.albedo_d[k * S + i];
-            e[68 + k] = a.v.albedo_s[k * S + i];
+            e[65 + k] = v.ad[k];
+            e[68 + k] = v.as[k];
         }
         e[71] = 1.0f;
     } else {
-        const uint32_t px = a.v.pixel[i];
-        const uint32_t P = a.n_pixels;
-        float ref = (a.film[px] + a.film[P + px] + a.film[2 * P + px]) * (1.0f / 3.0f);
 #pragma unroll
-        for (int k = 0; k < 3; ++k) e[65 + k] = box_cox(a.v.thp[k * S + i]);
+        for (int k = 0; k < 3; ++k) e[65 + k] = box_cox(v.thp[k]);
         e[68] = box_cox(ref);
-        if (a.stats) {
+        if (st) {
 #pragma unroll
             for (int k = 0; k < 3; ++k) {
-                e[69 + k] = box_cox(fmaxf(a.stats[(size_t)k * a.stats_stride + i], 0.0f));
-                e[72 + k] = box_cox(fmaxf(a.stats[(size_t)(3 + k) * a.stats_stride + i], 0.0f));
+                e[69 + k] = box_cox(fmaxf(st[k], 0.0f));
+                e[72 + k] = box_cox(fmaxf(st[3 + k], 0.0f));
             }
         } else {
 #pragma unroll
             for (int k = 0; k < 3; ++k) {
-                e[69 + k] = a.v.albedo_d[k * S + i];
-                e[72 + k] = a.v.albedo_s[k * S + i];
+                e[69 + k] = v.ad[k];
+                e[72 + k] = v.as[k];
             }
         }
         e[75] = 1.0f;
     }
+}
+
+// Non-grid channels of vertex i of a SoA batch.
+__device__ __forceinline__ void encode_tail(const EncodeArgs& a, int net, uint32_t i, float* e) {
+    VertexRec v = vertex_from_soa(a, i);
+    float st[6];
+    const bool have_st = net == NRRS_RRSNET && a.stats;
+    if (have_st) {
+#pragma unroll
+        for (int k = 0; k < 6; ++k) st[k] = a.stats[(size_t)k * a.stats_stride + i];
+    }
+    const float ref = net == NRRS_RRSNET ? film_mean(a.film, a.n_pixels, v.pixel) : 0.0f;
+    encode_tail_rec(v, net, ref, have_st ? st : nullptr, e);
 }
 
 template <typename TT>

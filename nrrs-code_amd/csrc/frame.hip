@@ -1,0 +1,313 @@
+// Batched uneven wavefront: all bounces of a frame in one persistent launch.
+//
+// Bounce b processes the queue of path ids left by bounce b-1 (bounce 0: every
+// primary path). For queue slot k the path vertex is drawn from the synthetic
+// recipe with seed_b = seed + b * 0x9E3779B97F4A7C15 (it stands in for the
+// closest-hit program that produces the inference queue, device.cu:97-108),
+// then: StatNet -> RRSNet (statistics passed in registers) -> q -> exact
+// normalisation sum -> split decision -> stable compaction of the survivors into
+// the next queue. Queue sizes never leave the device, so the host neither waits
+// nor reads anything back between bounces (the reference reads the queue size back
+// every bounce, integrator.cpp:1533-1555).
+//
+// Phases are separated by a software grid barrier. The grid never exceeds the
+// number of workgroups the device keeps resident, and every wait is bounded: a
+// barrier that does not complete within its spin limit sets status bit 0 and all
+// workgroups leave at their next barrier.
+#include "frame_common.hpp"
+#include "mfma_common.hpp"
+
+namespace nrrs {
+namespace {
+
+using namespace mfma;
+
+constexpr uint32_t kSpinLimit = 1u << 22;
+constexpr uint64_t kBounceSeedStep = 0x9E3779B97F4A7C15ull;
+
+// Bounded sense-by-generation barrier over all workgroups of the launch.
+__device__ __forceinline__ bool grid_sync(uint32_t* bar, uint32_t nblocks, uint32_t* status) {
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        __threadfence();
+        const uint32_t gen = __hip_atomic_load(bar + 1, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT);
+        const uint32_t arrived = __hip_atomic_fetch_add(bar, 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
+        if (arrived == nblocks - 1) {
+            __hip_atomic_store(bar, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            __hip_atomic_store(bar + 1, gen + 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+        } else {
+            uint32_t it = 0;
+            while (__hip_atomic_load(bar + 1, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) == gen) {
+                if (++it > kSpinLimit) {
+                    atomicOr(status, 1u);
+                    break;
+                }
+                __builtin_amdgcn_s_sleep(1);
+            }
+        }
+        __threadfence();
+    }
+    __syncthreads();
+    return (__hip_atomic_load(status, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) & 1u) == 0u;
+}
+
+template <int BLOCK>
+__device__ __forceinline__ uint32_t block_scan_excl(uint32_t v, uint32_t* total, uint32_t* wsum) {
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    uint32_t inc = v;
+#pragma unroll
+    for (int off = 1; off < 64; off <<= 1) {
+        uint32_t y = __shfl_up(inc, off, 64);
+        if (lane >= off) inc += y;
+    }
+    __syncthreads();
+    if (lane == 63) wsum[wave] = inc;
+    __syncthreads();
+    uint32_t base = 0, tot = 0;
+#pragma unroll
+    for (int w = 0; w < BLOCK / 64; ++w) {
+        const uint32_t x = wsum[w];
+        base += (w < wave) ? x : 0u;
+        tot += x;
+    }
+    *total = tot;
+    return base + inc - v;
+}
+
+template <typename P, int WAVES>
+__global__ __launch_bounds__(64 * WAVES) void k_frame(FrameParams f) {
+    constexpr int BLOCK = 64 * WAVES;
+    constexpr int kTilesS = tiles_of(5), kTilesR = tiles_of(4);
+    extern __shared__ u32x4 smem[];
+    u32x4* sWs = smem;
+    u32x4* sWr = smem + kTilesS * 64;
+    __shared__ uint32_t wsum[WAVES];
+    for (int k = threadIdx.x; k < kTilesS * 64; k += BLOCK) sWs[k] = reinterpret_cast<const u32x4*>(f.w_stat)[k];
+    for (int k = threadIdx.x; k < kTilesR * 64; k += BLOCK) sWr[k] = reinterpret_cast<const u32x4*>(f.w_rrs)[k];
+
+    const int lane = threadIdx.x & 63, h = lane >> 5, col = lane & 31, wave = threadIdx.x >> 6;
+    const uint32_t nb = gridDim.x;
+    const uint32_t cap = f.capacity;
+    const uint64_t seed = f.seed_dev ? *f.seed_dev : f.seed;
+
+    // frame set-up: queue 0 = all primary paths, per-bounce sums cleared
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        f.sizes[0] = f.n_primary < cap ? f.n_primary : cap;
+        for (uint32_t b = 0; b < f.n_bounces; ++b) f.sums[b] = 0ull;
+        if (f.n_primary > cap) atomicOr(f.status, 2u);
+    }
+    for (uint32_t k = blockIdx.x * BLOCK + threadIdx.x; k < f.n_primary && k < cap; k += nb * BLOCK) f.queue[k] = k;
+    if (!grid_sync(f.bar, nb, f.status)) return;
+
+    for (uint32_t b = 0; b < f.n_bounces; ++b) {
+        const uint32_t n = __hip_atomic_load(f.sizes + b, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        const uint32_t* qin = f.queue + (size_t)(b & 1u) * cap;
+        uint32_t* qout = f.queue + (size_t)((b + 1) & 1u) * cap;
+        const uint64_t seed_b = seed + (uint64_t)b * kBounceSeedStep;
+        if (f.trace)
+            for (uint32_t k = blockIdx.x * BLOCK + threadIdx.x; k < n; k += nb * BLOCK)
+                f.trace[(size_t)b * cap + k] = qin[k];
+
+        // ---- phase A: StatNet -> RRSNet -> q, exact normalisation sum
+        {
+            const uint32_t n_cb = (n + 31) / 32;
+            const uint32_t total_waves = nb * WAVES;
+            const uint32_t gw = blockIdx.x * WAVES + wave;
+            uint64_t term = 0;
+            for (uint32_t chunk = gw; chunk * f.cpw < n_cb; chunk += total_waves) {
+                for (uint32_t j = 0; j < f.cpw; ++j) {
+                    const uint32_t cb = chunk * f.cpw + j;
+                    if (cb >= n_cb) break;
+                    const uint32_t i = cb * 32 + col;
+                    const bool live = i < n;
+                    typename P::vec8 B[5];
+                    VertexRec v;
+                    float ref = 0.0f;
+                    if (live) {
+                        const SynthVertex sv = synth_vertex(seed_b, i);
+#pragma unroll
+                        for (int k = 0; k < 3; ++k) {
+                            v.x[k] = norm_coord(sv.pos[k], f.frame.c[k], f.frame.d[k]);
+                            v.dir[k] = sv.dir[k];
+                            v.nrm[k] = sv.nrm[k];
+                            v.ad[k] = sv.ad[k];
+                            v.as[k] = sv.as[k];
+                            v.thp[k] = sv.thp[k];
+                        }
+                        v.rough = sv.rough;
+                        v.pixel = qin[i];
+                        ref = film_mean(f.film, f.n_pixels, v.pixel);
+                        encode_half_rec<P>(v, f.grid, NRRS_STATNET, f.table_stat, h, 0.0f, nullptr, B);
+                    } else {
+#pragma unroll
+                        for (int kb = 0; kb < 5; ++kb) B[kb] = typename P::vec8{};
+                    }
+                    const f32x16 Ys = mlp_forward<P, 5>(sWs, B, lane);
+                    // statistics of the lane's vertex: outputs 0..3 in half 0, 4..5 in half 1
+                    float st[6];
+                    {
+                        float a[4], o[4];
+#pragma unroll
+                        for (int r = 0; r < 4; ++r) {
+                            a[r] = Ys[r];
+                            o[r] = __shfl_xor(a[r], 32, 64);
+                        }
+                        if (h == 0) {
+                            st[0] = a[0]; st[1] = a[1]; st[2] = a[2]; st[3] = a[3]; st[4] = o[0]; st[5] = o[1];
+                        } else {
+                            st[0] = o[0]; st[1] = o[1]; st[2] = o[2]; st[3] = o[3]; st[4] = a[0]; st[5] = a[1];
+                        }
+                    }
+                    if (live) encode_half_rec<P>(v, f.grid, NRRS_RRSNET, f.table_rrs, h, ref, st, B);
+                    const f32x16 Yr = mlp_forward<P, 4>(sWr, B, lane);
+                    if (live && h == 0) {
+                        const float q = rrs_window_clamp(f.d, activation_sigma(Yr[0]));
+                        f.q_scr[i] = q;
+                        if (f.q_out) f.q_out[(size_t)b * cap + i] = q;
+                        term += rrs_sum_term(f.d, q);
+                    }
+                }
+            }
+#pragma unroll
+            for (int o = 32; o > 0; o >>= 1) term += __shfl_xor(term, o, 64);
+            if (lane == 0 && term) atomicAdd(f.sums + b, (unsigned long long)term);
+        }
+        if (!grid_sync(f.bar, nb, f.status)) return;
+
+        // ---- phase B: normalise, draw, split counts per tile
+        const uint32_t n_tiles = (n + BLOCK - 1) / BLOCK;
+        {
+            const double sum = (double)__hip_atomic_load(f.sums + b, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) *
+                               0x1p-32;
+            const float sum_f = (float)sum;
+            const float rcp = sum_f > 0.0f ? 1.0f / sum_f : 0.0f;
+            float nf = (float)f.d.n_pixels * rcp;
+            nf = nf * (f.d.use_ceil ? 1.0f : f.d.rate);
+            for (uint32_t t = blockIdx.x; t < n_tiles; t += nb) {
+                const uint32_t i = t * BLOCK + threadIdx.x;
+                uint32_t si = 0;
+                if (i < n) {
+                    float r = f.q_scr[i];
+                    r = f.d.no_upscale ? r * (nf < 1.0f ? nf : 1.0f) : r * nf;
+                    const float u = u24(pcg_output(pcg_state_at(seed, 1000ull + b, i)));
+                    int s = (int)r;
+                    const float left = r - (float)s;
+                    s += (u <= left) ? 1 : 0;
+                    si = (uint32_t)min(s, (int)f.d.split_cap);
+                    f.s_scr[i] = (uint8_t)si;
+                }
+                uint32_t tot;
+                block_scan_excl<BLOCK>(si, &tot, wsum);
+                if (threadIdx.x == 0) f.tile_sums[t] = tot;
+            }
+        }
+        if (!grid_sync(f.bar, nb, f.status)) return;
+
+        // ---- phase C: scan of the tile totals (one workgroup)
+        if (blockIdx.x == 0) {
+            const uint32_t per = (n_tiles + BLOCK - 1) / BLOCK;
+            const uint32_t t0 = threadIdx.x * per;
+            uint32_t local = 0;
+            for (uint32_t t = t0; t < t0 + per && t < n_tiles; ++t) local += f.tile_sums[t];
+            uint32_t tot;
+            uint32_t run = block_scan_excl<BLOCK>(local, &tot, wsum);
+            for (uint32_t t = t0; t < t0 + per && t < n_tiles; ++t) {
+                f.tile_offsets[t] = run;
+                run += f.tile_sums[t];
+            }
+            if (threadIdx.x == 0) {
+                const uint32_t kept = tot < cap ? tot : cap;
+                __hip_atomic_store(f.sizes + b + 1, kept, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                if (tot > cap) atomicOr(f.status, 2u);
+            }
+        }
+        if (!grid_sync(f.bar, nb, f.status)) return;
+
+        // ---- phase D: stable scatter of the survivors into the next queue
+        for (uint32_t t = blockIdx.x; t < n_tiles; t += nb) {
+            const uint32_t i = t * BLOCK + threadIdx.x;
+            const uint32_t si = i < n ? f.s_scr[i] : 0u;
+            uint32_t tot;
+            const uint32_t pos = f.tile_offsets[t] + block_scan_excl<BLOCK>(si, &tot, wsum);
+            if (si) {
+                const uint32_t id = qin[i];
+                for (uint32_t c = 0; c < si; ++c)
+                    if (pos + c < cap) qout[pos + c] = id;
+            }
+        }
+        if (!grid_sync(f.bar, nb, f.status)) return;
+    }
+    if (f.trace) {
+        const uint32_t n = __hip_atomic_load(f.sizes + f.n_bounces, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        const uint32_t* qf = f.queue + (size_t)(f.n_bounces & 1u) * cap;
+        for (uint32_t k = blockIdx.x * BLOCK + threadIdx.x; k < n; k += nb * BLOCK)
+            f.trace[(size_t)f.n_bounces * cap + k] = qf[k];
+    }
+    if (f.seed_dev && f.advance_seed && blockIdx.x == 0 && threadIdx.x == 0) {
+        // the next replay of a captured frame draws a fresh frame; every workgroup has
+        // read the seed before the first barrier
+        *f.seed_dev = seed + 0xD1B54A32D192ED03ull;
+    }
+}
+
+__global__ void k_bounce_vertices(uint64_t seed, uint32_t bounce, const uint32_t* __restrict__ queue, uint32_t n,
+                                  nrrs_vertices_out o) {
+    const uint32_t i = blockIdx.x * 256 + threadIdx.x;
+    if (i >= n) return;
+    const SynthVertex v = synth_vertex(seed + (uint64_t)bounce * kBounceSeedStep, i);
+    const uint32_t S = o.stride;
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+        o.pos[k * S + i] = v.pos[k];
+        o.dir[k * S + i] = v.dir[k];
+        o.normal[k * S + i] = v.nrm[k];
+        o.albedo_d[k * S + i] = v.ad[k];
+        o.albedo_s[k * S + i] = v.as[k];
+        o.thp[k * S + i] = v.thp[k];
+    }
+    o.rough[i] = v.rough;
+    o.pixel[i] = queue[i];
+}
+
+template <typename P, int WAVES>
+hipError_t launch_frame_t(const FrameParams& f, uint32_t max_blocks, hipStream_t s) {
+    const size_t smem = (size_t)(tiles_of(5) + tiles_of(4)) * 64 * 16;
+    auto kern = k_frame<P, WAVES>;
+    int per_cu = 0;
+    hipError_t e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, reinterpret_cast<const void*>(kern),
+                                                                64 * WAVES, smem);
+    if (e != hipSuccess) return e;
+    int dev = 0, cus = 0;
+    if ((e = hipGetDevice(&dev)) != hipSuccess) return e;
+    if ((e = hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev)) != hipSuccess) return e;
+    if (per_cu < 1) return hipErrorInvalidConfiguration;
+    uint32_t blocks = (uint32_t)per_cu * (uint32_t)cus;  // all resident at once
+    if (max_blocks && blocks > max_blocks) blocks = max_blocks;
+    hipLaunchKernelGGL(kern, dim3(blocks), dim3(64 * WAVES), smem, s, f);
+    return hipGetLastError();
+}
+
+template <typename P>
+hipError_t launch_frame_p(const FrameParams& f, int waves, uint32_t max_blocks, hipStream_t s) {
+    if (waves == 1) return launch_frame_t<P, 1>(f, max_blocks, s);
+    if (waves == 2) return launch_frame_t<P, 2>(f, max_blocks, s);
+    if (waves == 4) return launch_frame_t<P, 4>(f, max_blocks, s);
+    return hipErrorInvalidValue;
+}
+
+}  // namespace
+
+hipError_t launch_frame(int precision, const FrameParams& f, int waves, uint32_t max_blocks, hipStream_t s) {
+    if (precision == NRRS_BF16) return launch_frame_p<BF16>(f, waves, max_blocks, s);
+    if (precision == NRRS_FP16) return launch_frame_p<F16>(f, waves, max_blocks, s);
+    return hipErrorInvalidValue;
+}
+
+hipError_t launch_bounce_vertices(uint64_t seed, uint32_t bounce, const uint32_t* queue, uint32_t n,
+                                  const nrrs_vertices_out& o, hipStream_t s) {
+    if (!n) return hipSuccess;
+    hipLaunchKernelGGL(k_bounce_vertices, dim3((n + 255) / 256), dim3(256), 0, s, seed, bounce, queue, n, o);
+    return hipGetLastError();
+}
+
+}  // namespace nrrs

@@ -1,0 +1,42 @@
+// Arguments of the persistent frame kernel.
+#pragma once
+
+#include "host_common.hpp"
+
+namespace nrrs {
+
+typedef uint32_t u32x4_t __attribute__((ext_vector_type(4)));
+
+struct FrameParams {
+    GridParams grid;
+    Frame frame;
+    DecideParams d;
+    const uint32_t* table_stat;   // FP16/BF16 grids (two features per word)
+    const uint32_t* table_rrs;
+    const u32x4_t* w_stat;        // packed MFMA tiles
+    const u32x4_t* w_rrs;
+    const float* film;
+    uint32_t n_pixels;
+    uint32_t n_primary, n_bounces, capacity, cpw;
+    uint64_t seed;
+    uint64_t* seed_dev;           // when set, the seed is read here (graph replays)
+    int advance_seed;
+    uint32_t* queue;              // [2][capacity]
+    uint32_t* sizes;              // [n_bounces + 1]
+    float* q_out;                 // [n_bounces][capacity] or null
+    uint32_t* trace;              // [n_bounces + 1][capacity] or null
+    uint32_t* status;
+    // scratch
+    float* q_scr;
+    uint8_t* s_scr;
+    uint32_t* tile_sums;
+    uint32_t* tile_offsets;
+    unsigned long long* sums;     // [n_bounces]
+    uint32_t* bar;                // [2] barrier count / generation
+};
+
+hipError_t launch_frame(int precision, const FrameParams& f, int waves, uint32_t max_blocks, hipStream_t s);
+hipError_t launch_bounce_vertices(uint64_t seed, uint32_t bounce, const uint32_t* queue, uint32_t n,
+                                  const nrrs_vertices_out& o, hipStream_t s);
+
+}  // namespace nrrs

@@ -97,6 +97,16 @@ struct nrrs_ctx {
     float* q_scratch = nullptr;             // [max_queue]
     uint8_t* s_scratch = nullptr;           // [max_queue]
     uint32_t max_tiles = 0;
+    // frame (persistent multi-bounce) state
+    uint32_t frame_cap = 0;
+    float* frame_q = nullptr;
+    uint8_t* frame_s = nullptr;
+    uint32_t* frame_tiles = nullptr;        // [2][cap/64 + 1]
+    unsigned long long* frame_sums = nullptr;  // [16]
+    uint32_t* frame_bar = nullptr;          // [2]
+    uint32_t frame_waves = 4, frame_m = 128;
+    hipGraph_t frame_graph = nullptr;
+    hipGraphExec_t frame_exec = nullptr;
     // training scratch
     float* train_stats = nullptr;           // [6][max_train] StatNet outputs of the batch
     float* scale_tmp = nullptr;             // device scalar for nrrs_backward's loss scale

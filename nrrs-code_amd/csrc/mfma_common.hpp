@@ -1,0 +1,175 @@
+#pragma once
+// MFMA building blocks of the mixed-precision inference path (v_mfma_f32_32x32x16_{bf16,f16},
+// FP32 accumulation), shared by the per-net kernels and the persistent frame kernel.
+//
+// Layout. A wavefront processes 32 vertices at a time; lane l owns vertex
+// l % 32 and half h = l / 32 of every 16-wide K block, which is exactly the B
+// operand layout of the 32x32x16 MFMA (B[k][n]: n = l % 32, k = 8h + j). The
+// encoding is computed directly in that layout (each half gathers 8 of the 16
+// grid levels). The weights are the A operand (A[m][k]: m = l % 32, k = 8h + j),
+// pre-packed per (layer, 32-row tile, K block) into 1 KiB LDS tiles.
+//
+// The accumulator of a layer (D[m][n]: n = l % 32, m = 8(r/4) + 4h + r%4 for
+// register r) is fed to the next layer without any shuffle: registers 8g..8g+7
+// of output tile t become K block 2t+g, and the next layer's weights are packed
+// with their input columns permuted to match, p(kb,h,j) = 16kb + 8(j/4) + 4h + j%4.
+#include "device_common.hpp"
+
+namespace nrrs {
+namespace mfma {
+
+
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef _Float16 f16x8 __attribute__((ext_vector_type(8)));
+typedef float f32x16 __attribute__((ext_vector_type(16)));
+typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
+
+struct BF16 {
+    typedef bf16x8 vec8;
+    __device__ static float2 pair(uint32_t u) {
+        return make_float2(__uint_as_float(u << 16), __uint_as_float(u & 0xffff0000u));
+    }
+    __device__ static vec8 pack(const float* x) {
+        vec8 v;
+#pragma unroll
+        for (int j = 0; j < 8; ++j) v[j] = (__bf16)x[j];
+        return v;
+    }
+    __device__ static f32x16 mma(u32x4 a, vec8 b, f32x16 c) {
+        return __builtin_amdgcn_mfma_f32_32x32x16_bf16(__builtin_bit_cast(vec8, a), b, c, 0, 0, 0);
+    }
+};
+
+struct F16 {
+    typedef f16x8 vec8;
+    __device__ static float2 pair(uint32_t u) {
+        _Float16 lo = __builtin_bit_cast(_Float16, (uint16_t)(u & 0xffffu));
+        _Float16 hi = __builtin_bit_cast(_Float16, (uint16_t)(u >> 16));
+        return make_float2((float)lo, (float)hi);
+    }
+    __device__ static vec8 pack(const float* x) {
+        vec8 v;
+#pragma unroll
+        for (int j = 0; j < 8; ++j) v[j] = (_Float16)x[j];
+        return v;
+    }
+    __device__ static f32x16 mma(u32x4 a, vec8 b, f32x16 c) {
+        return __builtin_amdgcn_mfma_f32_32x32x16_f16(__builtin_bit_cast(vec8, a), b, c, 0, 0, 0);
+    }
+};
+
+template <typename P>
+__device__ __forceinline__ uint16_t lp_bits(float f);
+template <>
+__device__ __forceinline__ uint16_t lp_bits<BF16>(float f) {
+    return __builtin_bit_cast(uint16_t, (__bf16)f);
+}
+template <>
+__device__ __forceinline__ uint16_t lp_bits<F16>(float f) {
+    return __builtin_bit_cast(uint16_t, (_Float16)f);
+}
+
+constexpr int kTileHalves = 64 * 8;  // one packed A tile: 64 lanes x 8 values
+
+__host__ __device__ constexpr int tiles_of(int n_layers) { return 10 + (n_layers - 2) * 8 + 4; }
+
+// Input column of the packed K element (layer, kb, h, j).
+__device__ __forceinline__ int packed_col(int layer, int kb, int h, int j) {
+    return layer == 0 ? kb * 16 + 8 * h + j : 16 * kb + 8 * (j >> 2) + 4 * h + (j & 3);
+}
+
+// The 40 channels of a vertex that half h of the wavefront feeds to layer 0:
+// channel kb*16 + 8h + j goes to element j of K block kb.
+template <typename P>
+__device__ __forceinline__ void encode_half_rec(const VertexRec& v, const GridParams& grid, int net,
+                                                const uint32_t* __restrict__ table, int h, float ref,
+                                                const float* st, typename P::vec8* B) {
+    float e[8];
+#pragma unroll
+    for (int kb = 0; kb < 2; ++kb) {
+#pragma unroll
+        for (int ll = 0; ll < 4; ++ll) {
+            const int l = 8 * kb + 4 * h + ll;
+            float2 f = make_float2(0.0f, 0.0f);
+            if (l < (int)grid.n_levels) f = grid_level_pairs<PairLoadLp<P>>(table, grid, l, v.x);
+            e[2 * ll] = f.x;
+            e[2 * ll + 1] = f.y;
+        }
+        B[kb] = P::pack(e);
+    }
+    float t[kEnc];
+    encode_tail_rec(v, net, ref, st, t);
+#pragma unroll
+    for (int kb = 2; kb < 5; ++kb) {
+#pragma unroll
+        for (int j = 0; j < 8; ++j) e[j] = h ? t[kb * 16 + 8 + j] : t[kb * 16 + j];
+        B[kb] = P::pack(e);
+    }
+}
+
+template <typename P>
+__device__ __forceinline__ void encode_half(const EncodeArgs& a, int net, const uint32_t* __restrict__ table,
+                                            uint32_t i, int h, typename P::vec8* B) {
+    VertexRec v = vertex_from_soa(a, i);
+    float st[6];
+    const bool have_st = net == NRRS_RRSNET && a.stats;
+    if (have_st) {
+#pragma unroll
+        for (int k = 0; k < 6; ++k) st[k] = a.stats[(size_t)k * a.stats_stride + i];
+    }
+    const float ref = net == NRRS_RRSNET ? film_mean(a.film, a.n_pixels, v.pixel) : 0.0f;
+    encode_half_rec<P>(v, a.grid, net, table, h, ref, have_st ? st : nullptr, B);
+}
+
+__device__ __forceinline__ u32x4 lds_tile(const u32x4* __restrict__ sW, int tile, int lane) {
+    return sW[tile * 64 + lane];
+}
+
+template <typename P>
+__device__ __forceinline__ void relu_to_operands(const f32x16* D, typename P::vec8* H) {
+#pragma unroll
+    for (int ot = 0; ot < 2; ++ot) {
+#pragma unroll
+        for (int g = 0; g < 2; ++g) {
+            float x[8];
+#pragma unroll
+            for (int j = 0; j < 8; ++j) x[j] = fmaxf(D[ot][8 * g + j], 0.0f);
+            H[2 * ot + g] = P::pack(x);
+        }
+    }
+}
+
+// Fused MLP of one net from the layer-0 operands B (5 K chunks); weights are the
+// packed A tiles at sW. Returns the output accumulator: output c of the lane's
+// vertex sits in lane half c / 4, register c % 4.
+template <typename P, int NL>
+__device__ __forceinline__ f32x16 mlp_forward(const u32x4* __restrict__ sW, const typename P::vec8* B, int lane) {
+    f32x16 D[2];
+#pragma unroll
+    for (int ot = 0; ot < 2; ++ot) {
+        D[ot] = f32x16{};
+#pragma unroll
+        for (int kb = 0; kb < 5; ++kb) D[ot] = P::mma(lds_tile(sW, ot * 5 + kb, lane), B[kb], D[ot]);
+    }
+    typename P::vec8 H[4];
+#pragma unroll
+    for (int layer = 1; layer < NL - 1; ++layer) {
+        relu_to_operands<P>(D, H);
+        const int base = 10 + (layer - 1) * 8;
+#pragma unroll
+        for (int ot = 0; ot < 2; ++ot) {
+            D[ot] = f32x16{};
+#pragma unroll
+            for (int kb = 0; kb < 4; ++kb) D[ot] = P::mma(lds_tile(sW, base + ot * 4 + kb, lane), H[kb], D[ot]);
+        }
+    }
+    relu_to_operands<P>(D, H);
+    f32x16 Y = f32x16{};
+    const int base = 10 + (NL - 2) * 8;
+#pragma unroll
+    for (int kb = 0; kb < 4; ++kb) Y = P::mma(lds_tile(sW, base + kb, lane), H[kb], Y);
+    return Y;
+}
+
+}  // namespace mfma
+}  // namespace nrrs

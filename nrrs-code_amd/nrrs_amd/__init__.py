@@ -131,6 +131,27 @@ class RrsSide(C.Structure):
 
 LOSS_EXTERNAL, LOSS_LL2, LOSS_RRS = 0, 1, 2
 
+
+class FrameArgs(C.Structure):
+    _fields_ = [
+        ("n_primary", C.c_uint32),
+        ("n_bounces", C.c_uint32),
+        ("seed", C.c_uint64),
+        ("seed_dev", C.c_void_p),
+        ("advance_seed", C.c_int32),
+        ("film_dev", C.c_void_p),
+        ("queue_dev", C.c_void_p),
+        ("capacity", C.c_uint32),
+        ("sizes_dev", C.c_void_p),
+        ("q_dev", C.c_void_p),
+        ("trace_dev", C.c_void_p),
+        ("status_dev", C.c_void_p),
+    ]
+
+
+class TileShape(C.Structure):
+    _fields_ = [("waves", C.c_uint32), ("m", C.c_uint32)]
+
 # name -> (restype, argtypes)
 _P = C.c_void_p
 _SIGS = {
@@ -159,6 +180,12 @@ _SIGS = {
     "nrrs_adam_step": (C.c_int, [_P, C.c_int32, _P]),
     "nrrs_train_state": (C.c_int, [_P, C.c_int32, C.POINTER(C.c_uint32), C.POINTER(C.c_float),
                                    C.POINTER(C.c_uint32)]),
+    "nrrs_frame_run": (C.c_int, [_P, C.POINTER(FrameArgs), _P]),
+    "nrrs_frame_set_shape": (C.c_int, [_P, C.POINTER(TileShape)]),
+    "nrrs_frame_autotune": (C.c_int, [_P, C.POINTER(FrameArgs), C.POINTER(TileShape), C.POINTER(C.c_float), _P]),
+    "nrrs_frame_graph_capture": (C.c_int, [_P, C.POINTER(FrameArgs), _P]),
+    "nrrs_frame_graph_launch": (C.c_int, [_P, _P]),
+    "nrrs_frame_vertices": (C.c_int, [C.c_uint64, C.c_uint32, _P, C.c_uint32, C.POINTER(VerticesOut), _P]),
 }
 
 _lib = None
@@ -289,6 +316,14 @@ class VertexBatch:
             setattr(o, k, getattr(self, k).data_ptr())
         o.stride = self.stride
         return o
+
+
+def frame_vertices(seed, bounce, queue, n, device="cuda", stream=None):
+    """Vertices a frame draws at `bounce` for the path-id queue `queue` (device int32)."""
+    vb = VertexBatch(n, device)
+    o = vb.c_out()
+    check(load().nrrs_frame_vertices(seed, bounce, _ptr(queue), n, C.byref(o), _stream(stream)), "frame_vertices")
+    return vb
 
 
 def synth_vertices(n, seed, n_pixels, device="cuda", stream=None):
@@ -431,6 +466,46 @@ class Context:
         step, sc, sk = C.c_uint32(), C.c_float(), C.c_uint32()
         check(self.lib.nrrs_train_state(self.h, net, C.byref(step), C.byref(sc), C.byref(sk)), "train_state")
         return dict(step=step.value, loss_scale=sc.value, skipped=sk.value)
+
+    # ---- batched uneven wavefront (one persistent launch per frame)
+    def frame_buffers(self, n_primary, n_bounces, capacity=None, trace=False, record_q=False, device="cuda"):
+        """Device buffers of one frame; returns (FrameArgs, dict of tensors)."""
+        import torch
+
+        cap = capacity or n_primary
+        b = dict(queue=torch.zeros(2 * cap, dtype=torch.int32, device=device),
+                 sizes=torch.zeros(n_bounces + 1, dtype=torch.int32, device=device),
+                 status=torch.zeros(1, dtype=torch.int32, device=device))
+        if trace:
+            b["trace"] = torch.zeros((n_bounces + 1, cap), dtype=torch.int32, device=device)
+        if record_q:
+            b["q"] = torch.zeros((n_bounces, cap), dtype=torch.float32, device=device)
+        a = FrameArgs()
+        a.n_primary, a.n_bounces, a.capacity = n_primary, n_bounces, cap
+        a.queue_dev = b["queue"].data_ptr()
+        a.sizes_dev = b["sizes"].data_ptr()
+        a.status_dev = b["status"].data_ptr()
+        a.trace_dev = b["trace"].data_ptr() if trace else None
+        a.q_dev = b["q"].data_ptr() if record_q else None
+        return a, b
+
+    def frame_run(self, args, stream=None):
+        return self.lib.nrrs_frame_run(self.h, C.byref(args), _stream(stream))
+
+    def frame_set_shape(self, waves, m):
+        check(self.lib.nrrs_frame_set_shape(self.h, C.byref(TileShape(waves, m))), "frame_set_shape")
+
+    def frame_autotune(self, args, stream=None):
+        best, ms = TileShape(), C.c_float()
+        check(self.lib.nrrs_frame_autotune(self.h, C.byref(args), C.byref(best), C.byref(ms), _stream(stream)),
+              "frame_autotune")
+        return (best.waves, best.m), ms.value
+
+    def frame_graph_capture(self, args, stream):
+        check(self.lib.nrrs_frame_graph_capture(self.h, C.byref(args), _stream(stream)), "frame_graph_capture")
+
+    def frame_graph_launch(self, stream=None):
+        check(self.lib.nrrs_frame_graph_launch(self.h, _stream(stream)), "frame_graph_launch")
 
     def train_step(self, vb, net_mask, film=None, target_L=None, side=None, loss=None, stream=None):
         v = vb.c_struct()
