@@ -42,6 +42,7 @@ def parse():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU-baseline leg")
     ap.add_argument("--cpu-sample", type=int, default=1 << 16)
     ap.add_argument("--precisions", default="bf16,fp16,fp32")
+    ap.add_argument("--no-frame", action="store_true", help="skip the multi-bounce wavefront and online loop")
     return ap.parse_args()
 
 
@@ -164,6 +165,112 @@ class Path:
         return e0.elapsed_time(e1) / 1e3 / reps
 
 
+def frame_wavefront(torch, na, np, n_frames=8):
+    """Batched uneven wavefront: 10 bounces from 1920x1080 primary vertices, one
+    persistent launch per frame (autotuned tile shape) replayed from a hipGraph,
+    against the same frame driven bounce by bounce from the host (queue size read
+    back every bounce, as the reference integrator does)."""
+    P = 1920 * 1080
+    nb = 10
+    ctx = na.Context(precision=na.BF16, n_pixels=P, max_queue=2 * P, max_train=1 << 18)
+    rng = np.random.default_rng(0)
+    for net in (na.STATNET, na.RRSNET):
+        p = ctx.params_get(net)
+        g, _ = ctx.param_count(net)
+        p[:g] = rng.uniform(-0.3, 0.3, g).astype(np.float32)
+        ctx.params_set(net, p)
+    film = torch.from_numpy(rng.lognormal(-1, 1, (3, P)).astype(np.float32)).cuda()
+    a, b = ctx.frame_buffers(P, nb, capacity=2 * P)
+    a.film_dev = film.data_ptr()
+    a.seed = SEED
+    shape, tune_ms = ctx.frame_autotune(a)
+    seed_dev = torch.tensor([SEED], dtype=torch.int64, device="cuda")
+    a.seed_dev = seed_dev.data_ptr()
+    a.advance_seed = 1
+    stream = torch.cuda.Stream()
+    ctx.frame_graph_capture(a, stream)
+    ctx.frame_graph_launch(stream)
+    stream.synchronize()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record(stream)
+    for _ in range(n_frames):
+        ctx.frame_graph_launch(stream)
+    e1.record(stream)
+    stream.synchronize()
+    ms = e0.elapsed_time(e1) / n_frames
+    sizes = b["sizes"].cpu().numpy().astype(np.int64)
+    status = int(b["status"].item())
+    queries = int(sizes[:nb].sum())
+    # the same frame bounce by bounce: per-bounce entry points, size read back each bounce
+    queue = torch.arange(P, dtype=torch.int32, device="cuda")
+    q = torch.empty(2 * P, device="cuda")
+    surv = torch.empty(2 * P, dtype=torch.int32, device="cuda")
+    ns = torch.zeros(1, dtype=torch.int32, device="cuda")
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    n = P
+    for bnc in range(nb):
+        vb = na.frame_vertices(SEED, bnc, queue, n)
+        args = ctx.decide_args(vb, film, q, surv, ns, q_mode=na.Q_RRSNET_CHAINED, seed=SEED, stream_id=1000 + bnc)
+        na.check(ctx.rrs_infer_decide(args), "rrs_infer_decide")
+        n = int(ns.item())
+        queue = surv[:n].clone()
+    torch.cuda.synchronize()
+    host_ms = 1e3 * (time.perf_counter() - t0)
+    return {"resolution": "1920x1080", "bounces": nb, "tile_shape": {"waves": shape[0], "m": shape[1]},
+            "autotune_best_ms": round(tune_ms, 3), "ms_per_frame_graph": round(ms, 3),
+            "queries_per_frame": queries, "rrs_queries_per_s": round(queries / (ms * 1e-3), 1),
+            "queue_sizes_last_frame": sizes.tolist(), "status": status,
+            "ms_per_frame_host_driven_bounces": round(host_ms, 3)}
+
+
+def online_loop(torch, na, np, n_frames=256, warm_frames=32):
+    """256-frame online loop: every frame one persistent wavefront launch (10 bounces of
+    1920x1080) then one training step on 2^18 sampled vertices: StatNet alone for the
+    first 32 frames, StatNet + RRSNet afterwards. Per-frame latency from HIP events."""
+    P = 1920 * 1080
+    nb = 10
+    B = 1 << 18
+    ctx = na.Context(precision=na.BF16, n_pixels=P, max_queue=2 * P, max_train=B)
+    film = torch.exp(torch.randn(3, P, device="cuda") - 1.0)
+    a, bufs = ctx.frame_buffers(P, nb, capacity=2 * P)
+    a.film_dev = film.data_ptr()
+    seed_dev = torch.tensor([SEED], dtype=torch.int64, device="cuda")
+    a.seed_dev = seed_dev.data_ptr()
+    a.advance_seed = 1
+    stream = torch.cuda.Stream()
+    ctx.frame_graph_capture(a, stream)
+    gen = torch.Generator(device="cuda")
+    gen.manual_seed(SEED)
+    dev = dict(device="cuda", dtype=torch.float32)
+    err = torch.rand(B, generator=gen, **dev) * 2.0
+    sw = torch.ones(B, **dev)
+    qt = torch.rand(B, generator=gen, **dev) * 5.0 + 0.05
+    side = ctx.side(error=err, sample_weight=sw, q_target=qt, error_sum=0.7 * P)
+    loss = torch.zeros(2, **dev)
+    ev = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)) for _ in range(n_frames)]
+    with torch.cuda.stream(stream):
+        t0 = time.perf_counter()
+        for f in range(n_frames):
+            ev[f][0].record(stream)
+            ctx.frame_graph_launch(stream)
+            vb = na.synth_vertices(B, SEED + 7919 * f, P, stream=stream)
+            L = torch.exp(torch.randn(3, B, generator=gen, **dev) * 1.5 - 1.0)
+            L = torch.where(torch.rand(B, generator=gen, **dev) < 0.05, L * 100.0, L)
+            mask = 1 if f < warm_frames else 3
+            na.check(ctx.train_step(vb, mask, film=film, target_L=L, side=side, loss=loss, stream=stream),
+                     "train_step")
+            ev[f][1].record(stream)
+        stream.synchronize()
+        wall = time.perf_counter() - t0
+    lat = np.array([e[0].elapsed_time(e[1]) for e in ev])
+    return {"frames": n_frames, "frames_per_s": round(n_frames / wall, 2),
+            "latency_ms_p50": round(float(np.percentile(lat, 50)), 3),
+            "latency_ms_p99": round(float(np.percentile(lat, 99)), 3),
+            "train_batch": B, "statnet_only_frames": warm_frames,
+            "final_loss": [round(x, 5) for x in loss.cpu().tolist()], "status": int(bufs["status"].item())}
+
+
 def main():
     args = parse()
     if args.gpus != 1:
@@ -243,6 +350,11 @@ def main():
             "rrs_queries": round(bf["rrs_queries_per_s"] / results["fp32"]["rrs_queries_per_s"], 3),
             "train_samples": round(bf["train_samples_per_s"] / results["fp32"]["train_samples_per_s"], 3),
             "step": round(bf["step_queries_per_s"] / results["fp32"]["step_queries_per_s"], 3)}
+    if not args.no_frame:
+        out["wavefront_frame"] = frame_wavefront(torch, na, np)
+        print(f"[bench] wavefront: {json.dumps(out['wavefront_frame'])}", file=sys.stderr, flush=True)
+        out["online_loop"] = online_loop(torch, na, np)
+        print(f"[bench] online: {json.dumps(out['online_loop'])}", file=sys.stderr, flush=True)
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(args.cpu_sample)
     print(json.dumps(out))

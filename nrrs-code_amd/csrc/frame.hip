@@ -25,19 +25,22 @@ using namespace mfma;
 constexpr uint32_t kSpinLimit = 1u << 22;
 constexpr uint64_t kBounceSeedStep = 0x9E3779B97F4A7C15ull;
 
-// Bounded sense-by-generation barrier over all workgroups of the launch.
+// Bounded sense-by-generation barrier over all workgroups of the launch. One
+// agent-scope release fence before arriving and one acquire fence after leaving;
+// the spin itself uses relaxed loads (an acquire load per spin iteration would
+// invalidate the L2 of every spinning workgroup's XCD on each poll).
 __device__ __forceinline__ bool grid_sync(uint32_t* bar, uint32_t nblocks, uint32_t* status) {
     __syncthreads();
     if (threadIdx.x == 0) {
-        __threadfence();
-        const uint32_t gen = __hip_atomic_load(bar + 1, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT);
-        const uint32_t arrived = __hip_atomic_fetch_add(bar, 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+        const uint32_t gen = __hip_atomic_load(bar + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        const uint32_t arrived = __hip_atomic_fetch_add(bar, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         if (arrived == nblocks - 1) {
             __hip_atomic_store(bar, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
             __hip_atomic_store(bar + 1, gen + 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
         } else {
             uint32_t it = 0;
-            while (__hip_atomic_load(bar + 1, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) == gen) {
+            while (__hip_atomic_load(bar + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == gen) {
                 if (++it > kSpinLimit) {
                     atomicOr(status, 1u);
                     break;
@@ -45,7 +48,7 @@ __device__ __forceinline__ bool grid_sync(uint32_t* bar, uint32_t nblocks, uint3
                 __builtin_amdgcn_s_sleep(1);
             }
         }
-        __threadfence();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
     }
     __syncthreads();
     return (__hip_atomic_load(status, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) & 1u) == 0u;
@@ -120,7 +123,7 @@ __global__ __launch_bounds__(64 * WAVES) void k_frame(FrameParams f) {
                     if (cb >= n_cb) break;
                     const uint32_t i = cb * 32 + col;
                     const bool live = i < n;
-                    typename P::vec8 B[5];
+                    typename P::vec8 B[5], Br[2];
                     VertexRec v;
                     float ref = 0.0f;
                     if (live) {
@@ -137,10 +140,13 @@ __global__ __launch_bounds__(64 * WAVES) void k_frame(FrameParams f) {
                         v.rough = sv.rough;
                         v.pixel = qin[i];
                         ref = film_mean(f.film, f.n_pixels, v.pixel);
+                        // both nets' grid gathers depend on the position only: issue them together
+                        encode_grid_half<P>(v, f.grid, f.table_rrs, h, Br);
                         encode_half_rec<P>(v, f.grid, NRRS_STATNET, f.table_stat, h, 0.0f, nullptr, B);
                     } else {
 #pragma unroll
                         for (int kb = 0; kb < 5; ++kb) B[kb] = typename P::vec8{};
+                        Br[0] = Br[1] = typename P::vec8{};
                     }
                     const f32x16 Ys = mlp_forward<P, 5>(sWs, B, lane);
                     // statistics of the lane's vertex: outputs 0..3 in half 0, 4..5 in half 1
@@ -158,7 +164,11 @@ __global__ __launch_bounds__(64 * WAVES) void k_frame(FrameParams f) {
                             st[0] = o[0]; st[1] = o[1]; st[2] = o[2]; st[3] = o[3]; st[4] = a[0]; st[5] = a[1];
                         }
                     }
-                    if (live) encode_half_rec<P>(v, f.grid, NRRS_RRSNET, f.table_rrs, h, ref, st, B);
+                    if (live) {
+                        B[0] = Br[0];
+                        B[1] = Br[1];
+                        encode_tail_half<P>(v, NRRS_RRSNET, h, ref, st, B);
+                    }
                     const f32x16 Yr = mlp_forward<P, 4>(sWr, B, lane);
                     if (live && h == 0) {
                         const float q = rrs_window_clamp(f.d, activation_sigma(Yr[0]));

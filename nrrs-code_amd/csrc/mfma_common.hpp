@@ -79,11 +79,11 @@ __device__ __forceinline__ int packed_col(int layer, int kb, int h, int j) {
 }
 
 // The 40 channels of a vertex that half h of the wavefront feeds to layer 0:
-// channel kb*16 + 8h + j goes to element j of K block kb.
+// channel kb*16 + 8h + j goes to element j of K block kb. Chunks 0-1 are the grid
+// (levels 4h..4h+3 and 8+4h..8+4h+3), chunks 2-4 the SH and per-vertex channels.
 template <typename P>
-__device__ __forceinline__ void encode_half_rec(const VertexRec& v, const GridParams& grid, int net,
-                                                const uint32_t* __restrict__ table, int h, float ref,
-                                                const float* st, typename P::vec8* B) {
+__device__ __forceinline__ void encode_grid_half(const VertexRec& v, const GridParams& grid,
+                                                 const uint32_t* __restrict__ table, int h, typename P::vec8* B) {
     float e[8];
 #pragma unroll
     for (int kb = 0; kb < 2; ++kb) {
@@ -97,7 +97,12 @@ __device__ __forceinline__ void encode_half_rec(const VertexRec& v, const GridPa
         }
         B[kb] = P::pack(e);
     }
-    float t[kEnc];
+}
+
+template <typename P>
+__device__ __forceinline__ void encode_tail_half(const VertexRec& v, int net, int h, float ref, const float* st,
+                                                 typename P::vec8* B) {
+    float t[kEnc], e[8];
     encode_tail_rec(v, net, ref, st, t);
 #pragma unroll
     for (int kb = 2; kb < 5; ++kb) {
@@ -105,6 +110,14 @@ __device__ __forceinline__ void encode_half_rec(const VertexRec& v, const GridPa
         for (int j = 0; j < 8; ++j) e[j] = h ? t[kb * 16 + 8 + j] : t[kb * 16 + j];
         B[kb] = P::pack(e);
     }
+}
+
+template <typename P>
+__device__ __forceinline__ void encode_half_rec(const VertexRec& v, const GridParams& grid, int net,
+                                                const uint32_t* __restrict__ table, int h, float ref,
+                                                const float* st, typename P::vec8* B) {
+    encode_grid_half<P>(v, grid, table, h, B);
+    encode_tail_half<P>(v, net, h, ref, st, B);
 }
 
 template <typename P>
