@@ -31,6 +31,9 @@ N_PIXELS = 1 << 20
 SEED = 0x4E525253 ^ 2
 HBM_PEAK_GBS = 8000.0
 BF16_PEAK_TFLOPS = 2500.0
+# HBM bytes per launch of the fused StatNet training kernel at 2^20 samples, from
+# rocprofv3 --pmc FETCH_SIZE and WRITE_SIZE (profiles/train_kernel_bf16_2p20_pmc.md), or None.
+TRAIN_TRAFFIC_BYTES = 8650115072
 MACS = {"statnet": 80 * 64 + 3 * 64 * 64 + 64 * 6, "rrsnet": 80 * 64 + 2 * 64 * 64 + 64 * 1}
 
 
@@ -40,9 +43,12 @@ def parse():
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU-baseline leg")
-    ap.add_argument("--cpu-sample", type=int, default=1 << 16)
+    ap.add_argument("--cpu-sample", type=int, default=1 << 18)
     ap.add_argument("--precisions", default="bf16,fp16,fp32")
     ap.add_argument("--no-frame", action="store_true", help="skip the multi-bounce wavefront and online loop")
+    ap.add_argument("--pmc-probe", action="store_true",
+                    help="only run a 256 MiB device copy and 3 BF16 StatNet training kernels at 2^20 "
+                         "(for rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE runs)")
     return ap.parse_args()
 
 
@@ -148,6 +154,27 @@ class Path:
                 "rrs_queries_per_s": self.n / (inf * 1e-3), "train_samples_per_s": self.n / (trn * 1e-3),
                 "step_queries_per_s": self.n * steps / wall, "loss": self.loss.cpu().tolist(),
                 "survivors": int(self.n_surv.item())}
+
+    def backward_statnet(self):
+        """One fused StatNet forward/loss/backward (k_train) at the context's precision."""
+        self.na.check(self.ctx.backward(self.na.STATNET, self.vb, self.na.LOSS_LL2, target_L=self.target_L,
+                                        loss_scale=4096.0), "backward")
+
+    def time_train_kernel(self, reps):
+        """Average duration of nrrs_backward(StatNet, NRRS_LL2): the fused training kernel
+        plus a one-element fill and the per-workgroup dW reduction, HIP events on the stream
+        they are launched on."""
+        torch = self.torch
+        stream = torch.cuda.current_stream()
+        self.backward_statnet()
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record(stream)
+        for _ in range(reps):
+            self.backward_statnet()
+        e1.record(stream)
+        torch.cuda.synchronize()
+        self.ctx.grads_zero(self.na.STATNET)
+        return e0.elapsed_time(e1) / 1e3 / reps
 
     def time_statnet_kernel(self, reps):
         """Average duration of the fused StatNet encode+MLP inference kernel (one
@@ -282,14 +309,24 @@ def main():
     import nrrs_amd as na
 
     torch.cuda.set_device(0)
+    if args.pmc_probe:
+        src = torch.ones(64 << 20, device="cuda")  # 256 MiB
+        dst = torch.empty_like(src)
+        dst.copy_(src)
+        path = Path(na.BF16, torch, na, np)
+        for _ in range(3):
+            path.backward_statnet()
+        torch.cuda.synchronize()
+        print(json.dumps({"pmc_probe": "done", "copy_bytes": src.numel() * 4}))
+        return
     precs = {"bf16": na.BF16, "fp16": na.FP16, "fp32": na.FP32}
     results, q_by = {}, {}
     for name in args.precisions.split(","):
         path = Path(precs[name], torch, na, np)
         r = path.run(args.steps if name == "bf16" else max(2, args.steps // 2), args.warmup)
         if name == "bf16":
-            kern_s = path.time_statnet_kernel(max(5, args.steps))
-            r["statnet_kernel_us"] = kern_s * 1e6
+            r["statnet_kernel_us"] = path.time_statnet_kernel(max(5, args.steps)) * 1e6
+            r["train_kernel_us"] = path.time_train_kernel(max(3, args.steps // 2)) * 1e6
         # q of the same (pre-training) parameters for the matched-error report
         fresh = Path(precs[name], torch, na, np)
         fresh.infer(0)
@@ -303,18 +340,22 @@ def main():
 
     bf = results["bf16"]
     n = N_VERTICES
-    # algorithmic traffic and work of the fused StatNet inference kernel per query:
-    # input record 64 B + 16 levels x 8 corners x 2 BF16 features + 6 FP32 outputs
-    bytes_q = 64 + 16 * 8 * 2 * 2 + 24
-    flops_q = 2 * MACS["statnet"]
-    kern_s = bf["statnet_kernel_us"] * 1e-6
-    gbs = bytes_q * n / kern_s / 1e9
-    tflops = flops_q * n / kern_s / 1e12
+    # Dominant kernel of the BF16 step: the fused StatNet training kernel (k_train, NRRS_LL2).
+    # Algorithmic bytes per sample: input record 64 B + target 12 B, 16 levels x 8 corners x
+    # 4 B of BF16 feature pairs gathered, and the same 512 B of packed-FP16 gradient atomics.
+    bytes_s = 64 + 12 + 16 * 8 * 4 + 16 * 8 * 4
+    flops_s = 3 * 2 * MACS["statnet"]  # forward, dX, dW
+    kern_s = bf["train_kernel_us"] * 1e-6
+    gbs = bytes_s * n / kern_s / 1e9
+    tflops = flops_s * n / kern_s / 1e12
     roofline = {"bound": "hbm", "achieved": round(gbs, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                "frac": round(gbs / HBM_PEAK_GBS, 4), "traffic": None,
-                "kernel": "k_net_mfma<BF16,5,6,EPI_RAW> (StatNet encode + MLP)",
-                "kernel_us": round(bf["statnet_kernel_us"], 2),
-                "mfma_tflops": round(tflops, 2), "mfma_frac": round(tflops / BF16_PEAK_TFLOPS, 4)}
+                "frac": round(gbs / HBM_PEAK_GBS, 4), "traffic": TRAIN_TRAFFIC_BYTES,
+                "kernel": "k_train<TBF16,5,6,NRRS_LOSS_LL2> (StatNet forward + loss + backward, 2^20 samples)",
+                "kernel_us": round(bf["train_kernel_us"], 1),
+                "algorithmic_bytes_per_launch": bytes_s * n,
+                "mfma_tflops": round(tflops, 2), "mfma_frac": round(tflops / BF16_PEAK_TFLOPS, 5),
+                "grid_atomics_per_launch": 16 * 8 * n,
+                "grid_atomic_rate_g_per_s": round(16 * 8 * n / kern_s / 1e9, 2)}
     matched = {}
     if "fp32" in q_by:
         for name in q_by:
