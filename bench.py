@@ -33,7 +33,7 @@ HBM_PEAK_GBS = 8000.0
 BF16_PEAK_TFLOPS = 2500.0
 # HBM bytes per launch of the fused StatNet training kernel at 2^20 samples, from
 # rocprofv3 --pmc FETCH_SIZE and WRITE_SIZE (profiles/train_kernel_bf16_2p20_pmc.md), or None.
-TRAIN_TRAFFIC_BYTES = 8650115072
+TRAIN_TRAFFIC_BYTES = 8651328117
 MACS = {"statnet": 80 * 64 + 3 * 64 * 64 + 64 * 6, "rrsnet": 80 * 64 + 2 * 64 * 64 + 64 * 1}
 
 
