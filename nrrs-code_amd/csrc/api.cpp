@@ -199,6 +199,18 @@ nrrs_status ensure_train(nrrs_ctx* c, int net) {
     NRRS_TRY(dmalloc(reinterpret_cast<uint8_t**>(&ns.w_fwd), (size_t)train_tiles_fwd(ns.n_layers) * 64 * ob));
     NRRS_TRY(dmalloc(reinterpret_cast<uint8_t**>(&ns.w_bwd), (size_t)train_tiles_bwd(ns.n_layers) * 64 * ob));
     NRRS_TRY(dmalloc(&ns.dw_partial, (size_t)c->max_train_blocks * ns.n_mlp));
+    {
+        // dense levels come first: their entries end where the first hashed level starts
+        uint32_t nd = 0;
+        for (uint32_t l = 0; l < c->grid.n_levels; ++l)
+            if ((c->grid.dense_mask >> l) & 1u) nd = (l + 1 < c->grid.n_levels) ? c->grid.offset[l + 1] : c->grid.n_entries;
+        ns.n_dense = nd;
+        const size_t bytes = (size_t)kDenseGradReplicas * nd * (c->cfg.precision == NRRS_FP32 ? 8 : 4);
+        if (bytes) {
+            NRRS_TRY(dmalloc(reinterpret_cast<uint8_t**>(&ns.dense_rep), bytes));
+            NRRS_HIP(hipMemset(ns.dense_rep, 0, bytes));
+        }
+    }
     NRRS_TRY(dmalloc(&ns.loss_scale, 1));
     NRRS_TRY(dmalloc(&ns.state, 4));
     NRRS_HIP(hipMemset(ns.grad, 0, np * 4));
@@ -252,6 +264,11 @@ nrrs_status run_backward(nrrs_ctx* c, int net, uint32_t n, const nrrs_vertices& 
     t.out_stride = out_stride;
     t.grad_grid = ns.grad;
     t.grad_grid_h2 = reinterpret_cast<_Float16 __attribute__((ext_vector_type(2)))*>(ns.grad_lp);
+    t.n_dense = ns.n_dense;
+    if (cfg.precision == NRRS_FP32)
+        t.dense_f32 = static_cast<float*>(ns.dense_rep);
+    else
+        t.dense_h2 = static_cast<_Float16 __attribute__((ext_vector_type(2)))*>(ns.dense_rep);
     t.dw_partial = ns.dw_partial;
     t.n_mlp = (uint32_t)ns.n_mlp;
     t.loss_out = loss_out;
@@ -262,9 +279,19 @@ nrrs_status run_backward(nrrs_ctx* c, int net, uint32_t n, const nrrs_vertices& 
     return hip_status(launch_reduce_partials(ns.dw_partial, blocks, (uint32_t)ns.n_mlp, ns.grad + ns.n_grid, s));
 }
 
+// Folds the dense-level replicas into the table gradient (and clears them).
+nrrs_status fold_dense(nrrs_ctx* c, int net, hipStream_t s) {
+    NetState& ns = c->net[net];
+    if (!ns.dense_rep) return NRRS_OK;
+    const bool f32 = c->cfg.precision == NRRS_FP32;
+    return hip_status(launch_reduce_dense(f32 ? static_cast<float*>(ns.dense_rep) : nullptr, f32 ? nullptr : ns.dense_rep,
+                                          ns.n_dense, ns.grad, f32 ? nullptr : ns.grad_lp, s));
+}
+
 nrrs_status run_adam(nrrs_ctx* c, int net, hipStream_t s) {
     NetState& ns = c->net[net];
     const nrrs_config& cfg = c->cfg;
+    NRRS_TRY(fold_dense(c, net, s));
     AdamArgs a{};
     a.params = ns.params;
     a.grad = ns.grad;
@@ -362,6 +389,7 @@ void free_ctx(nrrs_ctx* c) {
         hipFree(ns.mlp_lp);
         hipFree(ns.grad);
         hipFree(ns.grad_lp);
+        hipFree(ns.dense_rep);
         hipFree(ns.adam_m);
         hipFree(ns.adam_v);
         hipFree(ns.w_fwd);
@@ -695,6 +723,10 @@ nrrs_status nrrs_grads_zero(nrrs_ctx* ctx, int32_t net, void* stream) {
     NetState& ns = ctx->net[net];
     if (ns.grad_lp)
         NRRS_HIP(launch_fill_f32(reinterpret_cast<float*>(ns.grad_lp), 0.0f, ns.n_grid / 2, (hipStream_t)stream));
+    if (ns.dense_rep)
+        NRRS_HIP(launch_fill_f32(static_cast<float*>(ns.dense_rep), 0.0f,
+                                 (uint64_t)kDenseGradReplicas * ns.n_dense * (ctx->cfg.precision == NRRS_FP32 ? 2 : 1),
+                                 (hipStream_t)stream));
     return hip_status(launch_fill_f32(ns.grad, 0.0f, ns.n_grid + ns.n_mlp, (hipStream_t)stream));
 }
 
@@ -702,6 +734,7 @@ nrrs_status nrrs_grads_get(nrrs_ctx* ctx, int32_t net, float* host_dst) {
     if (!ctx || (net != 0 && net != 1) || !host_dst) return NRRS_ERR_ARG;
     NRRS_TRY(ensure_train(ctx, net));
     NetState& ns = ctx->net[net];
+    NRRS_TRY(fold_dense(ctx, net, nullptr));
     NRRS_HIP(hipDeviceSynchronize());
     NRRS_HIP(hipMemcpy(host_dst, ns.grad, (ns.n_grid + ns.n_mlp) * 4, hipMemcpyDeviceToHost));
     if (ns.grad_lp) {  // FP16 grid gradient of FP16/BF16 contexts

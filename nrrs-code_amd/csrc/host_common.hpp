@@ -71,6 +71,8 @@ struct NetState {
     bool train_ready = false;
     float* grad = nullptr;           // [grid | mlp]
     uint16_t* grad_lp = nullptr;     // FP16 grid gradient of FP16/BF16 contexts [n_grid]
+    void* dense_rep = nullptr;       // dense-level gradient replicas (FP32 pairs or FP16 pairs)
+    uint32_t n_dense = 0;            // table entries of the dense levels
     float* adam_m = nullptr;
     float* adam_v = nullptr;
     void* w_fwd = nullptr;           // packed training tiles (forward / transposed)

@@ -91,6 +91,7 @@ struct TF32 {
 
 constexpr int kWaves = 4;
 constexpr int kThreads = 64 * kWaves;
+constexpr uint32_t kDenseReplicas = kDenseGradReplicas;
 constexpr int kVert = 32 * kWaves;     // vertices per workgroup iteration
 constexpr int kStageLd = kVert + 8;    // LDS row stride (elements), breaks bank aliasing
 constexpr int kStageRowsD = 64;        // delta rows
@@ -424,33 +425,55 @@ __global__ __launch_bounds__(kThreads) void k_train(TrainArgs t) {
                 f32x16 G = f32x16{};
 #pragma unroll
                 for (int kb = 0; kb < 4; ++kb) G = P::mma16(load_op<P>(Wb, kb, lane), Dl[kb], G);
-                if (live) {
-                    float x[3];
-                    load_xn(t.a, i, x);
+                // Scatter to the table. Lanes 2k and 2k+1 take vertex k of this wave's 32
+                // (whose level gradients sit in lanes k and k+32, fetched with ds_bpermute), so
+                // that the two lanes of a pair update neighbouring table words in the same
+                // atomic instruction, which the memory pipeline merges into one request:
+                //  - FP16 table: lane parity = x bit of the corner; the x-adjacent corners are
+                //    consecutive entries in a dense level and the aligned pair {h & ~1, h | 1} of
+                //    a hashed level when the cell's x is even;
+                //  - FP32 table: lane parity = feature, both features of a corner are one 8-byte
+                //    entry.
+                const int pv = lane >> 1, pb = lane & 1;
+                const uint32_t iv = it * kVert + wave * 32 + pv;
+                const bool live_v = iv < t.n;
+                float x[3] = {0.0f, 0.0f, 0.0f};
+                if (live_v) load_xn(t.a, iv, x);
 #pragma unroll
-                    for (int q4 = 0; q4 < 4; ++q4)
+                for (int l = 0; l < kMaxLevels; ++l) {
+                    if (l >= (int)t.a.grid.n_levels) break;
+                    // features of level l: registers 4(l/4) + 2(l%2) + {0,1} of lane pv + 32((l/2)%2)
+                    const int k = 4 * (l >> 2) + 2 * (l & 1);
+                    const int src = 4 * (pv + 32 * ((l >> 1) & 1));
+                    const float g0 = __int_as_float(__builtin_amdgcn_ds_bpermute(src, __float_as_int(G[k])));
+                    const float g1 = __int_as_float(__builtin_amdgcn_ds_bpermute(src, __float_as_int(G[k + 1])));
+                    if (!live_v) continue;
+                    LevelCell lc = level_cell(t.a.grid, l, x);
+                    // Dense (coarse) levels are small and hit by every vertex: their updates go
+                    // to one of kDenseReplicas private copies (chosen by workgroup), summed before
+                    // the optimiser step, which spreads the contention on each entry.
+                    const bool dense = (t.a.grid.dense_mask >> l) & 1u;
+                    const size_t rep = dense ? (size_t)(blockIdx.x % kDenseReplicas) * t.n_dense : 0;
+                    if (t.grad_grid_h2) {
 #pragma unroll
-                        for (int pr = 0; pr < 2; ++pr) {
-                            // registers 4*q4 + 2*pr + {0,1} hold features 0/1 of level 4*q4 + 2h + pr
-                            const int l = 4 * q4 + 2 * h + pr;
-                            if (l < (int)t.a.grid.n_levels) {
-                                const float g0 = G[4 * q4 + 2 * pr], g1 = G[4 * q4 + 2 * pr + 1];
-                                LevelCell lc = level_cell(t.a.grid, l, x);
-#pragma unroll
-                                for (int cn = 0; cn < 8; ++cn) {
-                                    const uint32_t ent = t.a.grid.offset[l] + corner_index(t.a.grid, l, lc, cn);
-                                    const float w = corner_weight(lc, cn);
-                                    if (t.grad_grid_h2) {
-                                        // both features in one packed FP16 atomic
-                                        h2v gv = {(_Float16)(w * g0), (_Float16)(w * g1)};
-                                        __builtin_amdgcn_global_atomic_fadd_v2f16(t.grad_grid_h2 + ent, gv);
-                                    } else {
-                                        atomicAdd(t.grad_grid + 2 * (size_t)ent, w * g0);
-                                        atomicAdd(t.grad_grid + 2 * (size_t)ent + 1, w * g1);
-                                    }
-                                }
-                            }
+                        for (int yz = 0; yz < 4; ++yz) {
+                            const int cn = pb | (yz << 1);
+                            const uint32_t ent = t.a.grid.offset[l] + corner_index(t.a.grid, l, lc, cn);
+                            const float w = corner_weight(lc, cn);
+                            h2v gv = {(_Float16)(w * g0), (_Float16)(w * g1)};  // one packed FP16 atomic
+                            h2v* dst = dense ? t.dense_h2 + rep + ent : t.grad_grid_h2 + ent;
+                            __builtin_amdgcn_global_atomic_fadd_v2f16(dst, gv);
                         }
+                    } else {
+                        const float g = pb ? g1 : g0;
+#pragma unroll
+                        for (int cn = 0; cn < 8; ++cn) {
+                            const uint32_t ent = t.a.grid.offset[l] + corner_index(t.a.grid, l, lc, cn);
+                            const float w = corner_weight(lc, cn);
+                            float* dst = dense ? t.dense_f32 + 2 * (rep + ent) : t.grad_grid + 2 * (size_t)ent;
+                            atomicAdd(dst + pb, w * g);
+                        }
+                    }
                 }
             }
         }
@@ -579,6 +602,30 @@ hipError_t dispatch_train(const TrainArgs& t, int nl, int n_out, int loss, uint3
     return hipErrorInvalidValue;
 }
 
+// Sum of the dense-level gradient replicas into the table gradient; the replicas are cleared.
+__global__ void k_reduce_dense(float* __restrict__ rep_f32, _Float16* __restrict__ rep_h, uint32_t n_dense,
+                               float* __restrict__ grad, _Float16* __restrict__ grad_h) {
+    // element k of [n_dense][2]
+    for (uint32_t k = blockIdx.x * blockDim.x + threadIdx.x; k < 2 * n_dense; k += gridDim.x * blockDim.x) {
+        float s = 0.0f;
+        for (uint32_t r = 0; r < kDenseReplicas; ++r) {
+            const size_t o = (size_t)r * 2 * n_dense + k;
+            if (rep_h) {
+                s += (float)rep_h[o];
+                rep_h[o] = (_Float16)0.0f;
+            } else {
+                s += rep_f32[o];
+                rep_f32[o] = 0.0f;
+            }
+        }
+        if (grad_h) {
+            grad_h[k] = (_Float16)((float)grad_h[k] + s);
+        } else {
+            grad[k] += s;
+        }
+    }
+}
+
 __global__ void k_reduce_partials(const float* __restrict__ part, uint32_t n_blocks, uint32_t n_mlp,
                                   float* __restrict__ grad) {
     const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
@@ -614,6 +661,14 @@ hipError_t launch_train(int precision, const TrainArgs& t, int nl, int n_out, in
     if (precision == NRRS_FP32) return dispatch_train<TF32>(t, nl, n_out, loss, blocks, s);
     if (precision == NRRS_BF16) return dispatch_train<TBF16>(t, nl, n_out, loss, blocks, s);
     return dispatch_train<TF16>(t, nl, n_out, loss, blocks, s);
+}
+
+hipError_t launch_reduce_dense(float* rep_f32, void* rep_h, uint32_t n_dense, float* grad, void* grad_h,
+                               hipStream_t s) {
+    if (!n_dense) return hipSuccess;
+    hipLaunchKernelGGL(k_reduce_dense, dim3(512), dim3(256), 0, s, rep_f32, static_cast<_Float16*>(rep_h), n_dense,
+                       grad, static_cast<_Float16*>(grad_h));
+    return hipGetLastError();
 }
 
 hipError_t launch_reduce_partials(const float* part, uint32_t n_blocks, uint32_t n_mlp, float* grad, hipStream_t s) {

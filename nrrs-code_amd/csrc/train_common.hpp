@@ -5,6 +5,9 @@
 
 namespace nrrs {
 
+// Private copies of the dense-level table gradient (see the scatter in k_train).
+constexpr uint32_t kDenseGradReplicas = 16;
+
 struct TrainArgs {
     EncodeArgs a;                  // vertices, film, RRSNet statistics input, grid
     int net;
@@ -33,6 +36,9 @@ struct TrainArgs {
     uint32_t out_stride;
     float* grad_grid;              // [entries][2], atomically accumulated (FP32 contexts)
     _Float16 __attribute__((ext_vector_type(2)))* grad_grid_h2;  // [entries] FP16 pairs (FP16/BF16)
+    float* dense_f32;              // [replicas][n_dense][2] (FP32 contexts)
+    _Float16 __attribute__((ext_vector_type(2)))* dense_h2;  // [replicas][n_dense] (FP16/BF16)
+    uint32_t n_dense;              // entries of the dense levels (they come first in the table)
     float* dw_partial;             // [gridDim.x][n_mlp]
     uint32_t n_mlp;
     float* loss_out;               // device scalar (atomic add), or null
@@ -44,6 +50,8 @@ size_t train_op_bytes(int precision);
 hipError_t launch_pack_train(int precision, const float* W, int nl, int n_out, void* fwd, void* bwd, hipStream_t s);
 hipError_t launch_train(int precision, const TrainArgs& t, int nl, int n_out, int loss, uint32_t blocks,
                         hipStream_t s);
+hipError_t launch_reduce_dense(float* rep_f32, void* rep_h, uint32_t n_dense, float* grad, void* grad_h,
+                               hipStream_t s);
 hipError_t launch_reduce_partials(const float* part, uint32_t n_blocks, uint32_t n_mlp, float* grad, hipStream_t s);
 
 struct AdamArgs {
