@@ -272,7 +272,8 @@ nrrs_status run_backward(nrrs_ctx* c, int net, uint32_t n, const nrrs_vertices& 
     t.dw_partial = ns.dw_partial;
     t.n_mlp = (uint32_t)ns.n_mlp;
     t.loss_out = loss_out;
-    uint32_t iters = (n + 127) / 128;
+    const uint32_t vpi = train_vertices_per_iter(cfg.precision);
+    uint32_t iters = (n + vpi - 1) / vpi;
     uint32_t blocks = iters < c->max_train_blocks ? iters : c->max_train_blocks;
     if (!blocks) return NRRS_OK;
     NRRS_HIP(launch_train(cfg.precision, t, ns.n_layers, ns.n_out, kind, blocks, s));

@@ -31,8 +31,12 @@ typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
 typedef _Float16 h2v __attribute__((ext_vector_type(2)));
 
 // Operand policies: 8 values per lane per 16-deep K chunk.
+// kWaves: waves per workgroup. FP16/BF16 use 8 (one workgroup of 8 waves per CU, two
+// waves per SIMD, 256 VGPRs per lane); FP32 uses 4, since its FP32 staging of 8 waves
+// would not fit in LDS.
 struct TBF16 {
     static constexpr bool kF32Table = false;
+    static constexpr int kWaves = 8;
     typedef bf16x8 op;
     typedef __bf16 elem;
     __device__ static op pack(const float* x) {
@@ -51,6 +55,7 @@ struct TBF16 {
 };
 struct TF16 {
     static constexpr bool kF32Table = false;
+    static constexpr int kWaves = 8;
     typedef f16x8 op;
     typedef _Float16 elem;
     __device__ static op pack(const float* x) {
@@ -71,6 +76,7 @@ struct TF16 {
 };
 struct TF32 {
     static constexpr bool kF32Table = true;
+    static constexpr int kWaves = 4;
     typedef f32x8 op;
     typedef float elem;
     __device__ static op pack(const float* x) {
@@ -89,35 +95,38 @@ struct TF32 {
     }
 };
 
-constexpr int kWaves = 4;
-constexpr int kThreads = 64 * kWaves;
 constexpr uint32_t kDenseReplicas = kDenseGradReplicas;
-constexpr int kVert = 32 * kWaves;     // vertices per workgroup iteration
-constexpr int kStageLd = kVert + 8;    // LDS row stride (elements), breaks bank aliasing
 constexpr int kStageRowsD = 64;        // delta rows
 constexpr int kStageRowsH = 96;        // input rows (80 padded to 3 x 32)
 
 template <typename P>
+struct Geo {
+    static constexpr int kWaves = P::kWaves;
+    static constexpr int kThreads = 64 * kWaves;
+    static constexpr int kVert = 32 * kWaves;   // vertices per workgroup iteration
+    static constexpr int kStageLd = kVert + 8;  // LDS row stride (elements), breaks bank aliasing
+};
+
+template <typename P>
 struct Stage {
-    typename P::elem d[kStageRowsD * kStageLd];
-    typename P::elem h[kStageRowsH * kStageLd];
+    typename P::elem d[kStageRowsD * Geo<P>::kStageLd];
+    typename P::elem h[kStageRowsH * Geo<P>::kStageLd];
 };
 
 // Number of 32x32 dW tiles of layer l (out rows x in cols).
 __host__ __device__ constexpr int dw_tiles(int l, int nl) {
     return (l == 0 ? 2 * 3 : (l == nl - 1 ? 1 * 2 : 2 * 2));
 }
-// Accumulator slots a wave needs: ceil(tiles / waves) summed over layers.
-__host__ __device__ constexpr int dw_slots(int nl) {
+// dW tiles are numbered across layers (layer 0 first); wave w of W owns the tiles g
+// with g % W == w and accumulates tile g in register slot g / W.
+__host__ __device__ constexpr int dw_tile_base(int l, int nl) {
     int s = 0;
-    for (int l = 0; l < nl; ++l) s += (dw_tiles(l, nl) + kWaves - 1) / kWaves;
+    for (int k = 0; k < l; ++k) s += dw_tiles(k, nl);
     return s;
 }
-__host__ __device__ constexpr int dw_slot_base(int l, int nl) {
-    int s = 0;
-    for (int k = 0; k < l; ++k) s += (dw_tiles(k, nl) + kWaves - 1) / kWaves;
-    return s;
-}
+__host__ __device__ constexpr int dw_slots(int nl, int w) { return (dw_tile_base(nl, nl) + w - 1) / w; }
+// First tile at or after `base` owned by wave w of W.
+__device__ __forceinline__ int dw_first(int base, int w, int W) { return base + ((w - base) % W + W) % W; }
 
 template <typename P>
 __device__ __forceinline__ typename P::op load_op(const typename P::op* __restrict__ W, int tile, int lane) {
@@ -127,7 +136,7 @@ __device__ __forceinline__ typename P::op load_op(const typename P::op* __restri
 // LDS operand: 8 consecutive vertices of row `row`, starting at vertex v0.
 template <typename P>
 __device__ __forceinline__ typename P::op stage_op(const typename P::elem* s, int row, int v0) {
-    const typename P::elem* p = s + row * kStageLd + v0;
+    const typename P::elem* p = s + row * Geo<P>::kStageLd + v0;
     typename P::op v;
 #pragma unroll
     for (int j = 0; j < 8; ++j) v[j] = p[j];
@@ -189,61 +198,77 @@ __device__ __forceinline__ uint32_t recompute(const typename P::op* __restrict__
 }
 
 template <typename P, int NL, int NOUT, int LOSS>
-__global__ __launch_bounds__(kThreads) void k_train(TrainArgs t) {
+__global__ __launch_bounds__(64 * P::kWaves) void k_train(TrainArgs t) {
     typedef typename P::op op;
+    constexpr int kWaves = Geo<P>::kWaves, kThreads = Geo<P>::kThreads, kVert = Geo<P>::kVert;
+    constexpr int kStageLd = Geo<P>::kStageLd;
     extern __shared__ __align__(16) unsigned char smem_raw[];
     Stage<P>& st = *reinterpret_cast<Stage<P>*>(smem_raw);
-    __shared__ float loss_part[kWaves];
+    __shared__ float loss_part[kWaves];  // loss per wave, accumulated with LDS atomics
 
     const int lane = threadIdx.x & 63, h = lane >> 5, col = lane & 31, wave = threadIdx.x >> 6;
-    const op* __restrict__ Wf = reinterpret_cast<const op*>(t.w_fwd);
-    const op* __restrict__ Wb = reinterpret_cast<const op*>(t.w_bwd);
+    const op* Wf0 = reinterpret_cast<const op*>(t.w_fwd);
+    const op* Wb0 = reinterpret_cast<const op*>(t.w_bwd);
     const float S = *t.loss_scale;
-    constexpr int kSlots = dw_slots(NL);
+    constexpr int kSlots = dw_slots(NL, kWaves);
     f32x16 acc[kSlots];
 #pragma unroll
     for (int s = 0; s < kSlots; ++s) acc[s] = f32x16{};
-    float loss_acc = 0.0f;
+    if (lane == 0) loss_part[wave] = 0.0f;  // by the wave that accumulates into it
 
     // zero the padded input rows once (layer 0 has 80 of 96 rows)
     for (int k = threadIdx.x; k < 16 * kStageLd; k += kThreads) st.h[80 * kStageLd + k] = (typename P::elem)0.0f;
 
     const uint32_t n_iter = (t.n + kVert - 1) / kVert;
     for (uint32_t it = blockIdx.x; it < n_iter; it += gridDim.x) {
+        // With two waves per SIMD (256 VGPRs) the weight tiles are reloaded (from cache)
+        // in every iteration rather than held in registers across it: the opaque copy of
+        // the pointers stops the compiler from hoisting the loads (about 150 VGPRs).
+        const op* Wf = Wf0;
+        const op* Wb = Wb0;
+        if constexpr (kWaves == 8) asm volatile("" : "+s"(Wf), "+s"(Wb));
         const uint32_t i = it * kVert + wave * 32 + col;
         const bool live = i < t.n;
         const int vcol = wave * 32 + col;  // vertex column in the stage
 
-        // ---- encoding (B layout, natural channel order)
+        // ---- encoding (B layout, natural channel order): lane half h holds channels
+        // 16kb + 8h + j, so it gathers only levels 8kb + 4h + {0..3} of the grid
         op E[5];
         {
-            float e[kEnc];
+            float e8[8];
             if (live) {
-                if constexpr (P::kF32Table) {
-                    encode_full<float>(t.a, t.net, t.table_f32, i, e);
-                } else {
-                    float x[3];
-                    load_xn(t.a, i, x);
+                float x[3];
+                load_xn(t.a, i, x);
 #pragma unroll
-                    for (int l = 0; l < kMaxLevels; ++l) {
+                for (int kb = 0; kb < 2; ++kb) {
+#pragma unroll
+                    for (int ll = 0; ll < 4; ++ll) {
+                        const int l = 8 * kb + 4 * h + ll;
                         float2 f = make_float2(0.0f, 0.0f);
-                        if (l < (int)t.a.grid.n_levels)
-                            f = grid_level_pairs<PairLoadLp<P>>(t.table_lp, t.a.grid, l, x);
-                        e[2 * l] = f.x;
-                        e[2 * l + 1] = f.y;
+                        if (l < (int)t.a.grid.n_levels) {
+                            if constexpr (P::kF32Table)
+                                f = grid_level_pairs<PairLoad<float>>(t.table_f32, t.a.grid, l, x);
+                            else
+                                f = grid_level_pairs<PairLoadLp<P>>(t.table_lp, t.a.grid, l, x);
+                        }
+                        e8[2 * ll] = f.x;
+                        e8[2 * ll + 1] = f.y;
                     }
-                    encode_tail(t.a, t.net, i, e);
+                    E[kb] = P::pack(e8);
+                }
+                float e[kEnc];
+                encode_tail(t.a, t.net, i, e);
+#pragma unroll
+                for (int kb = 2; kb < 5; ++kb) {
+#pragma unroll
+                    for (int j = 0; j < 8; ++j) e8[j] = h ? e[kb * 16 + 8 + j] : e[kb * 16 + j];
+                    E[kb] = P::pack(e8);
                 }
             } else {
 #pragma unroll
-                for (int k = 0; k < kEnc; ++k) e[k] = 0.0f;
-            }
+                for (int j = 0; j < 8; ++j) e8[j] = 0.0f;
 #pragma unroll
-            for (int kb = 0; kb < 5; ++kb) {
-                float x8[8];
-#pragma unroll
-                for (int j = 0; j < 8; ++j) x8[j] = h ? e[kb * 16 + 8 + j] : e[kb * 16 + j];
-                E[kb] = P::pack(x8);
+                for (int kb = 0; kb < 5; ++kb) E[kb] = P::pack(e8);
             }
         }
 
@@ -277,7 +302,7 @@ __global__ __launch_bounds__(kThreads) void k_train(TrainArgs t) {
                         const float y = Y[r];
                         const float den = y * y + 0.01f;
                         const float d = y - tg;
-                        loss_acc += d * d / den * t.inv_norm;
+                        atomicAdd(&loss_part[wave], d * d / den * t.inv_norm);
                         dyr[r] = 2.0f * d / den * t.inv_norm;
                     }
                 }
@@ -310,7 +335,7 @@ __global__ __launch_bounds__(kThreads) void k_train(TrainArgs t) {
                                         t.gamma[2] * q + t.gamma[3] * (q - qt) * (q - qt);
                         const float fp = -(t.gamma[0] + t.gamma[1]) * vv / (qe * qe) + t.gamma[0] * t.ebar +
                                          t.gamma[1] * e + t.gamma[2] + 2.0f * t.gamma[3] * (q - qt);
-                        loss_acc += w * f * t.inv_norm;
+                        atomicAdd(&loss_part[wave], w * f * t.inv_norm);
                         dyr[0] = w * fp * activation_sigma_grad(y) * t.inv_norm;
                     }
                 }
@@ -377,9 +402,9 @@ __global__ __launch_bounds__(kThreads) void k_train(TrainArgs t) {
             // (3) dW tiles of this layer owned by this wave
             {
                 const int n_ti = layer == 0 ? 3 : 2;
-                const int n_tiles = dw_tiles(layer, NL);
-                int slot = dw_slot_base(layer, NL);
-                for (int tt = wave; tt < n_tiles; tt += kWaves, ++slot) {
+                const int base = dw_tile_base(layer, NL), n_tiles = dw_tiles(layer, NL);
+                for (int g = dw_first(base, wave, kWaves); g < base + n_tiles; g += kWaves) {
+                    const int tt = g - base, slot = g / kWaves;
                     const int to = tt / n_ti, ti = tt % n_ti;
                     f32x16 a = f32x16{};
 #pragma unroll
@@ -488,8 +513,9 @@ __global__ __launch_bounds__(kThreads) void k_train(TrainArgs t) {
         const int in = layer == 0 ? kEnc : kHidden;
         const int out = layer == NL - 1 ? NOUT : kHidden;
         const int woff = layer == 0 ? 0 : kHidden * kEnc + (layer - 1) * kHidden * kHidden;
-        int slot = dw_slot_base(layer, NL);
-        for (int tt = wave; tt < n_tiles; tt += kWaves, ++slot) {
+        const int base = dw_tile_base(layer, NL);
+        for (int g = dw_first(base, wave, kWaves); g < base + n_tiles; g += kWaves) {
+            const int tt = g - base, slot = g / kWaves;
             const int to = tt / n_ti, ti = tt % n_ti;
             f32x16 a = f32x16{};
 #pragma unroll
@@ -503,9 +529,6 @@ __global__ __launch_bounds__(kThreads) void k_train(TrainArgs t) {
             }
         }
     }
-#pragma unroll
-    for (int o = 32; o > 0; o >>= 1) loss_acc += __shfl_xor(loss_acc, o, 64);
-    if (lane == 0) loss_part[wave] = loss_acc;
     __syncthreads();
     if (threadIdx.x == 0 && t.loss_out) {
         float s = 0.0f;
@@ -585,9 +608,17 @@ hipError_t launch_train_one(const TrainArgs& t, uint32_t blocks, hipStream_t s) 
         if (e != hipSuccess) return e;
         attr = true;
     }
-    hipLaunchKernelGGL((k_train<P, NL, NOUT, LOSS>), dim3(blocks), dim3(kThreads), smem, s, t);
+    hipLaunchKernelGGL((k_train<P, NL, NOUT, LOSS>), dim3(blocks), dim3(Geo<P>::kThreads), smem, s, t);
     return hipGetLastError();
 }
+
+}  // namespace
+
+uint32_t train_vertices_per_iter(int precision) {
+    return precision == NRRS_FP32 ? Geo<TF32>::kVert : Geo<TBF16>::kVert;
+}
+
+namespace {
 
 template <typename P>
 hipError_t dispatch_train(const TrainArgs& t, int nl, int n_out, int loss, uint32_t blocks, hipStream_t s) {

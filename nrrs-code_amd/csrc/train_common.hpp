@@ -50,6 +50,8 @@ size_t train_op_bytes(int precision);
 hipError_t launch_pack_train(int precision, const float* W, int nl, int n_out, void* fwd, void* bwd, hipStream_t s);
 hipError_t launch_train(int precision, const TrainArgs& t, int nl, int n_out, int loss, uint32_t blocks,
                         hipStream_t s);
+// Vertices one training workgroup processes per iteration.
+uint32_t train_vertices_per_iter(int precision);
 hipError_t launch_reduce_dense(float* rep_f32, void* rep_h, uint32_t n_dense, float* grad, void* grad_h,
                                hipStream_t s);
 hipError_t launch_reduce_partials(const float* part, uint32_t n_blocks, uint32_t n_mlp, float* grad, hipStream_t s);
