@@ -211,6 +211,23 @@ nrrs_status ensure_train(nrrs_ctx* c, int net) {
             NRRS_HIP(hipMemset(ns.dense_rep, 0, bytes));
         }
     }
+    if (c->cfg.precision != NRRS_FP32) {
+        // Binned scatter of the hashed levels: segments of kSegEntries entries, each with room
+        // for 1.25x its expected share of a full batch's 8 records per vertex and level.
+        const uint32_t T = c->grid.hash_mask + 1u;
+        uint32_t n_hashed = 0;
+        for (uint32_t l = 0; l < c->grid.n_levels; ++l)
+            if (!((c->grid.dense_mask >> l) & 1u)) ++n_hashed;
+        if (n_hashed && T >= kSegEntries && T / kSegEntries <= kMaxSegBins) {
+            ns.seg_bins = T / kSegEntries;
+            ns.n_seg = n_hashed * ns.seg_bins;
+            const uint64_t expect = (uint64_t)c->cfg.max_train * 8u / ns.seg_bins;
+            ns.rec_cap = (uint32_t)((expect + expect / 4 + 4096) & ~63ull);  // even: 16-byte rows
+            NRRS_TRY(dmalloc(reinterpret_cast<uint8_t**>(&ns.rec_buf), (size_t)ns.n_seg * ns.rec_cap * sizeof(uint2)));
+            NRRS_TRY(dmalloc(&ns.seg_count, ns.n_seg));
+            NRRS_HIP(hipMemset(ns.seg_count, 0, (size_t)ns.n_seg * 4));
+        }
+    }
     NRRS_TRY(dmalloc(&ns.loss_scale, 1));
     NRRS_TRY(dmalloc(&ns.state, 4));
     NRRS_HIP(hipMemset(ns.grad, 0, np * 4));
@@ -269,6 +286,10 @@ nrrs_status run_backward(nrrs_ctx* c, int net, uint32_t n, const nrrs_vertices& 
         t.dense_f32 = static_cast<float*>(ns.dense_rep);
     else
         t.dense_h2 = static_cast<_Float16 __attribute__((ext_vector_type(2)))*>(ns.dense_rep);
+    t.rec_buf = ns.rec_buf;
+    t.seg_count = ns.seg_count;
+    t.rec_cap = ns.rec_cap;
+    t.seg_bins = ns.seg_bins;
     t.dw_partial = ns.dw_partial;
     t.n_mlp = (uint32_t)ns.n_mlp;
     t.loss_out = loss_out;
@@ -277,6 +298,18 @@ nrrs_status run_backward(nrrs_ctx* c, int net, uint32_t n, const nrrs_vertices& 
     uint32_t blocks = iters < c->max_train_blocks ? iters : c->max_train_blocks;
     if (!blocks) return NRRS_OK;
     NRRS_HIP(launch_train(cfg.precision, t, ns.n_layers, ns.n_out, kind, blocks, s));
+    if (ns.seg_bins) {
+        ApplyArgs ap{};
+        ap.rec = ns.rec_buf;
+        ap.seg_count = ns.seg_count;
+        ap.cap = ns.rec_cap;
+        ap.bins = ns.seg_bins;
+        uint32_t hl = 0;
+        for (uint32_t l = 0; l < c->grid.n_levels; ++l)
+            if (!((c->grid.dense_mask >> l) & 1u)) ap.level_off[hl++] = c->grid.offset[l];
+        ap.grad = reinterpret_cast<_Float16 __attribute__((ext_vector_type(2)))*>(ns.grad_lp);
+        NRRS_HIP(launch_apply_bins(ap, ns.n_seg, s));
+    }
     return hip_status(launch_reduce_partials(ns.dw_partial, blocks, (uint32_t)ns.n_mlp, ns.grad + ns.n_grid, s));
 }
 
@@ -391,6 +424,8 @@ void free_ctx(nrrs_ctx* c) {
         hipFree(ns.grad);
         hipFree(ns.grad_lp);
         hipFree(ns.dense_rep);
+        hipFree(ns.rec_buf);
+        hipFree(ns.seg_count);
         hipFree(ns.adam_m);
         hipFree(ns.adam_v);
         hipFree(ns.w_fwd);

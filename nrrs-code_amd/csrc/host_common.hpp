@@ -73,6 +73,11 @@ struct NetState {
     uint16_t* grad_lp = nullptr;     // FP16 grid gradient of FP16/BF16 contexts [n_grid]
     void* dense_rep = nullptr;       // dense-level gradient replicas (FP32 pairs or FP16 pairs)
     uint32_t n_dense = 0;            // table entries of the dense levels
+    uint2* rec_buf = nullptr;        // binned hashed-level gradient records (FP16/BF16)
+    uint32_t* seg_count = nullptr;   // records per segment
+    uint32_t rec_cap = 0;            // record capacity of a segment
+    uint32_t seg_bins = 0;           // segments per hashed level (0: direct atomics)
+    uint32_t n_seg = 0;              // segments over all hashed levels
     float* adam_m = nullptr;
     float* adam_v = nullptr;
     void* w_fwd = nullptr;           // packed training tiles (forward / transposed)

@@ -205,6 +205,7 @@ __global__ __launch_bounds__(64 * P::kWaves) void k_train(TrainArgs t) {
     extern __shared__ __align__(16) unsigned char smem_raw[];
     Stage<P>& st = *reinterpret_cast<Stage<P>*>(smem_raw);
     __shared__ float loss_part[kWaves];  // loss per wave, accumulated with LDS atomics
+    __shared__ uint32_t bin_cnt[kMaxSegBins], bin_gbase[kMaxSegBins], bin_lstart[kMaxSegBins + 1];
 
     const int lane = threadIdx.x & 63, h = lane >> 5, col = lane & 31, wave = threadIdx.x >> 6;
     const op* Wf0 = reinterpret_cast<const op*>(t.w_fwd);
@@ -215,6 +216,7 @@ __global__ __launch_bounds__(64 * P::kWaves) void k_train(TrainArgs t) {
 #pragma unroll
     for (int s = 0; s < kSlots; ++s) acc[s] = f32x16{};
     if (lane == 0) loss_part[wave] = 0.0f;  // by the wave that accumulates into it
+    if (threadIdx.x < kMaxSegBins) bin_cnt[threadIdx.x] = 0;  // first used after several barriers
 
     // zero the padded input rows once (layer 0 has 80 of 96 rows)
     for (int k = threadIdx.x; k < 16 * kStageLd; k += kThreads) st.h[80 * kStageLd + k] = (typename P::elem)0.0f;
@@ -450,34 +452,101 @@ __global__ __launch_bounds__(64 * P::kWaves) void k_train(TrainArgs t) {
                 f32x16 G = f32x16{};
 #pragma unroll
                 for (int kb = 0; kb < 4; ++kb) G = P::mma16(load_op<P>(Wb, kb, lane), Dl[kb], G);
-                // Scatter to the table. Lanes 2k and 2k+1 take vertex k of this wave's 32
-                // (whose level gradients sit in lanes k and k+32, fetched with ds_bpermute), so
-                // that the two lanes of a pair update neighbouring table words in the same
-                // atomic instruction, which the memory pipeline merges into one request:
+                // Scatter to the table. The level gradients are staged in LDS as
+                // [grid channel][vertex] (the delta rows are free once every wave is past its
+                // dW tiles). Lanes 2k and 2k+1 then take vertex k of this wave's 32, so that the
+                // two lanes of a pair update neighbouring table words in the same atomic
+                // instruction, which the memory pipeline merges into one request:
                 //  - FP16 table: lane parity = x bit of the corner; the x-adjacent corners are
                 //    consecutive entries in a dense level and the aligned pair {h & ~1, h | 1} of
                 //    a hashed level when the cell's x is even;
                 //  - FP32 table: lane parity = feature, both features of a corner are one 8-byte
                 //    entry.
+                __syncthreads();
+                float* gst = reinterpret_cast<float*>(st.d);  // [32][kVert]
+#pragma unroll
+                for (int r = 0; r < 16; ++r) gst[(8 * (r >> 2) + 4 * h + (r & 3)) * kVert + vcol] = G[r];
+                // the values a lane reads were written by its own wave
+                __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+                __builtin_amdgcn_wave_barrier();
+                __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
                 const int pv = lane >> 1, pb = lane & 1;
                 const uint32_t iv = it * kVert + wave * 32 + pv;
                 const bool live_v = iv < t.n;
                 float x[3] = {0.0f, 0.0f, 0.0f};
                 if (live_v) load_xn(t.a, iv, x);
+#pragma unroll 1
+                for (int l = 0; l < (int)t.a.grid.n_levels; ++l) {
+                    const float g0 = gst[(2 * l) * kVert + wave * 32 + pv];
+                    const float g1 = gst[(2 * l + 1) * kVert + wave * 32 + pv];
+                    const bool dense = (t.a.grid.dense_mask >> l) & 1u;
+                    if constexpr (!P::kF32Table) {
+                        if (!dense && t.seg_bins) {
+                            // Binned scatter of a hashed level (uniform branch: every thread
+                            // takes part in the barriers). The block's 8 records per vertex are
+                            // counted per segment in LDS, each segment reserves its run in
+                            // rec_buf with one global atomic, the records are sorted by segment
+                            // through LDS and written out as contiguous runs.
+                            uint32_t re[4], rk[4], rv[4];
+                            if (live_v) {
+                                LevelCell lc = level_cell(t.a.grid, l, x);
 #pragma unroll
-                for (int l = 0; l < kMaxLevels; ++l) {
-                    if (l >= (int)t.a.grid.n_levels) break;
-                    // features of level l: registers 4(l/4) + 2(l%2) + {0,1} of lane pv + 32((l/2)%2)
-                    const int k = 4 * (l >> 2) + 2 * (l & 1);
-                    const int src = 4 * (pv + 32 * ((l >> 1) & 1));
-                    const float g0 = __int_as_float(__builtin_amdgcn_ds_bpermute(src, __float_as_int(G[k])));
-                    const float g1 = __int_as_float(__builtin_amdgcn_ds_bpermute(src, __float_as_int(G[k + 1])));
+                                for (int yz = 0; yz < 4; ++yz) {
+                                    const int cn = pb | (yz << 1);
+                                    re[yz] = corner_index(t.a.grid, l, lc, cn);  // < T = seg_bins << kSegShift
+                                    const float w = corner_weight(lc, cn);
+                                    const h2v gv = {(_Float16)(w * g0), (_Float16)(w * g1)};
+                                    rv[yz] = __builtin_bit_cast(uint32_t, gv);
+                                    rk[yz] = atomicAdd(&bin_cnt[re[yz] >> kSegShift], 1u);
+                                }
+                            }
+                            __syncthreads();
+                            // segments of this level: hashed levels are numbered in table order
+                            const uint32_t seg0 =
+                                (uint32_t)(l - __builtin_popcount(t.a.grid.dense_mask & ((1u << l) - 1u))) * t.seg_bins;
+                            if (threadIdx.x < 64) {  // wave 0: reserve runs, exclusive scan of the counts
+                                const uint32_t b = threadIdx.x;
+                                const uint32_t c = b < t.seg_bins ? bin_cnt[b] : 0u;
+                                if (b < t.seg_bins) {
+                                    bin_cnt[b] = 0;
+                                    bin_gbase[b] = c ? atomicAdd(t.seg_count + seg0 + b, c) : 0u;
+                                }
+                                uint32_t inc = c;
+#pragma unroll
+                                for (int o = 1; o < 64; o <<= 1) {
+                                    const uint32_t y = __shfl_up(inc, o, 64);
+                                    if ((int)b >= o) inc += y;
+                                }
+                                if (b < t.seg_bins) bin_lstart[b] = inc - c;
+                                if (b == 63) bin_lstart[kMaxSegBins] = inc;
+                            }
+                            __syncthreads();
+                            uint2* stg = reinterpret_cast<uint2*>(st.h);  // free until the next iteration
+                            if (live_v) {
+#pragma unroll
+                                for (int yz = 0; yz < 4; ++yz)
+                                    stg[bin_lstart[re[yz] >> kSegShift] + rk[yz]] = make_uint2(re[yz], rv[yz]);
+                            }
+                            __syncthreads();
+                            const uint32_t total = bin_lstart[kMaxSegBins];
+                            for (uint32_t k = threadIdx.x; k < total; k += kThreads) {
+                                const uint2 r = stg[k];
+                                const uint32_t b = r.x >> kSegShift;
+                                const uint32_t pos = bin_gbase[b] + (k - bin_lstart[b]);
+                                if (pos < t.rec_cap)
+                                    t.rec_buf[(size_t)(seg0 + b) * t.rec_cap + pos] = r;
+                                else  // segment full (clustered input): add directly
+                                    __builtin_amdgcn_global_atomic_fadd_v2f16(t.grad_grid_h2 + t.a.grid.offset[l] + r.x,
+                                                                              __builtin_bit_cast(h2v, r.y));
+                            }
+                            continue;
+                        }
+                    }
                     if (!live_v) continue;
                     LevelCell lc = level_cell(t.a.grid, l, x);
                     // Dense (coarse) levels are small and hit by every vertex: their updates go
                     // to one of kDenseReplicas private copies (chosen by workgroup), summed before
                     // the optimiser step, which spreads the contention on each entry.
-                    const bool dense = (t.a.grid.dense_mask >> l) & 1u;
                     const size_t rep = dense ? (size_t)(blockIdx.x % kDenseReplicas) * t.n_dense : 0;
                     if (t.grad_grid_h2) {
 #pragma unroll
@@ -633,6 +702,78 @@ hipError_t dispatch_train(const TrainArgs& t, int nl, int n_out, int loss, uint3
     return hipErrorInvalidValue;
 }
 
+// Sum of the binned records of one segment (one workgroup per segment) into the FP16 table
+// gradient. Every FP16 value is an integer multiple of 2^-24 below 2^16, so the records are
+// summed exactly as 64-bit fixed point with integer LDS atomics (LDS float atomics are an
+// order of magnitude slower); the sum does not depend on the record order. A non-finite
+// record turns the segment's first entry into NaN, so the finite check of the optimiser
+// step sees it. The segment's count is cleared for the next launch.
+constexpr int kApplyThreads = 1024;
+constexpr int kApplyBatch = 4;
+
+__device__ __forceinline__ void apply_half(unsigned long long* acc, uint32_t slot, uint32_t h, uint32_t& bad) {
+    const uint32_t ex = (h >> 10) & 31u, m = h & 1023u;
+    if (ex == 31u) {
+        bad = 1u;
+        return;
+    }
+    const unsigned long long mag = ex ? (unsigned long long)(1024u + m) << (ex - 1u) : (unsigned long long)m;
+    if (mag) atomicAdd(&acc[slot], (h & 0x8000u) ? 0ull - mag : mag);
+}
+
+__device__ __forceinline__ void apply_record(unsigned long long* acc, uint32_t e, uint32_t bits, uint32_t& bad) {
+    e &= kSegEntries - 1u;
+    apply_half(acc, 2 * e, bits & 0xffffu, bad);
+    apply_half(acc, 2 * e + 1, bits >> 16, bad);
+}
+
+__global__ __launch_bounds__(kApplyThreads) void k_apply_bins(ApplyArgs a) {
+    extern __shared__ unsigned long long acc_s[];  // [kSegEntries][2], units of 2^-24
+    __shared__ uint32_t bad_s;
+    const uint32_t seg = blockIdx.x, hl = seg / a.bins, b = seg % a.bins;
+    for (uint32_t k = threadIdx.x; k < 2 * kSegEntries; k += kApplyThreads) acc_s[k] = 0ull;
+    if (threadIdx.x == 0) bad_s = 0u;
+    __syncthreads();
+    uint32_t bad = 0u;
+    const uint32_t n = min(a.seg_count[seg], a.cap);
+    const uint2* r = a.rec + (size_t)seg * a.cap;  // cap is even: 16-byte aligned
+    const uint4* r4 = static_cast<const uint4*>(__builtin_assume_aligned(r, 16));
+    const uint32_t n2 = n / 2;
+    constexpr uint32_t kStep = kApplyThreads * kApplyBatch;
+    uint32_t k0 = 0;
+    for (; k0 + kStep <= n2; k0 += kStep) {  // full batches: every load in bounds, all issued first
+        uint4 q[kApplyBatch];
+#pragma unroll
+        for (int u = 0; u < kApplyBatch; ++u) q[u] = r4[k0 + u * kApplyThreads + threadIdx.x];
+#pragma unroll
+        for (int u = 0; u < kApplyBatch; ++u) {
+            apply_record(acc_s, q[u].x, q[u].y, bad);
+            apply_record(acc_s, q[u].z, q[u].w, bad);
+        }
+    }
+    for (uint32_t k = k0 + threadIdx.x; k < n2; k += kApplyThreads) {
+        const uint4 q = r4[k];
+        apply_record(acc_s, q.x, q.y, bad);
+        apply_record(acc_s, q.z, q.w, bad);
+    }
+    if ((n & 1u) && threadIdx.x == 0) apply_record(acc_s, r[n - 1].x, r[n - 1].y, bad);
+    if (bad) bad_s = 1u;
+    __syncthreads();
+    h2v* g = a.grad + a.level_off[hl] + (size_t)b * kSegEntries;
+    for (uint32_t e = threadIdx.x; e < kSegEntries; e += kApplyThreads) {
+        const long long s0 = (long long)acc_s[2 * e], s1 = (long long)acc_s[2 * e + 1];
+        if (s0 != 0 || s1 != 0) {
+            const h2v o = g[e];
+            g[e] = h2v{(_Float16)((float)o.x + (float)((double)s0 * 0x1p-24)),
+                       (_Float16)((float)o.y + (float)((double)s1 * 0x1p-24))};
+        }
+    }
+    if (threadIdx.x == 0) {
+        if (bad_s) g[0] = h2v{(_Float16)__builtin_nanf(""), (_Float16)__builtin_nanf("")};
+        a.seg_count[seg] = 0;
+    }
+}
+
 // Sum of the dense-level gradient replicas into the table gradient; the replicas are cleared.
 __global__ void k_reduce_dense(float* __restrict__ rep_f32, _Float16* __restrict__ rep_h, uint32_t n_dense,
                                float* __restrict__ grad, _Float16* __restrict__ grad_h) {
@@ -692,6 +833,20 @@ hipError_t launch_train(int precision, const TrainArgs& t, int nl, int n_out, in
     if (precision == NRRS_FP32) return dispatch_train<TF32>(t, nl, n_out, loss, blocks, s);
     if (precision == NRRS_BF16) return dispatch_train<TBF16>(t, nl, n_out, loss, blocks, s);
     return dispatch_train<TF16>(t, nl, n_out, loss, blocks, s);
+}
+
+hipError_t launch_apply_bins(const ApplyArgs& a, uint32_t n_seg, hipStream_t s) {
+    if (!n_seg) return hipSuccess;
+    constexpr size_t smem = 2 * kSegEntries * sizeof(unsigned long long);
+    static bool attr = false;
+    if (!attr) {
+        hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(&k_apply_bins),
+                                           hipFuncAttributeMaxDynamicSharedMemorySize, (int)smem);
+        if (e != hipSuccess) return e;
+        attr = true;
+    }
+    hipLaunchKernelGGL(k_apply_bins, dim3(n_seg), dim3(kApplyThreads), smem, s, a);
+    return hipGetLastError();
 }
 
 hipError_t launch_reduce_dense(float* rep_f32, void* rep_h, uint32_t n_dense, float* grad, void* grad_h,

@@ -7,6 +7,18 @@ namespace nrrs {
 
 // Private copies of the dense-level table gradient (see the scatter in k_train).
 constexpr uint32_t kDenseGradReplicas = 16;
+// Segments of the binned hashed-level scatter.
+constexpr uint32_t kSegShift = 13;
+constexpr uint32_t kSegEntries = 1u << kSegShift;
+constexpr uint32_t kMaxSegBins = 64;
+
+struct ApplyArgs {
+    const uint2* rec;              // [segments][cap]
+    uint32_t* seg_count;           // [segments], cleared after use
+    uint32_t cap, bins;
+    uint32_t level_off[kMaxLevels];  // first table entry of hashed level hl
+    _Float16 __attribute__((ext_vector_type(2)))* grad;  // FP16 pair table gradient
+};
 
 struct TrainArgs {
     EncodeArgs a;                  // vertices, film, RRSNet statistics input, grid
@@ -39,6 +51,14 @@ struct TrainArgs {
     float* dense_f32;              // [replicas][n_dense][2] (FP32 contexts)
     _Float16 __attribute__((ext_vector_type(2)))* dense_h2;  // [replicas][n_dense] (FP16/BF16)
     uint32_t n_dense;              // entries of the dense levels (they come first in the table)
+    // Binned scatter of the hashed levels (FP16/BF16 contexts). Each hashed level is cut into
+    // seg_bins segments of kSegEntries entries; the kernel appends {entry within the level,
+    // FP16 pair} records to rec_buf[segment][..] and the apply kernel sums them into the
+    // table gradient. seg_bins == 0 selects direct atomics instead.
+    uint2* rec_buf;                // [hashed levels * seg_bins][rec_cap]
+    uint32_t* seg_count;           // [hashed levels * seg_bins] records appended so far
+    uint32_t rec_cap;              // records per segment; the excess is added atomically
+    uint32_t seg_bins;
     float* dw_partial;             // [gridDim.x][n_mlp]
     uint32_t n_mlp;
     float* loss_out;               // device scalar (atomic add), or null
@@ -50,6 +70,8 @@ size_t train_op_bytes(int precision);
 hipError_t launch_pack_train(int precision, const float* W, int nl, int n_out, void* fwd, void* bwd, hipStream_t s);
 hipError_t launch_train(int precision, const TrainArgs& t, int nl, int n_out, int loss, uint32_t blocks,
                         hipStream_t s);
+// Sums the binned records of n_seg segments into the table gradient.
+hipError_t launch_apply_bins(const ApplyArgs& a, uint32_t n_seg, hipStream_t s);
 // Vertices one training workgroup processes per iteration.
 uint32_t train_vertices_per_iter(int precision);
 hipError_t launch_reduce_dense(float* rep_f32, void* rep_h, uint32_t n_dense, float* grad, void* grad_h,
