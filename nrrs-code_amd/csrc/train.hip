@@ -798,13 +798,37 @@ __global__ void k_reduce_dense(float* __restrict__ rep_f32, _Float16* __restrict
     }
 }
 
-__global__ void k_reduce_partials(const float* __restrict__ part, uint32_t n_blocks, uint32_t n_mlp,
-                                  float* __restrict__ grad) {
-    const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
-    if (k >= n_mlp) return;
+// Sum over workgroups of the dW partials, in a fixed order: a block takes 64 parameters,
+// its 16 thread groups take every 16th partial row (8 loads in flight), then the groups
+// are summed in group order.
+constexpr int kRedParams = 64, kRedGroups = 16;
+
+__global__ __launch_bounds__(kRedParams* kRedGroups) void k_reduce_partials(const float* __restrict__ part,
+                                                                           uint32_t n_blocks, uint32_t n_mlp,
+                                                                           float* __restrict__ grad) {
+    __shared__ float red[kRedGroups][kRedParams];
+    const uint32_t pl = threadIdx.x % kRedParams, g = threadIdx.x / kRedParams;
+    const uint32_t k = blockIdx.x * kRedParams + pl;
     float s = 0.0f;
-    for (uint32_t b = 0; b < n_blocks; ++b) s += part[(size_t)b * n_mlp + k];
-    grad[k] += s;
+    if (k < n_mlp) {
+        uint32_t b = g;
+        for (; b + 7 * kRedGroups < n_blocks; b += 8 * kRedGroups) {
+            float v[8];
+#pragma unroll
+            for (int u = 0; u < 8; ++u) v[u] = part[(size_t)(b + u * kRedGroups) * n_mlp + k];
+#pragma unroll
+            for (int u = 0; u < 8; ++u) s += v[u];
+        }
+        for (; b < n_blocks; b += kRedGroups) s += part[(size_t)b * n_mlp + k];
+    }
+    red[g][pl] = s;
+    __syncthreads();
+    if (g == 0 && k < n_mlp) {
+        float t = 0.0f;
+#pragma unroll
+        for (int q = 0; q < kRedGroups; ++q) t += red[q][pl];
+        grad[k] += t;
+    }
 }
 
 }  // namespace
@@ -858,7 +882,8 @@ hipError_t launch_reduce_dense(float* rep_f32, void* rep_h, uint32_t n_dense, fl
 }
 
 hipError_t launch_reduce_partials(const float* part, uint32_t n_blocks, uint32_t n_mlp, float* grad, hipStream_t s) {
-    hipLaunchKernelGGL(k_reduce_partials, dim3((n_mlp + 255) / 256), dim3(256), 0, s, part, n_blocks, n_mlp, grad);
+    hipLaunchKernelGGL(k_reduce_partials, dim3((n_mlp + kRedParams - 1) / kRedParams), dim3(kRedParams * kRedGroups), 0,
+                       s, part, n_blocks, n_mlp, grad);
     return hipGetLastError();
 }
 
