@@ -31,9 +31,10 @@ N_PIXELS = 1 << 20
 SEED = 0x4E525253 ^ 2
 HBM_PEAK_GBS = 8000.0
 BF16_PEAK_TFLOPS = 2500.0
-# HBM bytes per launch of the fused StatNet training kernel at 2^20 samples, from
-# rocprofv3 --pmc FETCH_SIZE and WRITE_SIZE (profiles/train_kernel_bf16_2p20_pmc.md), or None.
-TRAIN_TRAFFIC_BYTES = 8651328117
+# HBM bytes of one StatNet training backward at 2^20 samples (k_train + k_apply_bins +
+# k_reduce_partials), from rocprofv3 --pmc FETCH_SIZE and WRITE_SIZE
+# (profiles/train_backward_bf16_2p20_pmc.md), or None.
+TRAIN_TRAFFIC_BYTES = 5484942336
 MACS = {"statnet": 80 * 64 + 3 * 64 * 64 + 64 * 6, "rrsnet": 80 * 64 + 2 * 64 * 64 + 64 * 1}
 
 
@@ -340,9 +341,12 @@ def main():
 
     bf = results["bf16"]
     n = N_VERTICES
-    # Dominant kernel of the BF16 step: the fused StatNet training kernel (k_train, NRRS_LL2).
-    # Algorithmic bytes per sample: input record 64 B + target 12 B, 16 levels x 8 corners x
-    # 4 B of BF16 feature pairs gathered, and the same 512 B of packed-FP16 gradient atomics.
+    # Dominant operation of the BF16 step: the StatNet training backward (nrrs_backward with
+    # NRRS_LL2): k_train (forward with recompute, loss, backward, dW, dense-level atomics and
+    # binned hashed-level gradient records), k_apply_bins (sums the records into the table
+    # gradient) and k_reduce_partials (dW). Algorithmic bytes per sample: input record 64 B +
+    # target 12 B, 16 levels x 8 corners x 4 B of BF16 feature pairs gathered, and the same
+    # 512 B of FP16 gradient pairs added to the table.
     bytes_s = 64 + 12 + 16 * 8 * 4 + 16 * 8 * 4
     flops_s = 3 * 2 * MACS["statnet"]  # forward, dX, dW
     kern_s = bf["train_kernel_us"] * 1e-6
@@ -350,12 +354,13 @@ def main():
     tflops = flops_s * n / kern_s / 1e12
     roofline = {"bound": "hbm", "achieved": round(gbs, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": round(gbs / HBM_PEAK_GBS, 4), "traffic": TRAIN_TRAFFIC_BYTES,
-                "kernel": "k_train<TBF16,5,6,NRRS_LOSS_LL2> (StatNet forward + loss + backward, 2^20 samples)",
+                "kernel": "nrrs_backward(StatNet, NRRS_LL2) at 2^20 samples: k_train<TBF16,5,6,NRRS_LOSS_LL2> + "
+                          "k_apply_bins + k_reduce_partials",
                 "kernel_us": round(bf["train_kernel_us"], 1),
                 "algorithmic_bytes_per_launch": bytes_s * n,
                 "mfma_tflops": round(tflops, 2), "mfma_frac": round(tflops / BF16_PEAK_TFLOPS, 5),
-                "grid_atomics_per_launch": 16 * 8 * n,
-                "grid_atomic_rate_g_per_s": round(16 * 8 * n / kern_s / 1e9, 2)}
+                "hashed_level_records_per_launch": 12 * 8 * n,
+                "dense_level_atomic_lanes_per_launch": 4 * 8 * n}
     matched = {}
     if "fp32" in q_by:
         for name in q_by:

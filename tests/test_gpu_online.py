@@ -1,0 +1,116 @@
+"""Online loop on the GPU: frames of persistent multi-bounce inference interleaved with
+training steps (StatNet alone first, then StatNet + RRSNet), and the drift of q after it.
+
+- Inference drift: the parameters the BF16 context has trained are read back and the
+  chained StatNet -> RRSNet q of a probe set is evaluated by the HIP path and by the FP64
+  oracle on the same parameters; the BF16 bound is the one of the untrained forward test.
+- Trajectory drift: an FP32 context trained on the same inputs from the same initial
+  parameters; q of the two trained networks on the probe set.
+"""
+import numpy as np
+import pytest
+
+import nrrs_amd as na
+
+pytestmark = pytest.mark.gpu
+
+P = 1 << 14       # pixels (128 x 128)
+B = 1 << 14       # training batch per frame
+FRAMES = 24
+WARM = 8          # StatNet-only frames
+BOUNCES = 4
+
+
+@pytest.fixture(scope="module")
+def torch_cuda():
+    import torch
+
+    assert torch.cuda.is_available()
+    return torch
+
+
+def _inputs(torch, f):
+    rng = np.random.default_rng(1000 + f)
+    L = rng.lognormal(-1.0, 1.5, (3, B))
+    L[:, rng.uniform(size=B) < 0.05] *= 100.0
+    qt = rng.uniform(0.05, 5.0, B).astype(np.float32)
+    qt[rng.uniform(size=B) < 0.1] = -1.0
+    return (torch.from_numpy(L.astype(np.float32)).cuda(),
+            dict(error=torch.from_numpy(rng.uniform(0, 2, B).astype(np.float32)).cuda(),
+                 sample_weight=torch.from_numpy((1.0 / rng.integers(1, 4, B)).astype(np.float32)).cuda(),
+                 q_target=torch.from_numpy(qt).cuda()))
+
+
+def _train(torch, prec, film):
+    cfg = na.default_config(precision=prec, n_pixels=P, max_queue=2 * P, max_train=B, lr_rrsnet=1e-3)
+    ctx = na.Context(cfg)
+    frame = None
+    if prec != na.FP32:  # the frame kernel runs in FP16/BF16 contexts
+        a, bufs = ctx.frame_buffers(P, BOUNCES, capacity=2 * P)
+        a.film_dev = film.data_ptr()
+        frame = (a, bufs)
+    loss = torch.zeros(2, device="cuda")
+    hist = []
+    for f in range(FRAMES):
+        if frame is not None:
+            frame[0].seed = 77 + f
+            na.check(ctx.frame_run(frame[0]), "frame_run")
+        vb = na.synth_vertices(B, 5000 + f, P)
+        L, sd = _inputs(torch, f)
+        side = ctx.side(error=sd["error"], sample_weight=sd["sample_weight"], q_target=sd["q_target"],
+                        error_sum=0.7 * P)
+        na.check(ctx.train_step(vb, 1 if f < WARM else 3, film=film, target_L=L, side=side, loss=loss),
+                 "train_step")
+        hist.append(loss.cpu().numpy().copy())
+    torch.cuda.synchronize()
+    if frame is not None:
+        assert int(frame[1]["status"].item()) == 0
+        sizes = frame[1]["sizes"].cpu().numpy()
+        assert sizes[0] == P
+    return ctx, np.array(hist)
+
+
+def _q_probe(torch, ctx, ora, vb, film_np, film):
+    stats = ctx.statnet_infer(vb)
+    raw = ctx.rrsnet_infer(vb, film, stats)
+    torch.cuda.synchronize()
+    hv = ora.HostVertices(vb.to_numpy())
+    q = ora.q_f64(ctx.cfg, na.Q_RRSNET, hv, raw.cpu().numpy()[None, :], film_np)
+    return stats.cpu().numpy(), raw.cpu().numpy(), q, hv
+
+
+def test_online_loop_q_drift(torch_cuda, ora):
+    torch = torch_cuda
+    rng = np.random.default_rng(9)
+    film_np = rng.lognormal(-1.0, 1.0, (3, P)).astype(np.float32)
+    film = torch.from_numpy(film_np).cuda()
+    ctx16, h16 = _train(torch, na.BF16, film)
+    ctx32, h32 = _train(torch, na.FP32, film)
+    assert np.all(np.isfinite(h16)) and np.all(np.isfinite(h32))
+
+    vb = na.synth_vertices(P, 424242, P)
+    st16, raw16, q16, hv = _q_probe(torch, ctx16, ora, vb, film_np, film)
+    _, _, q32, _ = _q_probe(torch, ctx32, ora, vb, film_np, film)
+
+    # inference drift of the trained BF16 networks against FP64 on the same parameters
+    ps = ctx16.params_get(na.STATNET).astype(np.float64)
+    pr = ctx16.params_get(na.RRSNET).astype(np.float64)
+    ref_stats = ora.net_forward(ctx16.cfg, na.STATNET, hv, film_np, None, ps).T
+    ref_raw = ora.net_forward(ctx16.cfg, na.RRSNET, hv, film_np, st16.astype(np.float64), pr)[:, 0]
+    q_ref = ora.q_f64(ctx16.cfg, na.Q_RRSNET, hv, ref_raw[None, :], film_np)
+    stat_err = np.max(np.abs(st16 - ref_stats)) / np.max(np.abs(ref_stats))
+    raw_err = np.max(np.abs(raw16 - ref_raw)) / np.max(np.abs(ref_raw))
+    q_rel = np.abs(q16 - q_ref) / q_ref
+    # trajectory drift BF16 vs FP32 training
+    t_rel = np.abs(q16 - q32) / q32
+    print(f"losses bf16 {h16[-1]} fp32 {h32[-1]}")
+    print(f"inference drift vs FP64: statnet {stat_err:.2e} rrsnet {raw_err:.2e} q max {q_rel.max():.2e} "
+          f"mean {q_rel.mean():.2e}")
+    print(f"trajectory drift bf16 vs fp32: q mean {t_rel.mean():.3e} p99 {np.percentile(t_rel, 99):.3e} "
+          f"max {t_rel.max():.3e}; mean q bf16 {q16.mean():.4f} fp32 {q32.mean():.4f}")
+    # BF16 MFMA forward bounds of the untrained-parameter tests (output range relative, q relative)
+    assert stat_err < 4e-2 and raw_err < 4e-2
+    assert q_rel.max() < 3e-2
+    # trained in BF16 vs FP32 from the same start on the same data: the nets agree on q
+    assert t_rel.mean() < 0.1
+    assert abs(q16.mean() - q32.mean()) / q32.mean() < 0.05
