@@ -214,6 +214,8 @@ typedef struct {
     const float* sample_weight; /* 1 / (training samples of the pixel)                      */
     const float* q_target;      /* q used when the path was traced; -1 = duplicate node      */
     float error_sum;            /* sum of the per-pixel error over the film                 */
+    const float* error_sum_dev; /* device scalar replacing error_sum when not NULL (the sum
+                                   nrrs_error_compute writes, integrator.cpp:2466-2471)      */
 } nrrs_rrs_side;
 
 /* One training step: StatNet (net_mask bit 0) on the NRRS_LL2 loss against target_L
@@ -289,6 +291,29 @@ nrrs_status nrrs_frame_graph_launch(nrrs_ctx* ctx, void* stream);
 /* The vertices bounce `bounce` of a frame draws for a queue (for tests and tools). */
 nrrs_status nrrs_frame_vertices(uint64_t seed, uint32_t bounce, const uint32_t* queue_dev, uint32_t n,
                                 const nrrs_vertices_out* out, void* stream);
+
+/* ---------------- per-pixel error feeding NRRS_RRS (train.cu:344-430) ---------------- */
+
+/* Running per-pixel radiance used by the error (nrrs_update_weighted_L, train.cu:344-363):
+ * weighted_L = weighted_L (1 - blend) + (L_frame / spp) blend per channel, all [3][n_pixels];
+ * spp_dev: samples of the pixel in L_frame. */
+nrrs_status nrrs_error_blend(nrrs_ctx* ctx, uint32_t n_pixels, const float* L_frame_dev, const uint32_t* spp_dev,
+                             float blend, float* weighted_L_dev, void* stream);
+
+/* Relative squared error per pixel (nrrs_calculate_error, train.cu:365-430):
+ * e = mean_c (L_c - ref_c)^2 / (ref_c^2 + 0.01), min(e, 100) when clamp; row 0 of error_dev
+ * holds e, row 1 e * sqrt(spp) (spp_dev, samples per pixel this frame; NULL: row 1 = e).
+ * error_sum_dev receives the sum of row 1 (integrator.cpp:2466-2471), summed in FP64 in a
+ * fixed order. */
+typedef struct {
+    const float* ref_dev;        /* [3][n] reference estimate (accumulated film)   */
+    const float* weighted_L_dev; /* [3][n] running estimate                        */
+    const float* spp_dev;        /* [n] or NULL                                    */
+    int32_t clamp;               /* clamp e at 100                                 */
+    float* error_dev;            /* [2][n]                                         */
+    float* error_sum_dev;        /* device scalar, or NULL                         */
+} nrrs_error_args;
+nrrs_status nrrs_error_compute(nrrs_ctx* ctx, uint32_t n_pixels, const nrrs_error_args* a, void* stream);
 
 #ifdef __cplusplus
 }

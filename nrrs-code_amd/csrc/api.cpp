@@ -269,6 +269,8 @@ nrrs_status run_backward(nrrs_ctx* c, int net, uint32_t n, const nrrs_vertices& 
         t.sample_weight = side->sample_weight;
         t.q_target = side->q_target;
         t.ebar = std::fmax(side->error_sum / (float)cfg.n_pixels, 1e-4f);
+        t.error_sum_dev = side->error_sum_dev;
+        t.inv_pixels = 1.0f / (float)cfg.n_pixels;
     }
     for (int k = 0; k < 4; ++k) t.gamma[k] = cfg.gamma[k];
     t.clamp_on = cfg.loss_clamp_on;
@@ -417,6 +419,7 @@ bool side_ok(const nrrs_rrs_side* side) {
 }
 
 void free_ctx(nrrs_ctx* c) {
+    hipFree(c->err_part);
     for (auto& ns : c->net) {
         hipFree(ns.params);
         hipFree(ns.grid_lp);
@@ -477,6 +480,9 @@ size_t nrrs_struct_size(int32_t which) {
         case 1: return sizeof(nrrs_vertices);
         case 2: return sizeof(nrrs_decide_args);
         case 3: return sizeof(nrrs_vertices_out);
+        case 4: return sizeof(nrrs_rrs_side);
+        case 5: return sizeof(nrrs_frame_args);
+        case 6: return sizeof(nrrs_error_args);
     }
     return 0;
 }
@@ -935,6 +941,19 @@ nrrs_status nrrs_frame_vertices(uint64_t seed, uint32_t bounce, const uint32_t* 
         !o->thp || !o->pixel || o->stride < n)
         return NRRS_ERR_ARG;
     return hip_status(launch_bounce_vertices(seed, bounce, queue_dev, n, *o, (hipStream_t)stream));
+}
+
+nrrs_status nrrs_error_blend(nrrs_ctx* ctx, uint32_t n_pixels, const float* L_frame_dev, const uint32_t* spp_dev,
+                             float blend, float* weighted_L_dev, void* stream) {
+    if (!ctx || (n_pixels && (!L_frame_dev || !spp_dev || !weighted_L_dev))) return NRRS_ERR_ARG;
+    return hip_status(launch_error_blend(n_pixels, L_frame_dev, spp_dev, blend, weighted_L_dev, (hipStream_t)stream));
+}
+
+nrrs_status nrrs_error_compute(nrrs_ctx* ctx, uint32_t n_pixels, const nrrs_error_args* a, void* stream) {
+    if (!ctx || !a || (n_pixels && (!a->ref_dev || !a->weighted_L_dev || !a->error_dev))) return NRRS_ERR_ARG;
+    if (!ctx->err_part) NRRS_TRY(dmalloc(&ctx->err_part, error_partial_count()));
+    return hip_status(launch_error_compute(n_pixels, a->ref_dev, a->weighted_L_dev, a->spp_dev, a->clamp, a->error_dev,
+                                           ctx->err_part, a->error_sum_dev, (hipStream_t)stream));
 }
 
 }  // extern "C"

@@ -98,6 +98,7 @@ struct nrrs_ctx {
     nrrs::NetState net[2];
     // RRS-stage scratch
     unsigned long long* sum_dev = nullptr;  // [1]
+    double* err_part = nullptr;             // per-block partial sums of the pixel error
     uint32_t* tile_sums = nullptr;          // [max tiles]
     uint32_t* tile_offsets = nullptr;       // [max tiles]
     float* stats_scratch = nullptr;         // [6][max_queue]
@@ -137,6 +138,10 @@ hipError_t launch_net_mfma(int precision, const EncodeArgs& a, int net, int n_la
                            hipStream_t s);
 
 hipError_t launch_zero_u64(unsigned long long* p, hipStream_t s);
+hipError_t launch_error_blend(uint32_t n, const float* Lf, const uint32_t* spp, float blend, float* wL, hipStream_t s);
+size_t error_partial_count();
+hipError_t launch_error_compute(uint32_t n, const float* ref, const float* wL, const float* spp, int clamp, float* err,
+                                double* part, float* sum, hipStream_t s);
 hipError_t launch_qsum(nrrs_ctx* ctx, const float* q, uint32_t n, hipStream_t s);
 hipError_t launch_decide(nrrs_ctx* ctx, const nrrs_decide_args& a, hipStream_t s);
 

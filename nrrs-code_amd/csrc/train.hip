@@ -211,6 +211,7 @@ __global__ __launch_bounds__(64 * P::kWaves) void k_train(TrainArgs t) {
     const op* Wf0 = reinterpret_cast<const op*>(t.w_fwd);
     const op* Wb0 = reinterpret_cast<const op*>(t.w_bwd);
     const float S = *t.loss_scale;
+    const float ebar = t.error_sum_dev ? fmaxf(*t.error_sum_dev * t.inv_pixels, 1e-4f) : t.ebar;
     constexpr int kSlots = dw_slots(NL, kWaves);
     f32x16 acc[kSlots];
 #pragma unroll
@@ -333,9 +334,9 @@ __global__ __launch_bounds__(64 * P::kWaves) void k_train(TrainArgs t) {
                         const float q = activation_sigma(y);
                         const float qe = q + 0.01f;
                         const float w = t.sample_weight[i];
-                        const float f = t.gamma[0] * (vv / qe + q * t.ebar) + t.gamma[1] * (vv / qe + q * e) +
+                        const float f = t.gamma[0] * (vv / qe + q * ebar) + t.gamma[1] * (vv / qe + q * e) +
                                         t.gamma[2] * q + t.gamma[3] * (q - qt) * (q - qt);
-                        const float fp = -(t.gamma[0] + t.gamma[1]) * vv / (qe * qe) + t.gamma[0] * t.ebar +
+                        const float fp = -(t.gamma[0] + t.gamma[1]) * vv / (qe * qe) + t.gamma[0] * ebar +
                                          t.gamma[1] * e + t.gamma[2] + 2.0f * t.gamma[3] * (q - qt);
                         atomicAdd(&loss_part[wave], w * f * t.inv_norm);
                         dyr[0] = w * fp * activation_sigma_grad(y) * t.inv_norm;

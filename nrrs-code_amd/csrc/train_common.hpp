@@ -38,6 +38,8 @@ struct TrainArgs {
     const float* sample_weight;
     const float* q_target;
     float ebar;                    // max(error_sum / pixels, 1e-4)
+    const float* error_sum_dev;    // device error sum replacing ebar's error_sum, or null
+    float inv_pixels;
     float gamma[4];
     int clamp_on;
     float clamp_max;

@@ -126,6 +126,7 @@ class RrsSide(C.Structure):
         ("sample_weight", C.c_void_p),
         ("q_target", C.c_void_p),
         ("error_sum", C.c_float),
+        ("error_sum_dev", C.c_void_p),
     ]
 
 
@@ -151,6 +152,17 @@ class FrameArgs(C.Structure):
 
 class TileShape(C.Structure):
     _fields_ = [("waves", C.c_uint32), ("m", C.c_uint32)]
+
+
+class ErrorArgs(C.Structure):
+    _fields_ = [
+        ("ref_dev", C.c_void_p),
+        ("weighted_L_dev", C.c_void_p),
+        ("spp_dev", C.c_void_p),
+        ("clamp", C.c_int32),
+        ("error_dev", C.c_void_p),
+        ("error_sum_dev", C.c_void_p),
+    ]
 
 # name -> (restype, argtypes)
 _P = C.c_void_p
@@ -186,6 +198,8 @@ _SIGS = {
     "nrrs_frame_graph_capture": (C.c_int, [_P, C.POINTER(FrameArgs), _P]),
     "nrrs_frame_graph_launch": (C.c_int, [_P, _P]),
     "nrrs_frame_vertices": (C.c_int, [C.c_uint64, C.c_uint32, _P, C.c_uint32, C.POINTER(VerticesOut), _P]),
+    "nrrs_error_blend": (C.c_int, [_P, C.c_uint32, _P, _P, C.c_float, _P, _P]),
+    "nrrs_error_compute": (C.c_int, [_P, C.c_uint32, C.POINTER(ErrorArgs), _P]),
 }
 
 _lib = None
@@ -207,7 +221,7 @@ def load(path=LIB_PATH):
         fn = getattr(lib, name)
         fn.restype = res
         fn.argtypes = args
-    for k, cls in enumerate((Config, Vertices, DecideArgs, VerticesOut)):
+    for k, cls in enumerate((Config, Vertices, DecideArgs, VerticesOut, RrsSide, FrameArgs, ErrorArgs)):
         if lib.nrrs_struct_size(k) != C.sizeof(cls):
             raise RuntimeError(f"ABI mismatch for {cls.__name__}: {lib.nrrs_struct_size(k)} != {C.sizeof(cls)}")
     _lib = lib
@@ -433,15 +447,34 @@ class Context:
 
     # ---- training
     @staticmethod
-    def side(ref_mean=None, error=None, sample_weight=None, q_target=None, error_sum=0.0):
-        """nrrs_rrs_side from device tensors (keeps no reference: the caller owns them)."""
+    def side(ref_mean=None, error=None, sample_weight=None, q_target=None, error_sum=0.0, error_sum_dev=None):
+        """nrrs_rrs_side from device tensors (keeps no reference: the caller owns them);
+        error_sum_dev, a device scalar, replaces error_sum when given."""
         sd = RrsSide()
         sd.ref_mean = None if ref_mean is None else ref_mean.data_ptr()
         sd.error = error.data_ptr()
         sd.sample_weight = sample_weight.data_ptr()
         sd.q_target = q_target.data_ptr()
         sd.error_sum = float(error_sum)
+        sd.error_sum_dev = None if error_sum_dev is None else error_sum_dev.data_ptr()
         return sd
+
+    def error_blend(self, L_frame, spp, blend, weighted_L, stream=None):
+        """weighted_L = weighted_L (1 - blend) + L_frame / spp * blend ([3][P] planes, in place)."""
+        check(self.lib.nrrs_error_blend(self.h, weighted_L.shape[1], _ptr(L_frame), _ptr(spp), blend,
+                                        _ptr(weighted_L), _stream(stream)), "error_blend")
+
+    def error_compute(self, ref, weighted_L, spp=None, clamp=True, stream=None):
+        """Per-pixel error [2][P] (row 1 scaled by sqrt(spp)) and its sum (device scalar)."""
+        import torch
+
+        n = ref.shape[1]
+        err = torch.empty(2, n, dtype=torch.float32, device=ref.device)
+        total = torch.empty(1, dtype=torch.float32, device=ref.device)
+        a = ErrorArgs(ref.data_ptr(), weighted_L.data_ptr(), None if spp is None else spp.data_ptr(), int(clamp),
+                      err.data_ptr(), total.data_ptr())
+        check(self.lib.nrrs_error_compute(self.h, n, C.byref(a), _stream(stream)), "error_compute")
+        return err, total
 
     def backward(self, net, vb, kind, film=None, stats=None, dy=None, target_L=None, side=None, loss_scale=1.0,
                  loss=None, stream=None):

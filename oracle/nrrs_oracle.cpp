@@ -702,4 +702,35 @@ void ora_adam(uint64_t n, double* p, double* m, double* v, const double* g, doub
     }
 }
 
+// nrrs_update_weighted_L (train.cu:344-363): L = L (1 - b) + (L_frame / spp) b per channel.
+void ora_error_blend(uint32_t n, const float* L_frame, const uint32_t* spp, double blend, double* weighted_L) {
+    for (uint32_t i = 0; i < n; ++i)
+        for (int c = 0; c < 3; ++c) {
+            const size_t k = (size_t)c * n + i;
+            weighted_L[k] = weighted_L[k] * (1.0 - blend) + ((double)L_frame[k] / (double)spp[i]) * blend;
+        }
+}
+
+// nrrs_calculate_error (train.cu:365-430): e = mean_c (L - ref)^2 / (ref^2 + 0.01), clamp at
+// 100, row 1 = e * sqrt(spp); the sum of row 1 as calcSum2PassAsync<false> forms it
+// (integrator.cpp:2466-2471).
+double ora_error_compute(uint32_t n, const double* ref, const double* weighted_L, const double* spp, int32_t clamp,
+                         double* err) {
+    double total = 0.0;
+    for (uint32_t i = 0; i < n; ++i) {
+        double e = 0.0;
+        for (int c = 0; c < 3; ++c) {
+            const double r = ref[(size_t)c * n + i], d = weighted_L[(size_t)c * n + i] - r;
+            e += d * d / (r * r + 1e-2);
+        }
+        e /= 3.0;
+        if (clamp) e = std::fmin(e, 100.0);
+        err[i] = e;
+        const double e2 = spp ? e * std::sqrt(spp[i]) : e;
+        err[(size_t)n + i] = e2;
+        total += e2;
+    }
+    return total;
+}
+
 }  // extern "C"

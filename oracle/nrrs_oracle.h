@@ -24,6 +24,8 @@
  *    restatement, pinned only against their own finite differences
  *    ("parity unpinned" w.r.t. the reference loss code).
  *  - Adam: hand-computed 3-step trajectory.
+ *  - Pixel error: closed forms (zero at the reference, L^2 / 0.01 on a black
+ *    reference, the clamp, sqrt(spp) scaling, blend weights 0 and 1).
  */
 #ifndef NRRS_ORACLE_H
 #define NRRS_ORACLE_H
@@ -106,6 +108,13 @@ double ora_train_grads(const nrrs_config* cfg, int32_t net, uint32_t n, const nr
 
 void ora_adam(uint64_t n, double* p, double* m, double* v, const double* g, double lr, double b1, double b2,
               double eps, double l2, uint32_t t);
+
+/* Per-pixel error pipeline (train.cu:344-430, integrator.cpp:2466-2471), planes [3][n]:
+ * running radiance blend, relative squared error (clamped at 100), row 1 scaled by sqrt(spp)
+ * (spp NULL: unscaled); returns the sum of row 1. */
+void ora_error_blend(uint32_t n, const float* L_frame, const uint32_t* spp, double blend, double* weighted_L);
+double ora_error_compute(uint32_t n, const double* ref, const double* weighted_L, const double* spp, int32_t clamp,
+                         double* err);
 
 #ifdef __cplusplus
 }

@@ -264,3 +264,28 @@ def synth_targets(n, seed):
     L = rng.lognormal(-1.0, 1.5, (3, n))
     L[:, rng.uniform(size=n) < 0.05] *= 100.0
     return L.astype(np.float32)
+
+
+def error_blend(L_frame, spp, blend, weighted_L):
+    """Running per-pixel radiance blend (in place on a float64 [3][n] array)."""
+    assert weighted_L.dtype == np.float64 and weighted_L.flags.c_contiguous
+    Lf = np.ascontiguousarray(L_frame, np.float32)
+    sp = np.ascontiguousarray(spp, np.uint32)
+    lib = load()
+    lib.ora_error_blend.argtypes = [C.c_uint32, C.c_void_p, C.c_void_p, C.c_double, C.c_void_p]
+    lib.ora_error_blend(weighted_L.shape[1], Lf.ctypes.data, sp.ctypes.data, blend, weighted_L.ctypes.data)
+
+
+def error_compute(ref, weighted_L, spp=None, clamp=True):
+    """Per-pixel error [2][n] and the sum of row 1 (FP64)."""
+    ref = np.ascontiguousarray(ref, np.float64)
+    wl = np.ascontiguousarray(weighted_L, np.float64)
+    sp = None if spp is None else np.ascontiguousarray(spp, np.float64)
+    n = ref.shape[1]
+    err = np.zeros((2, n), np.float64)
+    lib = load()
+    lib.ora_error_compute.restype = C.c_double
+    lib.ora_error_compute.argtypes = [C.c_uint32, C.c_void_p, C.c_void_p, C.c_void_p, C.c_int32, C.c_void_p]
+    total = lib.ora_error_compute(n, ref.ctypes.data, wl.ctypes.data, None if sp is None else sp.ctypes.data,
+                                  int(clamp), err.ctypes.data)
+    return err, total
