@@ -296,7 +296,8 @@ def online_loop(torch, na, np, n_frames=256, warm_frames=32):
             "latency_ms_p50": round(float(np.percentile(lat, 50)), 3),
             "latency_ms_p99": round(float(np.percentile(lat, 99)), 3),
             "train_batch": B, "statnet_only_frames": warm_frames,
-            "final_loss": [round(x, 5) for x in loss.cpu().tolist()], "status": int(bufs["status"].item())}
+            "final_loss": [round(x, 5) for x in loss.cpu().tolist()],
+            "queue_sizes_last_frame": bufs["sizes"].cpu().tolist(), "status": int(bufs["status"].item())}
 
 
 def main():
