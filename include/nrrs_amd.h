@@ -102,6 +102,7 @@ typedef struct {
     uint32_t max_queue;  /* capacity of one inference batch (vertices) */
     uint32_t max_train;  /* capacity of one training batch */
     uint64_t seed;       /* parameter initialisation seed */
+    uint32_t max_nodes;  /* capacity of the path-node buffer (0: 4 x max_queue) */
 } nrrs_config;
 
 /* SoA path vertices of one bounce (replaces NRRSInferenceWorkItem, workitem.h:9-15). */
@@ -219,7 +220,7 @@ typedef struct {
 } nrrs_rrs_side;
 
 /* One training step: StatNet (net_mask bit 0) on the NRRS_LL2 loss against target_L
- * [3][n], then RRSNet (bit 1) on the NRRS_RRS loss, its StatNet-statistics inputs taken
+ * [3][v->stride] (the vertices' row stride; entries 0..n-1 used), then RRSNet (bit 1) on the NRRS_RRS loss, its StatNet-statistics inputs taken
  * from the StatNet forward of the same batch; backward; Adam on each trained net.
  * loss_dev (optional, device, 2 floats) receives the two batch losses. */
 nrrs_status nrrs_train_step(nrrs_ctx* ctx, uint32_t net_mask, uint32_t n, const nrrs_vertices* v,
@@ -230,7 +231,8 @@ nrrs_status nrrs_train_step(nrrs_ctx* ctx, uint32_t net_mask, uint32_t n, const 
 typedef enum { NRRS_LOSS_EXTERNAL = 0, NRRS_LOSS_LL2 = 1, NRRS_LOSS_RRS = 2 } nrrs_loss_kind;
 
 /* Forward + backward of one net, accumulating dL/dparams (scaled by loss_scale) into the
- * context's gradient buffer. kind EXTERNAL: dy_dev [n_out][n] is dL/dy. LL2: target_L_dev.
+ * context's gradient buffer. kind EXTERNAL: dy_dev [n_out][n] is dL/dy. LL2: target_L_dev
+ * [3][v->stride].
  * RRS: side + stats_dev (StatNet statistics [6][n], also the RRSNet input). loss_dev: 1 float. */
 nrrs_status nrrs_backward(nrrs_ctx* ctx, int32_t net, uint32_t n, const nrrs_vertices* v,
                           const float* film_dev, const float* stats_dev, int32_t kind,
@@ -314,6 +316,68 @@ typedef struct {
     float* error_sum_dev;        /* device scalar, or NULL                         */
 } nrrs_error_args;
 nrrs_status nrrs_error_compute(nrrs_ctx* ctx, uint32_t n_pixels, const nrrs_error_args* a, void* stream);
+
+/* ---------------- training data from path nodes (workqueue.h:106-189, train.cu:432-592) ----------------
+ * The context holds a buffer of path nodes: every non-specular vertex of a training path is
+ * recorded with a link to the node of the previous vertex of its path; radiance found later
+ * along the path is added to the node and all its ancestors; at the end of the frame the
+ * nodes become training samples. Node indices are int32, -1 = none. */
+
+/* Empties the node buffer (NRRSPathNodesBuffer::reset). */
+nrrs_status nrrs_nodes_reset(nrrs_ctx* ctx, void* stream);
+
+/* Appends one node per vertex (NRRSPathNodesBuffer::recordNode, workqueue.h:142-168):
+ * vertex i -> node size + i, in vertex order. parent_dev[i]: node of the path's previous
+ * vertex (NULL: all -1); q_dev[i]: the q' used at this vertex (NULL: 1); depth: bounce.
+ * n_dev (optional, device): the vertex count, n then being its upper bound.
+ * node_out_dev[i] receives the new node, or parent_dev[i] when the buffer is full (the
+ * vertex then counts as invalid), as the reference's callers keep the previous node. */
+nrrs_status nrrs_nodes_record(nrrs_ctx* ctx, uint32_t n, const uint32_t* n_dev, const nrrs_vertices* v,
+                              const int32_t* parent_dev, const float* q_dev, uint32_t depth, int32_t* node_out_dev,
+                              void* stream);
+
+/* Adds radiance L_dev [3][L_stride] (entry i) to node node_dev[i] and every ancestor
+ * (NRRSPathNodesBuffer::recordRadiance, workqueue.h:122-134); nodes < 0 or not yet
+ * recorded are skipped. FP32 atomic adds: the sums are order-dependent in their last bits. */
+nrrs_status nrrs_nodes_add_radiance(nrrs_ctx* ctx, uint32_t n, const uint32_t* n_dev, const int32_t* node_dev,
+                                    const float* L_dev, uint32_t L_stride, void* stream);
+
+/* Node count and invalid (dropped) records, read back to the host (synchronises). */
+nrrs_status nrrs_nodes_size(nrrs_ctx* ctx, uint32_t* size, uint32_t* invalid);
+
+/* Training samples from the recorded nodes (nrrs_generate_training_data, train.cu:432-570,
+ * and nrrs_calculate_sample_weight, train.cu:572-592). Node j gives target
+ * L = max(L_j / thp_j per channel where thp_j > 1e-5 else 0, 0); it is dropped when L, its
+ * position or its direction has a NaN, or when ignore_zero_L and mean(L) == 0. Kept nodes
+ * become rows in node order (the reference's atomic push order is not reproducible).
+ * With mark_duplicates, a node whose parent has an earlier child (lower node index) gets
+ * q_target -1 and is not counted in pixel_samples; other rows get q_target = the node's q.
+ * copy_rrs_node multiplies the row's thp by the node's q. ref_mean = mean RGB of the film
+ * pixel; error / error_avg = rows 0 / 1 of error_dev at the pixel mapped to the error image
+ * (offsetFrame2Scaled, myFilm.h:6-15); sample_weight = 1 / pixel_samples[pixel] (0 if 0)
+ * after all rows are counted. pixel_samples accumulates (zero it per frame). Rows beyond
+ * capacity are dropped; n_out_dev receives the row count. */
+typedef struct {
+    const float* film_dev;     /* [3][n_pixels] */
+    const float* error_dev;    /* [2][n_err] per-pixel error, or NULL (errors 0) */
+    uint32_t n_err;            /* pixels of the error image */
+    uint32_t error_scale;      /* frame / error image, per axis (>= 1) */
+    uint32_t frame_width;
+    int32_t ignore_zero_L;
+    int32_t copy_rrs_node;
+    int32_t mark_duplicates;
+    uint32_t capacity;         /* rows of the outputs */
+    nrrs_vertices_out out;     /* training vertices (stride >= capacity) */
+    float* target_L;           /* [3][capacity] */
+    float* ref_mean;           /* [capacity] */
+    float* error;
+    float* error_avg;
+    float* q_target;
+    float* sample_weight;
+    uint32_t* pixel_samples;   /* [n_pixels] */
+    uint32_t* n_out_dev;       /* device scalar */
+} nrrs_training_args;
+nrrs_status nrrs_generate_training_data(nrrs_ctx* ctx, const nrrs_training_args* a, void* stream);
 
 #ifdef __cplusplus
 }

@@ -270,7 +270,7 @@ nrrs_status run_backward(nrrs_ctx* c, int net, uint32_t n, const nrrs_vertices& 
         t.q_target = side->q_target;
         t.ebar = std::fmax(side->error_sum / (float)cfg.n_pixels, 1e-4f);
         t.error_sum_dev = side->error_sum_dev;
-        t.inv_pixels = 1.0f / (float)cfg.n_pixels;
+        t.n_pixels_f = (float)cfg.n_pixels;
     }
     for (int k = 0; k < 4; ++k) t.gamma[k] = cfg.gamma[k];
     t.clamp_on = cfg.loss_clamp_on;
@@ -313,6 +313,39 @@ nrrs_status run_backward(nrrs_ctx* c, int net, uint32_t n, const nrrs_vertices& 
         NRRS_HIP(launch_apply_bins(ap, ns.n_seg, s));
     }
     return hip_status(launch_reduce_partials(ns.dw_partial, blocks, (uint32_t)ns.n_mlp, ns.grad + ns.n_grid, s));
+}
+
+// Path-node buffer, allocated on first use: 4 words x 3 channels x 7 planes + 4 scalars.
+nrrs_status ensure_nodes(nrrs_ctx* c) {
+    NodeBuf& nb = c->nodes;
+    if (nb.cap) return NRRS_OK;
+    const uint64_t cap64 = c->cfg.max_nodes ? c->cfg.max_nodes : 4ull * c->cfg.max_queue;
+    if (!cap64 || cap64 > 0x7FFFFFFFull) return NRRS_ERR_SHAPE;
+    const size_t cap = (size_t)cap64, n_tiles = cap / kNodeTile + 1;
+    const size_t words = cap * (7 * 3 + 2) + cap * 4 + 2 + n_tiles;
+    uint32_t* m = nullptr;
+    NRRS_TRY(dmalloc(&m, words));
+    c->nodes_mem = m;
+    float* f = reinterpret_cast<float*>(m);
+    nb.pos = f; f += 3 * cap;
+    nb.dir = f; f += 3 * cap;
+    nb.normal = f; f += 3 * cap;
+    nb.ad = f; f += 3 * cap;
+    nb.as = f; f += 3 * cap;
+    nb.thp = f; f += 3 * cap;
+    nb.L = f; f += 3 * cap;
+    nb.rough = f; f += cap;
+    nb.q = f; f += cap;
+    uint32_t* u = reinterpret_cast<uint32_t*>(f);
+    nb.parent = reinterpret_cast<int32_t*>(u); u += cap;
+    nb.pixel = u; u += cap;
+    nb.depth = u; u += cap;
+    nb.first = u; u += cap;
+    nb.cnt = u; u += 2;
+    nb.tiles = u;
+    NRRS_HIP(hipMemset(nb.cnt, 0, 8));
+    nb.cap = (uint32_t)cap;
+    return NRRS_OK;
 }
 
 // Folds the dense-level replicas into the table gradient (and clears them).
@@ -420,6 +453,7 @@ bool side_ok(const nrrs_rrs_side* side) {
 
 void free_ctx(nrrs_ctx* c) {
     hipFree(c->err_part);
+    hipFree(c->nodes_mem);
     for (auto& ns : c->net) {
         hipFree(ns.params);
         hipFree(ns.grid_lp);
@@ -483,6 +517,7 @@ size_t nrrs_struct_size(int32_t which) {
         case 4: return sizeof(nrrs_rrs_side);
         case 5: return sizeof(nrrs_frame_args);
         case 6: return sizeof(nrrs_error_args);
+        case 7: return sizeof(nrrs_training_args);
     }
     return 0;
 }
@@ -954,6 +989,51 @@ nrrs_status nrrs_error_compute(nrrs_ctx* ctx, uint32_t n_pixels, const nrrs_erro
     if (!ctx->err_part) NRRS_TRY(dmalloc(&ctx->err_part, error_partial_count()));
     return hip_status(launch_error_compute(n_pixels, a->ref_dev, a->weighted_L_dev, a->spp_dev, a->clamp, a->error_dev,
                                            ctx->err_part, a->error_sum_dev, (hipStream_t)stream));
+}
+
+nrrs_status nrrs_nodes_reset(nrrs_ctx* ctx, void* stream) {
+    if (!ctx) return NRRS_ERR_ARG;
+    NRRS_TRY(ensure_nodes(ctx));
+    return hip_status(launch_nodes_reset(ctx->nodes, (hipStream_t)stream));
+}
+
+nrrs_status nrrs_nodes_record(nrrs_ctx* ctx, uint32_t n, const uint32_t* n_dev, const nrrs_vertices* v,
+                              const int32_t* parent_dev, const float* q_dev, uint32_t depth, int32_t* node_out_dev,
+                              void* stream) {
+    if (!ctx || !vertices_ok(v, n) || (n && !node_out_dev)) return NRRS_ERR_ARG;
+    NRRS_TRY(ensure_nodes(ctx));
+    return hip_status(
+        launch_nodes_record(ctx->nodes, n, n_dev, *v, parent_dev, q_dev, depth, node_out_dev, (hipStream_t)stream));
+}
+
+nrrs_status nrrs_nodes_add_radiance(nrrs_ctx* ctx, uint32_t n, const uint32_t* n_dev, const int32_t* node_dev,
+                                    const float* L_dev, uint32_t L_stride, void* stream) {
+    if (!ctx || (n && (!node_dev || !L_dev || L_stride < n))) return NRRS_ERR_ARG;
+    NRRS_TRY(ensure_nodes(ctx));
+    return hip_status(launch_nodes_radiance(ctx->nodes, n, n_dev, node_dev, L_dev, L_stride, (hipStream_t)stream));
+}
+
+nrrs_status nrrs_nodes_size(nrrs_ctx* ctx, uint32_t* size, uint32_t* invalid) {
+    if (!ctx) return NRRS_ERR_ARG;
+    NRRS_TRY(ensure_nodes(ctx));
+    uint32_t h[2];
+    NRRS_HIP(hipDeviceSynchronize());
+    NRRS_HIP(hipMemcpy(h, ctx->nodes.cnt, 8, hipMemcpyDeviceToHost));
+    if (size) *size = h[0];
+    if (invalid) *invalid = h[1];
+    return NRRS_OK;
+}
+
+nrrs_status nrrs_generate_training_data(nrrs_ctx* ctx, const nrrs_training_args* a, void* stream) {
+    if (!ctx || !a) return NRRS_ERR_ARG;
+    const nrrs_vertices_out& o = a->out;
+    if (!a->film_dev || !a->target_L || !a->ref_mean || !a->error || !a->error_avg || !a->q_target ||
+        !a->sample_weight || !a->pixel_samples || !a->n_out_dev || !o.pos || !o.dir || !o.normal || !o.rough ||
+        !o.albedo_d || !o.albedo_s || !o.thp || !o.pixel || o.stride < a->capacity)
+        return NRRS_ERR_ARG;
+    if (a->error_dev && (!a->error_scale || !a->frame_width || a->frame_width % a->error_scale)) return NRRS_ERR_ARG;
+    NRRS_TRY(ensure_nodes(ctx));
+    return hip_status(launch_training_data(ctx->nodes, *a, ctx->cfg.n_pixels, (hipStream_t)stream));
 }
 
 }  // extern "C"

@@ -16,36 +16,6 @@ constexpr int kScanThreads = 1024;
 
 __global__ void k_zero_u64(unsigned long long* p) { *p = 0ull; }
 
-// Inclusive prefix sum over the 64 lanes of a wavefront.
-__device__ __forceinline__ uint32_t wave_inclusive_scan(uint32_t v) {
-    const int lane = threadIdx.x & 63;
-#pragma unroll
-    for (int off = 1; off < 64; off <<= 1) {
-        uint32_t y = __shfl_up(v, off, 64);
-        if (lane >= off) v += y;
-    }
-    return v;
-}
-
-// Exclusive prefix over the block; *total receives the block sum.
-template <int BLOCK>
-__device__ __forceinline__ uint32_t block_exclusive_scan(uint32_t v, uint32_t* total) {
-    __shared__ uint32_t wsum[BLOCK / 64];
-    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    uint32_t inc = wave_inclusive_scan(v);
-    if (lane == 63) wsum[wave] = inc;
-    __syncthreads();
-    uint32_t base = 0, tot = 0;
-#pragma unroll
-    for (int w = 0; w < BLOCK / 64; ++w) {
-        uint32_t x = wsum[w];
-        base += (w < wave) ? x : 0u;
-        tot += x;
-    }
-    *total = tot;
-    return base + inc - v;
-}
-
 struct SplitArgs {
     const float* q;
     uint32_t n;

@@ -59,6 +59,27 @@ struct EpiArgs {
 
 enum Epilogue { EPI_RAW = 0, EPI_RRS = 1, EPI_ADRRS = 2 };
 
+// Path-node buffer (SoA, capacity cap) and the scratch of the training-data pass.
+struct NodeBuf {
+    float* pos = nullptr;       // [3][cap]
+    float* dir = nullptr;       // [3][cap]
+    float* normal = nullptr;    // [3][cap]
+    float* rough = nullptr;     // [cap]
+    float* ad = nullptr;        // [3][cap]
+    float* as = nullptr;        // [3][cap]
+    float* thp = nullptr;       // [3][cap]
+    float* q = nullptr;         // [cap]
+    float* L = nullptr;         // [3][cap] radiance found after the node
+    int32_t* parent = nullptr;  // [cap]
+    uint32_t* pixel = nullptr;  // [cap]
+    uint32_t* depth = nullptr;  // [cap]
+    uint32_t* cnt = nullptr;    // [0] nodes, [1] invalid records
+    uint32_t* first = nullptr;  // [cap] lowest child of each node
+    uint32_t* tiles = nullptr;  // [cap / kNodeTile + 1] kept rows per tile, then row offsets
+    uint32_t cap = 0;
+};
+constexpr uint32_t kNodeTile = 1024;
+
 struct NetState {
     int n_layers = 0, n_out = 0;
     uint64_t n_grid = 0, n_mlp = 0;  // FP32 parameter counts
@@ -99,6 +120,8 @@ struct nrrs_ctx {
     // RRS-stage scratch
     unsigned long long* sum_dev = nullptr;  // [1]
     double* err_part = nullptr;             // per-block partial sums of the pixel error
+    nrrs::NodeBuf nodes;                    // path nodes of the training paths
+    void* nodes_mem = nullptr;
     uint32_t* tile_sums = nullptr;          // [max tiles]
     uint32_t* tile_offsets = nullptr;       // [max tiles]
     float* stats_scratch = nullptr;         // [6][max_queue]
@@ -138,6 +161,12 @@ hipError_t launch_net_mfma(int precision, const EncodeArgs& a, int net, int n_la
                            hipStream_t s);
 
 hipError_t launch_zero_u64(unsigned long long* p, hipStream_t s);
+hipError_t launch_nodes_reset(const NodeBuf& nb, hipStream_t s);
+hipError_t launch_nodes_record(const NodeBuf& nb, uint32_t n, const uint32_t* n_dev, const nrrs_vertices& v,
+                               const int32_t* parent, const float* q, uint32_t depth, int32_t* out, hipStream_t s);
+hipError_t launch_nodes_radiance(const NodeBuf& nb, uint32_t n, const uint32_t* n_dev, const int32_t* node,
+                                 const float* L, uint32_t L_stride, hipStream_t s);
+hipError_t launch_training_data(const NodeBuf& nb, const nrrs_training_args& a, uint32_t n_pixels, hipStream_t s);
 hipError_t launch_error_blend(uint32_t n, const float* Lf, const uint32_t* spp, float blend, float* wL, hipStream_t s);
 size_t error_partial_count();
 hipError_t launch_error_compute(uint32_t n, const float* ref, const float* wL, const float* spp, int clamp, float* err,

@@ -211,7 +211,7 @@ __global__ __launch_bounds__(64 * P::kWaves) void k_train(TrainArgs t) {
     const op* Wf0 = reinterpret_cast<const op*>(t.w_fwd);
     const op* Wb0 = reinterpret_cast<const op*>(t.w_bwd);
     const float S = *t.loss_scale;
-    const float ebar = t.error_sum_dev ? fmaxf(*t.error_sum_dev * t.inv_pixels, 1e-4f) : t.ebar;
+    const float ebar = t.error_sum_dev ? fmaxf(*t.error_sum_dev / t.n_pixels_f, 1e-4f) : t.ebar;
     constexpr int kSlots = dw_slots(NL, kWaves);
     f32x16 acc[kSlots];
 #pragma unroll
@@ -299,7 +299,7 @@ __global__ __launch_bounds__(64 * P::kWaves) void k_train(TrainArgs t) {
                 for (int r = 0; r < 4; ++r) {
                     const int c = 4 * h + r;
                     if (c < NOUT && c < active) {
-                        float tg = fmaxf(t.target_L[(size_t)(c % 3) * t.n + i], 0.0f);
+                        float tg = fmaxf(t.target_L[(size_t)(c % 3) * t.a.v.stride + i], 0.0f);
                         if (t.clamp_on) tg = fminf(tg, t.clamp_max);
                         if (c >= 3) tg = tg * tg;
                         const float y = Y[r];

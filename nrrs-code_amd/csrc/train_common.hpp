@@ -39,7 +39,7 @@ struct TrainArgs {
     const float* q_target;
     float ebar;                    // max(error_sum / pixels, 1e-4)
     const float* error_sum_dev;    // device error sum replacing ebar's error_sum, or null
-    float inv_pixels;
+    float n_pixels_f;
     float gamma[4];
     int clamp_on;
     float clamp_max;

@@ -67,6 +67,7 @@ class Config(C.Structure):
         ("max_queue", C.c_uint32),
         ("max_train", C.c_uint32),
         ("seed", C.c_uint64),
+        ("max_nodes", C.c_uint32),
     ]
 
 
@@ -154,6 +155,29 @@ class TileShape(C.Structure):
     _fields_ = [("waves", C.c_uint32), ("m", C.c_uint32)]
 
 
+class TrainingArgs(C.Structure):
+    _fields_ = [
+        ("film_dev", C.c_void_p),
+        ("error_dev", C.c_void_p),
+        ("n_err", C.c_uint32),
+        ("error_scale", C.c_uint32),
+        ("frame_width", C.c_uint32),
+        ("ignore_zero_L", C.c_int32),
+        ("copy_rrs_node", C.c_int32),
+        ("mark_duplicates", C.c_int32),
+        ("capacity", C.c_uint32),
+        ("out", VerticesOut),
+        ("target_L", C.c_void_p),
+        ("ref_mean", C.c_void_p),
+        ("error", C.c_void_p),
+        ("error_avg", C.c_void_p),
+        ("q_target", C.c_void_p),
+        ("sample_weight", C.c_void_p),
+        ("pixel_samples", C.c_void_p),
+        ("n_out_dev", C.c_void_p),
+    ]
+
+
 class ErrorArgs(C.Structure):
     _fields_ = [
         ("ref_dev", C.c_void_p),
@@ -200,6 +224,11 @@ _SIGS = {
     "nrrs_frame_vertices": (C.c_int, [C.c_uint64, C.c_uint32, _P, C.c_uint32, C.POINTER(VerticesOut), _P]),
     "nrrs_error_blend": (C.c_int, [_P, C.c_uint32, _P, _P, C.c_float, _P, _P]),
     "nrrs_error_compute": (C.c_int, [_P, C.c_uint32, C.POINTER(ErrorArgs), _P]),
+    "nrrs_nodes_reset": (C.c_int, [_P, _P]),
+    "nrrs_nodes_record": (C.c_int, [_P, C.c_uint32, _P, C.POINTER(Vertices), _P, _P, C.c_uint32, _P, _P]),
+    "nrrs_nodes_add_radiance": (C.c_int, [_P, C.c_uint32, _P, _P, _P, C.c_uint32, _P]),
+    "nrrs_nodes_size": (C.c_int, [_P, C.POINTER(C.c_uint32), C.POINTER(C.c_uint32)]),
+    "nrrs_generate_training_data": (C.c_int, [_P, C.POINTER(TrainingArgs), _P]),
 }
 
 _lib = None
@@ -221,7 +250,7 @@ def load(path=LIB_PATH):
         fn = getattr(lib, name)
         fn.restype = res
         fn.argtypes = args
-    for k, cls in enumerate((Config, Vertices, DecideArgs, VerticesOut, RrsSide, FrameArgs, ErrorArgs)):
+    for k, cls in enumerate((Config, Vertices, DecideArgs, VerticesOut, RrsSide, FrameArgs, ErrorArgs, TrainingArgs)):
         if lib.nrrs_struct_size(k) != C.sizeof(cls):
             raise RuntimeError(f"ABI mismatch for {cls.__name__}: {lib.nrrs_struct_size(k)} != {C.sizeof(cls)}")
     _lib = lib
@@ -458,6 +487,62 @@ class Context:
         sd.error_sum = float(error_sum)
         sd.error_sum_dev = None if error_sum_dev is None else error_sum_dev.data_ptr()
         return sd
+
+    # ---- path nodes and training samples
+    def nodes_reset(self, stream=None):
+        check(self.lib.nrrs_nodes_reset(self.h, _stream(stream)), "nodes_reset")
+
+    def nodes_record(self, vb, parent=None, q=None, depth=0, n_dev=None, stream=None):
+        """Appends one node per vertex; returns the int32 node (or kept parent) per vertex."""
+        import torch
+
+        out = torch.empty(max(vb.n, 1), dtype=torch.int32, device=vb.pos.device)
+        v = vb.c_struct()
+        check(self.lib.nrrs_nodes_record(self.h, vb.n, _ptr(n_dev), C.byref(v), _ptr(parent), _ptr(q), depth,
+                                         _ptr(out), _stream(stream)), "nodes_record")
+        return out[: vb.n]
+
+    def nodes_add_radiance(self, node, L, stream=None):
+        """Adds L [3][n] to node[i] and its ancestors."""
+        n = node.shape[0]
+        check(self.lib.nrrs_nodes_add_radiance(self.h, n, None, _ptr(node), _ptr(L), L.shape[1], _stream(stream)),
+              "nodes_add_radiance")
+
+    def nodes_size(self):
+        size, inval = C.c_uint32(), C.c_uint32()
+        check(self.lib.nrrs_nodes_size(self.h, C.byref(size), C.byref(inval)), "nodes_size")
+        return size.value, inval.value
+
+    def generate_training_data(self, film, capacity, error=None, error_scale=1, frame_width=None,
+                               ignore_zero_L=False, copy_rrs_node=False, mark_duplicates=True, pixel_samples=None,
+                               stream=None):
+        """Training samples from the recorded nodes; returns (VertexBatch, dict of device arrays)."""
+        import torch
+
+        dev = film.device
+        vb = VertexBatch(capacity, dev)
+        f = dict(dtype=torch.float32, device=dev)
+        P = film.shape[1]
+        o = dict(target_L=torch.zeros(3, capacity, **f), ref_mean=torch.zeros(capacity, **f),
+                 error=torch.zeros(capacity, **f), error_avg=torch.zeros(capacity, **f),
+                 q_target=torch.zeros(capacity, **f), sample_weight=torch.zeros(capacity, **f),
+                 pixel_samples=pixel_samples if pixel_samples is not None else
+                 torch.zeros(P, dtype=torch.int32, device=dev),
+                 n_out=torch.zeros(1, dtype=torch.int32, device=dev))
+        a = TrainingArgs()
+        a.film_dev = film.data_ptr()
+        a.error_dev = None if error is None else error.data_ptr()
+        a.n_err = 0 if error is None else error.shape[1]
+        a.error_scale = error_scale
+        a.frame_width = frame_width or 1
+        a.ignore_zero_L, a.copy_rrs_node, a.mark_duplicates = int(ignore_zero_L), int(copy_rrs_node), int(mark_duplicates)
+        a.capacity = capacity
+        a.out = vb.c_out()
+        for k in ("target_L", "ref_mean", "error", "error_avg", "q_target", "sample_weight", "pixel_samples"):
+            setattr(a, k, o[k].data_ptr())
+        a.n_out_dev = o["n_out"].data_ptr()
+        check(self.lib.nrrs_generate_training_data(self.h, C.byref(a), _stream(stream)), "generate_training_data")
+        return vb, o
 
     def error_blend(self, L_frame, spp, blend, weighted_L, stream=None):
         """weighted_L = weighted_L (1 - blend) + L_frame / spp * blend ([3][P] planes, in place)."""

@@ -702,6 +702,121 @@ void ora_adam(uint64_t n, double* p, double* m, double* v, const double* g, doub
     }
 }
 
+// NRRSPathNodesBuffer::recordNode (workqueue.h:142-168), one vertex after the other.
+void ora_nodes_record(ora_nodes* nb, uint32_t n, const nrrs_vertices* v, const int32_t* parent, const float* q,
+                      uint32_t depth, int32_t* out) {
+    const uint32_t C = nb->cap, S = v->stride;
+    for (uint32_t i = 0; i < n; ++i) {
+        const int32_t par = parent ? parent[i] : -1;
+        if (nb->size >= C) {
+            nb->invalid += 1;
+            out[i] = par;
+            continue;
+        }
+        const uint32_t j = nb->size++;
+        for (int c = 0; c < 3; ++c) {
+            nb->pos[c * C + j] = v->pos[c * S + i];
+            nb->dir[c * C + j] = v->dir[c * S + i];
+            nb->normal[c * C + j] = v->normal[c * S + i];
+            nb->ad[c * C + j] = v->albedo_d[c * S + i];
+            nb->as[c * C + j] = v->albedo_s[c * S + i];
+            nb->thp[c * C + j] = v->thp[c * S + i];
+            nb->L[c * C + j] = 0.0;
+        }
+        nb->rough[j] = v->rough[i];
+        nb->pixel[j] = v->pixel[i];
+        nb->parent[j] = par;
+        nb->q[j] = q ? q[i] : 1.0f;
+        nb->depth[j] = depth;
+        out[i] = (int32_t)j;
+    }
+}
+
+// NRRSPathNodesBuffer::recordRadiance (workqueue.h:122-134): the node and every ancestor.
+void ora_nodes_radiance(ora_nodes* nb, uint32_t n, const int32_t* node, const float* L, uint32_t Ls) {
+    const uint32_t C = nb->cap;
+    for (uint32_t i = 0; i < n; ++i) {
+        int32_t j = node[i];
+        if (j < 0 || (uint32_t)j >= nb->size) continue;
+        while (j != -1) {
+            for (int c = 0; c < 3; ++c) nb->L[c * C + j] += (double)L[c * Ls + i];
+            j = nb->parent[j];
+        }
+    }
+}
+
+// nrrs_generate_training_data (train.cu:432-570) with rows in node order, the duplicate
+// rule "the sibling with the lowest node index keeps its q target", then
+// nrrs_calculate_sample_weight (train.cu:572-592).
+uint32_t ora_training_data(const ora_nodes* nb, const float* film, uint32_t n_pixels, const float* err, uint32_t n_err,
+                           uint32_t scale, uint32_t width, int32_t ignore_zero_L, int32_t copy_rrs,
+                           int32_t mark_dup, uint32_t capacity, const nrrs_vertices_out* out, double* target_L,
+                           double* ref_mean, float* error, float* error_avg, float* q_target, double* sample_weight,
+                           uint32_t* pixel_samples) {
+    const uint32_t C = nb->cap, S = out->stride, size = nb->size;
+    std::vector<uint32_t> first(size, 0xFFFFFFFFu);
+    for (uint32_t j = 0; j < size; ++j) {
+        const int32_t p = nb->parent[j];
+        if (p >= 0 && (uint32_t)p < size && j < first[p]) first[p] = j;
+    }
+    uint32_t rows = 0;
+    for (uint32_t j = 0; j < size; ++j) {
+        double L[3];
+        bool nan = false;
+        for (int c = 0; c < 3; ++c) {
+            const double th = nb->thp[c * C + j];
+            double l = th > 1e-5 ? nb->L[c * C + j] / th : 0.0;
+            l = l < 0.0 ? 0.0 : l;
+            L[c] = l;
+            nan = nan || std::isnan(l) || std::isnan(nb->pos[c * C + j]) || std::isnan(nb->dir[c * C + j]);
+        }
+        if (nan) continue;
+        if (ignore_zero_L && !((L[0] + L[1] + L[2]) / 3.0 > 0.0)) continue;
+        if (rows >= capacity) {
+            ++rows;
+            continue;
+        }
+        const uint32_t r = rows++;
+        const float q = nb->q[j];
+        for (int c = 0; c < 3; ++c) {
+            out->pos[c * S + r] = nb->pos[c * C + j];
+            out->dir[c * S + r] = nb->dir[c * C + j];
+            out->normal[c * S + r] = nb->normal[c * C + j];
+            out->albedo_d[c * S + r] = nb->ad[c * C + j];
+            out->albedo_s[c * S + r] = nb->as[c * C + j];
+            out->thp[c * S + r] = copy_rrs ? nb->thp[c * C + j] * q : nb->thp[c * C + j];
+            target_L[(size_t)c * capacity + r] = L[c];
+        }
+        out->rough[r] = nb->rough[j];
+        const uint32_t px = nb->pixel[j];
+        out->pixel[r] = px;
+        ref_mean[r] = px < n_pixels
+                          ? ((double)film[px] + (double)film[n_pixels + px] + (double)film[2 * n_pixels + px]) / 3.0
+                          : 0.0;
+        float e0 = 0.0f, e1 = 0.0f;
+        if (err) {
+            const uint32_t off = ((px % width) / scale) + ((px / width) / scale) * (width / scale);
+            if (off < n_err) {
+                e0 = err[off];
+                e1 = err[(size_t)n_err + off];
+            }
+        }
+        error[r] = e0;
+        error_avg[r] = e1;
+        const int32_t p = nb->parent[j];
+        const bool dup = mark_dup && p >= 0 && (uint32_t)p < size && first[p] != j;
+        q_target[r] = dup ? -1.0f : q;
+        if (!dup && px < n_pixels) pixel_samples[px] += 1;
+    }
+    const uint32_t n_out = rows < capacity ? rows : capacity;
+    for (uint32_t r = 0; r < n_out; ++r) {
+        const uint32_t px = out->pixel[r];
+        const uint32_t k = px < n_pixels ? pixel_samples[px] : 0u;
+        sample_weight[r] = k ? 1.0 / (double)k : 0.0;
+    }
+    return n_out;
+}
+
 // nrrs_update_weighted_L (train.cu:344-363): L = L (1 - b) + (L_frame / spp) b per channel.
 void ora_error_blend(uint32_t n, const float* L_frame, const uint32_t* spp, double blend, double* weighted_L) {
     for (uint32_t i = 0; i < n; ++i)

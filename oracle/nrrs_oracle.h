@@ -24,6 +24,9 @@
  *    restatement, pinned only against their own finite differences
  *    ("parity unpinned" w.r.t. the reference loss code).
  *  - Adam: hand-computed 3-step trajectory.
+ *  - Path nodes and training rows: a hand-built path tree with known sums
+ *    (radiance reaches every ancestor once), division by throughput, the
+ *    duplicate rule and the sample weights counted by hand.
  *  - Pixel error: closed forms (zero at the reference, L^2 / 0.01 on a black
  *    reference, the clamp, sqrt(spp) scaling, blend weights 0 and 1).
  */
@@ -108,6 +111,31 @@ double ora_train_grads(const nrrs_config* cfg, int32_t net, uint32_t n, const nr
 
 void ora_adam(uint64_t n, double* p, double* m, double* v, const double* g, double lr, double b1, double b2,
               double eps, double l2, uint32_t t);
+
+/* Path-node buffer (workqueue.h:106-189), host SoA planes of capacity cap; L in FP64. */
+typedef struct {
+    float *pos, *dir, *normal, *rough, *ad, *as, *thp, *q;
+    double* L;
+    int32_t* parent;
+    uint32_t *pixel, *depth;
+    uint32_t cap, size, invalid;
+} ora_nodes;
+
+/* recordNode for vertices 0..n-1 in order (node = size + i, or the parent when full). */
+void ora_nodes_record(ora_nodes* nb, uint32_t n, const nrrs_vertices* v, const int32_t* parent, const float* q,
+                      uint32_t depth, int32_t* out);
+/* recordRadiance: L [3][Ls] of entry i added to node[i] and each ancestor. */
+void ora_nodes_radiance(ora_nodes* nb, uint32_t n, const int32_t* node, const float* L, uint32_t Ls);
+
+/* Training rows (nrrs_generate_training_data + nrrs_calculate_sample_weight,
+ * train.cu:432-592) in node order; returns the row count. Outputs of capacity rows:
+ * out (vertex planes), target_L [3][capacity] FP64, ref_mean FP64, error / error_avg,
+ * q_target, sample_weight FP64; pixel_samples [n_pixels] accumulates. */
+uint32_t ora_training_data(const ora_nodes* nb, const float* film, uint32_t n_pixels, const float* err, uint32_t n_err,
+                           uint32_t scale, uint32_t width, int32_t ignore_zero_L, int32_t copy_rrs,
+                           int32_t mark_dup, uint32_t capacity, const nrrs_vertices_out* out, double* target_L,
+                           double* ref_mean, float* error, float* error_avg, float* q_target, double* sample_weight,
+                           uint32_t* pixel_samples);
 
 /* Per-pixel error pipeline (train.cu:344-430, integrator.cpp:2466-2471), planes [3][n]:
  * running radiance blend, relative squared error (clamped at 100), row 1 scaled by sqrt(spp)

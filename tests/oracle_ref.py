@@ -289,3 +289,79 @@ def error_compute(ref, weighted_L, spp=None, clamp=True):
     total = lib.ora_error_compute(n, ref.ctypes.data, wl.ctypes.data, None if sp is None else sp.ctypes.data,
                                   int(clamp), err.ctypes.data)
     return err, total
+
+
+class _OraNodes(C.Structure):
+    _fields_ = [(k, C.c_void_p) for k in ("pos", "dir", "normal", "rough", "ad", "as_", "thp", "q", "L", "parent",
+                                          "pixel", "depth")] + [("cap", C.c_uint32), ("size", C.c_uint32),
+                                                                ("invalid", C.c_uint32)]
+
+
+class Nodes:
+    """Host path-node buffer of the oracle (numpy planes of capacity cap)."""
+
+    def __init__(self, cap):
+        self.cap = cap
+        f3 = lambda: np.zeros((3, cap), np.float32)  # noqa: E731
+        self.a = dict(pos=f3(), dir=f3(), normal=f3(), ad=f3(), as_=f3(), thp=f3(), rough=np.zeros(cap, np.float32),
+                      q=np.zeros(cap, np.float32), L=np.zeros((3, cap), np.float64),
+                      parent=np.zeros(cap, np.int32), pixel=np.zeros(cap, np.uint32), depth=np.zeros(cap, np.uint32))
+        self.s = _OraNodes()
+        for k, v in self.a.items():
+            setattr(self.s, k, v.ctypes.data)
+        self.s.cap = cap
+
+    @property
+    def size(self):
+        return self.s.size
+
+    @property
+    def invalid(self):
+        return self.s.invalid
+
+    def record(self, hv, parent=None, q=None, depth=0):
+        lib = load()
+        lib.ora_nodes_record.argtypes = [C.POINTER(_OraNodes), C.c_uint32, C.POINTER(na.Vertices), C.c_void_p,
+                                         C.c_void_p, C.c_uint32, C.c_void_p]
+        par = None if parent is None else np.ascontiguousarray(parent, np.int32)
+        qq = None if q is None else np.ascontiguousarray(q, np.float32)
+        out = np.zeros(hv.n, np.int32)
+        lib.ora_nodes_record(C.byref(self.s), hv.n, C.byref(hv.v), _p(par), _p(qq), depth, out.ctypes.data)
+        return out
+
+    def add_radiance(self, node, L):
+        lib = load()
+        lib.ora_nodes_radiance.argtypes = [C.POINTER(_OraNodes), C.c_uint32, C.c_void_p, C.c_void_p, C.c_uint32]
+        node = np.ascontiguousarray(node, np.int32)
+        L = np.ascontiguousarray(L, np.float32)
+        lib.ora_nodes_radiance(C.byref(self.s), node.size, node.ctypes.data, L.ctypes.data, L.shape[1])
+
+    def training_data(self, film, capacity, err=None, scale=1, width=1, ignore_zero_L=False, copy_rrs=False,
+                      mark_dup=True, pixel_samples=None):
+        lib = load()
+        lib.ora_training_data.restype = C.c_uint32
+        lib.ora_training_data.argtypes = [C.POINTER(_OraNodes), C.c_void_p, C.c_uint32, C.c_void_p, C.c_uint32,
+                                          C.c_uint32, C.c_uint32, C.c_int32, C.c_int32, C.c_int32, C.c_uint32,
+                                          C.POINTER(na.VerticesOut)] + [C.c_void_p] * 7
+        film = np.ascontiguousarray(film, np.float32)
+        P = film.shape[1]
+        e = None if err is None else np.ascontiguousarray(err, np.float32)
+        o = {k: np.zeros((3, capacity), np.float32) for k in na.VertexBatch.FIELDS3}
+        o["rough"] = np.zeros(capacity, np.float32)
+        o["pixel"] = np.zeros(capacity, np.uint32)
+        vo = na.VerticesOut()
+        for k in na.VertexBatch.FIELDS3 + ("rough", "pixel"):
+            setattr(vo, k, o[k].ctypes.data)
+        vo.stride = capacity
+        r = dict(target_L=np.zeros((3, capacity)), ref_mean=np.zeros(capacity), error=np.zeros(capacity, np.float32),
+                 error_avg=np.zeros(capacity, np.float32), q_target=np.zeros(capacity, np.float32),
+                 sample_weight=np.zeros(capacity),
+                 pixel_samples=np.zeros(P, np.uint32) if pixel_samples is None else
+                 np.ascontiguousarray(pixel_samples, np.uint32))
+        n = lib.ora_training_data(C.byref(self.s), film.ctypes.data, P, _p(e), 0 if e is None else e.shape[1], scale,
+                                  width, int(ignore_zero_L), int(copy_rrs), int(mark_dup), capacity, C.byref(vo),
+                                  *(r[k].ctypes.data for k in ("target_L", "ref_mean", "error", "error_avg",
+                                                               "q_target", "sample_weight", "pixel_samples")))
+        r["n"] = n
+        r["v"] = o
+        return r

@@ -66,4 +66,5 @@ def test_rrs_loss_reads_device_error_sum(torch_cuda):
         na.check(ctx.backward(na.RRSNET, vb, na.LOSS_RRS, film=film, stats=stats, side=side, loss=loss), "bwd")
         losses.append(float(loss.item()))
         ctx.grads_zero(na.RRSNET)
-    assert losses[0] == losses[1] and losses[0] != 0.0
+    # same ebar; the batch loss is summed across workgroups with atomics (order-dependent bits)
+    assert losses[0] != 0.0 and abs(losses[0] - losses[1]) <= 1e-6 * abs(losses[1])
