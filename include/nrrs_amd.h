@@ -137,6 +137,9 @@ typedef struct {
     uint32_t capacity;
     uint32_t* n_survivors_dev;    /* device: number of survivors (incl. specular)         */
     uint32_t* overflow_dev;       /* device: survivors dropped for lack of capacity, or NULL */
+    float* qn_stid_dev;           /* q' stored at its scatter item, qn_stid[stid_i] (the
+                                     reference's mScatterRayQueue->mRRS[stid],
+                                     integrator.cpp:1408), or NULL                        */
 } nrrs_decide_args;
 
 /* ---------------- configuration and lifetime ---------------- */
@@ -204,6 +207,17 @@ typedef struct {
 } nrrs_vertices_out;
 nrrs_status nrrs_synth_vertices(uint32_t n, uint64_t seed, uint32_t n_pixels,
                                 const nrrs_vertices_out* out, void* stream);
+
+/* Survivor consumer (handleIntersections, integrator.cpp:772-794): for k < min(*n_survivors_dev,
+ * capacity), row k of out is scatter item sid = survivors_dev[k] of the scatter-queue SoA
+ * `scatter` (indexed by stid; scatter->stid unused) with its throughput divided by rrs_dev[sid],
+ * the q' stored per scatter item (1 for specular items). node_in_dev[sid] is carried to
+ * node_out_dev[k] and sid to stid_out_dev[k] when those are given. Rows whose sid is not
+ * below scatter->stride are left unwritten. */
+nrrs_status nrrs_consume_survivors(nrrs_ctx* ctx, uint32_t capacity, const uint32_t* n_survivors_dev,
+                                   const uint32_t* survivors_dev, const nrrs_vertices* scatter, const float* rrs_dev,
+                                   const int32_t* node_in_dev, const nrrs_vertices_out* out, int32_t* node_out_dev,
+                                   uint32_t* stid_out_dev, void* stream);
 
 /* ---------------- training (network.cpp:196-295, myNetworkLL2.cpp:41-89) ---------------- */
 

@@ -1036,4 +1036,17 @@ nrrs_status nrrs_generate_training_data(nrrs_ctx* ctx, const nrrs_training_args*
     return hip_status(launch_training_data(ctx->nodes, *a, ctx->cfg.n_pixels, (hipStream_t)stream));
 }
 
+nrrs_status nrrs_consume_survivors(nrrs_ctx* ctx, uint32_t capacity, const uint32_t* n_survivors_dev,
+                                   const uint32_t* survivors_dev, const nrrs_vertices* scatter, const float* rrs_dev,
+                                   const int32_t* node_in_dev, const nrrs_vertices_out* out, int32_t* node_out_dev,
+                                   uint32_t* stid_out_dev, void* stream) {
+    if (!ctx || !out || !scatter || !vertices_ok(scatter, 0)) return NRRS_ERR_ARG;
+    if (capacity && (!n_survivors_dev || !survivors_dev || !rrs_dev || !out->pos || !out->dir || !out->normal ||
+                     !out->rough || !out->albedo_d || !out->albedo_s || !out->thp || !out->pixel ||
+                     out->stride < capacity))
+        return NRRS_ERR_ARG;
+    return hip_status(launch_consume(capacity, n_survivors_dev, survivors_dev, *scatter, rrs_dev, node_in_dev, *out,
+                                     node_out_dev, stid_out_dev, (hipStream_t)stream));
+}
+
 }  // extern "C"

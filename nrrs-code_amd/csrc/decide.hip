@@ -23,6 +23,8 @@ struct SplitArgs {
     const uint32_t* n_spec;
     uint64_t seed, stream;
     float* qn;
+    float* qn_stid;
+    const uint32_t* stid;
     uint8_t* s;
     uint32_t* tile_sums;
     DecideParams d;
@@ -43,6 +45,7 @@ __global__ __launch_bounds__(kTile) void k_split(SplitArgs a) {
         float r = a.q[i];
         r = a.d.no_upscale ? r * (nf < 1.0f ? nf : 1.0f) : r * nf;
         if (a.qn) a.qn[i] = r;
+        if (a.qn_stid) a.qn_stid[a.stid ? a.stid[i] : i] = r;
         float u = u24(pcg_output(pcg_state_at(a.seed, a.stream, i)));
         // integrator.cpp:1415-1416, 1454
         int s = (int)r;
@@ -114,7 +117,42 @@ __global__ __launch_bounds__(kTile) void k_qsum(const float* __restrict__ q, uin
     block_atomic_add_u64(term, sum);
 }
 
+// handleIntersections' consumer loop (integrator.cpp:772-794) on SoA rows.
+__global__ void k_consume(uint32_t capacity, const uint32_t* __restrict__ n_dev, const uint32_t* __restrict__ surv,
+                          nrrs_vertices sc, const float* __restrict__ rrs, const int32_t* __restrict__ node_in,
+                          nrrs_vertices_out o, int32_t* __restrict__ node_out, uint32_t* __restrict__ stid_out) {
+    const uint32_t n = min(*n_dev, capacity), S = sc.stride, So = o.stride;
+    for (uint32_t k = blockIdx.x * blockDim.x + threadIdx.x; k < n; k += gridDim.x * blockDim.x) {
+        const uint32_t sid = surv[k];
+        if (sid >= S) continue;
+        const float r = rrs[sid];
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+            o.pos[c * So + k] = sc.pos[c * S + sid];
+            o.dir[c * So + k] = sc.dir[c * S + sid];
+            o.normal[c * So + k] = sc.normal[c * S + sid];
+            o.albedo_d[c * So + k] = sc.albedo_d[c * S + sid];
+            o.albedo_s[c * So + k] = sc.albedo_s[c * S + sid];
+            o.thp[c * So + k] = sc.thp[c * S + sid] / r;
+        }
+        o.rough[k] = sc.rough[sid];
+        o.pixel[k] = sc.pixel[sid];
+        if (node_out) node_out[k] = node_in ? node_in[sid] : -1;
+        if (stid_out) stid_out[k] = sid;
+    }
+}
+
 }  // namespace
+
+hipError_t launch_consume(uint32_t capacity, const uint32_t* n_dev, const uint32_t* surv, const nrrs_vertices& sc,
+                          const float* rrs, const int32_t* node_in, const nrrs_vertices_out& o, int32_t* node_out,
+                          uint32_t* stid_out, hipStream_t s) {
+    if (!capacity) return hipSuccess;
+    const uint32_t blocks = std::min<uint32_t>((capacity + 255) / 256, 8192);
+    hipLaunchKernelGGL(k_consume, dim3(blocks), dim3(256), 0, s, capacity, n_dev, surv, sc, rrs, node_in, o, node_out,
+                       stid_out);
+    return hipGetLastError();
+}
 
 hipError_t launch_qsum(nrrs_ctx* ctx, const float* q, uint32_t n, hipStream_t s) {
     hipLaunchKernelGGL(k_zero_u64, dim3(1), dim3(1), 0, s, ctx->sum_dev);
@@ -139,6 +177,8 @@ hipError_t launch_decide(nrrs_ctx* ctx, const nrrs_decide_args& a, hipStream_t s
         sa.seed = a.pcg_seed;
         sa.stream = a.pcg_stream;
         sa.qn = a.qn_dev;
+        sa.qn_stid = a.qn_stid_dev;
+        sa.stid = a.v.stid;
         sa.s = a.split_dev ? a.split_dev : ctx->s_scratch;
         sa.tile_sums = ctx->tile_sums;
         sa.d = ctx->dec;

@@ -173,6 +173,9 @@ hipError_t launch_error_compute(uint32_t n, const float* ref, const float* wL, c
                                 double* part, float* sum, hipStream_t s);
 hipError_t launch_qsum(nrrs_ctx* ctx, const float* q, uint32_t n, hipStream_t s);
 hipError_t launch_decide(nrrs_ctx* ctx, const nrrs_decide_args& a, hipStream_t s);
+hipError_t launch_consume(uint32_t capacity, const uint32_t* n_dev, const uint32_t* surv, const nrrs_vertices& sc,
+                          const float* rrs, const int32_t* node_in, const nrrs_vertices_out& o, int32_t* node_out,
+                          uint32_t* stid_out, hipStream_t s);
 
 hipError_t launch_synth(uint32_t n, uint64_t seed, uint32_t n_pixels, const nrrs_vertices_out& o,
                         hipStream_t s);

@@ -117,6 +117,7 @@ class DecideArgs(C.Structure):
         ("capacity", C.c_uint32),
         ("n_survivors_dev", C.c_void_p),
         ("overflow_dev", C.c_void_p),
+        ("qn_stid_dev", C.c_void_p),
     ]
 
 
@@ -224,6 +225,8 @@ _SIGS = {
     "nrrs_frame_vertices": (C.c_int, [C.c_uint64, C.c_uint32, _P, C.c_uint32, C.POINTER(VerticesOut), _P]),
     "nrrs_error_blend": (C.c_int, [_P, C.c_uint32, _P, _P, C.c_float, _P, _P]),
     "nrrs_error_compute": (C.c_int, [_P, C.c_uint32, C.POINTER(ErrorArgs), _P]),
+    "nrrs_consume_survivors": (C.c_int, [_P, C.c_uint32, _P, _P, C.POINTER(Vertices), _P, _P,
+                                         C.POINTER(VerticesOut), _P, _P, _P]),
     "nrrs_nodes_reset": (C.c_int, [_P, _P]),
     "nrrs_nodes_record": (C.c_int, [_P, C.c_uint32, _P, C.POINTER(Vertices), _P, _P, C.c_uint32, _P, _P]),
     "nrrs_nodes_add_radiance": (C.c_int, [_P, C.c_uint32, _P, _P, _P, C.c_uint32, _P]),
@@ -449,7 +452,7 @@ class Context:
         return out
 
     def decide_args(self, vb, film, q, survivors, n_surv, overflow=None, q_mode=Q_RRSNET, spec=None,
-                    n_spec=None, seed=0, stream_id=0, qn=None, split=None):
+                    n_spec=None, seed=0, stream_id=0, qn=None, split=None, qn_stid=None):
         a = DecideArgs()
         a.n = vb.n
         a.v = vb.c_struct()
@@ -466,6 +469,7 @@ class Context:
         a.capacity = survivors.numel()
         a.n_survivors_dev = n_surv.data_ptr()
         a.overflow_dev = None if overflow is None else overflow.data_ptr()
+        a.qn_stid_dev = None if qn_stid is None else qn_stid.data_ptr()
         return a
 
     def rrs_infer_decide(self, args, stream=None):
@@ -487,6 +491,22 @@ class Context:
         sd.error_sum = float(error_sum)
         sd.error_sum_dev = None if error_sum_dev is None else error_sum_dev.data_ptr()
         return sd
+
+    def consume_survivors(self, survivors, n_survivors, scatter, rrs, capacity, node_in=None, stream=None):
+        """Next-bounce rows from the survivor queue: scatter item survivors[k], thp / rrs[sid].
+        Returns (VertexBatch, node_out int32, stid_out int32)."""
+        import torch
+
+        dev = survivors.device
+        vb = VertexBatch(capacity, dev)
+        node_out = torch.full((max(capacity, 1),), -2, dtype=torch.int32, device=dev)
+        stid_out = torch.full((max(capacity, 1),), -1, dtype=torch.int32, device=dev)
+        sv = scatter.c_struct()
+        o = vb.c_out()
+        check(self.lib.nrrs_consume_survivors(self.h, capacity, _ptr(n_survivors), _ptr(survivors), C.byref(sv),
+                                              _ptr(rrs), _ptr(node_in), C.byref(o), _ptr(node_out), _ptr(stid_out),
+                                              _stream(stream)), "consume_survivors")
+        return vb, node_out, stid_out
 
     # ---- path nodes and training samples
     def nodes_reset(self, stream=None):

@@ -702,6 +702,27 @@ void ora_adam(uint64_t n, double* p, double* m, double* v, const double* g, doub
     }
 }
 
+// handleIntersections (integrator.cpp:772-794): copy of scatter item sid, thp /= q'(sid).
+void ora_consume_survivors(uint32_t n, const uint32_t* surv, const nrrs_vertices* sc, const float* rrs,
+                           const int32_t* node_in, const nrrs_vertices_out* out, int32_t* node_out) {
+    const uint32_t S = sc->stride, So = out->stride;
+    for (uint32_t k = 0; k < n; ++k) {
+        const uint32_t sid = surv[k];
+        if (sid >= S) continue;
+        for (int c = 0; c < 3; ++c) {
+            out->pos[c * So + k] = sc->pos[c * S + sid];
+            out->dir[c * So + k] = sc->dir[c * S + sid];
+            out->normal[c * So + k] = sc->normal[c * S + sid];
+            out->albedo_d[c * So + k] = sc->albedo_d[c * S + sid];
+            out->albedo_s[c * So + k] = sc->albedo_s[c * S + sid];
+            out->thp[c * So + k] = sc->thp[c * S + sid] / rrs[sid];
+        }
+        out->rough[k] = sc->rough[sid];
+        out->pixel[k] = sc->pixel[sid];
+        node_out[k] = node_in ? node_in[sid] : -1;
+    }
+}
+
 // NRRSPathNodesBuffer::recordNode (workqueue.h:142-168), one vertex after the other.
 void ora_nodes_record(ora_nodes* nb, uint32_t n, const nrrs_vertices* v, const int32_t* parent, const float* q,
                       uint32_t depth, int32_t* out) {

@@ -137,6 +137,11 @@ uint32_t ora_training_data(const ora_nodes* nb, const float* film, uint32_t n_pi
                            double* ref_mean, float* error, float* error_avg, float* q_target, double* sample_weight,
                            uint32_t* pixel_samples);
 
+/* Survivor consumer (integrator.cpp:772-794): row k = scatter item surv[k] with thp / rrs[sid]
+ * for k < n; node_out[k] = node_in[sid] (or -1). Rows whose sid >= scatter->stride are skipped. */
+void ora_consume_survivors(uint32_t n, const uint32_t* surv, const nrrs_vertices* sc, const float* rrs,
+                           const int32_t* node_in, const nrrs_vertices_out* out, int32_t* node_out);
+
 /* Per-pixel error pipeline (train.cu:344-430, integrator.cpp:2466-2471), planes [3][n]:
  * running radiance blend, relative squared error (clamped at 100), row 1 scaled by sqrt(spp)
  * (spp NULL: unscaled); returns the sum of row 1. */

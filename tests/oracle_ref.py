@@ -365,3 +365,25 @@ class Nodes:
         r["n"] = n
         r["v"] = o
         return r
+
+
+def consume_survivors(surv, hv_scatter, rrs, node_in=None):
+    """Oracle rows of the survivor consumer: dict of planes and node_out."""
+    lib = load()
+    lib.ora_consume_survivors.argtypes = [C.c_uint32, C.c_void_p, C.POINTER(na.Vertices), C.c_void_p, C.c_void_p,
+                                          C.POINTER(na.VerticesOut), C.c_void_p]
+    surv = np.ascontiguousarray(surv, np.uint32)
+    n = surv.size
+    o = {k: np.zeros((3, max(n, 1)), np.float32) for k in na.VertexBatch.FIELDS3}
+    o["rough"] = np.zeros(max(n, 1), np.float32)
+    o["pixel"] = np.zeros(max(n, 1), np.uint32)
+    vo = na.VerticesOut()
+    for k in na.VertexBatch.FIELDS3 + ("rough", "pixel"):
+        setattr(vo, k, o[k].ctypes.data)
+    vo.stride = max(n, 1)
+    rrs = np.ascontiguousarray(rrs, np.float32)
+    ni = None if node_in is None else np.ascontiguousarray(node_in, np.int32)
+    node_out = np.zeros(max(n, 1), np.int32)
+    lib.ora_consume_survivors(n, surv.ctypes.data, C.byref(hv_scatter.v), rrs.ctypes.data, _p(ni), C.byref(vo),
+                              node_out.ctypes.data)
+    return o, node_out
