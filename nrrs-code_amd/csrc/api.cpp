@@ -173,7 +173,15 @@ hipError_t run_net(nrrs_ctx* c, int net, const EncodeArgs& a, int epi, uint32_t 
     const NetState& ns = c->net[net];
     if (c->cfg.precision == NRRS_FP32)
         return launch_net_fp32(a, net, ns.n_layers, ns.n_out, epi, n, ns.grid_f32, ns.mlp_f32, ep, s);
-    return launch_net_mfma(c->cfg.precision, a, net, ns.n_layers, ns.n_out, epi, n, ns.grid_lp, ns.mlp_lp, ep, s);
+    if (!c->grid_enc || n > c->grid_enc_cap)
+        return launch_net_mfma(c->cfg.precision, a, net, ns.n_layers, ns.n_out, epi, n, ns.grid_lp, ns.mlp_lp, ep, s);
+    // level-major grid gathers first, then the MLP kernel reads the encoding
+    EncodeArgs ae = a;
+    ae.grid_enc = c->grid_enc;
+    ae.grid_enc_stride = c->grid_enc_cap;
+    hipError_t e = launch_encode_levels(c->cfg.precision, ae, ns.grid_lp, n, c->grid_enc, c->grid_enc_cap, s);
+    if (e != hipSuccess) return e;
+    return launch_net_mfma(c->cfg.precision, ae, net, ns.n_layers, ns.n_out, epi, n, ns.grid_lp, ns.mlp_lp, ep, s);
 }
 
 // Packed training tiles from the FP32 masters (after every parameter change).
@@ -252,6 +260,13 @@ nrrs_status run_backward(nrrs_ctx* c, int net, uint32_t n, const nrrs_vertices& 
     const nrrs_config& cfg = c->cfg;
     TrainArgs t{};
     t.a = encode_args(c, v, film, stats, stats_stride);
+    if (cfg.precision != NRRS_FP32) {
+        // level-major grid gathers of the forward, read back by the training kernel
+        if (!c->grid_enc || n > c->grid_enc_cap) return NRRS_ERR_SHAPE;
+        t.a.grid_enc = c->grid_enc;
+        t.a.grid_enc_stride = c->grid_enc_cap;
+        NRRS_HIP(launch_encode_levels(cfg.precision, t.a, ns.grid_lp, n, c->grid_enc, c->grid_enc_cap, s));
+    }
     t.net = net;
     t.n = n;
     t.table_f32 = cfg.precision == NRRS_FP32 ? ns.grid_f32 : nullptr;
@@ -484,6 +499,7 @@ void free_ctx(nrrs_ctx* c) {
     hipFree(c->tile_sums);
     hipFree(c->tile_offsets);
     hipFree(c->stats_scratch);
+    hipFree(c->grid_enc);
     hipFree(c->q_scratch);
     hipFree(c->s_scratch);
     delete c;
@@ -640,6 +656,10 @@ nrrs_status nrrs_create(const nrrs_config* cfg, nrrs_ctx** out) {
     if (st == NRRS_OK) st = dmalloc(&c->tile_sums, c->max_tiles);
     if (st == NRRS_OK) st = dmalloc(&c->tile_offsets, c->max_tiles);
     if (st == NRRS_OK) st = dmalloc(&c->stats_scratch, (size_t)6 * cfg->max_queue);
+    if (st == NRRS_OK && cfg->precision != NRRS_FP32) {
+        c->grid_enc_cap = cfg->max_queue > cfg->max_train ? cfg->max_queue : cfg->max_train;
+        st = dmalloc(&c->grid_enc, (size_t)c->grid.n_levels * c->grid_enc_cap);
+    }
     if (st == NRRS_OK) st = dmalloc(&c->q_scratch, cfg->max_queue);
     if (st == NRRS_OK) st = dmalloc(&c->s_scratch, cfg->max_queue);
     if (st != NRRS_OK) {

@@ -39,6 +39,10 @@ struct EncodeArgs {
     uint32_t stats_stride;
     GridParams grid;
     Frame frame;
+    // Level-major grid encoding [level][grid_enc_stride] of packed FP16/BF16 feature pairs
+    // (k_encode_levels); when set, the MFMA kernels read it instead of gathering.
+    const uint32_t* grid_enc = nullptr;
+    uint32_t grid_enc_stride = 0;
 };
 
 struct DecideParams {
@@ -120,6 +124,8 @@ struct nrrs_ctx {
     // RRS-stage scratch
     unsigned long long* sum_dev = nullptr;  // [1]
     double* err_part = nullptr;             // per-block partial sums of the pixel error
+    uint32_t* grid_enc = nullptr;           // [levels][grid_enc_cap] level-major encoding (FP16/BF16)
+    uint32_t grid_enc_cap = 0;
     nrrs::NodeBuf nodes;                    // path nodes of the training paths
     void* nodes_mem = nullptr;
     uint32_t* tile_sums = nullptr;          // [max tiles]
@@ -156,6 +162,10 @@ hipError_t launch_net_fp32(const EncodeArgs& a, int net, int n_layers, int n_out
 size_t mfma_packed_halves(int n_layers);
 hipError_t launch_pack_lp(int precision, const float* grid, uint64_t n_grid, const float* W, int n_layers,
                           int n_out, uint16_t* grid_lp, uint16_t* W_lp, hipStream_t s);
+// Level-major grid encoding: one workgroup per (level, 256 vertices), levels in order, so the
+// whole GPU gathers from one level's table at a time (it stays in L2).
+hipError_t launch_encode_levels(int precision, const EncodeArgs& a, const uint16_t* table, uint32_t n, uint32_t* out,
+                                uint32_t stride, hipStream_t s);
 hipError_t launch_net_mfma(int precision, const EncodeArgs& a, int net, int n_layers, int n_out, int epi,
                            uint32_t n, const uint16_t* table, const uint16_t* Wp, const EpiArgs& ep,
                            hipStream_t s);

@@ -134,6 +134,38 @@ __device__ __forceinline__ void encode_half(const EncodeArgs& a, int net, const 
     encode_half_rec<P>(v, a.grid, net, table, h, ref, have_st ? st : nullptr, B);
 }
 
+// Grid part of the B operands from the level-major encoding: chunk kb of lane half h holds
+// levels 8kb + 4h + {0..3}, each a packed feature pair (feature 0 in the low half).
+template <typename P>
+__device__ __forceinline__ void grid_half_from_levels(const uint32_t* __restrict__ enc, uint32_t stride,
+                                                      uint32_t n_levels, uint32_t i, int h, typename P::vec8* B) {
+#pragma unroll
+    for (int kb = 0; kb < 2; ++kb) {
+        u32x4 u;
+#pragma unroll
+        for (int ll = 0; ll < 4; ++ll) {
+            const uint32_t l = 8 * kb + 4 * h + ll;
+            u[ll] = l < n_levels ? enc[(size_t)l * stride + i] : 0u;
+        }
+        B[kb] = __builtin_bit_cast(typename P::vec8, u);
+    }
+}
+
+// encode_half with the grid channels read from the level-major encoding.
+template <typename P>
+__device__ __forceinline__ void encode_half_pre(const EncodeArgs& a, int net, uint32_t i, int h, typename P::vec8* B) {
+    VertexRec v = vertex_from_soa(a, i);
+    float st[6];
+    const bool have_st = net == NRRS_RRSNET && a.stats;
+    if (have_st) {
+#pragma unroll
+        for (int k = 0; k < 6; ++k) st[k] = a.stats[(size_t)k * a.stats_stride + i];
+    }
+    const float ref = net == NRRS_RRSNET ? film_mean(a.film, a.n_pixels, v.pixel) : 0.0f;
+    grid_half_from_levels<P>(a.grid_enc, a.grid_enc_stride, a.grid.n_levels, i, h, B);
+    encode_tail_half<P>(v, net, h, ref, have_st ? st : nullptr, B);
+}
+
 __device__ __forceinline__ u32x4 lds_tile(const u32x4* __restrict__ sW, int tile, int lane) {
     return sW[tile * 64 + lane];
 }

@@ -50,6 +50,20 @@ __global__ void k_to_lp(const float* __restrict__ src, uint16_t* __restrict__ ds
         dst[i] = lp_bits<P>(src[i]);
 }
 
+// Level-major grid encoding: workgroup b takes level b / n_chunks and vertices
+// 256 (b % n_chunks) .. +255, so consecutive workgroups share one level's table.
+template <typename P>
+__global__ __launch_bounds__(256) void k_encode_levels(EncodeArgs a, const uint32_t* __restrict__ table, uint32_t n,
+                                                       uint32_t n_chunks, uint32_t* __restrict__ out,
+                                                       uint32_t stride) {
+    const uint32_t l = blockIdx.x / n_chunks, i = (blockIdx.x % n_chunks) * 256 + threadIdx.x;
+    if (i >= n || l >= a.grid.n_levels) return;
+    float x[3];
+    load_xn(a, i, x);
+    const float2 f = grid_level_pairs<PairLoadLp<P>>(table, a.grid, (int)l, x);
+    out[(size_t)l * stride + i] = (uint32_t)lp_bits<P>(f.x) | ((uint32_t)lp_bits<P>(f.y) << 16);
+}
+
 template <typename P, int NL, int NOUT, int EPI>
 __global__ __launch_bounds__(kBlockT) void k_net_mfma(EncodeArgs a, int net, uint32_t n,
                                                       const uint32_t* __restrict__ table,
@@ -68,7 +82,10 @@ __global__ __launch_bounds__(kBlockT) void k_net_mfma(EncodeArgs a, int net, uin
         const bool live = i < n;
         typename P::vec8 B[5];
         if (live) {
-            encode_half<P>(a, net, table, i, h, B);
+            if (a.grid_enc)
+                encode_half_pre<P>(a, net, i, h, B);
+            else
+                encode_half<P>(a, net, table, i, h, B);
         } else {
 #pragma unroll
             for (int kb = 0; kb < 5; ++kb) B[kb] = typename P::vec8{};
@@ -141,6 +158,21 @@ hipError_t dispatch(const EncodeArgs& a, int net, int n_layers, int n_out, int e
 }
 
 }  // namespace
+
+hipError_t launch_encode_levels(int precision, const EncodeArgs& a, const uint16_t* table, uint32_t n, uint32_t* out,
+                                uint32_t stride, hipStream_t s) {
+    if (!n) return hipSuccess;
+    const uint32_t n_chunks = (n + 255) / 256;
+    const dim3 grid(n_chunks * a.grid.n_levels);
+    const uint32_t* t = reinterpret_cast<const uint32_t*>(table);
+    if (precision == NRRS_BF16)
+        hipLaunchKernelGGL(k_encode_levels<BF16>, grid, dim3(256), 0, s, a, t, n, n_chunks, out, stride);
+    else if (precision == NRRS_FP16)
+        hipLaunchKernelGGL(k_encode_levels<F16>, grid, dim3(256), 0, s, a, t, n, n_chunks, out, stride);
+    else
+        return hipErrorInvalidValue;
+    return hipGetLastError();
+}
 
 size_t mfma_packed_halves(int n_layers) { return (size_t)tiles_of(n_layers) * kTileHalves; }
 

@@ -240,24 +240,34 @@ __global__ __launch_bounds__(64 * P::kWaves) void k_train(TrainArgs t) {
         {
             float e8[8];
             if (live) {
-                float x[3];
-                load_xn(t.a, i, x);
+                if constexpr (!P::kF32Table) {
+                    // FP16/BF16: the level-major encoding computed before this kernel
 #pragma unroll
-                for (int kb = 0; kb < 2; ++kb) {
+                    for (int kb = 0; kb < 2; ++kb) {
+                        u32x4 u;
 #pragma unroll
-                    for (int ll = 0; ll < 4; ++ll) {
-                        const int l = 8 * kb + 4 * h + ll;
-                        float2 f = make_float2(0.0f, 0.0f);
-                        if (l < (int)t.a.grid.n_levels) {
-                            if constexpr (P::kF32Table)
-                                f = grid_level_pairs<PairLoad<float>>(t.table_f32, t.a.grid, l, x);
-                            else
-                                f = grid_level_pairs<PairLoadLp<P>>(t.table_lp, t.a.grid, l, x);
+                        for (int ll = 0; ll < 4; ++ll) {
+                            const uint32_t l = 8 * kb + 4 * h + ll;
+                            u[ll] = l < t.a.grid.n_levels ? t.a.grid_enc[(size_t)l * t.a.grid_enc_stride + i] : 0u;
                         }
-                        e8[2 * ll] = f.x;
-                        e8[2 * ll + 1] = f.y;
+                        E[kb] = __builtin_bit_cast(op, u);
                     }
-                    E[kb] = P::pack(e8);
+                } else {
+                    float x[3];
+                    load_xn(t.a, i, x);
+#pragma unroll
+                    for (int kb = 0; kb < 2; ++kb) {
+#pragma unroll
+                        for (int ll = 0; ll < 4; ++ll) {
+                            const int l = 8 * kb + 4 * h + ll;
+                            float2 f = make_float2(0.0f, 0.0f);
+                            if (l < (int)t.a.grid.n_levels)
+                                f = grid_level_pairs<PairLoad<float>>(t.table_f32, t.a.grid, l, x);
+                            e8[2 * ll] = f.x;
+                            e8[2 * ll + 1] = f.y;
+                        }
+                        E[kb] = P::pack(e8);
+                    }
                 }
                 float e[kEnc];
                 encode_tail(t.a, t.net, i, e);
