@@ -4,6 +4,7 @@
 // json-carried raw addresses, and status codes instead of Log(Fatal).
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cmath>
 #include <cstring>
 #include <new>
@@ -220,20 +221,67 @@ nrrs_status ensure_train(nrrs_ctx* c, int net) {
         }
     }
     if (c->cfg.precision != NRRS_FP32) {
-        // Binned scatter of the hashed levels: segments of kSegEntries entries, each with room
-        // for 1.25x its expected share of a full batch's 8 records per vertex and level.
-        const uint32_t T = c->grid.hash_mask + 1u;
-        uint32_t n_hashed = 0;
-        for (uint32_t l = 0; l < c->grid.n_levels; ++l)
-            if (!((c->grid.dense_mask >> l) & 1u)) ++n_hashed;
-        if (n_hashed && T >= kSegEntries && T / kSegEntries <= kMaxSegBins) {
-            ns.seg_bins = T / kSegEntries;
-            ns.n_seg = n_hashed * ns.seg_bins;
-            const uint64_t expect = (uint64_t)c->cfg.max_train * 8u / ns.seg_bins;
-            ns.rec_cap = (uint32_t)((expect + expect / 4 + 4096) & ~63ull);  // even: 16-byte rows
-            NRRS_TRY(dmalloc(reinterpret_cast<uint8_t**>(&ns.rec_buf), (size_t)ns.n_seg * ns.rec_cap * sizeof(uint2)));
-            NRRS_TRY(dmalloc(&ns.seg_count, ns.n_seg));
-            NRRS_HIP(hipMemset(ns.seg_count, 0, (size_t)ns.n_seg * 4));
+        // Binned scatter: every level cut into segments of kSegEntries entries, each with room
+        // for 1.25x (hashed levels, spread by the hash) or 3x (dense levels, whose segments see
+        // the geometry) its share of a full batch's 8 records per vertex; what does not fit is
+        // added atomically.
+        const uint64_t T = (uint64_t)c->grid.hash_mask + 1u, B8 = 8ull * c->cfg.max_train;
+        std::vector<SegInfo> segs;
+        std::vector<uint32_t> direct, chunked;
+        std::vector<uint2> chunks;
+        bool ok = true;
+        uint64_t rec_total = 0;
+        for (uint32_t l = 0; l < c->grid.n_levels && ok; ++l) {
+            const bool dense = (c->grid.dense_mask >> l) & 1u;
+            const uint64_t r = c->grid.res[l], size = dense ? r * r * r : T;
+            const uint32_t nb = (uint32_t)((size + kSegEntries - 1) / kSegEntries);
+            if (nb > kMaxSegBins) {
+                ok = false;
+                break;
+            }
+            ns.seg_base[l] = (uint32_t)segs.size();
+            ns.seg_nbins[l] = nb;
+            for (uint32_t b = 0; b < nb; ++b) {
+                const uint64_t ne = std::min<uint64_t>(kSegEntries, size - (uint64_t)b * kSegEntries);
+                const uint64_t expect = B8 * ne / size;
+                uint64_t cap = dense ? std::min<uint64_t>(B8, 3 * expect + 4096) : expect + expect / 4 + 4096;
+                cap = (cap + 63) / 64 * 64;
+                const uint32_t idx = (uint32_t)segs.size();
+                segs.push_back(SegInfo{(uint32_t)rec_total, (uint32_t)cap, c->grid.offset[l] + b * kSegEntries,
+                                       (uint32_t)ne});
+                if (dense) {
+                    chunked.push_back(idx);
+                    for (uint64_t r0 = 0; r0 < cap; r0 += kApplyChunk) chunks.push_back(make_uint2(idx, (uint32_t)r0));
+                } else {
+                    direct.push_back(idx);
+                }
+                rec_total += cap;
+            }
+        }
+        if (ok && rec_total < 0xFFFFFFFFull) {
+            NRRS_TRY(dmalloc(&ns.rec_buf, rec_total));
+            NRRS_TRY(dmalloc(&ns.seg_count, segs.size()));
+            NRRS_HIP(hipMemset(ns.seg_count, 0, segs.size() * 4));
+            NRRS_TRY(dmalloc(reinterpret_cast<SegInfo**>(&ns.segs), segs.size()));
+            NRRS_HIP(hipMemcpy(ns.segs, segs.data(), segs.size() * sizeof(SegInfo), hipMemcpyHostToDevice));
+            ns.n_direct = (uint32_t)direct.size();
+            ns.n_chunked = (uint32_t)chunked.size();
+            ns.n_chunks = (uint32_t)chunks.size();
+            if (ns.n_direct) {
+                NRRS_TRY(dmalloc(&ns.direct, direct.size()));
+                NRRS_HIP(hipMemcpy(ns.direct, direct.data(), direct.size() * 4, hipMemcpyHostToDevice));
+            }
+            if (ns.n_chunks) {
+                NRRS_TRY(dmalloc(&ns.chunked, chunked.size()));
+                NRRS_HIP(hipMemcpy(ns.chunked, chunked.data(), chunked.size() * 4, hipMemcpyHostToDevice));
+                NRRS_TRY(dmalloc(&ns.chunks, chunks.size()));
+                NRRS_HIP(hipMemcpy(ns.chunks, chunks.data(), chunks.size() * sizeof(uint2), hipMemcpyHostToDevice));
+                NRRS_TRY(dmalloc(&ns.acc, 2 * (size_t)ns.n_dense));
+                NRRS_HIP(hipMemset(ns.acc, 0, 16 * (size_t)ns.n_dense));
+            }
+            NRRS_TRY(dmalloc(&ns.bad, 1));
+            NRRS_HIP(hipMemset(ns.bad, 0, 4));
+            ns.binned = true;
         }
     }
     NRRS_TRY(dmalloc(&ns.loss_scale, 1));
@@ -303,10 +351,14 @@ nrrs_status run_backward(nrrs_ctx* c, int net, uint32_t n, const nrrs_vertices& 
         t.dense_f32 = static_cast<float*>(ns.dense_rep);
     else
         t.dense_h2 = static_cast<_Float16 __attribute__((ext_vector_type(2)))*>(ns.dense_rep);
+    t.binned = ns.binned ? 1 : 0;
     t.rec_buf = ns.rec_buf;
     t.seg_count = ns.seg_count;
-    t.rec_cap = ns.rec_cap;
-    t.seg_bins = ns.seg_bins;
+    t.segs = static_cast<const SegInfo*>(ns.segs);
+    for (int l = 0; l < kMaxLevels; ++l) {
+        t.seg_base[l] = ns.seg_base[l];
+        t.seg_nbins[l] = ns.seg_nbins[l];
+    }
     t.dw_partial = ns.dw_partial;
     t.n_mlp = (uint32_t)ns.n_mlp;
     t.loss_out = loss_out;
@@ -315,17 +367,20 @@ nrrs_status run_backward(nrrs_ctx* c, int net, uint32_t n, const nrrs_vertices& 
     uint32_t blocks = iters < c->max_train_blocks ? iters : c->max_train_blocks;
     if (!blocks) return NRRS_OK;
     NRRS_HIP(launch_train(cfg.precision, t, ns.n_layers, ns.n_out, kind, blocks, s));
-    if (ns.seg_bins) {
+    if (ns.binned) {
         ApplyArgs ap{};
         ap.rec = ns.rec_buf;
         ap.seg_count = ns.seg_count;
-        ap.cap = ns.rec_cap;
-        ap.bins = ns.seg_bins;
-        uint32_t hl = 0;
-        for (uint32_t l = 0; l < c->grid.n_levels; ++l)
-            if (!((c->grid.dense_mask >> l) & 1u)) ap.level_off[hl++] = c->grid.offset[l];
+        ap.segs = static_cast<const SegInfo*>(ns.segs);
+        ap.direct = ns.direct;
+        ap.chunks = ns.chunks;
+        ap.chunked = ns.chunked;
+        ap.n_chunked = ns.n_chunked;
+        ap.acc = ns.acc;
+        ap.n_acc = ns.n_chunks ? ns.n_dense : 0;
+        ap.bad = ns.bad;
         ap.grad = reinterpret_cast<_Float16 __attribute__((ext_vector_type(2)))*>(ns.grad_lp);
-        NRRS_HIP(launch_apply_bins(ap, ns.n_seg, s));
+        NRRS_HIP(launch_apply_bins(ap, ns.n_direct, ns.n_chunks, s));
     }
     return hip_status(launch_reduce_partials(ns.dw_partial, blocks, (uint32_t)ns.n_mlp, ns.grad + ns.n_grid, s));
 }
@@ -366,7 +421,7 @@ nrrs_status ensure_nodes(nrrs_ctx* c) {
 // Folds the dense-level replicas into the table gradient (and clears them).
 nrrs_status fold_dense(nrrs_ctx* c, int net, hipStream_t s) {
     NetState& ns = c->net[net];
-    if (!ns.dense_rep) return NRRS_OK;
+    if (!ns.dense_rep || ns.binned) return NRRS_OK;  // binned contexts do not use the replicas
     const bool f32 = c->cfg.precision == NRRS_FP32;
     return hip_status(launch_reduce_dense(f32 ? static_cast<float*>(ns.dense_rep) : nullptr, f32 ? nullptr : ns.dense_rep,
                                           ns.n_dense, ns.grad, f32 ? nullptr : ns.grad_lp, s));
@@ -408,13 +463,19 @@ nrrs_status ensure_frame(nrrs_ctx* c, uint32_t cap) {
     hipFree(c->frame_q);
     hipFree(c->frame_s);
     hipFree(c->frame_tiles);
+    hipFree(c->frame_x);
+    hipFree(c->frame_genc);
     c->frame_q = nullptr;
     c->frame_s = nullptr;
     c->frame_tiles = nullptr;
+    c->frame_x = nullptr;
+    c->frame_genc = nullptr;
     c->frame_cap = 0;
     NRRS_TRY(dmalloc(&c->frame_q, cap));
     NRRS_TRY(dmalloc(&c->frame_s, cap));
     NRRS_TRY(dmalloc(&c->frame_tiles, 2 * ((size_t)cap / 64 + 1)));
+    NRRS_TRY(dmalloc(&c->frame_x, 3 * (size_t)cap));
+    NRRS_TRY(dmalloc(&c->frame_genc, 2 * (size_t)c->grid.n_levels * cap));
     if (!c->frame_sums) NRRS_TRY(dmalloc(&c->frame_sums, 16));
     if (!c->frame_bar) {
         NRRS_TRY(dmalloc(&c->frame_bar, 2));
@@ -454,6 +515,8 @@ nrrs_status frame_params(nrrs_ctx* c, const nrrs_frame_args* a, FrameParams* f) 
     f->trace = a->trace_dev;
     f->status = a->status_dev;
     f->q_scr = c->frame_q;
+    f->xbuf = c->frame_x;
+    f->genc = c->frame_genc;
     f->s_scr = c->frame_s;
     f->tile_sums = c->frame_tiles;
     f->tile_offsets = c->frame_tiles + (a->capacity / 64 + 1);
@@ -478,6 +541,12 @@ void free_ctx(nrrs_ctx* c) {
         hipFree(ns.dense_rep);
         hipFree(ns.rec_buf);
         hipFree(ns.seg_count);
+        hipFree(ns.segs);
+        hipFree(ns.direct);
+        hipFree(ns.chunked);
+        hipFree(ns.chunks);
+        hipFree(ns.acc);
+        hipFree(ns.bad);
         hipFree(ns.adam_m);
         hipFree(ns.adam_v);
         hipFree(ns.w_fwd);
@@ -493,6 +562,8 @@ void free_ctx(nrrs_ctx* c) {
     hipFree(c->frame_q);
     hipFree(c->frame_s);
     hipFree(c->frame_tiles);
+    hipFree(c->frame_x);
+    hipFree(c->frame_genc);
     hipFree(c->frame_sums);
     hipFree(c->frame_bar);
     hipFree(c->sum_dev);

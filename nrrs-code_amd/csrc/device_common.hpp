@@ -67,21 +67,24 @@ struct SynthVertex {
     float pos[3], dir[3], nrm[3], rough, ad[3], as[3], thp[3];
 };
 
+// Position of synthetic vertex i (streams 0 and 1 of the recipe).
+__device__ __forceinline__ void synth_position(uint64_t seed, uint64_t i, float* pos) {
+    uint64_t st = pcg_state_at(seed, 1, 3ull * i);
+    uint32_t a = pcg_next(st, 1), b = pcg_next(st, 1), c = pcg_next(st, 1);
+    uint64_t tst = pcg_state_at(seed, 0, 9ull * (a & 63u));
+    float T[9];
+#pragma unroll
+    for (int k = 0; k < 9; ++k) T[k] = u24(pcg_next(tst, 0));
+    float su = sqrtf(u24(b));
+    float b0 = 1.0f - su, b1 = u24(c) * su, b2 = 1.0f - b0 - b1;
+#pragma unroll
+    for (int k = 0; k < 3; ++k) pos[k] = b0 * T[k] + b1 * T[3 + k] + b2 * T[6 + k];
+}
+
 __device__ __forceinline__ SynthVertex synth_vertex(uint64_t seed, uint64_t i) {
     SynthVertex v;
     const float two_pi = 6.283185307179586f;
-    {
-        uint64_t st = pcg_state_at(seed, 1, 3ull * i);
-        uint32_t a = pcg_next(st, 1), b = pcg_next(st, 1), c = pcg_next(st, 1);
-        uint64_t tst = pcg_state_at(seed, 0, 9ull * (a & 63u));
-        float T[9];
-#pragma unroll
-        for (int k = 0; k < 9; ++k) T[k] = u24(pcg_next(tst, 0));
-        float su = sqrtf(u24(b));
-        float b0 = 1.0f - su, b1 = u24(c) * su, b2 = 1.0f - b0 - b1;
-#pragma unroll
-        for (int k = 0; k < 3; ++k) v.pos[k] = b0 * T[k] + b1 * T[3 + k] + b2 * T[6 + k];
-    }
+    synth_position(seed, i, v.pos);
 #pragma unroll
     for (int f = 0; f < 2; ++f) {
         float* dst = f ? v.nrm : v.dir;

@@ -4,7 +4,9 @@
 // primary path). For queue slot k the path vertex is drawn from the synthetic
 // recipe with seed_b = seed + b * 0x9E3779B97F4A7C15 (it stands in for the
 // closest-hit program that produces the inference queue, device.cu:97-108),
-// then: StatNet -> RRSNet (statistics passed in registers) -> q -> exact
+// then: level-major grid encodings of both nets (every workgroup walks the (net, level,
+// vertex chunk) items in the same order, so the GPU gathers from one level's table at a
+// time and it stays in L2) -> StatNet -> RRSNet (statistics passed in registers) -> q -> exact
 // normalisation sum -> split decision -> stable compaction of the survivors into
 // the next queue. Queue sizes never leave the device, so the host neither waits
 // nor reads anything back between bounces (the reference reads the queue size back
@@ -24,6 +26,7 @@ using namespace mfma;
 
 constexpr uint32_t kSpinLimit = 1u << 22;
 constexpr uint64_t kBounceSeedStep = 0x9E3779B97F4A7C15ull;
+constexpr uint32_t kA0Vpt = 1;  // vertices per thread in the grid-encoding phase
 
 // Bounded sense-by-generation barrier over all workgroups of the launch. One
 // agent-scope release fence before arriving and one acquire fence after leaving;
@@ -77,6 +80,13 @@ __device__ __forceinline__ uint32_t block_scan_excl(uint32_t v, uint32_t* total,
     return base + inc - v;
 }
 
+__device__ __forceinline__ void store_position(const FrameParams& f, uint64_t seed_b, uint32_t k) {
+    float p[3];
+    synth_position(seed_b, k, p);
+#pragma unroll
+    for (int c = 0; c < 3; ++c) f.xbuf[(size_t)c * f.capacity + k] = norm_coord(p[c], f.frame.c[c], f.frame.d[c]);
+}
+
 template <typename P, int WAVES>
 __global__ __launch_bounds__(64 * WAVES) void k_frame(FrameParams f) {
     constexpr int BLOCK = 64 * WAVES;
@@ -99,7 +109,10 @@ __global__ __launch_bounds__(64 * WAVES) void k_frame(FrameParams f) {
         for (uint32_t b = 0; b < f.n_bounces; ++b) f.sums[b] = 0ull;
         if (f.n_primary > cap) atomicOr(f.status, 2u);
     }
-    for (uint32_t k = blockIdx.x * BLOCK + threadIdx.x; k < f.n_primary && k < cap; k += nb * BLOCK) f.queue[k] = k;
+    for (uint32_t k = blockIdx.x * BLOCK + threadIdx.x; k < f.n_primary && k < cap; k += nb * BLOCK) {
+        f.queue[k] = k;
+        store_position(f, seed, k);
+    }
     if (!grid_sync(f.bar, nb, f.status)) return;
 
     for (uint32_t b = 0; b < f.n_bounces; ++b) {
@@ -110,6 +123,34 @@ __global__ __launch_bounds__(64 * WAVES) void k_frame(FrameParams f) {
         if (f.trace)
             for (uint32_t k = blockIdx.x * BLOCK + threadIdx.x; k < n; k += nb * BLOCK)
                 f.trace[(size_t)b * cap + k] = qin[k];
+
+        // ---- phase A0: level-major grid encodings of both nets; a thread takes kA0Vpt vertices
+        // of an item so that their gathers are in flight together (this kernel keeps few waves
+        // per CU resident)
+        {
+            constexpr uint32_t kSpan = BLOCK * kA0Vpt;
+            const uint32_t L = f.grid.n_levels, n_ch = (n + kSpan - 1) / kSpan, items = 2u * L * n_ch;
+            for (uint32_t it = blockIdx.x; it < items; it += nb) {
+                const uint32_t nl = it / n_ch, base = (it % n_ch) * kSpan + threadIdx.x;  // nl = net * L + level
+                const uint32_t* table = nl < L ? f.table_stat : f.table_rrs;
+                float2 e[kA0Vpt];
+#pragma unroll
+                for (uint32_t j = 0; j < kA0Vpt; ++j) {
+                    const uint32_t i = base + j * BLOCK;
+                    const uint32_t ic = i < n ? i : 0u;  // out-of-range lanes gather vertex 0, not stored
+                    const float x[3] = {f.xbuf[ic], f.xbuf[cap + ic], f.xbuf[2 * cap + ic]};
+                    e[j] = grid_level_pairs<PairLoadLp<P>>(table, f.grid, (int)(nl % L), x);
+                }
+#pragma unroll
+                for (uint32_t j = 0; j < kA0Vpt; ++j) {
+                    const uint32_t i = base + j * BLOCK;
+                    if (i < n)
+                        f.genc[(size_t)nl * cap + i] =
+                            (uint32_t)lp_bits<P>(e[j].x) | ((uint32_t)lp_bits<P>(e[j].y) << 16);
+                }
+            }
+        }
+        if (!grid_sync(f.bar, nb, f.status)) return;
 
         // ---- phase A: StatNet -> RRSNet -> q, exact normalisation sum
         {
@@ -140,9 +181,10 @@ __global__ __launch_bounds__(64 * WAVES) void k_frame(FrameParams f) {
                         v.rough = sv.rough;
                         v.pixel = qin[i];
                         ref = film_mean(f.film, f.n_pixels, v.pixel);
-                        // both nets' grid gathers depend on the position only: issue them together
-                        encode_grid_half<P>(v, f.grid, f.table_rrs, h, Br);
-                        encode_half_rec<P>(v, f.grid, NRRS_STATNET, f.table_stat, h, 0.0f, nullptr, B);
+                        const uint32_t L = f.grid.n_levels;
+                        grid_half_from_levels<P>(f.genc, cap, L, i, h, B);
+                        grid_half_from_levels<P>(f.genc + (size_t)L * cap, cap, L, i, h, Br);
+                        encode_tail_half<P>(v, NRRS_STATNET, h, 0.0f, nullptr, B);
                     } else {
 #pragma unroll
                         for (int kb = 0; kb < 5; ++kb) B[kb] = typename P::vec8{};
@@ -244,6 +286,13 @@ __global__ __launch_bounds__(64 * WAVES) void k_frame(FrameParams f) {
                 for (uint32_t c = 0; c < si; ++c)
                     if (pos + c < cap) qout[pos + c] = id;
             }
+        }
+        // positions of the next bounce (slot k draws vertex k of that bounce's seed); the
+        // current positions were last read in phase A0
+        if (b + 1 < f.n_bounces) {
+            const uint32_t n1 = __hip_atomic_load(f.sizes + b + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            for (uint32_t k = blockIdx.x * BLOCK + threadIdx.x; k < n1; k += nb * BLOCK)
+                store_position(f, seed + (uint64_t)(b + 1) * kBounceSeedStep, k);
         }
         if (!grid_sync(f.bar, nb, f.status)) return;
     }

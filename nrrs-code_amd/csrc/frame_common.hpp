@@ -32,6 +32,8 @@ struct FrameParams {
     uint32_t* tile_sums;
     uint32_t* tile_offsets;
     unsigned long long* sums;     // [n_bounces]
+    float* xbuf;                  // [3][capacity] normalised positions of the current bounce
+    uint32_t* genc;               // [2][levels][capacity] level-major grid encodings (StatNet, RRSNet)
     uint32_t* bar;                // [2] barrier count / generation
 };
 

@@ -98,11 +98,20 @@ struct NetState {
     uint16_t* grad_lp = nullptr;     // FP16 grid gradient of FP16/BF16 contexts [n_grid]
     void* dense_rep = nullptr;       // dense-level gradient replicas (FP32 pairs or FP16 pairs)
     uint32_t n_dense = 0;            // table entries of the dense levels
-    uint2* rec_buf = nullptr;        // binned hashed-level gradient records (FP16/BF16)
+    // binned table-gradient scatter (FP16/BF16 contexts); see train_common.hpp
+    bool binned = false;
+    uint2* rec_buf = nullptr;        // gradient records
     uint32_t* seg_count = nullptr;   // records per segment
-    uint32_t rec_cap = 0;            // record capacity of a segment
-    uint32_t seg_bins = 0;           // segments per hashed level (0: direct atomics)
-    uint32_t n_seg = 0;              // segments over all hashed levels
+    void* segs = nullptr;            // SegInfo per segment
+    uint32_t seg_base[kMaxLevels] = {}, seg_nbins[kMaxLevels] = {};
+    uint32_t* direct = nullptr;      // segments applied by one workgroup each
+    uint32_t n_direct = 0;
+    uint32_t* chunked = nullptr;     // segments applied in chunks
+    uint32_t n_chunked = 0;
+    uint2* chunks = nullptr;         // {segment, first record}
+    uint32_t n_chunks = 0;
+    unsigned long long* acc = nullptr;  // fixed-point sums of the chunked segments
+    uint32_t* bad = nullptr;
     float* adam_m = nullptr;
     float* adam_v = nullptr;
     void* w_fwd = nullptr;           // packed training tiles (forward / transposed)
@@ -139,6 +148,8 @@ struct nrrs_ctx {
     float* frame_q = nullptr;
     uint8_t* frame_s = nullptr;
     uint32_t* frame_tiles = nullptr;        // [2][cap/64 + 1]
+    float* frame_x = nullptr;               // [3][cap] positions of the current bounce
+    uint32_t* frame_genc = nullptr;         // [2][levels][cap] level-major encodings
     unsigned long long* frame_sums = nullptr;  // [16]
     uint32_t* frame_bar = nullptr;          // [2]
     uint32_t frame_waves = 4, frame_m = 128;

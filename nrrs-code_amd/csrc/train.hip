@@ -96,6 +96,7 @@ struct TF32 {
 };
 
 constexpr uint32_t kDenseReplicas = kDenseGradReplicas;
+constexpr int kBinGroup = 8;  // levels per reservation round of the binned scatter
 constexpr int kStageRowsD = 64;        // delta rows
 constexpr int kStageRowsH = 96;        // input rows (80 padded to 3 x 32)
 
@@ -205,7 +206,8 @@ __global__ __launch_bounds__(64 * P::kWaves) void k_train(TrainArgs t) {
     extern __shared__ __align__(16) unsigned char smem_raw[];
     Stage<P>& st = *reinterpret_cast<Stage<P>*>(smem_raw);
     __shared__ float loss_part[kWaves];  // loss per wave, accumulated with LDS atomics
-    __shared__ uint32_t bin_cnt[kMaxSegBins], bin_gbase[kMaxSegBins], bin_lstart[kMaxSegBins + 1];
+    // binned scatter: records per (level of the group, segment), then their reserved bases
+    __shared__ uint32_t grp_cnt[kBinGroup][kMaxSegBins], grp_gbase[kBinGroup][kMaxSegBins];
 
     const int lane = threadIdx.x & 63, h = lane >> 5, col = lane & 31, wave = threadIdx.x >> 6;
     const op* Wf0 = reinterpret_cast<const op*>(t.w_fwd);
@@ -217,7 +219,8 @@ __global__ __launch_bounds__(64 * P::kWaves) void k_train(TrainArgs t) {
 #pragma unroll
     for (int s = 0; s < kSlots; ++s) acc[s] = f32x16{};
     if (lane == 0) loss_part[wave] = 0.0f;  // by the wave that accumulates into it
-    if (threadIdx.x < kMaxSegBins) bin_cnt[threadIdx.x] = 0;  // first used after several barriers
+    for (int k = threadIdx.x; k < kBinGroup * (int)kMaxSegBins; k += kThreads)
+        (&grp_cnt[0][0])[k] = 0;  // first used after several barriers
 
     // zero the padded input rows once (layer 0 has 80 of 96 rows)
     for (int k = threadIdx.x; k < 16 * kStageLd; k += kThreads) st.h[80 * kStageLd + k] = (typename P::elem)0.0f;
@@ -486,73 +489,78 @@ __global__ __launch_bounds__(64 * P::kWaves) void k_train(TrainArgs t) {
                 const bool live_v = iv < t.n;
                 float x[3] = {0.0f, 0.0f, 0.0f};
                 if (live_v) load_xn(t.a, iv, x);
+                bool binned_done = false;
+                if constexpr (!P::kF32Table) {
+                    if (t.binned) {
+                        // Binned scatter (uniform branch: every thread takes part in the
+                        // barriers), levels in groups of kBinGroup. Pass 1 gives every record
+                        // its rank within (level, segment) with LDS atomics and keeps the ranks
+                        // in LDS; then one thread per (level, segment) reserves the block's run
+                        // in the record buffer with one global atomic, all at once; pass 2
+                        // recomputes each record and writes it at run base + rank.
+                        static_assert(kThreads >= kBinGroup * (int)kMaxSegBins, "one thread per (level, segment)");
+                        static_assert(sizeof(Stage<P>::h) >= kBinGroup * kThreads * 4 * sizeof(uint16_t), "ranks fit");
+                        uint16_t* ranks = reinterpret_cast<uint16_t*>(st.h);  // [kBinGroup][kThreads * 4]
+                        const uint32_t L = t.a.grid.n_levels;
+                        for (uint32_t lg0 = 0; lg0 < L; lg0 += kBinGroup) {
+                            const uint32_t ng = min((uint32_t)kBinGroup, L - lg0);
 #pragma unroll 1
-                for (int l = 0; l < (int)t.a.grid.n_levels; ++l) {
-                    const float g0 = gst[(2 * l) * kVert + wave * 32 + pv];
-                    const float g1 = gst[(2 * l + 1) * kVert + wave * 32 + pv];
-                    const bool dense = (t.a.grid.dense_mask >> l) & 1u;
-                    if constexpr (!P::kF32Table) {
-                        if (!dense && t.seg_bins) {
-                            // Binned scatter of a hashed level (uniform branch: every thread
-                            // takes part in the barriers). The block's 8 records per vertex are
-                            // counted per segment in LDS, each segment reserves its run in
-                            // rec_buf with one global atomic, the records are sorted by segment
-                            // through LDS and written out as contiguous runs.
-                            uint32_t re[4], rk[4], rv[4];
-                            if (live_v) {
-                                LevelCell lc = level_cell(t.a.grid, l, x);
+                            for (uint32_t lg = 0; lg < ng; ++lg) {
+                                if (!live_v) break;
+                                const int l = (int)(lg0 + lg);
+                                const LevelCell lc = level_cell(t.a.grid, l, x);
+#pragma unroll
+                                for (int yz = 0; yz < 4; ++yz) {
+                                    const uint32_t re = corner_index(t.a.grid, l, lc, pb | (yz << 1));  // < level size
+                                    const uint32_t rk = atomicAdd(&grp_cnt[lg][re >> kSegShift], 1u);
+                                    ranks[lg * (kThreads * 4) + threadIdx.x * 4 + yz] = (uint16_t)rk;
+                                }
+                            }
+                            __syncthreads();
+                            {
+                                const uint32_t lg = threadIdx.x / kMaxSegBins, b = threadIdx.x % kMaxSegBins;
+                                if (lg < ng) {
+                                    const uint32_t l = lg0 + lg, c = grp_cnt[lg][b];
+                                    grp_cnt[lg][b] = 0;
+                                    grp_gbase[lg][b] =
+                                        (c && b < t.seg_nbins[l]) ? atomicAdd(t.seg_count + t.seg_base[l] + b, c) : 0u;
+                                }
+                            }
+                            __syncthreads();
+#pragma unroll 1
+                            for (uint32_t lg = 0; lg < ng; ++lg) {
+                                if (!live_v) break;
+                                const int l = (int)(lg0 + lg);
+                                const float g0 = gst[(2 * l) * kVert + wave * 32 + pv];
+                                const float g1 = gst[(2 * l + 1) * kVert + wave * 32 + pv];
+                                const LevelCell lc = level_cell(t.a.grid, l, x);
 #pragma unroll
                                 for (int yz = 0; yz < 4; ++yz) {
                                     const int cn = pb | (yz << 1);
-                                    re[yz] = corner_index(t.a.grid, l, lc, cn);  // < T = seg_bins << kSegShift
+                                    const uint32_t re = corner_index(t.a.grid, l, lc, cn), b = re >> kSegShift;
                                     const float w = corner_weight(lc, cn);
                                     const h2v gv = {(_Float16)(w * g0), (_Float16)(w * g1)};
-                                    rv[yz] = __builtin_bit_cast(uint32_t, gv);
-                                    rk[yz] = atomicAdd(&bin_cnt[re[yz] >> kSegShift], 1u);
+                                    const uint32_t pos =
+                                        grp_gbase[lg][b] + ranks[lg * (kThreads * 4) + threadIdx.x * 4 + yz];
+                                    const SegInfo sg = t.segs[t.seg_base[l] + b];
+                                    if (pos < sg.cap)
+                                        t.rec_buf[(size_t)sg.rec_off + pos] =
+                                            make_uint2(re, __builtin_bit_cast(uint32_t, gv));
+                                    else  // segment full (clustered input): add directly
+                                        __builtin_amdgcn_global_atomic_fadd_v2f16(
+                                            t.grad_grid_h2 + t.a.grid.offset[l] + re, gv);
                                 }
                             }
-                            __syncthreads();
-                            // segments of this level: hashed levels are numbered in table order
-                            const uint32_t seg0 =
-                                (uint32_t)(l - __builtin_popcount(t.a.grid.dense_mask & ((1u << l) - 1u))) * t.seg_bins;
-                            if (threadIdx.x < 64) {  // wave 0: reserve runs, exclusive scan of the counts
-                                const uint32_t b = threadIdx.x;
-                                const uint32_t c = b < t.seg_bins ? bin_cnt[b] : 0u;
-                                if (b < t.seg_bins) {
-                                    bin_cnt[b] = 0;
-                                    bin_gbase[b] = c ? atomicAdd(t.seg_count + seg0 + b, c) : 0u;
-                                }
-                                uint32_t inc = c;
-#pragma unroll
-                                for (int o = 1; o < 64; o <<= 1) {
-                                    const uint32_t y = __shfl_up(inc, o, 64);
-                                    if ((int)b >= o) inc += y;
-                                }
-                                if (b < t.seg_bins) bin_lstart[b] = inc - c;
-                                if (b == 63) bin_lstart[kMaxSegBins] = inc;
-                            }
-                            __syncthreads();
-                            uint2* stg = reinterpret_cast<uint2*>(st.h);  // free until the next iteration
-                            if (live_v) {
-#pragma unroll
-                                for (int yz = 0; yz < 4; ++yz)
-                                    stg[bin_lstart[re[yz] >> kSegShift] + rk[yz]] = make_uint2(re[yz], rv[yz]);
-                            }
-                            __syncthreads();
-                            const uint32_t total = bin_lstart[kMaxSegBins];
-                            for (uint32_t k = threadIdx.x; k < total; k += kThreads) {
-                                const uint2 r = stg[k];
-                                const uint32_t b = r.x >> kSegShift;
-                                const uint32_t pos = bin_gbase[b] + (k - bin_lstart[b]);
-                                if (pos < t.rec_cap)
-                                    t.rec_buf[(size_t)(seg0 + b) * t.rec_cap + pos] = r;
-                                else  // segment full (clustered input): add directly
-                                    __builtin_amdgcn_global_atomic_fadd_v2f16(t.grad_grid_h2 + t.a.grid.offset[l] + r.x,
-                                                                              __builtin_bit_cast(h2v, r.y));
-                            }
-                            continue;
+                            __syncthreads();  // ranks and bases are reused by the next group
                         }
+                        binned_done = true;
                     }
+                }
+#pragma unroll 1
+                for (int l = 0; l < (int)t.a.grid.n_levels && !binned_done; ++l) {
+                    const float g0 = gst[(2 * l) * kVert + wave * 32 + pv];
+                    const float g1 = gst[(2 * l + 1) * kVert + wave * 32 + pv];
+                    const bool dense = (t.a.grid.dense_mask >> l) & 1u;
                     if (!live_v) continue;
                     LevelCell lc = level_cell(t.a.grid, l, x);
                     // Dense (coarse) levels are small and hit by every vertex: their updates go
@@ -713,12 +721,15 @@ hipError_t dispatch_train(const TrainArgs& t, int nl, int n_out, int loss, uint3
     return hipErrorInvalidValue;
 }
 
-// Sum of the binned records of one segment (one workgroup per segment) into the FP16 table
-// gradient. Every FP16 value is an integer multiple of 2^-24 below 2^16, so the records are
-// summed exactly as 64-bit fixed point with integer LDS atomics (LDS float atomics are an
-// order of magnitude slower); the sum does not depend on the record order. A non-finite
-// record turns the segment's first entry into NaN, so the finite check of the optimiser
-// step sees it. The segment's count is cleared for the next launch.
+// Sums of the binned records. Every FP16 value is an integer multiple of 2^-24 below 2^16, so
+// the records are summed exactly as 64-bit fixed point with integer LDS atomics (LDS float
+// atomics are an order of magnitude slower); the sums do not depend on the record order.
+// k_apply_bins: one workgroup per segment of a hashed level adds its sum to the table
+// gradient. k_apply_chunks: segments of the dense levels, which receive many records per
+// entry, are cut into chunks whose sums are added to a 64-bit accumulator with coalesced
+// integer atomics; k_apply_finish adds the accumulator to the table gradient. A non-finite
+// record turns a table-gradient entry into NaN, so the finite check of the optimiser step
+// sees it. Counts and accumulators are cleared for the next launch.
 constexpr int kApplyThreads = 1024;
 constexpr int kApplyBatch = 4;
 
@@ -738,18 +749,12 @@ __device__ __forceinline__ void apply_record(unsigned long long* acc, uint32_t e
     apply_half(acc, 2 * e + 1, bits >> 16, bad);
 }
 
-__global__ __launch_bounds__(kApplyThreads) void k_apply_bins(ApplyArgs a) {
-    extern __shared__ unsigned long long acc_s[];  // [kSegEntries][2], units of 2^-24
-    __shared__ uint32_t bad_s;
-    const uint32_t seg = blockIdx.x, hl = seg / a.bins, b = seg % a.bins;
-    for (uint32_t k = threadIdx.x; k < 2 * kSegEntries; k += kApplyThreads) acc_s[k] = 0ull;
-    if (threadIdx.x == 0) bad_s = 0u;
-    __syncthreads();
-    uint32_t bad = 0u;
-    const uint32_t n = min(a.seg_count[seg], a.cap);
-    const uint2* r = a.rec + (size_t)seg * a.cap;  // cap is even: 16-byte aligned
-    const uint4* r4 = static_cast<const uint4*>(__builtin_assume_aligned(r, 16));
-    const uint32_t n2 = n / 2;
+// Records [r0, r1) of a segment into the LDS sums.
+__device__ __forceinline__ void apply_range(unsigned long long* acc_s, const uint2* r, uint32_t r0, uint32_t r1,
+                                            uint32_t& bad) {
+    // r0 is even (chunks and segment rows start on 16-byte boundaries)
+    const uint4* r4 = static_cast<const uint4*>(__builtin_assume_aligned(r + r0, 16));
+    const uint32_t n2 = (r1 - r0) / 2;
     constexpr uint32_t kStep = kApplyThreads * kApplyBatch;
     uint32_t k0 = 0;
     for (; k0 + kStep <= n2; k0 += kStep) {  // full batches: every load in bounds, all issued first
@@ -767,21 +772,71 @@ __global__ __launch_bounds__(kApplyThreads) void k_apply_bins(ApplyArgs a) {
         apply_record(acc_s, q.x, q.y, bad);
         apply_record(acc_s, q.z, q.w, bad);
     }
-    if ((n & 1u) && threadIdx.x == 0) apply_record(acc_s, r[n - 1].x, r[n - 1].y, bad);
+    if (((r1 - r0) & 1u) && threadIdx.x == 0) apply_record(acc_s, r[r1 - 1].x, r[r1 - 1].y, bad);
+}
+
+__device__ __forceinline__ h2v add_fixed(h2v o, long long s0, long long s1) {
+    return h2v{(_Float16)((float)o.x + (float)((double)s0 * 0x1p-24)),
+               (_Float16)((float)o.y + (float)((double)s1 * 0x1p-24))};
+}
+
+__global__ __launch_bounds__(kApplyThreads) void k_apply_bins(ApplyArgs a) {
+    extern __shared__ unsigned long long acc_s[];  // [kSegEntries][2], units of 2^-24
+    __shared__ uint32_t bad_s;
+    const uint32_t seg = a.direct[blockIdx.x];
+    const SegInfo sg = a.segs[seg];
+    for (uint32_t k = threadIdx.x; k < 2 * kSegEntries; k += kApplyThreads) acc_s[k] = 0ull;
+    if (threadIdx.x == 0) bad_s = 0u;
+    __syncthreads();
+    uint32_t bad = 0u;
+    apply_range(acc_s, a.rec + sg.rec_off, 0, min(a.seg_count[seg], sg.cap), bad);
     if (bad) bad_s = 1u;
     __syncthreads();
-    h2v* g = a.grad + a.level_off[hl] + (size_t)b * kSegEntries;
-    for (uint32_t e = threadIdx.x; e < kSegEntries; e += kApplyThreads) {
+    h2v* g = a.grad + sg.entry0;
+    for (uint32_t e = threadIdx.x; e < sg.n_entries; e += kApplyThreads) {
         const long long s0 = (long long)acc_s[2 * e], s1 = (long long)acc_s[2 * e + 1];
-        if (s0 != 0 || s1 != 0) {
-            const h2v o = g[e];
-            g[e] = h2v{(_Float16)((float)o.x + (float)((double)s0 * 0x1p-24)),
-                       (_Float16)((float)o.y + (float)((double)s1 * 0x1p-24))};
-        }
+        if (s0 != 0 || s1 != 0) g[e] = add_fixed(g[e], s0, s1);
     }
     if (threadIdx.x == 0) {
         if (bad_s) g[0] = h2v{(_Float16)__builtin_nanf(""), (_Float16)__builtin_nanf("")};
         a.seg_count[seg] = 0;
+    }
+}
+
+__global__ __launch_bounds__(kApplyThreads) void k_apply_chunks(ApplyArgs a) {
+    extern __shared__ unsigned long long acc_s[];
+    const uint2 ch = a.chunks[blockIdx.x];  // {segment, first record}
+    const SegInfo sg = a.segs[ch.x];
+    const uint32_t n = min(a.seg_count[ch.x], sg.cap);
+    if (ch.y >= n) return;  // whole workgroup
+    for (uint32_t k = threadIdx.x; k < 2 * sg.n_entries; k += kApplyThreads) acc_s[k] = 0ull;
+    __syncthreads();
+    uint32_t bad = 0u;
+    apply_range(acc_s, a.rec + sg.rec_off, ch.y, min(ch.y + kApplyChunk, n), bad);
+    if (bad) atomicOr(a.bad, 1u);
+    __syncthreads();
+    unsigned long long* acc = a.acc + 2 * (size_t)sg.entry0;
+    for (uint32_t k = threadIdx.x; k < 2 * sg.n_entries; k += kApplyThreads) {
+        const unsigned long long v = acc_s[k];
+        if (v) atomicAdd(acc + k, v);
+    }
+}
+
+__global__ void k_apply_finish(ApplyArgs a) {
+    for (uint32_t e = blockIdx.x * blockDim.x + threadIdx.x; e < a.n_acc; e += gridDim.x * blockDim.x) {
+        const long long s0 = (long long)a.acc[2 * e], s1 = (long long)a.acc[2 * e + 1];
+        if (s0 != 0 || s1 != 0) {
+            a.grad[e] = add_fixed(a.grad[e], s0, s1);
+            a.acc[2 * e] = 0ull;
+            a.acc[2 * e + 1] = 0ull;
+        }
+    }
+    if (blockIdx.x == 0) {
+        for (uint32_t k = threadIdx.x; k < a.n_chunked; k += blockDim.x) a.seg_count[a.chunked[k]] = 0;
+        if (threadIdx.x == 0 && *a.bad) {
+            a.grad[0] = h2v{(_Float16)__builtin_nanf(""), (_Float16)__builtin_nanf("")};
+            *a.bad = 0u;
+        }
     }
 }
 
@@ -870,17 +925,22 @@ hipError_t launch_train(int precision, const TrainArgs& t, int nl, int n_out, in
     return dispatch_train<TF16>(t, nl, n_out, loss, blocks, s);
 }
 
-hipError_t launch_apply_bins(const ApplyArgs& a, uint32_t n_seg, hipStream_t s) {
-    if (!n_seg) return hipSuccess;
+hipError_t launch_apply_bins(const ApplyArgs& a, uint32_t n_direct, uint32_t n_chunks, hipStream_t s) {
     constexpr size_t smem = 2 * kSegEntries * sizeof(unsigned long long);
     static bool attr = false;
     if (!attr) {
-        hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(&k_apply_bins),
-                                           hipFuncAttributeMaxDynamicSharedMemorySize, (int)smem);
-        if (e != hipSuccess) return e;
+        for (const void* k : {reinterpret_cast<const void*>(&k_apply_bins), reinterpret_cast<const void*>(&k_apply_chunks)}) {
+            hipError_t e = hipFuncSetAttribute(k, hipFuncAttributeMaxDynamicSharedMemorySize, (int)smem);
+            if (e != hipSuccess) return e;
+        }
         attr = true;
     }
-    hipLaunchKernelGGL(k_apply_bins, dim3(n_seg), dim3(kApplyThreads), smem, s, a);
+    if (n_direct) hipLaunchKernelGGL(k_apply_bins, dim3(n_direct), dim3(kApplyThreads), smem, s, a);
+    if (n_chunks) {
+        hipLaunchKernelGGL(k_apply_chunks, dim3(n_chunks), dim3(kApplyThreads), smem, s, a);
+        const uint32_t blocks = std::min<uint32_t>((a.n_acc + 255) / 256, 2048);
+        hipLaunchKernelGGL(k_apply_finish, dim3(blocks), dim3(256), 0, s, a);
+    }
     return hipGetLastError();
 }
 

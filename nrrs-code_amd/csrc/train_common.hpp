@@ -7,16 +7,33 @@ namespace nrrs {
 
 // Private copies of the dense-level table gradient (see the scatter in k_train).
 constexpr uint32_t kDenseGradReplicas = 16;
-// Segments of the binned hashed-level scatter.
+// Segments of the binned table-gradient scatter: every level is cut into segments of
+// kSegEntries entries (a level has at most kMaxSegBins of them).
 constexpr uint32_t kSegShift = 13;
 constexpr uint32_t kSegEntries = 1u << kSegShift;
 constexpr uint32_t kMaxSegBins = 64;
+// Records one workgroup of the chunked apply sums (segments of the dense levels, where one
+// segment can receive several records per vertex).
+constexpr uint32_t kApplyChunk = 1u << 17;
+
+struct SegInfo {
+    uint32_t rec_off;    // first record slot in the record buffer
+    uint32_t cap;        // record slots; records beyond are added atomically by k_train
+    uint32_t entry0;     // first table entry of the segment
+    uint32_t n_entries;  // entries of the segment (the last one of a level can be short)
+};
 
 struct ApplyArgs {
-    const uint2* rec;              // [segments][cap]
-    uint32_t* seg_count;           // [segments], cleared after use
-    uint32_t cap, bins;
-    uint32_t level_off[kMaxLevels];  // first table entry of hashed level hl
+    const uint2* rec;                // records {entry in level, FP16 pair}
+    uint32_t* seg_count;             // records appended per segment, cleared after use
+    const SegInfo* segs;
+    const uint32_t* direct;          // segments summed by one workgroup each (hashed levels)
+    const uint2* chunks;             // {segment, first record} of the chunked apply (dense levels)
+    const uint32_t* chunked;         // the segments of the chunked apply
+    uint32_t n_chunked;
+    unsigned long long* acc;         // [n_acc][2] fixed-point sums of the chunked segments
+    uint32_t n_acc;                  // entries covered by acc (the dense levels come first)
+    uint32_t* bad;                   // non-finite record seen by the chunked apply
     _Float16 __attribute__((ext_vector_type(2)))* grad;  // FP16 pair table gradient
 };
 
@@ -53,14 +70,15 @@ struct TrainArgs {
     float* dense_f32;              // [replicas][n_dense][2] (FP32 contexts)
     _Float16 __attribute__((ext_vector_type(2)))* dense_h2;  // [replicas][n_dense] (FP16/BF16)
     uint32_t n_dense;              // entries of the dense levels (they come first in the table)
-    // Binned scatter of the hashed levels (FP16/BF16 contexts). Each hashed level is cut into
-    // seg_bins segments of kSegEntries entries; the kernel appends {entry within the level,
-    // FP16 pair} records to rec_buf[segment][..] and the apply kernel sums them into the
-    // table gradient. seg_bins == 0 selects direct atomics instead.
-    uint2* rec_buf;                // [hashed levels * seg_bins][rec_cap]
-    uint32_t* seg_count;           // [hashed levels * seg_bins] records appended so far
-    uint32_t rec_cap;              // records per segment; the excess is added atomically
-    uint32_t seg_bins;
+    // Binned scatter (FP16/BF16 contexts): the kernel appends {entry within the level, FP16
+    // pair} records to the segments of each level and the apply kernels sum them into the
+    // table gradient. binned == 0 selects direct atomics (and dense-level replicas) instead.
+    int binned;
+    uint2* rec_buf;
+    uint32_t* seg_count;
+    const SegInfo* segs;
+    uint32_t seg_base[kMaxLevels];   // first segment of each level
+    uint32_t seg_nbins[kMaxLevels];  // segments of each level
     float* dw_partial;             // [gridDim.x][n_mlp]
     uint32_t n_mlp;
     float* loss_out;               // device scalar (atomic add), or null
@@ -72,8 +90,9 @@ size_t train_op_bytes(int precision);
 hipError_t launch_pack_train(int precision, const float* W, int nl, int n_out, void* fwd, void* bwd, hipStream_t s);
 hipError_t launch_train(int precision, const TrainArgs& t, int nl, int n_out, int loss, uint32_t blocks,
                         hipStream_t s);
-// Sums the binned records of n_seg segments into the table gradient.
-hipError_t launch_apply_bins(const ApplyArgs& a, uint32_t n_seg, hipStream_t s);
+// Sums the binned records into the table gradient: n_direct segments with one workgroup
+// each, n_chunks chunks of the other segments, then their fixed-point sums.
+hipError_t launch_apply_bins(const ApplyArgs& a, uint32_t n_direct, uint32_t n_chunks, hipStream_t s);
 // Vertices one training workgroup processes per iteration.
 uint32_t train_vertices_per_iter(int precision);
 hipError_t launch_reduce_dense(float* rep_f32, void* rep_h, uint32_t n_dense, float* grad, void* grad_h,
