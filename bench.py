@@ -31,10 +31,10 @@ N_PIXELS = 1 << 20
 SEED = 0x4E525253 ^ 2
 HBM_PEAK_GBS = 8000.0
 BF16_PEAK_TFLOPS = 2500.0
-# HBM bytes of one StatNet training backward at 2^20 samples (k_train + k_apply_bins +
-# k_reduce_partials), from rocprofv3 --pmc FETCH_SIZE and WRITE_SIZE
-# (profiles/train_backward_bf16_2p20_pmc.md), or None.
-TRAIN_TRAFFIC_BYTES = 5484942336
+# HBM bytes of one StatNet training backward at 2^20 samples (k_encode_levels + k_train +
+# k_apply_bins + k_apply_chunks + k_apply_finish + k_reduce_partials), from rocprofv3 --pmc
+# FETCH_SIZE and WRITE_SIZE (profiles/train_backward_bf16_2p20_pmc.md), or None.
+TRAIN_TRAFFIC_BYTES = 2973085909
 MACS = {"statnet": 80 * 64 + 3 * 64 * 64 + 64 * 6, "rrsnet": 80 * 64 + 2 * 64 * 64 + 64 * 1}
 
 
@@ -343,9 +343,9 @@ def main():
     bf = results["bf16"]
     n = N_VERTICES
     # Dominant operation of the BF16 step: the StatNet training backward (nrrs_backward with
-    # NRRS_LL2): k_train (forward with recompute, loss, backward, dW, dense-level atomics and
-    # binned hashed-level gradient records), k_apply_bins (sums the records into the table
-    # gradient) and k_reduce_partials (dW). Algorithmic bytes per sample: input record 64 B +
+    # NRRS_LL2): k_encode_levels (level-major grid gathers), k_train (forward with recompute,
+    # loss, backward, dW, binned table-gradient records), k_apply_bins / k_apply_chunks /
+    # k_apply_finish (sum the records into the table gradient) and k_reduce_partials (dW). Algorithmic bytes per sample: input record 64 B +
     # target 12 B, 16 levels x 8 corners x 4 B of BF16 feature pairs gathered, and the same
     # 512 B of FP16 gradient pairs added to the table.
     bytes_s = 64 + 12 + 16 * 8 * 4 + 16 * 8 * 4
@@ -355,13 +355,13 @@ def main():
     tflops = flops_s * n / kern_s / 1e12
     roofline = {"bound": "hbm", "achieved": round(gbs, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": round(gbs / HBM_PEAK_GBS, 4), "traffic": TRAIN_TRAFFIC_BYTES,
-                "kernel": "nrrs_backward(StatNet, NRRS_LL2) at 2^20 samples: k_train<TBF16,5,6,NRRS_LOSS_LL2> + "
-                          "k_apply_bins + k_reduce_partials",
+                "kernel": "nrrs_backward(StatNet, NRRS_LL2) at 2^20 samples: k_encode_levels + "
+                          "k_train<TBF16,5,6,NRRS_LOSS_LL2> + k_apply_bins + k_apply_chunks + k_apply_finish + "
+                          "k_reduce_partials",
                 "kernel_us": round(bf["train_kernel_us"], 1),
                 "algorithmic_bytes_per_launch": bytes_s * n,
                 "mfma_tflops": round(tflops, 2), "mfma_frac": round(tflops / BF16_PEAK_TFLOPS, 5),
-                "hashed_level_records_per_launch": 12 * 8 * n,
-                "dense_level_atomic_lanes_per_launch": 4 * 8 * n}
+                "table_gradient_records_per_launch": 16 * 8 * n}
     matched = {}
     if "fp32" in q_by:
         for name in q_by:
