@@ -44,7 +44,7 @@ def parse():
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU-baseline leg")
-    ap.add_argument("--cpu-sample", type=int, default=1 << 18)
+    ap.add_argument("--cpu-sample", type=int, default=1 << 19)
     ap.add_argument("--precisions", default="bf16,fp16,fp32")
     ap.add_argument("--no-frame", action="store_true", help="skip the multi-bounce wavefront and online loop")
     ap.add_argument("--pmc-probe", action="store_true",
@@ -53,39 +53,77 @@ def parse():
     return ap.parse_args()
 
 
+def _cpu_model():
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
 def cpu_baseline(sample):
-    """FP64 oracle, one thread: the same step (chained inference + decision, then
-    StatNet and RRSNet forward/backward and Adam) on `sample` vertices."""
+    """CPU paths of the same step (chained inference + decision, then StatNet and RRSNet
+    forward/backward and Adam) on the host: the FP32 multi-threaded port on `sample`
+    vertices (median of 5 runs after one warm-up) and the FP64 oracle on one core over
+    sample / 8 vertices (median of 3)."""
+    import statistics
+
     import numpy as np
 
     sys.path.insert(0, os.path.join(ROOT, "tests"))
     import nrrs_amd as na
     import oracle_ref as ora
 
-    cfg = na.default_config(n_pixels=N_PIXELS, max_queue=sample, max_train=sample)
-    d = ora.synth_vertices(sample, SEED, N_PIXELS)
-    hv = ora.HostVertices(d)
-    rng = np.random.default_rng(1)
-    film = rng.lognormal(-1, 1, (3, N_PIXELS)).astype(np.float32)
-    ps = ora.init_params(cfg, na.STATNET, 1, 0.5)
-    pr = ora.init_params(cfg, na.RRSNET, 2, 0.5)
-    tL = ora.synth_targets(sample, 3)
-    side = ora.RrsSide(None, rng.uniform(0, 2, sample), np.ones(sample), rng.uniform(0.05, 5, sample), sample * 0.7)
-    t0 = time.perf_counter()
-    stats = np.ascontiguousarray(ora.net_forward(cfg, na.STATNET, hv, film, None, ps).T)
-    raw = ora.net_forward(cfg, na.RRSNET, hv, film, stats, pr)[:, 0]
-    q = ora.q_f64(cfg, na.Q_RRSNET_CHAINED, hv, raw, film)
-    ora.rrs_decide(cfg, q.astype(np.float32), seed=1, stream=0)
-    t1 = time.perf_counter()
-    for net, p, kind in ((na.STATNET, ps, na.LOSS_LL2), (na.RRSNET, pr, na.LOSS_RRS)):
-        _, g, _ = ora.train_grads(cfg, net, hv, p, kind, film=film, stats=stats if net == na.RRSNET else None,
-                                  target_L=tL, side=side)
-        m, v = np.zeros_like(p), np.zeros_like(p)
-        ora.adam(p, m, v, g, 1e-2, 0.9, 0.99, 1e-15, 0.0, 1)
-    dt = time.perf_counter() - t0
-    return {"value": round(sample / dt, 1), "unit": "queries/s", "cores": 1, "kind": "port",
-            "sample": f"{sample} vertices of the same step (inference + decision {t1 - t0:.2f} s, "
-                      f"training {dt - (t1 - t0):.2f} s), FP64 oracle, 1 thread, {dt:.2f} s total"}
+    threads = int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or min(16, os.cpu_count() or 1)
+
+    def inputs(m):
+        cfg = na.default_config(n_pixels=N_PIXELS, max_queue=m, max_train=m)
+        hv = ora.HostVertices(ora.synth_vertices(m, SEED, N_PIXELS))
+        rng = np.random.default_rng(1)
+        film = rng.lognormal(-1, 1, (3, N_PIXELS)).astype(np.float32)
+        ps = ora.init_params(cfg, na.STATNET, 1, 0.5)
+        pr = ora.init_params(cfg, na.RRSNET, 2, 0.5)
+        tL = ora.synth_targets(m, 3)
+        err, sw, qt = rng.uniform(0, 2, m), np.ones(m), rng.uniform(0.05, 5, m)
+        return cfg, hv, film, ps, pr, tL, err, sw, qt
+
+    cfg, hv, film, ps, pr, tL, err, sw, qt = inputs(sample)
+    runs = []
+    for _ in range(6):
+        r = ora.cpu_step_f32(cfg, hv, film, ps, pr, tL, err, sw, qt, sample * 0.7, threads, grads=False)
+        runs.append((r["total"], r["times"][0], r["times"][1]))
+    runs = runs[1:]
+    med = statistics.median(t[0] for t in runs)
+    t_inf = statistics.median(t[1] for t in runs)
+    t_tr = statistics.median(t[2] for t in runs)
+
+    m = max(sample // 8, 1)
+    cfg, hv, film, ps, pr, tL, err, sw, qt = inputs(m)
+    side = ora.RrsSide(None, err, sw, qt, m * 0.7)
+    t64 = []
+    for _ in range(3):
+        t0 = time.perf_counter()
+        stats = np.ascontiguousarray(ora.net_forward(cfg, na.STATNET, hv, film, None, ps).T)
+        raw = ora.net_forward(cfg, na.RRSNET, hv, film, stats, pr)[:, 0]
+        q = ora.q_f64(cfg, na.Q_RRSNET_CHAINED, hv, raw, film)
+        ora.rrs_decide(cfg, q.astype(np.float32), seed=1, stream=0)
+        for net, p, kind in ((na.STATNET, ps, na.LOSS_LL2), (na.RRSNET, pr, na.LOSS_RRS)):
+            _, g, _ = ora.train_grads(cfg, net, hv, p, kind, film=film, stats=stats if net == na.RRSNET else None,
+                                      target_L=tL, side=side)
+            pp, mm, vv = p.copy(), np.zeros_like(p), np.zeros_like(p)
+            ora.adam(pp, mm, vv, g, 1e-2, 0.9, 0.99, 1e-15, 0.0, 1)
+        t64.append(time.perf_counter() - t0)
+    med64 = statistics.median(t64)
+    host = f"{os.cpu_count()} logical CPUs, {_cpu_model()}"
+    return {"value": round(sample / med, 1), "unit": "queries/s", "cores": threads, "kind": "port",
+            "sample": f"{sample} vertices of the same step, FP32 C++ port on {threads} threads, median of 5 runs "
+                      f"after a warm-up: {med:.3f} s (inference + decision {t_inf:.3f} s, training {t_tr:.3f} s); "
+                      f"host: {host}",
+            "fp64_oracle": {"value": round(m / med64, 1), "unit": "queries/s", "cores": 1,
+                            "sample": f"{m} vertices, FP64 oracle on one thread, median of 3: {med64:.2f} s"}}
 
 
 class Path:

@@ -112,6 +112,18 @@ double ora_train_grads(const nrrs_config* cfg, int32_t net, uint32_t n, const nr
 void ora_adam(uint64_t n, double* p, double* m, double* v, const double* g, double lr, double b1, double b2,
               double eps, double l2, uint32_t t);
 
+/* FP32 multi-threaded CPU step (cpu_fp32.cpp; a timing baseline, checked against the FP64
+ * functions above): chained StatNet -> RRSNet inference, q, normalisation, split and
+ * compaction of n vertices, then one training step per net (StatNet on target_L with
+ * NRRS_LL2, RRSNet on the StatNet statistics with NRRS_RRS, loss scale 1) and one Adam step
+ * (t = 1) updating params_stat / params_rrs in place. q_out [n], grad_stat / grad_rrs
+ * ([grid | mlp]) optional; times[0] = inference + decision, times[1] = training, in
+ * seconds. Returns the total seconds. */
+double ora_cpu_step_f32(const nrrs_config* cfg, uint32_t n, const nrrs_vertices* v, const float* film,
+                        float* params_stat, float* params_rrs, const float* target_L, const float* err,
+                        const float* sw, const float* qt, double error_sum, int32_t threads, float* q_out,
+                        float* grad_stat, float* grad_rrs, double* times);
+
 /* Path-node buffer (workqueue.h:106-189), host SoA planes of capacity cap; L in FP64. */
 typedef struct {
     float *pos, *dir, *normal, *rough, *ad, *as, *thp, *q;

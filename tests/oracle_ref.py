@@ -387,3 +387,24 @@ def consume_survivors(surv, hv_scatter, rrs, node_in=None):
     lib.ora_consume_survivors(n, surv.ctypes.data, C.byref(hv_scatter.v), rrs.ctypes.data, _p(ni), C.byref(vo),
                               node_out.ctypes.data)
     return o, node_out
+
+
+def cpu_step_f32(cfg, hv, film, ps, pr, target_L, err, sw, qt, error_sum, threads, grads=True):
+    """FP32 multi-threaded CPU step; returns q, gradients, updated parameters and the times."""
+    lib = load()
+    lib.ora_cpu_step_f32.restype = C.c_double
+    lib.ora_cpu_step_f32.argtypes = [C.POINTER(na.Config), C.c_uint32, C.POINTER(na.Vertices)] + [C.c_void_p] * 7 + \
+        [C.c_double, C.c_int32] + [C.c_void_p] * 4
+    n = hv.n
+    ps32 = np.ascontiguousarray(ps, np.float32).copy()
+    pr32 = np.ascontiguousarray(pr, np.float32).copy()
+    f = lambda a: np.ascontiguousarray(a, np.float32)  # noqa: E731
+    film, tL, er, w, q_t = f(film), f(target_L), f(err), f(sw), f(qt)
+    q = np.zeros(n, np.float32)
+    gs = np.zeros(ps32.size, np.float32) if grads else None
+    gr = np.zeros(pr32.size, np.float32) if grads else None
+    times = np.zeros(2)
+    total = lib.ora_cpu_step_f32(C.byref(cfg), n, C.byref(hv.v), film.ctypes.data, ps32.ctypes.data, pr32.ctypes.data,
+                                 tL.ctypes.data, er.ctypes.data, w.ctypes.data, q_t.ctypes.data, error_sum, threads,
+                                 q.ctypes.data, _p(gs), _p(gr), times.ctypes.data)
+    return dict(q=q, grad_stat=gs, grad_rrs=gr, params_stat=ps32, params_rrs=pr32, times=times, total=total)
