@@ -210,8 +210,18 @@ __global__ __launch_bounds__(64 * P::kWaves) void k_train(TrainArgs t) {
     __shared__ uint32_t grp_cnt[kBinGroup][kMaxSegBins], grp_gbase[kBinGroup][kMaxSegBins];
 
     const int lane = threadIdx.x & 63, h = lane >> 5, col = lane & 31, wave = threadIdx.x >> 6;
-    const op* Wf0 = reinterpret_cast<const op*>(t.w_fwd);
-    const op* Wb0 = reinterpret_cast<const op*>(t.w_bwd);
+    // FP16/BF16: the packed forward and W^T tiles live in LDS behind the stage (they are read
+    // by every layer of every iteration); FP32: from global memory (cache)
+    constexpr int kTilesF = 10 + (NL - 2) * 8 + 4, kTilesB = 4 + (NL - 2) * 8 + 2;
+    op* sW = reinterpret_cast<op*>(smem_raw + sizeof(Stage<P>));
+    if constexpr (kWaves == 8) {
+        const op* gF = reinterpret_cast<const op*>(t.w_fwd);
+        const op* gB = reinterpret_cast<const op*>(t.w_bwd);
+        for (int k = threadIdx.x; k < (kTilesF + kTilesB) * 64; k += kThreads)
+            sW[k] = k < kTilesF * 64 ? gF[k] : gB[k - kTilesF * 64];
+    }
+    const op* Wf0 = kWaves == 8 ? sW : reinterpret_cast<const op*>(t.w_fwd);
+    const op* Wb0 = kWaves == 8 ? sW + kTilesF * 64 : reinterpret_cast<const op*>(t.w_bwd);
     const float S = *t.loss_scale;
     const float ebar = t.error_sum_dev ? fmaxf(*t.error_sum_dev / t.n_pixels_f, 1e-4f) : t.ebar;
     constexpr int kSlots = dw_slots(NL, kWaves);
@@ -224,15 +234,12 @@ __global__ __launch_bounds__(64 * P::kWaves) void k_train(TrainArgs t) {
 
     // zero the padded input rows once (layer 0 has 80 of 96 rows)
     for (int k = threadIdx.x; k < 16 * kStageLd; k += kThreads) st.h[80 * kStageLd + k] = (typename P::elem)0.0f;
+    __syncthreads();  // weight tiles in LDS before the first forward
 
     const uint32_t n_iter = (t.n + kVert - 1) / kVert;
     for (uint32_t it = blockIdx.x; it < n_iter; it += gridDim.x) {
-        // With two waves per SIMD (256 VGPRs) the weight tiles are reloaded (from cache)
-        // in every iteration rather than held in registers across it: the opaque copy of
-        // the pointers stops the compiler from hoisting the loads (about 150 VGPRs).
         const op* Wf = Wf0;
         const op* Wb = Wb0;
-        if constexpr (kWaves == 8) asm volatile("" : "+s"(Wf), "+s"(Wb));
         const uint32_t i = it * kVert + wave * 32 + col;
         const bool live = i < t.n;
         const int vcol = wave * 32 + col;  // vertex column in the stage
@@ -688,7 +695,10 @@ __global__ void k_pack_train(const float* __restrict__ W, int nl, int n_out, E* 
 
 template <typename P, int NL, int NOUT, int LOSS>
 hipError_t launch_train_one(const TrainArgs& t, uint32_t blocks, hipStream_t s) {
-    constexpr size_t smem = sizeof(Stage<P>);
+    constexpr size_t smem =
+        sizeof(Stage<P>) + (P::kWaves == 8 ? (size_t)(10 + (NL - 2) * 8 + 4 + 4 + (NL - 2) * 8 + 2) * 64 *
+                                                 sizeof(typename P::op)
+                                           : 0);
     static bool attr = false;
     if (!attr) {
         hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(&k_train<P, NL, NOUT, LOSS>),
