@@ -258,10 +258,24 @@ nrrs_status ensure_train(nrrs_ctx* c, int net) {
                 rec_total += cap;
             }
         }
-        if (ok && rec_total < 0xFFFFFFFFull) {
+        // slots and slot + list index stay below 2^32
+        if (ok && rec_total < 0xFFFF0000ull) {
             NRRS_TRY(dmalloc(&ns.rec_buf, rec_total));
+            NRRS_TRY(dmalloc(&ns.gdl, (size_t)c->grid.n_levels * c->cfg.max_train));
+            NRRS_TRY(dmalloc(&ns.bin_runs, (size_t)c->grid.n_levels * kMaxSegBins *
+                                               ((c->cfg.max_train + kRunVert - 1) / kRunVert)));
             NRRS_TRY(dmalloc(&ns.seg_count, segs.size()));
             NRRS_HIP(hipMemset(ns.seg_count, 0, segs.size() * 4));
+            {
+                std::vector<uint32_t> end((size_t)c->grid.n_levels * kMaxSegBins, 0u);
+                for (uint32_t l = 0; l < c->grid.n_levels; ++l)
+                    for (uint32_t b = 0; b < ns.seg_nbins[l]; ++b) {
+                        const SegInfo& sg = segs[ns.seg_base[l] + b];
+                        end[(size_t)l * kMaxSegBins + b] = sg.rec_off + sg.cap;
+                    }
+                NRRS_TRY(dmalloc(&ns.seg_end, end.size()));
+                NRRS_HIP(hipMemcpy(ns.seg_end, end.data(), end.size() * 4, hipMemcpyHostToDevice));
+            }
             NRRS_TRY(dmalloc(reinterpret_cast<SegInfo**>(&ns.segs), segs.size()));
             NRRS_HIP(hipMemcpy(ns.segs, segs.data(), segs.size() * sizeof(SegInfo), hipMemcpyHostToDevice));
             ns.n_direct = (uint32_t)direct.size();
@@ -306,6 +320,7 @@ nrrs_status run_backward(nrrs_ctx* c, int net, uint32_t n, const nrrs_vertices& 
                          uint32_t out_stride, float* loss_out, hipStream_t s) {
     NetState& ns = c->net[net];
     const nrrs_config& cfg = c->cfg;
+    if (n > cfg.max_train) return NRRS_ERR_SHAPE;  // per-sample training buffers hold max_train
     TrainArgs t{};
     t.a = encode_args(c, v, film, stats, stats_stride);
     if (cfg.precision != NRRS_FP32) {
@@ -313,7 +328,8 @@ nrrs_status run_backward(nrrs_ctx* c, int net, uint32_t n, const nrrs_vertices& 
         if (!c->grid_enc || n > c->grid_enc_cap) return NRRS_ERR_SHAPE;
         t.a.grid_enc = c->grid_enc;
         t.a.grid_enc_stride = c->grid_enc_cap;
-        NRRS_HIP(launch_encode_levels(cfg.precision, t.a, ns.grid_lp, n, c->grid_enc, c->grid_enc_cap, s));
+        NRRS_HIP(launch_encode_levels(cfg.precision, t.a, ns.grid_lp, n, c->grid_enc, c->grid_enc_cap, s,
+                                      ns.binned ? ns.bin_runs : nullptr));
     }
     t.net = net;
     t.n = n;
@@ -351,13 +367,9 @@ nrrs_status run_backward(nrrs_ctx* c, int net, uint32_t n, const nrrs_vertices& 
         t.dense_f32 = static_cast<float*>(ns.dense_rep);
     else
         t.dense_h2 = static_cast<_Float16 __attribute__((ext_vector_type(2)))*>(ns.dense_rep);
-    t.binned = ns.binned ? 1 : 0;
-    t.rec_buf = ns.rec_buf;
-    t.seg_count = ns.seg_count;
-    t.segs = static_cast<const SegInfo*>(ns.segs);
-    for (int l = 0; l < kMaxLevels; ++l) {
-        t.seg_base[l] = ns.seg_base[l];
-        t.seg_nbins[l] = ns.seg_nbins[l];
+    if (ns.binned) {
+        t.gdl = ns.gdl;
+        t.gdl_stride = cfg.max_train;
     }
     t.dw_partial = ns.dw_partial;
     t.n_mlp = (uint32_t)ns.n_mlp;
@@ -368,6 +380,23 @@ nrrs_status run_backward(nrrs_ctx* c, int net, uint32_t n, const nrrs_vertices& 
     if (!blocks) return NRRS_OK;
     NRRS_HIP(launch_train(cfg.precision, t, ns.n_layers, ns.n_out, kind, blocks, s));
     if (ns.binned) {
+        BinArgs bn{};
+        bn.a = t.a;
+        bn.gdl = ns.gdl;
+        bn.gdl_stride = cfg.max_train;
+        bn.n = n;
+        bn.rec_buf = ns.rec_buf;
+        bn.seg_count = ns.seg_count;
+        bn.runs = ns.bin_runs;
+        bn.n_runs = (n + kRunVert - 1) / kRunVert;
+        bn.seg_end = ns.seg_end;
+        bn.segs = static_cast<const SegInfo*>(ns.segs);
+        for (int l = 0; l < kMaxLevels; ++l) {
+            bn.seg_base[l] = ns.seg_base[l];
+            bn.seg_nbins[l] = ns.seg_nbins[l];
+        }
+        bn.grad = reinterpret_cast<_Float16 __attribute__((ext_vector_type(2)))*>(ns.grad_lp);
+        NRRS_HIP(launch_bin_levels(bn, s));
         ApplyArgs ap{};
         ap.rec = ns.rec_buf;
         ap.seg_count = ns.seg_count;
@@ -540,6 +569,9 @@ void free_ctx(nrrs_ctx* c) {
         hipFree(ns.grad_lp);
         hipFree(ns.dense_rep);
         hipFree(ns.rec_buf);
+        hipFree(ns.gdl);
+        hipFree(ns.bin_runs);
+        hipFree(ns.seg_end);
         hipFree(ns.seg_count);
         hipFree(ns.segs);
         hipFree(ns.direct);

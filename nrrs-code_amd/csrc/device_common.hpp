@@ -423,6 +423,22 @@ __device__ __forceinline__ uint32_t wave_inclusive_scan(uint32_t v) {
     return v;
 }
 
+// Increments LDS counter ctr[b] once for every active lane and returns a distinct value in
+// [old count, new count) to each lane (its rank among the lanes of that counter). When every
+// active lane hits one counter, a single atomic adds them all: per-lane atomics on one LDS
+// address are serialised.
+__device__ __forceinline__ uint32_t lds_count(uint32_t* ctr, uint32_t b) {
+    const uint64_t act = __ballot(1);
+    const uint32_t b0 = __builtin_amdgcn_readfirstlane(b);
+    if (__ballot(b == b0) == act) {
+        const uint32_t lt = __builtin_amdgcn_mbcnt_hi((uint32_t)(act >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)act, 0u));
+        uint32_t old = 0u;
+        if (lt == 0u) old = atomicAdd(&ctr[b0], (uint32_t)__popcll(act));
+        return __builtin_amdgcn_readfirstlane(old) + lt;
+    }
+    return atomicAdd(&ctr[b], 1u);
+}
+
 // Exclusive prefix over the block; *total receives the block sum. Contains a barrier.
 template <int BLOCK>
 __device__ __forceinline__ uint32_t block_exclusive_scan(uint32_t v, uint32_t* total) {

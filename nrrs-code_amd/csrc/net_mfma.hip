@@ -2,6 +2,7 @@
 // (v_mfma_f32_32x32x16_{bf16,f16}, FP32 accumulation). The operand layouts are
 // described in mfma_common.hpp.
 #include "mfma_common.hpp"
+#include "train_common.hpp"
 
 namespace nrrs {
 namespace {
@@ -53,15 +54,38 @@ __global__ void k_to_lp(const float* __restrict__ src, uint16_t* __restrict__ ds
 // Level-major grid encoding: workgroup b takes level b / n_chunks and vertices
 // 256 (b % n_chunks) .. +255, so consecutive workgroups share one level's table.
 template <typename P>
-__global__ __launch_bounds__(256) void k_encode_levels(EncodeArgs a, const uint32_t* __restrict__ table, uint32_t n,
-                                                       uint32_t n_chunks, uint32_t* __restrict__ out,
-                                                       uint32_t stride) {
-    const uint32_t l = blockIdx.x / n_chunks, i = (blockIdx.x % n_chunks) * 256 + threadIdx.x;
-    if (i >= n || l >= a.grid.n_levels) return;
+__global__ __launch_bounds__(kRunVert) void k_encode_levels(EncodeArgs a, const uint32_t* __restrict__ table, uint32_t n,
+                                                            uint32_t n_chunks, uint32_t* __restrict__ out,
+                                                            uint32_t stride, uint32_t* __restrict__ run_counts) {
+    const uint32_t l = blockIdx.x / n_chunks, chunk = blockIdx.x % n_chunks, i = chunk * kRunVert + threadIdx.x;
+    if (l >= a.grid.n_levels) return;  // whole workgroup
+    const bool live = i < n;
     float x[3];
-    load_xn(a, i, x);
-    const float2 f = grid_level_pairs<PairLoadLp<P>>(table, a.grid, (int)l, x);
-    out[(size_t)l * stride + i] = (uint32_t)lp_bits<P>(f.x) | ((uint32_t)lp_bits<P>(f.y) << 16);
+    if (live) {
+        load_xn(a, i, x);
+        const float2 f = grid_level_pairs<PairLoadLp<P>>(table, a.grid, (int)l, x);
+        out[(size_t)l * stride + i] = (uint32_t)lp_bits<P>(f.x) | ((uint32_t)lp_bits<P>(f.y) << 16);
+    }
+    if (run_counts) {  // uniform: corners per table segment of this level and chunk
+        __shared__ uint32_t hist[kMaxSegBins];
+        if (threadIdx.x < kMaxSegBins) hist[threadIdx.x] = 0u;
+        __syncthreads();
+        if (live) {
+            const LevelCell lc = level_cell(a.grid, (int)l, x);
+            const bool dense = (a.grid.dense_mask >> l) & 1u;  // few segments: aggregated atomics
+#pragma unroll
+            for (int cn = 0; cn < 8; ++cn) {
+                const uint32_t sb = corner_index(a.grid, (int)l, lc, cn) >> kSegShift;
+                if (dense)
+                    lds_count(hist, sb);
+                else
+                    atomicAdd(&hist[sb], 1u);
+            }
+        }
+        __syncthreads();
+        if (threadIdx.x < kMaxSegBins)
+            run_counts[((size_t)l * kMaxSegBins + threadIdx.x) * n_chunks + chunk] = hist[threadIdx.x];
+    }
 }
 
 template <typename P, int NL, int NOUT, int EPI>
@@ -160,15 +184,17 @@ hipError_t dispatch(const EncodeArgs& a, int net, int n_layers, int n_out, int e
 }  // namespace
 
 hipError_t launch_encode_levels(int precision, const EncodeArgs& a, const uint16_t* table, uint32_t n, uint32_t* out,
-                                uint32_t stride, hipStream_t s) {
+                                uint32_t stride, hipStream_t s, uint32_t* run_counts) {
     if (!n) return hipSuccess;
-    const uint32_t n_chunks = (n + 255) / 256;
+    const uint32_t n_chunks = (n + kRunVert - 1) / kRunVert;
     const dim3 grid(n_chunks * a.grid.n_levels);
     const uint32_t* t = reinterpret_cast<const uint32_t*>(table);
     if (precision == NRRS_BF16)
-        hipLaunchKernelGGL(k_encode_levels<BF16>, grid, dim3(256), 0, s, a, t, n, n_chunks, out, stride);
+        hipLaunchKernelGGL(k_encode_levels<BF16>, grid, dim3(kRunVert), 0, s, a, t, n, n_chunks, out, stride,
+                           run_counts);
     else if (precision == NRRS_FP16)
-        hipLaunchKernelGGL(k_encode_levels<F16>, grid, dim3(256), 0, s, a, t, n, n_chunks, out, stride);
+        hipLaunchKernelGGL(k_encode_levels<F16>, grid, dim3(kRunVert), 0, s, a, t, n, n_chunks, out, stride,
+                           run_counts);
     else
         return hipErrorInvalidValue;
     return hipGetLastError();

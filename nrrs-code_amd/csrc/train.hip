@@ -2,7 +2,8 @@
 // FP32 (v_mfma_f32_32x32x2_f32), FP16 and BF16 (v_mfma_f32_32x32x16_*) operands
 // with FP32 accumulation.
 //
-// One workgroup = 4 wavefronts = 128 vertices per iteration (persistent loop).
+// One workgroup = kWaves wavefronts = 32 kWaves vertices per iteration (persistent loop):
+// 8 waves and 256 vertices for FP16/BF16, 4 waves and 128 vertices for FP32.
 // Operand layout is the one of the inference kernel: lane l owns vertex l % 32
 // and element j of every 16-wide K chunk with k = 8(l/32) + j; layer outputs are
 // chained without shuffles through the permuted column order
@@ -11,12 +12,13 @@
 // Backward sweep, layer l = L-1 .. 0:
 //   1. recompute the layer input h_l from the encoding (only the encoding and one
 //      layer of activations are ever live: activations are recomputed, not stored);
-//   2. stage delta_l and h_l of the 128 vertices in LDS as [neuron][vertex];
+//   2. stage delta_l and h_l of the workgroup's vertices in LDS as [neuron][vertex];
 //   3. dW_l += delta_l h_l^T on MFMA with the vertices as the contraction index,
 //      each wave owning a fixed subset of the 32x32 dW tiles in its registers for
 //      the whole kernel (written once per workgroup, reduced deterministically);
 //   4. delta_{l-1} = (W_l^T delta_l) * [h_l > 0] on MFMA with pre-packed W^T tiles;
-//      at l = 0 the 32 grid channels of dL/denc are scatter-added into the table.
+//      at l = 0 the 32 grid channels of dL/denc are scatter-added into the table (FP32)
+//      or written out for the binned scatter k_bin_levels (FP16/BF16).
 #include "device_common.hpp"
 #include "train_common.hpp"
 
@@ -96,7 +98,6 @@ struct TF32 {
 };
 
 constexpr uint32_t kDenseReplicas = kDenseGradReplicas;
-constexpr int kBinGroup = 8;  // levels per reservation round of the binned scatter
 constexpr int kStageRowsD = 64;        // delta rows
 constexpr int kStageRowsH = 96;        // input rows (80 padded to 3 x 32)
 
@@ -206,8 +207,6 @@ __global__ __launch_bounds__(64 * P::kWaves) void k_train(TrainArgs t) {
     extern __shared__ __align__(16) unsigned char smem_raw[];
     Stage<P>& st = *reinterpret_cast<Stage<P>*>(smem_raw);
     __shared__ float loss_part[kWaves];  // loss per wave, accumulated with LDS atomics
-    // binned scatter: records per (level of the group, segment), then their reserved bases
-    __shared__ uint32_t grp_cnt[kBinGroup][kMaxSegBins], grp_gbase[kBinGroup][kMaxSegBins];
 
     const int lane = threadIdx.x & 63, h = lane >> 5, col = lane & 31, wave = threadIdx.x >> 6;
     // FP16/BF16: the packed forward and W^T tiles live in LDS behind the stage (they are read
@@ -229,8 +228,6 @@ __global__ __launch_bounds__(64 * P::kWaves) void k_train(TrainArgs t) {
 #pragma unroll
     for (int s = 0; s < kSlots; ++s) acc[s] = f32x16{};
     if (lane == 0) loss_part[wave] = 0.0f;  // by the wave that accumulates into it
-    for (int k = threadIdx.x; k < kBinGroup * (int)kMaxSegBins; k += kThreads)
-        (&grp_cnt[0][0])[k] = 0;  // first used after several barriers
 
     // zero the padded input rows once (layer 0 has 80 of 96 rows)
     for (int k = threadIdx.x; k < 16 * kStageLd; k += kThreads) st.h[80 * kStageLd + k] = (typename P::elem)0.0f;
@@ -473,125 +470,73 @@ __global__ __launch_bounds__(64 * P::kWaves) void k_train(TrainArgs t) {
                 f32x16 G = f32x16{};
 #pragma unroll
                 for (int kb = 0; kb < 4; ++kb) G = P::mma16(load_op<P>(Wb, kb, lane), Dl[kb], G);
-                // Scatter to the table. The level gradients are staged in LDS as
-                // [grid channel][vertex] (the delta rows are free once every wave is past its
-                // dW tiles). Lanes 2k and 2k+1 then take vertex k of this wave's 32, so that the
-                // two lanes of a pair update neighbouring table words in the same atomic
-                // instruction, which the memory pipeline merges into one request:
-                //  - FP16 table: lane parity = x bit of the corner; the x-adjacent corners are
-                //    consecutive entries in a dense level and the aligned pair {h & ~1, h | 1} of
-                //    a hashed level when the cell's x is even;
-                //  - FP32 table: lane parity = feature, both features of a corner are one 8-byte
-                //    entry.
-                __syncthreads();
-                float* gst = reinterpret_cast<float*>(st.d);  // [32][kVert]
+                if (t.gdl) {
+                    // Binned scatter (FP16/BF16): the level pairs of dL/denc go to HBM as
+                    // [level][vertex] float2 and k_bin_levels turns them into records. Lane
+                    // (h, col) holds channels 8(r/4) + 4h + r%4, so registers r and r + 1 (r even)
+                    // are features 0 and 1 of level 4(r/4) + 2h + (r%4)/2.
+                    if (live) {
 #pragma unroll
-                for (int r = 0; r < 16; ++r) gst[(8 * (r >> 2) + 4 * h + (r & 3)) * kVert + vcol] = G[r];
-                // the values a lane reads were written by its own wave
-                __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-                __builtin_amdgcn_wave_barrier();
-                __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-                const int pv = lane >> 1, pb = lane & 1;
-                const uint32_t iv = it * kVert + wave * 32 + pv;
-                const bool live_v = iv < t.n;
-                float x[3] = {0.0f, 0.0f, 0.0f};
-                if (live_v) load_xn(t.a, iv, x);
-                bool binned_done = false;
-                if constexpr (!P::kF32Table) {
-                    if (t.binned) {
-                        // Binned scatter (uniform branch: every thread takes part in the
-                        // barriers), levels in groups of kBinGroup. Pass 1 gives every record
-                        // its rank within (level, segment) with LDS atomics and keeps the ranks
-                        // in LDS; then one thread per (level, segment) reserves the block's run
-                        // in the record buffer with one global atomic, all at once; pass 2
-                        // recomputes each record and writes it at run base + rank.
-                        static_assert(kThreads >= kBinGroup * (int)kMaxSegBins, "one thread per (level, segment)");
-                        static_assert(sizeof(Stage<P>::h) >= kBinGroup * kThreads * 4 * sizeof(uint16_t), "ranks fit");
-                        uint16_t* ranks = reinterpret_cast<uint16_t*>(st.h);  // [kBinGroup][kThreads * 4]
-                        const uint32_t L = t.a.grid.n_levels;
-                        for (uint32_t lg0 = 0; lg0 < L; lg0 += kBinGroup) {
-                            const uint32_t ng = min((uint32_t)kBinGroup, L - lg0);
-#pragma unroll 1
-                            for (uint32_t lg = 0; lg < ng; ++lg) {
-                                if (!live_v) break;
-                                const int l = (int)(lg0 + lg);
-                                const LevelCell lc = level_cell(t.a.grid, l, x);
-#pragma unroll
-                                for (int yz = 0; yz < 4; ++yz) {
-                                    const uint32_t re = corner_index(t.a.grid, l, lc, pb | (yz << 1));  // < level size
-                                    const uint32_t rk = atomicAdd(&grp_cnt[lg][re >> kSegShift], 1u);
-                                    ranks[lg * (kThreads * 4) + threadIdx.x * 4 + yz] = (uint16_t)rk;
-                                }
-                            }
-                            __syncthreads();
-                            {
-                                const uint32_t lg = threadIdx.x / kMaxSegBins, b = threadIdx.x % kMaxSegBins;
-                                if (lg < ng) {
-                                    const uint32_t l = lg0 + lg, c = grp_cnt[lg][b];
-                                    grp_cnt[lg][b] = 0;
-                                    grp_gbase[lg][b] =
-                                        (c && b < t.seg_nbins[l]) ? atomicAdd(t.seg_count + t.seg_base[l] + b, c) : 0u;
-                                }
-                            }
-                            __syncthreads();
-#pragma unroll 1
-                            for (uint32_t lg = 0; lg < ng; ++lg) {
-                                if (!live_v) break;
-                                const int l = (int)(lg0 + lg);
-                                const float g0 = gst[(2 * l) * kVert + wave * 32 + pv];
-                                const float g1 = gst[(2 * l + 1) * kVert + wave * 32 + pv];
-                                const LevelCell lc = level_cell(t.a.grid, l, x);
-#pragma unroll
-                                for (int yz = 0; yz < 4; ++yz) {
-                                    const int cn = pb | (yz << 1);
-                                    const uint32_t re = corner_index(t.a.grid, l, lc, cn), b = re >> kSegShift;
-                                    const float w = corner_weight(lc, cn);
-                                    const h2v gv = {(_Float16)(w * g0), (_Float16)(w * g1)};
-                                    const uint32_t pos =
-                                        grp_gbase[lg][b] + ranks[lg * (kThreads * 4) + threadIdx.x * 4 + yz];
-                                    const SegInfo sg = t.segs[t.seg_base[l] + b];
-                                    if (pos < sg.cap)
-                                        t.rec_buf[(size_t)sg.rec_off + pos] =
-                                            make_uint2(re, __builtin_bit_cast(uint32_t, gv));
-                                    else  // segment full (clustered input): add directly
-                                        __builtin_amdgcn_global_atomic_fadd_v2f16(
-                                            t.grad_grid_h2 + t.a.grid.offset[l] + re, gv);
-                                }
-                            }
-                            __syncthreads();  // ranks and bases are reused by the next group
+                        for (int r = 0; r < 16; r += 2) {
+                            const uint32_t l = 4 * (r >> 2) + 2 * h + ((r & 3) >> 1);
+                            if (l < t.a.grid.n_levels)
+                                t.gdl[(size_t)l * t.gdl_stride + i] = make_float2(G[r], G[r + 1]);
                         }
-                        binned_done = true;
                     }
-                }
+                } else {
+                    // Direct scatter to the table. The level gradients are staged in LDS as
+                    // [grid channel][vertex] (the delta rows are free once every wave is past its
+                    // dW tiles). Lanes 2k and 2k+1 then take vertex k of this wave's 32, so that the
+                    // two lanes of a pair update neighbouring table words in the same atomic
+                    // instruction, which the memory pipeline merges into one request:
+                    //  - FP16 table: lane parity = x bit of the corner; the x-adjacent corners are
+                    //    consecutive entries in a dense level and the aligned pair {h & ~1, h | 1} of
+                    //    a hashed level when the cell's x is even;
+                    //  - FP32 table: lane parity = feature, both features of a corner are one 8-byte
+                    //    entry.
+                    __syncthreads();
+                    float* gst = reinterpret_cast<float*>(st.d);  // [32][kVert]
+#pragma unroll
+                    for (int r = 0; r < 16; ++r) gst[(8 * (r >> 2) + 4 * h + (r & 3)) * kVert + vcol] = G[r];
+                    // the values a lane reads were written by its own wave
+                    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+                    __builtin_amdgcn_wave_barrier();
+                    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+                    const int pv = lane >> 1, pb = lane & 1;
+                    const uint32_t iv = it * kVert + wave * 32 + pv;
+                    const bool live_v = iv < t.n;
+                    float x[3] = {0.0f, 0.0f, 0.0f};
+                    if (live_v) load_xn(t.a, iv, x);
 #pragma unroll 1
-                for (int l = 0; l < (int)t.a.grid.n_levels && !binned_done; ++l) {
-                    const float g0 = gst[(2 * l) * kVert + wave * 32 + pv];
-                    const float g1 = gst[(2 * l + 1) * kVert + wave * 32 + pv];
-                    const bool dense = (t.a.grid.dense_mask >> l) & 1u;
-                    if (!live_v) continue;
-                    LevelCell lc = level_cell(t.a.grid, l, x);
-                    // Dense (coarse) levels are small and hit by every vertex: their updates go
-                    // to one of kDenseReplicas private copies (chosen by workgroup), summed before
-                    // the optimiser step, which spreads the contention on each entry.
-                    const size_t rep = dense ? (size_t)(blockIdx.x % kDenseReplicas) * t.n_dense : 0;
-                    if (t.grad_grid_h2) {
+                    for (int l = 0; l < (int)t.a.grid.n_levels; ++l) {
+                        const float g0 = gst[(2 * l) * kVert + wave * 32 + pv];
+                        const float g1 = gst[(2 * l + 1) * kVert + wave * 32 + pv];
+                        const bool dense = (t.a.grid.dense_mask >> l) & 1u;
+                        if (!live_v) continue;
+                        LevelCell lc = level_cell(t.a.grid, l, x);
+                        // Dense (coarse) levels are small and hit by every vertex: their updates go
+                        // to one of kDenseReplicas private copies (chosen by workgroup), summed before
+                        // the optimiser step, which spreads the contention on each entry.
+                        const size_t rep = dense ? (size_t)(blockIdx.x % kDenseReplicas) * t.n_dense : 0;
+                        if (t.grad_grid_h2) {
 #pragma unroll
-                        for (int yz = 0; yz < 4; ++yz) {
-                            const int cn = pb | (yz << 1);
-                            const uint32_t ent = t.a.grid.offset[l] + corner_index(t.a.grid, l, lc, cn);
-                            const float w = corner_weight(lc, cn);
-                            h2v gv = {(_Float16)(w * g0), (_Float16)(w * g1)};  // one packed FP16 atomic
-                            h2v* dst = dense ? t.dense_h2 + rep + ent : t.grad_grid_h2 + ent;
-                            __builtin_amdgcn_global_atomic_fadd_v2f16(dst, gv);
-                        }
-                    } else {
-                        const float g = pb ? g1 : g0;
+                            for (int yz = 0; yz < 4; ++yz) {
+                                const int cn = pb | (yz << 1);
+                                const uint32_t ent = t.a.grid.offset[l] + corner_index(t.a.grid, l, lc, cn);
+                                const float w = corner_weight(lc, cn);
+                                h2v gv = {(_Float16)(w * g0), (_Float16)(w * g1)};  // one packed FP16 atomic
+                                h2v* dst = dense ? t.dense_h2 + rep + ent : t.grad_grid_h2 + ent;
+                                __builtin_amdgcn_global_atomic_fadd_v2f16(dst, gv);
+                            }
+                        } else {
+                            const float g = pb ? g1 : g0;
 #pragma unroll
-                        for (int cn = 0; cn < 8; ++cn) {
-                            const uint32_t ent = t.a.grid.offset[l] + corner_index(t.a.grid, l, lc, cn);
-                            const float w = corner_weight(lc, cn);
-                            float* dst = dense ? t.dense_f32 + 2 * (rep + ent) : t.grad_grid + 2 * (size_t)ent;
-                            atomicAdd(dst + pb, w * g);
+                            for (int cn = 0; cn < 8; ++cn) {
+                                const uint32_t ent = t.a.grid.offset[l] + corner_index(t.a.grid, l, lc, cn);
+                                const float w = corner_weight(lc, cn);
+                                float* dst = dense ? t.dense_f32 + 2 * (rep + ent) : t.grad_grid + 2 * (size_t)ent;
+                                atomicAdd(dst + pb, w * g);
+                            }
                         }
                     }
                 }
@@ -729,6 +674,117 @@ hipError_t dispatch_train(const TrainArgs& t, int nl, int n_out, int loss, uint3
         if (loss == NRRS_LOSS_RRS) return launch_train_one<P, 4, 1, NRRS_LOSS_RRS>(t, blocks, s);
     }
     return hipErrorInvalidValue;
+}
+
+// ---------------------------------------------------------------- binned scatter
+// The encode pass of the backward counted the corners of every (level, segment, kRunVert
+// vertices). k_bin_scan (one workgroup per segment) turns these counts into the first record
+// slot of every chunk's run, in chunk order, and the segment's total; the slots are clamped to
+// the segment's end, so that a full segment (clustered input) sends the rest to direct atomics.
+constexpr int kScanT = 256, kScanPer = 4;
+
+__global__ __launch_bounds__(kScanT) void k_bin_scan(BinArgs b) {
+    const uint32_t l = blockIdx.x / kMaxSegBins, sb = blockIdx.x % kMaxSegBins;
+    if (l >= b.a.grid.n_levels || sb >= b.seg_nbins[l]) return;  // whole workgroup
+    const uint32_t seg = b.seg_base[l] + sb;
+    const SegInfo sg = b.segs[seg];
+    const uint64_t end = (uint64_t)sg.rec_off + sg.cap;
+    uint32_t* r = b.runs + ((size_t)l * kMaxSegBins + sb) * b.n_runs;
+    uint64_t carry = sg.rec_off;
+    for (uint32_t c0 = 0; c0 < b.n_runs; c0 += kScanT * kScanPer) {
+        const uint32_t c = c0 + threadIdx.x * kScanPer;
+        uint32_t v[kScanPer], sum = 0u;
+#pragma unroll
+        for (int j = 0; j < kScanPer; ++j) {
+            v[j] = c + j < b.n_runs ? r[c + j] : 0u;
+            sum += v[j];
+        }
+        uint32_t tot;
+        uint64_t run = carry + block_exclusive_scan<kScanT>(sum, &tot);
+#pragma unroll
+        for (int j = 0; j < kScanPer; ++j) {
+            if (c + j < b.n_runs) r[c + j] = (uint32_t)(run < end ? run : end);
+            run += v[j];
+        }
+        carry += tot;
+        __syncthreads();  // the scan's LDS is reused
+    }
+    if (threadIdx.x == 0) b.seg_count[seg] = (uint32_t)(carry - sg.rec_off);
+}
+
+// k_bin_levels: workgroup b takes level b / chunks and kBinVert vertices (two runs of the scan),
+// so the GPU walks the levels one after the other. Each thread takes one vertex and its 8
+// corners:
+//  1. every record gets its rank within its segment from LDS atomics;
+//  2. wave 0, lane s = segment s: an exclusive scan of the counts places the segment's records
+//     in the workgroup's segment-ordered list; the workgroup's slots of segment s start at the
+//     first slot of its first run;
+//  3. each record {entry in level, FP16 pair w * dL/denc} is put into the LDS list at
+//     (segment start + rank);
+//  4. the list is copied out with consecutive threads on consecutive records, so the stores of
+//     a wave form a few contiguous runs rather than one cache line per lane.
+// Records beyond a full segment's slots are added directly to the table gradient with packed
+// FP16 atomics.
+constexpr int kBinVert = 2 * kRunVert;
+
+__global__ __launch_bounds__(kBinVert) void k_bin_levels(BinArgs b, uint32_t n_chunks) {
+    static_assert(kMaxSegBins == 64, "one lane per segment");
+    __shared__ uint32_t cnt[kMaxSegBins], start[kMaxSegBins], dst[kMaxSegBins], lim[kMaxSegBins], tot;
+    __shared__ uint2 list[kBinVert * 8];
+    const uint32_t l = blockIdx.x / n_chunks, chunk = blockIdx.x % n_chunks, i = chunk * kBinVert + threadIdx.x;
+    if (l >= b.a.grid.n_levels) return;  // whole workgroup
+    uint32_t first = 0u, end = 0u;       // wave 0: this workgroup's first slot and the segment end
+    if (threadIdx.x < kMaxSegBins) {
+        cnt[threadIdx.x] = 0u;
+        first = b.runs[((size_t)l * kMaxSegBins + threadIdx.x) * b.n_runs + chunk * (kBinVert / kRunVert)];
+        end = b.seg_end[l * kMaxSegBins + threadIdx.x];
+    }
+    __syncthreads();
+    const bool live = i < b.n;
+    // dense (coarse) levels have few segments, often one per wave: aggregated LDS atomics
+    const bool dense = (b.a.grid.dense_mask >> l) & 1u;
+    uint32_t re[8], rk[8];
+    LevelCell lc{};
+    float2 g = make_float2(0.0f, 0.0f);
+    if (live) {
+        float x[3];
+        load_xn(b.a, i, x);
+        lc = level_cell(b.a.grid, (int)l, x);
+        g = b.gdl[(size_t)l * b.gdl_stride + i];
+#pragma unroll
+        for (int cn = 0; cn < 8; ++cn) {
+            re[cn] = corner_index(b.a.grid, (int)l, lc, cn);  // < level size
+            rk[cn] = dense ? lds_count(cnt, re[cn] >> kSegShift) : atomicAdd(&cnt[re[cn] >> kSegShift], 1u);
+        }
+    }
+    __syncthreads();
+    if (threadIdx.x < kMaxSegBins) {
+        const uint32_t sb = threadIdx.x, c = cnt[sb];
+        const uint32_t inc = wave_inclusive_scan(c);
+        start[sb] = inc - c;
+        dst[sb] = first - (inc - c);  // list entry k of this segment goes to slot dst + k (mod 2^32)
+        lim[sb] = end;
+        if (sb == kMaxSegBins - 1) tot = inc;
+    }
+    __syncthreads();
+    if (live) {
+#pragma unroll
+        for (int cn = 0; cn < 8; ++cn) {
+            const float w = corner_weight(lc, cn);
+            const h2v gv = {(_Float16)(w * g.x), (_Float16)(w * g.y)};
+            list[start[re[cn] >> kSegShift] + rk[cn]] = make_uint2(re[cn], __builtin_bit_cast(uint32_t, gv));
+        }
+    }
+    __syncthreads();
+    const uint32_t n_rec = tot;
+    for (uint32_t k = threadIdx.x; k < n_rec; k += kBinVert) {
+        const uint2 r = list[k];
+        const uint32_t sb = r.x >> kSegShift, d = dst[sb] + k;
+        if (d < lim[sb])
+            b.rec_buf[d] = r;
+        else  // segment full: add directly
+            __builtin_amdgcn_global_atomic_fadd_v2f16(b.grad + b.a.grid.offset[l] + r.x, __builtin_bit_cast(h2v, r.y));
+    }
 }
 
 // Sums of the binned records. Every FP16 value is an integer multiple of 2^-24 below 2^16, so
@@ -951,6 +1007,17 @@ hipError_t launch_apply_bins(const ApplyArgs& a, uint32_t n_direct, uint32_t n_c
         const uint32_t blocks = std::min<uint32_t>((a.n_acc + 255) / 256, 2048);
         hipLaunchKernelGGL(k_apply_finish, dim3(blocks), dim3(256), 0, s, a);
     }
+    return hipGetLastError();
+}
+
+hipError_t launch_bin_levels(const BinArgs& b, hipStream_t s) {
+    if (!b.n) return hipSuccess;
+    if (b.n_runs != (b.n + kRunVert - 1) / kRunVert) return hipErrorInvalidValue;
+    const uint32_t n_chunks = (b.n + kBinVert - 1) / kBinVert;
+    const uint64_t blocks = (uint64_t)n_chunks * b.a.grid.n_levels;
+    if (blocks > 0x7FFFFFFFull) return hipErrorInvalidValue;
+    hipLaunchKernelGGL(k_bin_scan, dim3(b.a.grid.n_levels * kMaxSegBins), dim3(kScanT), 0, s, b);
+    hipLaunchKernelGGL(k_bin_levels, dim3((uint32_t)blocks), dim3(kBinVert), 0, s, b, n_chunks);
     return hipGetLastError();
 }
 

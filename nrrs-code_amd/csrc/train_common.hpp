@@ -12,6 +12,9 @@ constexpr uint32_t kDenseGradReplicas = 16;
 constexpr uint32_t kSegShift = 13;
 constexpr uint32_t kSegEntries = 1u << kSegShift;
 constexpr uint32_t kMaxSegBins = 64;
+// The encode pass counts the records of every (level, segment, kRunVert vertices); a scan of
+// these counts gives each run its first record slot.
+constexpr uint32_t kRunVert = 256;
 // Records one workgroup of the chunked apply sums (segments of the dense levels, where one
 // segment can receive several records per vertex).
 constexpr uint32_t kApplyChunk = 1u << 17;
@@ -70,15 +73,11 @@ struct TrainArgs {
     float* dense_f32;              // [replicas][n_dense][2] (FP32 contexts)
     _Float16 __attribute__((ext_vector_type(2)))* dense_h2;  // [replicas][n_dense] (FP16/BF16)
     uint32_t n_dense;              // entries of the dense levels (they come first in the table)
-    // Binned scatter (FP16/BF16 contexts): the kernel appends {entry within the level, FP16
-    // pair} records to the segments of each level and the apply kernels sum them into the
-    // table gradient. binned == 0 selects direct atomics (and dense-level replicas) instead.
-    int binned;
-    uint2* rec_buf;
-    uint32_t* seg_count;
-    const SegInfo* segs;
-    uint32_t seg_base[kMaxLevels];   // first segment of each level
-    uint32_t seg_nbins[kMaxLevels];  // segments of each level
+    // Binned scatter (FP16/BF16 contexts): the kernel writes dL/denc of the grid as
+    // [level][vertex] pairs (row stride gdl_stride) for launch_bin_levels instead of adding it
+    // to the table. Null selects direct atomics (and dense-level replicas).
+    float2* gdl;
+    uint32_t gdl_stride;
     float* dw_partial;             // [gridDim.x][n_mlp]
     uint32_t n_mlp;
     float* loss_out;               // device scalar (atomic add), or null
@@ -90,6 +89,26 @@ size_t train_op_bytes(int precision);
 hipError_t launch_pack_train(int precision, const float* W, int nl, int n_out, void* fwd, void* bwd, hipStream_t s);
 hipError_t launch_train(int precision, const TrainArgs& t, int nl, int n_out, int loss, uint32_t blocks,
                         hipStream_t s);
+struct BinArgs {
+    EncodeArgs a;                  // vertices and grid (positions of the records' corners)
+    const float2* gdl;             // [level][gdl_stride] dL/denc pairs written by the training kernel
+    uint32_t gdl_stride;
+    uint32_t n;
+    uint2* rec_buf;                // records {entry in level, FP16 pair}
+    uint32_t* seg_count;           // records per segment (written by the scan)
+    uint32_t* runs;                // [level][kMaxSegBins][n_runs]: record counts of the kRunVert-vertex
+                                   // chunks from the encode pass, replaced by their first slots
+    uint32_t n_runs;               // chunks: ceil(n / kRunVert)
+    const uint32_t* seg_end;       // [level][kMaxSegBins] one past the last slot of each segment
+    const SegInfo* segs;
+    uint32_t seg_base[kMaxLevels];   // first segment of each level
+    uint32_t seg_nbins[kMaxLevels];  // segments of each level
+    _Float16 __attribute__((ext_vector_type(2)))* grad;  // FP16 pair table gradient (full segments)
+};
+// Scans the run counts into first slots and segment totals, then writes the records
+// {entry, FP16 w * dL/denc} of every (vertex, level, corner) into the slots of their segment,
+// walking the levels one after the other.
+hipError_t launch_bin_levels(const BinArgs& b, hipStream_t s);
 // Sums the binned records into the table gradient: n_direct segments with one workgroup
 // each, n_chunks chunks of the other segments, then their fixed-point sums.
 hipError_t launch_apply_bins(const ApplyArgs& a, uint32_t n_direct, uint32_t n_chunks, hipStream_t s);
