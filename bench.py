@@ -32,9 +32,10 @@ SEED = 0x4E525253 ^ 2
 HBM_PEAK_GBS = 8000.0
 BF16_PEAK_TFLOPS = 2500.0
 # HBM bytes of one StatNet training backward at 2^20 samples (k_encode_levels + k_train +
-# k_apply_bins + k_apply_chunks + k_apply_finish + k_reduce_partials), from rocprofv3 --pmc
-# FETCH_SIZE and WRITE_SIZE (profiles/train_backward_bf16_2p20_pmc.md), or None.
-TRAIN_TRAFFIC_BYTES = 2973085909
+# k_bin_scan + k_bin_levels + k_apply_bins + k_apply_chunks + k_apply_finish +
+# k_reduce_partials), from rocprofv3 --pmc FETCH_SIZE and WRITE_SIZE
+# (profiles/train_backward_bf16_2p20_pmc.md), or None when not measured.
+TRAIN_TRAFFIC_BYTES = 3468319893
 MACS = {"statnet": 80 * 64 + 3 * 64 * 64 + 64 * 6, "rrsnet": 80 * 64 + 2 * 64 * 64 + 64 * 1}
 
 
@@ -381,9 +382,11 @@ def main():
     bf = results["bf16"]
     n = N_VERTICES
     # Dominant operation of the BF16 step: the StatNet training backward (nrrs_backward with
-    # NRRS_LL2): k_encode_levels (level-major grid gathers), k_train (forward with recompute,
-    # loss, backward, dW, binned table-gradient records), k_apply_bins / k_apply_chunks /
-    # k_apply_finish (sum the records into the table gradient) and k_reduce_partials (dW). Algorithmic bytes per sample: input record 64 B +
+    # NRRS_LL2): k_encode_levels (level-major grid gathers, record counts per segment), k_train
+    # (forward with recompute, loss, backward, dW, dL/denc per level), k_bin_scan and
+    # k_bin_levels (table-gradient records per segment), k_apply_bins / k_apply_chunks /
+    # k_apply_finish (sum the records into the table gradient) and k_reduce_partials (dW).
+    # Algorithmic bytes per sample: input record 64 B +
     # target 12 B, 16 levels x 8 corners x 4 B of BF16 feature pairs gathered, and the same
     # 512 B of FP16 gradient pairs added to the table.
     bytes_s = 64 + 12 + 16 * 8 * 4 + 16 * 8 * 4
@@ -394,8 +397,8 @@ def main():
     roofline = {"bound": "hbm", "achieved": round(gbs, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": round(gbs / HBM_PEAK_GBS, 4), "traffic": TRAIN_TRAFFIC_BYTES,
                 "kernel": "nrrs_backward(StatNet, NRRS_LL2) at 2^20 samples: k_encode_levels + "
-                          "k_train<TBF16,5,6,NRRS_LOSS_LL2> + k_apply_bins + k_apply_chunks + k_apply_finish + "
-                          "k_reduce_partials",
+                          "k_train<TBF16,5,6,NRRS_LOSS_LL2> + k_bin_scan + k_bin_levels + k_apply_bins + "
+                          "k_apply_chunks + k_apply_finish + k_reduce_partials",
                 "kernel_us": round(bf["train_kernel_us"], 1),
                 "algorithmic_bytes_per_launch": bytes_s * n,
                 "mfma_tflops": round(tflops, 2), "mfma_frac": round(tflops / BF16_PEAK_TFLOPS, 5),

@@ -262,8 +262,10 @@ nrrs_status ensure_train(nrrs_ctx* c, int net) {
         if (ok && rec_total < 0xFFFF0000ull) {
             NRRS_TRY(dmalloc(&ns.rec_buf, rec_total));
             NRRS_TRY(dmalloc(&ns.gdl, (size_t)c->grid.n_levels * c->cfg.max_train));
-            NRRS_TRY(dmalloc(&ns.bin_runs, (size_t)c->grid.n_levels * kMaxSegBins *
-                                               ((c->cfg.max_train + kRunVert - 1) / kRunVert)));
+            const uint32_t max_runs = (c->cfg.max_train + kRunVert - 1) / kRunVert;
+            NRRS_TRY(dmalloc(&ns.bin_runs, (size_t)c->grid.n_levels * kMaxSegBins * max_runs));
+            ns.bin_max_tiles = (max_runs + kBinScanRows - 1) / kBinScanRows;
+            NRRS_TRY(dmalloc(&ns.bin_tiles, (size_t)c->grid.n_levels * kMaxSegBins * ns.bin_max_tiles));
             NRRS_TRY(dmalloc(&ns.seg_count, segs.size()));
             NRRS_HIP(hipMemset(ns.seg_count, 0, segs.size() * 4));
             {
@@ -390,6 +392,8 @@ nrrs_status run_backward(nrrs_ctx* c, int net, uint32_t n, const nrrs_vertices& 
         bn.runs = ns.bin_runs;
         bn.n_runs = (n + kRunVert - 1) / kRunVert;
         bn.seg_end = ns.seg_end;
+        bn.tile_sums = ns.bin_tiles;
+        bn.max_tiles = ns.bin_max_tiles;
         bn.segs = static_cast<const SegInfo*>(ns.segs);
         for (int l = 0; l < kMaxLevels; ++l) {
             bn.seg_base[l] = ns.seg_base[l];
@@ -571,6 +575,7 @@ void free_ctx(nrrs_ctx* c) {
         hipFree(ns.rec_buf);
         hipFree(ns.gdl);
         hipFree(ns.bin_runs);
+        hipFree(ns.bin_tiles);
         hipFree(ns.seg_end);
         hipFree(ns.seg_count);
         hipFree(ns.segs);

@@ -102,7 +102,9 @@ struct NetState {
     bool binned = false;
     uint2* rec_buf = nullptr;        // gradient records
     float2* gdl = nullptr;           // [level][max_train] dL/denc pairs of the training kernel
-    uint32_t* bin_runs = nullptr;    // [level][segment][chunk] run counts / first slots
+    uint32_t* bin_runs = nullptr;    // [level][chunk][segment] run counts / first slots
+    uint32_t* bin_tiles = nullptr;   // [level][tile][segment] scan scratch
+    uint32_t bin_max_tiles = 0;
     uint32_t* seg_end = nullptr;     // [level][kMaxSegBins] end slot of every segment (0: none)
     uint32_t* seg_count = nullptr;   // records per segment
     void* segs = nullptr;            // SegInfo per segment
@@ -179,7 +181,7 @@ hipError_t launch_pack_lp(int precision, const float* grid, uint64_t n_grid, con
 // Level-major grid encoding: one workgroup per (level, 256 vertices), levels in order, so the
 // whole GPU gathers from one level's table at a time (it stays in L2). With run_counts, also
 // counts the corners of every (level, table segment, 256 vertices) for the binned scatter:
-// run_counts[(level * kMaxSegBins + segment) * ceil(n / 256) + chunk].
+// run_counts[(level * ceil(n / 256) + chunk) * kMaxSegBins + segment].
 hipError_t launch_encode_levels(int precision, const EncodeArgs& a, const uint16_t* table, uint32_t n, uint32_t* out,
                                 uint32_t stride, hipStream_t s, uint32_t* run_counts = nullptr);
 hipError_t launch_net_mfma(int precision, const EncodeArgs& a, int net, int n_layers, int n_out, int epi,

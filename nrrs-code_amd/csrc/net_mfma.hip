@@ -84,7 +84,7 @@ __global__ __launch_bounds__(kRunVert) void k_encode_levels(EncodeArgs a, const 
         }
         __syncthreads();
         if (threadIdx.x < kMaxSegBins)
-            run_counts[((size_t)l * kMaxSegBins + threadIdx.x) * n_chunks + chunk] = hist[threadIdx.x];
+            run_counts[((size_t)l * n_chunks + chunk) * kMaxSegBins + threadIdx.x] = hist[threadIdx.x];
     }
 }
 

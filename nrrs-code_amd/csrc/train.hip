@@ -199,8 +199,11 @@ __device__ __forceinline__ uint32_t recompute(const typename P::op* __restrict__
     return mask;
 }
 
-template <typename P, int NL, int NOUT, int LOSS>
+// BIN: dL/denc of the grid goes to t.gdl for the binned scatter (FP16/BF16); otherwise it is
+// added to the table with atomics. Separate instances keep the unused path's registers out.
+template <typename P, int NL, int NOUT, int LOSS, bool BIN>
 __global__ __launch_bounds__(64 * P::kWaves) void k_train(TrainArgs t) {
+    static_assert(!(BIN && P::kF32Table), "the binned scatter is for FP16/BF16 tables");
     typedef typename P::op op;
     constexpr int kWaves = Geo<P>::kWaves, kThreads = Geo<P>::kThreads, kVert = Geo<P>::kVert;
     constexpr int kStageLd = Geo<P>::kStageLd;
@@ -470,7 +473,7 @@ __global__ __launch_bounds__(64 * P::kWaves) void k_train(TrainArgs t) {
                 f32x16 G = f32x16{};
 #pragma unroll
                 for (int kb = 0; kb < 4; ++kb) G = P::mma16(load_op<P>(Wb, kb, lane), Dl[kb], G);
-                if (t.gdl) {
+                if constexpr (BIN) {
                     // Binned scatter (FP16/BF16): the level pairs of dL/denc go to HBM as
                     // [level][vertex] float2 and k_bin_levels turns them into records. Lane
                     // (h, col) holds channels 8(r/4) + 4h + r%4, so registers r and r + 1 (r even)
@@ -638,21 +641,32 @@ __global__ void k_pack_train(const float* __restrict__ W, int nl, int n_out, E* 
     }
 }
 
-template <typename P, int NL, int NOUT, int LOSS>
-hipError_t launch_train_one(const TrainArgs& t, uint32_t blocks, hipStream_t s) {
+template <typename P, int NL, int NOUT, int LOSS, bool BIN>
+hipError_t launch_train_one_bin(const TrainArgs& t, uint32_t blocks, hipStream_t s) {
     constexpr size_t smem =
         sizeof(Stage<P>) + (P::kWaves == 8 ? (size_t)(10 + (NL - 2) * 8 + 4 + 4 + (NL - 2) * 8 + 2) * 64 *
                                                  sizeof(typename P::op)
                                            : 0);
     static bool attr = false;
     if (!attr) {
-        hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(&k_train<P, NL, NOUT, LOSS>),
+        hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(&k_train<P, NL, NOUT, LOSS, BIN>),
                                            hipFuncAttributeMaxDynamicSharedMemorySize, (int)smem);
         if (e != hipSuccess) return e;
         attr = true;
     }
-    hipLaunchKernelGGL((k_train<P, NL, NOUT, LOSS>), dim3(blocks), dim3(Geo<P>::kThreads), smem, s, t);
+    hipLaunchKernelGGL((k_train<P, NL, NOUT, LOSS, BIN>), dim3(blocks), dim3(Geo<P>::kThreads), smem, s, t);
     return hipGetLastError();
+}
+
+template <typename P, int NL, int NOUT, int LOSS>
+hipError_t launch_train_one(const TrainArgs& t, uint32_t blocks, hipStream_t s) {
+    if constexpr (P::kF32Table) {
+        if (t.gdl) return hipErrorInvalidValue;
+        return launch_train_one_bin<P, NL, NOUT, LOSS, false>(t, blocks, s);
+    } else {
+        return t.gdl ? launch_train_one_bin<P, NL, NOUT, LOSS, true>(t, blocks, s)
+                     : launch_train_one_bin<P, NL, NOUT, LOSS, false>(t, blocks, s);
+    }
 }
 
 }  // namespace
@@ -677,39 +691,69 @@ hipError_t dispatch_train(const TrainArgs& t, int nl, int n_out, int loss, uint3
 }
 
 // ---------------------------------------------------------------- binned scatter
-// The encode pass of the backward counted the corners of every (level, segment, kRunVert
-// vertices). k_bin_scan (one workgroup per segment) turns these counts into the first record
-// slot of every chunk's run, in chunk order, and the segment's total; the slots are clamped to
-// the segment's end, so that a full segment (clustered input) sends the rest to direct atomics.
-constexpr int kScanT = 256, kScanPer = 4;
+// The encode pass of the backward counted the corners of every (level, kRunVert vertices,
+// segment), one 256-byte row of counts per chunk. Two kernels turn these counts into the first
+// record slot of every chunk's run (chunk order within each segment) and each segment's total,
+// over tiles of kScanRows chunks (wave g of a workgroup takes rows g * kScanRowsPerWave.., lane
+// s segment s): k_bin_tiles sums each tile; k_bin_scan adds the sums of the level's earlier
+// tiles, scans its own rows and writes the running slots. Slots are clamped to the segment's
+// end, so that a full segment (clustered input) sends the rest of its records to direct
+// atomics.
+constexpr int kScanWaves = 16, kScanRowsPerWave = 16, kScanRows = kScanWaves * kScanRowsPerWave;
+static_assert(kScanRows == (int)kBinScanRows, "tile size");
 
-__global__ __launch_bounds__(kScanT) void k_bin_scan(BinArgs b) {
-    const uint32_t l = blockIdx.x / kMaxSegBins, sb = blockIdx.x % kMaxSegBins;
-    if (l >= b.a.grid.n_levels || sb >= b.seg_nbins[l]) return;  // whole workgroup
-    const uint32_t seg = b.seg_base[l] + sb;
+__global__ __launch_bounds__(64 * kScanWaves) void k_bin_tiles(BinArgs b, uint32_t n_tiles) {
+    __shared__ uint32_t part[kScanWaves][kMaxSegBins];
+    const uint32_t l = blockIdx.x / n_tiles, tile = blockIdx.x % n_tiles, s = threadIdx.x & 63u, w = threadIdx.x >> 6;
+    const uint32_t r0 = tile * kScanRows + w * kScanRowsPerWave;
+    const uint32_t* r = b.runs + ((size_t)l * b.n_runs) * kMaxSegBins + s;
+    uint32_t sum = 0u;
+#pragma unroll
+    for (int k = 0; k < kScanRowsPerWave; ++k)
+        if (r0 + k < b.n_runs) sum += r[(size_t)(r0 + k) * kMaxSegBins];
+    part[w][s] = sum;
+    __syncthreads();
+    if (w == 0) {
+        uint32_t tot = 0u;
+#pragma unroll
+        for (int k = 0; k < kScanWaves; ++k) tot += part[k][s];
+        b.tile_sums[((size_t)l * n_tiles + tile) * kMaxSegBins + s] = tot;
+    }
+}
+
+__global__ __launch_bounds__(64 * kScanWaves) void k_bin_scan(BinArgs b, uint32_t n_tiles) {
+    static_assert(kMaxSegBins == 64, "one lane per segment");
+    __shared__ uint32_t part[kScanWaves][kMaxSegBins];
+    const uint32_t l = blockIdx.x / n_tiles, tile = blockIdx.x % n_tiles, s = threadIdx.x & 63u, w = threadIdx.x >> 6;
+    const uint32_t r0 = tile * kScanRows + w * kScanRowsPerWave;
+    uint32_t* r = b.runs + ((size_t)l * b.n_runs) * kMaxSegBins + s;
+    uint32_t v[kScanRowsPerWave], sum = 0u;
+#pragma unroll
+    for (int k = 0; k < kScanRowsPerWave; ++k) {
+        v[k] = r0 + k < b.n_runs ? r[(size_t)(r0 + k) * kMaxSegBins] : 0u;
+        sum += v[k];
+    }
+    uint64_t base = 0u;  // records of this segment in the level's earlier tiles
+    for (uint32_t t = 0; t < tile; ++t) base += b.tile_sums[((size_t)l * n_tiles + t) * kMaxSegBins + s];
+    part[w][s] = sum;
+    __syncthreads();
+    uint64_t before = base, total = base;
+#pragma unroll
+    for (int k = 0; k < kScanWaves; ++k) {
+        before += k < (int)w ? part[k][s] : 0u;
+        total += part[k][s];
+    }
+    if (s >= b.seg_nbins[l]) return;  // rows of segments the level does not have stay zero
+    const uint32_t seg = b.seg_base[l] + s;
     const SegInfo sg = b.segs[seg];
     const uint64_t end = (uint64_t)sg.rec_off + sg.cap;
-    uint32_t* r = b.runs + ((size_t)l * kMaxSegBins + sb) * b.n_runs;
-    uint64_t carry = sg.rec_off;
-    for (uint32_t c0 = 0; c0 < b.n_runs; c0 += kScanT * kScanPer) {
-        const uint32_t c = c0 + threadIdx.x * kScanPer;
-        uint32_t v[kScanPer], sum = 0u;
+    uint64_t run = sg.rec_off + before;
 #pragma unroll
-        for (int j = 0; j < kScanPer; ++j) {
-            v[j] = c + j < b.n_runs ? r[c + j] : 0u;
-            sum += v[j];
-        }
-        uint32_t tot;
-        uint64_t run = carry + block_exclusive_scan<kScanT>(sum, &tot);
-#pragma unroll
-        for (int j = 0; j < kScanPer; ++j) {
-            if (c + j < b.n_runs) r[c + j] = (uint32_t)(run < end ? run : end);
-            run += v[j];
-        }
-        carry += tot;
-        __syncthreads();  // the scan's LDS is reused
+    for (int k = 0; k < kScanRowsPerWave; ++k) {
+        if (r0 + k < b.n_runs) r[(size_t)(r0 + k) * kMaxSegBins] = (uint32_t)(run < end ? run : end);
+        run += v[k];
     }
-    if (threadIdx.x == 0) b.seg_count[seg] = (uint32_t)(carry - sg.rec_off);
+    if (w == 0 && tile == n_tiles - 1) b.seg_count[seg] = (uint32_t)total;
 }
 
 // k_bin_levels: workgroup b takes level b / chunks and kBinVert vertices (two runs of the scan),
@@ -736,7 +780,7 @@ __global__ __launch_bounds__(kBinVert) void k_bin_levels(BinArgs b, uint32_t n_c
     uint32_t first = 0u, end = 0u;       // wave 0: this workgroup's first slot and the segment end
     if (threadIdx.x < kMaxSegBins) {
         cnt[threadIdx.x] = 0u;
-        first = b.runs[((size_t)l * kMaxSegBins + threadIdx.x) * b.n_runs + chunk * (kBinVert / kRunVert)];
+        first = b.runs[((size_t)l * b.n_runs + chunk * (kBinVert / kRunVert)) * kMaxSegBins + threadIdx.x];
         end = b.seg_end[l * kMaxSegBins + threadIdx.x];
     }
     __syncthreads();
@@ -1016,7 +1060,10 @@ hipError_t launch_bin_levels(const BinArgs& b, hipStream_t s) {
     const uint32_t n_chunks = (b.n + kBinVert - 1) / kBinVert;
     const uint64_t blocks = (uint64_t)n_chunks * b.a.grid.n_levels;
     if (blocks > 0x7FFFFFFFull) return hipErrorInvalidValue;
-    hipLaunchKernelGGL(k_bin_scan, dim3(b.a.grid.n_levels * kMaxSegBins), dim3(kScanT), 0, s, b);
+    const uint32_t n_tiles = (b.n_runs + kScanRows - 1) / kScanRows;
+    if (n_tiles > b.max_tiles) return hipErrorInvalidValue;
+    hipLaunchKernelGGL(k_bin_tiles, dim3(b.a.grid.n_levels * n_tiles), dim3(64 * kScanWaves), 0, s, b, n_tiles);
+    hipLaunchKernelGGL(k_bin_scan, dim3(b.a.grid.n_levels * n_tiles), dim3(64 * kScanWaves), 0, s, b, n_tiles);
     hipLaunchKernelGGL(k_bin_levels, dim3((uint32_t)blocks), dim3(kBinVert), 0, s, b, n_chunks);
     return hipGetLastError();
 }

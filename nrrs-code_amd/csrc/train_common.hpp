@@ -15,6 +15,8 @@ constexpr uint32_t kMaxSegBins = 64;
 // The encode pass counts the records of every (level, segment, kRunVert vertices); a scan of
 // these counts gives each run its first record slot.
 constexpr uint32_t kRunVert = 256;
+// Chunks per tile of the scan of the run counts.
+constexpr uint32_t kBinScanRows = 256;
 // Records one workgroup of the chunked apply sums (segments of the dense levels, where one
 // segment can receive several records per vertex).
 constexpr uint32_t kApplyChunk = 1u << 17;
@@ -96,9 +98,11 @@ struct BinArgs {
     uint32_t n;
     uint2* rec_buf;                // records {entry in level, FP16 pair}
     uint32_t* seg_count;           // records per segment (written by the scan)
-    uint32_t* runs;                // [level][kMaxSegBins][n_runs]: record counts of the kRunVert-vertex
+    uint32_t* runs;                // [level][n_runs][kMaxSegBins]: record counts of the kRunVert-vertex
                                    // chunks from the encode pass, replaced by their first slots
     uint32_t n_runs;               // chunks: ceil(n / kRunVert)
+    uint32_t* tile_sums;           // [level][max_tiles][kMaxSegBins] scan scratch
+    uint32_t max_tiles;            // tiles of kBinScanRows chunks tile_sums holds per level
     const uint32_t* seg_end;       // [level][kMaxSegBins] one past the last slot of each segment
     const SegInfo* segs;
     uint32_t seg_base[kMaxLevels];   // first segment of each level

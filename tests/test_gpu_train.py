@@ -98,6 +98,55 @@ def test_backward_external_matches_fp64(torch_cuda, ora, prec, net):
         assert n_grid < GRID_NORM_TOL[prec]
 
 
+@pytest.mark.parametrize("case", ["table_2p20", "z_plane"])
+def test_low_precision_table_gradient_fallbacks(torch_cuda, ora, case):
+    """BF16 StatNet backward through the two table-gradient paths that the default
+    configuration does not take, against the FP64 oracle:
+    - table_2p20: a 2^20-entry hash table has 128 segments per level, more than the binned
+      scatter handles, so the gradient goes through direct packed-FP16 atomics;
+    - z_plane: every vertex on one z-plane packs the records of the dense levels 2 and 3 into
+      one or two segments each, beyond their record slots, so the records that do not fit are
+      added with packed-FP16 atomics (a few per table entry, hence FP16-accurate)."""
+    torch = torch_cuda
+    prec, net = na.BF16, na.STATNET
+    kw = dict(grid=dict(log2_table_size=20)) if case == "table_2p20" else {}
+    ctx, vb, film_np, film = _setup(torch, prec, **kw)
+    if case == "z_plane":
+        vb.pos[2, :N] = 0.37
+    hv = ora.HostVertices(vb.to_numpy())
+    if case == "z_plane":
+        # precondition: some segment of a dense level receives more records than its slots
+        # (slot count of a dense segment: min(8n, 3 x its share of 8n records + 4096))
+        lay = ora.grid_layout(ctx.cfg.grid)
+        idx = ora.grid_indices(ctx.cfg, hv)
+        over = False
+        for l in range(ctx.cfg.grid.n_levels):
+            if not lay["dense"][l]:
+                continue
+            size = int(lay["size"][l])
+            counts = np.bincount(idx[:, l, :].ravel() >> 13, minlength=(size + 8191) // 8192)
+            for b, c in enumerate(counts):
+                ne = min(8192, size - b * 8192)
+                cap = min(8 * N, 3 * (8 * N * ne // size) + 4096)
+                over |= c > (cap + 63) // 64 * 64
+        assert over
+    rng = np.random.default_rng(3)
+    y64 = ora.net_forward(ctx.cfg, net, hv, film_np, None, ctx.params_get(net).astype(np.float64)).T
+    dy = (1e-3 * y64 + rng.normal(0, 1e-5, (6, N))).astype(np.float32)
+    ctx.grads_zero(net)
+    na.check(ctx.backward(net, vb, na.LOSS_EXTERNAL, film=film, dy=torch.from_numpy(dy).cuda(),
+                          loss_scale=SCALE[prec]), "backward")
+    g_dev = ctx.grads_get(net) / SCALE[prec]
+    p = ctx.params_get(net).astype(np.float64)
+    _, g_ref, _ = ora.train_grads(ctx.cfg, net, hv, p, 0, film=film_np, dy_ext=dy.astype(np.float64))
+    g, _m = ctx.param_count(net)
+    n_grid = float(np.linalg.norm(g_dev[:g] - g_ref[:g]) / np.linalg.norm(g_ref[:g]))
+    e_mlp = _rel(g_dev[g:], g_ref[g:])
+    print(f"{case}: grid grad norm err {n_grid:.2e}, mlp grad err {e_mlp:.2e}")
+    assert e_mlp < GRAD_TOL[prec]
+    assert n_grid < GRID_NORM_TOL[prec]
+
+
 @pytest.mark.parametrize("prec", PRECISIONS)
 def test_ll2_and_rrs_loss_backward(torch_cuda, ora, prec):
     torch = torch_cuda
