@@ -223,9 +223,11 @@ nrrs_status ensure_train(nrrs_ctx* c, int net) {
     if (c->cfg.precision != NRRS_FP32) {
         // Binned scatter: every level cut into segments of kSegEntries entries, each with room
         // for 1.25x (hashed levels, spread by the hash) or 3x (dense levels, whose segments see
-        // the geometry) its share of a full batch's 8 records per vertex; what does not fit is
-        // added atomically.
+        // the geometry) its share of a full batch's records (4.5 per vertex on average: four
+        // x-neighbour pairs, one in eight split in two), at most 8 per vertex; what does not
+        // fit is added atomically.
         const uint64_t T = (uint64_t)c->grid.hash_mask + 1u, B8 = 8ull * c->cfg.max_train;
+        const uint64_t B45 = (9ull * c->cfg.max_train) / 2;
         std::vector<SegInfo> segs;
         std::vector<uint32_t> direct, chunked;
         std::vector<uint2> chunks;
@@ -243,7 +245,7 @@ nrrs_status ensure_train(nrrs_ctx* c, int net) {
             ns.seg_nbins[l] = nb;
             for (uint32_t b = 0; b < nb; ++b) {
                 const uint64_t ne = std::min<uint64_t>(kSegEntries, size - (uint64_t)b * kSegEntries);
-                const uint64_t expect = B8 * ne / size;
+                const uint64_t expect = B45 * ne / size;
                 uint64_t cap = dense ? std::min<uint64_t>(B8, 3 * expect + 4096) : expect + expect / 4 + 4096;
                 cap = (cap + 63) / 64 * 64;
                 const uint32_t idx = (uint32_t)segs.size();
@@ -260,7 +262,8 @@ nrrs_status ensure_train(nrrs_ctx* c, int net) {
         }
         // slots and slot + list index stay below 2^32
         if (ok && rec_total < 0xFFFF0000ull) {
-            NRRS_TRY(dmalloc(&ns.rec_buf, rec_total));
+            NRRS_TRY(dmalloc(&ns.rec_key, rec_total));
+            NRRS_TRY(dmalloc(&ns.rec_val, rec_total));
             NRRS_TRY(dmalloc(&ns.gdl, (size_t)c->grid.n_levels * c->cfg.max_train));
             const uint32_t max_runs = (c->cfg.max_train + kRunVert - 1) / kRunVert;
             NRRS_TRY(dmalloc(&ns.bin_runs, (size_t)c->grid.n_levels * kMaxSegBins * max_runs));
@@ -387,7 +390,8 @@ nrrs_status run_backward(nrrs_ctx* c, int net, uint32_t n, const nrrs_vertices& 
         bn.gdl = ns.gdl;
         bn.gdl_stride = cfg.max_train;
         bn.n = n;
-        bn.rec_buf = ns.rec_buf;
+        bn.rec_key = ns.rec_key;
+        bn.rec_val = ns.rec_val;
         bn.seg_count = ns.seg_count;
         bn.runs = ns.bin_runs;
         bn.n_runs = (n + kRunVert - 1) / kRunVert;
@@ -402,7 +406,8 @@ nrrs_status run_backward(nrrs_ctx* c, int net, uint32_t n, const nrrs_vertices& 
         bn.grad = reinterpret_cast<_Float16 __attribute__((ext_vector_type(2)))*>(ns.grad_lp);
         NRRS_HIP(launch_bin_levels(bn, s));
         ApplyArgs ap{};
-        ap.rec = ns.rec_buf;
+        ap.key = ns.rec_key;
+        ap.val = ns.rec_val;
         ap.seg_count = ns.seg_count;
         ap.segs = static_cast<const SegInfo*>(ns.segs);
         ap.direct = ns.direct;
@@ -572,7 +577,8 @@ void free_ctx(nrrs_ctx* c) {
         hipFree(ns.grad);
         hipFree(ns.grad_lp);
         hipFree(ns.dense_rep);
-        hipFree(ns.rec_buf);
+        hipFree(ns.rec_key);
+        hipFree(ns.rec_val);
         hipFree(ns.gdl);
         hipFree(ns.bin_runs);
         hipFree(ns.bin_tiles);

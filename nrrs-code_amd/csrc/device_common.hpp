@@ -194,6 +194,17 @@ __device__ __forceinline__ float corner_weight(const LevelCell& lc, int cn) {
     return (wx * wy) * wz;
 }
 
+// Record code of the x-adjacent corners e0 (x bit 0) and e1 (x bit 1) of one (y, z) of a cell,
+// for the binned table-gradient scatter: 0..2 when e1 = e0 ^ (2^(code+1) - 1), so that one
+// record {e0, code, both values} carries the pair (both entries then lie in one 8192-entry
+// segment); 3 when the corners need a record each. Dense levels have e1 = e0 + 1, hashed levels
+// e1 = e0 ^ (cx ^ (cx + 1)) (the x prime of the hash is 1): code 3 is left for about one pair
+// in eight.
+__device__ __forceinline__ uint32_t pair_code(uint32_t e0, uint32_t e1) {
+    const uint32_t m = e0 ^ e1;
+    return m == 1u ? 0u : m == 3u ? 1u : m == 7u ? 2u : 3u;
+}
+
 // Corner values of a level fetched as x-adjacent pairs. In a dense level the
 // corners (x, y, z) and (x+1, y, z) are consecutive entries; in a hashed level
 // with even x they are the aligned pair {h & ~1, h | 1} (the x prime of the

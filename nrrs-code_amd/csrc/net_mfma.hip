@@ -66,7 +66,7 @@ __global__ __launch_bounds__(kRunVert) void k_encode_levels(EncodeArgs a, const 
         const float2 f = grid_level_pairs<PairLoadLp<P>>(table, a.grid, (int)l, x);
         out[(size_t)l * stride + i] = (uint32_t)lp_bits<P>(f.x) | ((uint32_t)lp_bits<P>(f.y) << 16);
     }
-    if (run_counts) {  // uniform: corners per table segment of this level and chunk
+    if (run_counts) {  // uniform: table-gradient records per segment of this level and chunk
         __shared__ uint32_t hist[kMaxSegBins];
         if (threadIdx.x < kMaxSegBins) hist[threadIdx.x] = 0u;
         __syncthreads();
@@ -74,12 +74,17 @@ __global__ __launch_bounds__(kRunVert) void k_encode_levels(EncodeArgs a, const 
             const LevelCell lc = level_cell(a.grid, (int)l, x);
             const bool dense = (a.grid.dense_mask >> l) & 1u;  // few segments: aggregated atomics
 #pragma unroll
-            for (int cn = 0; cn < 8; ++cn) {
-                const uint32_t sb = corner_index(a.grid, (int)l, lc, cn) >> kSegShift;
-                if (dense)
-                    lds_count(hist, sb);
-                else
-                    atomicAdd(&hist[sb], 1u);
+            for (int yz = 0; yz < 4; ++yz) {
+                const uint32_t e0 = corner_index(a.grid, (int)l, lc, yz << 1);
+                const uint32_t e1 = corner_index(a.grid, (int)l, lc, (yz << 1) | 1);
+                const bool two = pair_code(e0, e1) == 3u;  // a record per corner
+                if (dense) {
+                    lds_count(hist, e0 >> kSegShift);
+                    if (two) lds_count(hist, e1 >> kSegShift);
+                } else {
+                    atomicAdd(&hist[e0 >> kSegShift], 1u);
+                    if (two) atomicAdd(&hist[e1 >> kSegShift], 1u);
+                }
             }
         }
         __syncthreads();

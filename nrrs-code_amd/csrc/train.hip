@@ -757,24 +757,26 @@ __global__ __launch_bounds__(64 * kScanWaves) void k_bin_scan(BinArgs b, uint32_
 }
 
 // k_bin_levels: workgroup b takes level b / chunks and kBinVert vertices (two runs of the scan),
-// so the GPU walks the levels one after the other. Each thread takes one vertex and its 8
-// corners:
+// so the GPU walks the levels one after the other. Each thread takes one vertex: per (y, z) of its
+// cell, one record for the x-adjacent corner pair or, for pair code 3, one per corner:
 //  1. every record gets its rank within its segment from LDS atomics;
 //  2. wave 0, lane s = segment s: an exclusive scan of the counts places the segment's records
 //     in the workgroup's segment-ordered list; the workgroup's slots of segment s start at the
 //     first slot of its first run;
-//  3. each record {entry in level, FP16 pair w * dL/denc} is put into the LDS list at
-//     (segment start + rank);
+//  3. each record {entry in level, code, FP16 w * dL/denc of its corners} is put into the LDS
+//     list at (segment start + rank);
 //  4. the list is copied out with consecutive threads on consecutive records, so the stores of
 //     a wave form a few contiguous runs rather than one cache line per lane.
 // Records beyond a full segment's slots are added directly to the table gradient with packed
 // FP16 atomics.
 constexpr int kBinVert = 2 * kRunVert;
+constexpr int kBinMaxRec = 8 * kBinVert;  // every pair split
 
 __global__ __launch_bounds__(kBinVert) void k_bin_levels(BinArgs b, uint32_t n_chunks) {
     static_assert(kMaxSegBins == 64, "one lane per segment");
     __shared__ uint32_t cnt[kMaxSegBins], start[kMaxSegBins], dst[kMaxSegBins], lim[kMaxSegBins], tot;
-    __shared__ uint2 list[kBinVert * 8];
+    __shared__ uint32_t list_key[kBinMaxRec];            // entry in level | code << 24
+    __shared__ unsigned long long list_val[kBinMaxRec];
     const uint32_t l = blockIdx.x / n_chunks, chunk = blockIdx.x % n_chunks, i = chunk * kBinVert + threadIdx.x;
     if (l >= b.a.grid.n_levels) return;  // whole workgroup
     uint32_t first = 0u, end = 0u;       // wave 0: this workgroup's first slot and the segment end
@@ -787,7 +789,7 @@ __global__ __launch_bounds__(kBinVert) void k_bin_levels(BinArgs b, uint32_t n_c
     const bool live = i < b.n;
     // dense (coarse) levels have few segments, often one per wave: aggregated LDS atomics
     const bool dense = (b.a.grid.dense_mask >> l) & 1u;
-    uint32_t re[8], rk[8];
+    uint32_t e[8], rk[8], code[4];  // e[2yz], e[2yz+1]: corners x = 0, 1 of (y, z) = yz
     LevelCell lc{};
     float2 g = make_float2(0.0f, 0.0f);
     if (live) {
@@ -796,9 +798,14 @@ __global__ __launch_bounds__(kBinVert) void k_bin_levels(BinArgs b, uint32_t n_c
         lc = level_cell(b.a.grid, (int)l, x);
         g = b.gdl[(size_t)l * b.gdl_stride + i];
 #pragma unroll
-        for (int cn = 0; cn < 8; ++cn) {
-            re[cn] = corner_index(b.a.grid, (int)l, lc, cn);  // < level size
-            rk[cn] = dense ? lds_count(cnt, re[cn] >> kSegShift) : atomicAdd(&cnt[re[cn] >> kSegShift], 1u);
+        for (int yz = 0; yz < 4; ++yz) {
+            e[2 * yz] = corner_index(b.a.grid, (int)l, lc, yz << 1);  // < level size
+            e[2 * yz + 1] = corner_index(b.a.grid, (int)l, lc, (yz << 1) | 1);
+            code[yz] = pair_code(e[2 * yz], e[2 * yz + 1]);
+            rk[2 * yz] = dense ? lds_count(cnt, e[2 * yz] >> kSegShift) : atomicAdd(&cnt[e[2 * yz] >> kSegShift], 1u);
+            if (code[yz] == 3u)
+                rk[2 * yz + 1] = dense ? lds_count(cnt, e[2 * yz + 1] >> kSegShift)
+                                       : atomicAdd(&cnt[e[2 * yz + 1] >> kSegShift], 1u);
         }
     }
     __syncthreads();
@@ -813,21 +820,44 @@ __global__ __launch_bounds__(kBinVert) void k_bin_levels(BinArgs b, uint32_t n_c
     __syncthreads();
     if (live) {
 #pragma unroll
-        for (int cn = 0; cn < 8; ++cn) {
-            const float w = corner_weight(lc, cn);
-            const h2v gv = {(_Float16)(w * g.x), (_Float16)(w * g.y)};
-            list[start[re[cn] >> kSegShift] + rk[cn]] = make_uint2(re[cn], __builtin_bit_cast(uint32_t, gv));
+        for (int yz = 0; yz < 4; ++yz) {
+            uint32_t v[2];
+#pragma unroll
+            for (int xb = 0; xb < 2; ++xb) {
+                const float w = corner_weight(lc, (yz << 1) | xb);
+                const h2v gv = {(_Float16)(w * g.x), (_Float16)(w * g.y)};
+                v[xb] = __builtin_bit_cast(uint32_t, gv);
+            }
+            const uint32_t e0 = e[2 * yz], e1 = e[2 * yz + 1];
+            const uint32_t k0 = start[e0 >> kSegShift] + rk[2 * yz];
+            if (code[yz] != 3u) {
+                list_key[k0] = e0 | (code[yz] << 24);
+                list_val[k0] = (unsigned long long)v[0] | ((unsigned long long)v[1] << 32);
+            } else {
+                const uint32_t k1 = start[e1 >> kSegShift] + rk[2 * yz + 1];
+                list_key[k0] = e0 | (3u << 24);
+                list_val[k0] = v[0];
+                list_key[k1] = e1 | (3u << 24);
+                list_val[k1] = v[1];
+            }
         }
     }
     __syncthreads();
     const uint32_t n_rec = tot;
     for (uint32_t k = threadIdx.x; k < n_rec; k += kBinVert) {
-        const uint2 r = list[k];
-        const uint32_t sb = r.x >> kSegShift, d = dst[sb] + k;
-        if (d < lim[sb])
-            b.rec_buf[d] = r;
-        else  // segment full: add directly
-            __builtin_amdgcn_global_atomic_fadd_v2f16(b.grad + b.a.grid.offset[l] + r.x, __builtin_bit_cast(h2v, r.y));
+        const uint32_t key = list_key[k], ent = key & 0xFFFFFFu, c = key >> 24, sb = ent >> kSegShift;
+        const unsigned long long v = list_val[k];
+        const uint32_t d = dst[sb] + k;
+        if (d < lim[sb]) {
+            b.rec_key[d] = (uint16_t)((ent & (kSegEntries - 1u)) | (c << kSegShift));
+            b.rec_val[d] = v;
+        } else {  // segment full: add directly
+            h2v* gl = b.grad + b.a.grid.offset[l];
+            __builtin_amdgcn_global_atomic_fadd_v2f16(gl + ent, __builtin_bit_cast(h2v, (uint32_t)v));
+            if (c != 3u)
+                __builtin_amdgcn_global_atomic_fadd_v2f16(gl + (ent ^ ((2u << c) - 1u)),
+                                                          __builtin_bit_cast(h2v, (uint32_t)(v >> 32)));
+        }
     }
 }
 
@@ -853,36 +883,50 @@ __device__ __forceinline__ void apply_half(unsigned long long* acc, uint32_t slo
     if (mag) atomicAdd(&acc[slot], (h & 0x8000u) ? 0ull - mag : mag);
 }
 
-__device__ __forceinline__ void apply_record(unsigned long long* acc, uint32_t e, uint32_t bits, uint32_t& bad) {
-    e &= kSegEntries - 1u;
+__device__ __forceinline__ void apply_pair(unsigned long long* acc, uint32_t e, uint32_t bits, uint32_t& bad) {
     apply_half(acc, 2 * e, bits & 0xffffu, bad);
     apply_half(acc, 2 * e + 1, bits >> 16, bad);
 }
 
-// Records [r0, r1) of a segment into the LDS sums.
-__device__ __forceinline__ void apply_range(unsigned long long* acc_s, const uint2* r, uint32_t r0, uint32_t r1,
+// One record (key, value) into the LDS sums of its segment.
+__device__ __forceinline__ void apply_record(unsigned long long* acc, uint32_t key, unsigned long long v,
+                                             uint32_t& bad) {
+    const uint32_t e = key & (kSegEntries - 1u), c = (key >> kSegShift) & 3u;
+    apply_pair(acc, e, (uint32_t)v, bad);
+    if (c != 3u) apply_pair(acc, e ^ ((2u << c) - 1u), (uint32_t)(v >> 32), bad);
+}
+
+// Records [r0, r1) of a segment into the LDS sums, two per thread and load.
+__device__ __forceinline__ void apply_range(unsigned long long* acc_s, const uint16_t* key,
+                                            const unsigned long long* val, uint32_t r0, uint32_t r1,
                                             uint32_t& bad) {
-    // r0 is even (chunks and segment rows start on 16-byte boundaries)
-    const uint4* r4 = static_cast<const uint4*>(__builtin_assume_aligned(r + r0, 16));
+    // r0 is even (segment slots and chunks start at multiples of 64): 4- and 16-byte loads
+    const uint32_t* k2 = static_cast<const uint32_t*>(__builtin_assume_aligned(key + r0, 4));
+    const uint4* v2 = static_cast<const uint4*>(__builtin_assume_aligned(val + r0, 16));
     const uint32_t n2 = (r1 - r0) / 2;
     constexpr uint32_t kStep = kApplyThreads * kApplyBatch;
-    uint32_t k0 = 0;
-    for (; k0 + kStep <= n2; k0 += kStep) {  // full batches: every load in bounds, all issued first
+    uint32_t j0 = 0;
+    for (; j0 + kStep <= n2; j0 += kStep) {  // full batches: every load in bounds, all issued first
+        uint32_t kk[kApplyBatch];
         uint4 q[kApplyBatch];
 #pragma unroll
-        for (int u = 0; u < kApplyBatch; ++u) q[u] = r4[k0 + u * kApplyThreads + threadIdx.x];
+        for (int u = 0; u < kApplyBatch; ++u) {
+            kk[u] = k2[j0 + u * kApplyThreads + threadIdx.x];
+            q[u] = v2[j0 + u * kApplyThreads + threadIdx.x];
+        }
 #pragma unroll
         for (int u = 0; u < kApplyBatch; ++u) {
-            apply_record(acc_s, q[u].x, q[u].y, bad);
-            apply_record(acc_s, q[u].z, q[u].w, bad);
+            apply_record(acc_s, kk[u] & 0xffffu, q[u].x | ((unsigned long long)q[u].y << 32), bad);
+            apply_record(acc_s, kk[u] >> 16, q[u].z | ((unsigned long long)q[u].w << 32), bad);
         }
     }
-    for (uint32_t k = k0 + threadIdx.x; k < n2; k += kApplyThreads) {
-        const uint4 q = r4[k];
-        apply_record(acc_s, q.x, q.y, bad);
-        apply_record(acc_s, q.z, q.w, bad);
+    for (uint32_t j = j0 + threadIdx.x; j < n2; j += kApplyThreads) {
+        const uint32_t kk = k2[j];
+        const uint4 q = v2[j];
+        apply_record(acc_s, kk & 0xffffu, q.x | ((unsigned long long)q.y << 32), bad);
+        apply_record(acc_s, kk >> 16, q.z | ((unsigned long long)q.w << 32), bad);
     }
-    if (((r1 - r0) & 1u) && threadIdx.x == 0) apply_record(acc_s, r[r1 - 1].x, r[r1 - 1].y, bad);
+    if (((r1 - r0) & 1u) && threadIdx.x == 0) apply_record(acc_s, key[r1 - 1], val[r1 - 1], bad);
 }
 
 __device__ __forceinline__ h2v add_fixed(h2v o, long long s0, long long s1) {
@@ -899,7 +943,7 @@ __global__ __launch_bounds__(kApplyThreads) void k_apply_bins(ApplyArgs a) {
     if (threadIdx.x == 0) bad_s = 0u;
     __syncthreads();
     uint32_t bad = 0u;
-    apply_range(acc_s, a.rec + sg.rec_off, 0, min(a.seg_count[seg], sg.cap), bad);
+    apply_range(acc_s, a.key + sg.rec_off, a.val + sg.rec_off, 0, min(a.seg_count[seg], sg.cap), bad);
     if (bad) bad_s = 1u;
     __syncthreads();
     h2v* g = a.grad + sg.entry0;
@@ -922,7 +966,7 @@ __global__ __launch_bounds__(kApplyThreads) void k_apply_chunks(ApplyArgs a) {
     for (uint32_t k = threadIdx.x; k < 2 * sg.n_entries; k += kApplyThreads) acc_s[k] = 0ull;
     __syncthreads();
     uint32_t bad = 0u;
-    apply_range(acc_s, a.rec + sg.rec_off, ch.y, min(ch.y + kApplyChunk, n), bad);
+    apply_range(acc_s, a.key + sg.rec_off, a.val + sg.rec_off, ch.y, min(ch.y + kApplyChunk, n), bad);
     if (bad) atomicOr(a.bad, 1u);
     __syncthreads();
     unsigned long long* acc = a.acc + 2 * (size_t)sg.entry0;

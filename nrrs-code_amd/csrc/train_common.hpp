@@ -28,8 +28,12 @@ struct SegInfo {
     uint32_t n_entries;  // entries of the segment (the last one of a level can be short)
 };
 
+// Table-gradient records: key (u16) = entry within the segment (bits 0-12) | pair code << 13
+// (see pair_code); value (u64) = FP16 pair of the key's entry (low half) and, for codes 0-2,
+// of its x-neighbour entry ^ (2^(code+1) - 1) (high half).
 struct ApplyArgs {
-    const uint2* rec;                // records {entry in level, FP16 pair}
+    const uint16_t* key;             // record keys, indexed like the slots of the segments
+    const unsigned long long* val;   // record values
     uint32_t* seg_count;             // records appended per segment, cleared after use
     const SegInfo* segs;
     const uint32_t* direct;          // segments summed by one workgroup each (hashed levels)
@@ -96,7 +100,8 @@ struct BinArgs {
     const float2* gdl;             // [level][gdl_stride] dL/denc pairs written by the training kernel
     uint32_t gdl_stride;
     uint32_t n;
-    uint2* rec_buf;                // records {entry in level, FP16 pair}
+    uint16_t* rec_key;             // record keys and values (see ApplyArgs)
+    unsigned long long* rec_val;
     uint32_t* seg_count;           // records per segment (written by the scan)
     uint32_t* runs;                // [level][n_runs][kMaxSegBins]: record counts of the kRunVert-vertex
                                    // chunks from the encode pass, replaced by their first slots
@@ -109,9 +114,9 @@ struct BinArgs {
     uint32_t seg_nbins[kMaxLevels];  // segments of each level
     _Float16 __attribute__((ext_vector_type(2)))* grad;  // FP16 pair table gradient (full segments)
 };
-// Scans the run counts into first slots and segment totals, then writes the records
-// {entry, FP16 w * dL/denc} of every (vertex, level, corner) into the slots of their segment,
-// walking the levels one after the other.
+// Scans the run counts into first slots and segment totals, then writes the records of every
+// (vertex, level) (FP16 w * dL/denc of its corners, x-neighbours paired) into the slots of
+// their segments, walking the levels one after the other.
 hipError_t launch_bin_levels(const BinArgs& b, hipStream_t s);
 // Sums the binned records into the table gradient: n_direct segments with one workgroup
 // each, n_chunks chunks of the other segments, then their fixed-point sums.

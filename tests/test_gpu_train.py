@@ -115,19 +115,23 @@ def test_low_precision_table_gradient_fallbacks(torch_cuda, ora, case):
         vb.pos[2, :N] = 0.37
     hv = ora.HostVertices(vb.to_numpy())
     if case == "z_plane":
-        # precondition: some segment of a dense level receives more records than its slots
-        # (slot count of a dense segment: min(8n, 3 x its share of 8n records + 4096))
+        # precondition: some segment of a dense level receives more records than its slots.
+        # Records: one per x-neighbour corner pair whose entries differ by 1, 3 or 7 (xor),
+        # else one per corner; slots of a dense segment: min(8n, 3 x its share of 4.5n + 4096).
         lay = ora.grid_layout(ctx.cfg.grid)
-        idx = ora.grid_indices(ctx.cfg, hv)
+        idx = ora.grid_indices(ctx.cfg, hv).astype(np.int64)  # [vertex][level][corner x | y<<1 | z<<2]
         over = False
         for l in range(ctx.cfg.grid.n_levels):
             if not lay["dense"][l]:
                 continue
             size = int(lay["size"][l])
-            counts = np.bincount(idx[:, l, :].ravel() >> 13, minlength=(size + 8191) // 8192)
+            e0, e1 = idx[:, l, 0::2].ravel(), idx[:, l, 1::2].ravel()
+            single = ~np.isin(e0 ^ e1, [1, 3, 7])
+            segs = np.concatenate([e0 >> 13, e1[single] >> 13])
+            counts = np.bincount(segs, minlength=(size + 8191) // 8192)
             for b, c in enumerate(counts):
                 ne = min(8192, size - b * 8192)
-                cap = min(8 * N, 3 * (8 * N * ne // size) + 4096)
+                cap = min(8 * N, 3 * (9 * N // 2 * ne // size) + 4096)
                 over |= c > (cap + 63) // 64 * 64
         assert over
     rng = np.random.default_rng(3)
