@@ -402,7 +402,7 @@ def main():
                 "kernel_us": round(bf["train_kernel_us"], 1),
                 "algorithmic_bytes_per_launch": bytes_s * n,
                 "mfma_tflops": round(tflops, 2), "mfma_frac": round(tflops / BF16_PEAK_TFLOPS, 5),
-                "table_gradient_records_per_launch": 16 * 8 * n}
+                "table_gradient_corner_updates_per_launch": 16 * 8 * n}
     matched = {}
     if "fp32" in q_by:
         for name in q_by:

@@ -228,6 +228,9 @@ nrrs_status ensure_train(nrrs_ctx* c, int net) {
         // fit is added atomically.
         const uint64_t T = (uint64_t)c->grid.hash_mask + 1u, B8 = 8ull * c->cfg.max_train;
         const uint64_t B45 = (9ull * c->cfg.max_train) / 2;
+        uint32_t chunk = kApplyChunkMin;
+        while (chunk < kApplyChunkMax && 2ull * chunk <= c->cfg.max_train / 8) chunk *= 2;
+        ns.apply_chunk = chunk;
         std::vector<SegInfo> segs;
         std::vector<uint32_t> direct, chunked;
         std::vector<uint2> chunks;
@@ -253,7 +256,7 @@ nrrs_status ensure_train(nrrs_ctx* c, int net) {
                                        (uint32_t)ne});
                 if (dense) {
                     chunked.push_back(idx);
-                    for (uint64_t r0 = 0; r0 < cap; r0 += kApplyChunk) chunks.push_back(make_uint2(idx, (uint32_t)r0));
+                    for (uint64_t r0 = 0; r0 < cap; r0 += chunk) chunks.push_back(make_uint2(idx, (uint32_t)r0));
                 } else {
                     direct.push_back(idx);
                 }
@@ -416,6 +419,7 @@ nrrs_status run_backward(nrrs_ctx* c, int net, uint32_t n, const nrrs_vertices& 
         ap.n_chunked = ns.n_chunked;
         ap.acc = ns.acc;
         ap.n_acc = ns.n_chunks ? ns.n_dense : 0;
+        ap.chunk = ns.apply_chunk;
         ap.bad = ns.bad;
         ap.grad = reinterpret_cast<_Float16 __attribute__((ext_vector_type(2)))*>(ns.grad_lp);
         NRRS_HIP(launch_apply_bins(ap, ns.n_direct, ns.n_chunks, s));

@@ -116,6 +116,7 @@ struct NetState {
     uint32_t n_chunked = 0;
     uint2* chunks = nullptr;         // {segment, first record}
     uint32_t n_chunks = 0;
+    uint32_t apply_chunk = 0;        // records per chunk of the chunked apply
     unsigned long long* acc = nullptr;  // fixed-point sums of the chunked segments
     uint32_t* bad = nullptr;
     float* adam_m = nullptr;

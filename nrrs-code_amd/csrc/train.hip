@@ -966,7 +966,7 @@ __global__ __launch_bounds__(kApplyThreads) void k_apply_chunks(ApplyArgs a) {
     for (uint32_t k = threadIdx.x; k < 2 * sg.n_entries; k += kApplyThreads) acc_s[k] = 0ull;
     __syncthreads();
     uint32_t bad = 0u;
-    apply_range(acc_s, a.key + sg.rec_off, a.val + sg.rec_off, ch.y, min(ch.y + kApplyChunk, n), bad);
+    apply_range(acc_s, a.key + sg.rec_off, a.val + sg.rec_off, ch.y, min(ch.y + a.chunk, n), bad);
     if (bad) atomicOr(a.bad, 1u);
     __syncthreads();
     unsigned long long* acc = a.acc + 2 * (size_t)sg.entry0;

@@ -18,8 +18,10 @@ constexpr uint32_t kRunVert = 256;
 // Chunks per tile of the scan of the run counts.
 constexpr uint32_t kBinScanRows = 256;
 // Records one workgroup of the chunked apply sums (segments of the dense levels, where one
-// segment can receive several records per vertex).
-constexpr uint32_t kApplyChunk = 1u << 17;
+// segment can receive several records per vertex): max_train / 8 rounded down to a power of
+// two in [kApplyChunkMin, kApplyChunkMax], so that a full batch gives a similar number of
+// chunks at every batch size.
+constexpr uint32_t kApplyChunkMin = 1u << 12, kApplyChunkMax = 1u << 17;
 
 struct SegInfo {
     uint32_t rec_off;    // first record slot in the record buffer
@@ -40,6 +42,7 @@ struct ApplyArgs {
     const uint2* chunks;             // {segment, first record} of the chunked apply (dense levels)
     const uint32_t* chunked;         // the segments of the chunked apply
     uint32_t n_chunked;
+    uint32_t chunk;                  // records per chunk (a power of two)
     unsigned long long* acc;         // [n_acc][2] fixed-point sums of the chunked segments
     uint32_t n_acc;                  // entries covered by acc (the dense levels come first)
     uint32_t* bad;                   // non-finite record seen by the chunked apply
