@@ -216,6 +216,7 @@ struct PairLoad;
 template <>
 struct PairLoad<float> {  // FP32 table: entry = 2 floats
     typedef float T;
+    static constexpr bool kQuad = false;
     __device__ __forceinline__ static float2 one(const float* t, uint32_t e) {
         return reinterpret_cast<const float2*>(t)[e];
     }
@@ -229,12 +230,21 @@ struct PairLoad<float> {  // FP32 table: entry = 2 floats
 template <typename Dec>
 struct PairLoadLp {  // FP16/BF16 table: entry = one 32-bit word, decoded by Dec::pair
     typedef uint32_t T;
+    static constexpr bool kQuad = true;  // hashed x-neighbours via aligned 4-entry loads
     __device__ __forceinline__ static float2 one(const uint32_t* t, uint32_t e) { return Dec::pair(t[e]); }
     __device__ __forceinline__ static void two(const uint32_t* t, uint32_t e, float2& a, float2& b) {
         uint2 q;
         __builtin_memcpy(&q, t + e, sizeof(q));
         a = Dec::pair(q.x);
         b = Dec::pair(q.y);
+    }
+    // Entries h0 and h1 of a level whose first entry is 16-byte aligned: one 16-byte load of
+    // the aligned quad of h0, which also holds h1 when h0 ^ h1 < 4; else a second load.
+    __device__ __forceinline__ static void x_pair(const uint32_t* t, uint32_t h0, uint32_t h1, float2& a, float2& b) {
+        const uint4 q = *reinterpret_cast<const uint4*>(t + (h0 & ~3u));
+        const uint32_t w[4] = {q.x, q.y, q.z, q.w};
+        a = Dec::pair(w[h0 & 3u]);
+        b = ((h0 ^ h1) < 4u) ? Dec::pair(w[h1 & 3u]) : Dec::pair(t[h1]);
     }
 };
 
@@ -250,6 +260,16 @@ __device__ __forceinline__ float2 grid_level_pairs(const typename L::T* __restri
         for (int yz = 0; yz < 4; ++yz) {
             const uint32_t cy = lc.c[1] + (yz & 1), cz = lc.c[2] + (yz >> 1);
             L::two(table, off + lc.c[0] + r * (cy + r * cz), v[2 * yz], v[2 * yz + 1]);
+        }
+    } else if constexpr (L::kQuad) {
+        // x-neighbours h0, h1 = h0 ^ (cx ^ (cx+1)) share an aligned quad of entries for 3 cells
+        // in 4 (the x prime of the hash is 1); level starts are 8-entry aligned
+#pragma unroll
+        for (int yz = 0; yz < 4; ++yz) {
+            const uint32_t cy = lc.c[1] + (yz & 1), cz = lc.c[2] + (yz >> 1);
+            const uint32_t yzh = (cy * 2654435761u) ^ (cz * 805459861u);
+            const uint32_t h0 = (lc.c[0] ^ yzh) & g.hash_mask, h1 = ((lc.c[0] + 1) ^ yzh) & g.hash_mask;
+            L::x_pair(table + off, h0, h1, v[2 * yz], v[2 * yz + 1]);
         }
     } else {
         const bool odd = lc.c[0] & 1u;
