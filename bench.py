@@ -48,6 +48,7 @@ def parse():
     ap.add_argument("--cpu-sample", type=int, default=1 << 19)
     ap.add_argument("--precisions", default="bf16,fp16,fp32")
     ap.add_argument("--no-frame", action="store_true", help="skip the multi-bounce wavefront and online loop")
+    ap.add_argument("--no-peaks", action="store_true", help="skip the copy and GEMM measurements of the GPU")
     ap.add_argument("--pmc-probe", action="store_true",
                     help="only run a 256 MiB device copy and 3 BF16 StatNet training kernels at 2^20 "
                          "(for rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE runs)")
@@ -63,6 +64,40 @@ def _cpu_model():
     except OSError:
         pass
     return "unknown"
+
+
+def measured_peaks(torch):
+    """What this GPU reaches on plain library work, beside the published peaks the roofline
+    fractions use: a 1 GiB device-to-device copy (read + write bytes per second) and a
+    8192^3 BF16 GEMM (torch.matmul, dense TFLOP/s), each the best of 5 after a warm-up."""
+    def best(fn, reps=5):
+        fn()
+        torch.cuda.synchronize()
+        ts = []
+        for _ in range(reps):
+            e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+            e0.record()
+            fn()
+            e1.record()
+            torch.cuda.synchronize()
+            ts.append(e0.elapsed_time(e1) * 1e-3)
+        return min(ts)
+
+    src = torch.empty(1 << 28, dtype=torch.float32, device="cuda")  # 1 GiB
+    src.fill_(1.0)
+    dst = torch.empty_like(src)
+    t_copy = best(lambda: dst.copy_(src))
+    copy_gbs = 2 * src.numel() * 4 / t_copy / 1e9
+    del src, dst
+    n = 8192
+    a = torch.randn(n, n, device="cuda").to(torch.bfloat16)
+    b = torch.randn(n, n, device="cuda").to(torch.bfloat16)
+    t_mm = best(lambda: torch.matmul(a, b))
+    del a, b
+    torch.cuda.empty_cache()
+    return {"copy_GBps": round(copy_gbs, 1), "copy_frac_of_peak": round(copy_gbs / HBM_PEAK_GBS, 3),
+            "bf16_gemm_TFLOPs": round(2 * n ** 3 / t_mm / 1e12, 1),
+            "bf16_gemm_frac_of_peak": round(2 * n ** 3 / t_mm / 1e12 / BF16_PEAK_TFLOPS, 3)}
 
 
 def cpu_baseline(sample):
@@ -123,6 +158,9 @@ def cpu_baseline(sample):
             "sample": f"{sample} vertices of the same step, FP32 C++ port on {threads} threads, median of 5 runs "
                       f"after a warm-up: {med:.3f} s (inference + decision {t_inf:.3f} s, training {t_tr:.3f} s); "
                       f"host: {host}",
+            # the two parts of the same runs, in the units of the GPU's rrs_queries_per_s and
+            # train_samples_per_s
+            "rrs_queries_per_s": round(sample / t_inf, 1), "train_samples_per_s": round(sample / t_tr, 1),
             "fp64_oracle": {"value": round(m / med64, 1), "unit": "queries/s", "cores": 1,
                             "sample": f"{m} vertices, FP64 oracle on one thread, median of 3: {med64:.2f} s"}}
 
@@ -438,6 +476,12 @@ def main():
             "rrs_queries": round(bf["rrs_queries_per_s"] / results["fp32"]["rrs_queries_per_s"], 3),
             "train_samples": round(bf["train_samples_per_s"] / results["fp32"]["train_samples_per_s"], 3),
             "step": round(bf["step_queries_per_s"] / results["fp32"]["step_queries_per_s"], 3)}
+    if not args.no_peaks:
+        pk = measured_peaks(torch)
+        out["measured_peaks"] = pk
+        roofline["frac_of_measured_copy"] = round(roofline["achieved"] / pk["copy_GBps"], 4)
+        roofline["mfma_frac_of_measured_gemm"] = round(roofline["mfma_tflops"] / pk["bf16_gemm_TFLOPs"], 5)
+        print(f"[bench] peaks: {json.dumps(pk)}", file=sys.stderr, flush=True)
     if not args.no_frame:
         out["wavefront_frame"] = frame_wavefront(torch, na, np)
         print(f"[bench] wavefront: {json.dumps(out['wavefront_frame'])}", file=sys.stderr, flush=True)
