@@ -164,8 +164,9 @@ __device__ __forceinline__ void fwd_layer(const typename P::op* __restrict__ Wf,
     }
 }
 
-// ReLU of accumulators into operands; mask bit 16*ot + r set when D[ot][r] > 0.
-template <typename P>
+// ReLU of accumulators into operands; with MASK, also the mask with bit 16*ot + r set when
+// D[ot][r] > 0 (the backward needs it for the last layer of a recompute only).
+template <typename P, bool MASK = true>
 __device__ __forceinline__ uint32_t relu_ops(const f32x16* D, typename P::op* H) {
     uint32_t mask = 0;
 #pragma unroll
@@ -176,7 +177,7 @@ __device__ __forceinline__ uint32_t relu_ops(const f32x16* D, typename P::op* H)
 #pragma unroll
             for (int j = 0; j < 8; ++j) {
                 float v = D[ot][8 * g + j];
-                mask |= (v > 0.0f ? 1u : 0u) << (16 * ot + 8 * g + j);
+                if (MASK) mask |= (v > 0.0f ? 1u : 0u) << (16 * ot + 8 * g + j);
                 x[j] = fmaxf(v, 0.0f);
             }
             H[2 * ot + g] = P::pack(x);
@@ -191,12 +192,11 @@ __device__ __forceinline__ uint32_t recompute(const typename P::op* __restrict__
                                               typename P::op* H, int lane) {
     f32x16 D[2];
     fwd_layer<P, NL>(Wf, 0, E, D, lane);
-    uint32_t mask = relu_ops<P>(D, H);
     for (int layer = 1; layer < upto; ++layer) {
+        relu_ops<P, false>(D, H);
         fwd_layer<P, NL>(Wf, layer, H, D, lane);
-        mask = relu_ops<P>(D, H);
     }
-    return mask;
+    return relu_ops<P, true>(D, H);  // mask of the last layer only
 }
 
 // BIN: dL/denc of the grid goes to t.gdl for the binned scatter (FP16/BF16); otherwise it is
