@@ -93,8 +93,11 @@ __global__ __launch_bounds__(kRunVert) void k_encode_levels(EncodeArgs a, const 
     }
 }
 
+// Four waves per SIMD (at most 128 registers): the weight tiles are read from LDS in every
+// iteration of the persistent loop rather than kept in registers, which would take 120-152
+// of them and leave one or two waves per SIMD to hide the latency of the loads.
 template <typename P, int NL, int NOUT, int EPI>
-__global__ __launch_bounds__(kBlockT) void k_net_mfma(EncodeArgs a, int net, uint32_t n,
+__global__ __launch_bounds__(kBlockT, 4) void k_net_mfma(EncodeArgs a, int net, uint32_t n,
                                                       const uint32_t* __restrict__ table,
                                                       const u32x4* __restrict__ Wp, EpiArgs ep) {
     extern __shared__ u32x4 sW[];
@@ -107,6 +110,7 @@ __global__ __launch_bounds__(kBlockT) void k_net_mfma(EncodeArgs a, int net, uin
     const uint32_t n_cb = (n + 31) / 32;
     uint64_t term = 0;
     for (uint32_t cb = blockIdx.x * kWaves + wave; cb < n_cb; cb += gridDim.x * kWaves) {
+        asm volatile("" ::: "memory");  // keeps the weight-tile loads from LDS inside the loop
         const uint32_t i = cb * 32 + col;
         const bool live = i < n;
         typename P::vec8 B[5];
