@@ -515,13 +515,13 @@ nrrs_status ensure_frame(nrrs_ctx* c, uint32_t cap) {
     c->frame_cap = 0;
     NRRS_TRY(dmalloc(&c->frame_q, cap));
     NRRS_TRY(dmalloc(&c->frame_s, cap));
-    NRRS_TRY(dmalloc(&c->frame_tiles, 2 * ((size_t)cap / 64 + 1)));
+    NRRS_TRY(dmalloc(&c->frame_tiles, kFrameMaxBlocks));
     NRRS_TRY(dmalloc(&c->frame_x, 3 * (size_t)cap));
     NRRS_TRY(dmalloc(&c->frame_genc, 2 * (size_t)c->grid.n_levels * cap));
     if (!c->frame_sums) NRRS_TRY(dmalloc(&c->frame_sums, 16));
     if (!c->frame_bar) {
-        NRRS_TRY(dmalloc(&c->frame_bar, 2));
-        NRRS_HIP(hipMemset(c->frame_bar, 0, 8));
+        NRRS_TRY(dmalloc(&c->frame_bar, kFrameBarWords));
+        NRRS_HIP(hipMemset(c->frame_bar, 0, kFrameBarWords * 4));
         NRRS_HIP(hipDeviceSynchronize());
     }
     c->frame_cap = cap;
@@ -561,7 +561,6 @@ nrrs_status frame_params(nrrs_ctx* c, const nrrs_frame_args* a, FrameParams* f) 
     f->genc = c->frame_genc;
     f->s_scr = c->frame_s;
     f->tile_sums = c->frame_tiles;
-    f->tile_offsets = c->frame_tiles + (a->capacity / 64 + 1);
     f->sums = c->frame_sums;
     f->bar = c->frame_bar;
     return NRRS_OK;

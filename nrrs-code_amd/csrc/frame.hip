@@ -27,18 +27,33 @@ using namespace mfma;
 constexpr uint32_t kSpinLimit = 1u << 22;
 constexpr uint64_t kBounceSeedStep = 0x9E3779B97F4A7C15ull;
 constexpr uint32_t kA0Vpt = 1;  // vertices per thread in the grid-encoding phase
+constexpr uint32_t kBarGroups = 8, kBarStride = 32;  // barrier groups; u32 words per counter line
 
-// Bounded sense-by-generation barrier over all workgroups of the launch. One
-// agent-scope release fence before arriving and one acquire fence after leaving;
-// the spin itself uses relaxed loads (an acquire load per spin iteration would
-// invalidate the L2 of every spinning workgroup's XCD on each poll).
+// Bounded sense-by-generation barrier over all workgroups of the launch, in two levels:
+// workgroup w arrives at the counter of group w % kBarGroups (each counter on a line of its
+// own), the last arrival of a group at the top counter, and the last of those advances the
+// generation. Arrivals at one counter are serialised by the memory system, so two levels cut
+// the serial chain from nblocks to nblocks / kBarGroups + kBarGroups. One agent-scope release
+// fence before arriving and one acquire fence after leaving; the spin itself uses relaxed
+// loads (an acquire load per spin iteration would invalidate the L2 of every spinning
+// workgroup's XCD on each poll). bar: [0] top count, [1] generation, [kBarStride * (1 + g)]
+// arrivals of group g.
 __device__ __forceinline__ bool grid_sync(uint32_t* bar, uint32_t nblocks, uint32_t* status) {
     __syncthreads();
     if (threadIdx.x == 0) {
         __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
         const uint32_t gen = __hip_atomic_load(bar + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        const uint32_t arrived = __hip_atomic_fetch_add(bar, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        if (arrived == nblocks - 1) {
+        const uint32_t g = blockIdx.x % kBarGroups, groups = nblocks < kBarGroups ? nblocks : kBarGroups;
+        const uint32_t in_group = nblocks / kBarGroups + (g < nblocks % kBarGroups ? 1u : 0u);
+        uint32_t* gc = bar + kBarStride * (1u + g);
+        bool last = false;
+        if (__hip_atomic_fetch_add(gc, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == in_group - 1u) {
+            // the reset completes before the top arrival (release): no workgroup can pass this
+            // barrier, and reach the group counter again, while the reset is in flight
+            __hip_atomic_store(gc, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            last = __hip_atomic_fetch_add(bar, 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT) == groups - 1u;
+        }
+        if (last) {
             __hip_atomic_store(bar, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
             __hip_atomic_store(bar + 1, gen + 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
         } else {
@@ -226,8 +241,11 @@ __global__ __launch_bounds__(64 * WAVES) void k_frame(FrameParams f) {
         }
         if (!grid_sync(f.bar, nb, f.status)) return;
 
-        // ---- phase B: normalise, draw, split counts per tile
-        const uint32_t n_tiles = (n + BLOCK - 1) / BLOCK;
+        // ---- phase B: normalise, draw, split counts. Workgroup w takes the contiguous tiles
+        // [w * tpw, (w + 1) * tpw), so that the queue is filled in tile order by the
+        // workgroups one after the other and only per-workgroup totals cross the barrier.
+        const uint32_t n_tiles = (n + BLOCK - 1) / BLOCK, tpw = (n_tiles + nb - 1) / nb;
+        const uint32_t t_begin = min(blockIdx.x * tpw, n_tiles), t_end = min(t_begin + tpw, n_tiles);
         {
             const double sum = (double)__hip_atomic_load(f.sums + b, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) *
                                0x1p-32;
@@ -235,9 +253,9 @@ __global__ __launch_bounds__(64 * WAVES) void k_frame(FrameParams f) {
             const float rcp = sum_f > 0.0f ? 1.0f / sum_f : 0.0f;
             float nf = (float)f.d.n_pixels * rcp;
             nf = nf * (f.d.use_ceil ? 1.0f : f.d.rate);
-            for (uint32_t t = blockIdx.x; t < n_tiles; t += nb) {
+            uint32_t cnt = 0;
+            for (uint32_t t = t_begin; t < t_end; ++t) {
                 const uint32_t i = t * BLOCK + threadIdx.x;
-                uint32_t si = 0;
                 if (i < n) {
                     float r = f.q_scr[i];
                     r = f.d.no_upscale ? r * (nf < 1.0f ? nf : 1.0f) : r * nf;
@@ -245,52 +263,52 @@ __global__ __launch_bounds__(64 * WAVES) void k_frame(FrameParams f) {
                     int s = (int)r;
                     const float left = r - (float)s;
                     s += (u <= left) ? 1 : 0;
-                    si = (uint32_t)min(s, (int)f.d.split_cap);
+                    const uint32_t si = (uint32_t)min(s, (int)f.d.split_cap);
                     f.s_scr[i] = (uint8_t)si;
+                    cnt += si;
                 }
+            }
+            uint32_t tot;
+            block_scan_excl<BLOCK>(cnt, &tot, wsum);
+            if (threadIdx.x == 0) f.tile_sums[blockIdx.x] = tot;  // survivors of this workgroup
+        }
+        if (!grid_sync(f.bar, nb, f.status)) return;
+
+        // ---- phase C: stable scatter of the survivors into the next queue. Every workgroup
+        // sums the totals of the workgroups before it (its first slot) and of all (the next
+        // queue size), then walks its tiles in order.
+        uint32_t n1;
+        {
+            uint32_t before = 0, all = 0;
+            for (uint32_t w = threadIdx.x; w < nb; w += BLOCK) {
+                const uint32_t v = f.tile_sums[w];
+                all += v;
+                before += w < blockIdx.x ? v : 0u;
+            }
+            uint32_t run, total;
+            block_scan_excl<BLOCK>(before, &run, wsum);
+            block_scan_excl<BLOCK>(all, &total, wsum);
+            n1 = total < cap ? total : cap;
+            if (blockIdx.x == 0 && threadIdx.x == 0) {
+                __hip_atomic_store(f.sizes + b + 1, n1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                if (total > cap) atomicOr(f.status, 2u);
+            }
+            for (uint32_t t = t_begin; t < t_end; ++t) {
+                const uint32_t i = t * BLOCK + threadIdx.x;
+                const uint32_t si = i < n ? f.s_scr[i] : 0u;
                 uint32_t tot;
-                block_scan_excl<BLOCK>(si, &tot, wsum);
-                if (threadIdx.x == 0) f.tile_sums[t] = tot;
-            }
-        }
-        if (!grid_sync(f.bar, nb, f.status)) return;
-
-        // ---- phase C: scan of the tile totals (one workgroup)
-        if (blockIdx.x == 0) {
-            const uint32_t per = (n_tiles + BLOCK - 1) / BLOCK;
-            const uint32_t t0 = threadIdx.x * per;
-            uint32_t local = 0;
-            for (uint32_t t = t0; t < t0 + per && t < n_tiles; ++t) local += f.tile_sums[t];
-            uint32_t tot;
-            uint32_t run = block_scan_excl<BLOCK>(local, &tot, wsum);
-            for (uint32_t t = t0; t < t0 + per && t < n_tiles; ++t) {
-                f.tile_offsets[t] = run;
-                run += f.tile_sums[t];
-            }
-            if (threadIdx.x == 0) {
-                const uint32_t kept = tot < cap ? tot : cap;
-                __hip_atomic_store(f.sizes + b + 1, kept, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-                if (tot > cap) atomicOr(f.status, 2u);
-            }
-        }
-        if (!grid_sync(f.bar, nb, f.status)) return;
-
-        // ---- phase D: stable scatter of the survivors into the next queue
-        for (uint32_t t = blockIdx.x; t < n_tiles; t += nb) {
-            const uint32_t i = t * BLOCK + threadIdx.x;
-            const uint32_t si = i < n ? f.s_scr[i] : 0u;
-            uint32_t tot;
-            const uint32_t pos = f.tile_offsets[t] + block_scan_excl<BLOCK>(si, &tot, wsum);
-            if (si) {
-                const uint32_t id = qin[i];
-                for (uint32_t c = 0; c < si; ++c)
-                    if (pos + c < cap) qout[pos + c] = id;
+                const uint32_t pos = run + block_scan_excl<BLOCK>(si, &tot, wsum);
+                if (si) {
+                    const uint32_t id = qin[i];
+                    for (uint32_t c = 0; c < si; ++c)
+                        if (pos + c < cap) qout[pos + c] = id;
+                }
+                run += tot;
             }
         }
         // positions of the next bounce (slot k draws vertex k of that bounce's seed); the
         // current positions were last read in phase A0
         if (b + 1 < f.n_bounces) {
-            const uint32_t n1 = __hip_atomic_load(f.sizes + b + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
             for (uint32_t k = blockIdx.x * BLOCK + threadIdx.x; k < n1; k += nb * BLOCK)
                 store_position(f, seed + (uint64_t)(b + 1) * kBounceSeedStep, k);
         }
@@ -342,6 +360,7 @@ hipError_t launch_frame_t(const FrameParams& f, uint32_t max_blocks, hipStream_t
     if (per_cu < 1) return hipErrorInvalidConfiguration;
     uint32_t blocks = (uint32_t)per_cu * (uint32_t)cus;  // all resident at once
     if (max_blocks && blocks > max_blocks) blocks = max_blocks;
+    if (blocks > kFrameMaxBlocks) blocks = kFrameMaxBlocks;  // the per-workgroup totals' scratch
     hipLaunchKernelGGL(kern, dim3(blocks), dim3(64 * WAVES), smem, s, f);
     return hipGetLastError();
 }

@@ -29,13 +29,17 @@ struct FrameParams {
     // scratch
     float* q_scr;
     uint8_t* s_scr;
-    uint32_t* tile_sums;
-    uint32_t* tile_offsets;
+    uint32_t* tile_sums;          // per-workgroup survivor totals (at least kFrameMaxBlocks)
     unsigned long long* sums;     // [n_bounces]
     float* xbuf;                  // [3][capacity] normalised positions of the current bounce
     uint32_t* genc;               // [2][levels][capacity] level-major grid encodings (StatNet, RRSNet)
-    uint32_t* bar;                // [2] barrier count / generation
+    uint32_t* bar;                // [kFrameBarWords] grid barrier
 };
+
+// Workgroups of a frame launch at most; the tile-sum scratch holds at least this many totals.
+constexpr uint32_t kFrameMaxBlocks = 4096;
+// u32 words of the grid barrier: top count, generation and 8 group counters on 128-byte lines.
+constexpr uint32_t kFrameBarWords = 32 * 9;
 
 hipError_t launch_frame(int precision, const FrameParams& f, int waves, uint32_t max_blocks, hipStream_t s);
 hipError_t launch_bounce_vertices(uint64_t seed, uint32_t bounce, const uint32_t* queue, uint32_t n,
