@@ -8,13 +8,26 @@
 namespace nrrs {
 namespace {
 
+// FP16 infinity / NaN: all exponent bits set (each half of a 32-bit word)
+__device__ __forceinline__ bool half_pair_bad(uint32_t w) {
+    return (w & 0x7c00u) == 0x7c00u || (w & 0x7c000000u) == 0x7c000000u;
+}
+
 __global__ void k_check_finite(const float* __restrict__ g, uint64_t n, const uint16_t* __restrict__ glp,
                                uint64_t n_lp, uint32_t* flag) {
     bool bad = false;
     const uint64_t i0 = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x, step = (uint64_t)gridDim.x * blockDim.x;
     for (uint64_t i = i0; i < n; i += step) bad |= !isfinite(g[i]);
-    // FP16 infinity / NaN: all exponent bits set
-    for (uint64_t i = i0; glp && i < n_lp; i += step) bad |= (glp[i] & 0x7c00u) == 0x7c00u;
+    if (glp) {
+        // 8 halves per 16-byte load (the buffer is 16-byte aligned), then the tail
+        const uint4* g8 = reinterpret_cast<const uint4*>(glp);
+        const uint64_t n8 = n_lp / 8;
+        for (uint64_t k = i0; k < n8; k += step) {
+            const uint4 q = g8[k];
+            bad |= half_pair_bad(q.x) | half_pair_bad(q.y) | half_pair_bad(q.z) | half_pair_bad(q.w);
+        }
+        for (uint64_t i = 8 * n8 + i0; i < n_lp; i += step) bad |= (glp[i] & 0x7c00u) == 0x7c00u;
+    }
     if (__any(bad) && (threadIdx.x & 63) == 0) atomicOr(flag, 1u);
 }
 

@@ -571,6 +571,9 @@ bool side_ok(const nrrs_rrs_side* side) {
 }
 
 void free_ctx(nrrs_ctx* c) {
+    if (c->ev_fork) hipEventDestroy(c->ev_fork);
+    if (c->ev_join) hipEventDestroy(c->ev_join);
+    if (c->aux_stream) hipStreamDestroy(c->aux_stream);
     hipFree(c->err_part);
     hipFree(c->nodes_mem);
     for (auto& ns : c->net) {
@@ -779,6 +782,10 @@ nrrs_status nrrs_create(const nrrs_config* cfg, nrrs_ctx** out) {
     }
     if (st == NRRS_OK) st = dmalloc(&c->q_scratch, cfg->max_queue);
     if (st == NRRS_OK) st = dmalloc(&c->s_scratch, cfg->max_queue);
+    // helper stream at the default priority, and the events of its fork and join
+    if (st == NRRS_OK) st = hip_status(hipStreamCreateWithFlags(&c->aux_stream, hipStreamNonBlocking));
+    if (st == NRRS_OK) st = hip_status(hipEventCreateWithFlags(&c->ev_fork, hipEventDisableTiming));
+    if (st == NRRS_OK) st = hip_status(hipEventCreateWithFlags(&c->ev_join, hipEventDisableTiming));
     if (st != NRRS_OK) {
         free_ctx(c);
         return st;
@@ -990,16 +997,41 @@ nrrs_status nrrs_train_step(nrrs_ctx* ctx, uint32_t net_mask, uint32_t n, const 
     hipStream_t s = (hipStream_t)stream;
     if (loss_dev) NRRS_HIP(launch_fill_f32(loss_dev, 0.0f, 2, s));
     const uint32_t ms = ctx->cfg.max_train;
+    bool forked = false;
+    // first-use allocations (they synchronise the device) before any fork
+    if (net_mask & 1u) NRRS_TRY(ensure_train(ctx, NRRS_STATNET));
+    if (net_mask & 2u) NRRS_TRY(ensure_train(ctx, NRRS_RRSNET));
     if (net_mask & 1u) {
-        NRRS_TRY(ensure_train(ctx, NRRS_STATNET));
         NetState& ns = ctx->net[NRRS_STATNET];
         // StatNet step (myNetworkLL2.cpp:41-89); its forward outputs feed RRSNet
         NRRS_TRY(run_backward(ctx, NRRS_STATNET, n, *v, film_dev, nullptr, 0, NRRS_LOSS_LL2, nullptr, target_L_dev,
                               nullptr, ns.loss_scale, ctx->train_stats, ms, loss_dev, s));
-        NRRS_TRY(run_adam(ctx, NRRS_STATNET, s));
+        if ((net_mask & 2u) && ctx->aux_stream) {
+            // The StatNet update touches only StatNet's buffers: it runs on the helper stream
+            // beside the RRSNet backward (HBM-bound Adam next to the L2-bound encode pass).
+            NRRS_HIP(hipEventRecord(ctx->ev_fork, s));
+            NRRS_HIP(hipStreamWaitEvent(ctx->aux_stream, ctx->ev_fork, 0));
+            const nrrs_status st = run_adam(ctx, NRRS_STATNET, ctx->aux_stream);
+            NRRS_HIP(hipEventRecord(ctx->ev_join, ctx->aux_stream));
+            forked = true;
+            if (st != NRRS_OK) {
+                NRRS_HIP(hipStreamWaitEvent(s, ctx->ev_join, 0));
+                return st;
+            }
+        } else {
+            NRRS_TRY(run_adam(ctx, NRRS_STATNET, s));
+        }
     }
+    // the helper stream's work joins the caller's stream on every return from here
+    struct Join {
+        nrrs_ctx* c;
+        hipStream_t s;
+        bool on;
+        ~Join() {
+            if (on) (void)hipStreamWaitEvent(s, c->ev_join, 0);
+        }
+    } join{ctx, s, forked};
     if (net_mask & 2u) {
-        NRRS_TRY(ensure_train(ctx, NRRS_RRSNET));
         if (!(net_mask & 1u)) {
             if (!ctx->train_stats) NRRS_TRY(dmalloc(&ctx->train_stats, (size_t)6 * ms));
             EncodeArgs ea = encode_args(ctx, *v, nullptr, nullptr, 0);

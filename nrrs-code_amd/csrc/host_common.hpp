@@ -154,11 +154,11 @@ struct nrrs_ctx {
     uint32_t frame_cap = 0;
     float* frame_q = nullptr;
     uint8_t* frame_s = nullptr;
-    uint32_t* frame_tiles = nullptr;        // [2][cap/64 + 1]
+    uint32_t* frame_tiles = nullptr;        // per-workgroup survivor totals of the frame kernel
     float* frame_x = nullptr;               // [3][cap] positions of the current bounce
     uint32_t* frame_genc = nullptr;         // [2][levels][cap] level-major encodings
     unsigned long long* frame_sums = nullptr;  // [16]
-    uint32_t* frame_bar = nullptr;          // [2]
+    uint32_t* frame_bar = nullptr;          // frame kernel's grid barrier
     uint32_t frame_waves = 4, frame_m = 128;
     hipGraph_t frame_graph = nullptr;
     hipGraphExec_t frame_exec = nullptr;
@@ -166,6 +166,10 @@ struct nrrs_ctx {
     float* train_stats = nullptr;           // [6][max_train] StatNet outputs of the batch
     float* scale_tmp = nullptr;             // device scalar for nrrs_backward's loss scale
     uint32_t max_train_blocks = 512;
+    // helper stream (default priority) on which a joint training step runs the StatNet update
+    // beside the RRSNet backward; joined to the caller's stream before the step returns
+    hipStream_t aux_stream = nullptr;
+    hipEvent_t ev_fork = nullptr, ev_join = nullptr;
 };
 
 namespace nrrs {
