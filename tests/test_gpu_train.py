@@ -151,6 +151,32 @@ def test_low_precision_table_gradient_fallbacks(torch_cuda, ora, case):
     assert n_grid < GRID_NORM_TOL[prec]
 
 
+@pytest.mark.parametrize("n", [777, 300])
+def test_low_precision_backward_ragged_batch(torch_cuda, ora, n):
+    """BF16 StatNet backward at batch sizes that are no multiple of the 256-vertex runs of the
+    record counts or of the 512-vertex workgroups of the record kernel (partial runs, a partial
+    last workgroup, scan tails), against the FP64 oracle."""
+    torch = torch_cuda
+    prec, net = na.BF16, na.STATNET
+    ctx, vb, film_np, film = _setup(torch, prec, n=n, seed=7)
+    hv = ora.HostVertices(vb.to_numpy())
+    rng = np.random.default_rng(11)
+    y64 = ora.net_forward(ctx.cfg, net, hv, film_np, None, ctx.params_get(net).astype(np.float64)).T
+    dy = (1e-3 * y64 + rng.normal(0, 1e-5, (6, n))).astype(np.float32)
+    ctx.grads_zero(net)
+    na.check(ctx.backward(net, vb, na.LOSS_EXTERNAL, film=film, dy=torch.from_numpy(dy).cuda(),
+                          loss_scale=SCALE[prec]), "backward")
+    g_dev = ctx.grads_get(net) / SCALE[prec]
+    p = ctx.params_get(net).astype(np.float64)
+    _, g_ref, _ = ora.train_grads(ctx.cfg, net, hv, p, 0, film=film_np, dy_ext=dy.astype(np.float64))
+    g, _m = ctx.param_count(net)
+    n_grid = float(np.linalg.norm(g_dev[:g] - g_ref[:g]) / np.linalg.norm(g_ref[:g]))
+    e_mlp = _rel(g_dev[g:], g_ref[g:])
+    print(f"n={n}: grid grad norm err {n_grid:.2e}, mlp grad err {e_mlp:.2e}")
+    assert e_mlp < GRAD_TOL[prec]
+    assert n_grid < GRID_NORM_TOL[prec]
+
+
 @pytest.mark.parametrize("prec", PRECISIONS)
 def test_ll2_and_rrs_loss_backward(torch_cuda, ora, prec):
     torch = torch_cuda
