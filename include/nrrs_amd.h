@@ -103,6 +103,12 @@ typedef struct {
     uint32_t max_train;  /* capacity of one training batch */
     uint64_t seed;       /* parameter initialisation seed */
     uint32_t max_nodes;  /* capacity of the path-node buffer (0: 4 x max_queue) */
+    /* FP16/BF16 contexts: 1 (default) keeps, beside each net's table, a paired table with both
+     * nets' entries interleaved (8 bytes per entry, 4 x the grid parameters in bytes), so that
+     * the chained inference, the joint training step and the frame kernel gather both nets'
+     * grids in one pass; 0 gathers each net's table in a pass of its own. Results are bitwise
+     * the same either way. No reference counterpart. */
+    uint32_t paired_tables;
 } nrrs_config;
 
 /* SoA path vertices of one bounce (replaces NRRSInferenceWorkItem, workitem.h:9-15). */

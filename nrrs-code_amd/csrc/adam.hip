@@ -55,8 +55,11 @@ __global__ void k_adam(AdamArgs a) {
         p -= a.lr * (m / c1) / (sqrtf(v / c2) + a.eps);
         a.params[i] = p;
         if (a.grid_lp && i < a.n_grid) {
-            a.grid_lp[i] = a.lp_precision == NRRS_BF16 ? __builtin_bit_cast(uint16_t, (__bf16)p)
-                                                        : __builtin_bit_cast(uint16_t, (_Float16)p);
+            const uint16_t b = a.lp_precision == NRRS_BF16 ? __builtin_bit_cast(uint16_t, (__bf16)p)
+                                                            : __builtin_bit_cast(uint16_t, (_Float16)p);
+            a.grid_lp[i] = b;
+            // parameter i = 2 e + f of entry e: half 4 e + 2 net + f of the paired table
+            if (a.grid_pair) a.grid_pair[(i >> 1) * 4 + 2 * a.pair_net + (i & 1)] = b;
         }
     }
 }

@@ -165,7 +165,8 @@ nrrs_status refresh_lp(nrrs_ctx* c, int net, hipStream_t s) {
     if (c->cfg.precision == NRRS_FP32) return NRRS_OK;
     NetState& ns = c->net[net];
     return hip_status(launch_pack_lp(c->cfg.precision, ns.grid_f32, ns.n_grid, ns.mlp_f32, ns.n_layers, ns.n_out,
-                                     ns.grid_lp, ns.mlp_lp, s));
+                                     ns.grid_lp, ns.mlp_lp, s, reinterpret_cast<uint16_t*>(c->grid_pair),
+                                     (uint32_t)net));
 }
 
 // One fused encode + MLP launch of `net` in the context's precision.
@@ -183,6 +184,21 @@ hipError_t run_net(nrrs_ctx* c, int net, const EncodeArgs& a, int epi, uint32_t 
     hipError_t e = launch_encode_levels(c->cfg.precision, ae, ns.grid_lp, n, c->grid_enc, c->grid_enc_cap, s);
     if (e != hipSuccess) return e;
     return launch_net_mfma(c->cfg.precision, ae, net, ns.n_layers, ns.n_out, epi, n, ns.grid_lp, ns.mlp_lp, ep, s);
+}
+
+// MLP launch of `net` reading the level-major encoding `enc` of a paired encode pass.
+hipError_t run_net_pre(nrrs_ctx* c, int net, const EncodeArgs& a, int epi, uint32_t n, const EpiArgs& ep,
+                       const uint32_t* enc, hipStream_t s) {
+    const NetState& ns = c->net[net];
+    EncodeArgs ae = a;
+    ae.grid_enc = enc;
+    ae.grid_enc_stride = c->grid_enc_cap;
+    return launch_net_mfma(c->cfg.precision, ae, net, ns.n_layers, ns.n_out, epi, n, ns.grid_lp, ns.mlp_lp, ep, s);
+}
+
+// Whether a batch of n vertices takes the paired encode pass (both nets in one gather).
+bool paired_pass(const nrrs_ctx* c, uint32_t n) {
+    return c->grid_pair && c->grid_enc2 && n && n <= c->grid_enc_cap;
 }
 
 // Packed training tiles from the FP32 masters (after every parameter change).
@@ -322,10 +338,12 @@ nrrs_status ensure_train(nrrs_ctx* c, int net) {
 }
 
 // Forward + loss + backward of one net into its gradient buffer.
+// `pre_enc`: the level-major encoding of this batch and net (and, for a binned net, its record
+// counts) already written by a paired encode pass; null = run the net's own encode pass.
 nrrs_status run_backward(nrrs_ctx* c, int net, uint32_t n, const nrrs_vertices& v, const float* film,
                          const float* stats, uint32_t stats_stride, int kind, const float* dy,
                          const float* target_L, const nrrs_rrs_side* side, const float* scale, float* y_out,
-                         uint32_t out_stride, float* loss_out, hipStream_t s) {
+                         uint32_t out_stride, float* loss_out, hipStream_t s, const uint32_t* pre_enc = nullptr) {
     NetState& ns = c->net[net];
     const nrrs_config& cfg = c->cfg;
     if (n > cfg.max_train) return NRRS_ERR_SHAPE;  // per-sample training buffers hold max_train
@@ -334,10 +352,11 @@ nrrs_status run_backward(nrrs_ctx* c, int net, uint32_t n, const nrrs_vertices& 
     if (cfg.precision != NRRS_FP32) {
         // level-major grid gathers of the forward, read back by the training kernel
         if (!c->grid_enc || n > c->grid_enc_cap) return NRRS_ERR_SHAPE;
-        t.a.grid_enc = c->grid_enc;
+        t.a.grid_enc = pre_enc ? pre_enc : c->grid_enc;
         t.a.grid_enc_stride = c->grid_enc_cap;
-        NRRS_HIP(launch_encode_levels(cfg.precision, t.a, ns.grid_lp, n, c->grid_enc, c->grid_enc_cap, s,
-                                      ns.binned ? ns.bin_runs : nullptr));
+        if (!pre_enc)
+            NRRS_HIP(launch_encode_levels(cfg.precision, t.a, ns.grid_lp, n, c->grid_enc, c->grid_enc_cap, s,
+                                          ns.binned ? ns.bin_runs : nullptr));
     }
     t.net = net;
     t.n = n;
@@ -482,6 +501,8 @@ nrrs_status run_adam(nrrs_ctx* c, int net, hipStream_t s) {
     a.n = ns.n_grid + ns.n_mlp;
     a.n_grid = ns.n_grid;
     a.grid_lp = ns.grid_lp;
+    a.grid_pair = reinterpret_cast<uint16_t*>(c->grid_pair);
+    a.pair_net = (uint32_t)net;
     a.lp_precision = cfg.precision;
     a.lr = net == NRRS_STATNET ? cfg.lr_statnet : cfg.lr_rrsnet;
     a.beta1 = cfg.adam_beta1;
@@ -540,6 +561,7 @@ nrrs_status frame_params(nrrs_ctx* c, const nrrs_frame_args* a, FrameParams* f) 
     f->d = c->dec;
     f->table_stat = reinterpret_cast<const uint32_t*>(c->net[NRRS_STATNET].grid_lp);
     f->table_rrs = reinterpret_cast<const uint32_t*>(c->net[NRRS_RRSNET].grid_lp);
+    f->table_pair = reinterpret_cast<const uint2*>(c->grid_pair);
     f->w_stat = reinterpret_cast<const u32x4_t*>(c->net[NRRS_STATNET].mlp_lp);
     f->w_rrs = reinterpret_cast<const u32x4_t*>(c->net[NRRS_RRSNET].mlp_lp);
     f->film = a->film_dev;
@@ -620,6 +642,8 @@ void free_ctx(nrrs_ctx* c) {
     hipFree(c->tile_offsets);
     hipFree(c->stats_scratch);
     hipFree(c->grid_enc);
+    hipFree(c->grid_enc2);
+    hipFree(c->grid_pair);
     hipFree(c->q_scratch);
     hipFree(c->s_scratch);
     delete c;
@@ -704,6 +728,7 @@ nrrs_status nrrs_default_config(nrrs_config* c) {
     c->max_queue = 1u << 14;
     c->max_train = 1u << 14;
     c->seed = 0x4E525253ull;
+    c->paired_tables = 1;
     return NRRS_OK;
 }
 
@@ -749,6 +774,8 @@ nrrs_status nrrs_create(const nrrs_config* cfg, nrrs_ctx** out) {
     c->dec.n_pixels = cfg->n_pixels;
 
     nrrs_status st = NRRS_OK;
+    // paired table: one 32-bit word per entry and net, n_grid / 2 entries
+    if (cfg->precision != NRRS_FP32 && cfg->paired_tables) st = dmalloc(&c->grid_pair, n_grid);
     std::vector<float> host;
     for (int net = 0; net < 2 && st == NRRS_OK; ++net) {
         const nrrs_mlp_config& m = net == NRRS_STATNET ? cfg->statnet : cfg->rrsnet;
@@ -779,6 +806,7 @@ nrrs_status nrrs_create(const nrrs_config* cfg, nrrs_ctx** out) {
     if (st == NRRS_OK && cfg->precision != NRRS_FP32) {
         c->grid_enc_cap = cfg->max_queue > cfg->max_train ? cfg->max_queue : cfg->max_train;
         st = dmalloc(&c->grid_enc, (size_t)c->grid.n_levels * c->grid_enc_cap);
+        if (st == NRRS_OK && c->grid_pair) st = dmalloc(&c->grid_enc2, (size_t)c->grid.n_levels * c->grid_enc_cap);
     }
     if (st == NRRS_OK) st = dmalloc(&c->q_scratch, cfg->max_queue);
     if (st == NRRS_OK) st = dmalloc(&c->s_scratch, cfg->max_queue);
@@ -884,16 +912,27 @@ nrrs_status nrrs_rrs_infer_decide(nrrs_ctx* ctx, const nrrs_decide_args* a, void
         NRRS_HIP(run_net(ctx, NRRS_STATNET, ea, EPI_ADRRS, a->n, ep, s));
     } else if (a->q_mode == NRRS_Q_RRSNET || a->q_mode == NRRS_Q_RRSNET_CHAINED) {
         const float* stats = nullptr;
+        const bool paired = a->q_mode == NRRS_Q_RRSNET_CHAINED && paired_pass(ctx, a->n);
         if (a->q_mode == NRRS_Q_RRSNET_CHAINED) {
             EncodeArgs ea = encode_args(ctx, a->v, nullptr, nullptr, 0);
             EpiArgs es{};
             es.out = ctx->stats_scratch;
             es.out_stride = ctx->cfg.max_queue;
-            NRRS_HIP(run_net(ctx, NRRS_STATNET, ea, EPI_RAW, a->n, es, s));
+            if (paired) {
+                // both nets' grid encodings in one pass over the paired table
+                NRRS_HIP(launch_encode_levels_dual(ctx->cfg.precision, ea, ctx->grid_pair, a->n, ctx->grid_enc,
+                                                   ctx->grid_enc2, ctx->grid_enc_cap, s));
+                NRRS_HIP(run_net_pre(ctx, NRRS_STATNET, ea, EPI_RAW, a->n, es, ctx->grid_enc, s));
+            } else {
+                NRRS_HIP(run_net(ctx, NRRS_STATNET, ea, EPI_RAW, a->n, es, s));
+            }
             stats = ctx->stats_scratch;
         }
         EncodeArgs ea = encode_args(ctx, a->v, a->film_dev, stats, ctx->cfg.max_queue);
-        NRRS_HIP(run_net(ctx, NRRS_RRSNET, ea, EPI_RRS, a->n, ep, s));
+        if (paired)
+            NRRS_HIP(run_net_pre(ctx, NRRS_RRSNET, ea, EPI_RRS, a->n, ep, ctx->grid_enc2, s));
+        else
+            NRRS_HIP(run_net(ctx, NRRS_RRSNET, ea, EPI_RRS, a->n, ep, s));
     } else {
         return NRRS_ERR_ARG;
     }
@@ -1001,11 +1040,24 @@ nrrs_status nrrs_train_step(nrrs_ctx* ctx, uint32_t net_mask, uint32_t n, const 
     // first-use allocations (they synchronise the device) before any fork
     if (net_mask & 1u) NRRS_TRY(ensure_train(ctx, NRRS_STATNET));
     if (net_mask & 2u) NRRS_TRY(ensure_train(ctx, NRRS_RRSNET));
+    // joint step: both nets' encodings (and the record counts, the same for both) in one pass
+    // over the paired table, before either net's parameters change
+    const bool paired = (net_mask & 3u) == 3u && ctx->cfg.precision != NRRS_FP32 && paired_pass(ctx, n);
+    if (paired) {
+        NetState& s0 = ctx->net[NRRS_STATNET];
+        NetState& s1 = ctx->net[NRRS_RRSNET];
+        EncodeArgs ea = encode_args(ctx, *v, nullptr, nullptr, 0);
+        uint32_t* rc = s0.binned ? s0.bin_runs : (s1.binned ? s1.bin_runs : nullptr);
+        uint32_t* rc2 = s0.binned && s1.binned ? s1.bin_runs : nullptr;
+        NRRS_HIP(launch_encode_levels_dual(ctx->cfg.precision, ea, ctx->grid_pair, n, ctx->grid_enc, ctx->grid_enc2,
+                                           ctx->grid_enc_cap, s, rc, rc2));
+    }
     if (net_mask & 1u) {
         NetState& ns = ctx->net[NRRS_STATNET];
         // StatNet step (myNetworkLL2.cpp:41-89); its forward outputs feed RRSNet
         NRRS_TRY(run_backward(ctx, NRRS_STATNET, n, *v, film_dev, nullptr, 0, NRRS_LOSS_LL2, nullptr, target_L_dev,
-                              nullptr, ns.loss_scale, ctx->train_stats, ms, loss_dev, s));
+                              nullptr, ns.loss_scale, ctx->train_stats, ms, loss_dev, s,
+                              paired ? ctx->grid_enc : nullptr));
         if ((net_mask & 2u) && ctx->aux_stream) {
             // The StatNet update touches only StatNet's buffers: it runs on the helper stream
             // beside the RRSNet backward (HBM-bound Adam next to the L2-bound encode pass).
@@ -1042,7 +1094,8 @@ nrrs_status nrrs_train_step(nrrs_ctx* ctx, uint32_t net_mask, uint32_t n, const 
         }
         NetState& ns = ctx->net[NRRS_RRSNET];
         NRRS_TRY(run_backward(ctx, NRRS_RRSNET, n, *v, film_dev, ctx->train_stats, ms, NRRS_LOSS_RRS, nullptr,
-                              nullptr, side, ns.loss_scale, nullptr, 0, loss_dev ? loss_dev + 1 : nullptr, s));
+                              nullptr, side, ns.loss_scale, nullptr, 0, loss_dev ? loss_dev + 1 : nullptr, s,
+                              paired ? ctx->grid_enc2 : nullptr));
         NRRS_TRY(run_adam(ctx, NRRS_RRSNET, s));
     }
     return NRRS_OK;

@@ -298,6 +298,61 @@ __device__ __forceinline__ float2 grid_level_pairs(const typename L::T* __restri
     return acc;
 }
 
+// Both nets' FP16/BF16 tables interleaved per entry (paired table): entry e is the 8 bytes
+// {StatNet word, RRSNet word}, so one gather serves both nets. The x-adjacent entries of a dense
+// level, and of a hashed level with even x (e1 = e0 ^ 1), come in one 16-byte load; the other
+// hashed x-pairs add an 8-byte load for the second entry (half the lanes). That is 6 loads per
+// hashed level for both nets against 2 x 5 with the per-net tables. Returns {StatNet f0, f1,
+// RRSNet f0, f1}; each net's sum runs in corner order 0..7 with the same roundings as
+// grid_level_pairs, so the values are bitwise those of the per-net tables.
+template <typename Dec>
+__device__ __forceinline__ float4 grid_level_pairs_dual(const uint2* __restrict__ table, const GridParams& g, int l,
+                                                        const float* x) {
+    LevelCell lc = level_cell(g, l, x);
+    const uint2* t = table + g.offset[l];
+    uint2 v[8];
+    if ((g.dense_mask >> l) & 1u) {
+        const uint32_t r = g.res[l];
+#pragma unroll
+        for (int yz = 0; yz < 4; ++yz) {
+            const uint32_t cy = lc.c[1] + (yz & 1), cz = lc.c[2] + (yz >> 1);
+            uint4 q;
+            __builtin_memcpy(&q, t + lc.c[0] + r * (cy + r * cz), sizeof(q));  // 8-byte aligned
+            v[2 * yz] = make_uint2(q.x, q.y);
+            v[2 * yz + 1] = make_uint2(q.z, q.w);
+        }
+    } else {
+        // level starts are 8-entry aligned, so the pair {h0 & ~1, h0 | 1} is 16-byte aligned
+#pragma unroll
+        for (int yz = 0; yz < 4; ++yz) {
+            const uint32_t cy = lc.c[1] + (yz & 1), cz = lc.c[2] + (yz >> 1);
+            const uint32_t yzh = (cy * 2654435761u) ^ (cz * 805459861u);
+            const uint32_t h0 = (lc.c[0] ^ yzh) & g.hash_mask, h1 = ((lc.c[0] + 1) ^ yzh) & g.hash_mask;
+            const uint4 q = *reinterpret_cast<const uint4*>(t + (h0 & ~1u));
+            const bool odd = h0 & 1u;
+            v[2 * yz] = make_uint2(odd ? q.z : q.x, odd ? q.w : q.y);
+            uint2 other = make_uint2(odd ? q.x : q.z, odd ? q.y : q.w);  // entry h0 ^ 1
+            if ((h0 ^ h1) != 1u) {  // global address space stated, else this load becomes a flat one
+                typedef const __attribute__((address_space(1))) unsigned long long GlobalWords;
+                const unsigned long long w = *(GlobalWords*)(t + h1);
+                other = make_uint2((uint32_t)w, (uint32_t)(w >> 32));
+            }
+            v[2 * yz + 1] = other;
+        }
+    }
+    float4 acc = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+#pragma unroll
+    for (int cn = 0; cn < 8; ++cn) {
+        const float w = corner_weight(lc, cn);
+        const float2 s = Dec::pair(v[cn].x), r = Dec::pair(v[cn].y);
+        acc.x = __fmaf_rn(w, s.x, acc.x);
+        acc.y = __fmaf_rn(w, s.y, acc.y);
+        acc.z = __fmaf_rn(w, r.x, acc.z);
+        acc.w = __fmaf_rn(w, r.y, acc.w);
+    }
+    return acc;
+}
+
 // ---------------------------------------------------------------- encoding
 // Network input layout (80 channels):
 //   [0,32)  hash grid, feature f of level l at 2l+f

@@ -141,12 +141,35 @@ __global__ __launch_bounds__(64 * WAVES) void k_frame(FrameParams f) {
 
         // ---- phase A0: level-major grid encodings of both nets; a thread takes kA0Vpt vertices
         // of an item so that their gathers are in flight together (this kernel keeps few waves
-        // per CU resident)
+        // per CU resident). With the paired table an item covers both nets of a level.
         {
             constexpr uint32_t kSpan = BLOCK * kA0Vpt;
-            const uint32_t L = f.grid.n_levels, n_ch = (n + kSpan - 1) / kSpan, items = 2u * L * n_ch;
+            const uint32_t L = f.grid.n_levels, n_ch = (n + kSpan - 1) / kSpan;
+            const bool paired = f.table_pair != nullptr;
+            const uint32_t items = (paired ? 1u : 2u) * L * n_ch;
             for (uint32_t it = blockIdx.x; it < items; it += nb) {
                 const uint32_t nl = it / n_ch, base = (it % n_ch) * kSpan + threadIdx.x;  // nl = net * L + level
+                if (paired) {
+                    float4 e[kA0Vpt];
+#pragma unroll
+                    for (uint32_t j = 0; j < kA0Vpt; ++j) {
+                        const uint32_t i = base + j * BLOCK;
+                        const uint32_t ic = i < n ? i : 0u;  // out-of-range lanes gather vertex 0, not stored
+                        const float x[3] = {f.xbuf[ic], f.xbuf[cap + ic], f.xbuf[2 * cap + ic]};
+                        e[j] = grid_level_pairs_dual<P>(f.table_pair, f.grid, (int)nl, x);
+                    }
+#pragma unroll
+                    for (uint32_t j = 0; j < kA0Vpt; ++j) {
+                        const uint32_t i = base + j * BLOCK;
+                        if (i < n) {
+                            f.genc[(size_t)nl * cap + i] =
+                                (uint32_t)lp_bits<P>(e[j].x) | ((uint32_t)lp_bits<P>(e[j].y) << 16);
+                            f.genc[(size_t)(L + nl) * cap + i] =
+                                (uint32_t)lp_bits<P>(e[j].z) | ((uint32_t)lp_bits<P>(e[j].w) << 16);
+                        }
+                    }
+                    continue;
+                }
                 const uint32_t* table = nl < L ? f.table_stat : f.table_rrs;
                 float2 e[kA0Vpt];
 #pragma unroll

@@ -13,6 +13,7 @@ struct FrameParams {
     DecideParams d;
     const uint32_t* table_stat;   // FP16/BF16 grids (two features per word)
     const uint32_t* table_rrs;
+    const uint2* table_pair;      // both nets' grids interleaved per entry, or null
     const u32x4_t* w_stat;        // packed MFMA tiles
     const u32x4_t* w_rrs;
     const float* film;

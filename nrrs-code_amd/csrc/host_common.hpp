@@ -141,7 +141,9 @@ struct nrrs_ctx {
     unsigned long long* sum_dev = nullptr;  // [1]
     double* err_part = nullptr;             // per-block partial sums of the pixel error
     uint32_t* grid_enc = nullptr;           // [levels][grid_enc_cap] level-major encoding (FP16/BF16)
+    uint32_t* grid_enc2 = nullptr;          // RRSNet's encoding of a paired pass, same shape
     uint32_t grid_enc_cap = 0;
+    uint32_t* grid_pair = nullptr;          // paired table: [entries][2 nets] FP16/BF16 words, or null
     nrrs::NodeBuf nodes;                    // path nodes of the training paths
     void* nodes_mem = nullptr;
     uint32_t* tile_sums = nullptr;          // [max tiles]
@@ -182,14 +184,23 @@ hipError_t launch_net_fp32(const EncodeArgs& a, int net, int n_layers, int n_out
                            const float* table, const float* W, const EpiArgs& ep, hipStream_t s);
 
 size_t mfma_packed_halves(int n_layers);
+// FP16/BF16 shadows of one net: packed MLP tiles and, with `grid`, the grid table; with
+// `grid_pair`, also the net's half of the paired table (both nets' entries interleaved).
 hipError_t launch_pack_lp(int precision, const float* grid, uint64_t n_grid, const float* W, int n_layers,
-                          int n_out, uint16_t* grid_lp, uint16_t* W_lp, hipStream_t s);
+                          int n_out, uint16_t* grid_lp, uint16_t* W_lp, hipStream_t s, uint16_t* grid_pair = nullptr,
+                          uint32_t net = 0);
 // Level-major grid encoding: one workgroup per (level, 256 vertices), levels in order, so the
 // whole GPU gathers from one level's table at a time (it stays in L2). With run_counts, also
 // counts the corners of every (level, table segment, 256 vertices) for the binned scatter:
 // run_counts[(level * ceil(n / 256) + chunk) * kMaxSegBins + segment].
 hipError_t launch_encode_levels(int precision, const EncodeArgs& a, const uint16_t* table, uint32_t n, uint32_t* out,
                                 uint32_t stride, hipStream_t s, uint32_t* run_counts = nullptr);
+// The same pass for both nets from the paired table (entry e = {StatNet word, RRSNet word}):
+// StatNet's encoding to out_stat, RRSNet's to out_rrs, the record counts (one set serves both
+// nets) to run_counts and run_counts2 where given.
+hipError_t launch_encode_levels_dual(int precision, const EncodeArgs& a, const uint32_t* pair_table, uint32_t n,
+                                     uint32_t* out_stat, uint32_t* out_rrs, uint32_t stride, hipStream_t s,
+                                     uint32_t* run_counts = nullptr, uint32_t* run_counts2 = nullptr);
 hipError_t launch_net_mfma(int precision, const EncodeArgs& a, int net, int n_layers, int n_out, int epi,
                            uint32_t n, const uint16_t* table, const uint16_t* Wp, const EpiArgs& ep,
                            hipStream_t s);

@@ -45,26 +45,44 @@ __global__ void k_pack_mlp(const float* __restrict__ W, int n_layers, int n_out,
     }
 }
 
+// FP32 grid -> FP16/BF16 shadow; with `pair`, also the net's half of the paired table
+// (parameter i = 2 e + f of entry e goes to half 4 e + 2 net + f).
 template <typename P>
-__global__ void k_to_lp(const float* __restrict__ src, uint16_t* __restrict__ dst, uint64_t n) {
-    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x)
-        dst[i] = lp_bits<P>(src[i]);
+__global__ void k_to_lp(const float* __restrict__ src, uint16_t* __restrict__ dst, uint64_t n,
+                        uint16_t* __restrict__ pair, uint32_t net) {
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x) {
+        const uint16_t b = lp_bits<P>(src[i]);
+        dst[i] = b;
+        if (pair) pair[(i >> 1) * 4 + 2 * net + (i & 1)] = b;
+    }
 }
 
 // Level-major grid encoding: workgroup b takes level b / n_chunks and vertices
 // 256 (b % n_chunks) .. +255, so consecutive workgroups share one level's table.
-template <typename P>
+// DUAL: `table` is the paired table of both nets; StatNet's encoding goes to `out`, RRSNet's
+// to `out2`, and the record counts (the same for both nets, whose tables share one layout) to
+// `run_counts` and, when given, `run_counts2`.
+template <typename P, bool DUAL>
 __global__ __launch_bounds__(kRunVert) void k_encode_levels(EncodeArgs a, const uint32_t* __restrict__ table, uint32_t n,
                                                             uint32_t n_chunks, uint32_t* __restrict__ out,
-                                                            uint32_t stride, uint32_t* __restrict__ run_counts) {
+                                                            uint32_t* __restrict__ out2, uint32_t stride,
+                                                            uint32_t* __restrict__ run_counts,
+                                                            uint32_t* __restrict__ run_counts2) {
     const uint32_t l = blockIdx.x / n_chunks, chunk = blockIdx.x % n_chunks, i = chunk * kRunVert + threadIdx.x;
     if (l >= a.grid.n_levels) return;  // whole workgroup
     const bool live = i < n;
     float x[3];
     if (live) {
         load_xn(a, i, x);
-        const float2 f = grid_level_pairs<PairLoadLp<P>>(table, a.grid, (int)l, x);
-        out[(size_t)l * stride + i] = (uint32_t)lp_bits<P>(f.x) | ((uint32_t)lp_bits<P>(f.y) << 16);
+        const size_t o = (size_t)l * stride + i;
+        if constexpr (DUAL) {
+            const float4 f = grid_level_pairs_dual<P>(reinterpret_cast<const uint2*>(table), a.grid, (int)l, x);
+            out[o] = (uint32_t)lp_bits<P>(f.x) | ((uint32_t)lp_bits<P>(f.y) << 16);
+            out2[o] = (uint32_t)lp_bits<P>(f.z) | ((uint32_t)lp_bits<P>(f.w) << 16);
+        } else {
+            const float2 f = grid_level_pairs<PairLoadLp<P>>(table, a.grid, (int)l, x);
+            out[o] = (uint32_t)lp_bits<P>(f.x) | ((uint32_t)lp_bits<P>(f.y) << 16);
+        }
     }
     if (run_counts) {  // uniform: table-gradient records per segment of this level and chunk
         __shared__ uint32_t hist[kMaxSegBins];
@@ -88,8 +106,11 @@ __global__ __launch_bounds__(kRunVert) void k_encode_levels(EncodeArgs a, const 
             }
         }
         __syncthreads();
-        if (threadIdx.x < kMaxSegBins)
-            run_counts[((size_t)l * n_chunks + chunk) * kMaxSegBins + threadIdx.x] = hist[threadIdx.x];
+        if (threadIdx.x < kMaxSegBins) {
+            const size_t o = ((size_t)l * n_chunks + chunk) * kMaxSegBins + threadIdx.x;
+            run_counts[o] = hist[threadIdx.x];
+            if (DUAL && run_counts2) run_counts2[o] = hist[threadIdx.x];
+        }
     }
 }
 
@@ -199,11 +220,28 @@ hipError_t launch_encode_levels(int precision, const EncodeArgs& a, const uint16
     const dim3 grid(n_chunks * a.grid.n_levels);
     const uint32_t* t = reinterpret_cast<const uint32_t*>(table);
     if (precision == NRRS_BF16)
-        hipLaunchKernelGGL(k_encode_levels<BF16>, grid, dim3(kRunVert), 0, s, a, t, n, n_chunks, out, stride,
-                           run_counts);
+        hipLaunchKernelGGL((k_encode_levels<BF16, false>), grid, dim3(kRunVert), 0, s, a, t, n, n_chunks, out,
+                           nullptr, stride, run_counts, nullptr);
     else if (precision == NRRS_FP16)
-        hipLaunchKernelGGL(k_encode_levels<F16>, grid, dim3(kRunVert), 0, s, a, t, n, n_chunks, out, stride,
-                           run_counts);
+        hipLaunchKernelGGL((k_encode_levels<F16, false>), grid, dim3(kRunVert), 0, s, a, t, n, n_chunks, out,
+                           nullptr, stride, run_counts, nullptr);
+    else
+        return hipErrorInvalidValue;
+    return hipGetLastError();
+}
+
+hipError_t launch_encode_levels_dual(int precision, const EncodeArgs& a, const uint32_t* pair_table, uint32_t n,
+                                     uint32_t* out_stat, uint32_t* out_rrs, uint32_t stride, hipStream_t s,
+                                     uint32_t* run_counts, uint32_t* run_counts2) {
+    if (!n) return hipSuccess;
+    const uint32_t n_chunks = (n + kRunVert - 1) / kRunVert;
+    const dim3 grid(n_chunks * a.grid.n_levels);
+    if (precision == NRRS_BF16)
+        hipLaunchKernelGGL((k_encode_levels<BF16, true>), grid, dim3(kRunVert), 0, s, a, pair_table, n, n_chunks,
+                           out_stat, out_rrs, stride, run_counts, run_counts2);
+    else if (precision == NRRS_FP16)
+        hipLaunchKernelGGL((k_encode_levels<F16, true>), grid, dim3(kRunVert), 0, s, a, pair_table, n, n_chunks,
+                           out_stat, out_rrs, stride, run_counts, run_counts2);
     else
         return hipErrorInvalidValue;
     return hipGetLastError();
@@ -212,14 +250,17 @@ hipError_t launch_encode_levels(int precision, const EncodeArgs& a, const uint16
 size_t mfma_packed_halves(int n_layers) { return (size_t)tiles_of(n_layers) * kTileHalves; }
 
 hipError_t launch_pack_lp(int precision, const float* grid, uint64_t n_grid, const float* W, int n_layers,
-                          int n_out, uint16_t* grid_lp, uint16_t* W_lp, hipStream_t s) {
+                          int n_out, uint16_t* grid_lp, uint16_t* W_lp, hipStream_t s, uint16_t* grid_pair,
+                          uint32_t net) {
     const int tiles = tiles_of(n_layers);
     if (precision == NRRS_BF16) {
         hipLaunchKernelGGL(k_pack_mlp<BF16>, dim3(tiles), dim3(64), 0, s, W, n_layers, n_out, W_lp);
-        if (grid) hipLaunchKernelGGL(k_to_lp<BF16>, dim3(2048), dim3(256), 0, s, grid, grid_lp, n_grid);
+        if (grid)
+            hipLaunchKernelGGL(k_to_lp<BF16>, dim3(2048), dim3(256), 0, s, grid, grid_lp, n_grid, grid_pair, net);
     } else if (precision == NRRS_FP16) {
         hipLaunchKernelGGL(k_pack_mlp<F16>, dim3(tiles), dim3(64), 0, s, W, n_layers, n_out, W_lp);
-        if (grid) hipLaunchKernelGGL(k_to_lp<F16>, dim3(2048), dim3(256), 0, s, grid, grid_lp, n_grid);
+        if (grid)
+            hipLaunchKernelGGL(k_to_lp<F16>, dim3(2048), dim3(256), 0, s, grid, grid_lp, n_grid, grid_pair, net);
     } else {
         return hipErrorInvalidValue;
     }

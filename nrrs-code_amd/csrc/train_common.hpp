@@ -139,6 +139,8 @@ struct AdamArgs {
     uint64_t n;
     uint64_t n_grid;      // leading grid parameters (their low-precision shadow is refreshed)
     uint16_t* grid_lp;    // or null
+    uint16_t* grid_pair;  // paired table of both nets (its half `pair_net` is refreshed) or null
+    uint32_t pair_net;
     int lp_precision;
     float lr, beta1, beta2, eps, l2;
     float* loss_scale;    // device

@@ -68,6 +68,7 @@ class Config(C.Structure):
         ("max_train", C.c_uint32),
         ("seed", C.c_uint64),
         ("max_nodes", C.c_uint32),
+        ("paired_tables", C.c_uint32),
     ]
 
 
