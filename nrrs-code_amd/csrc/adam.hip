@@ -58,8 +58,6 @@ __global__ void k_adam(AdamArgs a) {
             const uint16_t b = a.lp_precision == NRRS_BF16 ? __builtin_bit_cast(uint16_t, (__bf16)p)
                                                             : __builtin_bit_cast(uint16_t, (_Float16)p);
             a.grid_lp[i] = b;
-            // parameter i = 2 e + f of entry e: half 4 e + 2 net + f of the paired table
-            if (a.grid_pair) a.grid_pair[(i >> 1) * 4 + 2 * a.pair_net + (i & 1)] = b;
         }
     }
 }
@@ -81,6 +79,22 @@ __global__ void k_adam_finish(AdamArgs a) {
     st[3] = 0;
 }
 
+// Paired table from the two nets' FP16/BF16 grids: entry e = {StatNet word e, RRSNet word e}.
+// Two entries per thread: 8-byte reads, one 16-byte write (full lines, no partial writes).
+__global__ void k_pack_pair(const uint32_t* __restrict__ a, const uint32_t* __restrict__ b, uint4* __restrict__ pair,
+                            uint64_t n_entries) {
+    const uint64_t n2 = n_entries / 2;
+    for (uint64_t k = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; k < n2; k += (uint64_t)gridDim.x * blockDim.x) {
+        const uint2 x = reinterpret_cast<const uint2*>(a)[k], y = reinterpret_cast<const uint2*>(b)[k];
+        pair[k] = make_uint4(x.x, y.x, x.y, y.y);
+    }
+    if ((n_entries & 1) && blockIdx.x == 0 && threadIdx.x == 0) {
+        uint32_t* p = reinterpret_cast<uint32_t*>(pair) + 2 * (n_entries - 1);
+        p[0] = a[n_entries - 1];
+        p[1] = b[n_entries - 1];
+    }
+}
+
 __global__ void k_fill(float* p, float v, uint64_t n) {
     for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x)
         p[i] = v;
@@ -95,6 +109,14 @@ hipError_t launch_adam(const AdamArgs& a, hipStream_t s) {
                        a.grad_lp ? a.n_grid : 0, a.state + 3);
     hipLaunchKernelGGL(k_adam, dim3(2048), dim3(256), 0, s, a);
     hipLaunchKernelGGL(k_adam_finish, dim3(1), dim3(1), 0, s, a);
+    return hipGetLastError();
+}
+
+hipError_t launch_pack_pair(const uint16_t* stat_lp, const uint16_t* rrs_lp, uint32_t* pair, uint64_t n_entries,
+                            hipStream_t s) {
+    if (n_entries)
+        hipLaunchKernelGGL(k_pack_pair, dim3(2048), dim3(256), 0, s, reinterpret_cast<const uint32_t*>(stat_lp),
+                           reinterpret_cast<const uint32_t*>(rrs_lp), reinterpret_cast<uint4*>(pair), n_entries);
     return hipGetLastError();
 }
 

@@ -160,13 +160,20 @@ bool vertices_ok(const nrrs_vertices* v, uint32_t n) {
            v->pixel && v->stride >= n;
 }
 
+// Paired table from both nets' FP16/BF16 grids; after every change of either.
+nrrs_status refresh_pair(nrrs_ctx* c, hipStream_t s) {
+    if (!c->grid_pair || !c->net[NRRS_STATNET].grid_lp || !c->net[NRRS_RRSNET].grid_lp) return NRRS_OK;
+    return hip_status(launch_pack_pair(c->net[NRRS_STATNET].grid_lp, c->net[NRRS_RRSNET].grid_lp, c->grid_pair,
+                                       c->net[NRRS_STATNET].n_grid / 2, s));
+}
+
 // Low-precision shadows (grid table and packed MFMA tiles) from the FP32 masters.
 nrrs_status refresh_lp(nrrs_ctx* c, int net, hipStream_t s) {
     if (c->cfg.precision == NRRS_FP32) return NRRS_OK;
     NetState& ns = c->net[net];
-    return hip_status(launch_pack_lp(c->cfg.precision, ns.grid_f32, ns.n_grid, ns.mlp_f32, ns.n_layers, ns.n_out,
-                                     ns.grid_lp, ns.mlp_lp, s, reinterpret_cast<uint16_t*>(c->grid_pair),
-                                     (uint32_t)net));
+    NRRS_HIP(launch_pack_lp(c->cfg.precision, ns.grid_f32, ns.n_grid, ns.mlp_f32, ns.n_layers, ns.n_out,
+                            ns.grid_lp, ns.mlp_lp, s));
+    return refresh_pair(c, s);
 }
 
 // One fused encode + MLP launch of `net` in the context's precision.
@@ -501,8 +508,6 @@ nrrs_status run_adam(nrrs_ctx* c, int net, hipStream_t s) {
     a.n = ns.n_grid + ns.n_mlp;
     a.n_grid = ns.n_grid;
     a.grid_lp = ns.grid_lp;
-    a.grid_pair = reinterpret_cast<uint16_t*>(c->grid_pair);
-    a.pair_net = (uint32_t)net;
     a.lp_precision = cfg.precision;
     a.lr = net == NRRS_STATNET ? cfg.lr_statnet : cfg.lr_rrsnet;
     a.beta1 = cfg.adam_beta1;
@@ -1008,7 +1013,8 @@ nrrs_status nrrs_grads_get(nrrs_ctx* ctx, int32_t net, float* host_dst) {
 nrrs_status nrrs_adam_step(nrrs_ctx* ctx, int32_t net, void* stream) {
     if (!ctx || (net != 0 && net != 1)) return NRRS_ERR_ARG;
     NRRS_TRY(ensure_train(ctx, net));
-    return run_adam(ctx, net, (hipStream_t)stream);
+    NRRS_TRY(run_adam(ctx, net, (hipStream_t)stream));
+    return refresh_pair(ctx, (hipStream_t)stream);
 }
 
 nrrs_status nrrs_train_state(nrrs_ctx* ctx, int32_t net, uint32_t* step, float* loss_scale, uint32_t* skipped) {
@@ -1098,7 +1104,12 @@ nrrs_status nrrs_train_step(nrrs_ctx* ctx, uint32_t net_mask, uint32_t n, const 
                               paired ? ctx->grid_enc2 : nullptr));
         NRRS_TRY(run_adam(ctx, NRRS_RRSNET, s));
     }
-    return NRRS_OK;
+    // the paired table follows both nets' updated grids once the StatNet update has joined
+    if (join.on) {
+        NRRS_HIP(hipStreamWaitEvent(s, ctx->ev_join, 0));
+        join.on = false;
+    }
+    return refresh_pair(ctx, s);
 }
 
 nrrs_status nrrs_frame_set_shape(nrrs_ctx* ctx, const nrrs_tile_shape* shape) {

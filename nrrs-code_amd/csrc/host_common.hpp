@@ -184,11 +184,9 @@ hipError_t launch_net_fp32(const EncodeArgs& a, int net, int n_layers, int n_out
                            const float* table, const float* W, const EpiArgs& ep, hipStream_t s);
 
 size_t mfma_packed_halves(int n_layers);
-// FP16/BF16 shadows of one net: packed MLP tiles and, with `grid`, the grid table; with
-// `grid_pair`, also the net's half of the paired table (both nets' entries interleaved).
+// FP16/BF16 shadows of one net: packed MLP tiles and, with `grid`, the grid table.
 hipError_t launch_pack_lp(int precision, const float* grid, uint64_t n_grid, const float* W, int n_layers,
-                          int n_out, uint16_t* grid_lp, uint16_t* W_lp, hipStream_t s, uint16_t* grid_pair = nullptr,
-                          uint32_t net = 0);
+                          int n_out, uint16_t* grid_lp, uint16_t* W_lp, hipStream_t s);
 // Level-major grid encoding: one workgroup per (level, 256 vertices), levels in order, so the
 // whole GPU gathers from one level's table at a time (it stays in L2). With run_counts, also
 // counts the corners of every (level, table segment, 256 vertices) for the binned scatter:

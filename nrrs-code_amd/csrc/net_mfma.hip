@@ -45,16 +45,10 @@ __global__ void k_pack_mlp(const float* __restrict__ W, int n_layers, int n_out,
     }
 }
 
-// FP32 grid -> FP16/BF16 shadow; with `pair`, also the net's half of the paired table
-// (parameter i = 2 e + f of entry e goes to half 4 e + 2 net + f).
 template <typename P>
-__global__ void k_to_lp(const float* __restrict__ src, uint16_t* __restrict__ dst, uint64_t n,
-                        uint16_t* __restrict__ pair, uint32_t net) {
-    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x) {
-        const uint16_t b = lp_bits<P>(src[i]);
-        dst[i] = b;
-        if (pair) pair[(i >> 1) * 4 + 2 * net + (i & 1)] = b;
-    }
+__global__ void k_to_lp(const float* __restrict__ src, uint16_t* __restrict__ dst, uint64_t n) {
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x)
+        dst[i] = lp_bits<P>(src[i]);
 }
 
 // Level-major grid encoding: workgroup b takes level b / n_chunks and vertices
@@ -250,17 +244,16 @@ hipError_t launch_encode_levels_dual(int precision, const EncodeArgs& a, const u
 size_t mfma_packed_halves(int n_layers) { return (size_t)tiles_of(n_layers) * kTileHalves; }
 
 hipError_t launch_pack_lp(int precision, const float* grid, uint64_t n_grid, const float* W, int n_layers,
-                          int n_out, uint16_t* grid_lp, uint16_t* W_lp, hipStream_t s, uint16_t* grid_pair,
-                          uint32_t net) {
+                          int n_out, uint16_t* grid_lp, uint16_t* W_lp, hipStream_t s) {
     const int tiles = tiles_of(n_layers);
     if (precision == NRRS_BF16) {
         hipLaunchKernelGGL(k_pack_mlp<BF16>, dim3(tiles), dim3(64), 0, s, W, n_layers, n_out, W_lp);
         if (grid)
-            hipLaunchKernelGGL(k_to_lp<BF16>, dim3(2048), dim3(256), 0, s, grid, grid_lp, n_grid, grid_pair, net);
+            hipLaunchKernelGGL(k_to_lp<BF16>, dim3(2048), dim3(256), 0, s, grid, grid_lp, n_grid);
     } else if (precision == NRRS_FP16) {
         hipLaunchKernelGGL(k_pack_mlp<F16>, dim3(tiles), dim3(64), 0, s, W, n_layers, n_out, W_lp);
         if (grid)
-            hipLaunchKernelGGL(k_to_lp<F16>, dim3(2048), dim3(256), 0, s, grid, grid_lp, n_grid, grid_pair, net);
+            hipLaunchKernelGGL(k_to_lp<F16>, dim3(2048), dim3(256), 0, s, grid, grid_lp, n_grid);
     } else {
         return hipErrorInvalidValue;
     }

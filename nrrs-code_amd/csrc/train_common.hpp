@@ -139,8 +139,6 @@ struct AdamArgs {
     uint64_t n;
     uint64_t n_grid;      // leading grid parameters (their low-precision shadow is refreshed)
     uint16_t* grid_lp;    // or null
-    uint16_t* grid_pair;  // paired table of both nets (its half `pair_net` is refreshed) or null
-    uint32_t pair_net;
     int lp_precision;
     float lr, beta1, beta2, eps, l2;
     float* loss_scale;    // device
@@ -150,5 +148,9 @@ struct AdamArgs {
 };
 hipError_t launch_adam(const AdamArgs& a, hipStream_t s);
 hipError_t launch_fill_f32(float* p, float v, uint64_t n, hipStream_t s);
+// Paired table (entry e = {StatNet word e, RRSNet word e}) from the nets' FP16/BF16 grids,
+// both 16-byte aligned with n_entries words each.
+hipError_t launch_pack_pair(const uint16_t* stat_lp, const uint16_t* rrs_lp, uint32_t* pair, uint64_t n_entries,
+                            hipStream_t s);
 
 }  // namespace nrrs

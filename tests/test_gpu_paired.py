@@ -78,10 +78,19 @@ def test_joint_training_step_paired_equals_per_net(torch_cuda, ora, prec):
                         error_sum=0.7 * n)
         loss = torch.zeros(2, device="cuda")
         losses = []
-        for _ in range(3):
-            na.check(ctx.train_step(vb, 3, film=film, target_L=tL, side=side, loss=loss), "train_step")
+        # StatNet alone (its update then refreshes the paired table), then both nets jointly
+        for mask in (1, 1, 3, 3, 3):
+            na.check(ctx.train_step(vb, mask, film=film, target_L=tL, side=side, loss=loss), "train_step")
             losses.append(loss.cpu().numpy().copy())
-        res.append(dict(losses=np.array(losses), stat=ctx.params_get(na.STATNET), rrs=ctx.params_get(na.RRSNET)))
+        # the chained stage on the trained parameters reads the refreshed paired table
+        q = torch.empty(n, device="cuda")
+        surv = torch.empty(8 * n, dtype=torch.int32, device="cuda")
+        ns = torch.zeros(1, dtype=torch.int32, device="cuda")
+        args = ctx.decide_args(vb, film, q, surv, ns, q_mode=na.Q_RRSNET_CHAINED, seed=3, stream_id=2)
+        na.check(ctx.rrs_infer_decide(args), "rrs_infer_decide")
+        torch.cuda.synchronize()
+        res.append(dict(losses=np.array(losses), stat=ctx.params_get(na.STATNET), rrs=ctx.params_get(na.RRSNET),
+                        q=q.cpu().numpy()))
     a, b = res
     print(f"prec {prec}: StatNet losses {a['losses'][:, 0].tolist()} / {b['losses'][:, 0].tolist()}, "
           f"RRSNet {a['losses'][:, 1].tolist()} / {b['losses'][:, 1].tolist()}")
@@ -92,6 +101,7 @@ def test_joint_training_step_paired_equals_per_net(torch_cuda, ora, prec):
     # are bitwise equal; the loss values are summed with float atomics in no fixed order
     assert np.array_equal(a["stat"], b["stat"]) and np.array_equal(a["rrs"], b["rrs"])
     np.testing.assert_allclose(a["losses"], b["losses"], rtol=1e-5)
+    assert np.array_equal(a["q"], b["q"])
 
 
 def test_frame_paired_equals_per_net(torch_cuda):
