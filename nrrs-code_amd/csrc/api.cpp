@@ -193,16 +193,6 @@ hipError_t run_net(nrrs_ctx* c, int net, const EncodeArgs& a, int epi, uint32_t 
     return launch_net_mfma(c->cfg.precision, ae, net, ns.n_layers, ns.n_out, epi, n, ns.grid_lp, ns.mlp_lp, ep, s);
 }
 
-// MLP launch of `net` reading the level-major encoding `enc` of a paired encode pass.
-hipError_t run_net_pre(nrrs_ctx* c, int net, const EncodeArgs& a, int epi, uint32_t n, const EpiArgs& ep,
-                       const uint32_t* enc, hipStream_t s) {
-    const NetState& ns = c->net[net];
-    EncodeArgs ae = a;
-    ae.grid_enc = enc;
-    ae.grid_enc_stride = c->grid_enc_cap;
-    return launch_net_mfma(c->cfg.precision, ae, net, ns.n_layers, ns.n_out, epi, n, ns.grid_lp, ns.mlp_lp, ep, s);
-}
-
 // Whether a batch of n vertices takes the paired encode pass (both nets in one gather).
 bool paired_pass(const nrrs_ctx* c, uint32_t n) {
     return c->grid_pair && c->grid_enc2 && n && n <= c->grid_enc_cap;
@@ -811,7 +801,7 @@ nrrs_status nrrs_create(const nrrs_config* cfg, nrrs_ctx** out) {
     if (st == NRRS_OK && cfg->precision != NRRS_FP32) {
         c->grid_enc_cap = cfg->max_queue > cfg->max_train ? cfg->max_queue : cfg->max_train;
         st = dmalloc(&c->grid_enc, (size_t)c->grid.n_levels * c->grid_enc_cap);
-        if (st == NRRS_OK && c->grid_pair) st = dmalloc(&c->grid_enc2, (size_t)c->grid.n_levels * c->grid_enc_cap);
+        if (st == NRRS_OK) st = dmalloc(&c->grid_enc2, (size_t)c->grid.n_levels * c->grid_enc_cap);
     }
     if (st == NRRS_OK) st = dmalloc(&c->q_scratch, cfg->max_queue);
     if (st == NRRS_OK) st = dmalloc(&c->s_scratch, cfg->max_queue);
@@ -915,29 +905,37 @@ nrrs_status nrrs_rrs_infer_decide(nrrs_ctx* ctx, const nrrs_decide_args* a, void
     if (a->q_mode == NRRS_Q_STATNET_ADRRS) {
         EncodeArgs ea = encode_args(ctx, a->v, a->film_dev, nullptr, 0);
         NRRS_HIP(run_net(ctx, NRRS_STATNET, ea, EPI_ADRRS, a->n, ep, s));
+    } else if (a->q_mode == NRRS_Q_RRSNET_CHAINED && ctx->cfg.precision != NRRS_FP32 && ctx->grid_enc &&
+               ctx->grid_enc2 && a->n <= ctx->grid_enc_cap) {
+        // both nets' level-major grid encodings (one pass over the paired table, else a pass per
+        // net), then StatNet -> RRSNet -> q in one launch
+        const nrrs_config& cfg = ctx->cfg;
+        EncodeArgs ea = encode_args(ctx, a->v, a->film_dev, nullptr, 0);
+        ea.grid_enc = ctx->grid_enc;
+        ea.grid_enc_stride = ctx->grid_enc_cap;
+        if (paired_pass(ctx, a->n)) {
+            NRRS_HIP(launch_encode_levels_dual(cfg.precision, ea, ctx->grid_pair, a->n, ctx->grid_enc, ctx->grid_enc2,
+                                               ctx->grid_enc_cap, s));
+        } else {
+            NRRS_HIP(launch_encode_levels(cfg.precision, ea, ctx->net[NRRS_STATNET].grid_lp, a->n, ctx->grid_enc,
+                                          ctx->grid_enc_cap, s));
+            NRRS_HIP(launch_encode_levels(cfg.precision, ea, ctx->net[NRRS_RRSNET].grid_lp, a->n, ctx->grid_enc2,
+                                          ctx->grid_enc_cap, s));
+        }
+        NRRS_HIP(launch_net_chained(cfg.precision, ea, ctx->grid_enc2, a->n, ctx->net[NRRS_STATNET].mlp_lp,
+                                    ctx->net[NRRS_RRSNET].mlp_lp, ep, s));
     } else if (a->q_mode == NRRS_Q_RRSNET || a->q_mode == NRRS_Q_RRSNET_CHAINED) {
         const float* stats = nullptr;
-        const bool paired = a->q_mode == NRRS_Q_RRSNET_CHAINED && paired_pass(ctx, a->n);
-        if (a->q_mode == NRRS_Q_RRSNET_CHAINED) {
+        if (a->q_mode == NRRS_Q_RRSNET_CHAINED) {  // FP32, or a batch beyond the encoding buffers
             EncodeArgs ea = encode_args(ctx, a->v, nullptr, nullptr, 0);
             EpiArgs es{};
             es.out = ctx->stats_scratch;
             es.out_stride = ctx->cfg.max_queue;
-            if (paired) {
-                // both nets' grid encodings in one pass over the paired table
-                NRRS_HIP(launch_encode_levels_dual(ctx->cfg.precision, ea, ctx->grid_pair, a->n, ctx->grid_enc,
-                                                   ctx->grid_enc2, ctx->grid_enc_cap, s));
-                NRRS_HIP(run_net_pre(ctx, NRRS_STATNET, ea, EPI_RAW, a->n, es, ctx->grid_enc, s));
-            } else {
-                NRRS_HIP(run_net(ctx, NRRS_STATNET, ea, EPI_RAW, a->n, es, s));
-            }
+            NRRS_HIP(run_net(ctx, NRRS_STATNET, ea, EPI_RAW, a->n, es, s));
             stats = ctx->stats_scratch;
         }
         EncodeArgs ea = encode_args(ctx, a->v, a->film_dev, stats, ctx->cfg.max_queue);
-        if (paired)
-            NRRS_HIP(run_net_pre(ctx, NRRS_RRSNET, ea, EPI_RRS, a->n, ep, ctx->grid_enc2, s));
-        else
-            NRRS_HIP(run_net(ctx, NRRS_RRSNET, ea, EPI_RRS, a->n, ep, s));
+        NRRS_HIP(run_net(ctx, NRRS_RRSNET, ea, EPI_RRS, a->n, ep, s));
     } else {
         return NRRS_ERR_ARG;
     }

@@ -199,6 +199,11 @@ hipError_t launch_encode_levels(int precision, const EncodeArgs& a, const uint16
 hipError_t launch_encode_levels_dual(int precision, const EncodeArgs& a, const uint32_t* pair_table, uint32_t n,
                                      uint32_t* out_stat, uint32_t* out_rrs, uint32_t stride, hipStream_t s,
                                      uint32_t* run_counts = nullptr, uint32_t* run_counts2 = nullptr);
+// Chained StatNet (4 hidden, 6 outputs) -> RRSNet (3 hidden, 1 output) inference with the RRS
+// epilogue (q, exact sum) in one launch; a.grid_enc holds StatNet's level-major encoding,
+// enc_rrs RRSNet's (same stride), a.stats must be null. Ws / Wr: packed MFMA tiles.
+hipError_t launch_net_chained(int precision, const EncodeArgs& a, const uint32_t* enc_rrs, uint32_t n,
+                              const uint16_t* Ws, const uint16_t* Wr, const EpiArgs& ep, hipStream_t s);
 hipError_t launch_net_mfma(int precision, const EncodeArgs& a, int net, int n_layers, int n_out, int epi,
                            uint32_t n, const uint16_t* table, const uint16_t* Wp, const EpiArgs& ep,
                            hipStream_t s);

@@ -178,6 +178,94 @@ __global__ __launch_bounds__(kBlockT, 4) void k_net_mfma(EncodeArgs a, int net, 
     if (EPI != EPI_RAW) block_atomic_add_u64(term, ep.sum);
 }
 
+// Chained StatNet -> RRSNet inference in one launch: the StatNet statistics of a vertex stay in
+// registers (exchanged between the two lane halves) and feed the RRSNet input directly, which
+// saves writing and re-reading them and a second pass over the vertex records. The grid
+// channels of both nets come from level-major encodings (a.grid_enc: StatNet, enc_rrs: RRSNet).
+// Both nets' weight tiles sit in LDS (68 KB): 8-wave workgroups, two per CU, four waves per
+// SIMD (at most 128 registers). Epilogue as k_net_mfma's EPI_RRS: q and its exact sum term.
+constexpr int kChainWaves = 8;
+constexpr int kChainT = 64 * kChainWaves;
+
+template <typename P>
+__global__ __launch_bounds__(kChainT, 2) void k_net_chained(EncodeArgs a, const uint32_t* __restrict__ enc_rrs,
+                                                           uint32_t n, const u32x4* __restrict__ Ws,
+                                                           const u32x4* __restrict__ Wr, EpiArgs ep) {
+    extern __shared__ u32x4 sW[];
+    constexpr int kTilesS = tiles_of(5), kTilesR = tiles_of(4);
+    u32x4* sWs = sW;
+    u32x4* sWr = sW + kTilesS * 64;
+    for (int k = threadIdx.x; k < kTilesS * 64; k += kChainT) sWs[k] = Ws[k];
+    for (int k = threadIdx.x; k < kTilesR * 64; k += kChainT) sWr[k] = Wr[k];
+    __syncthreads();
+
+    const int lane = threadIdx.x & 63, h = lane >> 5, col = lane & 31;
+    const int wave = threadIdx.x >> 6;
+    const uint32_t n_cb = (n + 31) / 32;
+    uint64_t term = 0;
+    for (uint32_t cb = blockIdx.x * kChainWaves + wave; cb < n_cb; cb += gridDim.x * kChainWaves) {
+        asm volatile("" ::: "memory");  // keeps the weight-tile loads from LDS inside the loop
+        const uint32_t i = cb * 32 + col;
+        const bool live = i < n;
+        typename P::vec8 B[5];
+        if (live) {
+            encode_half_pre<P>(a, NRRS_STATNET, i, h, B);
+        } else {
+#pragma unroll
+            for (int kb = 0; kb < 5; ++kb) B[kb] = typename P::vec8{};
+        }
+        const f32x16 Ys = mlp_forward<P, 5>(sWs, B, lane);
+        // statistics of the lane's vertex: outputs 0..3 in half 0, 4..5 in half 1
+        float st[6];
+        {
+            float o[4];
+#pragma unroll
+            for (int r = 0; r < 4; ++r) o[r] = __shfl_xor(Ys[r], 32, 64);
+            if (h == 0) {
+                st[0] = Ys[0]; st[1] = Ys[1]; st[2] = Ys[2]; st[3] = Ys[3]; st[4] = o[0]; st[5] = o[1];
+            } else {
+                st[0] = o[0]; st[1] = o[1]; st[2] = o[2]; st[3] = o[3]; st[4] = Ys[0]; st[5] = Ys[1];
+            }
+        }
+        if (live) {
+            const VertexRec v = vertex_from_soa(a, i);
+            const float ref = film_mean(a.film, a.n_pixels, v.pixel);
+            grid_half_from_levels<P>(enc_rrs, a.grid_enc_stride, a.grid.n_levels, i, h, B);
+            encode_tail_half<P>(v, NRRS_RRSNET, h, ref, st, B);
+        } else {
+#pragma unroll
+            for (int kb = 0; kb < 5; ++kb) B[kb] = typename P::vec8{};
+        }
+        const f32x16 Yr = mlp_forward<P, 4>(sWr, B, lane);
+        if (live && h == 0) {
+            const float q = rrs_window_clamp(ep.d, activation_sigma(Yr[0]));
+            ep.q[i] = q;
+            term += rrs_sum_term(ep.d, q);
+        }
+    }
+    block_atomic_add_u64(term, ep.sum);
+}
+
+template <typename P>
+hipError_t launch_chained(const EncodeArgs& a, const uint32_t* enc_rrs, uint32_t n, const uint16_t* Ws,
+                          const uint16_t* Wr, const EpiArgs& ep, hipStream_t s) {
+    constexpr size_t smem = (size_t)(tiles_of(5) + tiles_of(4)) * kTileHalves * 2;
+    static bool attr = false;
+    if (!attr) {
+        hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(&k_net_chained<P>),
+                                           hipFuncAttributeMaxDynamicSharedMemorySize, (int)smem);
+        if (e != hipSuccess) return e;
+        attr = true;
+    }
+    const uint32_t n_cb = (n + 31) / 32;
+    uint32_t blocks = (n_cb + kChainWaves - 1) / kChainWaves;
+    blocks = blocks > 2048 ? 2048 : blocks;
+    if (!blocks) return hipSuccess;
+    hipLaunchKernelGGL(k_net_chained<P>, dim3(blocks), dim3(kChainT), smem, s, a, enc_rrs, n,
+                       reinterpret_cast<const u32x4*>(Ws), reinterpret_cast<const u32x4*>(Wr), ep);
+    return hipGetLastError();
+}
+
 template <typename P, int NL, int NOUT, int EPI>
 hipError_t launch_one(const EncodeArgs& a, int net, uint32_t n, const uint16_t* table, const uint16_t* Wp,
                       const EpiArgs& ep, hipStream_t s) {
@@ -258,6 +346,14 @@ hipError_t launch_pack_lp(int precision, const float* grid, uint64_t n_grid, con
         return hipErrorInvalidValue;
     }
     return hipGetLastError();
+}
+
+hipError_t launch_net_chained(int precision, const EncodeArgs& a, const uint32_t* enc_rrs, uint32_t n,
+                              const uint16_t* Ws, const uint16_t* Wr, const EpiArgs& ep, hipStream_t s) {
+    if (!a.grid_enc || !enc_rrs || a.stats) return hipErrorInvalidValue;
+    if (precision == NRRS_BF16) return launch_chained<BF16>(a, enc_rrs, n, Ws, Wr, ep, s);
+    if (precision == NRRS_FP16) return launch_chained<F16>(a, enc_rrs, n, Ws, Wr, ep, s);
+    return hipErrorInvalidValue;
 }
 
 hipError_t launch_net_mfma(int precision, const EncodeArgs& a, int net, int n_layers, int n_out, int epi,
