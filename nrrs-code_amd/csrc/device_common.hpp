@@ -27,22 +27,46 @@ __device__ __forceinline__ uint32_t pcg_output(uint64_t old) {
     return (xs >> rot) | (xs << ((32u - rot) & 31u));
 }
 
+// Jump-ahead table. The LCG step x -> a x + c with c = 2 stream + 1 (mod 2^64), taken k times,
+// is x -> A(k) x + c G(k) with A(k) = a^k and G(k) = 1 + a + ... + a^(k-1), both independent of
+// the stream. Entry [n][v] holds {A, G} for k = v 16^n, so a jump of `delta` steps composes one
+// entry per hexadecimal digit of delta (Brown 1994's jump-ahead, taken a digit rather than a
+// bit at a time; the maps are powers of one map, so the order of composition is free).
+struct PcgAffine {
+    uint64_t A, G;
+};
+struct PcgJumpTable {
+    PcgAffine e[16][16];
+};
+constexpr PcgJumpTable make_pcg_jump_table() {
+    PcgJumpTable t{};
+    uint64_t bA = kPcgMult, bG = 1;  // one step
+    for (int n = 0; n < 16; ++n) {
+        uint64_t A = 1, G = 0;  // zero steps
+        for (int v = 0; v < 16; ++v) {
+            t.e[n][v] = PcgAffine{A, G};
+            G = bA * G + bG;  // one more base step: 16^n steps
+            A = bA * A;
+        }
+        bA = A;  // 16^(n+1) steps
+        bG = G;
+    }
+    return t;
+}
+__constant__ const PcgJumpTable kPcgJump = make_pcg_jump_table();
+
 // State after seeding (pcg32_srandom_r), jumped `delta` steps ahead.
 __device__ __forceinline__ uint64_t pcg_state_at(uint64_t seed, uint64_t stream, uint64_t delta) {
     const uint64_t inc = (stream << 1u) | 1u;
     uint64_t state = inc;                     // state = 0; next()
     state = (state + seed) * kPcgMult + inc;  // state += seed; next()
-    uint64_t acc_mult = 1, acc_plus = 0, cur_mult = kPcgMult, cur_plus = inc;
-    while (delta) {
-        if (delta & 1u) {
-            acc_mult *= cur_mult;
-            acc_plus = acc_plus * cur_mult + cur_plus;
-        }
-        cur_plus = (cur_mult + 1) * cur_plus;
-        cur_mult *= cur_mult;
-        delta >>= 1u;
+    uint64_t A = 1, G = 0;
+    for (int n = 0; delta; ++n, delta >>= 4u) {
+        const PcgAffine e = kPcgJump.e[n][delta & 15u];
+        G = e.A * G + e.G;
+        A = e.A * A;
     }
-    return acc_mult * state + acc_plus;
+    return A * state + inc * G;
 }
 
 __device__ __forceinline__ uint32_t pcg_next(uint64_t& state, uint64_t stream) {
