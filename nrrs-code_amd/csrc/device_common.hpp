@@ -105,34 +105,77 @@ __device__ __forceinline__ void synth_position(uint64_t seed, uint64_t i, float*
     for (int k = 0; k < 3; ++k) pos[k] = b0 * T[k] + b1 * T[3 + k] + b2 * T[6 + k];
 }
 
+// Unit vector of stream 2 (direction) or 3 (normal), outputs 2i, 2i+1.
+__device__ __forceinline__ void synth_sphere(uint64_t seed, uint64_t stream, uint64_t i, float* dst) {
+    const float two_pi = 6.283185307179586f;
+    uint64_t st = pcg_state_at(seed, stream, 2ull * i);
+    float u1 = u24(pcg_next(st, stream)), u2 = u24(pcg_next(st, stream));
+    float z = 1.0f - 2.0f * u1;
+    float rr = sqrtf(fmaxf(0.0f, 1.0f - z * z));
+    float ph = two_pi * u2;
+    dst[0] = rr * cosf(ph);
+    dst[1] = rr * sinf(ph);
+    dst[2] = z;
+}
+
+// Albedo of stream 5 (diffuse) or 6 (specular), outputs 3i+k.
+__device__ __forceinline__ void synth_albedo(uint64_t seed, uint64_t stream, uint64_t i, float* dst) {
+    uint64_t st = pcg_state_at(seed, stream, 3ull * i);
+#pragma unroll
+    for (int k = 0; k < 3; ++k) dst[k] = u24(pcg_next(st, stream));
+}
+
+// Throughput channel k (stream 7, outputs 6i+2k and 6i+2k+1).
+__device__ __forceinline__ float synth_thp(uint64_t seed, uint64_t i, uint32_t k) {
+    const float two_pi = 6.283185307179586f;
+    uint64_t st = pcg_state_at(seed, 7, 6ull * i + 2ull * k);
+    float ua = u24(pcg_next(st, 7)), ub = u24(pcg_next(st, 7));
+    float nrm = sqrtf(-2.0f * logf(1.0f - ua)) * cosf(two_pi * ub);
+    return expf(-1.0f + nrm);
+}
+
+__device__ __forceinline__ float synth_rough(uint64_t seed, uint64_t i) {
+    return 0.05f + 0.95f * u24(pcg_output(pcg_state_at(seed, 4, i)));
+}
+
 __device__ __forceinline__ SynthVertex synth_vertex(uint64_t seed, uint64_t i) {
     SynthVertex v;
-    const float two_pi = 6.283185307179586f;
     synth_position(seed, i, v.pos);
+    synth_sphere(seed, 2, i, v.dir);
+    synth_sphere(seed, 3, i, v.nrm);
+    v.rough = synth_rough(seed, i);
+    synth_albedo(seed, 5, i, v.ad);
+    synth_albedo(seed, 6, i, v.as);
 #pragma unroll
-    for (int f = 0; f < 2; ++f) {
-        float* dst = f ? v.nrm : v.dir;
-        uint64_t st = pcg_state_at(seed, 2 + f, 2ull * i);
-        float u1 = u24(pcg_next(st, 2 + f)), u2 = u24(pcg_next(st, 2 + f));
-        float z = 1.0f - 2.0f * u1;
-        float rr = sqrtf(fmaxf(0.0f, 1.0f - z * z));
-        float ph = two_pi * u2;
-        dst[0] = rr * cosf(ph);
-        dst[1] = rr * sinf(ph);
-        dst[2] = z;
-    }
-    v.rough = 0.05f + 0.95f * u24(pcg_output(pcg_state_at(seed, 4, i)));
-    uint64_t sa = pcg_state_at(seed, 5, 3ull * i), ss = pcg_state_at(seed, 6, 3ull * i),
-             st = pcg_state_at(seed, 7, 6ull * i);
+    for (int k = 0; k < 3; ++k) v.thp[k] = synth_thp(seed, i, k);
+    return v;
+}
+
+// The non-position fields of synthetic vertex i drawn by the two halves of a wavefront that
+// hold the same vertex in lanes l and l ^ 32 (h = lane half): half h draws stream 2 + h (direction
+// / normal), 5 + h (diffuse / specular albedo) and throughput channels h and h + 2 (channel 3
+// is drawn and dropped), so that both halves run the same code on their own streams; the
+// other half's share comes over by lane exchange. Bitwise the fields of synth_vertex. Every
+// lane of the wavefront must take part.
+__device__ __forceinline__ void synth_fields_pair(uint64_t seed, uint64_t i, int h, float* dir, float* nrm,
+                                                  float* ad, float* as, float* thp, float* rough) {
+    float sph[3], alb[3];
+    synth_sphere(seed, 2 + h, i, sph);
+    synth_albedo(seed, 5 + h, i, alb);
+    const float t0 = synth_thp(seed, i, (uint32_t)h), t1 = synth_thp(seed, i, (uint32_t)h + 2u);
 #pragma unroll
     for (int k = 0; k < 3; ++k) {
-        v.ad[k] = u24(pcg_next(sa, 5));
-        v.as[k] = u24(pcg_next(ss, 6));
-        float ua = u24(pcg_next(st, 7)), ub = u24(pcg_next(st, 7));
-        float nrm = sqrtf(-2.0f * logf(1.0f - ua)) * cosf(two_pi * ub);
-        v.thp[k] = expf(-1.0f + nrm);
+        const float os = __shfl_xor(sph[k], 32, 64), oa = __shfl_xor(alb[k], 32, 64);
+        dir[k] = h ? os : sph[k];
+        nrm[k] = h ? sph[k] : os;
+        ad[k] = h ? oa : alb[k];
+        as[k] = h ? alb[k] : oa;
     }
-    return v;
+    const float o0 = __shfl_xor(t0, 32, 64), o1 = __shfl_xor(t1, 32, 64);
+    thp[0] = h ? o0 : t0;
+    thp[1] = h ? t0 : o0;
+    thp[2] = h ? o1 : t1;
+    *rough = synth_rough(seed, i);
 }
 
 // ---------------------------------------------------------------- scalar maps

@@ -205,18 +205,12 @@ __global__ __launch_bounds__(64 * WAVES) void k_frame(FrameParams f) {
                     typename P::vec8 B[5], Br[2];
                     VertexRec v;
                     float ref = 0.0f;
+                    // the vertex's fields, drawn by both lane halves (all lanes take part; the
+                    // grid channels come from the encodings, so the position is not needed here)
+                    synth_fields_pair(seed_b, live ? i : 0u, h, v.dir, v.nrm, v.ad, v.as, v.thp, &v.rough);
                     if (live) {
-                        const SynthVertex sv = synth_vertex(seed_b, i);
 #pragma unroll
-                        for (int k = 0; k < 3; ++k) {
-                            v.x[k] = norm_coord(sv.pos[k], f.frame.c[k], f.frame.d[k]);
-                            v.dir[k] = sv.dir[k];
-                            v.nrm[k] = sv.nrm[k];
-                            v.ad[k] = sv.ad[k];
-                            v.as[k] = sv.as[k];
-                            v.thp[k] = sv.thp[k];
-                        }
-                        v.rough = sv.rough;
+                        for (int k = 0; k < 3; ++k) v.x[k] = 0.0f;
                         v.pixel = qin[i];
                         ref = film_mean(f.film, f.n_pixels, v.pixel);
                         const uint32_t L = f.grid.n_levels;
