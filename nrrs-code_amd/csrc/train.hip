@@ -883,9 +883,12 @@ __device__ __forceinline__ void apply_half(unsigned long long* acc, uint32_t slo
     if (mag) atomicAdd(&acc[slot], (h & 0x8000u) ? 0ull - mag : mag);
 }
 
+// LDS sums as two planes, feature f of entry e at f * kSegEntries + e: consecutive entries are
+// 8 bytes apart, so the entries of a wave's records spread over all LDS banks (interleaved
+// features put every entry at a multiple of 16 bytes, on half of them).
 __device__ __forceinline__ void apply_pair(unsigned long long* acc, uint32_t e, uint32_t bits, uint32_t& bad) {
-    apply_half(acc, 2 * e, bits & 0xffffu, bad);
-    apply_half(acc, 2 * e + 1, bits >> 16, bad);
+    apply_half(acc, e, bits & 0xffffu, bad);
+    apply_half(acc, kSegEntries + e, bits >> 16, bad);
 }
 
 // One record (key, value) into the LDS sums of its segment.
@@ -935,7 +938,7 @@ __device__ __forceinline__ h2v add_fixed(h2v o, long long s0, long long s1) {
 }
 
 __global__ __launch_bounds__(kApplyThreads) void k_apply_bins(ApplyArgs a) {
-    extern __shared__ unsigned long long acc_s[];  // [kSegEntries][2], units of 2^-24
+    extern __shared__ unsigned long long acc_s[];  // [2][kSegEntries], units of 2^-24
     __shared__ uint32_t bad_s;
     const uint32_t seg = a.direct[blockIdx.x];
     const SegInfo sg = a.segs[seg];
@@ -948,7 +951,7 @@ __global__ __launch_bounds__(kApplyThreads) void k_apply_bins(ApplyArgs a) {
     __syncthreads();
     h2v* g = a.grad + sg.entry0;
     for (uint32_t e = threadIdx.x; e < sg.n_entries; e += kApplyThreads) {
-        const long long s0 = (long long)acc_s[2 * e], s1 = (long long)acc_s[2 * e + 1];
+        const long long s0 = (long long)acc_s[e], s1 = (long long)acc_s[kSegEntries + e];
         if (s0 != 0 || s1 != 0) g[e] = add_fixed(g[e], s0, s1);
     }
     if (threadIdx.x == 0) {
@@ -963,15 +966,16 @@ __global__ __launch_bounds__(kApplyThreads) void k_apply_chunks(ApplyArgs a) {
     const SegInfo sg = a.segs[ch.x];
     const uint32_t n = min(a.seg_count[ch.x], sg.cap);
     if (ch.y >= n) return;  // whole workgroup
-    for (uint32_t k = threadIdx.x; k < 2 * sg.n_entries; k += kApplyThreads) acc_s[k] = 0ull;
+    for (uint32_t k = threadIdx.x; k < 2 * sg.n_entries; k += kApplyThreads)
+        acc_s[(k & 1u) * kSegEntries + (k >> 1)] = 0ull;
     __syncthreads();
     uint32_t bad = 0u;
     apply_range(acc_s, a.key + sg.rec_off, a.val + sg.rec_off, ch.y, min(ch.y + a.chunk, n), bad);
     if (bad) atomicOr(a.bad, 1u);
     __syncthreads();
     unsigned long long* acc = a.acc + 2 * (size_t)sg.entry0;
-    for (uint32_t k = threadIdx.x; k < 2 * sg.n_entries; k += kApplyThreads) {
-        const unsigned long long v = acc_s[k];
+    for (uint32_t k = threadIdx.x; k < 2 * sg.n_entries; k += kApplyThreads) {  // global: [entry][2]
+        const unsigned long long v = acc_s[(k & 1u) * kSegEntries + (k >> 1)];
         if (v) atomicAdd(acc + k, v);
     }
 }
