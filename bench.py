@@ -35,7 +35,7 @@ BF16_PEAK_TFLOPS = 2500.0
 # k_bin_scan + k_bin_levels + k_apply_bins + k_apply_chunks + k_apply_finish +
 # k_reduce_partials), from rocprofv3 --pmc FETCH_SIZE and WRITE_SIZE
 # (profiles/train_backward_bf16_2p20_pmc.md), or None when not measured.
-TRAIN_TRAFFIC_BYTES = 2886379808
+TRAIN_TRAFFIC_BYTES = 2867101664
 MACS = {"statnet": 80 * 64 + 3 * 64 * 64 + 64 * 6, "rrsnet": 80 * 64 + 2 * 64 * 64 + 64 * 1}
 
 
@@ -50,8 +50,9 @@ def parse():
     ap.add_argument("--no-frame", action="store_true", help="skip the multi-bounce wavefront and online loop")
     ap.add_argument("--no-peaks", action="store_true", help="skip the copy and GEMM measurements of the GPU")
     ap.add_argument("--pmc-probe", action="store_true",
-                    help="only run a 256 MiB device copy and 3 BF16 StatNet training kernels at 2^20 "
-                         "(for rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE runs)")
+                    help="only run a 256 MiB device copy and, after 8 warm-up launches, 3 BF16 StatNet "
+                         "training backwards at 2^20 (for rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE runs; "
+                         "the profile summaries use the last 3 dispatches of each kernel)")
     return ap.parse_args()
 
 
@@ -393,7 +394,7 @@ def main():
         dst = torch.empty_like(src)
         dst.copy_(src)
         path = Path(na.BF16, torch, na, np)
-        for _ in range(3):
+        for _ in range(8 + 3):  # warm-up (clocks, caches, first-use allocations), then 3 measured
             path.backward_statnet()
         torch.cuda.synchronize()
         print(json.dumps({"pmc_probe": "done", "copy_bytes": src.numel() * 4}))
