@@ -287,6 +287,7 @@ nrrs_status ensure_train(nrrs_ctx* c, int net) {
             NRRS_TRY(dmalloc(&ns.bin_tiles, (size_t)c->grid.n_levels * kMaxSegBins * ns.bin_max_tiles));
             NRRS_TRY(dmalloc(&ns.seg_count, segs.size()));
             NRRS_HIP(hipMemset(ns.seg_count, 0, segs.size() * 4));
+            ns.n_segs = (uint32_t)segs.size();
             {
                 std::vector<uint32_t> end((size_t)c->grid.n_levels * kMaxSegBins, 0u);
                 for (uint32_t l = 0; l < c->grid.n_levels; ++l)
@@ -335,12 +336,60 @@ nrrs_status ensure_train(nrrs_ctx* c, int net) {
 }
 
 // Forward + loss + backward of one net into its gradient buffer.
+// Binned-scatter arguments of one net (its records, counts and segments).
+BinArgs bin_args(nrrs_ctx* c, int net, uint32_t n, const EncodeArgs& a) {
+    NetState& ns = c->net[net];
+    BinArgs bn{};
+    bn.a = a;
+    bn.gdl = ns.gdl;
+    bn.gdl_stride = c->cfg.max_train;
+    bn.n = n;
+    bn.rec_key = ns.rec_key;
+    bn.rec_val = ns.rec_val;
+    bn.seg_count = ns.seg_count;
+    bn.runs = ns.bin_runs;
+    bn.n_runs = (n + kRunVert - 1) / kRunVert;
+    bn.seg_end = ns.seg_end;
+    bn.tile_sums = ns.bin_tiles;
+    bn.max_tiles = ns.bin_max_tiles;
+    bn.segs = static_cast<const SegInfo*>(ns.segs);
+    for (int l = 0; l < kMaxLevels; ++l) {
+        bn.seg_base[l] = ns.seg_base[l];
+        bn.seg_nbins[l] = ns.seg_nbins[l];
+    }
+    bn.grad = reinterpret_cast<_Float16 __attribute__((ext_vector_type(2)))*>(ns.grad_lp);
+    return bn;
+}
+
+// Sums of one net's binned records (keys `key`, its own values) into its table gradient.
+hipError_t launch_apply(nrrs_ctx* c, int net, const uint16_t* key, hipStream_t s) {
+    NetState& ns = c->net[net];
+    ApplyArgs ap{};
+    ap.key = key;
+    ap.val = ns.rec_val;
+    ap.seg_count = ns.seg_count;
+    ap.segs = static_cast<const SegInfo*>(ns.segs);
+    ap.direct = ns.direct;
+    ap.chunks = ns.chunks;
+    ap.chunked = ns.chunked;
+    ap.n_chunked = ns.n_chunked;
+    ap.acc = ns.acc;
+    ap.n_acc = ns.n_chunks ? ns.n_dense : 0;
+    ap.chunk = ns.apply_chunk;
+    ap.bad = ns.bad;
+    ap.grad = reinterpret_cast<_Float16 __attribute__((ext_vector_type(2)))*>(ns.grad_lp);
+    return launch_apply_bins(ap, ns.n_direct, ns.n_chunks, s);
+}
+
 // `pre_enc`: the level-major encoding of this batch and net (and, for a binned net, its record
 // counts) already written by a paired encode pass; null = run the net's own encode pass.
+// `blocks_out`: stop after the training kernel and report its workgroups (the caller then
+// bins, applies and reduces).
 nrrs_status run_backward(nrrs_ctx* c, int net, uint32_t n, const nrrs_vertices& v, const float* film,
                          const float* stats, uint32_t stats_stride, int kind, const float* dy,
                          const float* target_L, const nrrs_rrs_side* side, const float* scale, float* y_out,
-                         uint32_t out_stride, float* loss_out, hipStream_t s, const uint32_t* pre_enc = nullptr) {
+                         uint32_t out_stride, float* loss_out, hipStream_t s, const uint32_t* pre_enc = nullptr,
+                         uint32_t* blocks_out = nullptr) {
     NetState& ns = c->net[net];
     const nrrs_config& cfg = c->cfg;
     if (n > cfg.max_train) return NRRS_ERR_SHAPE;  // per-sample training buffers hold max_train
@@ -403,42 +452,13 @@ nrrs_status run_backward(nrrs_ctx* c, int net, uint32_t n, const nrrs_vertices& 
     uint32_t blocks = iters < c->max_train_blocks ? iters : c->max_train_blocks;
     if (!blocks) return NRRS_OK;
     NRRS_HIP(launch_train(cfg.precision, t, ns.n_layers, ns.n_out, kind, blocks, s));
+    if (blocks_out) {  // the caller bins, applies and reduces (joint step)
+        *blocks_out = blocks;
+        return NRRS_OK;
+    }
     if (ns.binned) {
-        BinArgs bn{};
-        bn.a = t.a;
-        bn.gdl = ns.gdl;
-        bn.gdl_stride = cfg.max_train;
-        bn.n = n;
-        bn.rec_key = ns.rec_key;
-        bn.rec_val = ns.rec_val;
-        bn.seg_count = ns.seg_count;
-        bn.runs = ns.bin_runs;
-        bn.n_runs = (n + kRunVert - 1) / kRunVert;
-        bn.seg_end = ns.seg_end;
-        bn.tile_sums = ns.bin_tiles;
-        bn.max_tiles = ns.bin_max_tiles;
-        bn.segs = static_cast<const SegInfo*>(ns.segs);
-        for (int l = 0; l < kMaxLevels; ++l) {
-            bn.seg_base[l] = ns.seg_base[l];
-            bn.seg_nbins[l] = ns.seg_nbins[l];
-        }
-        bn.grad = reinterpret_cast<_Float16 __attribute__((ext_vector_type(2)))*>(ns.grad_lp);
-        NRRS_HIP(launch_bin_levels(bn, s));
-        ApplyArgs ap{};
-        ap.key = ns.rec_key;
-        ap.val = ns.rec_val;
-        ap.seg_count = ns.seg_count;
-        ap.segs = static_cast<const SegInfo*>(ns.segs);
-        ap.direct = ns.direct;
-        ap.chunks = ns.chunks;
-        ap.chunked = ns.chunked;
-        ap.n_chunked = ns.n_chunked;
-        ap.acc = ns.acc;
-        ap.n_acc = ns.n_chunks ? ns.n_dense : 0;
-        ap.chunk = ns.apply_chunk;
-        ap.bad = ns.bad;
-        ap.grad = reinterpret_cast<_Float16 __attribute__((ext_vector_type(2)))*>(ns.grad_lp);
-        NRRS_HIP(launch_apply_bins(ap, ns.n_direct, ns.n_chunks, s));
+        NRRS_HIP(launch_bin_levels(bin_args(c, net, n, t.a), s));
+        NRRS_HIP(launch_apply(c, net, ns.rec_key, s));
     }
     return hip_status(launch_reduce_partials(ns.dw_partial, blocks, (uint32_t)ns.n_mlp, ns.grad + ns.n_grid, s));
 }
@@ -1047,17 +1067,69 @@ nrrs_status nrrs_train_step(nrrs_ctx* ctx, uint32_t net_mask, uint32_t n, const 
     // joint step: both nets' encodings (and the record counts, the same for both) in one pass
     // over the paired table, before either net's parameters change
     const bool paired = (net_mask & 3u) == 3u && ctx->cfg.precision != NRRS_FP32 && paired_pass(ctx, n);
+    NetState& s0 = ctx->net[NRRS_STATNET];
+    NetState& s1 = ctx->net[NRRS_RRSNET];
+    // both nets binned with one segment layout: one record pass writes the records of both
+    const bool joint_records = paired && s0.binned && s1.binned && s0.n_segs == s1.n_segs;
     if (paired) {
-        NetState& s0 = ctx->net[NRRS_STATNET];
-        NetState& s1 = ctx->net[NRRS_RRSNET];
         EncodeArgs ea = encode_args(ctx, *v, nullptr, nullptr, 0);
         uint32_t* rc = s0.binned ? s0.bin_runs : (s1.binned ? s1.bin_runs : nullptr);
-        uint32_t* rc2 = s0.binned && s1.binned ? s1.bin_runs : nullptr;
+        uint32_t* rc2 = s0.binned && s1.binned && !joint_records ? s1.bin_runs : nullptr;
         NRRS_HIP(launch_encode_levels_dual(ctx->cfg.precision, ea, ctx->grid_pair, n, ctx->grid_enc, ctx->grid_enc2,
                                            ctx->grid_enc_cap, s, rc, rc2));
     }
+    if (joint_records) {
+        // both training kernels, then the scan and one record pass for both nets, then each
+        // net's apply, dW reduction and update (StatNet's on the helper stream)
+        uint32_t b0 = 0, b1 = 0;
+        NRRS_TRY(run_backward(ctx, NRRS_STATNET, n, *v, film_dev, nullptr, 0, NRRS_LOSS_LL2, nullptr, target_L_dev,
+                              nullptr, s0.loss_scale, ctx->train_stats, ms, loss_dev, s, ctx->grid_enc, &b0));
+        NRRS_TRY(run_backward(ctx, NRRS_RRSNET, n, *v, film_dev, ctx->train_stats, ms, NRRS_LOSS_RRS, nullptr,
+                              nullptr, side, s1.loss_scale, nullptr, 0, loss_dev ? loss_dev + 1 : nullptr, s,
+                              ctx->grid_enc2, &b1));
+        if (b0) {
+            BinArgs bn = bin_args(ctx, NRRS_STATNET, n, encode_args(ctx, *v, nullptr, nullptr, 0));
+            NRRS_HIP(launch_bin_scan(bn, s));
+            // RRSNet's records take StatNet's slots: its apply reads the same per-segment counts
+            NRRS_HIP(hipMemcpyAsync(s1.seg_count, s0.seg_count, (size_t)s0.n_segs * 4, hipMemcpyDeviceToDevice, s));
+            bn.gdl2 = s1.gdl;
+            bn.rec_val2 = s1.rec_val;
+            bn.grad2 = reinterpret_cast<_Float16 __attribute__((ext_vector_type(2)))*>(s1.grad_lp);
+            NRRS_HIP(launch_bin_records(bn, s));
+            NRRS_HIP(launch_apply(ctx, NRRS_STATNET, s0.rec_key, s));
+            NRRS_HIP(launch_reduce_partials(s0.dw_partial, b0, (uint32_t)s0.n_mlp, s0.grad + s0.n_grid, s));
+        }
+        bool forked = false;
+        if (ctx->aux_stream) {
+            NRRS_HIP(hipEventRecord(ctx->ev_fork, s));
+            NRRS_HIP(hipStreamWaitEvent(ctx->aux_stream, ctx->ev_fork, 0));
+            const nrrs_status st = run_adam(ctx, NRRS_STATNET, ctx->aux_stream);
+            NRRS_HIP(hipEventRecord(ctx->ev_join, ctx->aux_stream));
+            forked = true;
+            if (st != NRRS_OK) {
+                NRRS_HIP(hipStreamWaitEvent(s, ctx->ev_join, 0));
+                return st;
+            }
+        } else {
+            NRRS_TRY(run_adam(ctx, NRRS_STATNET, s));
+        }
+        nrrs_status st = NRRS_OK;
+        if (b1) {
+            hipError_t e = launch_apply(ctx, NRRS_RRSNET, s0.rec_key, s);
+            if (e == hipSuccess)
+                e = launch_reduce_partials(s1.dw_partial, b1, (uint32_t)s1.n_mlp, s1.grad + s1.n_grid, s);
+            st = hip_status(e);
+        }
+        if (st == NRRS_OK) st = run_adam(ctx, NRRS_RRSNET, s);
+        if (forked) {
+            const nrrs_status sj = hip_status(hipStreamWaitEvent(s, ctx->ev_join, 0));
+            if (st == NRRS_OK) st = sj;
+        }
+        if (st != NRRS_OK) return st;
+        return refresh_pair(ctx, s);
+    }
     if (net_mask & 1u) {
-        NetState& ns = ctx->net[NRRS_STATNET];
+        NetState& ns = s0;
         // StatNet step (myNetworkLL2.cpp:41-89); its forward outputs feed RRSNet
         NRRS_TRY(run_backward(ctx, NRRS_STATNET, n, *v, film_dev, nullptr, 0, NRRS_LOSS_LL2, nullptr, target_L_dev,
                               nullptr, ns.loss_scale, ctx->train_stats, ms, loss_dev, s,

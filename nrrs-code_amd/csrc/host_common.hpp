@@ -108,6 +108,7 @@ struct NetState {
     uint32_t bin_max_tiles = 0;
     uint32_t* seg_end = nullptr;     // [level][kMaxSegBins] end slot of every segment (0: none)
     uint32_t* seg_count = nullptr;   // records per segment
+    uint32_t n_segs = 0;
     void* segs = nullptr;            // SegInfo per segment
     uint32_t seg_base[kMaxLevels] = {}, seg_nbins[kMaxLevels] = {};
     uint32_t* direct = nullptr;      // segments applied by one workgroup each

@@ -763,20 +763,45 @@ __global__ __launch_bounds__(64 * kScanWaves) void k_bin_scan(BinArgs b, uint32_
 //  2. wave 0, lane s = segment s: an exclusive scan of the counts places the segment's records
 //     in the workgroup's segment-ordered list; the workgroup's slots of segment s start at the
 //     first slot of its first run;
-//  3. each record {entry in level, code, FP16 w * dL/denc of its corners} is put into the LDS
-//     list at (segment start + rank);
+//  3. each record {entry in level, code, vertex and corner} is put into the LDS list at
+//     (segment start + rank);
 //  4. the list is copied out with consecutive threads on consecutive records, so the stores of
-//     a wave form a few contiguous runs rather than one cache line per lane.
+//     a wave form a few contiguous runs rather than one cache line per lane; each record's value
+//     (FP16 w * dL/denc of its corners) is formed there from its vertex's cell fractions and
+//     dL/denc, staged in LDS (a 2-byte index per record instead of the 8-byte value keeps the
+//     list, and so the workgroup's LDS, small enough for more resident workgroups).
 // Records beyond a full segment's slots are added directly to the table gradient with packed
 // FP16 atomics.
 constexpr int kBinVert = 2 * kRunVert;
 constexpr int kBinMaxRec = 8 * kBinVert;  // every pair split
 
+// Value of a record: the FP16 pairs w * dL/denc of the corners (x = 0, 1) of (y, z) = yz, both in
+// one u64 for a paired record, corner xb alone for code 3.
+__device__ __forceinline__ unsigned long long record_value(const LevelCell& lc, int yz, float2 g, uint32_t code,
+                                                           uint32_t xb) {
+    uint32_t v[2];
+#pragma unroll
+    for (int x = 0; x < 2; ++x) {
+        const float w = corner_weight(lc, (yz << 1) | x);
+        const h2v gv = {(_Float16)(w * g.x), (_Float16)(w * g.y)};
+        v[x] = __builtin_bit_cast(uint32_t, gv);
+    }
+    if (code != 3u) return (unsigned long long)v[0] | ((unsigned long long)v[1] << 32);
+    return xb ? v[1] : v[0];
+}
+
+// DUAL: the records of both nets of a joint step in one pass. Their corners, and so their keys and
+// slots, are the same (one grid layout, one set of record counts); the values come from each
+// net's dL/denc (gdl, gdl2) and go to rec_val and rec_val2, the keys once to rec_key.
+template <bool DUAL>
 __global__ __launch_bounds__(kBinVert) void k_bin_levels(BinArgs b, uint32_t n_chunks) {
     static_assert(kMaxSegBins == 64, "one lane per segment");
     __shared__ uint32_t cnt[kMaxSegBins], start[kMaxSegBins], dst[kMaxSegBins], lim[kMaxSegBins], tot;
-    __shared__ uint32_t list_key[kBinMaxRec];            // entry in level | code << 24
-    __shared__ unsigned long long list_val[kBinMaxRec];
+    __shared__ uint32_t list_key[kBinMaxRec];  // entry in level | code << 24
+    __shared__ uint16_t list_src[kBinMaxRec];  // vertex in the workgroup << 3 | yz << 1 | x bit
+    __shared__ float vf[3][kBinVert];          // cell fractions and dL/denc of the workgroup's vertices:
+    __shared__ float2 vg[kBinVert];            // the values are formed at copy-out
+    __shared__ float2 vg2[DUAL ? kBinVert : 1];
     const uint32_t l = blockIdx.x / n_chunks, chunk = blockIdx.x % n_chunks, i = chunk * kBinVert + threadIdx.x;
     if (l >= b.a.grid.n_levels) return;  // whole workgroup
     uint32_t first = 0u, end = 0u;       // wave 0: this workgroup's first slot and the segment end
@@ -790,13 +815,14 @@ __global__ __launch_bounds__(kBinVert) void k_bin_levels(BinArgs b, uint32_t n_c
     // dense (coarse) levels have few segments, often one per wave: aggregated LDS atomics
     const bool dense = (b.a.grid.dense_mask >> l) & 1u;
     uint32_t e[8], rk[8], code[4];  // e[2yz], e[2yz+1]: corners x = 0, 1 of (y, z) = yz
-    LevelCell lc{};
-    float2 g = make_float2(0.0f, 0.0f);
     if (live) {
         float x[3];
         load_xn(b.a, i, x);
-        lc = level_cell(b.a.grid, (int)l, x);
-        g = b.gdl[(size_t)l * b.gdl_stride + i];
+        const LevelCell lc = level_cell(b.a.grid, (int)l, x);
+#pragma unroll
+        for (int k = 0; k < 3; ++k) vf[k][threadIdx.x] = lc.f[k];
+        vg[threadIdx.x] = b.gdl[(size_t)l * b.gdl_stride + i];
+        if (DUAL) vg2[threadIdx.x] = b.gdl2[(size_t)l * b.gdl_stride + i];
 #pragma unroll
         for (int yz = 0; yz < 4; ++yz) {
             e[2 * yz] = corner_index(b.a.grid, (int)l, lc, yz << 1);  // < level size
@@ -821,24 +847,15 @@ __global__ __launch_bounds__(kBinVert) void k_bin_levels(BinArgs b, uint32_t n_c
     if (live) {
 #pragma unroll
         for (int yz = 0; yz < 4; ++yz) {
-            uint32_t v[2];
-#pragma unroll
-            for (int xb = 0; xb < 2; ++xb) {
-                const float w = corner_weight(lc, (yz << 1) | xb);
-                const h2v gv = {(_Float16)(w * g.x), (_Float16)(w * g.y)};
-                v[xb] = __builtin_bit_cast(uint32_t, gv);
-            }
             const uint32_t e0 = e[2 * yz], e1 = e[2 * yz + 1];
+            const uint32_t src = (threadIdx.x << 3) | (uint32_t)(yz << 1);
             const uint32_t k0 = start[e0 >> kSegShift] + rk[2 * yz];
-            if (code[yz] != 3u) {
-                list_key[k0] = e0 | (code[yz] << 24);
-                list_val[k0] = (unsigned long long)v[0] | ((unsigned long long)v[1] << 32);
-            } else {
+            list_key[k0] = e0 | (code[yz] << 24);
+            list_src[k0] = (uint16_t)src;
+            if (code[yz] == 3u) {
                 const uint32_t k1 = start[e1 >> kSegShift] + rk[2 * yz + 1];
-                list_key[k0] = e0 | (3u << 24);
-                list_val[k0] = v[0];
                 list_key[k1] = e1 | (3u << 24);
-                list_val[k1] = v[1];
+                list_src[k1] = (uint16_t)(src | 1u);
             }
         }
     }
@@ -846,17 +863,30 @@ __global__ __launch_bounds__(kBinVert) void k_bin_levels(BinArgs b, uint32_t n_c
     const uint32_t n_rec = tot;
     for (uint32_t k = threadIdx.x; k < n_rec; k += kBinVert) {
         const uint32_t key = list_key[k], ent = key & 0xFFFFFFu, c = key >> 24, sb = ent >> kSegShift;
-        const unsigned long long v = list_val[k];
+        const uint32_t src = list_src[k], vi = src >> 3;
+        LevelCell lc;
+#pragma unroll
+        for (int q = 0; q < 3; ++q) lc.f[q] = vf[q][vi];
+        const int yz = (int)((src >> 1) & 3u);
+        const unsigned long long v = record_value(lc, yz, vg[vi], c, src & 1u);
+        const unsigned long long v2 = DUAL ? record_value(lc, yz, vg2[vi], c, src & 1u) : 0ull;
         const uint32_t d = dst[sb] + k;
         if (d < lim[sb]) {
             b.rec_key[d] = (uint16_t)((ent & (kSegEntries - 1u)) | (c << kSegShift));
             b.rec_val[d] = v;
+            if (DUAL) b.rec_val2[d] = v2;
         } else {  // segment full: add directly
+            const uint32_t partner = ent ^ ((2u << (c & 3u)) - 1u);
             h2v* gl = b.grad + b.a.grid.offset[l];
             __builtin_amdgcn_global_atomic_fadd_v2f16(gl + ent, __builtin_bit_cast(h2v, (uint32_t)v));
-            if (c != 3u)
-                __builtin_amdgcn_global_atomic_fadd_v2f16(gl + (ent ^ ((2u << c) - 1u)),
-                                                          __builtin_bit_cast(h2v, (uint32_t)(v >> 32)));
+            if (c != 3u) __builtin_amdgcn_global_atomic_fadd_v2f16(gl + partner, __builtin_bit_cast(h2v, (uint32_t)(v >> 32)));
+            if (DUAL) {
+                h2v* gl2 = b.grad2 + b.a.grid.offset[l];
+                __builtin_amdgcn_global_atomic_fadd_v2f16(gl2 + ent, __builtin_bit_cast(h2v, (uint32_t)v2));
+                if (c != 3u)
+                    __builtin_amdgcn_global_atomic_fadd_v2f16(gl2 + partner,
+                                                              __builtin_bit_cast(h2v, (uint32_t)(v2 >> 32)));
+            }
         }
     }
 }
@@ -1102,18 +1132,35 @@ hipError_t launch_apply_bins(const ApplyArgs& a, uint32_t n_direct, uint32_t n_c
     return hipGetLastError();
 }
 
-hipError_t launch_bin_levels(const BinArgs& b, hipStream_t s) {
+hipError_t launch_bin_scan(const BinArgs& b, hipStream_t s) {
     if (!b.n) return hipSuccess;
     if (b.n_runs != (b.n + kRunVert - 1) / kRunVert) return hipErrorInvalidValue;
-    const uint32_t n_chunks = (b.n + kBinVert - 1) / kBinVert;
-    const uint64_t blocks = (uint64_t)n_chunks * b.a.grid.n_levels;
-    if (blocks > 0x7FFFFFFFull) return hipErrorInvalidValue;
     const uint32_t n_tiles = (b.n_runs + kScanRows - 1) / kScanRows;
     if (n_tiles > b.max_tiles) return hipErrorInvalidValue;
     hipLaunchKernelGGL(k_bin_tiles, dim3(b.a.grid.n_levels * n_tiles), dim3(64 * kScanWaves), 0, s, b, n_tiles);
     hipLaunchKernelGGL(k_bin_scan, dim3(b.a.grid.n_levels * n_tiles), dim3(64 * kScanWaves), 0, s, b, n_tiles);
-    hipLaunchKernelGGL(k_bin_levels, dim3((uint32_t)blocks), dim3(kBinVert), 0, s, b, n_chunks);
     return hipGetLastError();
+}
+
+hipError_t launch_bin_records(const BinArgs& b, hipStream_t s) {
+    if (!b.n) return hipSuccess;
+    if (b.n_runs != (b.n + kRunVert - 1) / kRunVert) return hipErrorInvalidValue;
+    const bool dual = b.gdl2 != nullptr;
+    if (dual && (!b.rec_val2 || !b.grad2)) return hipErrorInvalidValue;
+    const uint32_t n_chunks = (b.n + kBinVert - 1) / kBinVert;
+    const uint64_t blocks = (uint64_t)n_chunks * b.a.grid.n_levels;
+    if (blocks > 0x7FFFFFFFull) return hipErrorInvalidValue;
+    if (dual)
+        hipLaunchKernelGGL(k_bin_levels<true>, dim3((uint32_t)blocks), dim3(kBinVert), 0, s, b, n_chunks);
+    else
+        hipLaunchKernelGGL(k_bin_levels<false>, dim3((uint32_t)blocks), dim3(kBinVert), 0, s, b, n_chunks);
+    return hipGetLastError();
+}
+
+hipError_t launch_bin_levels(const BinArgs& b, hipStream_t s) {
+    const hipError_t e = launch_bin_scan(b, s);
+    if (e != hipSuccess) return e;
+    return launch_bin_records(b, s);
 }
 
 hipError_t launch_reduce_dense(float* rep_f32, void* rep_h, uint32_t n_dense, float* grad, void* grad_h,

@@ -116,11 +116,20 @@ struct BinArgs {
     uint32_t seg_base[kMaxLevels];   // first segment of each level
     uint32_t seg_nbins[kMaxLevels];  // segments of each level
     _Float16 __attribute__((ext_vector_type(2)))* grad;  // FP16 pair table gradient (full segments)
+    // second net of a joint step (same grid, same record slots), or null: its dL/denc, its
+    // record values (the keys are shared) and its table gradient
+    const float2* gdl2;
+    unsigned long long* rec_val2;
+    _Float16 __attribute__((ext_vector_type(2)))* grad2;
 };
 // Scans the run counts into first slots and segment totals, then writes the records of every
 // (vertex, level) (FP16 w * dL/denc of its corners, x-neighbours paired) into the slots of
 // their segments, walking the levels one after the other.
 hipError_t launch_bin_levels(const BinArgs& b, hipStream_t s);
+// The two halves of launch_bin_levels: the scan, then the record pass (with gdl2 set, the
+// records of both nets of a joint step in one pass).
+hipError_t launch_bin_scan(const BinArgs& b, hipStream_t s);
+hipError_t launch_bin_records(const BinArgs& b, hipStream_t s);
 // Sums the binned records into the table gradient: n_direct segments with one workgroup
 // each, n_chunks chunks of the other segments, then their fixed-point sums.
 hipError_t launch_apply_bins(const ApplyArgs& a, uint32_t n_direct, uint32_t n_chunks, hipStream_t s);
