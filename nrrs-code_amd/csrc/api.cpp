@@ -1099,7 +1099,7 @@ nrrs_status nrrs_train_step(nrrs_ctx* ctx, uint32_t net_mask, uint32_t n, const 
             NRRS_HIP(launch_apply(ctx, NRRS_STATNET, s0.rec_key, s));
             NRRS_HIP(launch_reduce_partials(s0.dw_partial, b0, (uint32_t)s0.n_mlp, s0.grad + s0.n_grid, s));
         }
-        bool forked = false;
+        // the StatNet update touches only StatNet's buffers: helper stream, beside RRSNet's apply
         if (ctx->aux_stream) {
             NRRS_HIP(hipEventRecord(ctx->ev_fork, s));
             NRRS_HIP(hipStreamWaitEvent(ctx->aux_stream, ctx->ev_fork, 0));
@@ -1136,7 +1136,7 @@ nrrs_status nrrs_train_step(nrrs_ctx* ctx, uint32_t net_mask, uint32_t n, const 
                               paired ? ctx->grid_enc : nullptr));
         if ((net_mask & 2u) && ctx->aux_stream) {
             // The StatNet update touches only StatNet's buffers: it runs on the helper stream
-            // beside the RRSNet backward (HBM-bound Adam next to the L2-bound encode pass).
+            // beside the RRSNet backward.
             NRRS_HIP(hipEventRecord(ctx->ev_fork, s));
             NRRS_HIP(hipStreamWaitEvent(ctx->aux_stream, ctx->ev_fork, 0));
             const nrrs_status st = run_adam(ctx, NRRS_STATNET, ctx->aux_stream);
