@@ -125,3 +125,55 @@ def test_frame_paired_equals_per_net(torch_cuda):
         n = int(p["sizes"][bn])
         assert np.array_equal(p["trace"][bn, :n], s["trace"][bn, :n]), bn
         assert np.array_equal(p["q"][bn, :n], s["q"][bn, :n]), bn
+
+
+def test_joint_step_with_full_segments(torch_cuda, ora):
+    """Every vertex on one z-plane overfills the record slots of dense-level segments, so the
+    joint record pass adds the records that do not fit to both nets' table gradients with
+    packed FP16 atomics, whose order is not fixed. One joint step then matches the per-net path
+    except where a gradient's sign is decided by that order: Adam's first step moves each
+    parameter by lr * sign(gradient)."""
+    torch = torch_cuda
+    n = 4096
+    ctxs = _contexts(na.BF16, n)
+    vb = na.synth_vertices(n, 23, n)
+    vb.pos[2, :n] = 0.37
+    # precondition: some dense-level segment receives more records than its slots (records: one
+    # per x-neighbour corner pair whose entries differ by 1, 3 or 7, else one per corner; slots
+    # of a dense segment: min(8n, 3 x its share of 4.5n + 4096), rounded up to 64)
+    hv = ora.HostVertices(vb.to_numpy())
+    lay = ora.grid_layout(ctxs[0].cfg.grid)
+    idx = ora.grid_indices(ctxs[0].cfg, hv).astype(np.int64)
+    over = False
+    for l in range(ctxs[0].cfg.grid.n_levels):
+        if not lay["dense"][l]:
+            continue
+        size = int(lay["size"][l])
+        e0, e1 = idx[:, l, 0::2].ravel(), idx[:, l, 1::2].ravel()
+        single = ~np.isin(e0 ^ e1, [1, 3, 7])
+        counts = np.bincount(np.concatenate([e0 >> 13, e1[single] >> 13]), minlength=(size + 8191) // 8192)
+        for b, c in enumerate(counts):
+            ne = min(8192, size - b * 8192)
+            cap = min(8 * n, 3 * (9 * n // 2 * ne // size) + 4096)
+            over |= c > (cap + 63) // 64 * 64
+    assert over
+    rng = np.random.default_rng(7)
+    film = torch.from_numpy(rng.lognormal(-1, 1, (3, n)).astype(np.float32)).cuda()
+    tL = torch.from_numpy(ora.synth_targets(n, 13)).cuda()
+    keep = dict(error=torch.from_numpy(rng.uniform(0, 2, n).astype(np.float32)).cuda(),
+                sample_weight=torch.ones(n, device="cuda"),
+                q_target=torch.from_numpy(rng.uniform(0.05, 5.0, n).astype(np.float32)).cuda())
+    res = []
+    for ctx in ctxs:
+        side = ctx.side(error=keep["error"], sample_weight=keep["sample_weight"], q_target=keep["q_target"],
+                        error_sum=0.7 * n)
+        p0 = {net: ctx.params_get(net) for net in (na.STATNET, na.RRSNET)}
+        na.check(ctx.train_step(vb, 3, film=film, target_L=tL, side=side), "train_step")
+        res.append({net: ctx.params_get(net) - p0[net] for net in (na.STATNET, na.RRSNET)})
+    for net in (na.STATNET, na.RRSNET):
+        a, b = res[0][net], res[1][net]
+        moved = np.abs(b) > 0
+        differ = a != b
+        print(f"net {net}: {int(moved.sum())} parameters moved, {int(differ.sum())} differ")
+        assert moved.sum() > 1000
+        assert differ.sum() <= 1e-3 * moved.sum() + 8
