@@ -335,7 +335,6 @@ nrrs_status ensure_train(nrrs_ctx* c, int net) {
     return hip_status(hipDeviceSynchronize());
 }
 
-// Forward + loss + backward of one net into its gradient buffer.
 // Binned-scatter arguments of one net (its records, counts and segments).
 BinArgs bin_args(nrrs_ctx* c, int net, uint32_t n, const EncodeArgs& a) {
     NetState& ns = c->net[net];
@@ -381,6 +380,7 @@ hipError_t launch_apply(nrrs_ctx* c, int net, const uint16_t* key, hipStream_t s
     return launch_apply_bins(ap, ns.n_direct, ns.n_chunks, s);
 }
 
+// Forward + loss + backward of one net into its gradient buffer.
 // `pre_enc`: the level-major encoding of this batch and net (and, for a binned net, its record
 // counts) already written by a paired encode pass; null = run the net's own encode pass.
 // `blocks_out`: stop after the training kernel and report its workgroups (the caller then
