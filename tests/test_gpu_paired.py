@@ -177,3 +177,28 @@ def test_joint_step_with_full_segments(torch_cuda, ora):
         print(f"net {net}: {int(moved.sum())} parameters moved, {int(differ.sum())} differ")
         assert moved.sum() > 1000
         assert differ.sum() <= 1e-3 * moved.sum() + 8
+
+
+def test_paired_table_follows_single_net_updates(torch_cuda, ora):
+    """A StatNet update through nrrs_backward + nrrs_adam_step (not the joint step) refreshes the
+    paired table: the chained stage afterwards is bitwise the same as with per-net tables."""
+    torch = torch_cuda
+    n = 3001
+    ctxs = _contexts(na.BF16, n)
+    vb = na.synth_vertices(n, 24, n)
+    film = torch.from_numpy(np.random.default_rng(9).lognormal(-1, 1, (3, n)).astype(np.float32)).cuda()
+    tL = torch.from_numpy(ora.synth_targets(n, 14)).cuda()
+    qs = []
+    for ctx in ctxs:
+        for _ in range(2):
+            na.check(ctx.backward(na.STATNET, vb, na.LOSS_LL2, target_L=tL, loss_scale=4096.0), "backward")
+            ctx.adam_step(na.STATNET)
+        q = torch.empty(n, device="cuda")
+        surv = torch.empty(8 * n, dtype=torch.int32, device="cuda")
+        ns = torch.zeros(1, dtype=torch.int32, device="cuda")
+        args = ctx.decide_args(vb, film, q, surv, ns, q_mode=na.Q_RRSNET_CHAINED, seed=4, stream_id=6)
+        na.check(ctx.rrs_infer_decide(args), "rrs_infer_decide")
+        torch.cuda.synchronize()
+        qs.append(q.cpu().numpy())
+    assert ctxs[0].train_state(na.STATNET)["step"] == 2
+    assert np.array_equal(qs[0], qs[1])
