@@ -772,6 +772,9 @@ nrrs_status nrrs_create(const nrrs_config* cfg, nrrs_ctx** out) {
     if (!c) return NRRS_ERR_OOM;
     c->cfg = *cfg;
     c->device = dev;
+    // the training kernel keeps one workgroup per CU resident (its LDS stage and weight tiles):
+    // one persistent workgroup per CU, so the dW partials are one per CU as well
+    if (prop.multiProcessorCount > 0) c->max_train_blocks = (uint32_t)std::min(prop.multiProcessorCount, 1024);
     uint64_t n_grid = 0;
     if (!make_grid(cfg->grid, &c->grid, nullptr, nullptr, nullptr, nullptr, nullptr, &n_grid)) {
         delete c;
