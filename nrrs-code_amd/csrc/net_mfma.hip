@@ -257,9 +257,18 @@ hipError_t launch_chained(const EncodeArgs& a, const uint32_t* enc_rrs, uint32_t
         if (e != hipSuccess) return e;
         attr = true;
     }
+    // persistent: the resident workgroups only (two per CU), each loading the 68 KB of tiles once
+    static uint32_t resident = 0;
+    if (!resident) {
+        int dev = 0, cus = 0;
+        hipError_t e = hipGetDevice(&dev);
+        if (e == hipSuccess) e = hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
+        if (e != hipSuccess) return e;
+        resident = 2u * (uint32_t)(cus > 0 ? cus : 1);
+    }
     const uint32_t n_cb = (n + 31) / 32;
     uint32_t blocks = (n_cb + kChainWaves - 1) / kChainWaves;
-    blocks = blocks > 2048 ? 2048 : blocks;
+    blocks = blocks > resident ? resident : blocks;
     if (!blocks) return hipSuccess;
     hipLaunchKernelGGL(k_net_chained<P>, dim3(blocks), dim3(kChainT), smem, s, a, enc_rrs, n,
                        reinterpret_cast<const u32x4*>(Ws), reinterpret_cast<const u32x4*>(Wr), ep);
