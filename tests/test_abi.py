@@ -35,6 +35,7 @@ def test_struct_layouts_and_defaults():
     assert cfg.grid.n_levels == 16 and cfg.grid.log2_table_size == 19
     assert cfg.rrs_clamp_min == C.c_float(0.05).value and cfg.rrs_clamp_max == 20.0
     assert cfg.rrs_use_ceil == 1 and cfg.rrs_no_upscale == 1 and cfg.split_cap == 8
+    assert cfg.paired_tables == 1  # both nets' grids gathered in one pass by default
     assert lib.nrrs_status_string(na.ERR_OVERFLOW) == b"survivor queue overflow"
 
 
