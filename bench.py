@@ -256,8 +256,8 @@ class Path:
         return e0.elapsed_time(e1) / 1e3 / reps
 
     def time_statnet_kernel(self, reps):
-        """Average duration of the fused StatNet encode+MLP inference kernel (one
-        launch per call), HIP events on the stream it is launched on."""
+        """Average duration of one StatNet inference call (the level-major encode pass, then
+        the MFMA MLP kernel), HIP events on the stream they are launched on."""
         torch = self.torch
         stream = torch.cuda.current_stream()
         out = self.ctx.statnet_infer(self.vb)
