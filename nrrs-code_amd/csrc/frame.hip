@@ -222,6 +222,9 @@ __global__ __launch_bounds__(64 * WAVES) void k_frame(FrameParams f) {
                         for (int kb = 0; kb < 5; ++kb) B[kb] = typename P::vec8{};
                         Br[0] = Br[1] = typename P::vec8{};
                     }
+                    // K chunks 2 and 3 (SH-4 of the direction and the normal) are the same in
+                    // both nets' inputs
+                    const typename P::vec8 sh_dir = B[2], sh_nrm = B[3];
                     const f32x16 Ys = mlp_forward<P, 5>(sWs, B, lane);
                     // statistics of the lane's vertex: outputs 0..3 in half 0, 4..5 in half 1
                     float st[6];
@@ -242,6 +245,8 @@ __global__ __launch_bounds__(64 * WAVES) void k_frame(FrameParams f) {
                         B[0] = Br[0];
                         B[1] = Br[1];
                         encode_tail_half<P>(v, NRRS_RRSNET, h, ref, st, B);
+                        B[2] = sh_dir;  // (the recomputed chunks are dead and not evaluated)
+                        B[3] = sh_nrm;
                     }
                     const f32x16 Yr = mlp_forward<P, 4>(sWr, B, lane);
                     if (live && h == 0) {

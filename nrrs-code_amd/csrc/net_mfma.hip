@@ -214,6 +214,8 @@ __global__ __launch_bounds__(kChainT, 2) void k_net_chained(EncodeArgs a, const 
 #pragma unroll
             for (int kb = 0; kb < 5; ++kb) B[kb] = typename P::vec8{};
         }
+        // K chunks 2 and 3 (SH-4 of the direction and the normal) are the same in both nets' inputs
+        const typename P::vec8 sh_dir = B[2], sh_nrm = B[3];
         const f32x16 Ys = mlp_forward<P, 5>(sWs, B, lane);
         // statistics of the lane's vertex: outputs 0..3 in half 0, 4..5 in half 1
         float st[6];
@@ -232,6 +234,8 @@ __global__ __launch_bounds__(kChainT, 2) void k_net_chained(EncodeArgs a, const 
             const float ref = film_mean(a.film, a.n_pixels, v.pixel);
             grid_half_from_levels<P>(enc_rrs, a.grid_enc_stride, a.grid.n_levels, i, h, B);
             encode_tail_half<P>(v, NRRS_RRSNET, h, ref, st, B);
+            B[2] = sh_dir;  // (the recomputed chunks are dead and not evaluated)
+            B[3] = sh_nrm;
         } else {
 #pragma unroll
             for (int kb = 0; kb < 5; ++kb) B[kb] = typename P::vec8{};
