@@ -756,7 +756,7 @@ __global__ __launch_bounds__(64 * kScanWaves) void k_bin_scan(BinArgs b, uint32_
     if (w == 0 && tile == n_tiles - 1) b.seg_count[seg] = (uint32_t)total;
 }
 
-// k_bin_levels: workgroup b takes level b / chunks and kBinVert vertices (two runs of the scan),
+// k_bin_levels: workgroup b takes level b / chunks and kBinVert vertices (four runs of the scan),
 // so the GPU walks the levels one after the other. Each thread takes one vertex: per (y, z) of its
 // cell, one record for the x-adjacent corner pair or, for pair code 3, one per corner:
 //  1. every record gets its rank within its segment from LDS atomics;
@@ -772,7 +772,9 @@ __global__ __launch_bounds__(64 * kScanWaves) void k_bin_scan(BinArgs b, uint32_
 //     list, and so the workgroup's LDS, small enough for more resident workgroups).
 // Records beyond a full segment's slots are added directly to the table gradient with packed
 // FP16 atomics.
-constexpr int kBinVert = 2 * kRunVert;
+// 1024 threads: 71 KB of LDS (79 KB for both nets), two workgroups per CU; half the workgroups
+// of 512-vertex ones, so half the per-workgroup segment scans and copy-out tails (3% faster)
+constexpr int kBinVert = 4 * kRunVert;
 constexpr int kBinMaxRec = 8 * kBinVert;  // every pair split
 
 // Value of a record: the FP16 pairs w * dL/denc of the corners (x = 0, 1) of (y, z) = yz, both in
