@@ -188,10 +188,10 @@ size_t mfma_packed_halves(int n_layers);
 // FP16/BF16 shadows of one net: packed MLP tiles and, with `grid`, the grid table.
 hipError_t launch_pack_lp(int precision, const float* grid, uint64_t n_grid, const float* W, int n_layers,
                           int n_out, uint16_t* grid_lp, uint16_t* W_lp, hipStream_t s);
-// Level-major grid encoding: one workgroup per (level, 256 vertices), levels in order, so the
-// whole GPU gathers from one level's table at a time (it stays in L2). With run_counts, also
-// counts the corners of every (level, table segment, 256 vertices) for the binned scatter:
-// run_counts[(level * ceil(n / 256) + chunk) * kMaxSegBins + segment].
+// Level-major grid encoding: one workgroup per (level, kRunVert = 1024 vertices), levels in
+// order, so the whole GPU gathers from one level's table at a time (it stays in L2). With
+// run_counts, also counts the corners of every (level, table segment, 1024 vertices) for the
+// binned scatter: run_counts[(level * ceil(n / 1024) + chunk) * kMaxSegBins + segment].
 hipError_t launch_encode_levels(int precision, const EncodeArgs& a, const uint16_t* table, uint32_t n, uint32_t* out,
                                 uint32_t stride, hipStream_t s, uint32_t* run_counts = nullptr);
 // The same pass for both nets from the paired table (entry e = {StatNet word, RRSNet word}):

@@ -756,7 +756,7 @@ __global__ __launch_bounds__(64 * kScanWaves) void k_bin_scan(BinArgs b, uint32_
     if (w == 0 && tile == n_tiles - 1) b.seg_count[seg] = (uint32_t)total;
 }
 
-// k_bin_levels: workgroup b takes level b / chunks and kBinVert vertices (four runs of the scan),
+// k_bin_levels: workgroup b takes level b / chunks and kBinVert vertices (one run of the scan),
 // so the GPU walks the levels one after the other. Each thread takes one vertex: per (y, z) of its
 // cell, one record for the x-adjacent corner pair or, for pair code 3, one per corner:
 //  1. every record gets its rank within its segment from LDS atomics;
@@ -773,8 +773,10 @@ __global__ __launch_bounds__(64 * kScanWaves) void k_bin_scan(BinArgs b, uint32_
 // Records beyond a full segment's slots are added directly to the table gradient with packed
 // FP16 atomics.
 // 1024 threads: 71 KB of LDS (79 KB for both nets), two workgroups per CU; half the workgroups
-// of 512-vertex ones, so half the per-workgroup segment scans and copy-out tails (3% faster)
-constexpr int kBinVert = 4 * kRunVert;
+// of 512-vertex ones, so half the per-workgroup segment scans and copy-out tails (3% faster).
+// One run per workgroup: its slots of each segment start at the run's first slot.
+constexpr int kBinVert = kRunVert;
+static_assert(kBinVert == (int)kRunVert, "the first-slot read takes one run per workgroup");
 constexpr int kBinMaxRec = 8 * kBinVert;  // every pair split
 
 // Value of a record: the FP16 pairs w * dL/denc of the corners (x = 0, 1) of (y, z) = yz, both in

@@ -14,7 +14,7 @@ constexpr uint32_t kSegEntries = 1u << kSegShift;
 constexpr uint32_t kMaxSegBins = 64;
 // The encode pass counts the records of every (level, segment, kRunVert vertices); a scan of
 // these counts gives each run its first record slot.
-constexpr uint32_t kRunVert = 256;
+constexpr uint32_t kRunVert = 1024;
 // Chunks per tile of the scan of the run counts.
 constexpr uint32_t kBinScanRows = 256;
 // Records one workgroup of the chunked apply sums (segments of the dense levels, where one
