@@ -35,7 +35,7 @@ BF16_PEAK_TFLOPS = 2500.0
 # k_bin_scan + k_bin_levels + k_apply_bins + k_apply_chunks + k_apply_finish +
 # k_reduce_partials), from rocprofv3 --pmc FETCH_SIZE and WRITE_SIZE
 # (profiles/train_backward_bf16_2p20_pmc.md), or None when not measured.
-TRAIN_TRAFFIC_BYTES = 2867101664
+TRAIN_TRAFFIC_BYTES = 2709207040
 MACS = {"statnet": 80 * 64 + 3 * 64 * 64 + 64 * 6, "rrsnet": 80 * 64 + 2 * 64 * 64 + 64 * 1}
 
 

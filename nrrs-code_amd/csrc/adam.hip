@@ -31,35 +31,92 @@ __global__ void k_check_finite(const float* __restrict__ g, uint64_t n, const ui
     if (__any(bad) && (threadIdx.x & 63) == 0) atomicOr(flag, 1u);
 }
 
+struct AdamStep {
+    float inv_s, c1, c2;
+};
+
+__device__ __forceinline__ AdamStep adam_step(const AdamArgs& a) {
+    const uint32_t t = a.state[0] + 1;
+    return AdamStep{1.0f / *a.loss_scale, 1.0f - powf(a.beta1, (float)t), 1.0f - powf(a.beta2, (float)t)};
+}
+
+// One parameter: gradient g (loss-scaled), master p and moments m, v updated in place.
+__device__ __forceinline__ void adam_one(const AdamArgs& a, const AdamStep& k, float g, float& p, float& m, float& v) {
+    g = g * k.inv_s + a.l2 * p;
+    m = a.beta1 * m + (1.0f - a.beta1) * g;
+    v = a.beta2 * v + (1.0f - a.beta2) * g * g;
+    p -= a.lr * (m / k.c1) / (sqrtf(v / k.c2) + a.eps);
+}
+
+__device__ __forceinline__ uint16_t lp_bits(const AdamArgs& a, float p) {
+    return a.lp_precision == NRRS_BF16 ? __builtin_bit_cast(uint16_t, (__bf16)p) : __builtin_bit_cast(uint16_t, (_Float16)p);
+}
+
+// One parameter i (gradient read and cleared, update, shadow refreshed).
+__device__ __forceinline__ void adam_scalar(const AdamArgs& a, const AdamStep& k, uint32_t skip, uint64_t i) {
+    float g;
+    if (a.grad_lp && i < a.n_grid) {
+        g = (float)__builtin_bit_cast(_Float16, a.grad_lp[i]);
+        a.grad_lp[i] = 0;
+    } else {
+        g = a.grad[i];
+        a.grad[i] = 0.0f;
+    }
+    if (skip) return;
+    float p = a.params[i], m = a.m[i], v = a.v[i];
+    adam_one(a, k, g, p, m, v);
+    a.m[i] = m;
+    a.v[i] = v;
+    a.params[i] = p;
+    if (a.grid_lp && i < a.n_grid) a.grid_lp[i] = lp_bits(a, p);
+}
+
+// Any n_grid: one parameter per thread and iteration.
+__global__ void k_adam_scalar(AdamArgs a) {
+    const uint32_t skip = a.state[3];
+    const AdamStep k = adam_step(a);
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < a.n; i += (uint64_t)gridDim.x * blockDim.x)
+        adam_scalar(a, k, skip, i);
+}
+
+// Four consecutive parameters per thread and iteration: 16-byte loads and stores of the FP32
+// arrays, 8-byte ones of the FP16 gradient and the low-precision shadow. Needs n_grid % 4 == 0
+// (a group is all grid or all MLP); the last n % 4 parameters are taken one by one.
 __global__ void k_adam(AdamArgs a) {
     const uint32_t skip = a.state[3];
-    const float inv_s = 1.0f / *a.loss_scale;
-    const uint32_t t = a.state[0] + 1;
-    const float c1 = 1.0f - powf(a.beta1, (float)t), c2 = 1.0f - powf(a.beta2, (float)t);
-    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < a.n; i += (uint64_t)gridDim.x * blockDim.x) {
-        float g;
-        if (a.grad_lp && i < a.n_grid) {
-            g = (float)__builtin_bit_cast(_Float16, a.grad_lp[i]);
-            a.grad_lp[i] = 0;
+    const AdamStep k = adam_step(a);
+    const uint64_t n4 = a.n / 4, stride = (uint64_t)gridDim.x * blockDim.x;
+    for (uint64_t c = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; c < n4; c += stride) {
+        const uint64_t i = 4 * c;
+        const bool lp = a.grad_lp && i < a.n_grid;
+        float g[4];
+        if (lp) {
+            const uint2 w = reinterpret_cast<const uint2*>(a.grad_lp)[c];
+            const uint32_t h[4] = {w.x & 0xffffu, w.x >> 16, w.y & 0xffffu, w.y >> 16};
+#pragma unroll
+            for (int u = 0; u < 4; ++u) g[u] = (float)__builtin_bit_cast(_Float16, (uint16_t)h[u]);
+            reinterpret_cast<uint2*>(a.grad_lp)[c] = make_uint2(0u, 0u);
         } else {
-            g = a.grad[i];
-            a.grad[i] = 0.0f;
+            const float4 w = reinterpret_cast<const float4*>(a.grad)[c];
+            g[0] = w.x, g[1] = w.y, g[2] = w.z, g[3] = w.w;
+            reinterpret_cast<float4*>(a.grad)[c] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
         }
         if (skip) continue;
-        float p = a.params[i];
-        g = g * inv_s + a.l2 * p;
-        float m = a.beta1 * a.m[i] + (1.0f - a.beta1) * g;
-        float v = a.beta2 * a.v[i] + (1.0f - a.beta2) * g * g;
-        a.m[i] = m;
-        a.v[i] = v;
-        p -= a.lr * (m / c1) / (sqrtf(v / c2) + a.eps);
-        a.params[i] = p;
-        if (a.grid_lp && i < a.n_grid) {
-            const uint16_t b = a.lp_precision == NRRS_BF16 ? __builtin_bit_cast(uint16_t, (__bf16)p)
-                                                            : __builtin_bit_cast(uint16_t, (_Float16)p);
-            a.grid_lp[i] = b;
-        }
+        float4 p = reinterpret_cast<const float4*>(a.params)[c], m = reinterpret_cast<const float4*>(a.m)[c],
+               v = reinterpret_cast<const float4*>(a.v)[c];
+        adam_one(a, k, g[0], p.x, m.x, v.x);
+        adam_one(a, k, g[1], p.y, m.y, v.y);
+        adam_one(a, k, g[2], p.z, m.z, v.z);
+        adam_one(a, k, g[3], p.w, m.w, v.w);
+        reinterpret_cast<float4*>(a.m)[c] = m;
+        reinterpret_cast<float4*>(a.v)[c] = v;
+        reinterpret_cast<float4*>(a.params)[c] = p;
+        if (a.grid_lp && i < a.n_grid)
+            reinterpret_cast<uint2*>(a.grid_lp)[c] = make_uint2(lp_bits(a, p.x) | (uint32_t)lp_bits(a, p.y) << 16,
+                                                                lp_bits(a, p.z) | (uint32_t)lp_bits(a, p.w) << 16);
     }
+    for (uint64_t i = 4 * n4 + (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < a.n; i += stride)
+        adam_scalar(a, k, skip, i);
 }
 
 __global__ void k_adam_finish(AdamArgs a) {
@@ -107,7 +164,10 @@ hipError_t launch_adam(const AdamArgs& a, hipStream_t s) {
     const uint64_t off = a.grad_lp ? a.n_grid : 0;
     hipLaunchKernelGGL(k_check_finite, dim3(1024), dim3(256), 0, s, a.grad + off, a.n - off, a.grad_lp,
                        a.grad_lp ? a.n_grid : 0, a.state + 3);
-    hipLaunchKernelGGL(k_adam, dim3(2048), dim3(256), 0, s, a);
+    if (a.n_grid % 4 == 0)
+        hipLaunchKernelGGL(k_adam, dim3(2048), dim3(256), 0, s, a);
+    else
+        hipLaunchKernelGGL(k_adam_scalar, dim3(2048), dim3(256), 0, s, a);
     hipLaunchKernelGGL(k_adam_finish, dim3(1), dim3(1), 0, s, a);
     return hipGetLastError();
 }
