@@ -4,6 +4,8 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <atomic>
+
 #include "../../include/nrrs_amd.h"
 
 namespace nrrs {
@@ -183,6 +185,20 @@ hipError_t launch_encode(const EncodeArgs& a, int net, uint32_t n, const float* 
 hipError_t launch_grid_indices(const EncodeArgs& a, uint32_t n, uint32_t* idx, hipStream_t s);
 hipError_t launch_net_fp32(const EncodeArgs& a, int net, int n_layers, int n_out, int epi, uint32_t n,
                            const float* table, const float* W, const EpiArgs& ep, hipStream_t s);
+
+// Launch set-up that is done once per device: `done` is the caller's static table, one flag per
+// device ordinal (devices beyond kMaxDevices repeat the set-up on every launch).
+constexpr int kMaxDevices = 64;
+inline hipError_t set_dyn_lds_once(std::atomic<bool>* done, const void* kernel, size_t bytes) {
+    int d = 0;
+    hipError_t e = hipGetDevice(&d);
+    if (e != hipSuccess) return e;
+    const bool cached = d >= 0 && d < kMaxDevices;
+    if (cached && done[d].load(std::memory_order_acquire)) return hipSuccess;
+    e = hipFuncSetAttribute(kernel, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes);
+    if (e == hipSuccess && cached) done[d].store(true, std::memory_order_release);
+    return e;
+}
 
 size_t mfma_packed_halves(int n_layers);
 // FP16/BF16 shadows of one net: packed MLP tiles and, with `grid`, the grid table.

@@ -135,13 +135,9 @@ template <int NL, int NOUT, int EPI>
 hipError_t launch_one(const EncodeArgs& a, int net, uint32_t n, const float* table, const float* W,
                       const EpiArgs& ep, hipStream_t s) {
     constexpr size_t smem = sizeof(float) * mlp_floats<NL, NOUT>();
-    static bool attr = false;
-    if (!attr) {
-        hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(&k_net_fp32<NL, NOUT, EPI>),
-                                           hipFuncAttributeMaxDynamicSharedMemorySize, (int)smem);
-        if (e != hipSuccess) return e;
-        attr = true;
-    }
+    static std::atomic<bool> attr[kMaxDevices];
+    hipError_t e = set_dyn_lds_once(attr, reinterpret_cast<const void*>(&k_net_fp32<NL, NOUT, EPI>), smem);
+    if (e != hipSuccess) return e;
     dim3 grid((n + kBlock - 1) / kBlock);
     if (grid.x == 0) return hipSuccess;
     hipLaunchKernelGGL((k_net_fp32<NL, NOUT, EPI>), grid, dim3(kBlock), smem, s, a, net, n, table, W, ep);

@@ -254,21 +254,19 @@ template <typename P>
 hipError_t launch_chained(const EncodeArgs& a, const uint32_t* enc_rrs, uint32_t n, const uint16_t* Ws,
                           const uint16_t* Wr, const EpiArgs& ep, hipStream_t s) {
     constexpr size_t smem = (size_t)(tiles_of(5) + tiles_of(4)) * kTileHalves * 2;
-    static bool attr = false;
-    if (!attr) {
-        hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(&k_net_chained<P>),
-                                           hipFuncAttributeMaxDynamicSharedMemorySize, (int)smem);
-        if (e != hipSuccess) return e;
-        attr = true;
-    }
+    static std::atomic<bool> attr[kMaxDevices];
+    hipError_t e = set_dyn_lds_once(attr, reinterpret_cast<const void*>(&k_net_chained<P>), smem);
+    if (e != hipSuccess) return e;
     // persistent: the resident workgroups only (two per CU), each loading the 68 KB of tiles once
-    static uint32_t resident = 0;
+    static std::atomic<uint32_t> resident_of[kMaxDevices];
+    int dev = 0;
+    if ((e = hipGetDevice(&dev)) != hipSuccess) return e;
+    uint32_t resident = dev >= 0 && dev < kMaxDevices ? resident_of[dev].load(std::memory_order_relaxed) : 0u;
     if (!resident) {
-        int dev = 0, cus = 0;
-        hipError_t e = hipGetDevice(&dev);
-        if (e == hipSuccess) e = hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
-        if (e != hipSuccess) return e;
+        int cus = 0;
+        if ((e = hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev)) != hipSuccess) return e;
         resident = 2u * (uint32_t)(cus > 0 ? cus : 1);
+        if (dev >= 0 && dev < kMaxDevices) resident_of[dev].store(resident, std::memory_order_relaxed);
     }
     const uint32_t n_cb = (n + 31) / 32;
     uint32_t blocks = (n_cb + kChainWaves - 1) / kChainWaves;

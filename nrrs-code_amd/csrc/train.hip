@@ -647,13 +647,9 @@ hipError_t launch_train_one_bin(const TrainArgs& t, uint32_t blocks, hipStream_t
         sizeof(Stage<P>) + (P::kWaves == 8 ? (size_t)(10 + (NL - 2) * 8 + 4 + 4 + (NL - 2) * 8 + 2) * 64 *
                                                  sizeof(typename P::op)
                                            : 0);
-    static bool attr = false;
-    if (!attr) {
-        hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(&k_train<P, NL, NOUT, LOSS, BIN>),
-                                           hipFuncAttributeMaxDynamicSharedMemorySize, (int)smem);
-        if (e != hipSuccess) return e;
-        attr = true;
-    }
+    static std::atomic<bool> attr[kMaxDevices];
+    hipError_t e = set_dyn_lds_once(attr, reinterpret_cast<const void*>(&k_train<P, NL, NOUT, LOSS, BIN>), smem);
+    if (e != hipSuccess) return e;
     hipLaunchKernelGGL((k_train<P, NL, NOUT, LOSS, BIN>), dim3(blocks), dim3(Geo<P>::kThreads), smem, s, t);
     return hipGetLastError();
 }
@@ -1119,14 +1115,10 @@ hipError_t launch_train(int precision, const TrainArgs& t, int nl, int n_out, in
 
 hipError_t launch_apply_bins(const ApplyArgs& a, uint32_t n_direct, uint32_t n_chunks, hipStream_t s) {
     constexpr size_t smem = 2 * kSegEntries * sizeof(unsigned long long);
-    static bool attr = false;
-    if (!attr) {
-        for (const void* k : {reinterpret_cast<const void*>(&k_apply_bins), reinterpret_cast<const void*>(&k_apply_chunks)}) {
-            hipError_t e = hipFuncSetAttribute(k, hipFuncAttributeMaxDynamicSharedMemorySize, (int)smem);
-            if (e != hipSuccess) return e;
-        }
-        attr = true;
-    }
+    static std::atomic<bool> attr_bins[kMaxDevices], attr_chunks[kMaxDevices];
+    hipError_t e = set_dyn_lds_once(attr_bins, reinterpret_cast<const void*>(&k_apply_bins), smem);
+    if (e == hipSuccess) e = set_dyn_lds_once(attr_chunks, reinterpret_cast<const void*>(&k_apply_chunks), smem);
+    if (e != hipSuccess) return e;
     if (n_direct) hipLaunchKernelGGL(k_apply_bins, dim3(n_direct), dim3(kApplyThreads), smem, s, a);
     if (n_chunks) {
         hipLaunchKernelGGL(k_apply_chunks, dim3(n_chunks), dim3(kApplyThreads), smem, s, a);
