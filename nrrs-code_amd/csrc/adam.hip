@@ -71,7 +71,9 @@ __device__ __forceinline__ void adam_scalar(const AdamArgs& a, const AdamStep& k
     if (a.grid_lp && i < a.n_grid) a.grid_lp[i] = lp_bits(a, p);
 }
 
-// Any n_grid: one parameter per thread and iteration.
+// Any n_grid: one parameter per thread and iteration. The 16-level grids with base resolutions
+// 8-23 and T = 2^17 or 2^19 all have even entry counts, so n_grid is a multiple of 4 and k_adam
+// runs there; this kernel keeps any other layout correct.
 __global__ void k_adam_scalar(AdamArgs a) {
     const uint32_t skip = a.state[3];
     const AdamStep k = adam_step(a);
