@@ -27,6 +27,10 @@ COMMON_FLAGS = [
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
     # Hardware float atomics for the gradient scatter-adds.
     "-munsafe-fp-atomics",
+    # Device code fuses a * b + c only within one expression (C semantics), never across
+    # statements: the same source then rounds the same way in every kernel that inlines it,
+    # whatever basic blocks the surrounding code gives it.
+    "-ffp-contract=on",
     # Host FP32 expressions (grid frame, normalisation factor) round as written.
     "-Xarch_host", "-ffp-contract=off",
     "-Wall", "-Wno-unused-function",

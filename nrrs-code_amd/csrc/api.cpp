@@ -205,7 +205,7 @@ nrrs_status refresh_train(nrrs_ctx* c, int net, hipStream_t s) {
     return hip_status(launch_pack_train(c->cfg.precision, ns.mlp_f32, ns.n_layers, ns.n_out, ns.w_fwd, ns.w_bwd, s));
 }
 
-// Training buffers of one net, allocated on first use.
+// Training buffers of one net (nrrs_create, when the context trains).
 nrrs_status ensure_train(nrrs_ctx* c, int net) {
     NetState& ns = c->net[net];
     if (ns.train_ready) return NRRS_OK;
@@ -321,11 +321,12 @@ nrrs_status ensure_train(nrrs_ctx* c, int net) {
         }
     }
     NRRS_TRY(dmalloc(&ns.loss_scale, 1));
-    NRRS_TRY(dmalloc(&ns.state, 4));
+    NRRS_TRY(dmalloc(&ns.state, 8));
+    NRRS_TRY(dmalloc(&ns.loss_partial, c->max_train_blocks));
     NRRS_HIP(hipMemset(ns.grad, 0, np * 4));
     NRRS_HIP(hipMemset(ns.adam_m, 0, np * 4));
     NRRS_HIP(hipMemset(ns.adam_v, 0, np * 4));
-    NRRS_HIP(hipMemset(ns.state, 0, 16));
+    NRRS_HIP(hipMemset(ns.state, 0, 32));
     const float s0 = c->cfg.precision == NRRS_FP32 ? 1.0f : c->cfg.loss_scale_init;
     NRRS_HIP(hipMemcpy(ns.loss_scale, &s0, 4, hipMemcpyHostToDevice));
     if (!c->train_stats) NRRS_TRY(dmalloc(&c->train_stats, (size_t)6 * c->cfg.max_train));
@@ -446,12 +447,14 @@ nrrs_status run_backward(nrrs_ctx* c, int net, uint32_t n, const nrrs_vertices& 
     }
     t.dw_partial = ns.dw_partial;
     t.n_mlp = (uint32_t)ns.n_mlp;
-    t.loss_out = loss_out;
+    t.loss_partial = kind == NRRS_LOSS_EXTERNAL ? nullptr : ns.loss_partial;
     const uint32_t vpi = train_vertices_per_iter(cfg.precision);
     uint32_t iters = (n + vpi - 1) / vpi;
     uint32_t blocks = iters < c->max_train_blocks ? iters : c->max_train_blocks;
     if (!blocks) return NRRS_OK;
     NRRS_HIP(launch_train(cfg.precision, t, ns.n_layers, ns.n_out, kind, blocks, s));
+    if (kind != NRRS_LOSS_EXTERNAL)
+        NRRS_HIP(launch_loss_finish(ns.loss_partial, blocks, loss_out, ns.state + 4, s));
     if (blocks_out) {  // the caller bins, applies and reduces (joint step)
         *blocks_out = blocks;
         return NRRS_OK;
@@ -463,7 +466,7 @@ nrrs_status run_backward(nrrs_ctx* c, int net, uint32_t n, const nrrs_vertices& 
     return hip_status(launch_reduce_partials(ns.dw_partial, blocks, (uint32_t)ns.n_mlp, ns.grad + ns.n_grid, s));
 }
 
-// Path-node buffer, allocated on first use: 4 words x 3 channels x 7 planes + 4 scalars.
+// Path-node buffer (nrrs_create): 4 words x 3 channels x 7 planes + 4 scalars.
 nrrs_status ensure_nodes(nrrs_ctx* c) {
     NodeBuf& nb = c->nodes;
     if (nb.cap) return NRRS_OK;
@@ -535,31 +538,16 @@ nrrs_status run_adam(nrrs_ctx* c, int net, hipStream_t s) {
     return refresh_train(c, net, s);
 }
 
-// Frame scratch for queues of up to `cap` vertices (allocated outside any capture).
-nrrs_status ensure_frame(nrrs_ctx* c, uint32_t cap) {
-    if (c->frame_cap >= cap) return NRRS_OK;
-    hipFree(c->frame_q);
-    hipFree(c->frame_s);
-    hipFree(c->frame_tiles);
-    hipFree(c->frame_x);
-    hipFree(c->frame_genc);
-    c->frame_q = nullptr;
-    c->frame_s = nullptr;
-    c->frame_tiles = nullptr;
-    c->frame_x = nullptr;
-    c->frame_genc = nullptr;
-    c->frame_cap = 0;
+// Frame scratch for queues of up to `cap` vertices, allocated once by nrrs_create (so a
+// captured frame graph never holds pointers that a later call frees).
+nrrs_status alloc_frame(nrrs_ctx* c, uint32_t cap) {
     NRRS_TRY(dmalloc(&c->frame_q, cap));
     NRRS_TRY(dmalloc(&c->frame_s, cap));
     NRRS_TRY(dmalloc(&c->frame_tiles, kFrameMaxBlocks));
     NRRS_TRY(dmalloc(&c->frame_x, 3 * (size_t)cap));
     NRRS_TRY(dmalloc(&c->frame_genc, 2 * (size_t)c->grid.n_levels * cap));
-    if (!c->frame_sums) NRRS_TRY(dmalloc(&c->frame_sums, 16));
-    if (!c->frame_bar) {
-        NRRS_TRY(dmalloc(&c->frame_bar, kFrameBarWords));
-        NRRS_HIP(hipMemset(c->frame_bar, 0, kFrameBarWords * 4));
-        NRRS_HIP(hipDeviceSynchronize());
-    }
+    NRRS_TRY(dmalloc(&c->frame_sums, 16));
+    NRRS_TRY(dmalloc(&c->frame_bar, kFrameBarWords));
     c->frame_cap = cap;
     return NRRS_OK;
 }
@@ -569,7 +557,7 @@ nrrs_status frame_params(nrrs_ctx* c, const nrrs_frame_args* a, FrameParams* f) 
     if (a->n_bounces < 1 || a->n_bounces > 16 || a->capacity == 0) return NRRS_ERR_ARG;
     if (a->capacity > c->cfg.max_queue) return NRRS_ERR_SHAPE;
     if (c->cfg.precision == NRRS_FP32) return NRRS_ERR_UNSUPPORTED;
-    NRRS_TRY(ensure_frame(c, a->capacity));
+    if (a->capacity > c->frame_cap) return NRRS_ERR_SHAPE;
     *f = FrameParams{};
     f->grid = c->grid;
     f->frame = c->frame;
@@ -640,6 +628,7 @@ void free_ctx(nrrs_ctx* c) {
         hipFree(ns.dw_partial);
         hipFree(ns.loss_scale);
         hipFree(ns.state);
+        hipFree(ns.loss_partial);
     }
     hipFree(c->train_stats);
     hipFree(c->scale_tmp);
@@ -832,6 +821,15 @@ nrrs_status nrrs_create(const nrrs_config* cfg, nrrs_ctx** out) {
     if (st == NRRS_OK) st = hip_status(hipStreamCreateWithFlags(&c->aux_stream, hipStreamNonBlocking));
     if (st == NRRS_OK) st = hip_status(hipEventCreateWithFlags(&c->ev_fork, hipEventDisableTiming));
     if (st == NRRS_OK) st = hip_status(hipEventCreateWithFlags(&c->ev_join, hipEventDisableTiming));
+    // Everything the entry points use is allocated here, as NRRSNetwork::resize allocates its
+    // buffers up front (network.cpp:351-421): no later call allocates or synchronises, so a
+    // first call can be captured into a graph.
+    if (st == NRRS_OK && cfg->max_train) st = ensure_train(c, NRRS_STATNET);
+    if (st == NRRS_OK && cfg->max_train) st = ensure_train(c, NRRS_RRSNET);
+    if (st == NRRS_OK) st = ensure_nodes(c);
+    if (st == NRRS_OK) st = dmalloc(&c->err_part, error_partial_count());
+    if (st == NRRS_OK && cfg->precision != NRRS_FP32) st = alloc_frame(c, cfg->max_queue);
+    if (st == NRRS_OK) st = hip_status(hipDeviceSynchronize());
     if (st != NRRS_OK) {
         free_ctx(c);
         return st;
@@ -995,7 +993,7 @@ nrrs_status nrrs_backward(nrrs_ctx* ctx, int32_t net, uint32_t n, const nrrs_ver
         return NRRS_ERR_ARG;
     }
     if (net == NRRS_RRSNET && !film_dev) return NRRS_ERR_ARG;
-    NRRS_TRY(ensure_train(ctx, net));
+    if (!ctx->net[net].train_ready) return NRRS_ERR_SHAPE;  // max_train 0
     hipStream_t s = (hipStream_t)stream;
     NRRS_HIP(launch_fill_f32(ctx->scale_tmp, loss_scale, 1, s));
     if (loss_dev) NRRS_HIP(launch_fill_f32(loss_dev, 0.0f, 1, s));
@@ -1005,7 +1003,7 @@ nrrs_status nrrs_backward(nrrs_ctx* ctx, int32_t net, uint32_t n, const nrrs_ver
 
 nrrs_status nrrs_grads_zero(nrrs_ctx* ctx, int32_t net, void* stream) {
     if (!ctx || (net != 0 && net != 1)) return NRRS_ERR_ARG;
-    NRRS_TRY(ensure_train(ctx, net));
+    if (!ctx->net[net].train_ready) return NRRS_ERR_SHAPE;  // max_train 0
     NetState& ns = ctx->net[net];
     if (ns.grad_lp)
         NRRS_HIP(launch_fill_f32(reinterpret_cast<float*>(ns.grad_lp), 0.0f, ns.n_grid / 2, (hipStream_t)stream));
@@ -1018,7 +1016,7 @@ nrrs_status nrrs_grads_zero(nrrs_ctx* ctx, int32_t net, void* stream) {
 
 nrrs_status nrrs_grads_get(nrrs_ctx* ctx, int32_t net, float* host_dst) {
     if (!ctx || (net != 0 && net != 1) || !host_dst) return NRRS_ERR_ARG;
-    NRRS_TRY(ensure_train(ctx, net));
+    if (!ctx->net[net].train_ready) return NRRS_ERR_SHAPE;  // max_train 0
     NetState& ns = ctx->net[net];
     NRRS_TRY(fold_dense(ctx, net, nullptr));
     NRRS_HIP(hipDeviceSynchronize());
@@ -1033,23 +1031,29 @@ nrrs_status nrrs_grads_get(nrrs_ctx* ctx, int32_t net, float* host_dst) {
 
 nrrs_status nrrs_adam_step(nrrs_ctx* ctx, int32_t net, void* stream) {
     if (!ctx || (net != 0 && net != 1)) return NRRS_ERR_ARG;
-    NRRS_TRY(ensure_train(ctx, net));
+    if (!ctx->net[net].train_ready) return NRRS_ERR_SHAPE;  // max_train 0
     NRRS_TRY(run_adam(ctx, net, (hipStream_t)stream));
     return refresh_pair(ctx, (hipStream_t)stream);
 }
 
 nrrs_status nrrs_train_state(nrrs_ctx* ctx, int32_t net, uint32_t* step, float* loss_scale, uint32_t* skipped) {
     if (!ctx || (net != 0 && net != 1)) return NRRS_ERR_ARG;
-    NRRS_TRY(ensure_train(ctx, net));
+    if (!ctx->net[net].train_ready) return NRRS_ERR_SHAPE;  // max_train 0
     NetState& ns = ctx->net[net];
-    uint32_t st[4];
+    uint32_t st[8];
     float sc;
     NRRS_HIP(hipDeviceSynchronize());
-    NRRS_HIP(hipMemcpy(st, ns.state, 16, hipMemcpyDeviceToHost));
+    NRRS_HIP(hipMemcpy(st, ns.state, 32, hipMemcpyDeviceToHost));
     NRRS_HIP(hipMemcpy(&sc, ns.loss_scale, 4, hipMemcpyDeviceToHost));
     if (step) *step = st[0];
     if (loss_scale) *loss_scale = sc;
     if (skipped) *skipped = st[2];
+    if (st[4]) {
+        // a batch loss was NaN/Inf since the previous call (the reference stops on a NaN
+        // loss, integrator.cpp:2650-2656): report it once
+        NRRS_HIP(hipMemset(ns.state + 4, 0, 4));
+        return NRRS_ERR_NONFINITE;
+    }
     return NRRS_OK;
 }
 
@@ -1064,9 +1068,7 @@ nrrs_status nrrs_train_step(nrrs_ctx* ctx, uint32_t net_mask, uint32_t n, const 
     if (loss_dev) NRRS_HIP(launch_fill_f32(loss_dev, 0.0f, 2, s));
     const uint32_t ms = ctx->cfg.max_train;
     bool forked = false;
-    // first-use allocations (they synchronise the device) before any fork
-    if (net_mask & 1u) NRRS_TRY(ensure_train(ctx, NRRS_STATNET));
-    if (net_mask & 2u) NRRS_TRY(ensure_train(ctx, NRRS_RRSNET));
+    if (!ctx->net[NRRS_STATNET].train_ready || !ctx->net[NRRS_RRSNET].train_ready) return NRRS_ERR_SHAPE;
     // joint step: both nets' encodings (and the record counts, the same for both) in one pass
     // over the paired table, before either net's parameters change
     const bool paired = (net_mask & 3u) == 3u && ctx->cfg.precision != NRRS_FP32 && paired_pass(ctx, n);
@@ -1164,7 +1166,6 @@ nrrs_status nrrs_train_step(nrrs_ctx* ctx, uint32_t net_mask, uint32_t n, const 
     } join{ctx, s, forked};
     if (net_mask & 2u) {
         if (!(net_mask & 1u)) {
-            if (!ctx->train_stats) NRRS_TRY(dmalloc(&ctx->train_stats, (size_t)6 * ms));
             EncodeArgs ea = encode_args(ctx, *v, nullptr, nullptr, 0);
             EpiArgs es{};
             es.out = ctx->train_stats;
@@ -1292,14 +1293,13 @@ nrrs_status nrrs_error_blend(nrrs_ctx* ctx, uint32_t n_pixels, const float* L_fr
 
 nrrs_status nrrs_error_compute(nrrs_ctx* ctx, uint32_t n_pixels, const nrrs_error_args* a, void* stream) {
     if (!ctx || !a || (n_pixels && (!a->ref_dev || !a->weighted_L_dev || !a->error_dev))) return NRRS_ERR_ARG;
-    if (!ctx->err_part) NRRS_TRY(dmalloc(&ctx->err_part, error_partial_count()));
     return hip_status(launch_error_compute(n_pixels, a->ref_dev, a->weighted_L_dev, a->spp_dev, a->clamp, a->error_dev,
                                            ctx->err_part, a->error_sum_dev, (hipStream_t)stream));
 }
 
 nrrs_status nrrs_nodes_reset(nrrs_ctx* ctx, void* stream) {
     if (!ctx) return NRRS_ERR_ARG;
-    NRRS_TRY(ensure_nodes(ctx));
+    if (!ctx->nodes.cap) return NRRS_ERR_SHAPE;
     return hip_status(launch_nodes_reset(ctx->nodes, (hipStream_t)stream));
 }
 
@@ -1307,7 +1307,7 @@ nrrs_status nrrs_nodes_record(nrrs_ctx* ctx, uint32_t n, const uint32_t* n_dev, 
                               const int32_t* parent_dev, const float* q_dev, uint32_t depth, int32_t* node_out_dev,
                               void* stream) {
     if (!ctx || !vertices_ok(v, n) || (n && !node_out_dev)) return NRRS_ERR_ARG;
-    NRRS_TRY(ensure_nodes(ctx));
+    if (!ctx->nodes.cap) return NRRS_ERR_SHAPE;
     return hip_status(
         launch_nodes_record(ctx->nodes, n, n_dev, *v, parent_dev, q_dev, depth, node_out_dev, (hipStream_t)stream));
 }
@@ -1315,13 +1315,13 @@ nrrs_status nrrs_nodes_record(nrrs_ctx* ctx, uint32_t n, const uint32_t* n_dev, 
 nrrs_status nrrs_nodes_add_radiance(nrrs_ctx* ctx, uint32_t n, const uint32_t* n_dev, const int32_t* node_dev,
                                     const float* L_dev, uint32_t L_stride, void* stream) {
     if (!ctx || (n && (!node_dev || !L_dev || L_stride < n))) return NRRS_ERR_ARG;
-    NRRS_TRY(ensure_nodes(ctx));
+    if (!ctx->nodes.cap) return NRRS_ERR_SHAPE;
     return hip_status(launch_nodes_radiance(ctx->nodes, n, n_dev, node_dev, L_dev, L_stride, (hipStream_t)stream));
 }
 
 nrrs_status nrrs_nodes_size(nrrs_ctx* ctx, uint32_t* size, uint32_t* invalid) {
     if (!ctx) return NRRS_ERR_ARG;
-    NRRS_TRY(ensure_nodes(ctx));
+    if (!ctx->nodes.cap) return NRRS_ERR_SHAPE;
     uint32_t h[2];
     NRRS_HIP(hipDeviceSynchronize());
     NRRS_HIP(hipMemcpy(h, ctx->nodes.cnt, 8, hipMemcpyDeviceToHost));
@@ -1338,7 +1338,7 @@ nrrs_status nrrs_generate_training_data(nrrs_ctx* ctx, const nrrs_training_args*
         !o.albedo_d || !o.albedo_s || !o.thp || !o.pixel || o.stride < a->capacity)
         return NRRS_ERR_ARG;
     if (a->error_dev && (!a->error_scale || !a->frame_width || a->frame_width % a->error_scale)) return NRRS_ERR_ARG;
-    NRRS_TRY(ensure_nodes(ctx));
+    if (!ctx->nodes.cap) return NRRS_ERR_SHAPE;
     return hip_status(launch_training_data(ctx->nodes, *a, ctx->cfg.n_pixels, (hipStream_t)stream));
 }
 

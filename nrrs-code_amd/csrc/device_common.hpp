@@ -178,6 +178,16 @@ __device__ __forceinline__ void synth_fields_pair(uint64_t seed, uint64_t i, int
     *rough = synth_rough(seed, i);
 }
 
+// The FP32 value x, rounded to FP32 before whatever conversion follows. Without it, the FP16
+// conversion of a multiply-add result can be selected as one mixed-precision FMA that rounds
+// once, straight to FP16 (v_fma_mixlo_f16), where another kernel inlining the same code gets an
+// FP32 FMA and a conversion (two roundings, which differ in rare ties). Operands converted
+// through this helper round the same way in every kernel.
+__device__ __forceinline__ float f32_rounded(float x) {
+    asm("" : "+v"(x));
+    return x;
+}
+
 // ---------------------------------------------------------------- scalar maps
 // activationSigma (src/render/nrrs/net/networkcommon.h:61-68): softplus for x<0,
 // 0.5x + ln 2 above; C1 at 0.

@@ -14,8 +14,8 @@
 //
 // Phases are separated by a software grid barrier. The grid never exceeds the
 // number of workgroups the device keeps resident, and every wait is bounded: a
-// barrier that does not complete within its spin limit sets status bit 0 and all
-// workgroups leave at their next barrier.
+// barrier that does not complete within its spin limit sets status bit 0 and every
+// workgroup leaves at that barrier or the next one it reaches.
 #include "frame_common.hpp"
 #include "mfma_common.hpp"
 
@@ -37,39 +37,63 @@ constexpr uint32_t kBarGroups = 8, kBarStride = 32;  // barrier groups; u32 word
 // fence before arriving and one acquire fence after leaving; the spin itself uses relaxed
 // loads (an acquire load per spin iteration would invalidate the L2 of every spinning
 // workgroup's XCD on each poll). bar: [0] top count, [1] generation, [kBarStride * (1 + g)]
-// arrivals of group g.
+// arrivals of group g; zeroed by k_bar_reset before every launch.
+// A workgroup whose wait exceeds the spin limit sets status bit 0 and the abort bit of the
+// generation word. The generation only ever grows by atomic adds, so the bit stays: every
+// workgroup that is waiting at that barrier, arrives at it later or enters a later barrier sees
+// it and leaves. The decision is taken by one thread and shared through LDS, so a workgroup
+// never splits.
+constexpr uint32_t kBarAbort = 0x80000000u;
+
 __device__ __forceinline__ bool grid_sync(uint32_t* bar, uint32_t nblocks, uint32_t* status) {
+    __shared__ uint32_t pass;
     __syncthreads();
     if (threadIdx.x == 0) {
+        bool ok = true;
         __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
         const uint32_t gen = __hip_atomic_load(bar + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        const uint32_t g = blockIdx.x % kBarGroups, groups = nblocks < kBarGroups ? nblocks : kBarGroups;
-        const uint32_t in_group = nblocks / kBarGroups + (g < nblocks % kBarGroups ? 1u : 0u);
-        uint32_t* gc = bar + kBarStride * (1u + g);
-        bool last = false;
-        if (__hip_atomic_fetch_add(gc, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == in_group - 1u) {
-            // the reset completes before the top arrival (release): no workgroup can pass this
-            // barrier, and reach the group counter again, while the reset is in flight
-            __hip_atomic_store(gc, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-            last = __hip_atomic_fetch_add(bar, 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT) == groups - 1u;
-        }
-        if (last) {
-            __hip_atomic_store(bar, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-            __hip_atomic_store(bar + 1, gen + 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+        if (gen & kBarAbort) {
+            ok = false;  // an earlier wait timed out: leave without arriving
         } else {
-            uint32_t it = 0;
-            while (__hip_atomic_load(bar + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == gen) {
-                if (++it > kSpinLimit) {
-                    atomicOr(status, 1u);
-                    break;
+            const uint32_t g = blockIdx.x % kBarGroups, groups = nblocks < kBarGroups ? nblocks : kBarGroups;
+            const uint32_t in_group = nblocks / kBarGroups + (g < nblocks % kBarGroups ? 1u : 0u);
+            uint32_t* gc = bar + kBarStride * (1u + g);
+            bool last = false;
+            if (__hip_atomic_fetch_add(gc, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == in_group - 1u) {
+                // the reset completes before the top arrival (release): no workgroup can pass this
+                // barrier, and reach the group counter again, while the reset is in flight
+                __hip_atomic_store(gc, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                last = __hip_atomic_fetch_add(bar, 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT) == groups - 1u;
+            }
+            if (last) {
+                __hip_atomic_store(bar, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                const uint32_t old = __hip_atomic_fetch_add(bar + 1, 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+                ok = (old & kBarAbort) == 0u;
+            } else {
+                uint32_t it = 0, cur;
+                while ((cur = __hip_atomic_load(bar + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) == gen) {
+                    if (++it > kSpinLimit) {
+                        atomicOr(status, 1u);
+                        __hip_atomic_fetch_or(bar + 1, kBarAbort, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                        cur = kBarAbort;
+                        break;
+                    }
+                    __builtin_amdgcn_s_sleep(1);
                 }
-                __builtin_amdgcn_s_sleep(1);
+                ok = (cur & kBarAbort) == 0u;
             }
         }
         __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+        pass = ok ? 1u : 0u;
     }
     __syncthreads();
-    return (__hip_atomic_load(status, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) & 1u) == 0u;
+    return pass != 0u;
+}
+
+// Clears the barrier words before a frame launch (a kernel, so that a captured frame graph
+// holds it as a kernel node).
+__global__ void k_bar_reset(uint32_t* bar) {
+    for (uint32_t k = threadIdx.x; k < kFrameBarWords; k += blockDim.x) bar[k] = 0u;
 }
 
 template <int BLOCK>
@@ -383,6 +407,7 @@ hipError_t launch_frame_t(const FrameParams& f, uint32_t max_blocks, hipStream_t
     uint32_t blocks = (uint32_t)per_cu * (uint32_t)cus;  // all resident at once
     if (max_blocks && blocks > max_blocks) blocks = max_blocks;
     if (blocks > kFrameMaxBlocks) blocks = kFrameMaxBlocks;  // the per-workgroup totals' scratch
+    hipLaunchKernelGGL(k_bar_reset, dim3(1), dim3(256), 0, s, f.bar);
     hipLaunchKernelGGL(kern, dim3(blocks), dim3(64 * WAVES), smem, s, f);
     return hipGetLastError();
 }

@@ -94,7 +94,7 @@ struct NetState {
     float* mlp_f32 = nullptr;        // = params + n_grid: MLP weights, row-major W_l[out][in]
     uint16_t* grid_lp = nullptr;     // FP16/BF16 shadow of the grid
     uint16_t* mlp_lp = nullptr;      // FP16/BF16 shadow of the MLP (MFMA tile order)
-    // training state, allocated on first use
+    // training state (allocated by nrrs_create when max_train > 0)
     bool train_ready = false;
     float* grad = nullptr;           // [grid | mlp]
     uint16_t* grad_lp = nullptr;     // FP16 grid gradient of FP16/BF16 contexts [n_grid]
@@ -128,7 +128,9 @@ struct NetState {
     void* w_bwd = nullptr;
     float* dw_partial = nullptr;     // [max_train_blocks][n_mlp]
     float* loss_scale = nullptr;     // device scalar
-    uint32_t* state = nullptr;       // device: step, good steps, skipped, non-finite flag
+    uint32_t* state = nullptr;       // device [8]: step, good steps, skipped, non-finite gradient flag,
+                                     // non-finite batch losses
+    double* loss_partial = nullptr;  // [max_train_blocks] FP64 batch-loss partials of the training kernel
 };
 
 }  // namespace nrrs

@@ -50,7 +50,7 @@ struct F16 {
     __device__ static vec8 pack(const float* x) {
         vec8 v;
 #pragma unroll
-        for (int j = 0; j < 8; ++j) v[j] = (_Float16)x[j];
+        for (int j = 0; j < 8; ++j) v[j] = (_Float16)f32_rounded(x[j]);
         return v;
     }
     __device__ static f32x16 mma(u32x4 a, vec8 b, f32x16 c) {
@@ -66,7 +66,7 @@ __device__ __forceinline__ uint16_t lp_bits<BF16>(float f) {
 }
 template <>
 __device__ __forceinline__ uint16_t lp_bits<F16>(float f) {
-    return __builtin_bit_cast(uint16_t, (_Float16)f);
+    return __builtin_bit_cast(uint16_t, (_Float16)f32_rounded(f));
 }
 
 constexpr int kTileHalves = 64 * 8;  // one packed A tile: 64 lanes x 8 values

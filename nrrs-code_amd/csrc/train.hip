@@ -63,7 +63,7 @@ struct TF16 {
     __device__ static op pack(const float* x) {
         op v;
 #pragma unroll
-        for (int j = 0; j < 8; ++j) v[j] = (_Float16)x[j];
+        for (int j = 0; j < 8; ++j) v[j] = (_Float16)f32_rounded(x[j]);
         return v;
     }
     __device__ static float get(const op& v, int j) { return (float)v[j]; }
@@ -209,7 +209,7 @@ __global__ __launch_bounds__(64 * P::kWaves) void k_train(TrainArgs t) {
     constexpr int kStageLd = Geo<P>::kStageLd;
     extern __shared__ __align__(16) unsigned char smem_raw[];
     Stage<P>& st = *reinterpret_cast<Stage<P>*>(smem_raw);
-    __shared__ float loss_part[kWaves];  // loss per wave, accumulated with LDS atomics
+    __shared__ double loss_part[kWaves];  // loss per wave (FP64, summed in a fixed order)
 
     const int lane = threadIdx.x & 63, h = lane >> 5, col = lane & 31, wave = threadIdx.x >> 6;
     // FP16/BF16: the packed forward and W^T tiles live in LDS behind the stage (they are read
@@ -230,7 +230,7 @@ __global__ __launch_bounds__(64 * P::kWaves) void k_train(TrainArgs t) {
     f32x16 acc[kSlots];
 #pragma unroll
     for (int s = 0; s < kSlots; ++s) acc[s] = f32x16{};
-    if (lane == 0) loss_part[wave] = 0.0f;  // by the wave that accumulates into it
+    if (lane == 0) loss_part[wave] = 0.0;  // by the wave that accumulates into it
 
     // zero the padded input rows once (layer 0 has 80 of 96 rows)
     for (int k = threadIdx.x; k < 16 * kStageLd; k += kThreads) st.h[80 * kStageLd + k] = (typename P::elem)0.0f;
@@ -306,6 +306,7 @@ __global__ __launch_bounds__(64 * P::kWaves) void k_train(TrainArgs t) {
         }
         // output c sits in lane half c/4, register c%4
         float dyr[4] = {0.0f, 0.0f, 0.0f, 0.0f};
+        float lterm = 0.0f;  // this lane's loss terms of the iteration
         if (live) {
             if (LOSS == NRRS_LOSS_EXTERNAL) {
 #pragma unroll
@@ -325,7 +326,7 @@ __global__ __launch_bounds__(64 * P::kWaves) void k_train(TrainArgs t) {
                         const float y = Y[r];
                         const float den = y * y + 0.01f;
                         const float d = y - tg;
-                        atomicAdd(&loss_part[wave], d * d / den * t.inv_norm);
+                        lterm += d * d / den * t.inv_norm;
                         dyr[r] = 2.0f * d / den * t.inv_norm;
                     }
                 }
@@ -358,7 +359,7 @@ __global__ __launch_bounds__(64 * P::kWaves) void k_train(TrainArgs t) {
                                         t.gamma[2] * q + t.gamma[3] * (q - qt) * (q - qt);
                         const float fp = -(t.gamma[0] + t.gamma[1]) * vv / (qe * qe) + t.gamma[0] * ebar +
                                          t.gamma[1] * e + t.gamma[2] + 2.0f * t.gamma[3] * (q - qt);
-                        atomicAdd(&loss_part[wave], w * f * t.inv_norm);
+                        lterm += w * f * t.inv_norm;
                         dyr[0] = w * fp * activation_sigma_grad(y) * t.inv_norm;
                     }
                 }
@@ -370,6 +371,14 @@ __global__ __launch_bounds__(64 * P::kWaves) void k_train(TrainArgs t) {
                     if (c < NOUT) t.y_out[(size_t)c * t.out_stride + i] = Y[r];
                 }
             }
+        }
+        if constexpr (LOSS != NRRS_LOSS_EXTERNAL) {
+            // FP64 butterfly over the wave (every lane ends with the same total), added by lane 0
+            // to the wave's running sum: the batch loss does not depend on scheduling
+            double ls = (double)lterm;
+#pragma unroll
+            for (int o = 32; o > 0; o >>= 1) ls += __shfl_xor(ls, o, 64);
+            if (lane == 0) loss_part[wave] += ls;
         }
         // dy operand (k = c = 8h + j): half 0 holds outputs 0..7, fetching 4..7 from half 1
         op DY;
@@ -573,10 +582,10 @@ __global__ __launch_bounds__(64 * P::kWaves) void k_train(TrainArgs t) {
         }
     }
     __syncthreads();
-    if (threadIdx.x == 0 && t.loss_out) {
-        float s = 0.0f;
+    if (LOSS != NRRS_LOSS_EXTERNAL && threadIdx.x == 0 && t.loss_partial) {
+        double s = 0.0;
         for (int w = 0; w < kWaves; ++w) s += loss_part[w];
-        atomicAdd(t.loss_out, s);
+        t.loss_partial[blockIdx.x] = s;
     }
 }
 
@@ -1055,7 +1064,28 @@ __global__ void k_reduce_dense(float* __restrict__ rep_f32, _Float16* __restrict
 // Sum over workgroups of the dW partials, in a fixed order: a block takes 64 parameters,
 // its 16 thread groups take every 16th partial row (8 loads in flight), then the groups
 // are summed in group order.
-constexpr int kRedParams = 64, kRedGroups = 16;
+constexpr int kRedParams = 64, kRedGroups = 16, kLossThreads = 256;
+
+// Batch loss from the per-workgroup FP64 partials of the training kernel, summed in a fixed
+// order; a NaN/Inf total counts one non-finite step in *nonfinite.
+__global__ __launch_bounds__(kLossThreads) void k_loss_finish(const double* __restrict__ part, uint32_t n_blocks,
+                                                              float* loss_out, uint32_t* nonfinite) {
+    __shared__ double red[kLossThreads];
+    double s = 0.0;
+    for (uint32_t b = threadIdx.x; b < n_blocks; b += kLossThreads) s += part[b];
+    red[threadIdx.x] = s;
+    __syncthreads();
+#pragma unroll
+    for (int o = kLossThreads / 2; o > 0; o >>= 1) {
+        if ((int)threadIdx.x < o) red[threadIdx.x] += red[threadIdx.x + o];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        const double v = red[0];
+        if (loss_out) *loss_out = (float)v;
+        if (nonfinite && !isfinite(v)) *nonfinite += 1u;
+    }
+}
 
 __global__ __launch_bounds__(kRedParams* kRedGroups) void k_reduce_partials(const float* __restrict__ part,
                                                                            uint32_t n_blocks, uint32_t n_mlp,
@@ -1164,6 +1194,12 @@ hipError_t launch_reduce_dense(float* rep_f32, void* rep_h, uint32_t n_dense, fl
     if (!n_dense) return hipSuccess;
     hipLaunchKernelGGL(k_reduce_dense, dim3(512), dim3(256), 0, s, rep_f32, static_cast<_Float16*>(rep_h), n_dense,
                        grad, static_cast<_Float16*>(grad_h));
+    return hipGetLastError();
+}
+
+hipError_t launch_loss_finish(const double* part, uint32_t n_blocks, float* loss_out, uint32_t* nonfinite,
+                              hipStream_t s) {
+    hipLaunchKernelGGL(k_loss_finish, dim3(1), dim3(kLossThreads), 0, s, part, n_blocks, loss_out, nonfinite);
     return hipGetLastError();
 }
 

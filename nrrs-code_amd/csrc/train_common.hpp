@@ -89,7 +89,7 @@ struct TrainArgs {
     uint32_t gdl_stride;
     float* dw_partial;             // [gridDim.x][n_mlp]
     uint32_t n_mlp;
-    float* loss_out;               // device scalar (atomic add), or null
+    double* loss_partial;          // [gridDim.x] FP64 batch-loss partials (LL2 / RRS losses), or null
 };
 
 int train_tiles_fwd(int nl);
@@ -138,6 +138,10 @@ uint32_t train_vertices_per_iter(int precision);
 hipError_t launch_reduce_dense(float* rep_f32, void* rep_h, uint32_t n_dense, float* grad, void* grad_h,
                                hipStream_t s);
 hipError_t launch_reduce_partials(const float* part, uint32_t n_blocks, uint32_t n_mlp, float* grad, hipStream_t s);
+// Batch loss = FP64 sum of the training kernel's per-workgroup partials (fixed order), written
+// to loss_out (or not, when null); a non-finite sum increments *nonfinite.
+hipError_t launch_loss_finish(const double* part, uint32_t n_blocks, float* loss_out, uint32_t* nonfinite,
+                              hipStream_t s);
 
 struct AdamArgs {
     float* params;        // [n] FP32 masters
@@ -151,7 +155,8 @@ struct AdamArgs {
     int lp_precision;
     float lr, beta1, beta2, eps, l2;
     float* loss_scale;    // device
-    uint32_t* state;      // device: [0] step, [1] good steps, [2] skipped, [3] non-finite flag
+    uint32_t* state;      // device: [0] step, [1] good steps, [2] skipped, [3] non-finite gradient flag,
+                          // [4] non-finite batch losses (read and cleared by nrrs_train_state)
     int dynamic_scale;
     uint32_t growth_interval;
 };

@@ -207,3 +207,52 @@ def test_infer_decide_pipeline(torch_cuda, ora, mode, prec):
     flips = int(np.sum(ref64["s"] != ref["s"]))
     print(f"mode {mode}: {flips} of {n} split counts differ under the FP64 q")
     assert flips <= n * Q_TOL[prec] + 2
+
+
+@pytest.mark.parametrize("prec", [na.FP32, na.BF16])
+@pytest.mark.parametrize("knobs", [
+    dict(rrs_weight_window=1),                    # RRSWeightWindow (commoncudautils.h:14-28)
+    dict(rrs_use_ceil=0),                         # normaliseFactor * rate (integrator.cpp:1376-1377)
+    dict(rrs_no_upscale=0),                       # q' = q * nf also when nf > 1 (integrator.cpp:1391-1397)
+    dict(rrs_weight_window=1, rrs_use_ceil=0, rrs_no_upscale=0, rrs_normalize_rate=0.7),
+])
+def test_decision_branches(torch_cuda, ora, prec, knobs):
+    """The knobs of the RRS decision that the default configuration leaves off, through the full
+    stage: q against FP64, then the decision bit-exact against the oracle on the device's own q.
+    The film has four pixels per vertex, so the normalisation factor exceeds 1 and the
+    no-upscale switch decides whether q' grows."""
+    torch = torch_cuda
+    n, P = N_SMALL, 4 * N_SMALL
+    ctx = na.Context(na.default_config(precision=prec, n_pixels=P, max_queue=n, **knobs))
+    ps = random_params(ctx, na.STATNET, 51)
+    pr = random_params(ctx, na.RRSNET, 52)
+    vb = na.synth_vertices(n, 78, P)
+    film_np, film = make_film(torch, P, 9)
+    b = _decide_buffers(torch, n, 8 * n)
+    args = ctx.decide_args(vb, film, b["q"], b["survivors"], b["n_surv"], b["overflow"], q_mode=na.Q_RRSNET_CHAINED,
+                           seed=6, stream_id=2, qn=b["qn"], split=b["split"])
+    assert ctx.rrs_infer_decide(args) == na.OK
+    torch.cuda.synchronize()
+    q = b["q"][:n].cpu().numpy()
+    hv = ora.HostVertices(vb.to_numpy())
+    stats = ora.net_forward(ctx.cfg, na.STATNET, hv, film_np, None, ps.astype(np.float64)).T
+    raw = ora.net_forward(ctx.cfg, na.RRSNET, hv, film_np, stats, pr.astype(np.float64))[:, 0]
+    q64 = ora.q_f64(ctx.cfg, na.Q_RRSNET_CHAINED, hv, raw, film_np)
+    qerr = np.abs(q - q64) / q64
+    ref = ora.rrs_decide(ctx.cfg, q, seed=6, stream=2)
+    rate = 1.0 if ctx.cfg.rrs_use_ceil else ctx.cfg.rrs_normalize_rate
+    nf = P / ref["sum"] * rate
+    print(f"knobs {knobs} prec {prec}: q rel err max {qerr.max():.2e}; nf {nf:.3f}; "
+          f"{ref['n_survivors']} survivors; q' max {ref['qn'].max():.3f}")
+    assert qerr.max() < Q_TOL[prec]
+    if ctx.cfg.rrs_weight_window:
+        # inside the window [2 / (1 + size), 2 size / (1 + size)] q becomes exactly 1
+        assert np.any(q == 1.0)
+    assert nf > 1.0
+    if not ctx.cfg.rrs_no_upscale:
+        assert np.any(ref["qn"] > q)  # upscaled
+    ns = int(b["n_surv"].item())
+    assert ns == ref["n_survivors"] and int(b["overflow"].item()) == 0
+    assert np.array_equal(b["split"][:n].cpu().numpy(), ref["s"])
+    assert np.array_equal(b["qn"][:n].cpu().numpy(), ref["qn"])
+    assert np.array_equal(b["survivors"][:ns].cpu().numpy().astype(np.uint32), ref["survivors"])
