@@ -205,6 +205,17 @@ nrrs_status nrrs_rrs_infer_decide(nrrs_ctx* ctx, const nrrs_decide_args* a, void
 /* Decision only, from a given clamped q (a->q_dev is an input here). */
 nrrs_status nrrs_rrs_decide(nrrs_ctx* ctx, const nrrs_decide_args* a, void* stream);
 
+/* Allocation-free form of nrrs_rrs_decide. No counterpart in the reference: it is for callers
+ * that cannot allocate device memory (code under HIP graph capture, tight serving loops,
+ * harnesses that forbid hipMalloc) and may hold no context. The decision knobs and n_pixels come
+ * from `cfg`; a->q_dev is an input, as for nrrs_rrs_decide, and the results are the same. Its
+ * scratch (exact sum, tile totals, split counts) is a fixed array in the library's code object,
+ * one per device, holding up to NRRS_STATIC_DECIDE_MAX vertices: a->n above that returns
+ * NRRS_ERR_SHAPE. Every call on a device shares that device's scratch, so calls that overlap
+ * on one device (several streams or host threads) race; serialise them. */
+#define NRRS_STATIC_DECIDE_MAX (1u << 22)
+nrrs_status nrrs_rrs_decide_static(const nrrs_config* cfg, const nrrs_decide_args* a, void* stream);
+
 /* Synthetic vertices of the benchmark workloads, generated on the device from PCG32
  * (seed, field stream). out: buffers of a nrrs_vertices with stride >= n (written). */
 typedef struct {

@@ -106,6 +106,21 @@ Frame make_frame(const float* mn, const float* mx) {
     return f;
 }
 
+// The RRS-decision knobs of a configuration (integrator.h:265, render/test.json:63-67).
+DecideParams decide_params(const nrrs_config& cfg) {
+    DecideParams d;
+    d.clamp_min = cfg.rrs_clamp_min;
+    d.clamp_max = cfg.rrs_clamp_max;
+    d.rate = cfg.rrs_normalize_rate;
+    d.window_size = cfg.weight_window_size;
+    d.use_ceil = cfg.rrs_use_ceil;
+    d.no_upscale = cfg.rrs_no_upscale;
+    d.weight_window = cfg.rrs_weight_window;
+    d.split_cap = cfg.split_cap;
+    d.n_pixels = cfg.n_pixels;
+    return d;
+}
+
 uint64_t mlp_count(const nrrs_mlp_config& m) {
     return (uint64_t)kHidden * kEnc + (uint64_t)(m.n_hidden_layers - 1) * kHidden * kHidden +
            (uint64_t)m.n_outputs * kHidden;
@@ -770,15 +785,7 @@ nrrs_status nrrs_create(const nrrs_config* cfg, nrrs_ctx** out) {
         return NRRS_ERR_ARG;
     }
     c->frame = make_frame(cfg->aabb_min, cfg->aabb_max);
-    c->dec.clamp_min = cfg->rrs_clamp_min;
-    c->dec.clamp_max = cfg->rrs_clamp_max;
-    c->dec.rate = cfg->rrs_normalize_rate;
-    c->dec.window_size = cfg->weight_window_size;
-    c->dec.use_ceil = cfg->rrs_use_ceil;
-    c->dec.no_upscale = cfg->rrs_no_upscale;
-    c->dec.weight_window = cfg->rrs_weight_window;
-    c->dec.split_cap = cfg->split_cap;
-    c->dec.n_pixels = cfg->n_pixels;
+    c->dec = decide_params(*cfg);
 
     nrrs_status st = NRRS_OK;
     // paired table: one 32-bit word per entry and net, n_grid / 2 entries
@@ -968,6 +975,16 @@ nrrs_status nrrs_rrs_decide(nrrs_ctx* ctx, const nrrs_decide_args* a, void* stre
     hipStream_t s = (hipStream_t)stream;
     NRRS_HIP(launch_qsum(ctx, a->q_dev, a->n, s));
     return hip_status(launch_decide(ctx, *a, s));
+}
+
+nrrs_status nrrs_rrs_decide_static(const nrrs_config* cfg, const nrrs_decide_args* a, void* stream) {
+    if (!cfg || !a || !a->q_dev || !a->survivors_dev || !a->n_survivors_dev) return NRRS_ERR_ARG;
+    if (a->n_spec_dev && !a->spec_stid_dev) return NRRS_ERR_ARG;
+    if (cfg->n_pixels == 0 || cfg->split_cap == 0 || cfg->split_cap > 255) return NRRS_ERR_ARG;
+    static_assert(NRRS_STATIC_DECIDE_MAX == kStaticDecideMax, "scratch size of the allocation-free decision");
+    if (a->n > NRRS_STATIC_DECIDE_MAX) return NRRS_ERR_SHAPE;
+    const DecideParams d = decide_params(*cfg);
+    return hip_status(launch_decide_static(d, *a, (hipStream_t)stream));
 }
 
 nrrs_status nrrs_synth_vertices(uint32_t n, uint64_t seed, uint32_t n_pixels, const nrrs_vertices_out* o,

@@ -6,6 +6,8 @@
 // specular stids first (in the order given), then for each inference index i in
 // ascending order, s_i copies of stid_i. It is built by a tile reduce, a scan of
 // the tile totals and a scatter that re-scans each tile with wavefront prefix sums.
+#include <mutex>
+
 #include "device_common.hpp"
 
 namespace nrrs {
@@ -154,10 +156,93 @@ hipError_t launch_consume(uint32_t capacity, const uint32_t* n_dev, const uint32
     return hipGetLastError();
 }
 
-hipError_t launch_qsum(nrrs_ctx* ctx, const float* q, uint32_t n, hipStream_t s) {
-    hipLaunchKernelGGL(k_zero_u64, dim3(1), dim3(1), 0, s, ctx->sum_dev);
-    if (n) hipLaunchKernelGGL(k_qsum, dim3((n + kTile - 1) / kTile), dim3(kTile), 0, s, q, n, ctx->dec, ctx->sum_dev);
+namespace {
+
+// Scratch of one decision: the exact sum, the per-tile survivor totals and first slots, and
+// the split counts when the caller keeps none.
+struct DecideScratch {
+    unsigned long long* sum;
+    uint32_t* tile_sums;
+    uint32_t* tile_offsets;
+    uint8_t* s;
+    uint32_t max_tiles;
+};
+
+hipError_t qsum_into(const DecideParams& d, const float* q, uint32_t n, unsigned long long* sum, hipStream_t s) {
+    hipLaunchKernelGGL(k_zero_u64, dim3(1), dim3(1), 0, s, sum);
+    if (n) hipLaunchKernelGGL(k_qsum, dim3((n + kTile - 1) / kTile), dim3(kTile), 0, s, q, n, d, sum);
     return hipGetLastError();
+}
+
+hipError_t decide_with(const DecideParams& d, const DecideScratch& sc, const nrrs_decide_args& a, hipStream_t s) {
+    const uint32_t tiles = (a.n + kTile - 1) / kTile;
+    if (tiles > sc.max_tiles) return hipErrorInvalidValue;
+    uint8_t* split = a.split_dev ? a.split_dev : sc.s;
+    if (tiles) {
+        SplitArgs sa;
+        sa.q = a.q_dev;
+        sa.n = a.n;
+        sa.sum = sc.sum;
+        sa.n_spec = a.n_spec_dev;
+        sa.seed = a.pcg_seed;
+        sa.stream = a.pcg_stream;
+        sa.qn = a.qn_dev;
+        sa.qn_stid = a.qn_stid_dev;
+        sa.stid = a.v.stid;
+        sa.s = split;
+        sa.tile_sums = sc.tile_sums;
+        sa.d = d;
+        hipLaunchKernelGGL(k_split, dim3(tiles), dim3(kTile), 0, s, sa);
+    }
+    hipLaunchKernelGGL(k_scan_tiles, dim3(1), dim3(kScanThreads), 0, s, sc.tile_sums, tiles, a.n_spec_dev, a.capacity,
+                       sc.tile_offsets, a.n_survivors_dev, a.overflow_dev);
+    if (a.n_spec_dev && a.spec_stid_dev)
+        hipLaunchKernelGGL(k_spec_copy, dim3(64), dim3(256), 0, s, a.spec_stid_dev, a.n_spec_dev, a.capacity,
+                           a.survivors_dev);
+    if (tiles)
+        hipLaunchKernelGGL(k_scatter, dim3(tiles), dim3(kTile), 0, s, split, a.n, a.v.stid, sc.tile_offsets,
+                           a.capacity, a.survivors_dev);
+    return hipGetLastError();
+}
+
+DecideScratch ctx_scratch(nrrs_ctx* ctx) {
+    return DecideScratch{ctx->sum_dev, ctx->tile_sums, ctx->tile_offsets, ctx->s_scratch, ctx->max_tiles};
+}
+
+// Scratch of the allocation-free decision (nrrs_rrs_decide_static): one array per device, in
+// the code object, [0, 256) the sum, then the tile totals, the tile first slots and the split
+// counts of kStaticDecideMax vertices (about 4.3 MB). Host code uses the address that
+// hipGetSymbolAddress returns on each device, never the array's name.
+constexpr uint32_t kStaticTiles = kStaticDecideMax / kTile;
+constexpr size_t kStaticTileOff = 256, kStaticOffOff = kStaticTileOff + 4 * (size_t)kStaticTiles,
+                 kStaticSplitOff = kStaticOffOff + 4 * (size_t)kStaticTiles,
+                 kStaticScratchBytes = kStaticSplitOff + kStaticDecideMax;
+static __device__ __align__(256) uint8_t scratch[kStaticScratchBytes];
+
+std::once_flag g_scratch_once[kMaxDevices];
+uint8_t* g_scratch_ptr[kMaxDevices];
+hipError_t g_scratch_err[kMaxDevices];
+
+// The device address of `scratch` on the current device (looked up once per device).
+hipError_t static_scratch(uint8_t** out) {
+    int dev = 0;
+    hipError_t e = hipGetDevice(&dev);
+    if (e != hipSuccess) return e;
+    if (dev < 0 || dev >= kMaxDevices) return hipErrorInvalidDevice;
+    std::call_once(g_scratch_once[dev], [dev] {
+        void* p = nullptr;
+        g_scratch_err[dev] = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        g_scratch_ptr[dev] = g_scratch_err[dev] == hipSuccess ? static_cast<uint8_t*>(p) : nullptr;
+    });
+    if (g_scratch_err[dev] != hipSuccess) return g_scratch_err[dev];
+    *out = g_scratch_ptr[dev];
+    return hipSuccess;
+}
+
+}  // namespace
+
+hipError_t launch_qsum(nrrs_ctx* ctx, const float* q, uint32_t n, hipStream_t s) {
+    return qsum_into(ctx->dec, q, n, ctx->sum_dev, s);
 }
 
 hipError_t launch_zero_u64(unsigned long long* p, hipStream_t s) {
@@ -166,34 +251,22 @@ hipError_t launch_zero_u64(unsigned long long* p, hipStream_t s) {
 }
 
 hipError_t launch_decide(nrrs_ctx* ctx, const nrrs_decide_args& a, hipStream_t s) {
-    const uint32_t tiles = (a.n + kTile - 1) / kTile;
-    if (tiles > ctx->max_tiles) return hipErrorInvalidValue;
-    if (tiles) {
-        SplitArgs sa;
-        sa.q = a.q_dev;
-        sa.n = a.n;
-        sa.sum = ctx->sum_dev;
-        sa.n_spec = a.n_spec_dev;
-        sa.seed = a.pcg_seed;
-        sa.stream = a.pcg_stream;
-        sa.qn = a.qn_dev;
-        sa.qn_stid = a.qn_stid_dev;
-        sa.stid = a.v.stid;
-        sa.s = a.split_dev ? a.split_dev : ctx->s_scratch;
-        sa.tile_sums = ctx->tile_sums;
-        sa.d = ctx->dec;
-        hipLaunchKernelGGL(k_split, dim3(tiles), dim3(kTile), 0, s, sa);
-    }
-    hipLaunchKernelGGL(k_scan_tiles, dim3(1), dim3(kScanThreads), 0, s, ctx->tile_sums, tiles,
-                       a.n_spec_dev, a.capacity, ctx->tile_offsets, a.n_survivors_dev, a.overflow_dev);
-    if (a.n_spec_dev && a.spec_stid_dev)
-        hipLaunchKernelGGL(k_spec_copy, dim3(64), dim3(256), 0, s, a.spec_stid_dev, a.n_spec_dev,
-                           a.capacity, a.survivors_dev);
-    if (tiles)
-        hipLaunchKernelGGL(k_scatter, dim3(tiles), dim3(kTile), 0, s,
-                           a.split_dev ? a.split_dev : ctx->s_scratch, a.n, a.v.stid, ctx->tile_offsets,
-                           a.capacity, a.survivors_dev);
-    return hipGetLastError();
+    return decide_with(ctx->dec, ctx_scratch(ctx), a, s);
+}
+
+hipError_t launch_decide_static(const DecideParams& d, const nrrs_decide_args& a, hipStream_t s) {
+    if (a.n > kStaticDecideMax) return hipErrorInvalidValue;
+    uint8_t* base = nullptr;
+    hipError_t e = static_scratch(&base);
+    if (e != hipSuccess) return e;
+    DecideScratch sc;
+    sc.sum = reinterpret_cast<unsigned long long*>(base);
+    sc.tile_sums = reinterpret_cast<uint32_t*>(base + kStaticTileOff);
+    sc.tile_offsets = reinterpret_cast<uint32_t*>(base + kStaticOffOff);
+    sc.s = base + kStaticSplitOff;
+    sc.max_tiles = kStaticTiles;
+    if ((e = qsum_into(d, a.q_dev, a.n, sc.sum, s)) != hipSuccess) return e;
+    return decide_with(d, sc, a, s);
 }
 
 }  // namespace nrrs

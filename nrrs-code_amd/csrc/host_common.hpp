@@ -240,6 +240,11 @@ hipError_t launch_error_compute(uint32_t n, const float* ref, const float* wL, c
                                 double* part, float* sum, hipStream_t s);
 hipError_t launch_qsum(nrrs_ctx* ctx, const float* q, uint32_t n, hipStream_t s);
 hipError_t launch_decide(nrrs_ctx* ctx, const nrrs_decide_args& a, hipStream_t s);
+// Vertices nrrs_rrs_decide_static takes at most (its scratch is a fixed device array).
+constexpr uint32_t kStaticDecideMax = 1u << 22;
+// Normalisation sum + decision + compaction with the fixed device scratch instead of a
+// context's (allocates nothing; calls that overlap on one device race on that scratch).
+hipError_t launch_decide_static(const DecideParams& d, const nrrs_decide_args& a, hipStream_t s);
 hipError_t launch_consume(uint32_t capacity, const uint32_t* n_dev, const uint32_t* surv, const nrrs_vertices& sc,
                           const float* rrs, const int32_t* node_in, const nrrs_vertices_out& o, int32_t* node_out,
                           uint32_t* stid_out, hipStream_t s);

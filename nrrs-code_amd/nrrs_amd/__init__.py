@@ -208,6 +208,7 @@ _SIGS = {
     "nrrs_rrsnet_infer": (C.c_int, [_P, C.c_uint32, C.POINTER(Vertices), _P, _P, _P, _P]),
     "nrrs_rrs_infer_decide": (C.c_int, [_P, C.POINTER(DecideArgs), _P]),
     "nrrs_rrs_decide": (C.c_int, [_P, C.POINTER(DecideArgs), _P]),
+    "nrrs_rrs_decide_static": (C.c_int, [C.POINTER(Config), C.POINTER(DecideArgs), _P]),
     "nrrs_synth_vertices": (C.c_int, [C.c_uint32, C.c_uint64, C.c_uint32, C.POINTER(VerticesOut), _P]),
     "nrrs_train_step": (C.c_int, [_P, C.c_uint32, C.c_uint32, C.POINTER(Vertices), _P, _P, C.POINTER(RrsSide),
                                   _P, _P]),
@@ -650,3 +651,13 @@ class Context:
         v = vb.c_struct()
         return self.lib.nrrs_train_step(self.h, net_mask, vb.n, C.byref(v), _ptr(film), _ptr(target_L),
                                         None if side is None else C.byref(side), _ptr(loss), _stream(stream))
+
+
+
+STATIC_DECIDE_MAX = 1 << 22
+
+
+def rrs_decide_static(cfg, args, stream=None):
+    """nrrs_rrs_decide_static: the decision with the library's fixed device scratch (no context,
+    no allocation)."""
+    return load().nrrs_rrs_decide_static(C.byref(cfg), C.byref(args), _stream(stream))
