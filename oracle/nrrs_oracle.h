@@ -21,8 +21,13 @@
  *  - MLP forward/backward: finite differences on FP64.
  *  - Losses: the NRRS_LL2 / NRRS_RRS loss sources are not part of the reference
  *    checkout (tcnn submodule absent): these are the build's documented
- *    restatement, pinned only against their own finite differences
- *    ("parity unpinned" w.r.t. the reference loss code).
+ *    restatement ("parity unpinned" w.r.t. the reference loss code), pinned to
+ *    that documentation by finite differences, closed forms (LL2: zero at the
+ *    target, unit offsets) and, for NRRS_RRS, hand-evaluated single-sample values
+ *    of each gamma term and the minimisers of the efficiency terms (which fix the
+ *    thp^2 weights, the channel mean, ref^2, the clamp and the error floors);
+ *    planted mistakes in a copy of this source fail those checks
+ *    (tests/test_oracle_rrs_loss.py).
  *  - Adam: hand-computed 3-step trajectory.
  *  - Path nodes and training rows: a hand-built path tree with known sums
  *    (radiance reaches every ancestor once), division by throughput, the

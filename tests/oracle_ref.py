@@ -218,8 +218,17 @@ def _side_ptrs(side):
     return _p(side.ref_mean), _p(side.error), _p(side.sample_weight), _p(side.q_target), side.error_sum
 
 
-def loss(cfg, kind, hv, y, film=None, target_L=None, stats=None, side=None, y_den=None):
-    """y: [n_out][n] (FP64). Returns (loss, dy)."""
+def bind_loss(lib):
+    """Declares ora_loss on a loaded oracle library (the oracle itself, or a test variant)."""
+    name, (res, args) = "ora_loss", _SIGS_TRAIN["ora_loss"]
+    fn = getattr(lib, name)
+    fn.restype = res
+    fn.argtypes = args
+    return lib
+
+
+def loss(cfg, kind, hv, y, film=None, target_L=None, stats=None, side=None, y_den=None, lib=None):
+    """y: [n_out][n] (FP64). Returns (loss, dy). `lib`: another build of the oracle (bind_loss)."""
     y = np.ascontiguousarray(y, np.float64)
     dy = np.zeros_like(y)
     rm, er, sw, qt, es = _side_ptrs(side)
@@ -227,8 +236,9 @@ def loss(cfg, kind, hv, y, film=None, target_L=None, stats=None, side=None, y_de
     target_L = None if target_L is None else np.ascontiguousarray(target_L, np.float32)
     stats = None if stats is None else np.ascontiguousarray(stats, np.float64)
     y_den = None if y_den is None else np.ascontiguousarray(y_den, np.float64)
-    val = _load_train().ora_loss(C.byref(cfg), kind, hv.n, y.ctypes.data, _p(target_L), C.byref(hv.v), _p(film),
-                                 _p(stats), rm, er, sw, qt, es, _p(y_den), dy.ctypes.data)
+    lib = lib if lib is not None else _load_train()
+    val = lib.ora_loss(C.byref(cfg), kind, hv.n, y.ctypes.data, _p(target_L), C.byref(hv.v), _p(film), _p(stats), rm,
+                       er, sw, qt, es, _p(y_den), dy.ctypes.data)
     return val, dy
 
 
