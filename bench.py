@@ -13,9 +13,14 @@ T = 2^19, SH-4 direction/normal encodings, 64-wide StatNet 4x64->6 and RRSNet
 accumulation, FP32 master weights, dynamic loss scaling); the per-phase rates
 and the same step in FP32 and FP16 are reported beside it.
 
-The roofline entry covers the dominant kernel of the BF16 step, timed with HIP
-events on the stream it runs on. The CPU baseline is the FP64 oracle, single
-threaded, on a bounded sample of the same step.
+`ops` times every operation of the BF16 step (the library's event pairs around each
+launcher, nrrs_set_op_timing, over further timed steps) and rates each against its
+roofline; `roofline` is the entry of the step's dominant kernel, the paired
+level-major grid-encoding pass (k_encode_levels<BF16, true>, two launches per step).
+`matched_error` compares each precision's RRS stage (StatNet statistics, q) and both
+nets' gradients with the FP64 oracle on the first 2^16 vertices of the batch.
+`cpu_baseline` is the FP32 C++ port of the same step on the host's threads, with the
+single-threaded FP64 oracle beside it.
 """
 import argparse
 import json
@@ -31,12 +36,38 @@ N_PIXELS = 1 << 20
 SEED = 0x4E525253 ^ 2
 HBM_PEAK_GBS = 8000.0
 BF16_PEAK_TFLOPS = 2500.0
-# HBM bytes of one StatNet training backward at 2^20 samples (k_encode_levels + k_train +
-# k_bin_scan + k_bin_levels + k_apply_bins + k_apply_chunks + k_apply_finish +
-# k_reduce_partials), from rocprofv3 --pmc FETCH_SIZE and WRITE_SIZE
-# (profiles/train_backward_bf16_2p20_pmc.md), or None when not measured.
-TRAIN_TRAFFIC_BYTES = 2709207040
+# HBM bytes per launch of the paired encode pass at 2^20 vertices, from rocprofv3 --pmc
+# FETCH_SIZE and WRITE_SIZE (profiles/encode_pair_bf16_2p20_pmc.md), or None when not measured.
+ENCODE_TRAFFIC_BYTES = None
 MACS = {"statnet": 80 * 64 + 3 * 64 * 64 + 64 * 6, "rrsnet": 80 * 64 + 2 * 64 * 64 + 64 * 1}
+# Algorithmic work of one launch of each operation of the step (bytes it must move, or MFMA
+# flops), for a batch of n vertices and nets of n_grid + n_mlp parameters:
+#  - encode_pair(_counts): 12 B position + 16 levels x 8 corners x 8 B (both nets' BF16 feature
+#    pairs) + 2 x 64 B level-major encodings written = 1164 B per vertex;
+#  - mlp_chained: StatNet and RRSNet forward, 2 x (17 792 + 13 376) flops per query;
+#  - train_statnet / train_rrsnet: forward, dX and dW, 3 x 2 x MACs per sample;
+#  - bin_records_pair: per vertex and level 12 B position + 2 x 8 B dL/denc read and 4.5 records
+#    x (2 B key + 2 x 8 B values) written = 109 B;
+#  - apply_statnet / apply_rrsnet: per vertex and level 4.5 records x (2 B key + 8 B value)
+#    read = 45 B, and the FP16 table gradient read and written (4 B per grid parameter);
+#  - adam_statnet / adam_rrsnet: 28 B per parameter (gradient, p, m, v read; p, m, v written).
+def op_work(name, n, n_grid, n_mlp):
+    levels = 16
+    if name in ("encode_pair", "encode_pair_counts"):
+        return "bytes", 1164 * n
+    if name == "mlp_chained":
+        return "flops", 2 * (MACS["statnet"] + MACS["rrsnet"]) * n
+    if name in ("train_statnet", "train_rrsnet"):
+        return "flops", 6 * MACS["statnet" if name == "train_statnet" else "rrsnet"] * n
+    if name == "bin_records_pair":
+        return "bytes", 109 * levels * n
+    if name in ("apply_statnet", "apply_rrsnet"):
+        return "bytes", 45 * levels * n + 4 * n_grid
+    if name == "adam_statnet":
+        return "bytes", 28 * (n_grid + n_mlp["statnet"])
+    if name == "adam_rrsnet":
+        return "bytes", 28 * (n_grid + n_mlp["rrsnet"])
+    return None, 0
 
 
 def parse():
@@ -49,10 +80,11 @@ def parse():
     ap.add_argument("--precisions", default="bf16,fp16,fp32")
     ap.add_argument("--no-frame", action="store_true", help="skip the multi-bounce wavefront and online loop")
     ap.add_argument("--no-peaks", action="store_true", help="skip the copy and GEMM measurements of the GPU")
+    ap.add_argument("--no-oracle", action="store_true", help="skip the FP64-oracle error report")
     ap.add_argument("--pmc-probe", action="store_true",
-                    help="only run a 256 MiB device copy and, after 8 warm-up launches, 3 BF16 StatNet "
-                         "training backwards at 2^20 (for rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE runs; "
-                         "the profile summaries use the last 3 dispatches of each kernel)")
+                    help="only run a 256 MiB device copy and, after 8 warm-up steps, 3 BF16 steps at 2^20 "
+                         "(for rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE runs; the profile summaries use the "
+                         "dispatches of the last 3 steps)")
     return ap.parse_args()
 
 
@@ -234,41 +266,86 @@ class Path:
                 "step_queries_per_s": self.n * steps / wall, "loss": self.loss.cpu().tolist(),
                 "survivors": int(self.n_surv.item())}
 
-    def backward_statnet(self):
-        """One fused StatNet forward/loss/backward (k_train) at the context's precision."""
-        self.na.check(self.ctx.backward(self.na.STATNET, self.vb, self.na.LOSS_LL2, target_L=self.target_L,
-                                        loss_scale=4096.0), "backward")
+    def op_table(self, steps):
+        """Mean duration (ms) per step of every operation of the step, from the library's event
+        pairs around each launcher (nrrs_set_op_timing) over `steps` further steps."""
+        self.ctx.set_op_timing(True)
+        self.ctx.op_times()  # empty collection
+        acc, launches = {}, {}
+        for k in range(steps):
+            self.infer(100 + k)
+            self.train()
+            for name, ms in self.ctx.op_times():
+                acc[name] = acc.get(name, 0.0) + ms
+                launches[name] = launches.get(name, 0) + 1
+        self.ctx.set_op_timing(False)
+        return {k: {"ms_per_step": v / steps, "launches_per_step": launches[k] / steps} for k, v in acc.items()}
 
-    def time_train_kernel(self, reps):
-        """Average duration of nrrs_backward(StatNet, NRRS_LL2): the fused training kernel
-        plus a one-element fill and the per-workgroup dW reduction, HIP events on the stream
-        they are launched on."""
-        torch = self.torch
-        stream = torch.cuda.current_stream()
-        self.backward_statnet()
-        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
-        e0.record(stream)
-        for _ in range(reps):
-            self.backward_statnet()
-        e1.record(stream)
-        torch.cuda.synchronize()
-        self.ctx.grads_zero(self.na.STATNET)
-        return e0.elapsed_time(e1) / 1e3 / reps
 
-    def time_statnet_kernel(self, reps):
-        """Average duration of one StatNet inference call (the level-major encode pass, then
-        the MFMA MLP kernel), HIP events on the stream they are launched on."""
-        torch = self.torch
-        stream = torch.cuda.current_stream()
-        out = self.ctx.statnet_infer(self.vb)
-        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
-        e0.record(stream)
-        for _ in range(reps):
-            self.ctx.statnet_infer(self.vb)
-        e1.record(stream)
-        torch.cuda.synchronize()
-        del out
-        return e0.elapsed_time(e1) / 1e3 / reps
+def matched_errors(torch, na, np, precisions, m=1 << 16):
+    """Each precision against the FP64 oracle on the first m vertices of the benchmark batch,
+    with the parameters the contexts start from (the same in every precision): StatNet
+    statistics (nrrs_statnet_infer) and q of the chained RRS stage (relative to the statistic
+    range and to q), and both nets' gradients (nrrs_backward with NRRS_LL2, and NRRS_RRS fed with
+    the oracle's statistics: norm-wise for the table, largest MLP entry error relative to the
+    largest MLP gradient). The gradients are taken at a loss scale that keeps the FP16 table
+    gradient finite and divided by it."""
+    import copy
+
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    import oracle_ref as ora
+
+    ora.load()
+    out, ref = {}, None
+    for name, prec in precisions.items():
+        path = Path(prec, torch, na, np)
+        ctx = path.ctx
+        sub = copy.copy(path.vb)
+        sub.n = m
+        if ref is None:  # the oracle on this batch and these parameters (FP32 masters in FP64)
+            hv = ora.HostVertices(sub.to_numpy())
+            film = path.film.cpu().numpy()
+            tL = path.target_L[:, :m].cpu().numpy()
+            side = ora.RrsSide(None, path.err[:m].cpu().numpy(), path.sw[:m].cpu().numpy(),
+                               path.qt[:m].cpu().numpy(), 0.7 * N_PIXELS)
+            ps, pr = (ctx.params_get(net).astype(np.float64) for net in (na.STATNET, na.RRSNET))
+            s64 = np.ascontiguousarray(ora.net_forward(ctx.cfg, na.STATNET, hv, film, None, ps).T)
+            raw64 = ora.net_forward(ctx.cfg, na.RRSNET, hv, film, s64, pr)[:, 0]
+            q64 = ora.q_f64(ctx.cfg, na.Q_RRSNET_CHAINED, hv, raw64, film)
+            _, g0, _ = ora.train_grads(ctx.cfg, na.STATNET, hv, ps, na.LOSS_LL2, film=film, target_L=tL)
+            _, g1, _ = ora.train_grads(ctx.cfg, na.RRSNET, hv, pr, na.LOSS_RRS, film=film, stats=s64, side=side)
+            ref = dict(s64=s64, q64=q64, g=(g0, g1))
+        q = torch.empty(m, device="cuda")
+        surv = torch.empty(8 * m, dtype=torch.int32, device="cuda")
+        ns = torch.zeros(1, dtype=torch.int32, device="cuda")
+        args = ctx.decide_args(sub, path.film, q, surv, ns, q_mode=na.Q_RRSNET_CHAINED)
+        na.check(ctx.rrs_infer_decide(args), "rrs_infer_decide")
+        stats = ctx.statnet_infer(sub).cpu().numpy().astype(np.float64)
+        qd = q.cpu().numpy().astype(np.float64)
+        rq = np.abs(qd - ref["q64"]) / ref["q64"]
+        r = {"vertices": m,
+             "stats_err_max_rel_to_range": float(np.max(np.abs(stats - ref["s64"])) / np.max(np.abs(ref["s64"]))),
+             "q_rel_err_max": float(rq.max()), "q_rel_err_mean": float(rq.mean()),
+             "q_rel_err_p99_99": float(np.percentile(rq, 99.99))}
+        stats_ref = torch.from_numpy(ref["s64"].astype(np.float32)).cuda()
+        sd = ctx.side(error=path.err, sample_weight=path.sw, q_target=path.qt, error_sum=0.7 * N_PIXELS)
+        for net, key in ((na.STATNET, "statnet"), (na.RRSNET, "rrsnet")):
+            g_ref = ref["g"][net]
+            g, _ = ctx.param_count(net)
+            scale = 1.0 if prec == na.FP32 else float(2.0 ** np.clip(np.floor(np.log2(3e4 / np.abs(g_ref).max())), 0, 15))
+            ctx.grads_zero(net)
+            if net == na.STATNET:
+                rc = ctx.backward(net, sub, na.LOSS_LL2, target_L=path.target_L, loss_scale=scale)
+            else:
+                rc = ctx.backward(net, sub, na.LOSS_RRS, film=path.film, stats=stats_ref, side=sd, loss_scale=scale)
+            na.check(rc, "backward")
+            gd = ctx.grads_get(net).astype(np.float64) / scale
+            r[f"grad_{key}_grid_norm_rel_err"] = float(np.linalg.norm(gd[:g] - g_ref[:g]) / np.linalg.norm(g_ref[:g]))
+            r[f"grad_{key}_mlp_max_rel_err"] = float(np.max(np.abs(gd[g:] - g_ref[g:])) / np.max(np.abs(g_ref[g:])))
+        out[name] = r
+        del path, ctx
+        torch.cuda.empty_cache()
+    return out
 
 
 def frame_wavefront(torch, na, np, n_frames=8):
@@ -394,20 +471,22 @@ def main():
         dst = torch.empty_like(src)
         dst.copy_(src)
         path = Path(na.BF16, torch, na, np)
-        for _ in range(8 + 3):  # warm-up (clocks, caches, first-use allocations), then 3 measured
-            path.backward_statnet()
+        for k in range(8 + 3):  # warm-up (clocks, caches), then 3 measured steps
+            path.infer(k)
+            path.train()
         torch.cuda.synchronize()
         print(json.dumps({"pmc_probe": "done", "copy_bytes": src.numel() * 4}))
         return
     precs = {"bf16": na.BF16, "fp16": na.FP16, "fp32": na.FP32}
-    results, q_by = {}, {}
+    results, q_by, ops, n_mlp, n_grid = {}, {}, {}, {}, 0
     for name in args.precisions.split(","):
         path = Path(precs[name], torch, na, np)
         r = path.run(args.steps if name == "bf16" else max(2, args.steps // 2), args.warmup)
         if name == "bf16":
-            r["statnet_kernel_us"] = path.time_statnet_kernel(max(5, args.steps)) * 1e6
-            r["train_kernel_us"] = path.time_train_kernel(max(3, args.steps // 2)) * 1e6
-        # q of the same (pre-training) parameters for the matched-error report
+            ops = path.op_table(max(3, args.steps // 2))
+            n_grid, n_mlp["statnet"] = path.ctx.param_count(na.STATNET)
+            n_mlp["rrsnet"] = path.ctx.param_count(na.RRSNET)[1]
+        # q of the same (pre-training) parameters at the full batch, against the FP32 path
         fresh = Path(precs[name], torch, na, np)
         fresh.infer(0)
         q_by[name] = fresh.q.cpu().numpy()
@@ -420,34 +499,44 @@ def main():
 
     bf = results["bf16"]
     n = N_VERTICES
-    # Dominant operation of the BF16 step: the StatNet training backward (nrrs_backward with
-    # NRRS_LL2): k_encode_levels (level-major grid gathers, record counts per segment), k_train
-    # (forward with recompute, loss, backward, dW, dL/denc per level), k_bin_scan and
-    # k_bin_levels (table-gradient records per segment), k_apply_bins / k_apply_chunks /
-    # k_apply_finish (sum the records into the table gradient) and k_reduce_partials (dW).
-    # Algorithmic bytes per sample: input record 64 B +
-    # target 12 B, 16 levels x 8 corners x 4 B of BF16 feature pairs gathered, and the same
-    # 512 B of FP16 gradient pairs added to the table.
-    bytes_s = 64 + 12 + 16 * 8 * 4 + 16 * 8 * 4
-    flops_s = 3 * 2 * MACS["statnet"]  # forward, dX, dW
-    kern_s = bf["train_kernel_us"] * 1e-6
-    gbs = bytes_s * n / kern_s / 1e9
-    tflops = flops_s * n / kern_s / 1e12
-    roofline = {"bound": "hbm", "achieved": round(gbs, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                "frac": round(gbs / HBM_PEAK_GBS, 4), "traffic": TRAIN_TRAFFIC_BYTES,
-                "kernel": "nrrs_backward(StatNet, NRRS_LL2) at 2^20 samples: k_encode_levels + "
-                          "k_train<TBF16,5,6,NRRS_LOSS_LL2> + k_bin_scan + k_bin_levels + k_apply_bins + "
-                          "k_apply_chunks + k_apply_finish + k_reduce_partials",
-                "kernel_us": round(bf["train_kernel_us"], 1),
-                "algorithmic_bytes_per_launch": bytes_s * n,
-                "mfma_tflops": round(tflops, 2), "mfma_frac": round(tflops / BF16_PEAK_TFLOPS, 5),
-                "table_gradient_corner_updates_per_launch": 16 * 8 * n}
+    # every operation of the BF16 step against its roofline (HBM bytes, or MFMA flops)
+    op_rows = {}
+    for name, o in ops.items():
+        kind, work = op_work(name, n, n_grid, n_mlp)
+        row = {"ms_per_step": round(o["ms_per_step"], 4), "launches_per_step": o["launches_per_step"]}
+        if kind:
+            t = o["ms_per_step"] * 1e-3 / o["launches_per_step"]
+            if kind == "bytes":
+                row.update(bound="hbm", achieved=round(work / t / 1e9, 1), unit="GB/s",
+                           frac=round(work / t / 1e9 / HBM_PEAK_GBS, 4), work_per_launch=work)
+            else:
+                row.update(bound="mfma", achieved=round(work / t / 1e12, 2), unit="TFLOP/s",
+                           frac=round(work / t / 1e12 / BF16_PEAK_TFLOPS, 4), work_per_launch=work)
+        op_rows[name] = row
+    # dominant kernel of the step: the paired level-major encode pass (k_encode_levels<BF16, true>),
+    # launched by the RRS stage (encode_pair) and by the training step (encode_pair_counts, which
+    # also counts the table-gradient records); the entry uses the RRS stage's launch
+    enc = op_rows.get("encode_pair", {})
+    roofline = {"bound": "hbm", "achieved": enc.get("achieved"), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                "frac": enc.get("frac"), "traffic": ENCODE_TRAFFIC_BYTES,
+                "kernel": "k_encode_levels<BF16, true>: both nets' level-major hash-grid encodings from the paired "
+                          "table, 2^20 vertices (two launches per step: the RRS stage's and the training step's, the "
+                          "latter also counting the table-gradient records)",
+                "kernel_us": round(ops["encode_pair"]["ms_per_step"] * 1e3, 1) if "encode_pair" in ops else None,
+                "algorithmic_bytes_per_launch": 1164 * n,
+                "bytes_per_vertex": "12 B position + 16 levels x 8 corners x 8 B + 2 x 64 B encodings = 1164 B"}
     matched = {}
     if "fp32" in q_by:
         for name in q_by:
             if name != "fp32":
                 rel = np.abs(q_by[name] - q_by["fp32"]) / q_by["fp32"]
-                matched[name] = {"q_rel_err_vs_fp32_max": float(rel.max()), "q_rel_err_vs_fp32_mean": float(rel.mean())}
+                matched[name] = {"q_rel_err_vs_fp32_2p20_max": float(rel.max()),
+                                 "q_rel_err_vs_fp32_2p20_mean": float(rel.mean())}
+    if not args.no_oracle:
+        fp64 = matched_errors(torch, na, np, {k: precs[k] for k in args.precisions.split(",")})
+        for name, r in fp64.items():
+            matched.setdefault(name, {}).update({"vs_fp64_oracle": r})
+        print(f"[bench] matched error: {json.dumps(matched)}", file=sys.stderr, flush=True)
     out = {
         "metric": "RRS queries/s + train samples/s at 2^20 vertices, BF16 vs FP32 at matched error",
         "value": round(bf["step_queries_per_s"], 1),
@@ -471,6 +560,7 @@ def main():
         "train_samples_per_s": round(bf["train_samples_per_s"], 1),
         "matched_error": matched,
         "roofline": roofline,
+        "ops": op_rows,
     }
     if "fp32" in results:
         out["bf16_over_fp32"] = {
@@ -480,8 +570,8 @@ def main():
     if not args.no_peaks:
         pk = measured_peaks(torch)
         out["measured_peaks"] = pk
-        roofline["frac_of_measured_copy"] = round(roofline["achieved"] / pk["copy_GBps"], 4)
-        roofline["mfma_frac_of_measured_gemm"] = round(roofline["mfma_tflops"] / pk["bf16_gemm_TFLOPs"], 5)
+        if roofline["achieved"]:
+            roofline["frac_of_measured_copy"] = round(roofline["achieved"] / pk["copy_GBps"], 4)
         print(f"[bench] peaks: {json.dumps(pk)}", file=sys.stderr, flush=True)
     if not args.no_frame:
         out["wavefront_frame"] = frame_wavefront(torch, na, np)

@@ -283,6 +283,18 @@ nrrs_status nrrs_adam_step(nrrs_ctx* ctx, int32_t net, void* stream);
 nrrs_status nrrs_train_state(nrrs_ctx* ctx, int32_t net, uint32_t* step, float* loss_scale,
                              uint32_t* skipped);
 
+/* ---------------- operation timing (measurement support, no reference counterpart) ----------------
+ * With timing on, the library records a pair of HIP events on the launching stream around each
+ * operation of the RRS stage and of training (one launcher: one kernel or a few, e.g.
+ * "encode_pair", "mlp_chained", "decide", "train_statnet", "bin_records", "apply_rrsnet",
+ * "adam_statnet"), up to NRRS_MAX_OP_TIMES per collection. nrrs_op_times waits for the device,
+ * returns the names (static strings) and durations in ms of the operations recorded since the
+ * previous call, in launch order, and starts a new collection. Turning timing on creates the
+ * events (synchronous). */
+#define NRRS_MAX_OP_TIMES 256
+nrrs_status nrrs_set_op_timing(nrrs_ctx* ctx, int32_t on);
+nrrs_status nrrs_op_times(nrrs_ctx* ctx, uint32_t* count, const char** names, float* ms);
+
 /* ---------------- batched uneven wavefront: one persistent launch per frame ---------------- */
 
 /* One frame of `n_bounces` RRS stages. Queue 0 holds the primary paths [0, n_primary)
@@ -306,7 +318,7 @@ typedef struct {
     uint32_t* status_dev;     /* [1] bit 0 barrier timeout, bit 1 queue overflow (must start at 0) */
 } nrrs_frame_args;
 
-/* Tile shape of the frame kernel: wavefronts per workgroup (1, 2, 4) and vertices a
+/* Tile shape of the frame kernel: wavefronts per workgroup (1, 2, 4, 8) and vertices a
  * workgroup takes per step M (64, 128, 256; a multiple of 32 * waves). */
 typedef struct {
     uint32_t waves;

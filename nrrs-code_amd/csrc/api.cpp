@@ -31,6 +31,23 @@ nrrs_status hip_status(hipError_t e) {
     } while (0)
 #define NRRS_HIP(expr) NRRS_TRY(hip_status(expr))
 
+// Event pair around one operation when the context's operation timing is on (nrrs_set_op_timing).
+struct OpScope {
+    nrrs_ctx* c;
+    hipStream_t s;
+    int k = -1;
+    OpScope(nrrs_ctx* ctx, const char* name, hipStream_t stream) : c(ctx), s(stream) {
+        if (c->op_timing && c->op_count < NRRS_MAX_OP_TIMES) {
+            k = (int)c->op_count++;
+            c->op_name[k] = name;
+            (void)hipEventRecord(c->op_ev[2 * k], s);
+        }
+    }
+    ~OpScope() {
+        if (k >= 0) (void)hipEventRecord(c->op_ev[2 * k + 1], s);
+    }
+};
+
 struct Pcg32Host {
     uint64_t state, inc;
     Pcg32Host(uint64_t seed, uint64_t stream) {
@@ -178,6 +195,7 @@ bool vertices_ok(const nrrs_vertices* v, uint32_t n) {
 // Paired table from both nets' FP16/BF16 grids; after every change of either.
 nrrs_status refresh_pair(nrrs_ctx* c, hipStream_t s) {
     if (!c->grid_pair || !c->net[NRRS_STATNET].grid_lp || !c->net[NRRS_RRSNET].grid_lp) return NRRS_OK;
+    OpScope t(c, "pack_pair", s);
     return hip_status(launch_pack_pair(c->net[NRRS_STATNET].grid_lp, c->net[NRRS_RRSNET].grid_lp, c->grid_pair,
                                        c->net[NRRS_STATNET].n_grid / 2, s));
 }
@@ -195,6 +213,7 @@ nrrs_status refresh_lp(nrrs_ctx* c, int net, hipStream_t s) {
 hipError_t run_net(nrrs_ctx* c, int net, const EncodeArgs& a, int epi, uint32_t n, const EpiArgs& ep,
                    hipStream_t s) {
     const NetState& ns = c->net[net];
+    OpScope t(c, net == NRRS_STATNET ? "infer_statnet" : "infer_rrsnet", s);
     if (c->cfg.precision == NRRS_FP32)
         return launch_net_fp32(a, net, ns.n_layers, ns.n_out, epi, n, ns.grid_f32, ns.mlp_f32, ep, s);
     if (!c->grid_enc || n > c->grid_enc_cap)
@@ -416,9 +435,11 @@ nrrs_status run_backward(nrrs_ctx* c, int net, uint32_t n, const nrrs_vertices& 
         if (!c->grid_enc || n > c->grid_enc_cap) return NRRS_ERR_SHAPE;
         t.a.grid_enc = pre_enc ? pre_enc : c->grid_enc;
         t.a.grid_enc_stride = c->grid_enc_cap;
-        if (!pre_enc)
+        if (!pre_enc) {
+            OpScope ot(c, "encode_counts", s);
             NRRS_HIP(launch_encode_levels(cfg.precision, t.a, ns.grid_lp, n, c->grid_enc, c->grid_enc_cap, s,
                                           ns.binned ? ns.bin_runs : nullptr));
+        }
     }
     t.net = net;
     t.n = n;
@@ -467,17 +488,25 @@ nrrs_status run_backward(nrrs_ctx* c, int net, uint32_t n, const nrrs_vertices& 
     uint32_t iters = (n + vpi - 1) / vpi;
     uint32_t blocks = iters < c->max_train_blocks ? iters : c->max_train_blocks;
     if (!blocks) return NRRS_OK;
-    NRRS_HIP(launch_train(cfg.precision, t, ns.n_layers, ns.n_out, kind, blocks, s));
-    if (kind != NRRS_LOSS_EXTERNAL)
-        NRRS_HIP(launch_loss_finish(ns.loss_partial, blocks, loss_out, ns.state + 4, s));
+    {
+        OpScope ot(c, net == NRRS_STATNET ? "train_statnet" : "train_rrsnet", s);
+        NRRS_HIP(launch_train(cfg.precision, t, ns.n_layers, ns.n_out, kind, blocks, s));
+        if (kind != NRRS_LOSS_EXTERNAL)
+            NRRS_HIP(launch_loss_finish(ns.loss_partial, blocks, loss_out, ns.state + 4, s));
+    }
     if (blocks_out) {  // the caller bins, applies and reduces (joint step)
         *blocks_out = blocks;
         return NRRS_OK;
     }
     if (ns.binned) {
-        NRRS_HIP(launch_bin_levels(bin_args(c, net, n, t.a), s));
+        {
+            OpScope ot(c, net == NRRS_STATNET ? "bin_statnet" : "bin_rrsnet", s);
+            NRRS_HIP(launch_bin_levels(bin_args(c, net, n, t.a), s));
+        }
+        OpScope ot(c, net == NRRS_STATNET ? "apply_statnet" : "apply_rrsnet", s);
         NRRS_HIP(launch_apply(c, net, ns.rec_key, s));
     }
+    OpScope ot(c, net == NRRS_STATNET ? "reduce_statnet" : "reduce_rrsnet", s);
     return hip_status(launch_reduce_partials(ns.dw_partial, blocks, (uint32_t)ns.n_mlp, ns.grad + ns.n_grid, s));
 }
 
@@ -526,6 +555,7 @@ nrrs_status fold_dense(nrrs_ctx* c, int net, hipStream_t s) {
 nrrs_status run_adam(nrrs_ctx* c, int net, hipStream_t s) {
     NetState& ns = c->net[net];
     const nrrs_config& cfg = c->cfg;
+    OpScope t(c, net == NRRS_STATNET ? "adam_statnet" : "adam_rrsnet", s);
     NRRS_TRY(fold_dense(c, net, s));
     AdamArgs a{};
     a.params = ns.params;
@@ -614,6 +644,8 @@ void free_ctx(nrrs_ctx* c) {
     if (c->ev_fork) hipEventDestroy(c->ev_fork);
     if (c->ev_join) hipEventDestroy(c->ev_join);
     if (c->aux_stream) hipStreamDestroy(c->aux_stream);
+    for (auto& e : c->op_ev)
+        if (e) hipEventDestroy(e);
     hipFree(c->err_part);
     hipFree(c->nodes_mem);
     for (auto& ns : c->net) {
@@ -786,6 +818,12 @@ nrrs_status nrrs_create(const nrrs_config* cfg, nrrs_ctx** out) {
     }
     c->frame = make_frame(cfg->aabb_min, cfg->aabb_max);
     c->dec = decide_params(*cfg);
+    // default frame tile shape: BF16's split StatNet tiles (110 KB of LDS) leave one workgroup
+    // per CU, so it takes the 8-wave shape; FP16 (76 KB) fits two 4-wave workgroups
+    if (cfg->precision == NRRS_BF16) {
+        c->frame_waves = 8;
+        c->frame_m = 256;
+    }
 
     nrrs_status st = NRRS_OK;
     // paired table: one 32-bit word per entry and net, n_grid / 2 entries
@@ -925,7 +963,10 @@ nrrs_status nrrs_rrs_infer_decide(nrrs_ctx* ctx, const nrrs_decide_args* a, void
     NRRS_TRY(check_decide_args(ctx, a));
     if (!vertices_ok(&a->v, a->n) || !a->film_dev) return NRRS_ERR_ARG;
     hipStream_t s = (hipStream_t)stream;
-    NRRS_HIP(launch_zero_u64(ctx->sum_dev, s));
+    {
+        OpScope t(ctx, "zero_sum", s);
+        NRRS_HIP(launch_zero_u64(ctx->sum_dev, s));
+    }
     EpiArgs ep{};
     ep.q = a->q_dev;
     ep.sum = ctx->sum_dev;
@@ -942,14 +983,17 @@ nrrs_status nrrs_rrs_infer_decide(nrrs_ctx* ctx, const nrrs_decide_args* a, void
         ea.grid_enc = ctx->grid_enc;
         ea.grid_enc_stride = ctx->grid_enc_cap;
         if (paired_pass(ctx, a->n)) {
+            OpScope t(ctx, "encode_pair", s);
             NRRS_HIP(launch_encode_levels_dual(cfg.precision, ea, ctx->grid_pair, a->n, ctx->grid_enc, ctx->grid_enc2,
                                                ctx->grid_enc_cap, s));
         } else {
+            OpScope t(ctx, "encode_two", s);
             NRRS_HIP(launch_encode_levels(cfg.precision, ea, ctx->net[NRRS_STATNET].grid_lp, a->n, ctx->grid_enc,
                                           ctx->grid_enc_cap, s));
             NRRS_HIP(launch_encode_levels(cfg.precision, ea, ctx->net[NRRS_RRSNET].grid_lp, a->n, ctx->grid_enc2,
                                           ctx->grid_enc_cap, s));
         }
+        OpScope t(ctx, "mlp_chained", s);
         NRRS_HIP(launch_net_chained(cfg.precision, ea, ctx->grid_enc2, a->n, ctx->net[NRRS_STATNET].mlp_lp,
                                     ctx->net[NRRS_RRSNET].mlp_lp, ep, s));
     } else if (a->q_mode == NRRS_Q_RRSNET || a->q_mode == NRRS_Q_RRSNET_CHAINED) {
@@ -967,6 +1011,7 @@ nrrs_status nrrs_rrs_infer_decide(nrrs_ctx* ctx, const nrrs_decide_args* a, void
     } else {
         return NRRS_ERR_ARG;
     }
+    OpScope t(ctx, "decide", s);
     return hip_status(launch_decide(ctx, *a, s));
 }
 
@@ -1097,6 +1142,7 @@ nrrs_status nrrs_train_step(nrrs_ctx* ctx, uint32_t net_mask, uint32_t n, const 
         EncodeArgs ea = encode_args(ctx, *v, nullptr, nullptr, 0);
         uint32_t* rc = s0.binned ? s0.bin_runs : (s1.binned ? s1.bin_runs : nullptr);
         uint32_t* rc2 = s0.binned && s1.binned && !joint_records ? s1.bin_runs : nullptr;
+        OpScope t(ctx, "encode_pair_counts", s);
         NRRS_HIP(launch_encode_levels_dual(ctx->cfg.precision, ea, ctx->grid_pair, n, ctx->grid_enc, ctx->grid_enc2,
                                            ctx->grid_enc_cap, s, rc, rc2));
     }
@@ -1111,14 +1157,25 @@ nrrs_status nrrs_train_step(nrrs_ctx* ctx, uint32_t net_mask, uint32_t n, const 
                               ctx->grid_enc2, &b1));
         if (b0) {
             BinArgs bn = bin_args(ctx, NRRS_STATNET, n, encode_args(ctx, *v, nullptr, nullptr, 0));
-            NRRS_HIP(launch_bin_scan(bn, s));
-            // RRSNet's records take StatNet's slots: its apply reads the same per-segment counts
-            NRRS_HIP(hipMemcpyAsync(s1.seg_count, s0.seg_count, (size_t)s0.n_segs * 4, hipMemcpyDeviceToDevice, s));
+            {
+                OpScope t(ctx, "bin_scan", s);
+                NRRS_HIP(launch_bin_scan(bn, s));
+                // RRSNet's records take StatNet's slots: its apply reads the same per-segment counts
+                NRRS_HIP(hipMemcpyAsync(s1.seg_count, s0.seg_count, (size_t)s0.n_segs * 4, hipMemcpyDeviceToDevice,
+                                        s));
+            }
             bn.gdl2 = s1.gdl;
             bn.rec_val2 = s1.rec_val;
             bn.grad2 = reinterpret_cast<_Float16 __attribute__((ext_vector_type(2)))*>(s1.grad_lp);
-            NRRS_HIP(launch_bin_records(bn, s));
-            NRRS_HIP(launch_apply(ctx, NRRS_STATNET, s0.rec_key, s));
+            {
+                OpScope t(ctx, "bin_records_pair", s);
+                NRRS_HIP(launch_bin_records(bn, s));
+            }
+            {
+                OpScope t(ctx, "apply_statnet", s);
+                NRRS_HIP(launch_apply(ctx, NRRS_STATNET, s0.rec_key, s));
+            }
+            OpScope t(ctx, "reduce_statnet", s);
             NRRS_HIP(launch_reduce_partials(s0.dw_partial, b0, (uint32_t)s0.n_mlp, s0.grad + s0.n_grid, s));
         }
         // the StatNet update touches only StatNet's buffers: helper stream, beside RRSNet's apply
@@ -1137,9 +1194,15 @@ nrrs_status nrrs_train_step(nrrs_ctx* ctx, uint32_t net_mask, uint32_t n, const 
         }
         nrrs_status st = NRRS_OK;
         if (b1) {
-            hipError_t e = launch_apply(ctx, NRRS_RRSNET, s0.rec_key, s);
-            if (e == hipSuccess)
+            hipError_t e;
+            {
+                OpScope t(ctx, "apply_rrsnet", s);
+                e = launch_apply(ctx, NRRS_RRSNET, s0.rec_key, s);
+            }
+            if (e == hipSuccess) {
+                OpScope t(ctx, "reduce_rrsnet", s);
                 e = launch_reduce_partials(s1.dw_partial, b1, (uint32_t)s1.n_mlp, s1.grad + s1.n_grid, s);
+            }
             st = hip_status(e);
         }
         if (st == NRRS_OK) st = run_adam(ctx, NRRS_RRSNET, s);
@@ -1203,10 +1266,33 @@ nrrs_status nrrs_train_step(nrrs_ctx* ctx, uint32_t net_mask, uint32_t n, const 
     return refresh_pair(ctx, s);
 }
 
+nrrs_status nrrs_set_op_timing(nrrs_ctx* ctx, int32_t on) {
+    if (!ctx) return NRRS_ERR_ARG;
+    if (on && !ctx->op_ev[0]) {
+        for (auto& e : ctx->op_ev) NRRS_HIP(hipEventCreate(&e));
+    }
+    ctx->op_timing = on != 0;
+    ctx->op_count = 0;
+    return NRRS_OK;
+}
+
+nrrs_status nrrs_op_times(nrrs_ctx* ctx, uint32_t* count, const char** names, float* ms) {
+    if (!ctx || !count) return NRRS_ERR_ARG;
+    const uint32_t n = ctx->op_count;
+    NRRS_HIP(hipDeviceSynchronize());  // operations of every stream (the helper stream as well)
+    for (uint32_t k = 0; k < n; ++k) {
+        if (names) names[k] = ctx->op_name[k];
+        if (ms) NRRS_HIP(hipEventElapsedTime(ms + k, ctx->op_ev[2 * k], ctx->op_ev[2 * k + 1]));
+    }
+    *count = n;
+    ctx->op_count = 0;
+    return NRRS_OK;
+}
+
 nrrs_status nrrs_frame_set_shape(nrrs_ctx* ctx, const nrrs_tile_shape* shape) {
     if (!ctx || !shape) return NRRS_ERR_ARG;
     const uint32_t w = shape->waves, m = shape->m;
-    if ((w != 1 && w != 2 && w != 4) || (m != 64 && m != 128 && m != 256) || m % (32 * w)) return NRRS_ERR_ARG;
+    if ((w != 1 && w != 2 && w != 4 && w != 8) || (m != 64 && m != 128 && m != 256) || m % (32 * w)) return NRRS_ERR_ARG;
     ctx->frame_waves = w;
     ctx->frame_m = m;
     return NRRS_OK;
@@ -1223,7 +1309,8 @@ nrrs_status nrrs_frame_autotune(nrrs_ctx* ctx, const nrrs_frame_args* a, nrrs_ti
                                 void* stream) {
     if (!ctx) return NRRS_ERR_ARG;
     hipStream_t s = (hipStream_t)stream;
-    const nrrs_tile_shape shapes[] = {{1, 64}, {1, 128}, {1, 256}, {2, 64}, {2, 128}, {2, 256}, {4, 128}, {4, 256}};
+    const nrrs_tile_shape shapes[] = {{1, 64}, {1, 128}, {1, 256}, {2, 64}, {2, 128}, {2, 256}, {4, 128}, {4, 256},
+                                      {8, 256}};
     hipEvent_t e0, e1;
     NRRS_HIP(hipEventCreate(&e0));
     NRRS_HIP(hipEventCreate(&e1));

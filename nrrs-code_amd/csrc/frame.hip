@@ -129,7 +129,8 @@ __device__ __forceinline__ void store_position(const FrameParams& f, uint64_t se
 template <typename P, int WAVES>
 __global__ __launch_bounds__(64 * WAVES) void k_frame(FrameParams f) {
     constexpr int BLOCK = 64 * WAVES;
-    constexpr int kTilesS = tiles_of(5), kTilesR = tiles_of(4);
+    constexpr bool kSplit = split_of<P>(5);
+    constexpr int kTilesS = tiles_of(5, kSplit), kTilesR = tiles_of(4);
     extern __shared__ u32x4 smem[];
     u32x4* sWs = smem;
     u32x4* sWr = smem + kTilesS * 64;
@@ -226,7 +227,7 @@ __global__ __launch_bounds__(64 * WAVES) void k_frame(FrameParams f) {
                     if (cb >= n_cb) break;
                     const uint32_t i = cb * 32 + col;
                     const bool live = i < n;
-                    typename P::vec8 B[5], Br[2];
+                    typename P::vec8 B[5], Br[2], Bl[3];
                     VertexRec v;
                     float ref = 0.0f;
                     // the vertex's fields, drawn by both lane halves (all lanes take part; the
@@ -240,16 +241,18 @@ __global__ __launch_bounds__(64 * WAVES) void k_frame(FrameParams f) {
                         const uint32_t L = f.grid.n_levels;
                         grid_half_from_levels<P>(f.genc, cap, L, i, h, B);
                         grid_half_from_levels<P>(f.genc + (size_t)L * cap, cap, L, i, h, Br);
-                        encode_tail_half<P>(v, NRRS_STATNET, h, 0.0f, nullptr, B);
+                        encode_tail_half<P>(v, NRRS_STATNET, h, 0.0f, nullptr, B, kSplit ? Bl : nullptr);
                     } else {
 #pragma unroll
                         for (int kb = 0; kb < 5; ++kb) B[kb] = typename P::vec8{};
+#pragma unroll
+                        for (int kb = 0; kb < 3; ++kb) Bl[kb] = typename P::vec8{};
                         Br[0] = Br[1] = typename P::vec8{};
                     }
                     // K chunks 2 and 3 (SH-4 of the direction and the normal) are the same in
                     // both nets' inputs
                     const typename P::vec8 sh_dir = B[2], sh_nrm = B[3];
-                    const f32x16 Ys = mlp_forward<P, 5>(sWs, B, lane);
+                    const f32x16 Ys = mlp_forward<P, 5, kSplit>(sWs, B, Bl, lane);
                     // statistics of the lane's vertex: outputs 0..3 in half 0, 4..5 in half 1
                     float st[6];
                     {
@@ -272,7 +275,7 @@ __global__ __launch_bounds__(64 * WAVES) void k_frame(FrameParams f) {
                         B[2] = sh_dir;  // (the recomputed chunks are dead and not evaluated)
                         B[3] = sh_nrm;
                     }
-                    const f32x16 Yr = mlp_forward<P, 4>(sWr, B, lane);
+                    const f32x16 Yr = mlp_forward<P, 4>(sWr, B, nullptr, lane);
                     if (live && h == 0) {
                         const float q = rrs_window_clamp(f.d, activation_sigma(Yr[0]));
                         f.q_scr[i] = q;
@@ -394,7 +397,7 @@ __global__ void k_bounce_vertices(uint64_t seed, uint32_t bounce, const uint32_t
 
 template <typename P, int WAVES>
 hipError_t launch_frame_t(const FrameParams& f, uint32_t max_blocks, hipStream_t s) {
-    const size_t smem = (size_t)(tiles_of(5) + tiles_of(4)) * 64 * 16;
+    const size_t smem = (size_t)(tiles_of(5, split_of<P>(5)) + tiles_of(4)) * 64 * 16;
     auto kern = k_frame<P, WAVES>;
     int per_cu = 0;
     hipError_t e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, reinterpret_cast<const void*>(kern),
@@ -417,6 +420,7 @@ hipError_t launch_frame_p(const FrameParams& f, int waves, uint32_t max_blocks, 
     if (waves == 1) return launch_frame_t<P, 1>(f, max_blocks, s);
     if (waves == 2) return launch_frame_t<P, 2>(f, max_blocks, s);
     if (waves == 4) return launch_frame_t<P, 4>(f, max_blocks, s);
+    if (waves == 8) return launch_frame_t<P, 8>(f, max_blocks, s);
     return hipErrorInvalidValue;
 }
 

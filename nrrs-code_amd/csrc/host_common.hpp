@@ -177,6 +177,11 @@ struct nrrs_ctx {
     // beside the RRSNet backward; joined to the caller's stream before the step returns
     hipStream_t aux_stream = nullptr;
     hipEvent_t ev_fork = nullptr, ev_join = nullptr;
+    // operation timing (nrrs_set_op_timing): an event pair per recorded operation
+    bool op_timing = false;
+    uint32_t op_count = 0;
+    hipEvent_t op_ev[2 * NRRS_MAX_OP_TIMES] = {};
+    const char* op_name[NRRS_MAX_OP_TIMES] = {};
 };
 
 namespace nrrs {

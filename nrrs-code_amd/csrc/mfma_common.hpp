@@ -26,6 +26,9 @@ typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
 
 struct BF16 {
     typedef bf16x8 vec8;
+    // StatNet runs in split precision (see mlp_forward): with 8-bit mantissas its statistics,
+    // which reach q through a square root near zero, would carry q errors of about 1e-2
+    static constexpr bool kSplitStat = true;
     __device__ static float2 pair(uint32_t u) {
         return make_float2(__uint_as_float(u << 16), __uint_as_float(u & 0xffff0000u));
     }
@@ -42,6 +45,7 @@ struct BF16 {
 
 struct F16 {
     typedef f16x8 vec8;
+    static constexpr bool kSplitStat = false;
     __device__ static float2 pair(uint32_t u) {
         _Float16 lo = __builtin_bit_cast(_Float16, (uint16_t)(u & 0xffffu));
         _Float16 hi = __builtin_bit_cast(_Float16, (uint16_t)(u >> 16));
@@ -71,7 +75,24 @@ __device__ __forceinline__ uint16_t lp_bits<F16>(float f) {
 
 constexpr int kTileHalves = 64 * 8;  // one packed A tile: 64 lanes x 8 values
 
-__host__ __device__ constexpr int tiles_of(int n_layers) { return 10 + (n_layers - 2) * 8 + 4; }
+// Packed A tiles of a net, 1 KiB each. Plain: layer 0 (2 row tiles x 5 K chunks), each hidden
+// layer (2 x 4), the output layer (4 K chunks) and its residual tiles (4: lp(W - lp(W))).
+// Split (every layer in two-term precision but the grid chunks 0-1 of layer 0): layer 0, its
+// residual tiles of K chunks 2-4 (2 x 3), then every hidden layer's 8 tiles followed by their 8
+// residual tiles, then the output layer as above.
+__host__ __device__ constexpr int tiles_of(int n_layers, bool split = false) {
+    return 10 + (split ? 6 : 0) + (n_layers - 2) * 8 * (split ? 2 : 1) + 8;
+}
+// Whether a net runs in split precision: StatNet (5 layers) in precisions with kSplitStat.
+template <typename P>
+__host__ __device__ constexpr bool split_of(int n_layers) {
+    return P::kSplitStat && n_layers == 5;
+}
+// First tile of hidden layer l (1..n_layers-2) and of the output layer.
+__host__ __device__ constexpr int hidden_base(int layer, bool split) {
+    return 10 + (split ? 6 : 0) + (layer - 1) * 8 * (split ? 2 : 1);
+}
+__host__ __device__ constexpr int out_base(int n_layers, bool split) { return hidden_base(n_layers - 1, split); }
 
 // Input column of the packed K element (layer, kb, h, j).
 __device__ __forceinline__ int packed_col(int layer, int kb, int h, int j) {
@@ -99,30 +120,46 @@ __device__ __forceinline__ void encode_grid_half(const VertexRec& v, const GridP
     }
 }
 
+// Two-term operand of 8 FP32 values: hi = lp(x), lo = lp(x - hi) (the difference is exact in
+// FP32), so that hi + lo holds x to about 16 bits.
+template <typename P>
+__device__ __forceinline__ void pack_split(const float* x, typename P::vec8& hi, typename P::vec8& lo) {
+    float r[8];
+    hi = P::pack(x);
+#pragma unroll
+    for (int j = 0; j < 8; ++j) r[j] = x[j] - (float)hi[j];
+    lo = P::pack(r);
+}
+
+// K chunks 2-4 (SH-4 of the direction and the normal, per-vertex channels) of the lane's half;
+// with Bl, also their residual terms (Bl[0..2] for chunks 2..4).
 template <typename P>
 __device__ __forceinline__ void encode_tail_half(const VertexRec& v, int net, int h, float ref, const float* st,
-                                                 typename P::vec8* B) {
+                                                 typename P::vec8* B, typename P::vec8* Bl = nullptr) {
     float t[kEnc], e[8];
     encode_tail_rec(v, net, ref, st, t);
 #pragma unroll
     for (int kb = 2; kb < 5; ++kb) {
 #pragma unroll
         for (int j = 0; j < 8; ++j) e[j] = h ? t[kb * 16 + 8 + j] : t[kb * 16 + j];
-        B[kb] = P::pack(e);
+        if (Bl)
+            pack_split<P>(e, B[kb], Bl[kb - 2]);
+        else
+            B[kb] = P::pack(e);
     }
 }
 
 template <typename P>
 __device__ __forceinline__ void encode_half_rec(const VertexRec& v, const GridParams& grid, int net,
                                                 const uint32_t* __restrict__ table, int h, float ref,
-                                                const float* st, typename P::vec8* B) {
+                                                const float* st, typename P::vec8* B, typename P::vec8* Bl) {
     encode_grid_half<P>(v, grid, table, h, B);
-    encode_tail_half<P>(v, net, h, ref, st, B);
+    encode_tail_half<P>(v, net, h, ref, st, B, Bl);
 }
 
 template <typename P>
 __device__ __forceinline__ void encode_half(const EncodeArgs& a, int net, const uint32_t* __restrict__ table,
-                                            uint32_t i, int h, typename P::vec8* B) {
+                                            uint32_t i, int h, typename P::vec8* B, typename P::vec8* Bl = nullptr) {
     VertexRec v = vertex_from_soa(a, i);
     float st[6];
     const bool have_st = net == NRRS_RRSNET && a.stats;
@@ -131,7 +168,7 @@ __device__ __forceinline__ void encode_half(const EncodeArgs& a, int net, const 
         for (int k = 0; k < 6; ++k) st[k] = a.stats[(size_t)k * a.stats_stride + i];
     }
     const float ref = net == NRRS_RRSNET ? film_mean(a.film, a.n_pixels, v.pixel) : 0.0f;
-    encode_half_rec<P>(v, a.grid, net, table, h, ref, have_st ? st : nullptr, B);
+    encode_half_rec<P>(v, a.grid, net, table, h, ref, have_st ? st : nullptr, B, Bl);
 }
 
 // Grid part of the B operands from the level-major encoding: chunk kb of lane half h holds
@@ -153,7 +190,8 @@ __device__ __forceinline__ void grid_half_from_levels(const uint32_t* __restrict
 
 // encode_half with the grid channels read from the level-major encoding.
 template <typename P>
-__device__ __forceinline__ void encode_half_pre(const EncodeArgs& a, int net, uint32_t i, int h, typename P::vec8* B) {
+__device__ __forceinline__ void encode_half_pre(const EncodeArgs& a, int net, uint32_t i, int h, typename P::vec8* B,
+                                                typename P::vec8* Bl = nullptr) {
     VertexRec v = vertex_from_soa(a, i);
     float st[6];
     const bool have_st = net == NRRS_RRSNET && a.stats;
@@ -163,7 +201,7 @@ __device__ __forceinline__ void encode_half_pre(const EncodeArgs& a, int net, ui
     }
     const float ref = net == NRRS_RRSNET ? film_mean(a.film, a.n_pixels, v.pixel) : 0.0f;
     grid_half_from_levels<P>(a.grid_enc, a.grid_enc_stride, a.grid.n_levels, i, h, B);
-    encode_tail_half<P>(v, net, h, ref, have_st ? st : nullptr, B);
+    encode_tail_half<P>(v, net, h, ref, have_st ? st : nullptr, B, Bl);
 }
 
 __device__ __forceinline__ u32x4 lds_tile(const u32x4* __restrict__ sW, int tile, int lane) {
@@ -184,33 +222,77 @@ __device__ __forceinline__ void relu_to_operands(const f32x16* D, typename P::ve
     }
 }
 
+// ReLU of the last hidden layer as a two-term operand: H = lp(x) and L = lp(x - H) (the
+// difference is exact in FP32), so that H + L holds x to about 16 bits.
+template <typename P>
+__device__ __forceinline__ void relu_to_operands_split(const f32x16* D, typename P::vec8* H, typename P::vec8* L) {
+#pragma unroll
+    for (int ot = 0; ot < 2; ++ot) {
+#pragma unroll
+        for (int g = 0; g < 2; ++g) {
+            float x[8];
+#pragma unroll
+            for (int j = 0; j < 8; ++j) x[j] = fmaxf(D[ot][8 * g + j], 0.0f);
+            pack_split<P>(x, H[2 * ot + g], L[2 * ot + g]);
+        }
+    }
+}
+
 // Fused MLP of one net from the layer-0 operands B (5 K chunks); weights are the
-// packed A tiles at sW. Returns the output accumulator: output c of the lane's
-// vertex sits in lane half c / 4, register c % 4.
-template <typename P, int NL>
-__device__ __forceinline__ f32x16 mlp_forward(const u32x4* __restrict__ sW, const typename P::vec8* B, int lane) {
+// packed A tiles at sW (layout of tiles_of). Returns the output accumulator: output c of the
+// lane's vertex sits in lane half c / 4, register c % 4.
+// The output layer runs in three products, W_lo h_hi + W_hi h_lo + W_hi h_hi, with the weights
+// and the last hidden activations each split into two low-precision terms, so that its result
+// is about as accurate as an FP32 product of the FP32 activations (the outputs reach q through
+// a square root near zero, where an 8-bit-mantissa product is too coarse). SPLIT runs every
+// hidden layer the same way, and layer 0 too on its SH and per-vertex chunks (Bl: their residual
+// terms), the grid chunks staying single-term.
+template <typename P, int NL, bool SPLIT = false>
+__device__ __forceinline__ f32x16 mlp_forward(const u32x4* __restrict__ sW, const typename P::vec8* B,
+                                              const typename P::vec8* Bl, int lane) {
     f32x16 D[2];
 #pragma unroll
     for (int ot = 0; ot < 2; ++ot) {
         D[ot] = f32x16{};
+        if constexpr (SPLIT) {
+#pragma unroll
+            for (int kb = 2; kb < 5; ++kb) {
+                D[ot] = P::mma(lds_tile(sW, 10 + ot * 3 + (kb - 2), lane), B[kb], D[ot]);
+                D[ot] = P::mma(lds_tile(sW, ot * 5 + kb, lane), Bl[kb - 2], D[ot]);
+            }
+        }
 #pragma unroll
         for (int kb = 0; kb < 5; ++kb) D[ot] = P::mma(lds_tile(sW, ot * 5 + kb, lane), B[kb], D[ot]);
     }
-    typename P::vec8 H[4];
+    typename P::vec8 H[4], L[4];
 #pragma unroll
     for (int layer = 1; layer < NL - 1; ++layer) {
-        relu_to_operands<P>(D, H);
-        const int base = 10 + (layer - 1) * 8;
+        const int base = hidden_base(layer, SPLIT);
+        if constexpr (SPLIT)
+            relu_to_operands_split<P>(D, H, L);
+        else
+            relu_to_operands<P>(D, H);
 #pragma unroll
         for (int ot = 0; ot < 2; ++ot) {
             D[ot] = f32x16{};
+            if constexpr (SPLIT) {
+#pragma unroll
+                for (int kb = 0; kb < 4; ++kb) {
+                    D[ot] = P::mma(lds_tile(sW, base + 8 + ot * 4 + kb, lane), H[kb], D[ot]);
+                    D[ot] = P::mma(lds_tile(sW, base + ot * 4 + kb, lane), L[kb], D[ot]);
+                }
+            }
 #pragma unroll
             for (int kb = 0; kb < 4; ++kb) D[ot] = P::mma(lds_tile(sW, base + ot * 4 + kb, lane), H[kb], D[ot]);
         }
     }
-    relu_to_operands<P>(D, H);
+    relu_to_operands_split<P>(D, H, L);
     f32x16 Y = f32x16{};
-    const int base = 10 + (NL - 2) * 8;
+    const int base = out_base(NL, SPLIT);
+#pragma unroll
+    for (int kb = 0; kb < 4; ++kb) Y = P::mma(lds_tile(sW, base + 4 + kb, lane), H[kb], Y);
+#pragma unroll
+    for (int kb = 0; kb < 4; ++kb) Y = P::mma(lds_tile(sW, base + kb, lane), L[kb], Y);
 #pragma unroll
     for (int kb = 0; kb < 4; ++kb) Y = P::mma(lds_tile(sW, base + kb, lane), H[kb], Y);
     return Y;

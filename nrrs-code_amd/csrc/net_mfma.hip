@@ -12,26 +12,38 @@ using namespace mfma;
 constexpr int kWaves = 4;
 constexpr int kBlockT = 64 * kWaves;
 
-// One thread per (tile, lane): 8 weights of the A operand.
+// One thread per (tile, lane): 8 weights of the A operand, in the tile layout of tiles_of
+// (residual tiles hold lp(W - lp(W)); the difference is exact in FP32).
 template <typename P>
-__global__ void k_pack_mlp(const float* __restrict__ W, int n_layers, int n_out, uint16_t* __restrict__ out) {
+__global__ void k_pack_mlp(const float* __restrict__ W, int n_layers, int n_out, int split, uint16_t* __restrict__ out) {
     const int t = blockIdx.x;      // tile
     const int lane = threadIdx.x;  // 0..63
     int layer, ot, kb, base_w = 0, in = kEnc, rows = kHidden;
+    bool residual = false;
+    const int l0_tiles = split ? 16 : 10, per_hidden = split ? 16 : 8;
     if (t < 10) {
         layer = 0;
         ot = t / 5;
         kb = t % 5;
+    } else if (t < l0_tiles) {  // residuals of layer 0's K chunks 2-4
+        layer = 0;
+        ot = (t - 10) / 3;
+        kb = 2 + (t - 10) % 3;
+        residual = true;
     } else {
-        int u = t - 10;
-        layer = 1 + u / 8;
+        int u = t - l0_tiles;
+        layer = 1 + u / per_hidden;
         if (layer <= n_layers - 2) {
-            ot = (u % 8) / 4;
-            kb = u % 4;
+            const int v = u % per_hidden;
+            residual = v >= 8;
+            ot = (v % 8) / 4;
+            kb = v % 4;
         } else {
             layer = n_layers - 1;
             ot = 0;
-            kb = u - (n_layers - 2) * 8;
+            kb = u - (n_layers - 2) * per_hidden;  // 0..3: W, 4..7: residuals
+            residual = kb >= 4;
+            kb &= 3;
         }
         base_w = kHidden * kEnc + (layer - 1) * kHidden * kHidden;
         in = kHidden;
@@ -41,6 +53,7 @@ __global__ void k_pack_mlp(const float* __restrict__ W, int n_layers, int n_out,
 #pragma unroll
     for (int j = 0; j < 8; ++j) {
         float w = row < rows ? W[base_w + row * in + packed_col(layer, kb, h, j)] : 0.0f;
+        if (residual) w -= P::pair(lp_bits<P>(w)).x;
         out[((size_t)t * 64 + lane) * 8 + j] = lp_bits<P>(w);
     }
 }
@@ -116,7 +129,8 @@ __global__ __launch_bounds__(kBlockT, 4) void k_net_mfma(EncodeArgs a, int net, 
                                                       const uint32_t* __restrict__ table,
                                                       const u32x4* __restrict__ Wp, EpiArgs ep) {
     extern __shared__ u32x4 sW[];
-    constexpr int kTiles = tiles_of(NL);
+    constexpr bool kSplit = split_of<P>(NL);
+    constexpr int kTiles = tiles_of(NL, kSplit);
     for (int k = threadIdx.x; k < kTiles * 64; k += kBlockT) sW[k] = Wp[k];
     __syncthreads();
 
@@ -128,17 +142,19 @@ __global__ __launch_bounds__(kBlockT, 4) void k_net_mfma(EncodeArgs a, int net, 
         asm volatile("" ::: "memory");  // keeps the weight-tile loads from LDS inside the loop
         const uint32_t i = cb * 32 + col;
         const bool live = i < n;
-        typename P::vec8 B[5];
+        typename P::vec8 B[5], Bl[3];
         if (live) {
             if (a.grid_enc)
-                encode_half_pre<P>(a, net, i, h, B);
+                encode_half_pre<P>(a, net, i, h, B, kSplit ? Bl : nullptr);
             else
-                encode_half<P>(a, net, table, i, h, B);
+                encode_half<P>(a, net, table, i, h, B, kSplit ? Bl : nullptr);
         } else {
 #pragma unroll
             for (int kb = 0; kb < 5; ++kb) B[kb] = typename P::vec8{};
+#pragma unroll
+            for (int kb = 0; kb < 3; ++kb) Bl[kb] = typename P::vec8{};
         }
-        f32x16 Y = mlp_forward<P, NL>(sW, B, lane);
+        f32x16 Y = mlp_forward<P, NL, kSplit>(sW, B, Bl, lane);
 
         // output c of vertex i sits in lane half c / 4, register c % 4
         if (EPI == EPI_RAW) {
@@ -182,17 +198,25 @@ __global__ __launch_bounds__(kBlockT, 4) void k_net_mfma(EncodeArgs a, int net, 
 // registers (exchanged between the two lane halves) and feed the RRSNet input directly, which
 // saves writing and re-reading them and a second pass over the vertex records. The grid
 // channels of both nets come from level-major encodings (a.grid_enc: StatNet, enc_rrs: RRSNet).
-// Both nets' weight tiles sit in LDS (68 KB): 8-wave workgroups, two per CU, four waves per
-// SIMD (at most 128 registers). Epilogue as k_net_mfma's EPI_RRS: q and its exact sum term.
-constexpr int kChainWaves = 8;
-constexpr int kChainT = 64 * kChainWaves;
+// Both nets' weight tiles sit in LDS: FP16 76 KB, 8-wave workgroups, two per CU; BF16 (split
+// StatNet) 106 KB, one 16-wave workgroup per CU. Four waves per SIMD either way (at most 128
+// registers). Epilogue as k_net_mfma's EPI_RRS: q and its exact sum term.
+template <typename P>
+struct Chain {
+    static constexpr int kWaves = P::kSplitStat ? 16 : 8;
+    static constexpr int kThreads = 64 * kWaves;
+    static constexpr int kPerCu = P::kSplitStat ? 1 : 2;
+    static constexpr int kTilesS = tiles_of(5, split_of<P>(5)), kTilesR = tiles_of(4);
+};
 
 template <typename P>
-__global__ __launch_bounds__(kChainT, 2) void k_net_chained(EncodeArgs a, const uint32_t* __restrict__ enc_rrs,
-                                                           uint32_t n, const u32x4* __restrict__ Ws,
-                                                           const u32x4* __restrict__ Wr, EpiArgs ep) {
+__global__ __launch_bounds__(Chain<P>::kThreads) void k_net_chained(EncodeArgs a, const uint32_t* __restrict__ enc_rrs,
+                                                                   uint32_t n, const u32x4* __restrict__ Ws,
+                                                                   const u32x4* __restrict__ Wr, EpiArgs ep) {
     extern __shared__ u32x4 sW[];
-    constexpr int kTilesS = tiles_of(5), kTilesR = tiles_of(4);
+    constexpr int kChainWaves = Chain<P>::kWaves, kChainT = Chain<P>::kThreads;
+    constexpr int kTilesS = Chain<P>::kTilesS, kTilesR = Chain<P>::kTilesR;
+    constexpr bool kSplit = split_of<P>(5);
     u32x4* sWs = sW;
     u32x4* sWr = sW + kTilesS * 64;
     for (int k = threadIdx.x; k < kTilesS * 64; k += kChainT) sWs[k] = Ws[k];
@@ -207,16 +231,18 @@ __global__ __launch_bounds__(kChainT, 2) void k_net_chained(EncodeArgs a, const 
         asm volatile("" ::: "memory");  // keeps the weight-tile loads from LDS inside the loop
         const uint32_t i = cb * 32 + col;
         const bool live = i < n;
-        typename P::vec8 B[5];
+        typename P::vec8 B[5], Bl[3];
         if (live) {
-            encode_half_pre<P>(a, NRRS_STATNET, i, h, B);
+            encode_half_pre<P>(a, NRRS_STATNET, i, h, B, kSplit ? Bl : nullptr);
         } else {
 #pragma unroll
             for (int kb = 0; kb < 5; ++kb) B[kb] = typename P::vec8{};
+#pragma unroll
+            for (int kb = 0; kb < 3; ++kb) Bl[kb] = typename P::vec8{};
         }
         // K chunks 2 and 3 (SH-4 of the direction and the normal) are the same in both nets' inputs
         const typename P::vec8 sh_dir = B[2], sh_nrm = B[3];
-        const f32x16 Ys = mlp_forward<P, 5>(sWs, B, lane);
+        const f32x16 Ys = mlp_forward<P, 5, kSplit>(sWs, B, Bl, lane);
         // statistics of the lane's vertex: outputs 0..3 in half 0, 4..5 in half 1
         float st[6];
         {
@@ -240,7 +266,7 @@ __global__ __launch_bounds__(kChainT, 2) void k_net_chained(EncodeArgs a, const 
 #pragma unroll
             for (int kb = 0; kb < 5; ++kb) B[kb] = typename P::vec8{};
         }
-        const f32x16 Yr = mlp_forward<P, 4>(sWr, B, lane);
+        const f32x16 Yr = mlp_forward<P, 4>(sWr, B, nullptr, lane);
         if (live && h == 0) {
             const float q = rrs_window_clamp(ep.d, activation_sigma(Yr[0]));
             ep.q[i] = q;
@@ -253,11 +279,12 @@ __global__ __launch_bounds__(kChainT, 2) void k_net_chained(EncodeArgs a, const 
 template <typename P>
 hipError_t launch_chained(const EncodeArgs& a, const uint32_t* enc_rrs, uint32_t n, const uint16_t* Ws,
                           const uint16_t* Wr, const EpiArgs& ep, hipStream_t s) {
-    constexpr size_t smem = (size_t)(tiles_of(5) + tiles_of(4)) * kTileHalves * 2;
+    constexpr size_t smem = (size_t)(Chain<P>::kTilesS + Chain<P>::kTilesR) * kTileHalves * 2;
+    constexpr int kChainWaves = Chain<P>::kWaves;
     static std::atomic<bool> attr[kMaxDevices];
     hipError_t e = set_dyn_lds_once(attr, reinterpret_cast<const void*>(&k_net_chained<P>), smem);
     if (e != hipSuccess) return e;
-    // persistent: the resident workgroups only (two per CU), each loading the 68 KB of tiles once
+    // persistent: the resident workgroups only, each loading both nets' tiles once
     static std::atomic<uint32_t> resident_of[kMaxDevices];
     int dev = 0;
     if ((e = hipGetDevice(&dev)) != hipSuccess) return e;
@@ -265,14 +292,14 @@ hipError_t launch_chained(const EncodeArgs& a, const uint32_t* enc_rrs, uint32_t
     if (!resident) {
         int cus = 0;
         if ((e = hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev)) != hipSuccess) return e;
-        resident = 2u * (uint32_t)(cus > 0 ? cus : 1);
+        resident = (uint32_t)Chain<P>::kPerCu * (uint32_t)(cus > 0 ? cus : 1);
         if (dev >= 0 && dev < kMaxDevices) resident_of[dev].store(resident, std::memory_order_relaxed);
     }
     const uint32_t n_cb = (n + 31) / 32;
     uint32_t blocks = (n_cb + kChainWaves - 1) / kChainWaves;
     blocks = blocks > resident ? resident : blocks;
     if (!blocks) return hipSuccess;
-    hipLaunchKernelGGL(k_net_chained<P>, dim3(blocks), dim3(kChainT), smem, s, a, enc_rrs, n,
+    hipLaunchKernelGGL(k_net_chained<P>, dim3(blocks), dim3(Chain<P>::kThreads), smem, s, a, enc_rrs, n,
                        reinterpret_cast<const u32x4*>(Ws), reinterpret_cast<const u32x4*>(Wr), ep);
     return hipGetLastError();
 }
@@ -280,7 +307,10 @@ hipError_t launch_chained(const EncodeArgs& a, const uint32_t* enc_rrs, uint32_t
 template <typename P, int NL, int NOUT, int EPI>
 hipError_t launch_one(const EncodeArgs& a, int net, uint32_t n, const uint16_t* table, const uint16_t* Wp,
                       const EpiArgs& ep, hipStream_t s) {
-    constexpr size_t smem = (size_t)tiles_of(NL) * kTileHalves * 2;
+    constexpr size_t smem = (size_t)tiles_of(NL, split_of<P>(NL)) * kTileHalves * 2;
+    static std::atomic<bool> attr[kMaxDevices];
+    hipError_t e = set_dyn_lds_once(attr, reinterpret_cast<const void*>(&k_net_mfma<P, NL, NOUT, EPI>), smem);
+    if (e != hipSuccess) return e;
     const uint32_t n_cb = (n + 31) / 32;
     uint32_t blocks = (n_cb + kWaves - 1) / kWaves;
     blocks = blocks > 4096 ? 4096 : blocks;
@@ -340,17 +370,21 @@ hipError_t launch_encode_levels_dual(int precision, const EncodeArgs& a, const u
     return hipGetLastError();
 }
 
-size_t mfma_packed_halves(int n_layers) { return (size_t)tiles_of(n_layers) * kTileHalves; }
+// Room for the larger (split) layout.
+size_t mfma_packed_halves(int n_layers) { return (size_t)tiles_of(n_layers, true) * kTileHalves; }
 
 hipError_t launch_pack_lp(int precision, const float* grid, uint64_t n_grid, const float* W, int n_layers,
                           int n_out, uint16_t* grid_lp, uint16_t* W_lp, hipStream_t s) {
-    const int tiles = tiles_of(n_layers);
     if (precision == NRRS_BF16) {
-        hipLaunchKernelGGL(k_pack_mlp<BF16>, dim3(tiles), dim3(64), 0, s, W, n_layers, n_out, W_lp);
+        const int split = split_of<BF16>(n_layers);
+        hipLaunchKernelGGL(k_pack_mlp<BF16>, dim3(tiles_of(n_layers, split)), dim3(64), 0, s, W, n_layers, n_out, split,
+                           W_lp);
         if (grid)
             hipLaunchKernelGGL(k_to_lp<BF16>, dim3(2048), dim3(256), 0, s, grid, grid_lp, n_grid);
     } else if (precision == NRRS_FP16) {
-        hipLaunchKernelGGL(k_pack_mlp<F16>, dim3(tiles), dim3(64), 0, s, W, n_layers, n_out, W_lp);
+        const int split = split_of<F16>(n_layers);
+        hipLaunchKernelGGL(k_pack_mlp<F16>, dim3(tiles_of(n_layers, split)), dim3(64), 0, s, W, n_layers, n_out, split,
+                           W_lp);
         if (grid)
             hipLaunchKernelGGL(k_to_lp<F16>, dim3(2048), dim3(256), 0, s, grid, grid_lp, n_grid);
     } else {

@@ -219,6 +219,8 @@ _SIGS = {
     "nrrs_adam_step": (C.c_int, [_P, C.c_int32, _P]),
     "nrrs_train_state": (C.c_int, [_P, C.c_int32, C.POINTER(C.c_uint32), C.POINTER(C.c_float),
                                    C.POINTER(C.c_uint32)]),
+    "nrrs_set_op_timing": (C.c_int, [_P, C.c_int32]),
+    "nrrs_op_times": (C.c_int, [_P, C.POINTER(C.c_uint32), _P, _P]),
     "nrrs_frame_run": (C.c_int, [_P, C.POINTER(FrameArgs), _P]),
     "nrrs_frame_set_shape": (C.c_int, [_P, C.POINTER(TileShape)]),
     "nrrs_frame_autotune": (C.c_int, [_P, C.POINTER(FrameArgs), C.POINTER(TileShape), C.POINTER(C.c_float), _P]),
@@ -606,6 +608,20 @@ class Context:
         step, sc, sk = C.c_uint32(), C.c_float(), C.c_uint32()
         check(self.lib.nrrs_train_state(self.h, net, C.byref(step), C.byref(sc), C.byref(sk)), "train_state")
         return dict(step=step.value, loss_scale=sc.value, skipped=sk.value)
+
+    # ---- operation timing
+    MAX_OP_TIMES = 256
+
+    def set_op_timing(self, on):
+        check(self.lib.nrrs_set_op_timing(self.h, int(on)), "set_op_timing")
+
+    def op_times(self):
+        """[(name, ms)] of the operations recorded since the previous call (waits for the device)."""
+        n = C.c_uint32()
+        names = (C.c_char_p * self.MAX_OP_TIMES)()
+        ms = (C.c_float * self.MAX_OP_TIMES)()
+        check(self.lib.nrrs_op_times(self.h, C.byref(n), names, ms), "op_times")
+        return [(names[k].decode(), ms[k]) for k in range(n.value)]
 
     # ---- batched uneven wavefront (one persistent launch per frame)
     def frame_buffers(self, n_primary, n_bounces, capacity=None, trace=False, record_q=False, device="cuda"):

@@ -82,7 +82,7 @@ def test_frame_results_independent_of_tile_shape(torch_cuda):
     torch = torch_cuda
     ctx, _, film = _ctx(torch)
     ref = _run(torch, ctx, film, 99, shape=(4, 128))
-    for shape in ((1, 64), (1, 256), (2, 64), (2, 128), (4, 256)):
+    for shape in ((1, 64), (1, 256), (2, 64), (2, 128), (4, 256), (8, 256)):
         r = _run(torch, ctx, film, 99, shape=shape)
         assert np.array_equal(r["sizes"], ref["sizes"]), shape
         for b in range(BOUNCES + 1):

@@ -83,8 +83,10 @@ def _rel_err(a, b):
 # Output error relative to the output range, per precision of the MLP contractions:
 # FP32 vector path, FP16 and BF16 MFMA with FP32 accumulation (8 / 11-bit mantissas).
 OUT_TOL = {na.FP32: 1e-5, na.FP16: 5e-3, na.BF16: 4e-2}
-# Relative error of q = activationSigma(raw) against FP64.
-Q_TOL = {na.FP32: 1e-4, na.FP16: 5e-3, na.BF16: 3e-2}
+# Relative error of q = activationSigma(raw) against FP64 (the north-star bound is 1e-2), and
+# of the StatNet/ADRRS q relative to its range.
+Q_TOL = {na.FP32: 1e-4, na.FP16: 5e-3, na.BF16: 1e-2}
+Q_TOL_ADRRS = {na.FP32: 1e-4, na.FP16: 5e-3, na.BF16: 3e-2}
 PRECISIONS = [na.FP32, na.FP16, na.BF16]
 
 
@@ -192,7 +194,7 @@ def test_infer_decide_pipeline(torch_cuda, ora, mode, prec):
     if mode == na.Q_STATNET_ADRRS:
         # q = mean(clamp(L) thp / I): relative error of L near 0 is unbounded in low
         # precision, so the bound is on the error against the q range, and the mean
-        assert np.max(np.abs(q - q64)) / q64.max() < Q_TOL[prec] and qerr.mean() < Q_TOL[prec]
+        assert np.max(np.abs(q - q64)) / q64.max() < Q_TOL_ADRRS[prec] and qerr.mean() < Q_TOL_ADRRS[prec]
     else:
         assert qerr.max() < Q_TOL[prec]
     # the decision from the device's own FP32 q is bit-exact
@@ -206,7 +208,7 @@ def test_infer_decide_pipeline(torch_cuda, ora, mode, prec):
     ref64 = ora.rrs_decide(ctx.cfg, q64.astype(np.float32), seed=5, stream=1)
     flips = int(np.sum(ref64["s"] != ref["s"]))
     print(f"mode {mode}: {flips} of {n} split counts differ under the FP64 q")
-    assert flips <= n * Q_TOL[prec] + 2
+    assert flips <= n * Q_TOL_ADRRS[prec] + 2
 
 
 @pytest.mark.parametrize("prec", [na.FP32, na.BF16])
