@@ -77,7 +77,7 @@ def parse():
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU-baseline leg")
     ap.add_argument("--cpu-sample", type=int, default=1 << 19)
-    ap.add_argument("--precisions", default="bf16,fp16,fp32")
+    ap.add_argument("--precisions", default="bf16,fp16,fp32,fp32_plain")
     ap.add_argument("--no-frame", action="store_true", help="skip the multi-bounce wavefront and online loop")
     ap.add_argument("--no-peaks", action="store_true", help="skip the copy and GEMM measurements of the GPU")
     ap.add_argument("--no-oracle", action="store_true", help="skip the FP64-oracle error report")
@@ -201,11 +201,11 @@ def cpu_baseline(sample):
 class Path:
     """One precision: context, inputs and the timed step."""
 
-    def __init__(self, precision, torch, na, np):
+    def __init__(self, precision, torch, na, np, **cfg):
         self.torch, self.na = torch, na
         n = N_VERTICES
         self.n = n
-        self.ctx = na.Context(precision=precision, n_pixels=N_PIXELS, max_queue=n, max_train=n)
+        self.ctx = na.Context(precision=precision, n_pixels=N_PIXELS, max_queue=n, max_train=n, **cfg)
         rng = np.random.default_rng(0)
         for net in (na.STATNET, na.RRSNET):
             p = self.ctx.params_get(net)
@@ -477,20 +477,24 @@ def main():
         torch.cuda.synchronize()
         print(json.dumps({"pmc_probe": "done", "copy_bytes": src.numel() * 4}))
         return
-    precs = {"bf16": na.BF16, "fp16": na.FP16, "fp32": na.FP32}
+    precs = {"bf16": na.BF16, "fp16": na.FP16, "fp32": na.FP32, "fp32_plain": na.FP32}
+    # fp32: level-major grid encoding pass before the network kernel (the default); fp32_plain:
+    # every level gathered per vertex inside the network kernel
+    extra = {"fp32_plain": dict(fp32_level_major=0)}
     results, q_by, ops, n_mlp, n_grid = {}, {}, {}, {}, 0
     for name in args.precisions.split(","):
-        path = Path(precs[name], torch, na, np)
+        path = Path(precs[name], torch, na, np, **extra.get(name, {}))
         r = path.run(args.steps if name == "bf16" else max(2, args.steps // 2), args.warmup)
         if name == "bf16":
             ops = path.op_table(max(3, args.steps // 2))
             n_grid, n_mlp["statnet"] = path.ctx.param_count(na.STATNET)
             n_mlp["rrsnet"] = path.ctx.param_count(na.RRSNET)[1]
         # q of the same (pre-training) parameters at the full batch, against the FP32 path
-        fresh = Path(precs[name], torch, na, np)
-        fresh.infer(0)
-        q_by[name] = fresh.q.cpu().numpy()
-        del fresh
+        if name != "fp32_plain":
+            fresh = Path(precs[name], torch, na, np)
+            fresh.infer(0)
+            q_by[name] = fresh.q.cpu().numpy()
+            del fresh
         results[name] = r
         del path
         torch.cuda.empty_cache()
@@ -533,7 +537,7 @@ def main():
                 matched[name] = {"q_rel_err_vs_fp32_2p20_max": float(rel.max()),
                                  "q_rel_err_vs_fp32_2p20_mean": float(rel.mean())}
     if not args.no_oracle:
-        fp64 = matched_errors(torch, na, np, {k: precs[k] for k in args.precisions.split(",")})
+        fp64 = matched_errors(torch, na, np, {k: precs[k] for k in args.precisions.split(",") if k != "fp32_plain"})
         for name, r in fp64.items():
             matched.setdefault(name, {}).update({"vs_fp64_oracle": r})
         print(f"[bench] matched error: {json.dumps(matched)}", file=sys.stderr, flush=True)
@@ -562,11 +566,12 @@ def main():
         "roofline": roofline,
         "ops": op_rows,
     }
-    if "fp32" in results:
-        out["bf16_over_fp32"] = {
-            "rrs_queries": round(bf["rrs_queries_per_s"] / results["fp32"]["rrs_queries_per_s"], 3),
-            "train_samples": round(bf["train_samples_per_s"] / results["fp32"]["train_samples_per_s"], 3),
-            "step": round(bf["step_queries_per_s"] / results["fp32"]["step_queries_per_s"], 3)}
+    for ref, key in (("fp32", "bf16_over_fp32"), ("fp32_plain", "bf16_over_fp32_plain")):
+        if ref in results:
+            out[key] = {
+                "rrs_queries": round(bf["rrs_queries_per_s"] / results[ref]["rrs_queries_per_s"], 3),
+                "train_samples": round(bf["train_samples_per_s"] / results[ref]["train_samples_per_s"], 3),
+                "step": round(bf["step_queries_per_s"] / results[ref]["step_queries_per_s"], 3)}
     if not args.no_peaks:
         pk = measured_peaks(torch)
         out["measured_peaks"] = pk

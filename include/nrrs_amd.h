@@ -109,6 +109,11 @@ typedef struct {
      * grids in one pass; 0 gathers each net's table in a pass of its own. Results are bitwise
      * the same either way. No reference counterpart. */
     uint32_t paired_tables;
+    /* FP32 contexts: 1 (default) gathers the grid level by level over a batch into a level-major
+     * encoding that the network kernel reads (each level's table stays in L2 meanwhile); 0
+     * gathers all levels of a vertex inside the network kernel. Results are bitwise the same.
+     * No reference counterpart. */
+    uint32_t fp32_level_major;
 } nrrs_config;
 
 /* SoA path vertices of one bounce (replaces NRRSInferenceWorkItem, workitem.h:9-15). */

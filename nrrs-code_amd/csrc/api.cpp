@@ -214,8 +214,17 @@ hipError_t run_net(nrrs_ctx* c, int net, const EncodeArgs& a, int epi, uint32_t 
                    hipStream_t s) {
     const NetState& ns = c->net[net];
     OpScope t(c, net == NRRS_STATNET ? "infer_statnet" : "infer_rrsnet", s);
-    if (c->cfg.precision == NRRS_FP32)
-        return launch_net_fp32(a, net, ns.n_layers, ns.n_out, epi, n, ns.grid_f32, ns.mlp_f32, ep, s);
+    if (c->cfg.precision == NRRS_FP32) {
+        if (!c->grid_enc_f32 || n > c->grid_enc_cap)
+            return launch_net_fp32(a, net, ns.n_layers, ns.n_out, epi, n, ns.grid_f32, ns.mlp_f32, ep, s);
+        // level-major grid gathers first, then the network kernel reads the encoding
+        EncodeArgs ae = a;
+        ae.grid_enc_f32 = c->grid_enc_f32;
+        ae.grid_enc_stride = c->grid_enc_cap;
+        hipError_t e = launch_encode_levels_f32(a, ns.grid_f32, n, c->grid_enc_f32, c->grid_enc_cap, s);
+        if (e != hipSuccess) return e;
+        return launch_net_fp32(ae, net, ns.n_layers, ns.n_out, epi, n, ns.grid_f32, ns.mlp_f32, ep, s);
+    }
     if (!c->grid_enc || n > c->grid_enc_cap)
         return launch_net_mfma(c->cfg.precision, a, net, ns.n_layers, ns.n_out, epi, n, ns.grid_lp, ns.mlp_lp, ep, s);
     // level-major grid gathers first, then the MLP kernel reads the encoding
@@ -694,6 +703,7 @@ void free_ctx(nrrs_ctx* c) {
     hipFree(c->stats_scratch);
     hipFree(c->grid_enc);
     hipFree(c->grid_enc2);
+    hipFree(c->grid_enc_f32);
     hipFree(c->grid_pair);
     hipFree(c->q_scratch);
     hipFree(c->s_scratch);
@@ -780,6 +790,7 @@ nrrs_status nrrs_default_config(nrrs_config* c) {
     c->max_train = 1u << 14;
     c->seed = 0x4E525253ull;
     c->paired_tables = 1;
+    c->fp32_level_major = 1;
     return NRRS_OK;
 }
 
@@ -855,6 +866,10 @@ nrrs_status nrrs_create(const nrrs_config* cfg, nrrs_ctx** out) {
     if (st == NRRS_OK) st = dmalloc(&c->tile_sums, c->max_tiles);
     if (st == NRRS_OK) st = dmalloc(&c->tile_offsets, c->max_tiles);
     if (st == NRRS_OK) st = dmalloc(&c->stats_scratch, (size_t)6 * cfg->max_queue);
+    if (st == NRRS_OK && cfg->precision == NRRS_FP32 && cfg->fp32_level_major) {
+        c->grid_enc_cap = cfg->max_queue > cfg->max_train ? cfg->max_queue : cfg->max_train;
+        st = dmalloc(&c->grid_enc_f32, (size_t)c->grid.n_levels * c->grid_enc_cap);
+    }
     if (st == NRRS_OK && cfg->precision != NRRS_FP32) {
         c->grid_enc_cap = cfg->max_queue > cfg->max_train ? cfg->max_queue : cfg->max_train;
         st = dmalloc(&c->grid_enc, (size_t)c->grid.n_levels * c->grid_enc_cap);

@@ -45,6 +45,9 @@ struct EncodeArgs {
     // (k_encode_levels); when set, the MFMA kernels read it instead of gathering.
     const uint32_t* grid_enc = nullptr;
     uint32_t grid_enc_stride = 0;
+    // FP32 contexts: level-major grid encoding [level][grid_enc_stride] of FP32 feature pairs
+    // (k_encode_levels_f32); when set, the FP32 network kernel reads it instead of gathering.
+    const float2* grid_enc_f32 = nullptr;
 };
 
 struct DecideParams {
@@ -149,6 +152,7 @@ struct nrrs_ctx {
     uint32_t* grid_enc2 = nullptr;          // RRSNet's encoding of a paired pass, same shape
     uint32_t grid_enc_cap = 0;
     uint32_t* grid_pair = nullptr;          // paired table: [entries][2 nets] FP16/BF16 words, or null
+    float2* grid_enc_f32 = nullptr;         // FP32 level-major encoding [levels][grid_enc_cap], or null
     nrrs::NodeBuf nodes;                    // path nodes of the training paths
     void* nodes_mem = nullptr;
     uint32_t* tile_sums = nullptr;          // [max tiles]
@@ -192,6 +196,10 @@ hipError_t launch_encode(const EncodeArgs& a, int net, uint32_t n, const float* 
 hipError_t launch_grid_indices(const EncodeArgs& a, uint32_t n, uint32_t* idx, hipStream_t s);
 hipError_t launch_net_fp32(const EncodeArgs& a, int net, int n_layers, int n_out, int epi, uint32_t n,
                            const float* table, const float* W, const EpiArgs& ep, hipStream_t s);
+// Level-major FP32 grid encoding of one net's table: out[l * stride + i] = the feature pair of
+// level l of vertex i, one level after the other over the batch.
+hipError_t launch_encode_levels_f32(const EncodeArgs& a, const float* table, uint32_t n, float2* out, uint32_t stride,
+                                    hipStream_t s);
 
 // Launch set-up that is done once per device: `done` is the caller's static table, one flag per
 // device ordinal (devices beyond kMaxDevices repeat the set-up on every launch).

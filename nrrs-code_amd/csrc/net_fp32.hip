@@ -57,6 +57,21 @@ __device__ __forceinline__ void dense_fp32(const float* __restrict__ sW, const f
     }
 }
 
+// Level-major FP32 grid encoding: workgroup b takes level b / n_chunks and kChunk vertices, so
+// that the GPU walks the levels one after the other and each level's table (4 MB at T = 2^19)
+// stays in L2 while it is gathered.
+constexpr uint32_t kChunk = 1024;
+
+__global__ __launch_bounds__(kChunk) void k_encode_levels_f32(EncodeArgs a, const float* __restrict__ table, uint32_t n,
+                                                              uint32_t n_chunks, float2* __restrict__ out,
+                                                              uint32_t stride) {
+    const uint32_t l = blockIdx.x / n_chunks, i = (blockIdx.x % n_chunks) * kChunk + threadIdx.x;
+    if (l >= a.grid.n_levels || i >= n) return;
+    float x[3];
+    load_xn(a, i, x);
+    out[(size_t)l * stride + i] = grid_level_pairs<PairLoad<float>>(table, a.grid, (int)l, x);
+}
+
 template <int NL, int NOUT>
 constexpr int mlp_floats() {
     return kHidden * kEnc + (NL - 2) * kHidden * kHidden + NOUT * kHidden;
@@ -79,7 +94,17 @@ __global__ __launch_bounds__(kBlock) void k_net_fp32(EncodeArgs a, int net, uint
     {
         float h[kEnc];
         float t[kHidden];
-        if (live) {
+        if (live && a.grid_enc_f32) {
+            // grid channels from the level-major encoding, the rest computed here
+#pragma unroll
+            for (int l = 0; l < kMaxLevels; ++l) {
+                const float2 f = l < (int)a.grid.n_levels ? a.grid_enc_f32[(size_t)l * a.grid_enc_stride + i]
+                                                           : make_float2(0.0f, 0.0f);
+                h[2 * l] = f.x;
+                h[2 * l + 1] = f.y;
+            }
+            encode_tail(a, net, i, h);
+        } else if (live) {
             encode_full<float>(a, net, table, i, h);
         } else {
 #pragma unroll
@@ -150,6 +175,15 @@ hipError_t launch_encode(const EncodeArgs& a, int net, uint32_t n, const float* 
                          hipStream_t s) {
     if (!n) return hipSuccess;
     hipLaunchKernelGGL(k_encode, dim3((n + kBlock - 1) / kBlock), dim3(kBlock), 0, s, a, net, n, table, enc);
+    return hipGetLastError();
+}
+
+hipError_t launch_encode_levels_f32(const EncodeArgs& a, const float* table, uint32_t n, float2* out, uint32_t stride,
+                                    hipStream_t s) {
+    if (!n) return hipSuccess;
+    const uint32_t n_chunks = (n + kChunk - 1) / kChunk;
+    hipLaunchKernelGGL(k_encode_levels_f32, dim3(n_chunks * a.grid.n_levels), dim3(kChunk), 0, s, a, table, n,
+                       n_chunks, out, stride);
     return hipGetLastError();
 }
 

@@ -69,6 +69,7 @@ class Config(C.Structure):
         ("seed", C.c_uint64),
         ("max_nodes", C.c_uint32),
         ("paired_tables", C.c_uint32),
+        ("fp32_level_major", C.c_uint32),
     ]
 
 

@@ -258,3 +258,31 @@ def test_decision_branches(torch_cuda, ora, prec, knobs):
     assert np.array_equal(b["split"][:n].cpu().numpy(), ref["s"])
     assert np.array_equal(b["qn"][:n].cpu().numpy(), ref["qn"])
     assert np.array_equal(b["survivors"][:ns].cpu().numpy().astype(np.uint32), ref["survivors"])
+
+
+def test_fp32_level_major_equals_per_vertex_gathers(torch_cuda, ora):
+    """FP32 contexts gather the grid level by level into a level-major encoding by default
+    (fp32_level_major = 1); the per-vertex gathers inside the network kernel (0) give the same
+    bits: StatNet statistics and the chained stage's q, q', splits and survivors."""
+    torch = torch_cuda
+    n = 3001
+    out = []
+    for lm in (1, 0):
+        ctx = na.Context(na.default_config(precision=na.FP32, n_pixels=n, max_queue=n, fp32_level_major=lm))
+        random_params(ctx, na.STATNET, 61)
+        random_params(ctx, na.RRSNET, 62)
+        vb = na.synth_vertices(n, 79, n)
+        film_np, film = make_film(torch, n, 10)
+        b = _decide_buffers(torch, n, 8 * n)
+        args = ctx.decide_args(vb, film, b["q"], b["survivors"], b["n_surv"], b["overflow"],
+                               q_mode=na.Q_RRSNET_CHAINED, seed=7, stream_id=3, qn=b["qn"], split=b["split"])
+        assert ctx.rrs_infer_decide(args) == na.OK
+        st = ctx.statnet_infer(vb)
+        torch.cuda.synchronize()
+        r = {k: v.cpu().numpy() for k, v in b.items()}
+        r["stats"] = st.cpu().numpy()
+        out.append(r)
+    for k in ("q", "qn", "split", "n_surv", "stats"):
+        assert np.array_equal(out[0][k], out[1][k]), k
+    ns = int(out[0]["n_surv"][0])
+    assert np.array_equal(out[0]["survivors"][:ns], out[1]["survivors"][:ns])
