@@ -365,14 +365,18 @@ __device__ __forceinline__ float2 grid_level_pairs(const typename L::T* __restri
             }
         }
     }
-    float2 acc = make_float2(0.0f, 0.0f);
+    // corners with x bit 0 and with x bit 1 summed apart, in (y, z) order, then added: the order
+    // of the two-lane gathers (grid_level_half), which give the same bits
+    float2 a0 = make_float2(0.0f, 0.0f), a1 = make_float2(0.0f, 0.0f);
 #pragma unroll
-    for (int cn = 0; cn < 8; ++cn) {
-        const float w = corner_weight(lc, cn);
-        acc.x = __fmaf_rn(w, v[cn].x, acc.x);
-        acc.y = __fmaf_rn(w, v[cn].y, acc.y);
+    for (int yz = 0; yz < 4; ++yz) {
+        const float w0 = corner_weight(lc, yz << 1), w1 = corner_weight(lc, (yz << 1) | 1);
+        a0.x = __fmaf_rn(w0, v[2 * yz].x, a0.x);
+        a0.y = __fmaf_rn(w0, v[2 * yz].y, a0.y);
+        a1.x = __fmaf_rn(w1, v[2 * yz + 1].x, a1.x);
+        a1.y = __fmaf_rn(w1, v[2 * yz + 1].y, a1.y);
     }
-    return acc;
+    return make_float2(a0.x + a1.x, a0.y + a1.y);
 }
 
 // Both nets' FP16/BF16 tables interleaved per entry (paired table): entry e is the 8 bytes
@@ -417,15 +421,55 @@ __device__ __forceinline__ float4 grid_level_pairs_dual(const uint2* __restrict_
             v[2 * yz + 1] = other;
         }
     }
+    // the corner order of grid_level_pairs
+    float4 a0 = make_float4(0.0f, 0.0f, 0.0f, 0.0f), a1 = a0;
+#pragma unroll
+    for (int yz = 0; yz < 4; ++yz) {
+        const float w0 = corner_weight(lc, yz << 1), w1 = corner_weight(lc, (yz << 1) | 1);
+        const float2 s0 = Dec::pair(v[2 * yz].x), r0 = Dec::pair(v[2 * yz].y);
+        const float2 s1 = Dec::pair(v[2 * yz + 1].x), r1 = Dec::pair(v[2 * yz + 1].y);
+        a0.x = __fmaf_rn(w0, s0.x, a0.x);
+        a0.y = __fmaf_rn(w0, s0.y, a0.y);
+        a0.z = __fmaf_rn(w0, r0.x, a0.z);
+        a0.w = __fmaf_rn(w0, r0.y, a0.w);
+        a1.x = __fmaf_rn(w1, s1.x, a1.x);
+        a1.y = __fmaf_rn(w1, s1.y, a1.y);
+        a1.z = __fmaf_rn(w1, r1.x, a1.z);
+        a1.w = __fmaf_rn(w1, r1.y, a1.w);
+    }
+    return make_float4(a0.x + a1.x, a0.y + a1.y, a0.z + a1.z, a0.w + a1.w);
+}
+
+// Two-lane form of the gathers: lanes 2k and 2k + 1 hold the same vertex and gather its corners
+// with x bit xb = lane & 1, one 8-byte entry (both nets) or one 4-byte entry (one net) per (y, z),
+// so that the two entries of an x-neighbour pair, which lie in one cache line in all but about
+// one cell in 16 of a hashed level (their entries differ in the low bits only), come in one
+// request of one load instruction. Returns the lane's partial sums over its four corners in
+// (y, z) order; partner lanes add them as a0 + a1 (the order of grid_level_pairs). `table` is a
+// paired table (DUAL) or a one-net table of 32-bit FP16/BF16 pairs.
+template <typename Dec, bool DUAL>
+__device__ __forceinline__ float4 grid_level_half(const uint32_t* __restrict__ table, const GridParams& g, int l,
+                                                  const LevelCell& lc, uint32_t xb) {
+    const bool dense = (g.dense_mask >> l) & 1u;
+    const uint32_t cx = lc.c[0] + xb, r = g.res[l], off = g.offset[l];
     float4 acc = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
 #pragma unroll
-    for (int cn = 0; cn < 8; ++cn) {
-        const float w = corner_weight(lc, cn);
-        const float2 s = Dec::pair(v[cn].x), r = Dec::pair(v[cn].y);
-        acc.x = __fmaf_rn(w, s.x, acc.x);
-        acc.y = __fmaf_rn(w, s.y, acc.y);
-        acc.z = __fmaf_rn(w, r.x, acc.z);
-        acc.w = __fmaf_rn(w, r.y, acc.w);
+    for (int yz = 0; yz < 4; ++yz) {
+        const uint32_t cy = lc.c[1] + (yz & 1), cz = lc.c[2] + (yz >> 1);
+        const uint32_t e = dense ? cx + r * (cy + r * cz) : (cx ^ (cy * 2654435761u) ^ (cz * 805459861u)) & g.hash_mask;
+        const float w = corner_weight(lc, (int)xb | (yz << 1));
+        if constexpr (DUAL) {
+            const uint2 v = reinterpret_cast<const uint2*>(table)[off + e];
+            const float2 s = Dec::pair(v.x), q = Dec::pair(v.y);
+            acc.x = __fmaf_rn(w, s.x, acc.x);
+            acc.y = __fmaf_rn(w, s.y, acc.y);
+            acc.z = __fmaf_rn(w, q.x, acc.z);
+            acc.w = __fmaf_rn(w, q.y, acc.w);
+        } else {
+            const float2 s = Dec::pair(table[off + e]);
+            acc.x = __fmaf_rn(w, s.x, acc.x);
+            acc.y = __fmaf_rn(w, s.y, acc.y);
+        }
     }
     return acc;
 }

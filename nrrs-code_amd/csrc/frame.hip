@@ -26,7 +26,6 @@ using namespace mfma;
 
 constexpr uint32_t kSpinLimit = 1u << 22;
 constexpr uint64_t kBounceSeedStep = 0x9E3779B97F4A7C15ull;
-constexpr uint32_t kA0Vpt = 1;  // vertices per thread in the grid-encoding phase
 constexpr uint32_t kBarGroups = 8, kBarStride = 32;  // barrier groups; u32 words per counter line
 
 // Bounded sense-by-generation barrier over all workgroups of the launch, in two levels:
@@ -164,52 +163,42 @@ __global__ __launch_bounds__(64 * WAVES) void k_frame(FrameParams f) {
             for (uint32_t k = blockIdx.x * BLOCK + threadIdx.x; k < n; k += nb * BLOCK)
                 f.trace[(size_t)b * cap + k] = qin[k];
 
-        // ---- phase A0: level-major grid encodings of both nets; a thread takes kA0Vpt vertices
-        // of an item so that their gathers are in flight together (this kernel keeps few waves
-        // per CU resident). With the paired table an item covers both nets of a level.
+        // ---- phase A0: level-major grid encodings of both nets. Items are (net, level, BLOCK
+        // vertices), walked by all workgroups in the same order; lanes 2k and 2k + 1 gather the x bit 0
+        // and x bit 1 corners of vertices k and BLOCK/2 + k of an item (grid_level_half), as the
+        // stand-alone encode pass does. With the paired table an item covers both nets of a level.
         {
-            constexpr uint32_t kSpan = BLOCK * kA0Vpt;
-            const uint32_t L = f.grid.n_levels, n_ch = (n + kSpan - 1) / kSpan;
+            const uint32_t L = f.grid.n_levels, n_ch = (n + BLOCK - 1) / BLOCK;
             const bool paired = f.table_pair != nullptr;
             const uint32_t items = (paired ? 1u : 2u) * L * n_ch;
+            const uint32_t xb = threadIdx.x & 1u, kk = threadIdx.x >> 1;
             for (uint32_t it = blockIdx.x; it < items; it += nb) {
-                const uint32_t nl = it / n_ch, base = (it % n_ch) * kSpan + threadIdx.x;  // nl = net * L + level
-                if (paired) {
-                    float4 e[kA0Vpt];
+                const uint32_t nl = it / n_ch, base = (it % n_ch) * BLOCK;  // nl = net * L + level
+                const int lev = (int)(nl % L);
+                const uint32_t* table = paired ? reinterpret_cast<const uint32_t*>(f.table_pair)
+                                               : (nl < L ? f.table_stat : f.table_rrs);
 #pragma unroll
-                    for (uint32_t j = 0; j < kA0Vpt; ++j) {
-                        const uint32_t i = base + j * BLOCK;
-                        const uint32_t ic = i < n ? i : 0u;  // out-of-range lanes gather vertex 0, not stored
-                        const float x[3] = {f.xbuf[ic], f.xbuf[cap + ic], f.xbuf[2 * cap + ic]};
-                        e[j] = grid_level_pairs_dual<P>(f.table_pair, f.grid, (int)nl, x);
-                    }
-#pragma unroll
-                    for (uint32_t j = 0; j < kA0Vpt; ++j) {
-                        const uint32_t i = base + j * BLOCK;
-                        if (i < n) {
-                            f.genc[(size_t)nl * cap + i] =
-                                (uint32_t)lp_bits<P>(e[j].x) | ((uint32_t)lp_bits<P>(e[j].y) << 16);
-                            f.genc[(size_t)(L + nl) * cap + i] =
-                                (uint32_t)lp_bits<P>(e[j].z) | ((uint32_t)lp_bits<P>(e[j].w) << 16);
-                        }
-                    }
-                    continue;
-                }
-                const uint32_t* table = nl < L ? f.table_stat : f.table_rrs;
-                float2 e[kA0Vpt];
-#pragma unroll
-                for (uint32_t j = 0; j < kA0Vpt; ++j) {
-                    const uint32_t i = base + j * BLOCK;
+                for (uint32_t half = 0; half < 2; ++half) {
+                    const uint32_t i = base + half * (BLOCK / 2) + kk;
                     const uint32_t ic = i < n ? i : 0u;  // out-of-range lanes gather vertex 0, not stored
                     const float x[3] = {f.xbuf[ic], f.xbuf[cap + ic], f.xbuf[2 * cap + ic]};
-                    e[j] = grid_level_pairs<PairLoadLp<P>>(table, f.grid, (int)(nl % L), x);
-                }
-#pragma unroll
-                for (uint32_t j = 0; j < kA0Vpt; ++j) {
-                    const uint32_t i = base + j * BLOCK;
-                    if (i < n)
-                        f.genc[(size_t)nl * cap + i] =
-                            (uint32_t)lp_bits<P>(e[j].x) | ((uint32_t)lp_bits<P>(e[j].y) << 16);
+                    const LevelCell lc = level_cell(f.grid, lev, x);
+                    const float4 acc = paired ? grid_level_half<P, true>(table, f.grid, lev, lc, xb)
+                                              : grid_level_half<P, false>(table, f.grid, lev, lc, xb);
+                    float4 oth;
+                    oth.x = __shfl_xor(acc.x, 1, 64);
+                    oth.y = __shfl_xor(acc.y, 1, 64);
+                    oth.z = __shfl_xor(acc.z, 1, 64);
+                    oth.w = __shfl_xor(acc.w, 1, 64);
+                    const float4 a0 = xb ? oth : acc, a1 = xb ? acc : oth;
+                    if (i < n) {
+                        if (paired && xb)
+                            f.genc[(size_t)(L + nl) * cap + i] =
+                                (uint32_t)lp_bits<P>(a0.z + a1.z) | ((uint32_t)lp_bits<P>(a0.w + a1.w) << 16);
+                        else if (!xb)
+                            f.genc[(size_t)nl * cap + i] =
+                                (uint32_t)lp_bits<P>(a0.x + a1.x) | ((uint32_t)lp_bits<P>(a0.y + a1.y) << 16);
+                    }
                 }
             }
         }

@@ -64,46 +64,62 @@ __global__ void k_to_lp(const float* __restrict__ src, uint16_t* __restrict__ ds
         dst[i] = lp_bits<P>(src[i]);
 }
 
-// Level-major grid encoding: workgroup b takes level b / n_chunks and vertices
-// 256 (b % n_chunks) .. +255, so consecutive workgroups share one level's table.
-// DUAL: `table` is the paired table of both nets; StatNet's encoding goes to `out`, RRSNet's
-// to `out2`, and the record counts (the same for both nets, whose tables share one layout) to
-// `run_counts` and, when given, `run_counts2`.
+// Level-major grid encoding: workgroup b takes level b / n_chunks and the kRunVert vertices of
+// chunk b % n_chunks, so consecutive workgroups share one level's table and the GPU walks the
+// levels one after the other (each level's table stays in L2 while it is gathered). The pass is
+// bound by the L2 request rate of its random gathers, so each vertex is taken by two lanes
+// (grid_level_half): lane 2k + xb gathers the corners with x bit xb, and the two entries of an
+// x-neighbour pair come in one request. Lanes 2k and 2k + 1 take vertices k and kRunVert/2 + k of
+// the chunk in turn.
+// DUAL: `table` is the paired table of both nets; StatNet's encoding goes to `out` (written by
+// the even lane), RRSNet's to `out2` (odd lane), and the record counts (the same for both nets,
+// whose tables share one layout) to `run_counts` and, when given, `run_counts2`.
 template <typename P, bool DUAL>
 __global__ __launch_bounds__(kRunVert) void k_encode_levels(EncodeArgs a, const uint32_t* __restrict__ table, uint32_t n,
                                                             uint32_t n_chunks, uint32_t* __restrict__ out,
                                                             uint32_t* __restrict__ out2, uint32_t stride,
                                                             uint32_t* __restrict__ run_counts,
                                                             uint32_t* __restrict__ run_counts2) {
-    const uint32_t l = blockIdx.x / n_chunks, chunk = blockIdx.x % n_chunks, i = chunk * kRunVert + threadIdx.x;
+    const uint32_t l = blockIdx.x / n_chunks, chunk = blockIdx.x % n_chunks;
     if (l >= a.grid.n_levels) return;  // whole workgroup
-    const bool live = i < n;
-    float x[3];
-    if (live) {
-        load_xn(a, i, x);
-        const size_t o = (size_t)l * stride + i;
-        if constexpr (DUAL) {
-            const float4 f = grid_level_pairs_dual<P>(reinterpret_cast<const uint2*>(table), a.grid, (int)l, x);
-            out[o] = (uint32_t)lp_bits<P>(f.x) | ((uint32_t)lp_bits<P>(f.y) << 16);
-            out2[o] = (uint32_t)lp_bits<P>(f.z) | ((uint32_t)lp_bits<P>(f.w) << 16);
-        } else {
-            const float2 f = grid_level_pairs<PairLoadLp<P>>(table, a.grid, (int)l, x);
-            out[o] = (uint32_t)lp_bits<P>(f.x) | ((uint32_t)lp_bits<P>(f.y) << 16);
-        }
-    }
-    if (run_counts) {  // uniform: table-gradient records per segment of this level and chunk
-        __shared__ uint32_t hist[kMaxSegBins];
+    const uint32_t xb = threadIdx.x & 1u, k = threadIdx.x >> 1;
+    __shared__ uint32_t hist[kMaxSegBins];
+    if (run_counts) {
         if (threadIdx.x < kMaxSegBins) hist[threadIdx.x] = 0u;
         __syncthreads();
-        if (live) {
-            const LevelCell lc = level_cell(a.grid, (int)l, x);
-            const bool dense = (a.grid.dense_mask >> l) & 1u;  // few segments: aggregated atomics
+    }
+    const bool dense = (a.grid.dense_mask >> l) & 1u;
 #pragma unroll
-            for (int yz = 0; yz < 4; ++yz) {
+    for (uint32_t half = 0; half < 2; ++half) {
+        const uint32_t i = chunk * kRunVert + half * (kRunVert / 2) + k;
+        const bool live = i < n;
+        float x[3] = {0.0f, 0.0f, 0.0f};  // lanes past the batch gather cell 0 (in bounds), store nothing
+        if (live) load_xn(a, i, x);
+        const LevelCell lc = level_cell(a.grid, (int)l, x);
+        const float4 acc = grid_level_half<P, DUAL>(table, a.grid, (int)l, lc, xb);
+        float4 oth;
+        oth.x = __shfl_xor(acc.x, 1, 64);
+        oth.y = __shfl_xor(acc.y, 1, 64);
+        if constexpr (DUAL) {
+            oth.z = __shfl_xor(acc.z, 1, 64);
+            oth.w = __shfl_xor(acc.w, 1, 64);
+        }
+        const float4 a0 = xb ? oth : acc, a1 = xb ? acc : oth;
+        if (live) {
+            const size_t o = (size_t)l * stride + i;
+            if (DUAL && xb)
+                out2[o] = (uint32_t)lp_bits<P>(a0.z + a1.z) | ((uint32_t)lp_bits<P>(a0.w + a1.w) << 16);
+            else if (!xb)
+                out[o] = (uint32_t)lp_bits<P>(a0.x + a1.x) | ((uint32_t)lp_bits<P>(a0.y + a1.y) << 16);
+        }
+        if (run_counts && live) {  // records of this vertex's (y, z) rows 2xb and 2xb + 1
+#pragma unroll
+            for (int j = 0; j < 2; ++j) {
+                const int yz = 2 * (int)xb + j;
                 const uint32_t e0 = corner_index(a.grid, (int)l, lc, yz << 1);
                 const uint32_t e1 = corner_index(a.grid, (int)l, lc, (yz << 1) | 1);
                 const bool two = pair_code(e0, e1) == 3u;  // a record per corner
-                if (dense) {
+                if (dense) {  // few segments: aggregated atomics
                     lds_count(hist, e0 >> kSegShift);
                     if (two) lds_count(hist, e1 >> kSegShift);
                 } else {
@@ -112,6 +128,8 @@ __global__ __launch_bounds__(kRunVert) void k_encode_levels(EncodeArgs a, const 
                 }
             }
         }
+    }
+    if (run_counts) {  // uniform: table-gradient records per segment of this level and chunk
         __syncthreads();
         if (threadIdx.x < kMaxSegBins) {
             const size_t o = ((size_t)l * n_chunks + chunk) * kMaxSegBins + threadIdx.x;
