@@ -38,7 +38,7 @@ HBM_PEAK_GBS = 8000.0
 BF16_PEAK_TFLOPS = 2500.0
 # HBM bytes per launch of the paired encode pass at 2^20 vertices, from rocprofv3 --pmc
 # FETCH_SIZE and WRITE_SIZE (profiles/encode_pair_bf16_2p20_pmc.md), or None when not measured.
-ENCODE_TRAFFIC_BYTES = None
+ENCODE_TRAFFIC_BYTES = 1204578005
 MACS = {"statnet": 80 * 64 + 3 * 64 * 64 + 64 * 6, "rrsnet": 80 * 64 + 2 * 64 * 64 + 64 * 1}
 # Algorithmic work of one launch of each operation of the step (bytes it must move, or MFMA
 # flops), for a batch of n vertices and nets of n_grid + n_mlp parameters:
