@@ -19,6 +19,9 @@ roofline; `roofline` is the entry of the step's dominant kernel, the paired
 level-major grid-encoding pass (k_encode_levels<BF16, true>, two launches per step).
 `matched_error` compares each precision's RRS stage (StatNet statistics, q) and both
 nets' gradients with the FP64 oracle on the first 2^16 vertices of the batch.
+`wavefront_frame` and `online_loop` time the 10-bounce persistent frame launch and the
+256-frame online loop; `online_loop.q_drift_vs_fp64` is the q drift of a reduced online loop
+trained in BF16 against the FP64 oracle trained on the same seed stream.
 `cpu_baseline` is the FP32 C++ port of the same step on the host's threads, with the
 single-threaded FP64 oracle beside it.
 """
@@ -455,6 +458,32 @@ def online_loop(torch, na, np, n_frames=256, warm_frames=32):
             "queue_sizes_last_frame": bufs["sizes"].cpu().tolist(), "status": int(bufs["status"].item())}
 
 
+def online_drift(torch, na, np, frames=16, warm=4, n=4096):
+    """Drift of q against the FP64 oracle trained on the same seed stream, on a reduced online
+    loop (the FP64 oracle cannot follow 2^18 samples per frame): `frames` frames of 4096 pixels,
+    each one 4-bounce persistent frame launch and one BF16 training step on `n` samples (StatNet
+    alone for `warm` frames), against the oracle's FP64 steps on the same inputs from the same
+    initial parameters; q of a 4096-vertex probe set after every frame, each side on its own
+    parameters (tests/online_drift.py)."""
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    import online_drift as od
+    import oracle_ref as ora
+
+    cfg = od.config(na, na.BF16, n, 4096)
+    p0 = od.initial_params(na, cfg)
+    t0 = time.perf_counter()
+    q_ref, l_ref = od.oracle_trajectory(na, ora, cfg, p0, frames, warm, n, 4096)
+    t_ora = time.perf_counter() - t0
+    q_dev, _, status = od.device_trajectory(torch, na, cfg, p0, frames, warm, n, 4096)
+    c = od.drift_curve(q_dev, q_ref)
+    return {"frames": frames, "statnet_only_frames": warm, "train_batch": n, "pixels": 4096, "probe": 4096,
+            "mean_per_frame": [float(f"{x:.3e}") for x in c[:, 0]],
+            "p99_per_frame": [float(f"{x:.3e}") for x in c[:, 1]],
+            "max_per_frame": [float(f"{x:.3e}") for x in c[:, 2]],
+            "first_row": "untrained start (same parameters on both sides)", "frame_status": status,
+            "fp64_oracle_s": round(t_ora, 1)}
+
+
 def main():
     args = parse()
     if args.gpus != 1:
@@ -582,6 +611,8 @@ def main():
         out["wavefront_frame"] = frame_wavefront(torch, na, np)
         print(f"[bench] wavefront: {json.dumps(out['wavefront_frame'])}", file=sys.stderr, flush=True)
         out["online_loop"] = online_loop(torch, na, np)
+        if not args.no_oracle:
+            out["online_loop"]["q_drift_vs_fp64"] = online_drift(torch, na, np)
         print(f"[bench] online: {json.dumps(out['online_loop'])}", file=sys.stderr, flush=True)
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(args.cpu_sample)

@@ -6,6 +6,9 @@ training steps (StatNet alone first, then StatNet + RRSNet), and the drift of q 
   oracle on the same parameters; the BF16 bound is the one of the untrained forward test.
 - Trajectory drift: an FP32 context trained on the same inputs from the same initial
   parameters; q of the two trained networks on the probe set.
+- Drift against FP64 training: a reduced loop (16 frames of 4096 samples, StatNet alone for 4)
+  trained on the device in FP32, FP16 and BF16 and by the FP64 oracle on the same seed stream
+  (tests/online_drift.py); q of a probe set after every frame, each side on its own parameters.
 """
 import numpy as np
 import pytest
@@ -114,3 +117,47 @@ def test_online_loop_q_drift(torch_cuda, ora):
     # trained in BF16 vs FP32 from the same start on the same data: the nets agree on q
     assert t_rel.mean() < 0.1
     assert abs(q16.mean() - q32.mean()) / q32.mean() < 0.05
+
+
+# Reduced online loop against an FP64 oracle trained on the same seed stream
+DRIFT = dict(frames=16, warm=4, n=4096, n_probe=4096)
+DRIFT_P = 4096  # pixels
+Q_TOL_START = {na.FP32: 1e-4, na.FP16: 5e-3, na.BF16: 1e-2}  # untrained inference error (test_gpu_rrs)
+LOSS_TOL = {na.FP32: 2e-4, na.FP16: 1e-2, na.BF16: 5e-2}
+# (mean, max) of |q - q_fp64| / q_fp64 over the probe set, at every frame: FP32 stays at its
+# rounding level; FP16/BF16 keep the mean within the 1e-2 RRS-factor target
+DRIFT_TOL = {na.FP32: (1e-5, 1e-4), na.FP16: (1e-2, 4e-2), na.BF16: (1e-2, 4e-2)}
+
+
+@pytest.fixture(scope="module")
+def fp64_trajectory(torch_cuda, ora):
+    import online_drift as od
+
+    cfg = od.config(na, na.FP32, DRIFT["n"], DRIFT_P)
+    p0 = od.initial_params(na, cfg)
+    q, losses = od.oracle_trajectory(na, ora, cfg, p0, **DRIFT)
+    return p0, q, losses
+
+
+@pytest.mark.parametrize("prec", [na.FP32, na.FP16, na.BF16])
+def test_online_loop_drift_vs_fp64_trajectory(torch_cuda, fp64_trajectory, prec):
+    import online_drift as od
+
+    p0, q_ref, l_ref = fp64_trajectory
+    cfg = od.config(na, prec, DRIFT["n"], DRIFT_P)
+    q_dev, l_dev, status = od.device_trajectory(torch_cuda, na, cfg, p0, **DRIFT)
+    assert status == 0
+    curve = od.drift_curve(q_dev, q_ref)
+    print(f"prec {prec}: q drift vs the FP64 trajectory per frame (mean / p99 / max):")
+    for f, (m, p, x) in enumerate(curve):
+        print(f"  frame {f - 1:3d}: {m:.3e} {p:.3e} {x:.3e}   loss dev {l_dev[f - 1] if f else ''} "
+              f"fp64 {l_ref[f - 1] if f else ''}")
+    assert np.all(np.isfinite(l_dev[:, 0])) and np.all(np.isfinite(l_dev[DRIFT["warm"]:, 1]))
+    # frame -1: the same parameters on both sides, the inference error of the precision
+    assert curve[0, 2] < Q_TOL_START[prec]
+    # the batch losses of every step follow the FP64 ones (the bounds of the joint-step test)
+    trained = ~np.isnan(l_ref)
+    assert np.all(np.abs(l_dev[trained] - l_ref[trained]) / np.abs(l_ref[trained]) < LOSS_TOL[prec])
+    # q of the trained networks stays with the FP64 trajectory over all 16 frames
+    assert np.all(curve[:, 0] < DRIFT_TOL[prec][0]), curve[:, 0]
+    assert np.all(curve[:, 2] < DRIFT_TOL[prec][1]), curve[:, 2]
