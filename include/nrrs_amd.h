@@ -85,7 +85,11 @@ typedef struct {
     /* Losses (2net-simple.json:3-4, 145-148) */
     int32_t loss_clamp_on;  /* 1  */
     float loss_clamp_max;   /* 10 */
-    int32_t train_sigma;    /* must be 0 (channels 3-5 model E[L^2]) */
+    /* train_sigma (network.h:158, train.cu:196-200): 0 = StatNet channels 3-5 model E[L^2];
+     * 1 = they model sigma through activationSigma, E[L^2] = sigma^2 + L^2. With 1 the
+     * RRSNet training input takes activationSigma(max(L2, 0)) where inference takes
+     * max(L2, 0), as the reference's two input builders do. */
+    int32_t train_sigma;    /* 0 */
     uint32_t ll2_step;      /* 0: L only, >=1: L and L^2 */
     uint32_t rrs_step;      /* 3: on-policy RRSNet loss; <3: RRSNet gradient is zero */
     float gamma[4];         /* 0.01, 0.05, 0.0, 0.01 */

@@ -439,6 +439,10 @@ nrrs_status run_backward(nrrs_ctx* c, int net, uint32_t n, const nrrs_vertices& 
     if (n > cfg.max_train) return NRRS_ERR_SHAPE;  // per-sample training buffers hold max_train
     TrainArgs t{};
     t.a = encode_args(c, v, film, stats, stats_stride);
+    // the RRSNet training input takes activationSigma(max(L2, 0)) when StatNet models sigma
+    // (train.cu:196-200); inference keeps max(L2, 0)
+    t.a.sigma_l2 = cfg.train_sigma && net == NRRS_RRSNET;
+    t.train_sigma = cfg.train_sigma;
     if (cfg.precision != NRRS_FP32) {
         // level-major grid gathers of the forward, read back by the training kernel
         if (!c->grid_enc || n > c->grid_enc_cap) return NRRS_ERR_SHAPE;
@@ -803,7 +807,8 @@ nrrs_status nrrs_grid_layout(const nrrs_grid_config* g, uint32_t* res, uint32_t*
 nrrs_status nrrs_create(const nrrs_config* cfg, nrrs_ctx** out) {
     if (!cfg || !out) return NRRS_ERR_ARG;
     *out = nullptr;
-    if (!valid_shapes(*cfg) || cfg->train_sigma) return NRRS_ERR_UNSUPPORTED;
+    if (!valid_shapes(*cfg)) return NRRS_ERR_UNSUPPORTED;
+    if (cfg->train_sigma != 0 && cfg->train_sigma != 1) return NRRS_ERR_ARG;
     if (cfg->precision != NRRS_FP32 && cfg->precision != NRRS_FP16 && cfg->precision != NRRS_BF16)
         return NRRS_ERR_UNSUPPORTED;
     if (cfg->max_queue == 0 || cfg->n_pixels == 0 || cfg->split_cap == 0 || cfg->split_cap > 255 ||

@@ -519,8 +519,10 @@ __device__ __forceinline__ float film_mean(const float* film, uint32_t n_pixels,
 }
 
 // Non-grid channels [32,80) of a vertex. RRSNet: ref = pixel mean, st = StatNet
-// statistics (L_rgb, L2_rgb) or null.
-__device__ __forceinline__ void encode_tail_rec(const VertexRec& v, int net, float ref, const float* st, float* e) {
+// statistics (L_rgb, L2_rgb) or null; sigma_l2: the L2 channels are sigma outputs and enter
+// as activationSigma(max(L2, 0)) (the training input with train_sigma, train.cu:196-200).
+__device__ __forceinline__ void encode_tail_rec(const VertexRec& v, int net, float ref, const float* st, float* e,
+                                                bool sigma_l2 = false) {
     sh4(v.dir[0], v.dir[1], v.dir[2], e + 32);
     sh4(v.nrm[0], v.nrm[1], v.nrm[2], e + 48);
 #pragma unroll
@@ -541,7 +543,8 @@ __device__ __forceinline__ void encode_tail_rec(const VertexRec& v, int net, flo
 #pragma unroll
             for (int k = 0; k < 3; ++k) {
                 e[69 + k] = box_cox(fmaxf(st[k], 0.0f));
-                e[72 + k] = box_cox(fmaxf(st[3 + k], 0.0f));
+                const float l2 = fmaxf(st[3 + k], 0.0f);
+                e[72 + k] = box_cox(sigma_l2 ? activation_sigma(l2) : l2);
             }
         } else {
 #pragma unroll
@@ -564,7 +567,7 @@ __device__ __forceinline__ void encode_tail(const EncodeArgs& a, int net, uint32
         for (int k = 0; k < 6; ++k) st[k] = a.stats[(size_t)k * a.stats_stride + i];
     }
     const float ref = net == NRRS_RRSNET ? film_mean(a.film, a.n_pixels, v.pixel) : 0.0f;
-    encode_tail_rec(v, net, ref, have_st ? st : nullptr, e);
+    encode_tail_rec(v, net, ref, have_st ? st : nullptr, e, a.sigma_l2 != 0);
 }
 
 template <typename TT>

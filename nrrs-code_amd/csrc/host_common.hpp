@@ -48,6 +48,8 @@ struct EncodeArgs {
     // FP32 contexts: level-major grid encoding [level][grid_enc_stride] of FP32 feature pairs
     // (k_encode_levels_f32); when set, the FP32 network kernel reads it instead of gathering.
     const float2* grid_enc_f32 = nullptr;
+    // RRSNet training input with sigma outputs: channels 72-74 take activationSigma(max(L2, 0))
+    int sigma_l2 = 0;
 };
 
 struct DecideParams {

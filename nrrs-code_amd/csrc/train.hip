@@ -307,6 +307,13 @@ __global__ __launch_bounds__(64 * P::kWaves) void k_train(TrainArgs t) {
         // output c sits in lane half c/4, register c%4
         float dyr[4] = {0.0f, 0.0f, 0.0f, 0.0f};
         float lterm = 0.0f;  // this lane's loss terms of the iteration
+        // With sigma outputs, output c >= 3 needs L output c - 3: c = 3 sits in this lane (half 0),
+        // c = 4, 5 in lane half 1 need outputs 1, 2 of the other half
+        float y_mu[4] = {0.0f, 0.0f, 0.0f, 0.0f};
+        if (LOSS == NRRS_LOSS_LL2 && NOUT == 6) {
+#pragma unroll
+            for (int r = 0; r < 3; ++r) y_mu[r] = __shfl_xor(Y[r], 32);
+        }
         if (live) {
             if (LOSS == NRRS_LOSS_EXTERNAL) {
 #pragma unroll
@@ -324,10 +331,22 @@ __global__ __launch_bounds__(64 * P::kWaves) void k_train(TrainArgs t) {
                         if (t.clamp_on) tg = fminf(tg, t.clamp_max);
                         if (c >= 3) tg = tg * tg;
                         const float y = Y[r];
-                        const float den = y * y + 0.01f;
-                        const float d = y - tg;
-                        lterm += d * d / den * t.inv_norm;
-                        dyr[r] = 2.0f * d / den * t.inv_norm;
+                        if (c >= 3 && t.train_sigma) {
+                            // second moment sigma^2 + mu^2 against L^2, mu (output c - 3) held
+                            // constant: the gradient reaches only the sigma output
+                            const float mu = c == 3 ? Y[0] : y_mu[c - 3];
+                            const float sg = activation_sigma(y);
+                            const float m2 = sg * sg + mu * mu;
+                            const float den = m2 * m2 + 0.01f;
+                            const float d = m2 - tg;
+                            lterm += d * d / den * t.inv_norm;
+                            dyr[r] = 2.0f * d / den * 2.0f * sg * activation_sigma_grad(y) * t.inv_norm;
+                        } else {
+                            const float den = y * y + 0.01f;
+                            const float d = y - tg;
+                            lterm += d * d / den * t.inv_norm;
+                            dyr[r] = 2.0f * d / den * t.inv_norm;
+                        }
                     }
                 }
             } else if (LOSS == NRRS_LOSS_RRS) {
@@ -346,7 +365,16 @@ __global__ __launch_bounds__(64 * P::kWaves) void k_train(TrainArgs t) {
 #pragma unroll
                         for (int c = 0; c < 3; ++c) {
                             const float th = t.a.v.thp[c * S3 + i];
-                            m2 += th * th * fmaxf(t.a.stats[(size_t)(3 + c) * t.a.stats_stride + i], 0.0f);
+                            const float s2 = t.a.stats[(size_t)(3 + c) * t.a.stats_stride + i];
+                            float l2;
+                            if (t.train_sigma) {  // E[L^2] = sigma^2 + max(L, 0)^2
+                                const float sg = activation_sigma(s2);
+                                const float mu = fmaxf(t.a.stats[(size_t)c * t.a.stats_stride + i], 0.0f);
+                                l2 = sg * sg + mu * mu;
+                            } else {
+                                l2 = fmaxf(s2, 0.0f);
+                            }
+                            m2 += th * th * l2;
                         }
                         float vv = m2 * (1.0f / 3.0f) / (ref * ref + 0.01f);
                         if (t.clamp_on) vv = fminf(vv, t.clamp_max);

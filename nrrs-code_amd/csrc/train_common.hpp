@@ -72,6 +72,7 @@ struct TrainArgs {
     float gamma[4];
     int clamp_on;
     float clamp_max;
+    int train_sigma;               // StatNet channels 3-5 model sigma (1) or E[L^2] (0)
     uint32_t ll2_step, rrs_step;
     float inv_norm;                // loss normalisation
     // outputs

@@ -106,6 +106,7 @@ uint32_t corner_index(const Level& L, uint32_t log2T, uint32_t cx, uint32_t cy, 
 }
 
 double sigma_d(double x) { return x < 0 ? std::log1p(std::exp(x)) : 0.5 * x + 0.6931471805599453; }
+double sigma_grad_d(double x) { return x < 0 ? std::exp(x) / (1.0 + std::exp(x)) : 0.5; }
 
 float sigma_f(float x) {
     // activationSigma (src/render/nrrs/net/networkcommon.h:61-68) in FP32
@@ -218,8 +219,16 @@ uint64_t ora_param_count(const nrrs_config* cfg, int32_t net, uint64_t* n_grid, 
     return ng + nm;
 }
 
-void ora_encode(const nrrs_config* cfg, int32_t net, uint32_t n, const nrrs_vertices* v,
-                const float* film, const double* stats, const double* params, double* enc) {
+}  // extern "C"
+
+namespace {
+
+// Encoding of n vertices [n][80]. `training`: the RRSNet input of a training step, which with
+// train_sigma takes activationSigma(max(L2, 0)) (nrrs_generate_training_data_from_0to1,
+// src/render/nrrs/train.cu:196-200) where the inference input takes max(L2, 0)
+// (nrrs_generate_inference_data_from_0to1, train.cu:41-112).
+void encode_impl(const nrrs_config* cfg, int32_t net, uint32_t n, const nrrs_vertices* v, const float* film,
+                 const double* stats, const double* params, double* enc, bool training) {
     const nrrs_grid_config& g = cfg->grid;
     std::vector<Level> L = levels_of(g, nullptr);
     float c[3], d[3];
@@ -273,7 +282,9 @@ void ora_encode(const nrrs_config* cfg, int32_t net, uint32_t n, const nrrs_vert
             for (int k = 0; k < 3; ++k) {
                 if (stats) {
                     e[69 + k] = bc_d(std::max(stats[(size_t)k * n + i], 0.0));
-                    e[72 + k] = bc_d(std::max(stats[(size_t)(3 + k) * n + i], 0.0));
+                    double l2 = std::max(stats[(size_t)(3 + k) * n + i], 0.0);
+                    if (training && cfg->train_sigma) l2 = sigma_d(l2);
+                    e[72 + k] = bc_d(l2);
                 } else {
                     e[69 + k] = v->albedo_d[k * S + i];
                     e[72 + k] = v->albedo_s[k * S + i];
@@ -282,6 +293,20 @@ void ora_encode(const nrrs_config* cfg, int32_t net, uint32_t n, const nrrs_vert
             e[75] = 1.0;
         }
     }
+}
+
+}  // namespace
+
+extern "C" {
+
+void ora_encode(const nrrs_config* cfg, int32_t net, uint32_t n, const nrrs_vertices* v,
+                const float* film, const double* stats, const double* params, double* enc) {
+    encode_impl(cfg, net, n, v, film, stats, params, enc, false);
+}
+
+void ora_encode_train(const nrrs_config* cfg, int32_t net, uint32_t n, const nrrs_vertices* v,
+                      const float* film, const double* stats, const double* params, double* enc) {
+    encode_impl(cfg, net, n, v, film, stats, params, enc, true);
 }
 
 void ora_net_forward(const nrrs_config* cfg, int32_t net, uint32_t n, const nrrs_vertices* v,
@@ -480,7 +505,10 @@ namespace {
 // relative L2 of the 3 (step 0) or 6 (step >= 1) outputs against t = (L, L^2), L
 // clamped to loss_clamp_max when loss_clamp_on; the denominator y^2 + 0.01 is a
 // constant of the gradient (relative-L2 convention). Loss = mean over samples and
-// active channels.
+// active channels. With train_sigma, output 3 + c is a sigma: its prediction of L^2 is
+// m = activationSigma(y_{3+c})^2 + y_c^2 (as the integrator reads the outputs,
+// integrator.cpp:655-670), scored as (m - L^2)^2 / (m^2 + 0.01) with y_c and the
+// denominator constants of the gradient. y_den supplies those constants (null: y itself).
 double loss_ll2(const nrrs_config* cfg, uint32_t n, const double* y, const float* tL, const double* y_den,
                 double* dy) {
     const int active = cfg->ll2_step == 0 ? 3 : 6;
@@ -492,12 +520,22 @@ double loss_ll2(const nrrs_config* cfg, uint32_t n, const double* y, const float
             if (cfg->loss_clamp_on) t = std::fmin(t, (double)cfg->loss_clamp_max);
             if (c >= 3) t = t * t;
             double yy = y[(size_t)c * n + i];
-            double yd = y_den ? y_den[(size_t)c * n + i] : yy;
+            const double* yc = y_den ? y_den : y;  // constants of the gradient
+            double yd = yc[(size_t)c * n + i];
             double den = yd * yd + 0.01;
             double d = yy - t;
+            double slope = 1.0;  // d(prediction)/dy
+            if (c >= 3 && cfg->train_sigma) {
+                double mu = yc[(size_t)(c - 3) * n + i];
+                double sg = sigma_d(yy), sgd = sigma_d(yd);
+                double m = sg * sg + mu * mu, md = sgd * sgd + mu * mu;
+                den = md * md + 0.01;
+                d = m - t;
+                slope = 2.0 * sg * sigma_grad_d(yy);
+            }
             if (c < active) {
                 loss += d * d / den * norm;
-                if (dy) dy[(size_t)c * n + i] = 2.0 * d / den * norm;
+                if (dy) dy[(size_t)c * n + i] = 2.0 * d / den * slope * norm;
             } else if (dy) {
                 dy[(size_t)c * n + i] = 0.0;
             }
@@ -506,11 +544,11 @@ double loss_ll2(const nrrs_config* cfg, uint32_t n, const double* y, const float
     return loss;
 }
 
-double sigma_grad_d(double x) { return x < 0 ? std::exp(x) / (1.0 + std::exp(x)) : 0.5; }
-
 // NRRS_RRS restated (nrrs_rrs.h is not in the reference checkout). With q = sigma(y),
 // v = mean_c(thp_c^2 max(L2_c,0)) / (ref^2 + 0.01) (clamped at loss_clamp_max when
-// loss_clamp_on), E = max(error_sum / pixels, 1e-4), e = max(error, 1e-4):
+// loss_clamp_on; with train_sigma the second moment is activationSigma(s_{3+c})^2 +
+// max(s_c, 0)^2 of the StatNet outputs s), E = max(error_sum / pixels, 1e-4),
+// e = max(error, 1e-4):
 //   f(q) = g1 (v/(q+0.01) + q E) + g2 (v/(q+0.01) + q e) + g3 q + g4 (q - q_t)^2
 // loss = (1/n) sum over samples with q_t >= 0 of sample_weight * f(q); zero for step < 3.
 double loss_rrs(const nrrs_config* cfg, uint32_t n, const double* y, const nrrs_vertices* v,
@@ -535,7 +573,12 @@ double loss_rrs(const nrrs_config* cfg, uint32_t n, const double* y, const nrrs_
         double m2 = 0;
         for (int c = 0; c < 3; ++c) {
             double t = v->thp[c * S + i];
-            m2 += t * t * std::fmax(stats[(size_t)(3 + c) * n + i], 0.0);
+            double l2 = std::fmax(stats[(size_t)(3 + c) * n + i], 0.0);
+            if (cfg->train_sigma) {
+                double sg = sigma_d(stats[(size_t)(3 + c) * n + i]), mu = std::fmax(stats[(size_t)c * n + i], 0.0);
+                l2 = sg * sg + mu * mu;
+            }
+            m2 += t * t * l2;
         }
         double vv = m2 / 3.0 / (ref * ref + 0.01);
         if (cfg->loss_clamp_on) vv = std::fmin(vv, (double)cfg->loss_clamp_max);
@@ -579,7 +622,7 @@ double ora_train_grads(const nrrs_config* cfg, int32_t net, uint32_t n, const nr
     mlp_dims(cfg, net, &nl, &no);
     std::fill(grad, grad + ng + nm, 0.0);
     std::vector<double> enc((size_t)n * ORA_ENC_WIDTH);
-    ora_encode(cfg, net, n, v, film, stats, params, enc.data());
+    encode_impl(cfg, net, n, v, film, stats, params, enc.data(), true);
     const double* W = params + ng;
     double* dW = grad + ng;
     // forward, keeping every layer input

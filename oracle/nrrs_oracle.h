@@ -67,9 +67,13 @@ double ora_box_cox(double x);
 double ora_weight_window(double q, double size);
 void ora_sh4(double x, double y, double z, double* out16);
 
-/* enc: [n][80]; params: [grid | mlp] of that net (FP64); stats: [6][n] or NULL. */
+/* enc: [n][80]; params: [grid | mlp] of that net (FP64); stats: [6][n] or NULL. ora_encode
+ * is the inference input, ora_encode_train the input of a training step (they differ in the
+ * RRSNet L2 channels when train_sigma is set). */
 void ora_encode(const nrrs_config* cfg, int32_t net, uint32_t n, const nrrs_vertices* v,
                 const float* film, const double* stats, const double* params, double* enc);
+void ora_encode_train(const nrrs_config* cfg, int32_t net, uint32_t n, const nrrs_vertices* v,
+                      const float* film, const double* stats, const double* params, double* enc);
 
 /* out: [n][n_outputs] */
 void ora_net_forward(const nrrs_config* cfg, int32_t net, uint32_t n, const nrrs_vertices* v,

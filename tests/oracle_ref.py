@@ -27,6 +27,7 @@ _SIGS = {
     "ora_weight_window": (C.c_double, [C.c_double, C.c_double]),
     "ora_sh4": (None, [C.c_double, C.c_double, C.c_double, _P]),
     "ora_encode": (None, [C.POINTER(na.Config), C.c_int32, C.c_uint32, C.POINTER(na.Vertices), _P, _P, _P, _P]),
+    "ora_encode_train": (None, [C.POINTER(na.Config), C.c_int32, C.c_uint32, C.POINTER(na.Vertices), _P, _P, _P, _P]),
     "ora_net_forward": (None, [C.POINTER(na.Config), C.c_int32, C.c_uint32, C.POINTER(na.Vertices), _P, _P, _P, _P]),
     "ora_param_count": (C.c_uint64, [C.POINTER(na.Config), C.c_int32, C.POINTER(C.c_uint64), C.POINTER(C.c_uint64)]),
     "ora_q": (None, [C.POINTER(na.Config), C.c_int32, C.c_uint32, _P, C.POINTER(na.Vertices), _P, _P]),
@@ -124,13 +125,15 @@ def grid_indices(cfg, hv):
     return idx
 
 
-def encode(cfg, net, hv, film, stats, params):
+def encode(cfg, net, hv, film, stats, params, training=False):
+    """Network input [n][80]; `training`: the input of a training step (with train_sigma the
+    RRSNet L2 channels take activationSigma(max(L2, 0)) there)."""
     enc = np.zeros((hv.n, na.ENC_WIDTH), np.float64)
     film = None if film is None else np.ascontiguousarray(film, np.float32)
     stats = None if stats is None else np.ascontiguousarray(stats, np.float64)
     params = np.ascontiguousarray(params, np.float64)
-    load().ora_encode(C.byref(cfg), net, hv.n, C.byref(hv.v), _p(film), _p(stats), params.ctypes.data,
-                      enc.ctypes.data)
+    fn = load().ora_encode_train if training else load().ora_encode
+    fn(C.byref(cfg), net, hv.n, C.byref(hv.v), _p(film), _p(stats), params.ctypes.data, enc.ctypes.data)
     return enc
 
 

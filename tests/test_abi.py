@@ -45,5 +45,7 @@ def test_invalid_arguments_rejected_without_device():
     bad = na.default_config(grid=dict(n_features=4))
     h = C.c_void_p()
     assert lib.nrrs_grid_layout(C.byref(bad.grid), None, None, None, None, None, None) == na.ERR_ARG
-    unsupported = na.default_config(train_sigma=1)
+    unsupported = na.default_config(precision=7)
     assert lib.nrrs_create(C.byref(unsupported), C.byref(h)) == na.ERR_UNSUPPORTED
+    bad_sigma = na.default_config(train_sigma=2)
+    assert lib.nrrs_create(C.byref(bad_sigma), C.byref(h)) == na.ERR_ARG

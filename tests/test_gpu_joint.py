@@ -97,9 +97,21 @@ def _rel_norm(a, b):
 def test_joint_step_matches_fp64(torch_cuda, ora, prec, n, paired):
     if prec == na.FP32 and paired == 0:
         pytest.skip("FP32 contexts have no paired table")
+    _joint_step_vs_fp64(torch_cuda, ora, prec, n, paired)
+
+
+@pytest.mark.parametrize("prec", PRECISIONS)
+def test_joint_step_train_sigma_matches_fp64(torch_cuda, ora, prec):
+    """train_sigma = 1 (network.h:158): StatNet's outputs 3-5 are sigmas (NRRS_LL2 scores
+    sigma^2 + L^2 against L^2); RRSNet's training input takes activationSigma(max(L2, 0))
+    (train.cu:196-200) and NRRS_RRS the second moments sigma^2 + max(L, 0)^2."""
+    _joint_step_vs_fp64(torch_cuda, ora, prec, 4096, 1, train_sigma=1)
+
+
+def _joint_step_vs_fp64(torch_cuda, ora, prec, n, paired, **kw):
     torch = torch_cuda
     ctx = na.Context(na.default_config(precision=prec, n_pixels=n, max_queue=n, max_train=n, paired_tables=paired,
-                                       **LINEAR_ADAM))
+                                       **LINEAR_ADAM, **kw))
     vb, film_np, tL, side = _batch(torch, ora, ctx, n, seed=n % 97)
     p0 = [ctx.params_get(net).astype(np.float64) for net in (na.STATNET, na.RRSNET)]
     loss = _device_step(torch, ctx, vb, film_np, tL, side)

@@ -178,9 +178,13 @@ def test_low_precision_backward_ragged_batch(torch_cuda, ora, n):
 
 
 @pytest.mark.parametrize("prec", PRECISIONS)
-def test_ll2_and_rrs_loss_backward(torch_cuda, ora, prec):
+@pytest.mark.parametrize("train_sigma", [0, 1])
+def test_ll2_and_rrs_loss_backward(torch_cuda, ora, prec, train_sigma):
+    """Both losses through nrrs_backward against the FP64 oracle; with train_sigma the StatNet
+    outputs 3-5 are sigmas (LL2 scores sigma^2 + L^2 against L^2) and RRSNet's training input
+    and loss read them as such (statistics of both signs)."""
     torch = torch_cuda
-    ctx, vb, film_np, film = _setup(torch, prec, seed=2)
+    ctx, vb, film_np, film = _setup(torch, prec, seed=2, train_sigma=train_sigma)
     hv = ora.HostVertices(vb.to_numpy())
     tL = ora.synth_targets(N, 4)
     loss = torch.zeros(1, device="cuda")
@@ -196,6 +200,8 @@ def test_ll2_and_rrs_loss_backward(torch_cuda, ora, prec):
 
     sd = _rrs_side_np(N, 5)
     stats_np = np.abs(np.random.default_rng(6).normal(0.5, 0.5, (6, N))).astype(np.float32)
+    if train_sigma:
+        stats_np[3:] = np.random.default_rng(7).normal(0.0, 1.5, (3, N))
     keep = {k: torch.from_numpy(v).cuda() for k, v in sd.items() if k != "error_sum"}
     side = ctx.side(error=keep["error"], sample_weight=keep["sample_weight"], q_target=keep["q_target"],
                     error_sum=sd["error_sum"])

@@ -149,7 +149,7 @@ def test_rrs_loss_pinned(ora):
 
 # Plausible mistakes, each a (find, replace) on the oracle source's loss_rrs.
 MUTANTS = {
-    "thp instead of thp^2": ("m2 += t * t * std::fmax", "m2 += t * std::fmax"),
+    "thp instead of thp^2": ("m2 += t * t * l2;", "m2 += t * l2;"),
     "channel sum instead of mean": ("double vv = m2 / 3.0 / (ref * ref + 0.01);",
                                     "double vv = m2 / (ref * ref + 0.01);"),
     "ref instead of ref^2": ("double vv = m2 / 3.0 / (ref * ref + 0.01);", "double vv = m2 / 3.0 / (ref + 0.01);"),
@@ -161,8 +161,8 @@ MUTANTS = {
                                  "gam[0] * (vv / qe + q * e) + gam[1] * (vv / qe + q * Ebar)"),
     "no +0.01 on q": ("double qe = q + 0.01;", "double qe = q;"),
     "sample weight dropped": ("loss += sw[i] * f / n;", "loss += f / n;"),
-    "negative L2 kept": ("m2 += t * t * std::fmax(stats[(size_t)(3 + c) * n + i], 0.0);",
-                         "m2 += t * t * stats[(size_t)(3 + c) * n + i];"),
+    "negative L2 kept": ("double l2 = std::fmax(stats[(size_t)(3 + c) * n + i], 0.0);",
+                         "double l2 = stats[(size_t)(3 + c) * n + i];"),
 }
 
 
