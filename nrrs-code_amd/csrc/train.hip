@@ -31,6 +31,33 @@ typedef float f32x8 __attribute__((ext_vector_type(8)));
 typedef float f32x16 __attribute__((ext_vector_type(16)));
 typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
 typedef _Float16 h2v __attribute__((ext_vector_type(2)));
+typedef short i16x8 __attribute__((ext_vector_type(8)));
+typedef unsigned short u16x8 __attribute__((ext_vector_type(8)));
+
+// 16-bit operands: ReLU on the rounded values (a negative FP16/BF16 value has its sign bit set,
+// so as an int16 it is negative; rounding keeps the sign, so this is round(max(x, 0)) for every
+// finite x), and delta * [h > 0] for ReLU outputs h (h > 0 iff its bits are nonzero; a positive
+// input that rounds to zero, below 2^-25 in FP16, counts as inactive).
+template <typename OP>
+__device__ __forceinline__ OP relu16(OP v) {
+    return __builtin_bit_cast(OP, __builtin_elementwise_max(__builtin_bit_cast(i16x8, v), i16x8{}));
+}
+// 0xffff in each 16-bit half of h that is nonzero: min(half, 1), then 0 - that. Written as
+// packed 16-bit VALU operations in asm, since the compiler otherwise rewrites the mask into
+// per-element FP32 compares and selects.
+__device__ __forceinline__ uint32_t nonzero16_mask(uint32_t h) {
+    uint32_t m;
+    asm("v_pk_min_u16 %0, %1, %2\n\tv_pk_sub_u16 %0, 0, %0" : "=&v"(m) : "v"(h), "s"(0x00010001u));
+    return m;
+}
+template <typename OP>
+__device__ __forceinline__ OP mask16(OP d, OP h) {
+    u32x4 dd = __builtin_bit_cast(u32x4, d);
+    const u32x4 hh = __builtin_bit_cast(u32x4, h);
+#pragma unroll
+    for (int k = 0; k < 4; ++k) dd[k] &= nonzero16_mask(hh[k]);
+    return __builtin_bit_cast(OP, dd);
+}
 
 // Operand policies: 8 values per lane per 16-deep K chunk.
 // kWaves: waves per workgroup. FP16/BF16 use 8 (one workgroup of 8 waves per CU, two
@@ -47,6 +74,8 @@ struct TBF16 {
         for (int j = 0; j < 8; ++j) v[j] = (__bf16)x[j];
         return v;
     }
+    __device__ static op relu_pack(const float* x) { return relu16(pack(x)); }
+    __device__ static op mask_by(op d, op h) { return mask16(d, h); }
     __device__ static float get(const op& v, int j) { return (float)v[j]; }
     __device__ static f32x16 mma16(op a, op b, f32x16 c) {
         return __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, b, c, 0, 0, 0);
@@ -66,6 +95,8 @@ struct TF16 {
         for (int j = 0; j < 8; ++j) v[j] = (_Float16)f32_rounded(x[j]);
         return v;
     }
+    __device__ static op relu_pack(const float* x) { return relu16(pack(x)); }
+    __device__ static op mask_by(op d, op h) { return mask16(d, h); }
     __device__ static float get(const op& v, int j) { return (float)v[j]; }
     __device__ static f32x16 mma16(op a, op b, f32x16 c) {
         return __builtin_amdgcn_mfma_f32_32x32x16_f16(a, b, c, 0, 0, 0);
@@ -86,6 +117,17 @@ struct TF32 {
 #pragma unroll
         for (int j = 0; j < 8; ++j) v[j] = x[j];
         return v;
+    }
+    __device__ static op relu_pack(const float* x) {
+        op v;
+#pragma unroll
+        for (int j = 0; j < 8; ++j) v[j] = fmaxf(x[j], 0.0f);
+        return v;
+    }
+    __device__ static op mask_by(op d, op h) {
+#pragma unroll
+        for (int j = 0; j < 8; ++j) d[j] = h[j] > 0.0f ? d[j] : 0.0f;
+        return d;
     }
     __device__ static float get(const op& v, int j) { return v[j]; }
     // A 16-deep chunk as 8 K=2 MFMAs: step s pairs k = s (lanes 0-31) with k = 8 + s
@@ -164,39 +206,33 @@ __device__ __forceinline__ void fwd_layer(const typename P::op* __restrict__ Wf,
     }
 }
 
-// ReLU of accumulators into operands; with MASK, also the mask with bit 16*ot + r set when
-// D[ot][r] > 0 (the backward needs it for the last layer of a recompute only).
-template <typename P, bool MASK = true>
-__device__ __forceinline__ uint32_t relu_ops(const f32x16* D, typename P::op* H) {
-    uint32_t mask = 0;
+// ReLU of accumulators into operands (the ReLU outputs double as the mask [h > 0] of the
+// backward, P::mask_by).
+template <typename P>
+__device__ __forceinline__ void relu_ops(const f32x16* D, typename P::op* H) {
 #pragma unroll
     for (int ot = 0; ot < 2; ++ot) {
 #pragma unroll
         for (int g = 0; g < 2; ++g) {
             float x[8];
 #pragma unroll
-            for (int j = 0; j < 8; ++j) {
-                float v = D[ot][8 * g + j];
-                if (MASK) mask |= (v > 0.0f ? 1u : 0u) << (16 * ot + 8 * g + j);
-                x[j] = fmaxf(v, 0.0f);
-            }
-            H[2 * ot + g] = P::pack(x);
+            for (int j = 0; j < 8; ++j) x[j] = D[ot][8 * g + j];
+            H[2 * ot + g] = P::relu_pack(x);
         }
     }
-    return mask;
 }
 
 // Input of layer `upto` (>= 1): forward through layers 0..upto-1 from the encoding.
 template <typename P, int NL>
-__device__ __forceinline__ uint32_t recompute(const typename P::op* __restrict__ Wf, int upto, const typename P::op* E,
-                                              typename P::op* H, int lane) {
+__device__ __forceinline__ void recompute(const typename P::op* __restrict__ Wf, int upto, const typename P::op* E,
+                                          typename P::op* H, int lane) {
     f32x16 D[2];
     fwd_layer<P, NL>(Wf, 0, E, D, lane);
     for (int layer = 1; layer < upto; ++layer) {
-        relu_ops<P, false>(D, H);
+        relu_ops<P>(D, H);
         fwd_layer<P, NL>(Wf, layer, H, D, lane);
     }
-    return relu_ops<P, true>(D, H);  // mask of the last layer only
+    relu_ops<P>(D, H);
 }
 
 // BIN: dL/denc of the grid goes to t.gdl for the binned scatter (FP16/BF16); otherwise it is
@@ -211,7 +247,10 @@ __global__ __launch_bounds__(64 * P::kWaves) void k_train(TrainArgs t) {
     Stage<P>& st = *reinterpret_cast<Stage<P>*>(smem_raw);
     __shared__ double loss_part[kWaves];  // loss per wave (FP64, summed in a fixed order)
 
-    const int lane = threadIdx.x & 63, h = lane >> 5, col = lane & 31, wave = threadIdx.x >> 6;
+    // the wave index in a scalar register: tile ownership and slot choices below are then
+    // scalar branches
+    const int lane = threadIdx.x & 63, h = lane >> 5, col = lane & 31;
+    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     // FP16/BF16: the packed forward and W^T tiles live in LDS behind the stage (they are read
     // by every layer of every iteration); FP32: from global memory (cache)
     constexpr int kTilesF = 10 + (NL - 2) * 8 + 4, kTilesB = 4 + (NL - 2) * 8 + 2;
@@ -297,7 +336,7 @@ __global__ __launch_bounds__(64 * P::kWaves) void k_train(TrainArgs t) {
 
         // ---- forward to the output, loss, dL/dy
         op H[4];
-        uint32_t mask = recompute<P, NL>(Wf, NL - 1, E, H, lane);
+        recompute<P, NL>(Wf, NL - 1, E, H, lane);
         f32x16 Y = f32x16{};
         {
             const int base = 10 + (NL - 2) * 8;
@@ -429,7 +468,7 @@ __global__ __launch_bounds__(64 * P::kWaves) void k_train(TrainArgs t) {
         for (int layer = NL - 1; layer >= 0; --layer) {
             const bool last = layer == NL - 1;
             // (1) layer input h_l and its ReLU mask
-            if (!last && layer > 0) mask = recompute<P, NL>(Wf, layer, E, H, lane);
+            if (!last && layer > 0) recompute<P, NL>(Wf, layer, E, H, lane);
             // (2) stage delta_l rows [out neuron][vertex] and h_l rows [in neuron][vertex]
             __syncthreads();  // previous layer's dW reads are done
             if (last) {
@@ -466,17 +505,19 @@ __global__ __launch_bounds__(64 * P::kWaves) void k_train(TrainArgs t) {
                 for (int g = dw_first(base, wave, kWaves); g < base + n_tiles; g += kWaves) {
                     const int tt = g - base, slot = g / kWaves;
                     const int to = tt / n_ti, ti = tt % n_ti;
-                    f32x16 a = f32x16{};
+                    // the MFMAs accumulate straight into the tile's register slot; the slot is
+                    // wave-uniform, so its choice is a scalar branch
 #pragma unroll
-                    for (int s = 0; s < kVert / 16; ++s) {
-                        op A = stage_op<P>(st.d, to * 32 + col, 16 * s + 8 * h);
-                        op B = stage_op<P>(st.h, ti * 32 + col, 16 * s + 8 * h);
-                        a = P::mma16(A, B, a);
+                    for (int s2 = 0; s2 < kSlots; ++s2) {
+                        if (s2 == slot) {
+#pragma unroll
+                            for (int s = 0; s < kVert / 16; ++s) {
+                                op A = stage_op<P>(st.d, to * 32 + col, 16 * s + 8 * h);
+                                op B = stage_op<P>(st.h, ti * 32 + col, 16 * s + 8 * h);
+                                acc[s2] = P::mma16(A, B, acc[s2]);
+                            }
+                        }
                     }
-                    // registers are indexed statically: select the slot by a switch
-#pragma unroll
-                    for (int s2 = 0; s2 < kSlots; ++s2)
-                        if (s2 == slot) acc[s2] += a;
                 }
             }
             // (4) propagate delta to the layer input
@@ -501,9 +542,8 @@ __global__ __launch_bounds__(64 * P::kWaves) void k_train(TrainArgs t) {
                     for (int g = 0; g < 2; ++g) {
                         float x8[8];
 #pragma unroll
-                        for (int j = 0; j < 8; ++j)
-                            x8[j] = ((mask >> (16 * ot + 8 * g + j)) & 1u) ? D[ot][8 * g + j] : 0.0f;
-                        Dl[2 * ot + g] = P::pack(x8);
+                        for (int j = 0; j < 8; ++j) x8[j] = D[ot][8 * g + j];
+                        Dl[2 * ot + g] = P::mask_by(P::pack(x8), H[2 * ot + g]);
                     }
             } else {
                 // dL/denc for the 32 grid channels: rows c = 8(r/4) + 4h + r%4
