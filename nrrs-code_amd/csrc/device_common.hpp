@@ -560,14 +560,16 @@ __device__ __forceinline__ void encode_tail_rec(const VertexRec& v, int net, flo
 // Non-grid channels of vertex i of a SoA batch.
 __device__ __forceinline__ void encode_tail(const EncodeArgs& a, int net, uint32_t i, float* e) {
     VertexRec v = vertex_from_soa(a, i);
-    float st[6];
-    const bool have_st = net == NRRS_RRSNET && a.stats;
-    if (have_st) {
+    const float ref = net == NRRS_RRSNET ? film_mean(a.film, a.n_pixels, v.pixel) : 0.0f;
+    // two calls rather than a selected pointer, which would put st in memory
+    if (net == NRRS_RRSNET && a.stats) {
+        float st[6];
 #pragma unroll
         for (int k = 0; k < 6; ++k) st[k] = a.stats[(size_t)k * a.stats_stride + i];
+        encode_tail_rec(v, net, ref, st, e, a.sigma_l2 != 0);
+    } else {
+        encode_tail_rec(v, net, ref, nullptr, e);
     }
-    const float ref = net == NRRS_RRSNET ? film_mean(a.film, a.n_pixels, v.pixel) : 0.0f;
-    encode_tail_rec(v, net, ref, have_st ? st : nullptr, e, a.sigma_l2 != 0);
 }
 
 template <typename TT>

@@ -161,14 +161,15 @@ template <typename P>
 __device__ __forceinline__ void encode_half(const EncodeArgs& a, int net, const uint32_t* __restrict__ table,
                                             uint32_t i, int h, typename P::vec8* B, typename P::vec8* Bl = nullptr) {
     VertexRec v = vertex_from_soa(a, i);
-    float st[6];
-    const bool have_st = net == NRRS_RRSNET && a.stats;
-    if (have_st) {
+    const float ref = net == NRRS_RRSNET ? film_mean(a.film, a.n_pixels, v.pixel) : 0.0f;
+    if (net == NRRS_RRSNET && a.stats) {  // two calls rather than a selected pointer (st stays in registers)
+        float st[6];
 #pragma unroll
         for (int k = 0; k < 6; ++k) st[k] = a.stats[(size_t)k * a.stats_stride + i];
+        encode_half_rec<P>(v, a.grid, net, table, h, ref, st, B, Bl);
+    } else {
+        encode_half_rec<P>(v, a.grid, net, table, h, ref, nullptr, B, Bl);
     }
-    const float ref = net == NRRS_RRSNET ? film_mean(a.film, a.n_pixels, v.pixel) : 0.0f;
-    encode_half_rec<P>(v, a.grid, net, table, h, ref, have_st ? st : nullptr, B, Bl);
 }
 
 // Grid part of the B operands from the level-major encoding: chunk kb of lane half h holds
@@ -193,15 +194,16 @@ template <typename P>
 __device__ __forceinline__ void encode_half_pre(const EncodeArgs& a, int net, uint32_t i, int h, typename P::vec8* B,
                                                 typename P::vec8* Bl = nullptr) {
     VertexRec v = vertex_from_soa(a, i);
-    float st[6];
-    const bool have_st = net == NRRS_RRSNET && a.stats;
-    if (have_st) {
-#pragma unroll
-        for (int k = 0; k < 6; ++k) st[k] = a.stats[(size_t)k * a.stats_stride + i];
-    }
     const float ref = net == NRRS_RRSNET ? film_mean(a.film, a.n_pixels, v.pixel) : 0.0f;
     grid_half_from_levels<P>(a.grid_enc, a.grid_enc_stride, a.grid.n_levels, i, h, B);
-    encode_tail_half<P>(v, net, h, ref, have_st ? st : nullptr, B, Bl);
+    if (net == NRRS_RRSNET && a.stats) {  // two calls rather than a selected pointer (st stays in registers)
+        float st[6];
+#pragma unroll
+        for (int k = 0; k < 6; ++k) st[k] = a.stats[(size_t)k * a.stats_stride + i];
+        encode_tail_half<P>(v, net, h, ref, st, B, Bl);
+    } else {
+        encode_tail_half<P>(v, net, h, ref, nullptr, B, Bl);
+    }
 }
 
 __device__ __forceinline__ u32x4 lds_tile(const u32x4* __restrict__ sW, int tile, int lane) {

@@ -10,8 +10,9 @@
 // p(kb,h,j) = 16kb + 8(j/4) + 4h + j%4.
 //
 // Backward sweep, layer l = L-1 .. 0:
-//   1. recompute the layer input h_l from the encoding (only the encoding and one
-//      layer of activations are ever live: activations are recomputed, not stored);
+//   1. recompute the layer input h_l from the encoding, or from the input of layer 2 kept
+//      from the forward (only the encoding, that checkpoint and one layer of activations
+//      are ever live: activations are recomputed, not stored);
 //   2. stage delta_l and h_l of the workgroup's vertices in LDS as [neuron][vertex];
 //   3. dW_l += delta_l h_l^T on MFMA with the vertices as the contraction index,
 //      each wave owning a fixed subset of the 32x32 dW tiles in its registers for
@@ -222,18 +223,26 @@ __device__ __forceinline__ void relu_ops(const f32x16* D, typename P::op* H) {
     }
 }
 
-// Input of layer `upto` (>= 1): forward through layers 0..upto-1 from the encoding.
+// Input of layer `upto` (> from): forward through layers from..upto-1, starting from X, the
+// input of layer `from` (the encoding when from = 0).
 template <typename P, int NL>
-__device__ __forceinline__ void recompute(const typename P::op* __restrict__ Wf, int upto, const typename P::op* E,
-                                          typename P::op* H, int lane) {
+__device__ __forceinline__ void forward_to(const typename P::op* __restrict__ Wf, int from, const typename P::op* X,
+                                           int upto, typename P::op* H, int lane) {
     f32x16 D[2];
-    fwd_layer<P, NL>(Wf, 0, E, D, lane);
-    for (int layer = 1; layer < upto; ++layer) {
+    fwd_layer<P, NL>(Wf, from, X, D, lane);
+    for (int layer = from + 1; layer < upto; ++layer) {
         relu_ops<P>(D, H);
         fwd_layer<P, NL>(Wf, layer, H, D, lane);
     }
     relu_ops<P>(D, H);
 }
+
+// The input of this layer is kept from the forward pass (a checkpoint) for the backward sweep:
+// the inputs of the layers above it are recomputed from it, those below from the encoding. Of
+// the recompute the StatNet (5 layers) then runs 2 layers per iteration instead of 6, the
+// RRSNet (4 layers) 1 instead of 3. 0: no checkpoint.
+template <int NL>
+constexpr int kCkpt = NL >= 4 ? 2 : 0;
 
 // BIN: dL/denc of the grid goes to t.gdl for the binned scatter (FP16/BF16); otherwise it is
 // added to the table with atomics. Separate instances keep the unused path's registers out.
@@ -336,7 +345,19 @@ __global__ __launch_bounds__(64 * P::kWaves) void k_train(TrainArgs t) {
 
         // ---- forward to the output, loss, dL/dy
         op H[4];
-        recompute<P, NL>(Wf, NL - 1, E, H, lane);
+        constexpr int C = kCkpt<NL>;
+        op Hc[4];  // input of layer C (the checkpoint)
+        if constexpr (C > 0) {
+            forward_to<P, NL>(Wf, 0, E, C, Hc, lane);
+            if constexpr (NL - 1 > C) {
+                forward_to<P, NL>(Wf, C, Hc, NL - 1, H, lane);
+            } else {
+#pragma unroll
+                for (int kb = 0; kb < 4; ++kb) H[kb] = Hc[kb];
+            }
+        } else {
+            forward_to<P, NL>(Wf, 0, E, NL - 1, H, lane);
+        }
         f32x16 Y = f32x16{};
         {
             const int base = 10 + (NL - 2) * 8;
@@ -373,7 +394,8 @@ __global__ __launch_bounds__(64 * P::kWaves) void k_train(TrainArgs t) {
                         if (c >= 3 && t.train_sigma) {
                             // second moment sigma^2 + mu^2 against L^2, mu (output c - 3) held
                             // constant: the gradient reaches only the sigma output
-                            const float mu = c == 3 ? Y[0] : y_mu[c - 3];
+                            // (indices static in r: a lane-dependent index would put y_mu in memory)
+                            const float mu = h == 0 ? Y[0] : (r == 0 ? y_mu[1] : y_mu[2]);
                             const float sg = activation_sigma(y);
                             const float m2 = sg * sg + mu * mu;
                             const float den = m2 * m2 + 0.01f;
@@ -468,7 +490,16 @@ __global__ __launch_bounds__(64 * P::kWaves) void k_train(TrainArgs t) {
         for (int layer = NL - 1; layer >= 0; --layer) {
             const bool last = layer == NL - 1;
             // (1) layer input h_l and its ReLU mask
-            if (!last && layer > 0) recompute<P, NL>(Wf, layer, E, H, lane);
+            if (!last && layer > 0) {
+                if (C > 0 && layer > C) {
+                    forward_to<P, NL>(Wf, C, Hc, layer, H, lane);
+                } else if (C > 0 && layer == C) {
+#pragma unroll
+                    for (int kb = 0; kb < 4; ++kb) H[kb] = Hc[kb];
+                } else {
+                    forward_to<P, NL>(Wf, 0, E, layer, H, lane);
+                }
+            }
             // (2) stage delta_l rows [out neuron][vertex] and h_l rows [in neuron][vertex]
             __syncthreads();  // previous layer's dW reads are done
             if (last) {
