@@ -486,7 +486,7 @@ __global__ __launch_bounds__(64 * P::kWaves) void k_train(TrainArgs t) {
         op Dl[4];  // delta of the current layer as operands (k-chunks of its output neurons)
 #pragma unroll
         for (int kb = 0; kb < 4; ++kb) Dl[kb] = DY;  // only Dl[0] is used for the output layer
-#pragma unroll 1
+#pragma unroll
         for (int layer = NL - 1; layer >= 0; --layer) {
             const bool last = layer == NL - 1;
             // (1) layer input h_l and its ReLU mask
