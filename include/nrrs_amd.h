@@ -118,6 +118,17 @@ typedef struct {
      * gathers all levels of a vertex inside the network kernel. Results are bitwise the same.
      * No reference counterpart. */
     uint32_t fp32_level_major;
+    /* Two-headed StatNet (ll2_use_two_head / ll2_two_head_type, myNetworkLL2.cpp:21-29,
+     * 296-322): 0 = one head (default); 1 = PartShare (twn_PartShare.cpp: shared encoding and
+     * base net, a tail per head), with statnet.n_hidden_layers = 3 (base: 2 layers, tails: 2);
+     * 2 = JustEncoding (twn_JustEncoding.cpp: shared encoding, a net per head), with
+     * statnet.n_hidden_layers = 2. Each head is 32 wide; the two heads of a layer form one
+     * 64-wide layer whose weights between the heads are held at zero (set to zero by
+     * nrrs_params_set, never trained). Outputs 0-2 (L) come from head 1, 3-5 (L^2) from head 2;
+     * NRRS_LL2 over the 6 outputs is the mean of the two heads' losses. AllSelf (3: two nets
+     * with their own encodings) returns NRRS_ERR_UNSUPPORTED, and so does the frame kernel
+     * (nrrs_frame_*) for two-headed StatNets. */
+    int32_t ll2_two_head;
 } nrrs_config;
 
 /* SoA path vertices of one bounce (replaces NRRSInferenceWorkItem, workitem.h:9-15). */

@@ -144,9 +144,28 @@ uint64_t mlp_count(const nrrs_mlp_config& m) {
 }
 
 bool valid_shapes(const nrrs_config& c) {
-    // Shapes the fused kernels are compiled for.
-    return c.statnet.n_hidden_layers == 4 && c.statnet.n_outputs == 6 &&
+    // Shapes the fused kernels are compiled for: StatNet 4 hidden -> 6 (one head), 3 hidden
+    // (PartShare) or 2 hidden (JustEncoding) -> 6; RRSNet 3 hidden -> 1.
+    static const uint32_t stat_hidden[3] = {4, 3, 2};
+    if (c.ll2_two_head < 0 || c.ll2_two_head > 2) return false;
+    return c.statnet.n_hidden_layers == stat_hidden[c.ll2_two_head] && c.statnet.n_outputs == 6 &&
            c.rrsnet.n_hidden_layers == 3 && c.rrsnet.n_outputs == 1;
+}
+
+// Zero the weights between the two heads of a two-headed StatNet (mlp: that net's MLP part).
+void zero_between_heads(const nrrs_config& c, int net, float* mlp) {
+    if (net != NRRS_STATNET || !c.ll2_two_head) return;
+    const int nl = (int)c.statnet.n_hidden_layers + 1, first = head_first_layer(c.ll2_two_head, nl);
+    size_t pos = 0;
+    int in = kEnc;
+    for (int layer = 0; layer < nl; ++layer) {
+        const int o = layer == nl - 1 ? (int)c.statnet.n_outputs : kHidden;
+        for (int r = 0; r < o; ++r)
+            for (int k = 0; k < in; ++k)
+                if (head_masked(first, layer, nl, r, k)) mlp[pos + (size_t)r * in + k] = 0.0f;
+        pos += (size_t)o * in;
+        in = o;
+    }
 }
 
 template <typename T>
@@ -157,7 +176,9 @@ nrrs_status dmalloc(T** p, size_t count) {
 }
 
 // Parameters of one net: grid U[-1e-4, 1e-4] (the tcnn grid initialisation), then
-// every MLP layer Xavier-uniform, from PCG32(seed, 100 + net).
+// every MLP layer Xavier-uniform, from PCG32(seed, 100 + net). A two-headed StatNet takes the
+// fan-in and fan-out of each head's own layer (32 wide, 3 outputs) from the split on, and
+// zeros between the heads.
 void init_params(const nrrs_config& c, int net, uint64_t n_grid, std::vector<float>& out) {
     const nrrs_mlp_config& m = net == NRRS_STATNET ? c.statnet : c.rrsnet;
     out.resize(n_grid + mlp_count(m));
@@ -166,12 +187,16 @@ void init_params(const nrrs_config& c, int net, uint64_t n_grid, std::vector<flo
     size_t pos = n_grid;
     int in = kEnc;
     const int nl = (int)m.n_hidden_layers + 1;
+    const int first = net == NRRS_STATNET ? head_first_layer(c.ll2_two_head, nl) : -1;
     for (int layer = 0; layer < nl; ++layer) {
         int o = layer == nl - 1 ? (int)m.n_outputs : kHidden;
-        double lim = std::sqrt(6.0 / (double)(in + o));
+        const bool split = first >= 0 && layer >= first;
+        const int fan_out = split ? o / 2 : o, fan_in = split && layer > first ? in / 2 : in;
+        double lim = std::sqrt(6.0 / (double)(fan_in + fan_out));
         for (int k = 0; k < o * in; ++k) out[pos++] = (float)(lim * (2.0 * r.uniform() - 1.0));
         in = o;
     }
+    zero_between_heads(c, net, out.data() + n_grid);
 }
 
 EncodeArgs encode_args(nrrs_ctx* ctx, const nrrs_vertices& v, const float* film, const float* stats,
@@ -443,6 +468,7 @@ nrrs_status run_backward(nrrs_ctx* c, int net, uint32_t n, const nrrs_vertices& 
     // (train.cu:196-200); inference keeps max(L2, 0)
     t.a.sigma_l2 = cfg.train_sigma && net == NRRS_RRSNET;
     t.train_sigma = cfg.train_sigma;
+    t.head_first = net == NRRS_STATNET ? head_first_layer(cfg.ll2_two_head, ns.n_layers) : -1;
     if (cfg.precision != NRRS_FP32) {
         // level-major grid gathers of the forward, read back by the training kernel
         if (!c->grid_enc || n > c->grid_enc_cap) return NRRS_ERR_SHAPE;
@@ -611,6 +637,8 @@ nrrs_status alloc_frame(nrrs_ctx* c, uint32_t cap) {
 }
 
 nrrs_status frame_params(nrrs_ctx* c, const nrrs_frame_args* a, FrameParams* f) {
+    // the frame kernel runs the one-headed StatNet of the benchmark configuration
+    if (c->cfg.ll2_two_head) return NRRS_ERR_UNSUPPORTED;
     if (!a || !a->film_dev || !a->queue_dev || !a->sizes_dev || !a->status_dev) return NRRS_ERR_ARG;
     if (a->n_bounces < 1 || a->n_bounces > 16 || a->capacity == 0) return NRRS_ERR_ARG;
     if (a->capacity > c->cfg.max_queue) return NRRS_ERR_SHAPE;
@@ -795,6 +823,7 @@ nrrs_status nrrs_default_config(nrrs_config* c) {
     c->seed = 0x4E525253ull;
     c->paired_tables = 1;
     c->fp32_level_major = 1;
+    c->ll2_two_head = 0;
     return NRRS_OK;
 }
 
@@ -893,7 +922,7 @@ nrrs_status nrrs_create(const nrrs_config* cfg, nrrs_ctx** out) {
     if (st == NRRS_OK && cfg->max_train) st = ensure_train(c, NRRS_RRSNET);
     if (st == NRRS_OK) st = ensure_nodes(c);
     if (st == NRRS_OK) st = dmalloc(&c->err_part, error_partial_count());
-    if (st == NRRS_OK && cfg->precision != NRRS_FP32) st = alloc_frame(c, cfg->max_queue);
+    if (st == NRRS_OK && cfg->precision != NRRS_FP32 && !cfg->ll2_two_head) st = alloc_frame(c, cfg->max_queue);
     if (st == NRRS_OK) st = hip_status(hipDeviceSynchronize());
     if (st != NRRS_OK) {
         free_ctx(c);
@@ -931,7 +960,13 @@ nrrs_status nrrs_params_set(nrrs_ctx* ctx, int32_t net, const float* host_src) {
     NetState& ns = ctx->net[net];
     NRRS_HIP(hipDeviceSynchronize());
     NRRS_HIP(hipMemcpy(ns.grid_f32, host_src, ns.n_grid * 4, hipMemcpyHostToDevice));
-    NRRS_HIP(hipMemcpy(ns.mlp_f32, host_src + ns.n_grid, ns.n_mlp * 4, hipMemcpyHostToDevice));
+    if (net == NRRS_STATNET && ctx->cfg.ll2_two_head) {  // the weights between the heads stay zero
+        std::vector<float> mlp(host_src + ns.n_grid, host_src + ns.n_grid + ns.n_mlp);
+        zero_between_heads(ctx->cfg, net, mlp.data());
+        NRRS_HIP(hipMemcpy(ns.mlp_f32, mlp.data(), ns.n_mlp * 4, hipMemcpyHostToDevice));
+    } else {
+        NRRS_HIP(hipMemcpy(ns.mlp_f32, host_src + ns.n_grid, ns.n_mlp * 4, hipMemcpyHostToDevice));
+    }
     NRRS_TRY(refresh_lp(ctx, net, nullptr));
     NRRS_TRY(refresh_train(ctx, net, nullptr));
     return hip_status(hipDeviceSynchronize());
@@ -994,8 +1029,8 @@ nrrs_status nrrs_rrs_infer_decide(nrrs_ctx* ctx, const nrrs_decide_args* a, void
     if (a->q_mode == NRRS_Q_STATNET_ADRRS) {
         EncodeArgs ea = encode_args(ctx, a->v, a->film_dev, nullptr, 0);
         NRRS_HIP(run_net(ctx, NRRS_STATNET, ea, EPI_ADRRS, a->n, ep, s));
-    } else if (a->q_mode == NRRS_Q_RRSNET_CHAINED && ctx->cfg.precision != NRRS_FP32 && ctx->grid_enc &&
-               ctx->grid_enc2 && a->n <= ctx->grid_enc_cap) {
+    } else if (a->q_mode == NRRS_Q_RRSNET_CHAINED && ctx->cfg.precision != NRRS_FP32 && !ctx->cfg.ll2_two_head &&
+               ctx->grid_enc && ctx->grid_enc2 && a->n <= ctx->grid_enc_cap) {
         // both nets' level-major grid encodings (one pass over the paired table, else a pass per
         // net), then StatNet -> RRSNet -> q in one launch
         const nrrs_config& cfg = ctx->cfg;
@@ -1018,7 +1053,7 @@ nrrs_status nrrs_rrs_infer_decide(nrrs_ctx* ctx, const nrrs_decide_args* a, void
                                     ctx->net[NRRS_RRSNET].mlp_lp, ep, s));
     } else if (a->q_mode == NRRS_Q_RRSNET || a->q_mode == NRRS_Q_RRSNET_CHAINED) {
         const float* stats = nullptr;
-        if (a->q_mode == NRRS_Q_RRSNET_CHAINED) {  // FP32, or a batch beyond the encoding buffers
+        if (a->q_mode == NRRS_Q_RRSNET_CHAINED) {  // FP32, two-headed StatNet, or a batch beyond the encoding buffers
             EncodeArgs ea = encode_args(ctx, a->v, nullptr, nullptr, 0);
             EpiArgs es{};
             es.out = ctx->stats_scratch;

@@ -128,7 +128,7 @@ __device__ __forceinline__ void store_position(const FrameParams& f, uint64_t se
 template <typename P, int WAVES>
 __global__ __launch_bounds__(64 * WAVES) void k_frame(FrameParams f) {
     constexpr int BLOCK = 64 * WAVES;
-    constexpr bool kSplit = split_of<P>(5);
+    constexpr bool kSplit = split_of<P>(6);
     constexpr int kTilesS = tiles_of(5, kSplit), kTilesR = tiles_of(4);
     extern __shared__ u32x4 smem[];
     u32x4* sWs = smem;
@@ -386,7 +386,7 @@ __global__ void k_bounce_vertices(uint64_t seed, uint32_t bounce, const uint32_t
 
 template <typename P, int WAVES>
 hipError_t launch_frame_t(const FrameParams& f, uint32_t max_blocks, hipStream_t s) {
-    const size_t smem = (size_t)(tiles_of(5, split_of<P>(5)) + tiles_of(4)) * 64 * 16;
+    const size_t smem = (size_t)(tiles_of(5, split_of<P>(6)) + tiles_of(4)) * 64 * 16;
     auto kern = k_frame<P, WAVES>;
     int per_cu = 0;
     hipError_t e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, reinterpret_cast<const void*>(kern),

@@ -16,6 +16,21 @@ constexpr int kMaxLevels = 16;
 constexpr int kMaxLayers = 7;  // n_hidden_layers <= 6
 constexpr uint64_t kPcgMult = 6364136223846793005ull;
 
+// Two-headed StatNet (nrrs_config.ll2_two_head): the first layer split into the two 32-wide
+// heads, or -1 for one head. PartShare: the tails' two layers (hidden, output); JustEncoding:
+// every layer.
+__host__ __device__ inline int head_first_layer(int two_head, int n_layers) {
+    return two_head == 1 ? n_layers - 2 : (two_head == 2 ? 0 : -1);
+}
+// Whether W_layer[row][col] joins the two heads (held at zero): from the layer after the split
+// on, output rows belong to head row >= 32 (the output layer: row >= 3, L then L^2) and input
+// columns to head col >= 32; the split layer itself reads the shared input whole.
+__host__ __device__ inline bool head_masked(int first, int layer, int n_layers, int row, int col) {
+    if (first < 0 || layer <= first) return false;
+    const bool rh = row >= (layer == n_layers - 1 ? 3 : 32), ch = col >= 32;
+    return rh != ch;
+}
+
 // Per-level hash-grid constants, passed to kernels by value.
 struct GridParams {
     float scale[kMaxLevels];

@@ -83,10 +83,11 @@ constexpr int kTileHalves = 64 * 8;  // one packed A tile: 64 lanes x 8 values
 __host__ __device__ constexpr int tiles_of(int n_layers, bool split = false) {
     return 10 + (split ? 6 : 0) + (n_layers - 2) * 8 * (split ? 2 : 1) + 8;
 }
-// Whether a net runs in split precision: StatNet (5 layers) in precisions with kSplitStat.
+// Whether a net runs in split precision: the StatNet (6 outputs, one or two heads) in
+// precisions with kSplitStat.
 template <typename P>
-__host__ __device__ constexpr bool split_of(int n_layers) {
-    return P::kSplitStat && n_layers == 5;
+__host__ __device__ constexpr bool split_of(int n_out) {
+    return P::kSplitStat && n_out == 6;
 }
 // First tile of hidden layer l (1..n_layers-2) and of the output layer.
 __host__ __device__ constexpr int hidden_base(int layer, bool split) {

@@ -194,12 +194,22 @@ hipError_t launch_grid_indices(const EncodeArgs& a, uint32_t n, uint32_t* idx, h
 }
 
 // Supported shapes: StatNet 4 hidden -> 6, RRSNet 3 hidden -> 1 (the benchmark
-// configurations); other shapes report hipErrorInvalidValue.
+// configurations), the two-headed StatNets 3 and 2 hidden -> 6; other shapes report
+// hipErrorInvalidValue.
 hipError_t launch_net_fp32(const EncodeArgs& a, int net, int n_layers, int n_out, int epi, uint32_t n,
                            const float* table, const float* W, const EpiArgs& ep, hipStream_t s) {
     if (n_layers == 5 && n_out == 6) {
         if (epi == EPI_RAW) return launch_one<5, 6, EPI_RAW>(a, net, n, table, W, ep, s);
         if (epi == EPI_ADRRS) return launch_one<5, 6, EPI_ADRRS>(a, net, n, table, W, ep, s);
+    }
+    // two-headed StatNets (PartShare, JustEncoding): the weights between the heads are zeros
+    if (n_layers == 4 && n_out == 6) {
+        if (epi == EPI_RAW) return launch_one<4, 6, EPI_RAW>(a, net, n, table, W, ep, s);
+        if (epi == EPI_ADRRS) return launch_one<4, 6, EPI_ADRRS>(a, net, n, table, W, ep, s);
+    }
+    if (n_layers == 3 && n_out == 6) {
+        if (epi == EPI_RAW) return launch_one<3, 6, EPI_RAW>(a, net, n, table, W, ep, s);
+        if (epi == EPI_ADRRS) return launch_one<3, 6, EPI_ADRRS>(a, net, n, table, W, ep, s);
     }
     if (n_layers == 4 && n_out == 1) {
         if (epi == EPI_RAW) return launch_one<4, 1, EPI_RAW>(a, net, n, table, W, ep, s);

@@ -147,7 +147,7 @@ __global__ __launch_bounds__(kBlockT, 4) void k_net_mfma(EncodeArgs a, int net, 
                                                       const uint32_t* __restrict__ table,
                                                       const u32x4* __restrict__ Wp, EpiArgs ep) {
     extern __shared__ u32x4 sW[];
-    constexpr bool kSplit = split_of<P>(NL);
+    constexpr bool kSplit = split_of<P>(NOUT);
     constexpr int kTiles = tiles_of(NL, kSplit);
     for (int k = threadIdx.x; k < kTiles * 64; k += kBlockT) sW[k] = Wp[k];
     __syncthreads();
@@ -224,7 +224,7 @@ struct Chain {
     static constexpr int kWaves = P::kSplitStat ? 16 : 8;
     static constexpr int kThreads = 64 * kWaves;
     static constexpr int kPerCu = P::kSplitStat ? 1 : 2;
-    static constexpr int kTilesS = tiles_of(5, split_of<P>(5)), kTilesR = tiles_of(4);
+    static constexpr int kTilesS = tiles_of(5, split_of<P>(6)), kTilesR = tiles_of(4);
 };
 
 template <typename P>
@@ -234,7 +234,7 @@ __global__ __launch_bounds__(Chain<P>::kThreads) void k_net_chained(EncodeArgs a
     extern __shared__ u32x4 sW[];
     constexpr int kChainWaves = Chain<P>::kWaves, kChainT = Chain<P>::kThreads;
     constexpr int kTilesS = Chain<P>::kTilesS, kTilesR = Chain<P>::kTilesR;
-    constexpr bool kSplit = split_of<P>(5);
+    constexpr bool kSplit = split_of<P>(6);
     u32x4* sWs = sW;
     u32x4* sWr = sW + kTilesS * 64;
     for (int k = threadIdx.x; k < kTilesS * 64; k += kChainT) sWs[k] = Ws[k];
@@ -325,7 +325,7 @@ hipError_t launch_chained(const EncodeArgs& a, const uint32_t* enc_rrs, uint32_t
 template <typename P, int NL, int NOUT, int EPI>
 hipError_t launch_one(const EncodeArgs& a, int net, uint32_t n, const uint16_t* table, const uint16_t* Wp,
                       const EpiArgs& ep, hipStream_t s) {
-    constexpr size_t smem = (size_t)tiles_of(NL, split_of<P>(NL)) * kTileHalves * 2;
+    constexpr size_t smem = (size_t)tiles_of(NL, split_of<P>(NOUT)) * kTileHalves * 2;
     static std::atomic<bool> attr[kMaxDevices];
     hipError_t e = set_dyn_lds_once(attr, reinterpret_cast<const void*>(&k_net_mfma<P, NL, NOUT, EPI>), smem);
     if (e != hipSuccess) return e;
@@ -348,6 +348,15 @@ hipError_t dispatch(const EncodeArgs& a, int net, int n_layers, int n_out, int e
     if (n_layers == 4 && n_out == 1) {
         if (epi == EPI_RAW) return launch_one<P, 4, 1, EPI_RAW>(a, net, n, table, Wp, ep, s);
         if (epi == EPI_RRS) return launch_one<P, 4, 1, EPI_RRS>(a, net, n, table, Wp, ep, s);
+    }
+    // two-headed StatNets (PartShare, JustEncoding): the weights between the heads are zeros
+    if (n_layers == 4 && n_out == 6) {
+        if (epi == EPI_RAW) return launch_one<P, 4, 6, EPI_RAW>(a, net, n, table, Wp, ep, s);
+        if (epi == EPI_ADRRS) return launch_one<P, 4, 6, EPI_ADRRS>(a, net, n, table, Wp, ep, s);
+    }
+    if (n_layers == 3 && n_out == 6) {
+        if (epi == EPI_RAW) return launch_one<P, 3, 6, EPI_RAW>(a, net, n, table, Wp, ep, s);
+        if (epi == EPI_ADRRS) return launch_one<P, 3, 6, EPI_ADRRS>(a, net, n, table, Wp, ep, s);
     }
     return hipErrorInvalidValue;
 }
@@ -394,13 +403,13 @@ size_t mfma_packed_halves(int n_layers) { return (size_t)tiles_of(n_layers, true
 hipError_t launch_pack_lp(int precision, const float* grid, uint64_t n_grid, const float* W, int n_layers,
                           int n_out, uint16_t* grid_lp, uint16_t* W_lp, hipStream_t s) {
     if (precision == NRRS_BF16) {
-        const int split = split_of<BF16>(n_layers);
+        const int split = split_of<BF16>(n_out);
         hipLaunchKernelGGL(k_pack_mlp<BF16>, dim3(tiles_of(n_layers, split)), dim3(64), 0, s, W, n_layers, n_out, split,
                            W_lp);
         if (grid)
             hipLaunchKernelGGL(k_to_lp<BF16>, dim3(2048), dim3(256), 0, s, grid, grid_lp, n_grid);
     } else if (precision == NRRS_FP16) {
-        const int split = split_of<F16>(n_layers);
+        const int split = split_of<F16>(n_out);
         hipLaunchKernelGGL(k_pack_mlp<F16>, dim3(tiles_of(n_layers, split)), dim3(64), 0, s, W, n_layers, n_out, split,
                            W_lp);
         if (grid)

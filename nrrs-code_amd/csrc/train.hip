@@ -676,7 +676,8 @@ __global__ __launch_bounds__(64 * P::kWaves) void k_train(TrainArgs t) {
             for (int r = 0; r < 16; ++r) {
                 const int row = to * 32 + 8 * (r >> 2) + 4 * h + (r & 3);
                 const int cc = ti * 32 + col;
-                if (row < out && cc < in) part[woff + row * in + cc] = a[r];
+                if (row < out && cc < in)
+                    part[woff + row * in + cc] = head_masked(t.head_first, layer, NL, row, cc) ? 0.0f : a[r];
             }
         }
     }
@@ -786,6 +787,15 @@ hipError_t dispatch_train(const TrainArgs& t, int nl, int n_out, int loss, uint3
     if (nl == 5 && n_out == 6) {
         if (loss == NRRS_LOSS_EXTERNAL) return launch_train_one<P, 5, 6, NRRS_LOSS_EXTERNAL>(t, blocks, s);
         if (loss == NRRS_LOSS_LL2) return launch_train_one<P, 5, 6, NRRS_LOSS_LL2>(t, blocks, s);
+    }
+    // two-headed StatNets: PartShare (4 layers), JustEncoding (3 layers)
+    if (nl == 4 && n_out == 6) {
+        if (loss == NRRS_LOSS_EXTERNAL) return launch_train_one<P, 4, 6, NRRS_LOSS_EXTERNAL>(t, blocks, s);
+        if (loss == NRRS_LOSS_LL2) return launch_train_one<P, 4, 6, NRRS_LOSS_LL2>(t, blocks, s);
+    }
+    if (nl == 3 && n_out == 6) {
+        if (loss == NRRS_LOSS_EXTERNAL) return launch_train_one<P, 3, 6, NRRS_LOSS_EXTERNAL>(t, blocks, s);
+        if (loss == NRRS_LOSS_LL2) return launch_train_one<P, 3, 6, NRRS_LOSS_LL2>(t, blocks, s);
     }
     if (nl == 4 && n_out == 1) {
         if (loss == NRRS_LOSS_EXTERNAL) return launch_train_one<P, 4, 1, NRRS_LOSS_EXTERNAL>(t, blocks, s);

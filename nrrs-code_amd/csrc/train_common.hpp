@@ -89,6 +89,8 @@ struct TrainArgs {
     float2* gdl;
     uint32_t gdl_stride;
     float* dw_partial;             // [gridDim.x][n_mlp]
+    int head_first;                // two-headed StatNet: first split layer (-1: one head);
+                                   // weights between the heads get a zero gradient
     uint32_t n_mlp;
     double* loss_partial;          // [gridDim.x] FP64 batch-loss partials (LL2 / RRS losses), or null
 };

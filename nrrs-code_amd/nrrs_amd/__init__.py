@@ -70,6 +70,7 @@ class Config(C.Structure):
         ("max_nodes", C.c_uint32),
         ("paired_tables", C.c_uint32),
         ("fp32_level_major", C.c_uint32),
+        ("ll2_two_head", C.c_int32),
     ]
 
 

@@ -122,6 +122,33 @@ void mlp_dims(const nrrs_config* cfg, int net, int* n_layers, int* n_out) {
     *n_out = (int)m.n_outputs;
 }
 
+// Which inputs each neuron reads, per layer ([layer][row * in + col], 1 = connected).
+// Two-headed StatNets restated (twn_PartShare.cpp:93-169, twn_JustEncoding.cpp; the heads are
+// 32 wide here): head 1 predicts L (outputs 0-2), head 2 L^2 (outputs 3-5). PartShare: a base
+// net (all layers but the last two) on the encoding, then per head a tail of two layers on the
+// base net's output; JustEncoding: per head a whole net on the encoding. A head's first layer
+// reads the shared input whole; its later layers read only the same head's neurons
+// (neurons 0-31 of a hidden layer are head 1, 32-63 head 2).
+std::vector<std::vector<char>> connections(const nrrs_config* cfg, int net) {
+    int nl, no;
+    mlp_dims(cfg, net, &nl, &no);
+    std::vector<std::vector<char>> conn(nl);
+    const bool two = net == NRRS_STATNET && cfg->ll2_two_head != 0;
+    const int head_start = cfg->ll2_two_head == 1 ? nl - 2 : 0;  // first layer of the heads
+    int in = ORA_ENC_WIDTH;
+    for (int layer = 0; layer < nl; ++layer) {
+        const int o = layer == nl - 1 ? no : ORA_HIDDEN;
+        conn[layer].assign((size_t)o * in, 1);
+        if (two && layer > head_start)
+            for (int r = 0; r < o; ++r) {
+                const int head = layer == nl - 1 ? r / 3 : r / 32;
+                for (int k = 0; k < in; ++k) conn[layer][(size_t)r * in + k] = (k / 32 == head);
+            }
+        in = o;
+    }
+    return conn;
+}
+
 }  // namespace
 
 extern "C" {
@@ -318,6 +345,7 @@ void ora_net_forward(const nrrs_config* cfg, int32_t net, uint32_t n, const nrrs
     std::vector<double> enc((size_t)n * ORA_ENC_WIDTH);
     ora_encode(cfg, net, n, v, film, stats, params, enc.data());
     const double* W = params + ng;
+    const std::vector<std::vector<char>> conn = connections(cfg, net);
     std::vector<double> h(ORA_ENC_WIDTH), t(ORA_HIDDEN);
     for (uint32_t i = 0; i < n; ++i) {
         int in = ORA_ENC_WIDTH;
@@ -328,7 +356,8 @@ void ora_net_forward(const nrrs_config* cfg, int32_t net, uint32_t n, const nrrs
             t.assign(o, 0.0);
             for (int r = 0; r < o; ++r) {
                 double acc = 0;
-                for (int k = 0; k < in; ++k) acc += w[(size_t)r * in + k] * h[k];
+                for (int k = 0; k < in; ++k)
+                    if (conn[layer][(size_t)r * in + k]) acc += w[(size_t)r * in + k] * h[k];
                 t[r] = layer == nl - 1 ? acc : std::max(acc, 0.0);
             }
             w += (size_t)o * in;
@@ -487,10 +516,11 @@ void ora_init_params(const nrrs_config* cfg, int32_t net, uint64_t seed, double 
     mlp_dims(cfg, net, &nl, &no);
     double* w = params + ng;
     int in = ORA_ENC_WIDTH;
+    const std::vector<std::vector<char>> conn = connections(cfg, net);
     for (int layer = 0; layer < nl; ++layer) {
         int o = layer == nl - 1 ? no : ORA_HIDDEN;
         double lim = std::sqrt(6.0 / (double)(in + o));
-        for (int k = 0; k < o * in; ++k) w[k] = lim * (2.0 * u24(r.next()) - 1.0);
+        for (int k = 0; k < o * in; ++k) w[k] = conn[layer][k] ? lim * (2.0 * u24(r.next()) - 1.0) : 0.0;
         w += (size_t)o * in;
         in = o;
     }
@@ -625,6 +655,7 @@ double ora_train_grads(const nrrs_config* cfg, int32_t net, uint32_t n, const nr
     encode_impl(cfg, net, n, v, film, stats, params, enc.data(), true);
     const double* W = params + ng;
     double* dW = grad + ng;
+    const std::vector<std::vector<char>> conn = connections(cfg, net);
     // forward, keeping every layer input
     std::vector<std::vector<double>> act(nl);  // act[l] = input of layer l, [n][in_l]
     act[0] = enc;
@@ -639,7 +670,9 @@ double ora_train_grads(const nrrs_config* cfg, int32_t net, uint32_t n, const nr
             for (uint32_t i = 0; i < n; ++i)
                 for (int r = 0; r < o; ++r) {
                     double acc = 0;
-                    for (int k = 0; k < in; ++k) acc += w[(size_t)r * in + k] * act[layer][(size_t)i * in + k];
+                    for (int k = 0; k < in; ++k)
+                        if (conn[layer][(size_t)r * in + k])
+                            acc += w[(size_t)r * in + k] * act[layer][(size_t)i * in + k];
                     out[(size_t)i * o + r] = layer == nl - 1 ? acc : std::max(acc, 0.0);
                 }
             if (layer < nl - 1) {
@@ -689,6 +722,7 @@ double ora_train_grads(const nrrs_config* cfg, int32_t net, uint32_t n, const nr
                 double dr = delta[(size_t)i * o + r];
                 if (dr == 0.0) continue;
                 for (int k = 0; k < in; ++k) {
+                    if (!conn[layer][(size_t)r * in + k]) continue;
                     dw[(size_t)r * in + k] += dr * a[(size_t)i * in + k];
                     prev[(size_t)i * in + k] += w[(size_t)r * in + k] * dr;
                 }

@@ -49,3 +49,7 @@ def test_invalid_arguments_rejected_without_device():
     assert lib.nrrs_create(C.byref(unsupported), C.byref(h)) == na.ERR_UNSUPPORTED
     bad_sigma = na.default_config(train_sigma=2)
     assert lib.nrrs_create(C.byref(bad_sigma), C.byref(h)) == na.ERR_ARG
+    # two-headed StatNets: AllSelf (3) and a depth other than the variant's are not built
+    for t, hidden in ((3, 4), (1, 4), (2, 3)):
+        cfg = na.default_config(ll2_two_head=t, statnet=dict(n_hidden_layers=hidden, n_outputs=6))
+        assert lib.nrrs_create(C.byref(cfg), C.byref(h)) == na.ERR_UNSUPPORTED
