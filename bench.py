@@ -63,9 +63,11 @@ def op_work(name, n, n_grid, n_mlp):
     if name in ("train_statnet", "train_rrsnet"):
         return "flops", 6 * MACS["statnet" if name == "train_statnet" else "rrsnet"] * n
     if name == "bin_records_pair":
-        return "bytes", 109 * levels * n
+        # per sample and level: 12 B position + 2 x 8 B dL/denc + 4.5 records x (4 B key + 2 x 4 B values)
+        return "bytes", 82 * levels * n
     if name in ("apply_statnet", "apply_rrsnet"):
-        return "bytes", 45 * levels * n + 4 * n_grid
+        # per sample and level 4.5 records x (4 B key + 4 B value); the FP16 table gradient read and written
+        return "bytes", 36 * levels * n + 4 * n_grid
     if name == "adam_statnet":
         return "bytes", 28 * (n_grid + n_mlp["statnet"])
     if name == "adam_rrsnet":

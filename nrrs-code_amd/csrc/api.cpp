@@ -430,7 +430,7 @@ BinArgs bin_args(nrrs_ctx* c, int net, uint32_t n, const EncodeArgs& a) {
 }
 
 // Sums of one net's binned records (keys `key`, its own values) into its table gradient.
-hipError_t launch_apply(nrrs_ctx* c, int net, const uint16_t* key, hipStream_t s) {
+hipError_t launch_apply(nrrs_ctx* c, int net, const uint32_t* key, hipStream_t s) {
     NetState& ns = c->net[net];
     ApplyArgs ap{};
     ap.key = key;

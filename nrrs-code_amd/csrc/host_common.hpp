@@ -122,8 +122,8 @@ struct NetState {
     uint32_t n_dense = 0;            // table entries of the dense levels
     // binned table-gradient scatter (FP16/BF16 contexts); see train_common.hpp
     bool binned = false;
-    uint16_t* rec_key = nullptr;     // gradient records: keys
-    unsigned long long* rec_val = nullptr;  // and values
+    uint32_t* rec_key = nullptr;     // gradient records: keys
+    uint32_t* rec_val = nullptr;     // and values
     float2* gdl = nullptr;           // [level][max_train] dL/denc pairs of the training kernel
     uint32_t* bin_runs = nullptr;    // [level][chunk][segment] run counts / first slots
     uint32_t* bin_tiles = nullptr;   // [level][tile][segment] scan scratch

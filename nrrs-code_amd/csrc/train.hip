@@ -895,17 +895,31 @@ constexpr int kBinMaxRec = 8 * kBinVert;  // every pair split
 
 // Value of a record: the FP16 pairs w * dL/denc of the corners (x = 0, 1) of (y, z) = yz, both in
 // one u64 for a paired record, corner xb alone for code 3.
-__device__ __forceinline__ unsigned long long record_value(const LevelCell& lc, int yz, float2 g, uint32_t code,
-                                                           uint32_t xb) {
-    uint32_t v[2];
-#pragma unroll
-    for (int x = 0; x < 2; ++x) {
-        const float w = corner_weight(lc, (yz << 1) | x);
-        const h2v gv = {(_Float16)(w * g.x), (_Float16)(w * g.y)};
-        v[x] = __builtin_bit_cast(uint32_t, gv);
+// Value of a record: for a pair (codes 0-2), the FP16 pair of w_y w_z g, the (y, z) weight of the
+// pair times dL/denc, which the apply kernels split between the two x corners with the x
+// fraction carried in the key; for a single corner (code 3), the FP16 pair of its full weight
+// times dL/denc.
+__device__ __forceinline__ uint32_t record_value(const LevelCell& lc, int yz, float2 g, uint32_t code, uint32_t xb) {
+    float w;
+    if (code != 3u) {
+        const float wy = (yz & 1) ? lc.f[1] : 1.0f - lc.f[1];
+        const float wz = (yz & 2) ? lc.f[2] : 1.0f - lc.f[2];
+        w = wy * wz;
+    } else {
+        w = corner_weight(lc, (yz << 1) | (int)xb);
     }
-    if (code != 3u) return (unsigned long long)v[0] | ((unsigned long long)v[1] << 32);
-    return xb ? v[1] : v[0];
+    const h2v gv = {(_Float16)(w * g.x), (_Float16)(w * g.y)};
+    return __builtin_bit_cast(uint32_t, gv);
+}
+
+// x fraction of a cell in 2^-16 units (truncated: 0..65535), the key's upper half
+__device__ __forceinline__ uint32_t x_fraction16(float fx) { return (uint32_t)(fx * 65536.0f); }
+
+// The two corner values of a pair record, for the direct FP16 atomics of a full segment.
+__device__ __forceinline__ void pair_corners(const LevelCell& lc, int yz, float2 g, h2v& v0, h2v& v1) {
+    const float w0 = corner_weight(lc, yz << 1), w1 = corner_weight(lc, (yz << 1) | 1);
+    v0 = h2v{(_Float16)(w0 * g.x), (_Float16)(w0 * g.y)};
+    v1 = h2v{(_Float16)(w1 * g.x), (_Float16)(w1 * g.y)};
 }
 
 // DUAL: the records of both nets of a joint step in one pass. Their corners, and so their keys and
@@ -986,24 +1000,27 @@ __global__ __launch_bounds__(kBinVert) void k_bin_levels(BinArgs b, uint32_t n_c
 #pragma unroll
         for (int q = 0; q < 3; ++q) lc.f[q] = vf[q][vi];
         const int yz = (int)((src >> 1) & 3u);
-        const unsigned long long v = record_value(lc, yz, vg[vi], c, src & 1u);
-        const unsigned long long v2 = DUAL ? record_value(lc, yz, vg2[vi], c, src & 1u) : 0ull;
         const uint32_t d = dst[sb] + k;
         if (d < lim[sb]) {
-            b.rec_key[d] = (uint16_t)((ent & (kSegEntries - 1u)) | (c << kSegShift));
-            b.rec_val[d] = v;
-            if (DUAL) b.rec_val2[d] = v2;
+            const uint32_t u = c != 3u ? x_fraction16(lc.f[0]) : 0u;
+            b.rec_key[d] = (ent & (kSegEntries - 1u)) | (c << kSegShift) | (u << 16);
+            b.rec_val[d] = record_value(lc, yz, vg[vi], c, src & 1u);
+            if (DUAL) b.rec_val2[d] = record_value(lc, yz, vg2[vi], c, src & 1u);
         } else {  // segment full: add directly
             const uint32_t partner = ent ^ ((2u << (c & 3u)) - 1u);
-            h2v* gl = b.grad + b.a.grid.offset[l];
-            __builtin_amdgcn_global_atomic_fadd_v2f16(gl + ent, __builtin_bit_cast(h2v, (uint32_t)v));
-            if (c != 3u) __builtin_amdgcn_global_atomic_fadd_v2f16(gl + partner, __builtin_bit_cast(h2v, (uint32_t)(v >> 32)));
-            if (DUAL) {
-                h2v* gl2 = b.grad2 + b.a.grid.offset[l];
-                __builtin_amdgcn_global_atomic_fadd_v2f16(gl2 + ent, __builtin_bit_cast(h2v, (uint32_t)v2));
-                if (c != 3u)
-                    __builtin_amdgcn_global_atomic_fadd_v2f16(gl2 + partner,
-                                                              __builtin_bit_cast(h2v, (uint32_t)(v2 >> 32)));
+#pragma unroll
+            for (int net2 = 0; net2 < (DUAL ? 2 : 1); ++net2) {
+                h2v* gl = (net2 ? b.grad2 : b.grad) + b.a.grid.offset[l];
+                const float2 g = net2 ? vg2[vi] : vg[vi];
+                if (c != 3u) {
+                    h2v v0, v1;
+                    pair_corners(lc, yz, g, v0, v1);
+                    __builtin_amdgcn_global_atomic_fadd_v2f16(gl + ent, v0);
+                    __builtin_amdgcn_global_atomic_fadd_v2f16(gl + partner, v1);
+                } else {
+                    __builtin_amdgcn_global_atomic_fadd_v2f16(
+                        gl + ent, __builtin_bit_cast(h2v, record_value(lc, yz, g, c, src & 1u)));
+                }
             }
         }
     }
@@ -1019,65 +1036,79 @@ __global__ __launch_bounds__(kBinVert) void k_bin_levels(BinArgs b, uint32_t n_c
 // record turns a table-gradient entry into NaN, so the finite check of the optimiser step
 // sees it. Counts and accumulators are cleared for the next launch.
 constexpr int kApplyThreads = 1024;
-constexpr int kApplyBatch = 4;
+constexpr int kApplyBatch = 2;
 
-__device__ __forceinline__ void apply_half(unsigned long long* acc, uint32_t slot, uint32_t h, uint32_t& bad) {
+// An FP16 value as a signed count of 2^-24 (exact); a non-finite value sets `bad`.
+__device__ __forceinline__ long long fixed_of_half(uint32_t h, uint32_t& bad) {
     const uint32_t ex = (h >> 10) & 31u, m = h & 1023u;
     if (ex == 31u) {
         bad = 1u;
+        return 0;
+    }
+    const long long mag = ex ? (long long)(1024u + m) << (ex - 1u) : (long long)m;
+    return (h & 0x8000u) ? -mag : mag;
+}
+
+// One feature of a record into the LDS sums of plane `plane` (feature f of entry e at
+// f * kSegEntries + e: consecutive entries are 8 bytes apart, so the entries of a wave's records
+// spread over all LDS banks). A pair record's value v is split exactly between its two corners:
+// the x = 1 corner (the partner) takes v u / 2^16, truncated toward zero, the x = 0 corner the
+// rest, so the two parts sum to v and the sums do not depend on the record order.
+__device__ __forceinline__ void apply_feature(unsigned long long* acc, uint32_t e, uint32_t partner, uint32_t c,
+                                              uint32_t u, uint32_t h, uint32_t& bad) {
+    const long long v = fixed_of_half(h, bad);
+    if (!v) return;
+    if (c == 3u) {
+        atomicAdd(&acc[e], (unsigned long long)v);
         return;
     }
-    const unsigned long long mag = ex ? (unsigned long long)(1024u + m) << (ex - 1u) : (unsigned long long)m;
-    if (mag) atomicAdd(&acc[slot], (h & 0x8000u) ? 0ull - mag : mag);
+    const unsigned long long mag = (unsigned long long)(v < 0 ? -v : v), t = (mag * u) >> 16;  // < 2^56
+    const unsigned long long x0 = mag - t;
+    if (x0) atomicAdd(&acc[e], v < 0 ? 0ull - x0 : x0);
+    if (t) atomicAdd(&acc[partner], v < 0 ? 0ull - t : t);
 }
 
-// LDS sums as two planes, feature f of entry e at f * kSegEntries + e: consecutive entries are
-// 8 bytes apart, so the entries of a wave's records spread over all LDS banks (interleaved
-// features put every entry at a multiple of 16 bytes, on half of them).
-__device__ __forceinline__ void apply_pair(unsigned long long* acc, uint32_t e, uint32_t bits, uint32_t& bad) {
-    apply_half(acc, e, bits & 0xffffu, bad);
-    apply_half(acc, kSegEntries + e, bits >> 16, bad);
+// One record (key, value) into the LDS sums of its segment. Key: entry within the segment
+// (bits 0-12) | pair code << 13 | x fraction in 2^-16 units << 16; value: FP16 pair (features 0, 1).
+__device__ __forceinline__ void apply_record(unsigned long long* acc, uint32_t key, uint32_t v, uint32_t& bad) {
+    const uint32_t e = key & (kSegEntries - 1u), c = (key >> kSegShift) & 3u, u = key >> 16;
+    const uint32_t partner = (e ^ ((2u << c) - 1u)) & (kSegEntries - 1u);  // unused for c = 3
+    apply_feature(acc, e, partner, c, u, v & 0xffffu, bad);
+    apply_feature(acc + kSegEntries, e, partner, c, u, v >> 16, bad);
 }
 
-// One record (key, value) into the LDS sums of its segment.
-__device__ __forceinline__ void apply_record(unsigned long long* acc, uint32_t key, unsigned long long v,
-                                             uint32_t& bad) {
-    const uint32_t e = key & (kSegEntries - 1u), c = (key >> kSegShift) & 3u;
-    apply_pair(acc, e, (uint32_t)v, bad);
-    if (c != 3u) apply_pair(acc, e ^ ((2u << c) - 1u), (uint32_t)(v >> 32), bad);
-}
-
-// Records [r0, r1) of a segment into the LDS sums, two per thread and load.
-__device__ __forceinline__ void apply_range(unsigned long long* acc_s, const uint16_t* key,
-                                            const unsigned long long* val, uint32_t r0, uint32_t r1,
-                                            uint32_t& bad) {
-    // r0 is even (segment slots and chunks start at multiples of 64): 4- and 16-byte loads
-    const uint32_t* k2 = static_cast<const uint32_t*>(__builtin_assume_aligned(key + r0, 4));
-    const uint4* v2 = static_cast<const uint4*>(__builtin_assume_aligned(val + r0, 16));
-    const uint32_t n2 = (r1 - r0) / 2;
+// Records [r0, r1) of a segment into the LDS sums, four per thread and load.
+__device__ __forceinline__ void apply_range(unsigned long long* acc_s, const uint32_t* key, const uint32_t* val,
+                                            uint32_t r0, uint32_t r1, uint32_t& bad) {
+    // r0 is a multiple of 64 (segment slots and chunks start there): 16-byte loads
+    const uint4* k4 = static_cast<const uint4*>(__builtin_assume_aligned(key + r0, 16));
+    const uint4* v4 = static_cast<const uint4*>(__builtin_assume_aligned(val + r0, 16));
+    const uint32_t n4 = (r1 - r0) / 4;
     constexpr uint32_t kStep = kApplyThreads * kApplyBatch;
     uint32_t j0 = 0;
-    for (; j0 + kStep <= n2; j0 += kStep) {  // full batches: every load in bounds, all issued first
-        uint32_t kk[kApplyBatch];
-        uint4 q[kApplyBatch];
+    for (; j0 + kStep <= n4; j0 += kStep) {  // full batches: every load in bounds, all issued first
+        uint4 kk[kApplyBatch], q[kApplyBatch];
 #pragma unroll
-        for (int u = 0; u < kApplyBatch; ++u) {
-            kk[u] = k2[j0 + u * kApplyThreads + threadIdx.x];
-            q[u] = v2[j0 + u * kApplyThreads + threadIdx.x];
+        for (int b = 0; b < kApplyBatch; ++b) {
+            kk[b] = k4[j0 + b * kApplyThreads + threadIdx.x];
+            q[b] = v4[j0 + b * kApplyThreads + threadIdx.x];
         }
 #pragma unroll
-        for (int u = 0; u < kApplyBatch; ++u) {
-            apply_record(acc_s, kk[u] & 0xffffu, q[u].x | ((unsigned long long)q[u].y << 32), bad);
-            apply_record(acc_s, kk[u] >> 16, q[u].z | ((unsigned long long)q[u].w << 32), bad);
+        for (int b = 0; b < kApplyBatch; ++b) {
+            apply_record(acc_s, kk[b].x, q[b].x, bad);
+            apply_record(acc_s, kk[b].y, q[b].y, bad);
+            apply_record(acc_s, kk[b].z, q[b].z, bad);
+            apply_record(acc_s, kk[b].w, q[b].w, bad);
         }
     }
-    for (uint32_t j = j0 + threadIdx.x; j < n2; j += kApplyThreads) {
-        const uint32_t kk = k2[j];
-        const uint4 q = v2[j];
-        apply_record(acc_s, kk & 0xffffu, q.x | ((unsigned long long)q.y << 32), bad);
-        apply_record(acc_s, kk >> 16, q.z | ((unsigned long long)q.w << 32), bad);
+    for (uint32_t j = j0 + threadIdx.x; j < n4; j += kApplyThreads) {
+        const uint4 kk = k4[j], q = v4[j];
+        apply_record(acc_s, kk.x, q.x, bad);
+        apply_record(acc_s, kk.y, q.y, bad);
+        apply_record(acc_s, kk.z, q.z, bad);
+        apply_record(acc_s, kk.w, q.w, bad);
     }
-    if (((r1 - r0) & 1u) && threadIdx.x == 0) apply_record(acc_s, key[r1 - 1], val[r1 - 1], bad);
+    for (uint32_t r = r0 + 4 * n4 + threadIdx.x; r < r1; r += kApplyThreads) apply_record(acc_s, key[r], val[r], bad);
 }
 
 __device__ __forceinline__ h2v add_fixed(h2v o, long long s0, long long s1) {

@@ -30,12 +30,14 @@ struct SegInfo {
     uint32_t n_entries;  // entries of the segment (the last one of a level can be short)
 };
 
-// Table-gradient records: key (u16) = entry within the segment (bits 0-12) | pair code << 13
-// (see pair_code); value (u64) = FP16 pair of the key's entry (low half) and, for codes 0-2,
-// of its x-neighbour entry ^ (2^(code+1) - 1) (high half).
+// Table-gradient records: key (u32) = entry within the segment (bits 0-12) | pair code << 13
+// (see pair_code) | x fraction of the cell in 2^-16 units << 16 (pair codes 0-2); value (u32) =
+// FP16 pair of w_y w_z dL/denc for a pair of the key's entry (x bit 0) and its x-neighbour
+// entry ^ (2^(code+1) - 1), split between the two by the x fraction when summed; for code 3
+// the FP16 pair of the key's entry alone.
 struct ApplyArgs {
-    const uint16_t* key;             // record keys, indexed like the slots of the segments
-    const unsigned long long* val;   // record values
+    const uint32_t* key;             // record keys, indexed like the slots of the segments
+    const uint32_t* val;             // record values
     uint32_t* seg_count;             // records appended per segment, cleared after use
     const SegInfo* segs;
     const uint32_t* direct;          // segments summed by one workgroup each (hashed levels)
@@ -106,8 +108,8 @@ struct BinArgs {
     const float2* gdl;             // [level][gdl_stride] dL/denc pairs written by the training kernel
     uint32_t gdl_stride;
     uint32_t n;
-    uint16_t* rec_key;             // record keys and values (see ApplyArgs)
-    unsigned long long* rec_val;
+    uint32_t* rec_key;             // record keys and values (see ApplyArgs)
+    uint32_t* rec_val;
     uint32_t* seg_count;           // records per segment (written by the scan)
     uint32_t* runs;                // [level][n_runs][kMaxSegBins]: record counts of the kRunVert-vertex
                                    // chunks from the encode pass, replaced by their first slots
@@ -122,7 +124,7 @@ struct BinArgs {
     // second net of a joint step (same grid, same record slots), or null: its dL/denc, its
     // record values (the keys are shared) and its table gradient
     const float2* gdl2;
-    unsigned long long* rec_val2;
+    uint32_t* rec_val2;
     _Float16 __attribute__((ext_vector_type(2)))* grad2;
 };
 // Scans the run counts into first slots and segment totals, then writes the records of every
