@@ -50,9 +50,9 @@ MACS = {"statnet": 80 * 64 + 3 * 64 * 64 + 64 * 6, "rrsnet": 80 * 64 + 2 * 64 * 
 #  - mlp_chained: StatNet and RRSNet forward, 2 x (17 792 + 13 376) flops per query;
 #  - train_statnet / train_rrsnet: forward, dX and dW, 3 x 2 x MACs per sample;
 #  - bin_records_pair: per vertex and level 12 B position + 2 x 8 B dL/denc read and 4.5 records
-#    x (2 B key + 2 x 8 B values) written = 109 B;
-#  - apply_statnet / apply_rrsnet: per vertex and level 4.5 records x (2 B key + 8 B value)
-#    read = 45 B, and the FP16 table gradient read and written (4 B per grid parameter);
+#    x (4 B key + 2 x 4 B values) written = 82 B;
+#  - apply_statnet / apply_rrsnet: per vertex and level 4.5 records x (4 B key + 4 B value)
+#    read = 36 B, and the FP16 table gradient read and written (4 B per grid parameter);
 #  - adam_statnet / adam_rrsnet: 28 B per parameter (gradient, p, m, v read; p, m, v written).
 def op_work(name, n, n_grid, n_mlp):
     levels = 16
@@ -63,10 +63,8 @@ def op_work(name, n, n_grid, n_mlp):
     if name in ("train_statnet", "train_rrsnet"):
         return "flops", 6 * MACS["statnet" if name == "train_statnet" else "rrsnet"] * n
     if name == "bin_records_pair":
-        # per sample and level: 12 B position + 2 x 8 B dL/denc + 4.5 records x (4 B key + 2 x 4 B values)
         return "bytes", 82 * levels * n
     if name in ("apply_statnet", "apply_rrsnet"):
-        # per sample and level 4.5 records x (4 B key + 4 B value); the FP16 table gradient read and written
         return "bytes", 36 * levels * n + 4 * n_grid
     if name == "adam_statnet":
         return "bytes", 28 * (n_grid + n_mlp["statnet"])
