@@ -1038,43 +1038,33 @@ __global__ __launch_bounds__(kBinVert) void k_bin_levels(BinArgs b, uint32_t n_c
 constexpr int kApplyThreads = 1024;
 constexpr int kApplyBatch = 2;
 
-// An FP16 value as a signed count of 2^-24 (exact); a non-finite value sets `bad`.
-__device__ __forceinline__ long long fixed_of_half(uint32_t h, uint32_t& bad) {
-    const uint32_t ex = (h >> 10) & 31u, m = h & 1023u;
-    if (ex == 31u) {
-        bad = 1u;
-        return 0;
-    }
-    const long long mag = ex ? (long long)(1024u + m) << (ex - 1u) : (long long)m;
-    return (h & 0x8000u) ? -mag : mag;
-}
-
-// One feature of a record into the LDS sums of plane `plane` (feature f of entry e at
+// One feature of a record into the LDS sums of plane `acc` (feature f of entry e at
 // f * kSegEntries + e: consecutive entries are 8 bytes apart, so the entries of a wave's records
-// spread over all LDS banks). A pair record's value v is split exactly between its two corners:
-// the x = 1 corner (the partner) takes v u / 2^16, truncated toward zero, the x = 0 corner the
-// rest, so the two parts sum to v and the sums do not depend on the record order.
-__device__ __forceinline__ void apply_feature(unsigned long long* acc, uint32_t e, uint32_t partner, uint32_t c,
-                                              uint32_t u, uint32_t h, uint32_t& bad) {
-    const long long v = fixed_of_half(h, bad);
-    if (!v) return;
-    if (c == 3u) {
-        atomicAdd(&acc[e], (unsigned long long)v);
-        return;
-    }
-    const unsigned long long mag = (unsigned long long)(v < 0 ? -v : v), t = (mag * u) >> 16;  // < 2^56
-    const unsigned long long x0 = mag - t;
-    if (x0) atomicAdd(&acc[e], v < 0 ? 0ull - x0 : x0);
-    if (t) atomicAdd(&acc[partner], v < 0 ? 0ull - t : t);
+// spread over all LDS banks). The FP16 value is an exact count of 2^-24 (its magnitude is below
+// 2^41); a pair record's value is split exactly between its two corners: the x = 1 corner (the
+// partner) takes v u / 2^16, truncated toward zero, the x = 0 corner the rest, so the two parts
+// sum to v and the sums do not depend on the record order. A single-corner record has u = 0, so
+// its partner receives zero. Branch-free: zero adds cost less than divergent lanes; a
+// non-finite value adds nothing and sets `bad`.
+__device__ __forceinline__ void apply_feature(unsigned long long* acc, uint32_t e, uint32_t partner, uint32_t u,
+                                              uint32_t h, uint32_t& bad) {
+    const uint32_t ex = (h >> 10) & 31u, m = h & 1023u;
+    bad |= ex == 31u ? 1u : 0u;
+    const unsigned long long mag =
+        ex == 31u ? 0ull : (ex ? (unsigned long long)(1024u + m) << (ex - 1u) : (unsigned long long)m);
+    const unsigned long long t = (mag * u) >> 16, x0 = mag - t;  // mag * u < 2^57
+    const bool neg = (h & 0x8000u) != 0u;
+    atomicAdd(&acc[e], neg ? 0ull - x0 : x0);
+    atomicAdd(&acc[partner], neg ? 0ull - t : t);
 }
 
 // One record (key, value) into the LDS sums of its segment. Key: entry within the segment
 // (bits 0-12) | pair code << 13 | x fraction in 2^-16 units << 16; value: FP16 pair (features 0, 1).
 __device__ __forceinline__ void apply_record(unsigned long long* acc, uint32_t key, uint32_t v, uint32_t& bad) {
     const uint32_t e = key & (kSegEntries - 1u), c = (key >> kSegShift) & 3u, u = key >> 16;
-    const uint32_t partner = (e ^ ((2u << c) - 1u)) & (kSegEntries - 1u);  // unused for c = 3
-    apply_feature(acc, e, partner, c, u, v & 0xffffu, bad);
-    apply_feature(acc + kSegEntries, e, partner, c, u, v >> 16, bad);
+    const uint32_t partner = e ^ ((2u << c) - 1u);  // c = 3: e ^ 15, in the segment, receives zero
+    apply_feature(acc, e, partner, u, v & 0xffffu, bad);
+    apply_feature(acc + kSegEntries, e, partner, u, v >> 16, bad);
 }
 
 // Records [r0, r1) of a segment into the LDS sums, four per thread and load.
