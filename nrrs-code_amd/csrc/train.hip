@@ -1049,9 +1049,11 @@ constexpr int kApplyBatch = 2;
 __device__ __forceinline__ void apply_feature(unsigned long long* acc, uint32_t e, uint32_t partner, uint32_t u,
                                               uint32_t h, uint32_t& bad) {
     const uint32_t ex = (h >> 10) & 31u, m = h & 1023u;
-    bad |= ex == 31u ? 1u : 0u;
+    const uint32_t norm = ex != 0u ? 1u : 0u, finite = ex != 31u ? 1u : 0u;
+    bad |= finite ^ 1u;
+    // written as masks and shifts: the compiler turns nested selects here into branches
     const unsigned long long mag =
-        ex == 31u ? 0ull : (ex ? (unsigned long long)(1024u + m) << (ex - 1u) : (unsigned long long)m);
+        ((unsigned long long)(m | (norm << 10)) << (ex - norm)) & (0ull - (unsigned long long)finite);
     const unsigned long long t = (mag * u) >> 16, x0 = mag - t;  // mag * u < 2^57
     const bool neg = (h & 0x8000u) != 0u;
     atomicAdd(&acc[e], neg ? 0ull - x0 : x0);
