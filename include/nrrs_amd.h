@@ -129,6 +129,14 @@ typedef struct {
      * with their own encodings) returns NRRS_ERR_UNSUPPORTED, and so does the frame kernel
      * (nrrs_frame_*) for two-headed StatNets. */
     int32_t ll2_two_head;
+    /* AID RRSNet input (net1_aid_net2, network.cpp:31, 209-219, 242-258, 303-308, 340-343;
+     * NRRSNetworkInput1AID, train.h:58-68): 0 = RRSNet reads the StatNet statistics (default);
+     * 1 = RRSNet reads position, direction, roughness, reference mean and throughput only
+     * (its normal, albedo and statistics channels are zero) and runs without StatNet at
+     * inference (NRRS_Q_RRSNET). Training still runs StatNet: the NRRS_RRS loss takes its
+     * statistics of the batch (the reference's ll2 buffer). NRRS_Q_RRSNET_CHAINED and the frame
+     * kernel return NRRS_ERR_UNSUPPORTED with 1. */
+    int32_t rrs_aid;
 } nrrs_config;
 
 /* SoA path vertices of one bounce (replaces NRRSInferenceWorkItem, workitem.h:9-15). */

@@ -209,6 +209,7 @@ EncodeArgs encode_args(nrrs_ctx* ctx, const nrrs_vertices& v, const float* film,
     a.stats_stride = stats_stride;
     a.grid = ctx->grid;
     a.frame = ctx->frame;
+    a.aid = ctx->cfg.rrs_aid;
     return a;
 }
 
@@ -637,8 +638,9 @@ nrrs_status alloc_frame(nrrs_ctx* c, uint32_t cap) {
 }
 
 nrrs_status frame_params(nrrs_ctx* c, const nrrs_frame_args* a, FrameParams* f) {
-    // the frame kernel runs the one-headed StatNet of the benchmark configuration
-    if (c->cfg.ll2_two_head) return NRRS_ERR_UNSUPPORTED;
+    // the frame kernel runs the one-headed StatNet of the benchmark configuration and chains its
+    // statistics into RRSNet
+    if (c->cfg.ll2_two_head || c->cfg.rrs_aid) return NRRS_ERR_UNSUPPORTED;
     if (!a || !a->film_dev || !a->queue_dev || !a->sizes_dev || !a->status_dev) return NRRS_ERR_ARG;
     if (a->n_bounces < 1 || a->n_bounces > 16 || a->capacity == 0) return NRRS_ERR_ARG;
     if (a->capacity > c->cfg.max_queue) return NRRS_ERR_SHAPE;
@@ -824,6 +826,7 @@ nrrs_status nrrs_default_config(nrrs_config* c) {
     c->paired_tables = 1;
     c->fp32_level_major = 1;
     c->ll2_two_head = 0;
+    c->rrs_aid = 0;
     return NRRS_OK;
 }
 
@@ -838,6 +841,7 @@ nrrs_status nrrs_create(const nrrs_config* cfg, nrrs_ctx** out) {
     *out = nullptr;
     if (!valid_shapes(*cfg)) return NRRS_ERR_UNSUPPORTED;
     if (cfg->train_sigma != 0 && cfg->train_sigma != 1) return NRRS_ERR_ARG;
+    if (cfg->rrs_aid != 0 && cfg->rrs_aid != 1) return NRRS_ERR_ARG;
     if (cfg->precision != NRRS_FP32 && cfg->precision != NRRS_FP16 && cfg->precision != NRRS_BF16)
         return NRRS_ERR_UNSUPPORTED;
     if (cfg->max_queue == 0 || cfg->n_pixels == 0 || cfg->split_cap == 0 || cfg->split_cap > 255 ||
@@ -922,7 +926,8 @@ nrrs_status nrrs_create(const nrrs_config* cfg, nrrs_ctx** out) {
     if (st == NRRS_OK && cfg->max_train) st = ensure_train(c, NRRS_RRSNET);
     if (st == NRRS_OK) st = ensure_nodes(c);
     if (st == NRRS_OK) st = dmalloc(&c->err_part, error_partial_count());
-    if (st == NRRS_OK && cfg->precision != NRRS_FP32 && !cfg->ll2_two_head) st = alloc_frame(c, cfg->max_queue);
+    if (st == NRRS_OK && cfg->precision != NRRS_FP32 && !cfg->ll2_two_head && !cfg->rrs_aid)
+        st = alloc_frame(c, cfg->max_queue);
     if (st == NRRS_OK) st = hip_status(hipDeviceSynchronize());
     if (st != NRRS_OK) {
         free_ctx(c);
@@ -1017,6 +1022,8 @@ static nrrs_status check_decide_args(nrrs_ctx* ctx, const nrrs_decide_args* a) {
 nrrs_status nrrs_rrs_infer_decide(nrrs_ctx* ctx, const nrrs_decide_args* a, void* stream) {
     NRRS_TRY(check_decide_args(ctx, a));
     if (!vertices_ok(&a->v, a->n) || !a->film_dev) return NRRS_ERR_ARG;
+    // the AID input has no statistics channels: RRSNet runs alone (network.cpp:303-343)
+    if (a->q_mode == NRRS_Q_RRSNET_CHAINED && ctx->cfg.rrs_aid) return NRRS_ERR_UNSUPPORTED;
     hipStream_t s = (hipStream_t)stream;
     {
         OpScope t(ctx, "zero_sum", s);

@@ -521,10 +521,19 @@ __device__ __forceinline__ float film_mean(const float* film, uint32_t n_pixels,
 // Non-grid channels [32,80) of a vertex. RRSNet: ref = pixel mean, st = StatNet
 // statistics (L_rgb, L2_rgb) or null; sigma_l2: the L2 channels are sigma outputs and enter
 // as activationSigma(max(L2, 0)) (the training input with train_sigma, train.cu:196-200).
+// aid: RRSNet's AID input (NRRSNetworkInput1AID, train.h:58-68; train.cu:114-153, 242-267):
+// position, direction, roughness, reference mean and throughput; the normal, albedo and
+// statistics channels are zero.
 __device__ __forceinline__ void encode_tail_rec(const VertexRec& v, int net, float ref, const float* st, float* e,
-                                                bool sigma_l2 = false) {
+                                                bool sigma_l2 = false, bool aid = false) {
+    aid = aid && net == NRRS_RRSNET;
     sh4(v.dir[0], v.dir[1], v.dir[2], e + 32);
-    sh4(v.nrm[0], v.nrm[1], v.nrm[2], e + 48);
+    if (aid) {
+#pragma unroll
+        for (int k = 48; k < 64; ++k) e[k] = 0.0f;
+    } else {
+        sh4(v.nrm[0], v.nrm[1], v.nrm[2], e + 48);
+    }
 #pragma unroll
     for (int k = 64; k < kEnc; ++k) e[k] = 0.0f;
     e[64] = warp_roughness(v.rough);
@@ -539,7 +548,9 @@ __device__ __forceinline__ void encode_tail_rec(const VertexRec& v, int net, flo
 #pragma unroll
         for (int k = 0; k < 3; ++k) e[65 + k] = box_cox(v.thp[k]);
         e[68] = box_cox(ref);
-        if (st) {
+        if (aid) {
+            // channels 69-74 stay zero
+        } else if (st) {
 #pragma unroll
             for (int k = 0; k < 3; ++k) {
                 e[69 + k] = box_cox(fmaxf(st[k], 0.0f));
@@ -562,13 +573,13 @@ __device__ __forceinline__ void encode_tail(const EncodeArgs& a, int net, uint32
     VertexRec v = vertex_from_soa(a, i);
     const float ref = net == NRRS_RRSNET ? film_mean(a.film, a.n_pixels, v.pixel) : 0.0f;
     // two calls rather than a selected pointer, which would put st in memory
-    if (net == NRRS_RRSNET && a.stats) {
+    if (net == NRRS_RRSNET && a.stats && !a.aid) {
         float st[6];
 #pragma unroll
         for (int k = 0; k < 6; ++k) st[k] = a.stats[(size_t)k * a.stats_stride + i];
         encode_tail_rec(v, net, ref, st, e, a.sigma_l2 != 0);
     } else {
-        encode_tail_rec(v, net, ref, nullptr, e);
+        encode_tail_rec(v, net, ref, nullptr, e, false, a.aid != 0);
     }
 }
 

@@ -65,6 +65,9 @@ struct EncodeArgs {
     const float2* grid_enc_f32 = nullptr;
     // RRSNet training input with sigma outputs: channels 72-74 take activationSigma(max(L2, 0))
     int sigma_l2 = 0;
+    // RRSNet's AID input: no normal, albedo or statistics channels (the statistics, when set,
+    // still feed the NRRS_RRS loss)
+    int aid = 0;
 };
 
 struct DecideParams {

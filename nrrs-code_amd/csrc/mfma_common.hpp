@@ -136,9 +136,10 @@ __device__ __forceinline__ void pack_split(const float* x, typename P::vec8& hi,
 // with Bl, also their residual terms (Bl[0..2] for chunks 2..4).
 template <typename P>
 __device__ __forceinline__ void encode_tail_half(const VertexRec& v, int net, int h, float ref, const float* st,
-                                                 typename P::vec8* B, typename P::vec8* Bl = nullptr) {
+                                                 typename P::vec8* B, typename P::vec8* Bl = nullptr,
+                                                 bool aid = false) {
     float t[kEnc], e[8];
-    encode_tail_rec(v, net, ref, st, t);
+    encode_tail_rec(v, net, ref, st, t, false, aid);
 #pragma unroll
     for (int kb = 2; kb < 5; ++kb) {
 #pragma unroll
@@ -153,9 +154,10 @@ __device__ __forceinline__ void encode_tail_half(const VertexRec& v, int net, in
 template <typename P>
 __device__ __forceinline__ void encode_half_rec(const VertexRec& v, const GridParams& grid, int net,
                                                 const uint32_t* __restrict__ table, int h, float ref,
-                                                const float* st, typename P::vec8* B, typename P::vec8* Bl) {
+                                                const float* st, typename P::vec8* B, typename P::vec8* Bl,
+                                                bool aid = false) {
     encode_grid_half<P>(v, grid, table, h, B);
-    encode_tail_half<P>(v, net, h, ref, st, B, Bl);
+    encode_tail_half<P>(v, net, h, ref, st, B, Bl, aid);
 }
 
 template <typename P>
@@ -163,13 +165,13 @@ __device__ __forceinline__ void encode_half(const EncodeArgs& a, int net, const 
                                             uint32_t i, int h, typename P::vec8* B, typename P::vec8* Bl = nullptr) {
     VertexRec v = vertex_from_soa(a, i);
     const float ref = net == NRRS_RRSNET ? film_mean(a.film, a.n_pixels, v.pixel) : 0.0f;
-    if (net == NRRS_RRSNET && a.stats) {  // two calls rather than a selected pointer (st stays in registers)
+    if (net == NRRS_RRSNET && a.stats && !a.aid) {  // two calls rather than a selected pointer (st stays in registers)
         float st[6];
 #pragma unroll
         for (int k = 0; k < 6; ++k) st[k] = a.stats[(size_t)k * a.stats_stride + i];
         encode_half_rec<P>(v, a.grid, net, table, h, ref, st, B, Bl);
     } else {
-        encode_half_rec<P>(v, a.grid, net, table, h, ref, nullptr, B, Bl);
+        encode_half_rec<P>(v, a.grid, net, table, h, ref, nullptr, B, Bl, a.aid != 0);
     }
 }
 
@@ -197,13 +199,13 @@ __device__ __forceinline__ void encode_half_pre(const EncodeArgs& a, int net, ui
     VertexRec v = vertex_from_soa(a, i);
     const float ref = net == NRRS_RRSNET ? film_mean(a.film, a.n_pixels, v.pixel) : 0.0f;
     grid_half_from_levels<P>(a.grid_enc, a.grid_enc_stride, a.grid.n_levels, i, h, B);
-    if (net == NRRS_RRSNET && a.stats) {  // two calls rather than a selected pointer (st stays in registers)
+    if (net == NRRS_RRSNET && a.stats && !a.aid) {  // two calls rather than a selected pointer (st stays in registers)
         float st[6];
 #pragma unroll
         for (int k = 0; k < 6; ++k) st[k] = a.stats[(size_t)k * a.stats_stride + i];
         encode_tail_half<P>(v, net, h, ref, st, B, Bl);
     } else {
-        encode_tail_half<P>(v, net, h, ref, nullptr, B, Bl);
+        encode_tail_half<P>(v, net, h, ref, nullptr, B, Bl, a.aid != 0);
     }
 }
 

@@ -71,6 +71,7 @@ class Config(C.Structure):
         ("paired_tables", C.c_uint32),
         ("fp32_level_major", C.c_uint32),
         ("ll2_two_head", C.c_int32),
+        ("rrs_aid", C.c_int32),
     ]
 
 

@@ -288,8 +288,13 @@ void encode_impl(const nrrs_config* cfg, int32_t net, uint32_t n, const nrrs_ver
             e[2 * l] = f0;
             e[2 * l + 1] = f1;
         }
+        // AID RRSNet input (NRRSNetworkInput1AID, src/render/nrrs/train.h:58-68; built by
+        // nrrs_generate_inference_data_aid, train.cu:242-267, and for training by
+        // nrrs_generate_training_data_from_0to1_aid, train.cu:114-153): position, direction, roughness,
+        // reference mean and throughput only; the normal, albedo and statistics channels stay zero
+        const bool aid = net == NRRS_RRSNET && cfg->rrs_aid != 0;
         ora_sh4(v->dir[i], v->dir[S + i], v->dir[2 * S + i], e + 32);
-        ora_sh4(v->normal[i], v->normal[S + i], v->normal[2 * S + i], e + 48);
+        if (!aid) ora_sh4(v->normal[i], v->normal[S + i], v->normal[2 * S + i], e + 48);
         // warp_roughness (src/render/nrrs/train.h:95)
         e[64] = 1.0 - std::exp(-(double)v->rough[i]);
         if (net == NRRS_STATNET) {
@@ -306,7 +311,7 @@ void encode_impl(const nrrs_config* cfg, int32_t net, uint32_t n, const nrrs_ver
                           (double)film[2 * (size_t)cfg->n_pixels + px]) / 3.0;
             for (int k = 0; k < 3; ++k) e[65 + k] = bc_d((double)v->thp[k * S + i]);
             e[68] = bc_d(ref);
-            for (int k = 0; k < 3; ++k) {
+            for (int k = 0; k < 3 && !aid; ++k) {
                 if (stats) {
                     e[69 + k] = bc_d(std::max(stats[(size_t)k * n + i], 0.0));
                     double l2 = std::max(stats[(size_t)(3 + k) * n + i], 0.0);
