@@ -49,8 +49,8 @@ MACS = {"statnet": 80 * 64 + 3 * 64 * 64 + 64 * 6, "rrsnet": 80 * 64 + 2 * 64 * 
 #    pairs) + 2 x 64 B level-major encodings written = 1164 B per vertex;
 #  - mlp_chained: StatNet and RRSNet forward, 2 x (17 792 + 13 376) flops per query;
 #  - train_statnet / train_rrsnet: forward, dX and dW, 3 x 2 x MACs per sample;
-#  - bin_records_pair: per vertex and level 12 B position + 2 x 8 B dL/denc read and 4.5 records
-#    x (4 B key + 2 x 4 B values) written = 82 B;
+#  - bin_records_pair: per vertex and level 12 B position + 2 x 4 B dL/denc (FP16 pairs) read and
+#    4.5 records x (4 B key + 2 x 4 B values) written = 74 B;
 #  - apply_statnet / apply_rrsnet: per vertex and level 4.5 records x (4 B key + 4 B value)
 #    read = 36 B, and the FP16 table gradient read and written (4 B per grid parameter);
 #  - adam_statnet / adam_rrsnet: 28 B per parameter (gradient, p, m, v read; p, m, v written).
@@ -63,7 +63,7 @@ def op_work(name, n, n_grid, n_mlp):
     if name in ("train_statnet", "train_rrsnet"):
         return "flops", 6 * MACS["statnet" if name == "train_statnet" else "rrsnet"] * n
     if name == "bin_records_pair":
-        return "bytes", 82 * levels * n
+        return "bytes", 74 * levels * n
     if name in ("apply_statnet", "apply_rrsnet"):
         return "bytes", 36 * levels * n + 4 * n_grid
     if name == "adam_statnet":

@@ -127,7 +127,7 @@ struct NetState {
     bool binned = false;
     uint32_t* rec_key = nullptr;     // gradient records: keys
     uint32_t* rec_val = nullptr;     // and values
-    float2* gdl = nullptr;           // [level][max_train] dL/denc pairs of the training kernel
+    uint32_t* gdl = nullptr;         // [level][max_train] dL/denc FP16 pairs of the training kernel
     uint32_t* bin_runs = nullptr;    // [level][chunk][segment] run counts / first slots
     uint32_t* bin_tiles = nullptr;   // [level][tile][segment] scan scratch
     uint32_t bin_max_tiles = 0;

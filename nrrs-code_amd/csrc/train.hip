@@ -583,15 +583,17 @@ __global__ __launch_bounds__(64 * P::kWaves) void k_train(TrainArgs t) {
                 for (int kb = 0; kb < 4; ++kb) G = P::mma16(load_op<P>(Wb, kb, lane), Dl[kb], G);
                 if constexpr (BIN) {
                     // Binned scatter (FP16/BF16): the level pairs of dL/denc go to HBM as
-                    // [level][vertex] float2 and k_bin_levels turns them into records. Lane
-                    // (h, col) holds channels 8(r/4) + 4h + r%4, so registers r and r + 1 (r even)
-                    // are features 0 and 1 of level 4(r/4) + 2h + (r%4)/2.
+                    // [level][vertex] FP16 pairs (the precision of the records they become) and
+                    // k_bin_levels turns them into records. Lane (h, col) holds channels
+                    // 8(r/4) + 4h + r%4, so registers r and r + 1 (r even) are features 0 and 1 of
+                    // level 4(r/4) + 2h + (r%4)/2.
                     if (live) {
 #pragma unroll
                         for (int r = 0; r < 16; r += 2) {
                             const uint32_t l = 4 * (r >> 2) + 2 * h + ((r & 3) >> 1);
                             if (l < t.a.grid.n_levels)
-                                t.gdl[(size_t)l * t.gdl_stride + i] = make_float2(G[r], G[r + 1]);
+                                t.gdl[(size_t)l * t.gdl_stride + i] =
+                                    __builtin_bit_cast(uint32_t, h2v{(_Float16)G[r], (_Float16)G[r + 1]});
                         }
                     }
                 } else {
@@ -912,6 +914,12 @@ __device__ __forceinline__ uint32_t record_value(const LevelCell& lc, int yz, fl
     return __builtin_bit_cast(uint32_t, gv);
 }
 
+// dL/denc pair of the training kernel (FP16 pair) as FP32 (exact)
+__device__ __forceinline__ float2 unpack_h2(uint32_t u) {
+    const h2v v = __builtin_bit_cast(h2v, u);
+    return make_float2((float)v.x, (float)v.y);
+}
+
 // x fraction of a cell in 2^-16 units (truncated: 0..65535), the key's upper half
 __device__ __forceinline__ uint32_t x_fraction16(float fx) { return (uint32_t)(fx * 65536.0f); }
 
@@ -953,8 +961,8 @@ __global__ __launch_bounds__(kBinVert) void k_bin_levels(BinArgs b, uint32_t n_c
         const LevelCell lc = level_cell(b.a.grid, (int)l, x);
 #pragma unroll
         for (int k = 0; k < 3; ++k) vf[k][threadIdx.x] = lc.f[k];
-        vg[threadIdx.x] = b.gdl[(size_t)l * b.gdl_stride + i];
-        if (DUAL) vg2[threadIdx.x] = b.gdl2[(size_t)l * b.gdl_stride + i];
+        vg[threadIdx.x] = unpack_h2(b.gdl[(size_t)l * b.gdl_stride + i]);
+        if (DUAL) vg2[threadIdx.x] = unpack_h2(b.gdl2[(size_t)l * b.gdl_stride + i]);
 #pragma unroll
         for (int yz = 0; yz < 4; ++yz) {
             e[2 * yz] = corner_index(b.a.grid, (int)l, lc, yz << 1);  // < level size

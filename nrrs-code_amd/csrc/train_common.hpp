@@ -86,9 +86,9 @@ struct TrainArgs {
     _Float16 __attribute__((ext_vector_type(2)))* dense_h2;  // [replicas][n_dense] (FP16/BF16)
     uint32_t n_dense;              // entries of the dense levels (they come first in the table)
     // Binned scatter (FP16/BF16 contexts): the kernel writes dL/denc of the grid as
-    // [level][vertex] pairs (row stride gdl_stride) for launch_bin_levels instead of adding it
-    // to the table. Null selects direct atomics (and dense-level replicas).
-    float2* gdl;
+    // [level][vertex] FP16 pairs (row stride gdl_stride) for launch_bin_levels instead of
+    // adding it to the table. Null selects direct atomics (and dense-level replicas).
+    uint32_t* gdl;
     uint32_t gdl_stride;
     float* dw_partial;             // [gridDim.x][n_mlp]
     int head_first;                // two-headed StatNet: first split layer (-1: one head);
@@ -105,7 +105,7 @@ hipError_t launch_train(int precision, const TrainArgs& t, int nl, int n_out, in
                         hipStream_t s);
 struct BinArgs {
     EncodeArgs a;                  // vertices and grid (positions of the records' corners)
-    const float2* gdl;             // [level][gdl_stride] dL/denc pairs written by the training kernel
+    const uint32_t* gdl;           // [level][gdl_stride] dL/denc FP16 pairs written by the training kernel
     uint32_t gdl_stride;
     uint32_t n;
     uint32_t* rec_key;             // record keys and values (see ApplyArgs)
@@ -123,7 +123,7 @@ struct BinArgs {
     _Float16 __attribute__((ext_vector_type(2)))* grad;  // FP16 pair table gradient (full segments)
     // second net of a joint step (same grid, same record slots), or null: its dL/denc, its
     // record values (the keys are shared) and its table gradient
-    const float2* gdl2;
+    const uint32_t* gdl2;
     uint32_t* rec_val2;
     _Float16 __attribute__((ext_vector_type(2)))* grad2;
 };
