@@ -942,9 +942,14 @@ __global__ __launch_bounds__(kBinVert) void k_bin_levels(BinArgs b, uint32_t n_c
     __shared__ float vf[3][kBinVert];          // cell fractions and dL/denc of the workgroup's vertices:
     __shared__ float2 vg[kBinVert];            // the values are formed at copy-out
     __shared__ float2 vg2[DUAL ? kBinVert : 1];
-    const uint32_t l = blockIdx.x / n_chunks, chunk = blockIdx.x % n_chunks, i = chunk * kBinVert + threadIdx.x;
-    if (l >= b.a.grid.n_levels) return;  // whole workgroup
-    uint32_t first = 0u, end = 0u;       // wave 0: this workgroup's first slot and the segment end
+    // Workgroups are dispatched to the 8 XCDs in turn (workgroup w to XCD w % 8): slot w / 8 of
+    // XCD w % 8 takes level (w / 8) % L of chunk 8 ((w / 8) / L) + w % 8, so the L levels of a
+    // chunk run together on one XCD and read its positions from HBM once, then from that XCD's
+    // L2. Every mapping is correct (the slots come from the scan); this one saves traffic.
+    const uint32_t L = b.a.grid.n_levels, slot = blockIdx.x >> 3;
+    const uint32_t l = slot % L, chunk = (slot / L) * 8u + (blockIdx.x & 7u), i = chunk * kBinVert + threadIdx.x;
+    if (chunk >= n_chunks) return;  // whole workgroup
+    uint32_t first = 0u, end = 0u;  // wave 0: this workgroup's first slot and the segment end
     if (threadIdx.x < kMaxSegBins) {
         cnt[threadIdx.x] = 0u;
         first = b.runs[((size_t)l * b.n_runs + chunk * (kBinVert / kRunVert)) * kMaxSegBins + threadIdx.x];
@@ -1314,7 +1319,7 @@ hipError_t launch_bin_records(const BinArgs& b, hipStream_t s) {
     const bool dual = b.gdl2 != nullptr;
     if (dual && (!b.rec_val2 || !b.grad2)) return hipErrorInvalidValue;
     const uint32_t n_chunks = (b.n + kBinVert - 1) / kBinVert;
-    const uint64_t blocks = (uint64_t)n_chunks * b.a.grid.n_levels;
+    const uint64_t blocks = 8ull * ((n_chunks + 7) / 8) * b.a.grid.n_levels;  // XCD-major order, see k_bin_levels
     if (blocks > 0x7FFFFFFFull) return hipErrorInvalidValue;
     if (dual)
         hipLaunchKernelGGL(k_bin_levels<true>, dim3((uint32_t)blocks), dim3(kBinVert), 0, s, b, n_chunks);
