@@ -43,7 +43,8 @@ typedef enum { NRRS_FP32 = 0, NRRS_FP16 = 1, NRRS_BF16 = 2 } nrrs_precision;
 typedef struct nrrs_ctx nrrs_ctx;
 
 /* Network ids, as NRRSNetwork::LL2_NET / RRS_NET (src/render/nrrs/net/network.h:211-212). */
-typedef enum { NRRS_STATNET = 0, NRRS_RRSNET = 1 } nrrs_net;
+/* NRRS_STATNET2: the second StatNet of AllSelf (ll2_two_head = 3), the L^2 head. */
+typedef enum { NRRS_STATNET = 0, NRRS_RRSNET = 1, NRRS_STATNET2 = 2 } nrrs_net;
 
 /* Where the RRS factor q comes from (src/render/nrrs/integrator.cpp:1184-1256). */
 typedef enum {
@@ -125,9 +126,13 @@ typedef struct {
      * statnet.n_hidden_layers = 2. Each head is 32 wide; the two heads of a layer form one
      * 64-wide layer whose weights between the heads are held at zero (set to zero by
      * nrrs_params_set, never trained). Outputs 0-2 (L) come from head 1, 3-5 (L^2) from head 2;
-     * NRRS_LL2 over the 6 outputs is the mean of the two heads' losses. AllSelf (3: two nets
-     * with their own encodings) returns NRRS_ERR_UNSUPPORTED, and so does the frame kernel
-     * (nrrs_frame_*) for two-headed StatNets. */
+     * NRRS_LL2 over the 6 outputs is the mean of the two heads' losses. 3 = AllSelf
+     * (twn_AllSelf.cpp: two nets, each with its own encoding, optimizer and loss), with
+     * statnet.n_hidden_layers = 2: net NRRS_STATNET predicts L (outputs 0-2), net NRRS_STATNET2
+     * L^2 (outputs 3-5); each is a 64-wide StatNet whose output rows of the other head are
+     * held at zero, trained on its head's half of NRRS_LL2 with Adam at lr_statnet / 5
+     * (twn_AllSelf.cpp:9-16); train_sigma = 1 is unsupported with it. The frame kernel
+     * (nrrs_frame_*) returns NRRS_ERR_UNSUPPORTED for every two-headed StatNet. */
     int32_t ll2_two_head;
     /* AID RRSNet input (net1_aid_net2, network.cpp:31, 209-219, 242-258, 303-308, 340-343;
      * NRRSNetworkInput1AID, train.h:58-68): 0 = RRSNet reads the StatNet statistics (default);

@@ -145,17 +145,43 @@ uint64_t mlp_count(const nrrs_mlp_config& m) {
 
 bool valid_shapes(const nrrs_config& c) {
     // Shapes the fused kernels are compiled for: StatNet 4 hidden -> 6 (one head), 3 hidden
-    // (PartShare) or 2 hidden (JustEncoding) -> 6; RRSNet 3 hidden -> 1.
-    static const uint32_t stat_hidden[3] = {4, 3, 2};
-    if (c.ll2_two_head < 0 || c.ll2_two_head > 2) return false;
+    // (PartShare) or 2 hidden (JustEncoding, and each net of AllSelf) -> 6; RRSNet 3 hidden -> 1.
+    static const uint32_t stat_hidden[4] = {4, 3, 2, 2};
+    if (c.ll2_two_head < 0 || c.ll2_two_head > 3) return false;
     return c.statnet.n_hidden_layers == stat_hidden[c.ll2_two_head] && c.statnet.n_outputs == 6 &&
            c.rrsnet.n_hidden_layers == 3 && c.rrsnet.n_outputs == 1;
 }
 
-// Zero the weights between the two heads of a two-headed StatNet (mlp: that net's MLP part).
+// AllSelf (ll2_two_head = 3): the StatNet of L (NRRS_STATNET) and the StatNet of L^2
+// (NRRS_STATNET2), each with its own table, optimizer and loss (twn_AllSelf.cpp).
+bool all_self(const nrrs_config& c) { return c.ll2_two_head == 3; }
+int n_nets(const nrrs_config& c) { return all_self(c) ? 3 : 2; }
+bool net_ok(const nrrs_ctx* c, int32_t net) {
+    return net == NRRS_STATNET || net == NRRS_RRSNET || (net == NRRS_STATNET2 && all_self(c->cfg));
+}
+// What a net's kernels compute: NRRS_RRSNET, or NRRS_STATNET for either StatNet.
+int kind_of(int net) { return net == NRRS_RRSNET ? NRRS_RRSNET : NRRS_STATNET; }
+const nrrs_mlp_config& mlp_of(const nrrs_config& c, int net) { return net == NRRS_RRSNET ? c.rrsnet : c.statnet; }
+// AllSelf: the outputs a StatNet predicts and trains (3 head .. 3 head + 2); -1 = all.
+int head_of(const nrrs_config& c, int net) {
+    return all_self(c) && net != NRRS_RRSNET ? (net == NRRS_STATNET ? 0 : 1) : -1;
+}
+const char* op_label(int net, const char* statnet, const char* rrsnet, const char* statnet2) {
+    return net == NRRS_STATNET ? statnet : (net == NRRS_RRSNET ? rrsnet : statnet2);
+}
+
+// Zero the weights between the two heads of a two-headed StatNet (mlp: that net's MLP part); an
+// AllSelf StatNet's output rows of the other head.
 void zero_between_heads(const nrrs_config& c, int net, float* mlp) {
-    if (net != NRRS_STATNET || !c.ll2_two_head) return;
+    if (net == NRRS_RRSNET || !c.ll2_two_head) return;
     const int nl = (int)c.statnet.n_hidden_layers + 1, first = head_first_layer(c.ll2_two_head, nl);
+    if (all_self(c)) {
+        const size_t out0 = (size_t)kHidden * kEnc + (size_t)(nl - 2) * kHidden * kHidden;
+        for (int r = 0; r < (int)c.statnet.n_outputs; ++r)
+            if (r / 3 != head_of(c, net))
+                for (int k = 0; k < kHidden; ++k) mlp[out0 + (size_t)r * kHidden + k] = 0.0f;
+        return;
+    }
     size_t pos = 0;
     int in = kEnc;
     for (int layer = 0; layer < nl; ++layer) {
@@ -178,9 +204,9 @@ nrrs_status dmalloc(T** p, size_t count) {
 // Parameters of one net: grid U[-1e-4, 1e-4] (the tcnn grid initialisation), then
 // every MLP layer Xavier-uniform, from PCG32(seed, 100 + net). A two-headed StatNet takes the
 // fan-in and fan-out of each head's own layer (32 wide, 3 outputs) from the split on, and
-// zeros between the heads.
+// zeros between the heads; an AllSelf StatNet has 3 outputs.
 void init_params(const nrrs_config& c, int net, uint64_t n_grid, std::vector<float>& out) {
-    const nrrs_mlp_config& m = net == NRRS_STATNET ? c.statnet : c.rrsnet;
+    const nrrs_mlp_config& m = mlp_of(c, net);
     out.resize(n_grid + mlp_count(m));
     Pcg32Host r(c.seed, 100 + (uint64_t)net);
     for (uint64_t k = 0; k < n_grid; ++k) out[k] = (float)(1e-4 * (2.0 * r.uniform() - 1.0));
@@ -191,7 +217,8 @@ void init_params(const nrrs_config& c, int net, uint64_t n_grid, std::vector<flo
     for (int layer = 0; layer < nl; ++layer) {
         int o = layer == nl - 1 ? (int)m.n_outputs : kHidden;
         const bool split = first >= 0 && layer >= first;
-        const int fan_out = split ? o / 2 : o, fan_in = split && layer > first ? in / 2 : in;
+        const int fan_out = split || (head_of(c, net) >= 0 && layer == nl - 1) ? o / 2 : o;
+        const int fan_in = split && layer > first ? in / 2 : in;
         double lim = std::sqrt(6.0 / (double)(fan_in + fan_out));
         for (int k = 0; k < o * in; ++k) out[pos++] = (float)(lim * (2.0 * r.uniform() - 1.0));
         in = o;
@@ -239,27 +266,42 @@ nrrs_status refresh_lp(nrrs_ctx* c, int net, hipStream_t s) {
 hipError_t run_net(nrrs_ctx* c, int net, const EncodeArgs& a, int epi, uint32_t n, const EpiArgs& ep,
                    hipStream_t s) {
     const NetState& ns = c->net[net];
-    OpScope t(c, net == NRRS_STATNET ? "infer_statnet" : "infer_rrsnet", s);
+    OpScope t(c, op_label(net, "infer_statnet", "infer_rrsnet", "infer_statnet2"), s);
+    const int k = kind_of(net);
     if (c->cfg.precision == NRRS_FP32) {
         if (!c->grid_enc_f32 || n > c->grid_enc_cap)
-            return launch_net_fp32(a, net, ns.n_layers, ns.n_out, epi, n, ns.grid_f32, ns.mlp_f32, ep, s);
+            return launch_net_fp32(a, k, ns.n_layers, ns.n_out, epi, n, ns.grid_f32, ns.mlp_f32, ep, s);
         // level-major grid gathers first, then the network kernel reads the encoding
         EncodeArgs ae = a;
         ae.grid_enc_f32 = c->grid_enc_f32;
         ae.grid_enc_stride = c->grid_enc_cap;
         hipError_t e = launch_encode_levels_f32(a, ns.grid_f32, n, c->grid_enc_f32, c->grid_enc_cap, s);
         if (e != hipSuccess) return e;
-        return launch_net_fp32(ae, net, ns.n_layers, ns.n_out, epi, n, ns.grid_f32, ns.mlp_f32, ep, s);
+        return launch_net_fp32(ae, k, ns.n_layers, ns.n_out, epi, n, ns.grid_f32, ns.mlp_f32, ep, s);
     }
     if (!c->grid_enc || n > c->grid_enc_cap)
-        return launch_net_mfma(c->cfg.precision, a, net, ns.n_layers, ns.n_out, epi, n, ns.grid_lp, ns.mlp_lp, ep, s);
+        return launch_net_mfma(c->cfg.precision, a, k, ns.n_layers, ns.n_out, epi, n, ns.grid_lp, ns.mlp_lp, ep, s);
     // level-major grid gathers first, then the MLP kernel reads the encoding
     EncodeArgs ae = a;
     ae.grid_enc = c->grid_enc;
     ae.grid_enc_stride = c->grid_enc_cap;
     hipError_t e = launch_encode_levels(c->cfg.precision, ae, ns.grid_lp, n, c->grid_enc, c->grid_enc_cap, s);
     if (e != hipSuccess) return e;
-    return launch_net_mfma(c->cfg.precision, ae, net, ns.n_layers, ns.n_out, epi, n, ns.grid_lp, ns.mlp_lp, ep, s);
+    return launch_net_mfma(c->cfg.precision, ae, k, ns.n_layers, ns.n_out, epi, n, ns.grid_lp, ns.mlp_lp, ep, s);
+}
+
+// StatNet statistics [6][stride] of a batch; AllSelf runs both its nets, each writing its head's
+// outputs.
+hipError_t run_statnets(nrrs_ctx* c, const EncodeArgs& a, float* out, uint32_t stride, uint32_t n, hipStream_t s) {
+    EpiArgs ep{};
+    ep.out = out;
+    ep.out_stride = stride;
+    if (!all_self(c->cfg)) return run_net(c, NRRS_STATNET, a, EPI_RAW, n, ep, s);
+    ep.head = 0;
+    const hipError_t e = run_net(c, NRRS_STATNET, a, EPI_RAW, n, ep, s);
+    if (e != hipSuccess) return e;
+    ep.head = 1;
+    return run_net(c, NRRS_STATNET2, a, EPI_RAW, n, ep, s);
 }
 
 // Whether a batch of n vertices takes the paired encode pass (both nets in one gather).
@@ -470,6 +512,7 @@ nrrs_status run_backward(nrrs_ctx* c, int net, uint32_t n, const nrrs_vertices& 
     t.a.sigma_l2 = cfg.train_sigma && net == NRRS_RRSNET;
     t.train_sigma = cfg.train_sigma;
     t.head_first = net == NRRS_STATNET ? head_first_layer(cfg.ll2_two_head, ns.n_layers) : -1;
+    t.head = head_of(cfg, net);
     if (cfg.precision != NRRS_FP32) {
         // level-major grid gathers of the forward, read back by the training kernel
         if (!c->grid_enc || n > c->grid_enc_cap) return NRRS_ERR_SHAPE;
@@ -481,7 +524,7 @@ nrrs_status run_backward(nrrs_ctx* c, int net, uint32_t n, const nrrs_vertices& 
                                           ns.binned ? ns.bin_runs : nullptr));
         }
     }
-    t.net = net;
+    t.net = kind_of(net);
     t.n = n;
     t.table_f32 = cfg.precision == NRRS_FP32 ? ns.grid_f32 : nullptr;
     t.table_lp = reinterpret_cast<const uint32_t*>(ns.grid_lp);
@@ -529,7 +572,7 @@ nrrs_status run_backward(nrrs_ctx* c, int net, uint32_t n, const nrrs_vertices& 
     uint32_t blocks = iters < c->max_train_blocks ? iters : c->max_train_blocks;
     if (!blocks) return NRRS_OK;
     {
-        OpScope ot(c, net == NRRS_STATNET ? "train_statnet" : "train_rrsnet", s);
+        OpScope ot(c, op_label(net, "train_statnet", "train_rrsnet", "train_statnet2"), s);
         NRRS_HIP(launch_train(cfg.precision, t, ns.n_layers, ns.n_out, kind, blocks, s));
         if (kind != NRRS_LOSS_EXTERNAL)
             NRRS_HIP(launch_loss_finish(ns.loss_partial, blocks, loss_out, ns.state + 4, s));
@@ -540,13 +583,13 @@ nrrs_status run_backward(nrrs_ctx* c, int net, uint32_t n, const nrrs_vertices& 
     }
     if (ns.binned) {
         {
-            OpScope ot(c, net == NRRS_STATNET ? "bin_statnet" : "bin_rrsnet", s);
+            OpScope ot(c, op_label(net, "bin_statnet", "bin_rrsnet", "bin_statnet2"), s);
             NRRS_HIP(launch_bin_levels(bin_args(c, net, n, t.a), s));
         }
-        OpScope ot(c, net == NRRS_STATNET ? "apply_statnet" : "apply_rrsnet", s);
+        OpScope ot(c, op_label(net, "apply_statnet", "apply_rrsnet", "apply_statnet2"), s);
         NRRS_HIP(launch_apply(c, net, ns.rec_key, s));
     }
-    OpScope ot(c, net == NRRS_STATNET ? "reduce_statnet" : "reduce_rrsnet", s);
+    OpScope ot(c, op_label(net, "reduce_statnet", "reduce_rrsnet", "reduce_statnet2"), s);
     return hip_status(launch_reduce_partials(ns.dw_partial, blocks, (uint32_t)ns.n_mlp, ns.grad + ns.n_grid, s));
 }
 
@@ -595,7 +638,7 @@ nrrs_status fold_dense(nrrs_ctx* c, int net, hipStream_t s) {
 nrrs_status run_adam(nrrs_ctx* c, int net, hipStream_t s) {
     NetState& ns = c->net[net];
     const nrrs_config& cfg = c->cfg;
-    OpScope t(c, net == NRRS_STATNET ? "adam_statnet" : "adam_rrsnet", s);
+    OpScope t(c, op_label(net, "adam_statnet", "adam_rrsnet", "adam_statnet2"), s);
     NRRS_TRY(fold_dense(c, net, s));
     AdamArgs a{};
     a.params = ns.params;
@@ -607,7 +650,8 @@ nrrs_status run_adam(nrrs_ctx* c, int net, hipStream_t s) {
     a.n_grid = ns.n_grid;
     a.grid_lp = ns.grid_lp;
     a.lp_precision = cfg.precision;
-    a.lr = net == NRRS_STATNET ? cfg.lr_statnet : cfg.lr_rrsnet;
+    a.lr = net == NRRS_RRSNET ? cfg.lr_rrsnet : cfg.lr_statnet;
+    if (net != NRRS_RRSNET && all_self(cfg)) a.lr /= 5.0f;  // AllSelf (twn_AllSelf.cpp:9-16)
     a.beta1 = cfg.adam_beta1;
     a.beta2 = cfg.adam_beta2;
     a.eps = cfg.adam_eps;
@@ -842,6 +886,7 @@ nrrs_status nrrs_create(const nrrs_config* cfg, nrrs_ctx** out) {
     if (!valid_shapes(*cfg)) return NRRS_ERR_UNSUPPORTED;
     if (cfg->train_sigma != 0 && cfg->train_sigma != 1) return NRRS_ERR_ARG;
     if (cfg->rrs_aid != 0 && cfg->rrs_aid != 1) return NRRS_ERR_ARG;
+    if (all_self(*cfg) && cfg->train_sigma) return NRRS_ERR_UNSUPPORTED;
     if (cfg->precision != NRRS_FP32 && cfg->precision != NRRS_FP16 && cfg->precision != NRRS_BF16)
         return NRRS_ERR_UNSUPPORTED;
     if (cfg->max_queue == 0 || cfg->n_pixels == 0 || cfg->split_cap == 0 || cfg->split_cap > 255 ||
@@ -878,8 +923,8 @@ nrrs_status nrrs_create(const nrrs_config* cfg, nrrs_ctx** out) {
     // paired table: one 32-bit word per entry and net, n_grid / 2 entries
     if (cfg->precision != NRRS_FP32 && cfg->paired_tables) st = dmalloc(&c->grid_pair, n_grid);
     std::vector<float> host;
-    for (int net = 0; net < 2 && st == NRRS_OK; ++net) {
-        const nrrs_mlp_config& m = net == NRRS_STATNET ? cfg->statnet : cfg->rrsnet;
+    for (int net = 0; net < n_nets(*cfg) && st == NRRS_OK; ++net) {
+        const nrrs_mlp_config& m = mlp_of(*cfg, net);
         NetState& ns = c->net[net];
         ns.n_layers = (int)m.n_hidden_layers + 1;
         ns.n_out = (int)m.n_outputs;
@@ -924,6 +969,7 @@ nrrs_status nrrs_create(const nrrs_config* cfg, nrrs_ctx** out) {
     // first call can be captured into a graph.
     if (st == NRRS_OK && cfg->max_train) st = ensure_train(c, NRRS_STATNET);
     if (st == NRRS_OK && cfg->max_train) st = ensure_train(c, NRRS_RRSNET);
+    if (st == NRRS_OK && cfg->max_train && all_self(*cfg)) st = ensure_train(c, NRRS_STATNET2);
     if (st == NRRS_OK) st = ensure_nodes(c);
     if (st == NRRS_OK) st = dmalloc(&c->err_part, error_partial_count());
     if (st == NRRS_OK && cfg->precision != NRRS_FP32 && !cfg->ll2_two_head && !cfg->rrs_aid)
@@ -945,14 +991,14 @@ nrrs_status nrrs_destroy(nrrs_ctx* ctx) {
 }
 
 nrrs_status nrrs_param_count(nrrs_ctx* ctx, int32_t net, uint64_t* n_grid, uint64_t* n_mlp) {
-    if (!ctx || (net != 0 && net != 1)) return NRRS_ERR_ARG;
+    if (!ctx || !net_ok(ctx, net)) return NRRS_ERR_ARG;
     if (n_grid) *n_grid = ctx->net[net].n_grid;
     if (n_mlp) *n_mlp = ctx->net[net].n_mlp;
     return NRRS_OK;
 }
 
 nrrs_status nrrs_params_get(nrrs_ctx* ctx, int32_t net, float* host_dst) {
-    if (!ctx || !host_dst || (net != 0 && net != 1)) return NRRS_ERR_ARG;
+    if (!ctx || !host_dst || !net_ok(ctx, net)) return NRRS_ERR_ARG;
     const NetState& ns = ctx->net[net];
     NRRS_HIP(hipDeviceSynchronize());
     NRRS_HIP(hipMemcpy(host_dst, ns.grid_f32, ns.n_grid * 4, hipMemcpyDeviceToHost));
@@ -961,11 +1007,11 @@ nrrs_status nrrs_params_get(nrrs_ctx* ctx, int32_t net, float* host_dst) {
 }
 
 nrrs_status nrrs_params_set(nrrs_ctx* ctx, int32_t net, const float* host_src) {
-    if (!ctx || !host_src || (net != 0 && net != 1)) return NRRS_ERR_ARG;
+    if (!ctx || !host_src || !net_ok(ctx, net)) return NRRS_ERR_ARG;
     NetState& ns = ctx->net[net];
     NRRS_HIP(hipDeviceSynchronize());
     NRRS_HIP(hipMemcpy(ns.grid_f32, host_src, ns.n_grid * 4, hipMemcpyHostToDevice));
-    if (net == NRRS_STATNET && ctx->cfg.ll2_two_head) {  // the weights between the heads stay zero
+    if (net != NRRS_RRSNET && ctx->cfg.ll2_two_head) {  // the weights between the heads stay zero
         std::vector<float> mlp(host_src + ns.n_grid, host_src + ns.n_grid + ns.n_mlp);
         zero_between_heads(ctx->cfg, net, mlp.data());
         NRRS_HIP(hipMemcpy(ns.mlp_f32, mlp.data(), ns.n_mlp * 4, hipMemcpyHostToDevice));
@@ -979,10 +1025,10 @@ nrrs_status nrrs_params_set(nrrs_ctx* ctx, int32_t net, const float* host_src) {
 
 nrrs_status nrrs_encode(nrrs_ctx* ctx, int32_t net, uint32_t n, const nrrs_vertices* v,
                         const float* film_dev, const float* stats_dev, float* enc_dev, void* stream) {
-    if (!ctx || (net != 0 && net != 1) || !vertices_ok(v, n) || !enc_dev) return NRRS_ERR_ARG;
+    if (!ctx || !net_ok(ctx, net) || !vertices_ok(v, n) || !enc_dev) return NRRS_ERR_ARG;
     if (net == NRRS_RRSNET && !film_dev) return NRRS_ERR_ARG;
     EncodeArgs a = encode_args(ctx, *v, film_dev, net == NRRS_RRSNET ? stats_dev : nullptr, n);
-    return hip_status(launch_encode(a, net, n, ctx->net[net].grid_f32, enc_dev, (hipStream_t)stream));
+    return hip_status(launch_encode(a, kind_of(net), n, ctx->net[net].grid_f32, enc_dev, (hipStream_t)stream));
 }
 
 nrrs_status nrrs_grid_indices(nrrs_ctx* ctx, uint32_t n, const nrrs_vertices* v, uint32_t* idx_dev,
@@ -996,10 +1042,7 @@ nrrs_status nrrs_statnet_infer(nrrs_ctx* ctx, uint32_t n, const nrrs_vertices* v
                                void* stream) {
     if (!ctx || !vertices_ok(v, n) || !stats_dev) return NRRS_ERR_ARG;
     EncodeArgs a = encode_args(ctx, *v, nullptr, nullptr, 0);
-    EpiArgs ep{};
-    ep.out = stats_dev;
-    ep.out_stride = n;
-    return hip_status(run_net(ctx, NRRS_STATNET, a, EPI_RAW, n, ep, (hipStream_t)stream));
+    return hip_status(run_statnets(ctx, a, stats_dev, n, n, (hipStream_t)stream));
 }
 
 nrrs_status nrrs_rrsnet_infer(nrrs_ctx* ctx, uint32_t n, const nrrs_vertices* v, const float* film_dev,
@@ -1062,10 +1105,7 @@ nrrs_status nrrs_rrs_infer_decide(nrrs_ctx* ctx, const nrrs_decide_args* a, void
         const float* stats = nullptr;
         if (a->q_mode == NRRS_Q_RRSNET_CHAINED) {  // FP32, two-headed StatNet, or a batch beyond the encoding buffers
             EncodeArgs ea = encode_args(ctx, a->v, nullptr, nullptr, 0);
-            EpiArgs es{};
-            es.out = ctx->stats_scratch;
-            es.out_stride = ctx->cfg.max_queue;
-            NRRS_HIP(run_net(ctx, NRRS_STATNET, ea, EPI_RAW, a->n, es, s));
+            NRRS_HIP(run_statnets(ctx, ea, ctx->stats_scratch, ctx->cfg.max_queue, a->n, s));
             stats = ctx->stats_scratch;
         }
         EncodeArgs ea = encode_args(ctx, a->v, a->film_dev, stats, ctx->cfg.max_queue);
@@ -1105,12 +1145,12 @@ nrrs_status nrrs_synth_vertices(uint32_t n, uint64_t seed, uint32_t n_pixels, co
 nrrs_status nrrs_backward(nrrs_ctx* ctx, int32_t net, uint32_t n, const nrrs_vertices* v, const float* film_dev,
                           const float* stats_dev, int32_t kind, const float* dy_dev, const float* target_L_dev,
                           const nrrs_rrs_side* side, float loss_scale, float* loss_dev, void* stream) {
-    if (!ctx || (net != 0 && net != 1) || !vertices_ok(v, n)) return NRRS_ERR_ARG;
+    if (!ctx || !net_ok(ctx, net) || !vertices_ok(v, n)) return NRRS_ERR_ARG;
     if (n > ctx->cfg.max_train) return NRRS_ERR_SHAPE;
     if (kind == NRRS_LOSS_EXTERNAL) {
         if (!dy_dev) return NRRS_ERR_ARG;
     } else if (kind == NRRS_LOSS_LL2) {
-        if (net != NRRS_STATNET || !target_L_dev) return NRRS_ERR_ARG;
+        if (net == NRRS_RRSNET || !target_L_dev) return NRRS_ERR_ARG;
     } else if (kind == NRRS_LOSS_RRS) {
         if (net != NRRS_RRSNET || !side_ok(side) || !stats_dev) return NRRS_ERR_ARG;
     } else {
@@ -1126,7 +1166,7 @@ nrrs_status nrrs_backward(nrrs_ctx* ctx, int32_t net, uint32_t n, const nrrs_ver
 }
 
 nrrs_status nrrs_grads_zero(nrrs_ctx* ctx, int32_t net, void* stream) {
-    if (!ctx || (net != 0 && net != 1)) return NRRS_ERR_ARG;
+    if (!ctx || !net_ok(ctx, net)) return NRRS_ERR_ARG;
     if (!ctx->net[net].train_ready) return NRRS_ERR_SHAPE;  // max_train 0
     NetState& ns = ctx->net[net];
     if (ns.grad_lp)
@@ -1139,7 +1179,7 @@ nrrs_status nrrs_grads_zero(nrrs_ctx* ctx, int32_t net, void* stream) {
 }
 
 nrrs_status nrrs_grads_get(nrrs_ctx* ctx, int32_t net, float* host_dst) {
-    if (!ctx || (net != 0 && net != 1) || !host_dst) return NRRS_ERR_ARG;
+    if (!ctx || !net_ok(ctx, net) || !host_dst) return NRRS_ERR_ARG;
     if (!ctx->net[net].train_ready) return NRRS_ERR_SHAPE;  // max_train 0
     NetState& ns = ctx->net[net];
     NRRS_TRY(fold_dense(ctx, net, nullptr));
@@ -1154,14 +1194,14 @@ nrrs_status nrrs_grads_get(nrrs_ctx* ctx, int32_t net, float* host_dst) {
 }
 
 nrrs_status nrrs_adam_step(nrrs_ctx* ctx, int32_t net, void* stream) {
-    if (!ctx || (net != 0 && net != 1)) return NRRS_ERR_ARG;
+    if (!ctx || !net_ok(ctx, net)) return NRRS_ERR_ARG;
     if (!ctx->net[net].train_ready) return NRRS_ERR_SHAPE;  // max_train 0
     NRRS_TRY(run_adam(ctx, net, (hipStream_t)stream));
     return refresh_pair(ctx, (hipStream_t)stream);
 }
 
 nrrs_status nrrs_train_state(nrrs_ctx* ctx, int32_t net, uint32_t* step, float* loss_scale, uint32_t* skipped) {
-    if (!ctx || (net != 0 && net != 1)) return NRRS_ERR_ARG;
+    if (!ctx || !net_ok(ctx, net)) return NRRS_ERR_ARG;
     if (!ctx->net[net].train_ready) return NRRS_ERR_SHAPE;  // max_train 0
     NetState& ns = ctx->net[net];
     uint32_t st[8];
@@ -1193,9 +1233,11 @@ nrrs_status nrrs_train_step(nrrs_ctx* ctx, uint32_t net_mask, uint32_t n, const 
     const uint32_t ms = ctx->cfg.max_train;
     bool forked = false;
     if (!ctx->net[NRRS_STATNET].train_ready || !ctx->net[NRRS_RRSNET].train_ready) return NRRS_ERR_SHAPE;
+    const bool two_stat = all_self(ctx->cfg);  // AllSelf: a second StatNet with its own table
     // joint step: both nets' encodings (and the record counts, the same for both) in one pass
     // over the paired table, before either net's parameters change
-    const bool paired = (net_mask & 3u) == 3u && ctx->cfg.precision != NRRS_FP32 && paired_pass(ctx, n);
+    const bool paired =
+        (net_mask & 3u) == 3u && ctx->cfg.precision != NRRS_FP32 && !two_stat && paired_pass(ctx, n);
     NetState& s0 = ctx->net[NRRS_STATNET];
     NetState& s1 = ctx->net[NRRS_RRSNET];
     // both nets binned with one segment layout: one record pass writes the records of both
@@ -1281,12 +1323,18 @@ nrrs_status nrrs_train_step(nrrs_ctx* ctx, uint32_t net_mask, uint32_t n, const 
         NRRS_TRY(run_backward(ctx, NRRS_STATNET, n, *v, film_dev, nullptr, 0, NRRS_LOSS_LL2, nullptr, target_L_dev,
                               nullptr, ns.loss_scale, ctx->train_stats, ms, loss_dev, s,
                               paired ? ctx->grid_enc : nullptr));
+        // AllSelf: the StatNet of L^2 (outputs 3-5 of the statistics; its loss adds to loss_dev[0])
+        if (two_stat)
+            NRRS_TRY(run_backward(ctx, NRRS_STATNET2, n, *v, film_dev, nullptr, 0, NRRS_LOSS_LL2, nullptr,
+                                  target_L_dev, nullptr, ctx->net[NRRS_STATNET2].loss_scale, ctx->train_stats, ms,
+                                  loss_dev, s));
         if ((net_mask & 2u) && ctx->aux_stream) {
             // The StatNet update touches only StatNet's buffers: it runs on the helper stream
             // beside the RRSNet backward.
             NRRS_HIP(hipEventRecord(ctx->ev_fork, s));
             NRRS_HIP(hipStreamWaitEvent(ctx->aux_stream, ctx->ev_fork, 0));
-            const nrrs_status st = run_adam(ctx, NRRS_STATNET, ctx->aux_stream);
+            nrrs_status st = run_adam(ctx, NRRS_STATNET, ctx->aux_stream);
+            if (st == NRRS_OK && two_stat) st = run_adam(ctx, NRRS_STATNET2, ctx->aux_stream);
             NRRS_HIP(hipEventRecord(ctx->ev_join, ctx->aux_stream));
             forked = true;
             if (st != NRRS_OK) {
@@ -1295,6 +1343,7 @@ nrrs_status nrrs_train_step(nrrs_ctx* ctx, uint32_t net_mask, uint32_t n, const 
             }
         } else {
             NRRS_TRY(run_adam(ctx, NRRS_STATNET, s));
+            if (two_stat) NRRS_TRY(run_adam(ctx, NRRS_STATNET2, s));
         }
     }
     // the helper stream's work joins the caller's stream on every return from here
@@ -1309,10 +1358,7 @@ nrrs_status nrrs_train_step(nrrs_ctx* ctx, uint32_t net_mask, uint32_t n, const 
     if (net_mask & 2u) {
         if (!(net_mask & 1u)) {
             EncodeArgs ea = encode_args(ctx, *v, nullptr, nullptr, 0);
-            EpiArgs es{};
-            es.out = ctx->train_stats;
-            es.out_stride = ms;
-            NRRS_HIP(run_net(ctx, NRRS_STATNET, ea, EPI_RAW, n, es, s));
+            NRRS_HIP(run_statnets(ctx, ea, ctx->train_stats, ms, n, s));
         }
         NetState& ns = ctx->net[NRRS_RRSNET];
         NRRS_TRY(run_backward(ctx, NRRS_RRSNET, n, *v, film_dev, ctx->train_stats, ms, NRRS_LOSS_RRS, nullptr,

@@ -81,6 +81,7 @@ struct DecideParams {
 struct EpiArgs {
     float* out;                  // EPI_RAW: [n_out][out_stride]
     uint32_t out_stride;
+    int head = -1;               // EPI_RAW: >= 0 writes outputs 3 head .. 3 head + 2 only (AllSelf)
     float* q;                    // EPI_RRS/EPI_ADRRS: [n]
     unsigned long long* sum;     // fixed-point normalisation sum
     DecideParams d;
@@ -164,7 +165,7 @@ struct nrrs_ctx {
     nrrs::GridParams grid;
     nrrs::Frame frame;
     nrrs::DecideParams dec;
-    nrrs::NetState net[2];
+    nrrs::NetState net[3];  // StatNet, RRSNet, and AllSelf's second StatNet (ll2_two_head = 3)
     // RRS-stage scratch
     unsigned long long* sum_dev = nullptr;  // [1]
     double* err_part = nullptr;             // per-block partial sums of the pixel error

@@ -125,7 +125,8 @@ __global__ __launch_bounds__(kBlock) void k_net_fp32(EncodeArgs a, int net, uint
     if (EPI == EPI_RAW) {
         if (live) {
 #pragma unroll
-            for (int j = 0; j < NOUT; ++j) ep.out[(size_t)j * ep.out_stride + i] = y[j];
+            for (int j = 0; j < NOUT; ++j)
+                if (ep.head < 0 || j / 3 == ep.head) ep.out[(size_t)j * ep.out_stride + i] = y[j];
         }
         return;
     }

@@ -180,7 +180,7 @@ __global__ __launch_bounds__(kBlockT, 4) void k_net_mfma(EncodeArgs a, int net, 
 #pragma unroll
                 for (int r = 0; r < 4; ++r) {
                     const int c = 4 * h + r;
-                    if (c < NOUT) ep.out[(size_t)c * ep.out_stride + i] = Y[r];
+                    if (c < NOUT && (ep.head < 0 || c / 3 == ep.head)) ep.out[(size_t)c * ep.out_stride + i] = Y[r];
                 }
             }
         } else if (live && h == 0) {

@@ -386,7 +386,7 @@ __global__ __launch_bounds__(64 * P::kWaves) void k_train(TrainArgs t) {
 #pragma unroll
                 for (int r = 0; r < 4; ++r) {
                     const int c = 4 * h + r;
-                    if (c < NOUT && c < active) {
+                    if (c < NOUT && c < active && (t.head < 0 || c / 3 == t.head)) {
                         float tg = fmaxf(t.target_L[(size_t)(c % 3) * t.a.v.stride + i], 0.0f);
                         if (t.clamp_on) tg = fminf(tg, t.clamp_max);
                         if (c >= 3) tg = tg * tg;
@@ -457,7 +457,7 @@ __global__ __launch_bounds__(64 * P::kWaves) void k_train(TrainArgs t) {
 #pragma unroll
                 for (int r = 0; r < 4; ++r) {
                     const int c = 4 * h + r;
-                    if (c < NOUT) t.y_out[(size_t)c * t.out_stride + i] = Y[r];
+                    if (c < NOUT && (t.head < 0 || c / 3 == t.head)) t.y_out[(size_t)c * t.out_stride + i] = Y[r];
                 }
             }
         }
@@ -1227,7 +1227,7 @@ __global__ __launch_bounds__(kLossThreads) void k_loss_finish(const double* __re
     }
     if (threadIdx.x == 0) {
         const double v = red[0];
-        if (loss_out) *loss_out = (float)v;
+        if (loss_out) *loss_out += (float)v;  // the caller clears it (AllSelf's two nets add)
         if (nonfinite && !isfinite(v)) *nonfinite += 1u;
     }
 }

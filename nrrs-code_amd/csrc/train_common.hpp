@@ -91,6 +91,8 @@ struct TrainArgs {
     uint32_t* gdl;
     uint32_t gdl_stride;
     float* dw_partial;             // [gridDim.x][n_mlp]
+    int head = -1;                 // AllSelf StatNet: NRRS_LL2 terms and y_out of outputs
+                                   // 3 head .. 3 head + 2 only (-1: every output)
     int head_first;                // two-headed StatNet: first split layer (-1: one head);
                                    // weights between the heads get a zero gradient
     uint32_t n_mlp;

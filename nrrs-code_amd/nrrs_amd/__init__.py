@@ -14,7 +14,7 @@ LIB_PATH = os.path.join(os.path.dirname(_HERE), "lib", "libnrrs_amd.so")
 
 OK, ERR_ARG, ERR_SHAPE, ERR_OOM, ERR_OVERFLOW, ERR_NONFINITE, ERR_HIP, ERR_UNSUPPORTED, ERR_NO_DEVICE = range(9)
 FP32, FP16, BF16 = 0, 1, 2
-STATNET, RRSNET = 0, 1
+STATNET, RRSNET, STATNET2 = 0, 1, 2  # STATNET2: the second StatNet of AllSelf (ll2_two_head 3)
 Q_RRSNET, Q_RRSNET_CHAINED, Q_STATNET_ADRRS = 0, 1, 2
 ENC_WIDTH = 80
 

@@ -117,7 +117,7 @@ double bc_d(double x) { return 2.0 * (std::sqrt(x) - 1.0); }
 
 
 void mlp_dims(const nrrs_config* cfg, int net, int* n_layers, int* n_out) {
-    const nrrs_mlp_config& m = net == NRRS_STATNET ? cfg->statnet : cfg->rrsnet;
+    const nrrs_mlp_config& m = net == NRRS_RRSNET ? cfg->rrsnet : cfg->statnet;
     *n_layers = (int)m.n_hidden_layers + 1;
     *n_out = (int)m.n_outputs;
 }
@@ -128,12 +128,15 @@ void mlp_dims(const nrrs_config* cfg, int net, int* n_layers, int* n_out) {
 // net (all layers but the last two) on the encoding, then per head a tail of two layers on the
 // base net's output; JustEncoding: per head a whole net on the encoding. A head's first layer
 // reads the shared input whole; its later layers read only the same head's neurons
-// (neurons 0-31 of a hidden layer are head 1, 32-63 head 2).
+// (neurons 0-31 of a hidden layer are head 1, 32-63 head 2). AllSelf (twn_AllSelf.cpp): two
+// whole nets with their own encodings, net NRRS_STATNET for L and net 2 for L^2; each is the
+// one-head StatNet whose output rows of the other head are not connected.
 std::vector<std::vector<char>> connections(const nrrs_config* cfg, int net) {
     int nl, no;
     mlp_dims(cfg, net, &nl, &no);
     std::vector<std::vector<char>> conn(nl);
-    const bool two = net == NRRS_STATNET && cfg->ll2_two_head != 0;
+    const bool all_self = net != NRRS_RRSNET && cfg->ll2_two_head == 3;
+    const bool two = net == NRRS_STATNET && (cfg->ll2_two_head == 1 || cfg->ll2_two_head == 2);
     const int head_start = cfg->ll2_two_head == 1 ? nl - 2 : 0;  // first layer of the heads
     int in = ORA_ENC_WIDTH;
     for (int layer = 0; layer < nl; ++layer) {
@@ -144,6 +147,10 @@ std::vector<std::vector<char>> connections(const nrrs_config* cfg, int net) {
                 const int head = layer == nl - 1 ? r / 3 : r / 32;
                 for (int k = 0; k < in; ++k) conn[layer][(size_t)r * in + k] = (k / 32 == head);
             }
+        if (all_self && layer == nl - 1)
+            for (int r = 0; r < o; ++r)
+                if (r / 3 != (net == NRRS_STATNET ? 0 : 1))
+                    for (int k = 0; k < in; ++k) conn[layer][(size_t)r * in + k] = 0;
         in = o;
     }
     return conn;
@@ -297,7 +304,7 @@ void encode_impl(const nrrs_config* cfg, int32_t net, uint32_t n, const nrrs_ver
         if (!aid) ora_sh4(v->normal[i], v->normal[S + i], v->normal[2 * S + i], e + 48);
         // warp_roughness (src/render/nrrs/train.h:95)
         e[64] = 1.0 - std::exp(-(double)v->rough[i]);
-        if (net == NRRS_STATNET) {
+        if (net != NRRS_RRSNET) {  // StatNet (either net of AllSelf)
             for (int k = 0; k < 3; ++k) {
                 e[65 + k] = v->albedo_d[k * S + i];
                 e[68 + k] = v->albedo_s[k * S + i];
@@ -544,8 +551,9 @@ namespace {
 // m = activationSigma(y_{3+c})^2 + y_c^2 (as the integrator reads the outputs,
 // integrator.cpp:655-670), scored as (m - L^2)^2 / (m^2 + 0.01) with y_c and the
 // denominator constants of the gradient. y_den supplies those constants (null: y itself).
+// head >= 0: the terms of outputs 3 head .. 3 head + 2 only (the loss of one AllSelf net).
 double loss_ll2(const nrrs_config* cfg, uint32_t n, const double* y, const float* tL, const double* y_den,
-                double* dy) {
+                double* dy, int head = -1) {
     const int active = cfg->ll2_step == 0 ? 3 : 6;
     const double norm = 1.0 / ((double)n * active);
     double loss = 0;
@@ -568,7 +576,7 @@ double loss_ll2(const nrrs_config* cfg, uint32_t n, const double* y, const float
                 d = m - t;
                 slope = 2.0 * sg * sigma_grad_d(yy);
             }
-            if (c < active) {
+            if (c < active && (head < 0 || c / 3 == head)) {
                 loss += d * d / den * norm;
                 if (dy) dy[(size_t)c * n + i] = 2.0 * d / den * slope * norm;
             } else if (dy) {
@@ -695,8 +703,16 @@ double ora_train_grads(const nrrs_config* cfg, int32_t net, uint32_t n, const nr
     if (kind == NRRS_LOSS_EXTERNAL) {
         std::copy(dy_ext, dy_ext + (size_t)no * n, dy.begin());
     } else {
-        loss = ora_loss(cfg, kind, n, y.data(), target_L, v, film, stats, ref_mean, err, sw, qt, error_sum,
-                        nullptr, dy.data());
+        // AllSelf: each net trains on its own head's loss, half of the NRRS_LL2 mean over 6 outputs
+        // ((loss1 + loss2) / 2, twn_AllSelf.cpp:103-108)
+        const int head = kind == NRRS_LOSS_LL2 && net != NRRS_RRSNET && cfg->ll2_two_head == 3
+                             ? (net == NRRS_STATNET ? 0 : 1)
+                             : -1;
+        if (head >= 0)
+            loss = loss_ll2(cfg, n, y.data(), target_L, nullptr, dy.data(), head);
+        else
+            loss = ora_loss(cfg, kind, n, y.data(), target_L, v, film, stats, ref_mean, err, sw, qt, error_sum,
+                            nullptr, dy.data());
     }
     for (auto& d : dy) d *= loss_scale;
     if (dy_out) std::copy(dy.begin(), dy.end(), dy_out);

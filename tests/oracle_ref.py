@@ -138,7 +138,7 @@ def encode(cfg, net, hv, film, stats, params, training=False):
 
 
 def net_forward(cfg, net, hv, film, stats, params):
-    n_out = cfg.statnet.n_outputs if net == na.STATNET else cfg.rrsnet.n_outputs
+    n_out = cfg.rrsnet.n_outputs if net == na.RRSNET else cfg.statnet.n_outputs
     out = np.zeros((hv.n, n_out), np.float64)
     film = None if film is None else np.ascontiguousarray(film, np.float32)
     stats = None if stats is None else np.ascontiguousarray(stats, np.float64)
@@ -249,7 +249,7 @@ def train_grads(cfg, net, hv, params, kind, film=None, stats=None, dy_ext=None, 
                 loss_scale=1.0):
     """FP64 (loss, grad [n_params], dy [n_out][n])."""
     g, m = param_count(cfg, net)
-    n_out = cfg.statnet.n_outputs if net == na.STATNET else cfg.rrsnet.n_outputs
+    n_out = cfg.rrsnet.n_outputs if net == na.RRSNET else cfg.statnet.n_outputs
     grad = np.zeros(g + m, np.float64)
     dy = np.zeros((n_out, hv.n), np.float64)
     params = np.ascontiguousarray(params, np.float64)

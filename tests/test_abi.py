@@ -49,7 +49,11 @@ def test_invalid_arguments_rejected_without_device():
     assert lib.nrrs_create(C.byref(unsupported), C.byref(h)) == na.ERR_UNSUPPORTED
     bad_sigma = na.default_config(train_sigma=2)
     assert lib.nrrs_create(C.byref(bad_sigma), C.byref(h)) == na.ERR_ARG
-    # two-headed StatNets: AllSelf (3) and a depth other than the variant's are not built
-    for t, hidden in ((3, 4), (1, 4), (2, 3)):
+    # two-headed StatNets: a depth other than the variant's, an unknown variant, and AllSelf with
+    # sigma outputs are not built
+    for t, hidden in ((3, 4), (1, 4), (2, 3), (4, 2)):
         cfg = na.default_config(ll2_two_head=t, statnet=dict(n_hidden_layers=hidden, n_outputs=6))
         assert lib.nrrs_create(C.byref(cfg), C.byref(h)) == na.ERR_UNSUPPORTED
+    cfg = na.default_config(ll2_two_head=3, train_sigma=1, statnet=dict(n_hidden_layers=2, n_outputs=6))
+    assert lib.nrrs_create(C.byref(cfg), C.byref(h)) == na.ERR_UNSUPPORTED
+    assert lib.nrrs_create(C.byref(na.default_config(rrs_aid=2)), C.byref(h)) == na.ERR_ARG

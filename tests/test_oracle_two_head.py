@@ -116,3 +116,74 @@ def test_two_head_backward(ora, t):
         assert np.all(G[0][0:32] == 0.0) and np.all(G[1][0:32] == 0.0)
     else:  # PartShare: head 1's tail; the base net is shared
         assert np.all(G[2][0:32] == 0.0) and np.any(G[0] != 0.0)
+
+
+# AllSelf (ll2_two_head 3, twn_AllSelf.cpp): two whole nets with their own encodings, each
+# 80-64-64-3, NRRS_STATNET for L and STATNET2 for L^2, each stored as the 6-output StatNet whose
+# output rows of the other head are not connected.
+ALL_SELF = dict(ll2_two_head=3, statnet=dict(n_hidden_layers=2, n_outputs=6))
+
+
+def _all_self_numpy(W, e, head):
+    relu = lambda x: np.maximum(x, 0.0)  # noqa: E731
+    rows = slice(3 * head, 3 * head + 3)
+    return (W[2][rows] @ relu(W[1] @ relu(W[0] @ e.T))).T
+
+
+def _all_self_setup(ora, **kw):
+    cfg, hv, film = train_setup(ora, **ALL_SELF, **kw)
+    p = [ora.init_params(cfg, net, 3 + net, 0.3) for net in (na.STATNET, na.STATNET2)]
+    return cfg, hv, film, p
+
+
+def test_all_self_forward_is_two_networks(ora):
+    cfg, hv, film, p = _all_self_setup(ora)
+    g, _ = ora.param_count(cfg, na.STATNET)
+    assert ora.param_count(cfg, na.STATNET2) == ora.param_count(cfg, na.STATNET)
+    for head, (net, pn) in enumerate(zip((na.STATNET, na.STATNET2), p)):
+        W = _layers(pn, g, 3)
+        other = slice(3 * (1 - head), 3 * (1 - head) + 3)
+        assert not W[2][other].any()  # the initialisation leaves the other head's rows at zero
+        W[2][other] = np.random.default_rng(4).normal(0, 1, (3, 64))  # and the forward ignores them
+        y = ora.net_forward(cfg, net, hv, film, None, pn)
+        e = ora.encode(cfg, net, hv, film, None, pn)
+        assert np.allclose(y[:, 3 * head:3 * head + 3], _all_self_numpy(W, e, head), rtol=1e-12, atol=1e-12)
+        assert not y[:, other].any()
+    # each net reads its own table: the encodings differ only in the grid channels
+    e0 = ora.encode(cfg, na.STATNET, hv, film, None, p[0])
+    e1 = ora.encode(cfg, na.STATNET2, hv, film, None, p[1])
+    assert not np.array_equal(e0[:, :32], e1[:, :32]) and np.array_equal(e0[:, 32:], e1[:, 32:])
+
+
+def test_all_self_backward(ora):
+    """Each net's gradient is that of NRRS_LL2 over the six statistics (L from the first net,
+    L^2 from the second): central differences of the full loss, and the two nets' loss values
+    add up to it ((loss1 + loss2) / 2, twn_AllSelf.cpp:103-108)."""
+    cfg, hv, film, p = _all_self_setup(ora, ll2_step=1)
+    tL = ora.synth_targets(hv.n, 1)
+
+    def stats(pa, pb):
+        ya = ora.net_forward(cfg, na.STATNET, hv, film, None, pa).T
+        yb = ora.net_forward(cfg, na.STATNET2, hv, film, None, pb).T
+        return np.vstack([ya[:3], yb[3:]])
+
+    y0 = stats(*p)
+    full = ora.loss(cfg, na.LOSS_LL2, hv, y0, target_L=tL)[0]
+    vals = []
+    g, _ = ora.param_count(cfg, na.STATNET)
+    for head, net in enumerate((na.STATNET, na.STATNET2)):
+        val, grad, _ = ora.train_grads(cfg, net, hv, p[head], na.LOSS_LL2, film=film, target_L=tL)
+        vals.append(val)
+
+        def loss_of(q, head=head):
+            pa, pb = (q, p[1]) if head == 0 else (p[0], q)
+            return ora.loss(cfg, na.LOSS_LL2, hv, stats(pa, pb), target_L=tL, y_den=y0)[0]
+
+        assert fd_check(ora, cfg, net, hv, p[head], loss_of, grad) < 1e-4
+        W = _layers(grad, g, 3)
+        assert not W[2][3 * (1 - head):3 * (1 - head) + 3].any() and W[2][3 * head:3 * head + 3].any()
+    assert vals[0] + vals[1] == pytest.approx(full, rel=1e-12)
+    # L only (ll2_step 0): the L^2 net has no loss and no gradient
+    cfg0, _, _, _ = _all_self_setup(ora, ll2_step=0)
+    val2, grad2, _ = ora.train_grads(cfg0, na.STATNET2, hv, p[1], na.LOSS_LL2, film=film, target_L=tL)
+    assert val2 == 0.0 and not grad2.any()
