@@ -19,13 +19,19 @@ __global__ void k_check_finite(const float* __restrict__ g, uint64_t n, const ui
     const uint64_t i0 = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x, step = (uint64_t)gridDim.x * blockDim.x;
     for (uint64_t i = i0; i < n; i += step) bad |= !isfinite(g[i]);
     if (glp) {
-        // 8 halves per 16-byte load (the buffer is 16-byte aligned), then the tail
+        // 8 halves per 16-byte load (the buffer is 16-byte aligned), four loads in flight per
+        // thread, then the tail
         const uint4* g8 = reinterpret_cast<const uint4*>(glp);
         const uint64_t n8 = n_lp / 8;
-        for (uint64_t k = i0; k < n8; k += step) {
-            const uint4 q = g8[k];
-            bad |= half_pair_bad(q.x) | half_pair_bad(q.y) | half_pair_bad(q.z) | half_pair_bad(q.w);
+        auto quad_bad = [](const uint4& q) {
+            return half_pair_bad(q.x) | half_pair_bad(q.y) | half_pair_bad(q.z) | half_pair_bad(q.w);
+        };
+        uint64_t k = i0;
+        for (; k + 3 * step < n8; k += 4 * step) {
+            const uint4 q0 = g8[k], q1 = g8[k + step], q2 = g8[k + 2 * step], q3 = g8[k + 3 * step];
+            bad |= quad_bad(q0) | quad_bad(q1) | quad_bad(q2) | quad_bad(q3);
         }
+        for (; k < n8; k += step) bad |= quad_bad(g8[k]);
         for (uint64_t i = 8 * n8 + i0; i < n_lp; i += step) bad |= (glp[i] & 0x7c00u) == 0x7c00u;
     }
     if (__any(bad) && (threadIdx.x & 63) == 0) atomicOr(flag, 1u);
@@ -121,22 +127,7 @@ __global__ void k_adam(AdamArgs a) {
         adam_scalar(a, k, skip, i);
 }
 
-__global__ void k_adam_finish(AdamArgs a) {
-    uint32_t* st = a.state;
-    if (st[3]) {
-        st[2] += 1;
-        st[1] = 0;
-        if (a.dynamic_scale) *a.loss_scale = fmaxf(*a.loss_scale * 0.5f, 1.0f);
-    } else {
-        st[0] += 1;
-        st[1] += 1;
-        if (a.dynamic_scale && st[1] >= a.growth_interval) {
-            *a.loss_scale = fminf(*a.loss_scale * 2.0f, 65536.0f * 65536.0f);
-            st[1] = 0;
-        }
-    }
-    st[3] = 0;
-}
+__global__ void k_adam_finish(AdamArgs a) { adam_finish_state(a); }
 
 // Paired table from the two nets' FP16/BF16 grids: entry e = {StatNet word e, RRSNet word e}.
 // Two entries per thread: 8-byte reads, one 16-byte write (full lines, no partial writes).
@@ -161,16 +152,16 @@ __global__ void k_fill(float* p, float v, uint64_t n) {
 
 }  // namespace
 
-hipError_t launch_adam(const AdamArgs& a, hipStream_t s) {
+hipError_t launch_adam(const AdamArgs& a, hipStream_t s, bool finish) {
     // with an FP16 grid gradient only the MLP part of `grad` is live
     const uint64_t off = a.grad_lp ? a.n_grid : 0;
-    hipLaunchKernelGGL(k_check_finite, dim3(1024), dim3(256), 0, s, a.grad + off, a.n - off, a.grad_lp,
+    hipLaunchKernelGGL(k_check_finite, dim3(2048), dim3(256), 0, s, a.grad + off, a.n - off, a.grad_lp,
                        a.grad_lp ? a.n_grid : 0, a.state + 3);
     if (a.n_grid % 4 == 0)
         hipLaunchKernelGGL(k_adam, dim3(2048), dim3(256), 0, s, a);
     else
         hipLaunchKernelGGL(k_adam_scalar, dim3(2048), dim3(256), 0, s, a);
-    hipLaunchKernelGGL(k_adam_finish, dim3(1), dim3(1), 0, s, a);
+    if (finish) hipLaunchKernelGGL(k_adam_finish, dim3(1), dim3(1), 0, s, a);
     return hipGetLastError();
 }
 

@@ -660,6 +660,13 @@ nrrs_status run_adam(nrrs_ctx* c, int net, hipStream_t s) {
     a.state = ns.state;
     a.dynamic_scale = cfg.precision != NRRS_FP32;
     a.growth_interval = cfg.loss_scale_growth_interval ? cfg.loss_scale_growth_interval : 1000;
+    if (cfg.precision != NRRS_FP32 && ns.train_ready) {
+        // the update, then the state and both MLP tile layouts in one launch (the grid shadow is
+        // refreshed inside the Adam kernel)
+        NRRS_HIP(launch_adam(a, s, false));
+        return hip_status(launch_after_adam(cfg.precision, a, ns.mlp_f32, ns.n_layers, ns.n_out, ns.mlp_lp, ns.w_fwd,
+                                            ns.w_bwd, s));
+    }
     NRRS_HIP(launch_adam(a, s));
     // MLP shadows (the grid shadow is refreshed inside the Adam kernel)
     if (cfg.precision != NRRS_FP32)

@@ -100,6 +100,51 @@ __device__ __forceinline__ int packed_col(int layer, int kb, int h, int j) {
     return layer == 0 ? kb * 16 + 8 * h + j : 16 * kb + 8 * (j >> 2) + 4 * h + (j & 3);
 }
 
+// Tile t, lane `lane` (0..63) of the packed A tiles of an MLP (layout of tiles_of): 8 weights of
+// the A operand (residual tiles hold lp(W - lp(W)); the difference is exact in FP32).
+template <typename P>
+__device__ __forceinline__ void pack_mlp_tile(const float* __restrict__ W, int n_layers, int n_out, int split,
+                                              uint16_t* __restrict__ out, int t, int lane) {
+    int layer, ot, kb, base_w = 0, in = kEnc, rows = kHidden;
+    bool residual = false;
+    const int l0_tiles = split ? 16 : 10, per_hidden = split ? 16 : 8;
+    if (t < 10) {
+        layer = 0;
+        ot = t / 5;
+        kb = t % 5;
+    } else if (t < l0_tiles) {  // residuals of layer 0's K chunks 2-4
+        layer = 0;
+        ot = (t - 10) / 3;
+        kb = 2 + (t - 10) % 3;
+        residual = true;
+    } else {
+        int u = t - l0_tiles;
+        layer = 1 + u / per_hidden;
+        if (layer <= n_layers - 2) {
+            const int v = u % per_hidden;
+            residual = v >= 8;
+            ot = (v % 8) / 4;
+            kb = v % 4;
+        } else {
+            layer = n_layers - 1;
+            ot = 0;
+            kb = u - (n_layers - 2) * per_hidden;  // 0..3: W, 4..7: residuals
+            residual = kb >= 4;
+            kb &= 3;
+        }
+        base_w = kHidden * kEnc + (layer - 1) * kHidden * kHidden;
+        in = kHidden;
+        rows = layer == n_layers - 1 ? n_out : kHidden;
+    }
+    const int h = lane >> 5, row = ot * 32 + (lane & 31);
+#pragma unroll
+    for (int j = 0; j < 8; ++j) {
+        float w = row < rows ? W[base_w + row * in + packed_col(layer, kb, h, j)] : 0.0f;
+        if (residual) w -= P::pair(lp_bits<P>(w)).x;
+        out[((size_t)t * 64 + lane) * 8 + j] = lp_bits<P>(w);
+    }
+}
+
 // The 40 channels of a vertex that half h of the wavefront feeds to layer 0:
 // channel kb*16 + 8h + j goes to element j of K block kb. Chunks 0-1 are the grid
 // (levels 4h..4h+3 and 8+4h..8+4h+3), chunks 2-4 the SH and per-vertex channels.

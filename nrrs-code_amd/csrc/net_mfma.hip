@@ -16,46 +16,7 @@ constexpr int kBlockT = 64 * kWaves;
 // (residual tiles hold lp(W - lp(W)); the difference is exact in FP32).
 template <typename P>
 __global__ void k_pack_mlp(const float* __restrict__ W, int n_layers, int n_out, int split, uint16_t* __restrict__ out) {
-    const int t = blockIdx.x;      // tile
-    const int lane = threadIdx.x;  // 0..63
-    int layer, ot, kb, base_w = 0, in = kEnc, rows = kHidden;
-    bool residual = false;
-    const int l0_tiles = split ? 16 : 10, per_hidden = split ? 16 : 8;
-    if (t < 10) {
-        layer = 0;
-        ot = t / 5;
-        kb = t % 5;
-    } else if (t < l0_tiles) {  // residuals of layer 0's K chunks 2-4
-        layer = 0;
-        ot = (t - 10) / 3;
-        kb = 2 + (t - 10) % 3;
-        residual = true;
-    } else {
-        int u = t - l0_tiles;
-        layer = 1 + u / per_hidden;
-        if (layer <= n_layers - 2) {
-            const int v = u % per_hidden;
-            residual = v >= 8;
-            ot = (v % 8) / 4;
-            kb = v % 4;
-        } else {
-            layer = n_layers - 1;
-            ot = 0;
-            kb = u - (n_layers - 2) * per_hidden;  // 0..3: W, 4..7: residuals
-            residual = kb >= 4;
-            kb &= 3;
-        }
-        base_w = kHidden * kEnc + (layer - 1) * kHidden * kHidden;
-        in = kHidden;
-        rows = layer == n_layers - 1 ? n_out : kHidden;
-    }
-    const int h = lane >> 5, row = ot * 32 + (lane & 31);
-#pragma unroll
-    for (int j = 0; j < 8; ++j) {
-        float w = row < rows ? W[base_w + row * in + packed_col(layer, kb, h, j)] : 0.0f;
-        if (residual) w -= P::pair(lp_bits<P>(w)).x;
-        out[((size_t)t * 64 + lane) * 8 + j] = lp_bits<P>(w);
-    }
+    pack_mlp_tile<P>(W, n_layers, n_out, split, out, blockIdx.x, threadIdx.x);
 }
 
 template <typename P>

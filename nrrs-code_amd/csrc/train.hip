@@ -21,6 +21,7 @@
 //      at l = 0 the 32 grid channels of dL/denc are scatter-added into the table (FP32)
 //      or written out for the binned scatter k_bin_levels (FP16/BF16).
 #include "device_common.hpp"
+#include "mfma_common.hpp"
 #include "train_common.hpp"
 
 namespace nrrs {
@@ -398,15 +399,15 @@ __global__ __launch_bounds__(64 * P::kWaves) void k_train(TrainArgs t) {
                             const float mu = h == 0 ? Y[0] : (r == 0 ? y_mu[1] : y_mu[2]);
                             const float sg = activation_sigma(y);
                             const float m2 = sg * sg + mu * mu;
-                            const float den = m2 * m2 + 0.01f;
+                            const float rden = 1.0f / (m2 * m2 + 0.01f);  // one division for both terms
                             const float d = m2 - tg;
-                            lterm += d * d / den * t.inv_norm;
-                            dyr[r] = 2.0f * d / den * 2.0f * sg * activation_sigma_grad(y) * t.inv_norm;
+                            lterm += d * d * rden * t.inv_norm;
+                            dyr[r] = 2.0f * d * rden * 2.0f * sg * activation_sigma_grad(y) * t.inv_norm;
                         } else {
-                            const float den = y * y + 0.01f;
+                            const float rden = 1.0f / (y * y + 0.01f);
                             const float d = y - tg;
-                            lterm += d * d / den * t.inv_norm;
-                            dyr[r] = 2.0f * d / den * t.inv_norm;
+                            lterm += d * d * rden * t.inv_norm;
+                            dyr[r] = 2.0f * d * rden * t.inv_norm;
                         }
                     }
                 }
@@ -444,9 +445,10 @@ __global__ __launch_bounds__(64 * P::kWaves) void k_train(TrainArgs t) {
                         const float q = activation_sigma(y);
                         const float qe = q + 0.01f;
                         const float w = t.sample_weight[i];
-                        const float f = t.gamma[0] * (vv / qe + q * ebar) + t.gamma[1] * (vv / qe + q * e) +
+                        const float rq = 1.0f / qe, vq = vv * rq;  // v / (q + 0.01), one division
+                        const float f = t.gamma[0] * (vq + q * ebar) + t.gamma[1] * (vq + q * e) +
                                         t.gamma[2] * q + t.gamma[3] * (q - qt) * (q - qt);
-                        const float fp = -(t.gamma[0] + t.gamma[1]) * vv / (qe * qe) + t.gamma[0] * ebar +
+                        const float fp = -(t.gamma[0] + t.gamma[1]) * vq * rq + t.gamma[0] * ebar +
                                          t.gamma[1] * e + t.gamma[2] + 2.0f * t.gamma[3] * (q - qt);
                         lterm += w * f * t.inv_norm;
                         dyr[0] = w * fp * activation_sigma_grad(y) * t.inv_norm;
@@ -700,9 +702,9 @@ __global__ __launch_bounds__(64 * P::kWaves) void k_train(TrainArgs t) {
 __device__ __forceinline__ int pcol(int kb, int h, int j) { return 16 * kb + 8 * (j >> 2) + 4 * h + (j & 3); }
 
 template <typename E>
-__global__ void k_pack_train(const float* __restrict__ W, int nl, int n_out, E* __restrict__ fwd, E* __restrict__ bwd,
-                             int n_fwd, int n_bwd) {
-    const int tile = blockIdx.x, lane = threadIdx.x, h = lane >> 5, r32 = lane & 31;
+__device__ __forceinline__ void pack_train_tile(const float* __restrict__ W, int nl, int n_out, E* __restrict__ fwd,
+                                                E* __restrict__ bwd, int n_fwd, int n_bwd, int tile, int lane) {
+    const int h = lane >> 5, r32 = lane & 31;
     auto woff = [&](int layer) { return layer == 0 ? 0 : kHidden * kEnc + (layer - 1) * kHidden * kHidden; };
     if (tile < n_fwd) {
         int layer, ot, kb;
@@ -749,6 +751,31 @@ __global__ void k_pack_train(const float* __restrict__ W, int nl, int n_out, E* 
             w = c < n_out ? W[woff(nl - 1) + c * kHidden + ot * 32 + r32] : 0.0f;
         }
         bwd[((size_t)bt * 64 + lane) * 8 + j] = (E)w;
+    }
+}
+
+// One wave per tile.
+template <typename E>
+__global__ void k_pack_train(const float* __restrict__ W, int nl, int n_out, E* __restrict__ fwd, E* __restrict__ bwd,
+                             int n_fwd, int n_bwd) {
+    pack_train_tile<E>(W, nl, n_out, fwd, bwd, n_fwd, n_bwd, blockIdx.x, threadIdx.x);
+}
+
+// After an optimizer step of an FP16/BF16 net, one wave per workgroup: workgroup 0 updates the
+// step and loss-scale state, the next n_inf the inference tiles of the MLP (P's layout, with
+// the split-precision residual tiles where the net has them), the rest its training tiles (E).
+// Three dependent small launches become one.
+template <typename P, typename E>
+__global__ void k_after_adam(AdamArgs a, const float* __restrict__ W, int nl, int n_out, int split,
+                             uint16_t* __restrict__ inf, int n_inf, E* __restrict__ fwd, E* __restrict__ bwd, int n_fwd,
+                             int n_bwd) {
+    const int b = (int)blockIdx.x - 1;
+    if (b < 0) {
+        if (threadIdx.x == 0) adam_finish_state(a);
+    } else if (b < n_inf) {
+        mfma::pack_mlp_tile<P>(W, nl, n_out, split, inf, b, threadIdx.x);
+    } else {
+        pack_train_tile<E>(W, nl, n_out, fwd, bwd, n_fwd, n_bwd, b - n_inf, threadIdx.x);
     }
 }
 
@@ -1278,6 +1305,23 @@ hipError_t launch_pack_train(int precision, const float* W, int nl, int n_out, v
     else
         hipLaunchKernelGGL(k_pack_train<_Float16>, dim3(nf + nb), dim3(64), 0, s, W, nl, n_out,
                            static_cast<_Float16*>(fwd), static_cast<_Float16*>(bwd), nf, nb);
+    return hipGetLastError();
+}
+
+hipError_t launch_after_adam(int precision, const AdamArgs& a, const float* W, int nl, int n_out, uint16_t* mlp_lp,
+                             void* fwd, void* bwd, hipStream_t s) {
+    const int nf = train_tiles_fwd(nl), nb = train_tiles_bwd(nl);
+    if (precision == NRRS_BF16) {
+        const int split = mfma::split_of<mfma::BF16>(n_out), ni = mfma::tiles_of(nl, split);
+        hipLaunchKernelGGL((k_after_adam<mfma::BF16, __bf16>), dim3(1 + ni + nf + nb), dim3(64), 0, s, a, W, nl, n_out,
+                           split, mlp_lp, ni, static_cast<__bf16*>(fwd), static_cast<__bf16*>(bwd), nf, nb);
+    } else if (precision == NRRS_FP16) {
+        const int split = mfma::split_of<mfma::F16>(n_out), ni = mfma::tiles_of(nl, split);
+        hipLaunchKernelGGL((k_after_adam<mfma::F16, _Float16>), dim3(1 + ni + nf + nb), dim3(64), 0, s, a, W, nl,
+                           n_out, split, mlp_lp, ni, static_cast<_Float16*>(fwd), static_cast<_Float16*>(bwd), nf, nb);
+    } else {
+        return hipErrorInvalidValue;
+    }
     return hipGetLastError();
 }
 

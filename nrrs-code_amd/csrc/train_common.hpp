@@ -167,7 +167,32 @@ struct AdamArgs {
     int dynamic_scale;
     uint32_t growth_interval;
 };
-hipError_t launch_adam(const AdamArgs& a, hipStream_t s);
+// Loss-scale and step state after an optimizer step (one thread): a step whose gradient had a
+// NaN/Inf is counted as skipped and halves the scale; growth_interval finite steps in a row
+// double it.
+__device__ inline void adam_finish_state(const AdamArgs& a) {
+    uint32_t* st = a.state;
+    if (st[3]) {
+        st[2] += 1;
+        st[1] = 0;
+        if (a.dynamic_scale) *a.loss_scale = fmaxf(*a.loss_scale * 0.5f, 1.0f);
+    } else {
+        st[0] += 1;
+        st[1] += 1;
+        if (a.dynamic_scale && st[1] >= a.growth_interval) {
+            *a.loss_scale = fminf(*a.loss_scale * 2.0f, 65536.0f * 65536.0f);
+            st[1] = 0;
+        }
+    }
+    st[3] = 0;
+}
+// The finite check and the update; with `finish` also the state update (else the caller runs
+// adam_finish_state, as launch_after_adam does).
+hipError_t launch_adam(const AdamArgs& a, hipStream_t s, bool finish = true);
+// After launch_adam(..., false) of an FP16/BF16 net: the state update, the inference tiles
+// (mlp_lp) and the training tiles (fwd, bwd) of the updated MLP in one launch.
+hipError_t launch_after_adam(int precision, const AdamArgs& a, const float* W, int nl, int n_out, uint16_t* mlp_lp,
+                             void* fwd, void* bwd, hipStream_t s);
 hipError_t launch_fill_f32(float* p, float v, uint64_t n, hipStream_t s);
 // Paired table (entry e = {StatNet word e, RRSNet word e}) from the nets' FP16/BF16 grids,
 // both 16-byte aligned with n_entries words each.
