@@ -13,12 +13,14 @@ __device__ __forceinline__ bool half_pair_bad(uint32_t w) {
     return (w & 0x7c00u) == 0x7c00u || (w & 0x7c000000u) == 0x7c000000u;
 }
 
+// scan_flag: null, or the flag that says whether the FP16 table gradient needs the scan (its
+// producers report non-finite entries themselves otherwise).
 __global__ void k_check_finite(const float* __restrict__ g, uint64_t n, const uint16_t* __restrict__ glp,
-                               uint64_t n_lp, uint32_t* flag) {
+                               uint64_t n_lp, uint32_t* flag, const uint32_t* scan_flag) {
     bool bad = false;
     const uint64_t i0 = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x, step = (uint64_t)gridDim.x * blockDim.x;
     for (uint64_t i = i0; i < n; i += step) bad |= !isfinite(g[i]);
-    if (glp) {
+    if (glp && (!scan_flag || *scan_flag)) {
         // 8 halves per 16-byte load (the buffer is 16-byte aligned), four loads in flight per
         // thread, then the tail
         const uint4* g8 = reinterpret_cast<const uint4*>(glp);
@@ -155,8 +157,9 @@ __global__ void k_fill(float* p, float v, uint64_t n) {
 hipError_t launch_adam(const AdamArgs& a, hipStream_t s, bool finish) {
     // with an FP16 grid gradient only the MLP part of `grad` is live
     const uint64_t off = a.grad_lp ? a.n_grid : 0;
-    hipLaunchKernelGGL(k_check_finite, dim3(2048), dim3(256), 0, s, a.grad + off, a.n - off, a.grad_lp,
-                       a.grad_lp ? a.n_grid : 0, a.state + 3);
+    // with producer-side reporting the table scan is the rare case (a full segment): fewer blocks
+    hipLaunchKernelGGL(k_check_finite, dim3(a.scan_on_flag ? 256 : 2048), dim3(256), 0, s, a.grad + off, a.n - off,
+                       a.grad_lp, a.grad_lp ? a.n_grid : 0, a.state + 3, a.scan_on_flag ? a.state + 5 : nullptr);
     if (a.n_grid % 4 == 0)
         hipLaunchKernelGGL(k_adam, dim3(2048), dim3(256), 0, s, a);
     else

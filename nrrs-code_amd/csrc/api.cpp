@@ -469,6 +469,7 @@ BinArgs bin_args(nrrs_ctx* c, int net, uint32_t n, const EncodeArgs& a) {
         bn.seg_nbins[l] = ns.seg_nbins[l];
     }
     bn.grad = reinterpret_cast<_Float16 __attribute__((ext_vector_type(2)))*>(ns.grad_lp);
+    bn.scan_flag = ns.state + 5;
     return bn;
 }
 
@@ -489,6 +490,7 @@ hipError_t launch_apply(nrrs_ctx* c, int net, const uint32_t* key, hipStream_t s
     ap.chunk = ns.apply_chunk;
     ap.bad = ns.bad;
     ap.grad = reinterpret_cast<_Float16 __attribute__((ext_vector_type(2)))*>(ns.grad_lp);
+    ap.nonfinite = ns.state + 3;
     return launch_apply_bins(ap, ns.n_direct, ns.n_chunks, s);
 }
 
@@ -660,6 +662,7 @@ nrrs_status run_adam(nrrs_ctx* c, int net, hipStream_t s) {
     a.state = ns.state;
     a.dynamic_scale = cfg.precision != NRRS_FP32;
     a.growth_interval = cfg.loss_scale_growth_interval ? cfg.loss_scale_growth_interval : 1000;
+    a.scan_on_flag = ns.binned;  // the apply kernels report non-finite table-gradient entries
     if (cfg.precision != NRRS_FP32 && ns.train_ready) {
         // the update, then the state and both MLP tile layouts in one launch (the grid shadow is
         // refreshed inside the Adam kernel)
@@ -1277,6 +1280,7 @@ nrrs_status nrrs_train_step(nrrs_ctx* ctx, uint32_t net_mask, uint32_t n, const 
             }
             bn.gdl2 = s1.gdl;
             bn.rec_val2 = s1.rec_val;
+            bn.scan_flag2 = s1.state + 5;
             bn.grad2 = reinterpret_cast<_Float16 __attribute__((ext_vector_type(2)))*>(s1.grad_lp);
             {
                 OpScope t(ctx, "bin_records_pair", s);

@@ -1046,7 +1046,9 @@ __global__ __launch_bounds__(kBinVert) void k_bin_levels(BinArgs b, uint32_t n_c
             b.rec_key[d] = (ent & (kSegEntries - 1u)) | (c << kSegShift) | (u << 16);
             b.rec_val[d] = record_value(lc, yz, vg[vi], c, src & 1u);
             if (DUAL) b.rec_val2[d] = record_value(lc, yz, vg2[vi], c, src & 1u);
-        } else {  // segment full: add directly
+        } else {  // segment full: add directly (the optimizer then scans the table gradient)
+            if (b.scan_flag) atomicOr(b.scan_flag, 1u);
+            if (DUAL && b.scan_flag2) atomicOr(b.scan_flag2, 1u);
             const uint32_t partner = ent ^ ((2u << (c & 3u)) - 1u);
 #pragma unroll
             for (int net2 = 0; net2 < (DUAL ? 2 : 1); ++net2) {
@@ -1161,12 +1163,21 @@ __global__ __launch_bounds__(kApplyThreads) void k_apply_bins(ApplyArgs a) {
     if (bad) bad_s = 1u;
     __syncthreads();
     h2v* g = a.grad + sg.entry0;
+    bool inf = false;  // a written entry is not finite (an FP16 overflow of the sum)
     for (uint32_t e = threadIdx.x; e < sg.n_entries; e += kApplyThreads) {
         const long long s0 = (long long)acc_s[e], s1 = (long long)acc_s[kSegEntries + e];
-        if (s0 != 0 || s1 != 0) g[e] = add_fixed(g[e], s0, s1);
+        if (s0 != 0 || s1 != 0) {
+            const h2v v = add_fixed(g[e], s0, s1);
+            g[e] = v;
+            inf |= !isfinite((float)v.x) || !isfinite((float)v.y);
+        }
     }
+    if (a.nonfinite && __any(inf) && (threadIdx.x & 63) == 0) atomicOr(a.nonfinite, 1u);
     if (threadIdx.x == 0) {
-        if (bad_s) g[0] = h2v{(_Float16)__builtin_nanf(""), (_Float16)__builtin_nanf("")};
+        if (bad_s) {
+            g[0] = h2v{(_Float16)__builtin_nanf(""), (_Float16)__builtin_nanf("")};
+            if (a.nonfinite) atomicOr(a.nonfinite, 1u);
+        }
         a.seg_count[seg] = 0;
     }
 }
@@ -1192,18 +1203,23 @@ __global__ __launch_bounds__(kApplyThreads) void k_apply_chunks(ApplyArgs a) {
 }
 
 __global__ void k_apply_finish(ApplyArgs a) {
+    bool inf = false;
     for (uint32_t e = blockIdx.x * blockDim.x + threadIdx.x; e < a.n_acc; e += gridDim.x * blockDim.x) {
         const long long s0 = (long long)a.acc[2 * e], s1 = (long long)a.acc[2 * e + 1];
         if (s0 != 0 || s1 != 0) {
-            a.grad[e] = add_fixed(a.grad[e], s0, s1);
+            const h2v v = add_fixed(a.grad[e], s0, s1);
+            a.grad[e] = v;
+            inf |= !isfinite((float)v.x) || !isfinite((float)v.y);
             a.acc[2 * e] = 0ull;
             a.acc[2 * e + 1] = 0ull;
         }
     }
+    if (a.nonfinite && __any(inf) && (threadIdx.x & 63) == 0) atomicOr(a.nonfinite, 1u);
     if (blockIdx.x == 0) {
         for (uint32_t k = threadIdx.x; k < a.n_chunked; k += blockDim.x) a.seg_count[a.chunked[k]] = 0;
         if (threadIdx.x == 0 && *a.bad) {
             a.grad[0] = h2v{(_Float16)__builtin_nanf(""), (_Float16)__builtin_nanf("")};
+            if (a.nonfinite) atomicOr(a.nonfinite, 1u);
             *a.bad = 0u;
         }
     }

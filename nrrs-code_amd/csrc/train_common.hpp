@@ -49,6 +49,8 @@ struct ApplyArgs {
     uint32_t n_acc;                  // entries covered by acc (the dense levels come first)
     uint32_t* bad;                   // non-finite record seen by the chunked apply
     _Float16 __attribute__((ext_vector_type(2)))* grad;  // FP16 pair table gradient
+    uint32_t* nonfinite;             // the optimizer's non-finite-gradient flag: set when a sum or a
+                                     // gradient entry written is not finite
 };
 
 struct TrainArgs {
@@ -123,6 +125,10 @@ struct BinArgs {
     uint32_t seg_base[kMaxLevels];   // first segment of each level
     uint32_t seg_nbins[kMaxLevels];  // segments of each level
     _Float16 __attribute__((ext_vector_type(2)))* grad;  // FP16 pair table gradient (full segments)
+    // set when a full segment's records are added to the table gradient directly (their sums
+    // are then not checked by the apply kernels; the optimizer scans the table), or null
+    uint32_t* scan_flag;
+    uint32_t* scan_flag2;
     // second net of a joint step (same grid, same record slots), or null: its dL/denc, its
     // record values (the keys are shared) and its table gradient
     const uint32_t* gdl2;
@@ -163,7 +169,12 @@ struct AdamArgs {
     float lr, beta1, beta2, eps, l2;
     float* loss_scale;    // device
     uint32_t* state;      // device: [0] step, [1] good steps, [2] skipped, [3] non-finite gradient flag,
-                          // [4] non-finite batch losses (read and cleared by nrrs_train_state)
+                          // [4] non-finite batch losses (read and cleared by nrrs_train_state),
+                          // [5] the table gradient was written by direct atomics (scan it)
+    // Binned contexts, whose apply kernels report non-finite table-gradient entries themselves:
+    // the finite check scans the FP16 table gradient only when state[5] is set. Else false: it
+    // always scans.
+    bool scan_on_flag;
     int dynamic_scale;
     uint32_t growth_interval;
 };
@@ -185,6 +196,7 @@ __device__ inline void adam_finish_state(const AdamArgs& a) {
         }
     }
     st[3] = 0;
+    st[5] = 0;
 }
 // The finite check and the update; with `finish` also the state update (else the caller runs
 // adam_finish_state, as launch_after_adam does).
