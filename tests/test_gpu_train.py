@@ -149,6 +149,16 @@ def test_low_precision_table_gradient_fallbacks(torch_cuda, ora, case):
     print(f"{case}: grid grad norm err {n_grid:.2e}, mlp grad err {e_mlp:.2e}")
     assert e_mlp < GRAD_TOL[prec]
     assert n_grid < GRID_NORM_TOL[prec]
+    # the optimizer step after these gradients is taken; after a backward whose scale overflows
+    # FP16 (records and direct atomics alike) it is skipped
+    ctx.adam_step(net)
+    assert ctx.train_state(net)["step"] == 1 and ctx.train_state(net)["skipped"] == 0
+    ctx.grads_zero(net)
+    na.check(ctx.backward(net, vb, na.LOSS_EXTERNAL, film=film, dy=torch.from_numpy(dy).cuda(), loss_scale=2.0**40),
+             "backward")
+    ctx.adam_step(net)
+    st = ctx.train_state(net)
+    assert st["step"] == 1 and st["skipped"] == 1
 
 
 @pytest.mark.parametrize("n", [777, 300])
