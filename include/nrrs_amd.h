@@ -142,7 +142,33 @@ typedef struct {
      * statistics of the batch (the reference's ll2 buffer). NRRS_Q_RRSNET_CHAINED and the frame
      * kernel return NRRS_ERR_UNSUPPORTED with 1. */
     int32_t rrs_aid;
+    /* Network set. 0 = the benchmark nets (default): 64-wide ReLU MLPs on the 80-channel input
+     * (grid, SH-4 of direction and normal, per-vertex channels). 1 = the reference's shipped
+     * nets (2net-simple.json:29-75, 210-247, 250-291): 32-wide hidden layers with LeakyReLU
+     * (slope 0.01), no bias. StatNet reads NRRSNetworkInput0 (train.h:22-29: position, lat-long
+     * direction, warped roughness) encoded as HashGrid (cfg.grid) + OneBlob of the two
+     * direction coordinates (4 bins each) + OneBlob of the roughness (8 bins). RRSNet reads
+     * NRRSNetworkInput1 (train.h:39-56: Box-Cox of the StatNet statistics, of the throughput
+     * and of the reference mean, the roughness and a constant 1) and has no grid; with
+     * rrs_aid it reads NRRSNetworkInput1AID instead (train.h:58-68): HashGrid + OneBlob of the
+     * direction (4 bins each) + OneBlob of the roughness (4 bins) + Box-Cox of the reference mean
+     * and the throughput. The layers keep the 64-wide, 80-channel storage of set 0: the weights
+     * outside the 32-wide net are held at zero (nrrs_params_set zeroes them; they receive zero
+     * gradients). Requires ll2_two_head = 0 and statnet / rrsnet of 3 hidden layers; without
+     * rrs_aid, RRSNet always reads statistics (NRRS_Q_RRSNET is refused); the frame kernel
+     * returns NRRS_ERR_UNSUPPORTED. nrrs_shipped_config fills the whole shipped configuration. */
+    int32_t net_set;
+    /* Ema optimizer (Ema{Adam}, 2net-simple.json:16-18, 171-173): 0 = off (default); in (0, 1):
+     * every Adam step also updates the debiased exponential moving average of each net's
+     * parameters with this decay, and the inference entry points (nrrs_statnet_infer,
+     * nrrs_rrsnet_infer, nrrs_rrs_infer_decide, nrrs_frame_*) read the averaged parameters;
+     * training reads the trained ones. */
+    float ema_decay;
 } nrrs_config;
+
+/* Network sets of nrrs_config.net_set. */
+#define NRRS_NETS_BENCH 0
+#define NRRS_NETS_SHIPPED 1
 
 /* SoA path vertices of one bounce (replaces NRRSInferenceWorkItem, workitem.h:9-15). */
 typedef struct {
@@ -185,6 +211,15 @@ typedef struct {
 
 /* Fills the synthetic-benchmark defaults (16-level grid, 64-wide MLPs, render/test.json RRS knobs). */
 nrrs_status nrrs_default_config(nrrs_config* cfg);
+
+/* Fills the reference's shipped configuration (common/configs/nrrs/nrrs/2net-simple.json with
+ * the render/test.json RRS knobs): net_set NRRS_NETS_SHIPPED; grid 8 levels x 2 features,
+ * T = 2^19, base 16, scale 2.0 (lines 46-51); StatNet and RRSNet 3 hidden layers
+ * (lines 74, 246); NRRS_LL2 with clamp 10 (lines 3-4); NRRS_RRS gammas 0.01, 0.05, 0, 0.01
+ * (lines 145-148); Adam lr 5e-3 / 5e-4, beta (0.9, 0.99), eps 1e-15, l2 3e-6 inside
+ * Ema(decay 0.95) (lines 16-28, 171-183). Scene, capacities and precision as
+ * nrrs_default_config. */
+nrrs_status nrrs_shipped_config(nrrs_config* cfg);
 
 /* sizeof of the ABI structs, so a binding can check its own layout (no reference counterpart). */
 size_t nrrs_struct_size(int32_t which); /* 0 config, 1 vertices, 2 decide_args, 3 vertices_out */

@@ -146,10 +146,33 @@ uint64_t mlp_count(const nrrs_mlp_config& m) {
 bool valid_shapes(const nrrs_config& c) {
     // Shapes the fused kernels are compiled for: StatNet 4 hidden -> 6 (one head), 3 hidden
     // (PartShare) or 2 hidden (JustEncoding, and each net of AllSelf) -> 6; RRSNet 3 hidden -> 1.
+    // The shipped nets: one-headed StatNet 3 hidden -> 6, RRSNet 3 hidden -> 1.
     static const uint32_t stat_hidden[4] = {4, 3, 2, 2};
     if (c.ll2_two_head < 0 || c.ll2_two_head > 3) return false;
+    if (c.net_set == NRRS_NETS_SHIPPED)
+        return c.ll2_two_head == 0 && c.statnet.n_hidden_layers == 3 && c.statnet.n_outputs == 6 &&
+               c.rrsnet.n_hidden_layers == 3 && c.rrsnet.n_outputs == 1;
     return c.statnet.n_hidden_layers == stat_hidden[c.ll2_two_head] && c.statnet.n_outputs == 6 &&
            c.rrsnet.n_hidden_layers == 3 && c.rrsnet.n_outputs == 1;
+}
+
+// Whether a net has a hash grid: every net but the shipped RRSNet without the AID input, which
+// reads NRRSNetworkInput1 alone (network.cpp:503-507 builds it without an encoding).
+bool has_grid(const nrrs_config& c, int net) {
+    return !(net == NRRS_RRSNET && c.net_set == NRRS_NETS_SHIPPED && !c.rrs_aid);
+}
+
+// Shipped nets: the live part of the 64-wide, 80-channel storage. Hidden layers are 32 wide;
+// layer 0 reads the net's input channels (StatNet: grid and OneBlob channels; RRSNet: the 16
+// NRRSNetworkInput1 channels [64,80); AID RRSNet: grid, OneBlob, BC(thp) and BC(ref mean)).
+bool shipped_live(const nrrs_config& c, int net, int layer, int n_layers, int row, int col) {
+    if (layer < n_layers - 1 && row >= 32) return false;
+    if (layer > 0) return col < 32;
+    const int grid_ch = has_grid(c, net) ? 2 * (int)c.grid.n_levels : 0;
+    if (col < grid_ch) return true;
+    if (net != NRRS_RRSNET) return col >= 32 && col < 48;
+    if (!c.rrs_aid) return col >= 64;
+    return (col >= 32 && col < 44) || (col >= 65 && col < 69);
 }
 
 // AllSelf (ll2_two_head = 3): the StatNet of L (NRRS_STATNET) and the StatNet of L^2
@@ -173,6 +196,21 @@ const char* op_label(int net, const char* statnet, const char* rrsnet, const cha
 // Zero the weights between the two heads of a two-headed StatNet (mlp: that net's MLP part); an
 // AllSelf StatNet's output rows of the other head.
 void zero_between_heads(const nrrs_config& c, int net, float* mlp) {
+    if (c.net_set == NRRS_NETS_SHIPPED) {  // the weights outside the 32-wide net
+        const nrrs_mlp_config& m = net == NRRS_RRSNET ? c.rrsnet : c.statnet;
+        const int nl = (int)m.n_hidden_layers + 1;
+        size_t pos = 0;
+        int in = kEnc;
+        for (int layer = 0; layer < nl; ++layer) {
+            const int o = layer == nl - 1 ? (int)m.n_outputs : kHidden;
+            for (int r = 0; r < o; ++r)
+                for (int k = 0; k < in; ++k)
+                    if (!shipped_live(c, net, layer, nl, r, k)) mlp[pos + (size_t)r * in + k] = 0.0f;
+            pos += (size_t)o * in;
+            in = o;
+        }
+        return;
+    }
     if (net == NRRS_RRSNET || !c.ll2_two_head) return;
     const int nl = (int)c.statnet.n_hidden_layers + 1, first = head_first_layer(c.ll2_two_head, nl);
     if (all_self(c)) {
@@ -217,8 +255,12 @@ void init_params(const nrrs_config& c, int net, uint64_t n_grid, std::vector<flo
     for (int layer = 0; layer < nl; ++layer) {
         int o = layer == nl - 1 ? (int)m.n_outputs : kHidden;
         const bool split = first >= 0 && layer >= first;
-        const int fan_out = split || (head_of(c, net) >= 0 && layer == nl - 1) ? o / 2 : o;
-        const int fan_in = split && layer > first ? in / 2 : in;
+        int fan_out = split || (head_of(c, net) >= 0 && layer == nl - 1) ? o / 2 : o;
+        int fan_in = split && layer > first ? in / 2 : in;
+        if (c.net_set == NRRS_NETS_SHIPPED) {  // 32-wide layers; the inputs of the net's own encoding
+            fan_out = layer == nl - 1 ? o : 32;
+            fan_in = layer > 0 ? 32 : (net == NRRS_RRSNET && !c.rrs_aid ? 16 : 32);
+        }
         double lim = std::sqrt(6.0 / (double)(fan_in + fan_out));
         for (int k = 0; k < o * in; ++k) out[pos++] = (float)(lim * (2.0 * r.uniform() - 1.0));
         in = o;
@@ -237,6 +279,7 @@ EncodeArgs encode_args(nrrs_ctx* ctx, const nrrs_vertices& v, const float* film,
     a.grid = ctx->grid;
     a.frame = ctx->frame;
     a.aid = ctx->cfg.rrs_aid;
+    a.net_set = ctx->cfg.net_set;
     return a;
 }
 
@@ -263,9 +306,11 @@ nrrs_status refresh_lp(nrrs_ctx* c, int net, hipStream_t s) {
 }
 
 // One fused encode + MLP launch of `net` in the context's precision.
-hipError_t run_net(nrrs_ctx* c, int net, const EncodeArgs& a, int epi, uint32_t n, const EpiArgs& ep,
+hipError_t run_net(nrrs_ctx* c, int net, const EncodeArgs& a_in, int epi, uint32_t n, const EpiArgs& ep,
                    hipStream_t s) {
     const NetState& ns = c->net[net];
+    EncodeArgs a = a_in;
+    a.grid = ns.grid;  // the net's own grid (none for the shipped RRSNet)
     OpScope t(c, op_label(net, "infer_statnet", "infer_rrsnet", "infer_statnet2"), s);
     const int k = kind_of(net);
     if (c->cfg.precision == NRRS_FP32) {
@@ -334,9 +379,10 @@ nrrs_status ensure_train(nrrs_ctx* c, int net) {
     NRRS_TRY(dmalloc(&ns.dw_partial, (size_t)c->max_train_blocks * ns.n_mlp));
     {
         // dense levels come first: their entries end where the first hashed level starts
+        const GridParams& gp = ns.grid;
         uint32_t nd = 0;
-        for (uint32_t l = 0; l < c->grid.n_levels; ++l)
-            if ((c->grid.dense_mask >> l) & 1u) nd = (l + 1 < c->grid.n_levels) ? c->grid.offset[l + 1] : c->grid.n_entries;
+        for (uint32_t l = 0; l < gp.n_levels; ++l)
+            if ((gp.dense_mask >> l) & 1u) nd = (l + 1 < gp.n_levels) ? gp.offset[l + 1] : gp.n_entries;
         ns.n_dense = nd;
         const size_t bytes = (size_t)kDenseGradReplicas * nd * (c->cfg.precision == NRRS_FP32 ? 8 : 4);
         if (bytes) {
@@ -344,7 +390,7 @@ nrrs_status ensure_train(nrrs_ctx* c, int net) {
             NRRS_HIP(hipMemset(ns.dense_rep, 0, bytes));
         }
     }
-    if (c->cfg.precision != NRRS_FP32) {
+    if (c->cfg.precision != NRRS_FP32 && ns.grid.n_levels) {
         // Binned scatter: every level cut into segments of kSegEntries entries, each with room
         // for 1.25x (hashed levels, spread by the hash) or 3x (dense levels, whose segments see
         // the geometry) its share of a full batch's records (4.5 per vertex on average: four
@@ -509,6 +555,7 @@ nrrs_status run_backward(nrrs_ctx* c, int net, uint32_t n, const nrrs_vertices& 
     if (n > cfg.max_train) return NRRS_ERR_SHAPE;  // per-sample training buffers hold max_train
     TrainArgs t{};
     t.a = encode_args(c, v, film, stats, stats_stride);
+    t.a.grid = ns.grid;
     // the RRSNet training input takes activationSigma(max(L2, 0)) when StatNet models sigma
     // (train.cu:196-200); inference keeps max(L2, 0)
     t.a.sigma_l2 = cfg.train_sigma && net == NRRS_RRSNET;
@@ -694,7 +741,7 @@ nrrs_status alloc_frame(nrrs_ctx* c, uint32_t cap) {
 nrrs_status frame_params(nrrs_ctx* c, const nrrs_frame_args* a, FrameParams* f) {
     // the frame kernel runs the one-headed StatNet of the benchmark configuration and chains its
     // statistics into RRSNet
-    if (c->cfg.ll2_two_head || c->cfg.rrs_aid) return NRRS_ERR_UNSUPPORTED;
+    if (c->cfg.ll2_two_head || c->cfg.rrs_aid || c->cfg.net_set != NRRS_NETS_BENCH) return NRRS_ERR_UNSUPPORTED;
     if (!a || !a->film_dev || !a->queue_dev || !a->sizes_dev || !a->status_dev) return NRRS_ERR_ARG;
     if (a->n_bounces < 1 || a->n_bounces > 16 || a->capacity == 0) return NRRS_ERR_ARG;
     if (a->capacity > c->cfg.max_queue) return NRRS_ERR_SHAPE;
@@ -881,6 +928,40 @@ nrrs_status nrrs_default_config(nrrs_config* c) {
     c->fp32_level_major = 1;
     c->ll2_two_head = 0;
     c->rrs_aid = 0;
+    c->net_set = NRRS_NETS_BENCH;
+    c->ema_decay = 0.0f;
+    return NRRS_OK;
+}
+
+nrrs_status nrrs_shipped_config(nrrs_config* c) {
+    NRRS_TRY(nrrs_default_config(c));
+    c->net_set = NRRS_NETS_SHIPPED;
+    c->grid.n_levels = 8;  // 2net-simple.json:46-51
+    c->grid.n_features = 2;
+    c->grid.log2_table_size = 19;
+    c->grid.base_resolution = 16;
+    c->grid.per_level_scale = 2.0f;
+    c->statnet.n_hidden_layers = 3;  // lines 74, 246
+    c->statnet.n_outputs = 6;
+    c->rrsnet.n_hidden_layers = 3;
+    c->rrsnet.n_outputs = 1;
+    c->loss_clamp_on = 1;  // lines 3-5
+    c->loss_clamp_max = 10.0f;
+    c->train_sigma = 0;
+    c->gamma[0] = 0.01f;  // lines 145-148
+    c->gamma[1] = 0.05f;
+    c->gamma[2] = 0.0f;
+    c->gamma[3] = 0.01f;
+    // the integrator switches the RRS loss to its on-policy step 3 (NRRSNetwork::setStep,
+    // network.cpp:578-590) once the StatNet is trained; the JSON's step 1 is its start value
+    c->rrs_step = 3;
+    c->lr_statnet = 5e-3f;  // lines 16-28
+    c->lr_rrsnet = 5e-4f;   // lines 171-183
+    c->adam_beta1 = 0.9f;
+    c->adam_beta2 = 0.99f;
+    c->adam_eps = 1e-15f;
+    c->adam_l2_reg = 3e-6f;
+    c->ema_decay = 0.95f;
     return NRRS_OK;
 }
 
@@ -893,6 +974,9 @@ nrrs_status nrrs_grid_layout(const nrrs_grid_config* g, uint32_t* res, uint32_t*
 nrrs_status nrrs_create(const nrrs_config* cfg, nrrs_ctx** out) {
     if (!cfg || !out) return NRRS_ERR_ARG;
     *out = nullptr;
+    if (cfg->net_set != NRRS_NETS_BENCH && cfg->net_set != NRRS_NETS_SHIPPED) return NRRS_ERR_ARG;
+    if (!(cfg->ema_decay >= 0.0f && cfg->ema_decay < 1.0f)) return NRRS_ERR_ARG;
+    if (cfg->ema_decay != 0.0f) return NRRS_ERR_UNSUPPORTED;
     if (!valid_shapes(*cfg)) return NRRS_ERR_UNSUPPORTED;
     if (cfg->train_sigma != 0 && cfg->train_sigma != 1) return NRRS_ERR_ARG;
     if (cfg->rrs_aid != 0 && cfg->rrs_aid != 1) return NRRS_ERR_ARG;
@@ -930,15 +1014,23 @@ nrrs_status nrrs_create(const nrrs_config* cfg, nrrs_ctx** out) {
     }
 
     nrrs_status st = NRRS_OK;
-    // paired table: one 32-bit word per entry and net, n_grid / 2 entries
-    if (cfg->precision != NRRS_FP32 && cfg->paired_tables) st = dmalloc(&c->grid_pair, n_grid);
+    // paired table: one 32-bit word per entry and net, n_grid / 2 entries (both nets with a grid)
+    if (cfg->precision != NRRS_FP32 && cfg->paired_tables && has_grid(*cfg, NRRS_RRSNET))
+        st = dmalloc(&c->grid_pair, n_grid);
     std::vector<float> host;
     for (int net = 0; net < n_nets(*cfg) && st == NRRS_OK; ++net) {
         const nrrs_mlp_config& m = mlp_of(*cfg, net);
         NetState& ns = c->net[net];
         ns.n_layers = (int)m.n_hidden_layers + 1;
         ns.n_out = (int)m.n_outputs;
-        ns.n_grid = n_grid;
+        if (has_grid(*cfg, net)) {
+            ns.grid = c->grid;
+            ns.n_grid = n_grid;
+        } else {
+            ns.grid = GridParams{};  // n_levels 0: the encoders write no grid channels
+            ns.grid.hash_mask = c->grid.hash_mask;
+            ns.n_grid = 0;
+        }
         ns.n_mlp = mlp_count(m);
         if ((st = dmalloc(&ns.params, ns.n_grid + ns.n_mlp)) != NRRS_OK) break;
         ns.grid_f32 = ns.params;
@@ -947,10 +1039,10 @@ nrrs_status nrrs_create(const nrrs_config* cfg, nrrs_ctx** out) {
             if ((st = dmalloc(&ns.grid_lp, ns.n_grid)) != NRRS_OK) break;
             if ((st = dmalloc(&ns.mlp_lp, mfma_packed_halves(ns.n_layers))) != NRRS_OK) break;
         }
-        init_params(*cfg, net, n_grid, host);
+        init_params(*cfg, net, ns.n_grid, host);
         st = hip_status(hipMemcpy(ns.grid_f32, host.data(), ns.n_grid * 4, hipMemcpyHostToDevice));
         if (st == NRRS_OK)
-            st = hip_status(hipMemcpy(ns.mlp_f32, host.data() + n_grid, ns.n_mlp * 4, hipMemcpyHostToDevice));
+            st = hip_status(hipMemcpy(ns.mlp_f32, host.data() + ns.n_grid, ns.n_mlp * 4, hipMemcpyHostToDevice));
         if (st == NRRS_OK) st = refresh_lp(c, net, nullptr);
         if (st == NRRS_OK) st = hip_status(hipDeviceSynchronize());
     }
@@ -982,7 +1074,8 @@ nrrs_status nrrs_create(const nrrs_config* cfg, nrrs_ctx** out) {
     if (st == NRRS_OK && cfg->max_train && all_self(*cfg)) st = ensure_train(c, NRRS_STATNET2);
     if (st == NRRS_OK) st = ensure_nodes(c);
     if (st == NRRS_OK) st = dmalloc(&c->err_part, error_partial_count());
-    if (st == NRRS_OK && cfg->precision != NRRS_FP32 && !cfg->ll2_two_head && !cfg->rrs_aid)
+    if (st == NRRS_OK && cfg->precision != NRRS_FP32 && !cfg->ll2_two_head && !cfg->rrs_aid &&
+        cfg->net_set == NRRS_NETS_BENCH)
         st = alloc_frame(c, cfg->max_queue);
     if (st == NRRS_OK) st = hip_status(hipDeviceSynchronize());
     if (st != NRRS_OK) {
@@ -1021,7 +1114,8 @@ nrrs_status nrrs_params_set(nrrs_ctx* ctx, int32_t net, const float* host_src) {
     NetState& ns = ctx->net[net];
     NRRS_HIP(hipDeviceSynchronize());
     NRRS_HIP(hipMemcpy(ns.grid_f32, host_src, ns.n_grid * 4, hipMemcpyHostToDevice));
-    if (net != NRRS_RRSNET && ctx->cfg.ll2_two_head) {  // the weights between the heads stay zero
+    // the weights between the heads, or outside the shipped 32-wide nets, stay zero
+    if ((net != NRRS_RRSNET && ctx->cfg.ll2_two_head) || ctx->cfg.net_set == NRRS_NETS_SHIPPED) {
         std::vector<float> mlp(host_src + ns.n_grid, host_src + ns.n_grid + ns.n_mlp);
         zero_between_heads(ctx->cfg, net, mlp.data());
         NRRS_HIP(hipMemcpy(ns.mlp_f32, mlp.data(), ns.n_mlp * 4, hipMemcpyHostToDevice));
@@ -1038,6 +1132,7 @@ nrrs_status nrrs_encode(nrrs_ctx* ctx, int32_t net, uint32_t n, const nrrs_verti
     if (!ctx || !net_ok(ctx, net) || !vertices_ok(v, n) || !enc_dev) return NRRS_ERR_ARG;
     if (net == NRRS_RRSNET && !film_dev) return NRRS_ERR_ARG;
     EncodeArgs a = encode_args(ctx, *v, film_dev, net == NRRS_RRSNET ? stats_dev : nullptr, n);
+    a.grid = ctx->net[net].grid;
     return hip_status(launch_encode(a, kind_of(net), n, ctx->net[net].grid_f32, enc_dev, (hipStream_t)stream));
 }
 
@@ -1058,6 +1153,8 @@ nrrs_status nrrs_statnet_infer(nrrs_ctx* ctx, uint32_t n, const nrrs_vertices* v
 nrrs_status nrrs_rrsnet_infer(nrrs_ctx* ctx, uint32_t n, const nrrs_vertices* v, const float* film_dev,
                               const float* stats_dev, float* raw_dev, void* stream) {
     if (!ctx || !vertices_ok(v, n) || !film_dev || !raw_dev) return NRRS_ERR_ARG;
+    // the shipped RRSNet without the AID input reads the StatNet statistics (NRRSNetworkInput1)
+    if (!stats_dev && ctx->cfg.net_set == NRRS_NETS_SHIPPED && !ctx->cfg.rrs_aid) return NRRS_ERR_ARG;
     EncodeArgs a = encode_args(ctx, *v, film_dev, stats_dev, n);
     EpiArgs ep{};
     ep.out = raw_dev;
@@ -1077,6 +1174,9 @@ nrrs_status nrrs_rrs_infer_decide(nrrs_ctx* ctx, const nrrs_decide_args* a, void
     if (!vertices_ok(&a->v, a->n) || !a->film_dev) return NRRS_ERR_ARG;
     // the AID input has no statistics channels: RRSNet runs alone (network.cpp:303-343)
     if (a->q_mode == NRRS_Q_RRSNET_CHAINED && ctx->cfg.rrs_aid) return NRRS_ERR_UNSUPPORTED;
+    // the shipped RRSNet without the AID input runs only on StatNet statistics
+    if (a->q_mode == NRRS_Q_RRSNET && ctx->cfg.net_set == NRRS_NETS_SHIPPED && !ctx->cfg.rrs_aid)
+        return NRRS_ERR_UNSUPPORTED;
     hipStream_t s = (hipStream_t)stream;
     {
         OpScope t(ctx, "zero_sum", s);
@@ -1090,7 +1190,7 @@ nrrs_status nrrs_rrs_infer_decide(nrrs_ctx* ctx, const nrrs_decide_args* a, void
         EncodeArgs ea = encode_args(ctx, a->v, a->film_dev, nullptr, 0);
         NRRS_HIP(run_net(ctx, NRRS_STATNET, ea, EPI_ADRRS, a->n, ep, s));
     } else if (a->q_mode == NRRS_Q_RRSNET_CHAINED && ctx->cfg.precision != NRRS_FP32 && !ctx->cfg.ll2_two_head &&
-               ctx->grid_enc && ctx->grid_enc2 && a->n <= ctx->grid_enc_cap) {
+               ctx->cfg.net_set == NRRS_NETS_BENCH && ctx->grid_enc && ctx->grid_enc2 && a->n <= ctx->grid_enc_cap) {
         // both nets' level-major grid encodings (one pass over the paired table, else a pass per
         // net), then StatNet -> RRSNet -> q in one launch
         const nrrs_config& cfg = ctx->cfg;

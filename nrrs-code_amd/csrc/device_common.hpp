@@ -214,6 +214,42 @@ __device__ __forceinline__ float norm_coord(float p, float c, float d) {
     return fminf(fmaxf(x, 0.0f), 1.0f);
 }
 
+// ---------------------------------------------------------------- shipped-net encodings
+// worldToLatLong (KiRaRay, called at device.cu:102 and workqueue.h:160; the library is not in
+// the checkout): u = atan2(x, -z) / (2 pi) + 1/2, v = acos(y) / pi of the normalised direction.
+__device__ __forceinline__ void lat_long(const float* d, float* uv) {
+    const float inv = 1.0f / sqrtf(d[0] * d[0] + d[1] * d[1] + d[2] * d[2]);
+    const float x = d[0] * inv, y = d[1] * inv, z = d[2] * inv;
+    uv[0] = atan2f(x, -z) * 0.15915494309189535f + 0.5f;
+    uv[1] = acosf(fminf(fmaxf(y, -1.0f), 1.0f)) * 0.3183098861837907f;
+}
+
+// OneBlob encoding of x in [0,1] over `bins` bins (tiny-cuda-nn's "OneBlob", 2net-simple.json:
+// 53-66): bin k holds the mass that a biweight kernel K(u) = 15/16 (1 - u^2)^2 of radius one bin
+// width, centred at x and wrapped periodically onto [0,1), puts on [k / bins, (k+1) / bins).
+// With C the kernel's CDF, C(u) = 1/2 + 15/16 u (1 - 2u^2/3 + u^4/5) on [-1,1],
+// bin k = sum over s in {-1,0,1} of C(bins ((k+1)/bins - x - s)) - C(bins (k/bins - x - s)).
+// The bins sum to 1.
+__device__ __forceinline__ float quartic_cdf(float u) {
+    u = fminf(fmaxf(u, -1.0f), 1.0f);
+    const float u2 = u * u;
+    return 0.5f + 0.9375f * u * (1.0f - u2 * (2.0f / 3.0f) + u2 * u2 * 0.2f);
+}
+__device__ __forceinline__ float one_blob_cdf(float b, float x, float bins) {
+    return quartic_cdf((b - x) * bins) + quartic_cdf((b - x - 1.0f) * bins) + quartic_cdf((b - x + 1.0f) * bins);
+}
+template <int BINS>
+__device__ __forceinline__ void one_blob(float x, float* o) {
+    const float fb = (float)BINS;
+    float left = one_blob_cdf(0.0f, x, fb);
+#pragma unroll
+    for (int k = 0; k < BINS; ++k) {
+        const float right = one_blob_cdf((float)(k + 1) / fb, x, fb);
+        o[k] = right - left;
+        left = right;
+    }
+}
+
 // ---------------------------------------------------------------- SH degree 4
 // Real spherical harmonics, bands 0..3 (Sloan 2008), written to o[0..15].
 __device__ __forceinline__ void sh4(float x, float y, float z, float* o) {
@@ -475,7 +511,7 @@ __device__ __forceinline__ float4 grid_level_half(const uint32_t* __restrict__ t
 }
 
 // ---------------------------------------------------------------- encoding
-// Network input layout (80 channels):
+// Network input layout (80 channels) of the benchmark nets (net_set 0):
 //   [0,32)  hash grid, feature f of level l at 2l+f
 //   [32,48) SH4(direction)      [48,64) SH4(normal)
 //   StatNet [64] warp(rough) [65,68) albedo_d [68,71) albedo_s [71] 1 [72,80) 0
@@ -484,6 +520,14 @@ __device__ __forceinline__ float4 grid_level_half(const uint32_t* __restrict__ t
 //           [75] 1 [76,80) 0
 // The constant channel stands in for the bias the fused MLP does not have
 // (the reference injects 1 as its padding channel, train.cu:94-96).
+// Shipped nets (net_set 1; 2net-simple.json:29-68, 210-291), the other channels zero:
+//   StatNet [0, 2 levels) hash grid, [32,36) OneBlob(u), [36,40) OneBlob(v) of the lat-long
+//           direction, [40,48) OneBlob(warp(rough), 8 bins)                    (32 inputs)
+//   RRSNet  [64] warp(rough) [65,68) BC(thp) [68] BC(ref mean) [69,72) BC(max(L,0))
+//           [72,75) BC(max(L2,0)) [75] 1 [76,80) 0 (NRRSNetworkInput1, no grid; 16 inputs)
+//   RRSNet with rrs_aid: [0, 2 levels) hash grid, [32,40) OneBlob(u, v), [40,44)
+//           OneBlob(warp(rough), 4 bins), [65,68) BC(thp), [68] BC(ref mean)   (32 inputs)
+// The OneBlob sums give the StatNet and the AID RRSNet their constant.
 __device__ __forceinline__ void load_xn(const EncodeArgs& a, uint32_t i, float* x) {
     const uint32_t S = a.v.stride;
 #pragma unroll
@@ -525,8 +569,43 @@ __device__ __forceinline__ float film_mean(const float* film, uint32_t n_pixels,
 // position, direction, roughness, reference mean and throughput; the normal, albedo and
 // statistics channels are zero.
 __device__ __forceinline__ void encode_tail_rec(const VertexRec& v, int net, float ref, const float* st, float* e,
-                                                bool sigma_l2 = false, bool aid = false) {
+                                                bool sigma_l2 = false, bool aid = false, int set = NRRS_NETS_BENCH) {
     aid = aid && net == NRRS_RRSNET;
+    if (set == NRRS_NETS_SHIPPED) {
+#pragma unroll
+        for (int k = 32; k < kEnc; ++k) e[k] = 0.0f;
+        const float wr = warp_roughness(v.rough);
+        if (net != NRRS_RRSNET || aid) {
+            // NRRSNetworkInput0 (train.h:22-29, nrrs_generate_inference_data, train.cu:269-287) and
+            // NRRSNetworkInput1AID (train.h:58-68, train.cu:114-153, 242-267)
+            float uv[2];
+            lat_long(v.dir, uv);
+            one_blob<4>(uv[0], e + 32);
+            one_blob<4>(uv[1], e + 36);
+            if (aid) {
+                one_blob<4>(wr, e + 40);
+#pragma unroll
+                for (int k = 0; k < 3; ++k) e[65 + k] = box_cox(v.thp[k]);
+                e[68] = box_cox(ref);
+            } else {
+                one_blob<8>(wr, e + 40);
+            }
+        } else {
+            // NRRSNetworkInput1 (train.h:39-56, nrrs_generate_inference_data_from_0to1,
+            // train.cu:41-112; training: train.cu:155-240); without statistics they count as zero
+            e[64] = wr;
+#pragma unroll
+            for (int k = 0; k < 3; ++k) {
+                e[65 + k] = box_cox(v.thp[k]);
+                const float l = st ? fmaxf(st[k], 0.0f) : 0.0f, l2 = st ? fmaxf(st[3 + k], 0.0f) : 0.0f;
+                e[69 + k] = box_cox(l);
+                e[72 + k] = box_cox(sigma_l2 ? activation_sigma(l2) : l2);
+            }
+            e[68] = box_cox(ref);
+            e[75] = 1.0f;
+        }
+        return;
+    }
     sh4(v.dir[0], v.dir[1], v.dir[2], e + 32);
     if (aid) {
 #pragma unroll
@@ -577,9 +656,9 @@ __device__ __forceinline__ void encode_tail(const EncodeArgs& a, int net, uint32
         float st[6];
 #pragma unroll
         for (int k = 0; k < 6; ++k) st[k] = a.stats[(size_t)k * a.stats_stride + i];
-        encode_tail_rec(v, net, ref, st, e, a.sigma_l2 != 0);
+        encode_tail_rec(v, net, ref, st, e, a.sigma_l2 != 0, false, a.net_set);
     } else {
-        encode_tail_rec(v, net, ref, nullptr, e, false, a.aid != 0);
+        encode_tail_rec(v, net, ref, nullptr, e, false, a.aid != 0, a.net_set);
     }
 }
 

@@ -15,6 +15,10 @@ constexpr int kHidden = 64;    // MLP width
 constexpr int kMaxLevels = 16;
 constexpr int kMaxLayers = 7;  // n_hidden_layers <= 6
 constexpr uint64_t kPcgMult = 6364136223846793005ull;
+// Hidden-layer activations: ReLU (the benchmark nets) and LeakyReLU with slope 0.01 (the
+// shipped nets, 2net-simple.json:72, 240; tiny-cuda-nn's LeakyReLU slope)
+enum Activation { ACT_RELU = 0, ACT_LEAKY = 1 };
+constexpr float kLeakySlope = 0.01f;
 
 // Two-headed StatNet (nrrs_config.ll2_two_head): the first layer split into the two 32-wide
 // heads, or -1 for one head. PartShare: the tails' two layers (hidden, output); JustEncoding:
@@ -68,6 +72,8 @@ struct EncodeArgs {
     // RRSNet's AID input: no normal, albedo or statistics channels (the statistics, when set,
     // still feed the NRRS_RRS loss)
     int aid = 0;
+    // network set of the non-grid channels (NRRS_NETS_BENCH / NRRS_NETS_SHIPPED)
+    int net_set = 0;
 };
 
 struct DecideParams {
@@ -112,6 +118,7 @@ constexpr uint32_t kNodeTile = 1024;
 
 struct NetState {
     int n_layers = 0, n_out = 0;
+    GridParams grid{};               // the net's grid (n_levels = 0: a net without one)
     uint64_t n_grid = 0, n_mlp = 0;  // FP32 parameter counts
     float* params = nullptr;         // FP32 masters [grid | mlp]
     float* grid_f32 = nullptr;       // = params: grid [entries][2]

@@ -182,9 +182,9 @@ __device__ __forceinline__ void pack_split(const float* x, typename P::vec8& hi,
 template <typename P>
 __device__ __forceinline__ void encode_tail_half(const VertexRec& v, int net, int h, float ref, const float* st,
                                                  typename P::vec8* B, typename P::vec8* Bl = nullptr,
-                                                 bool aid = false) {
+                                                 bool aid = false, int set = NRRS_NETS_BENCH) {
     float t[kEnc], e[8];
-    encode_tail_rec(v, net, ref, st, t, false, aid);
+    encode_tail_rec(v, net, ref, st, t, false, aid, set);
 #pragma unroll
     for (int kb = 2; kb < 5; ++kb) {
 #pragma unroll
@@ -200,9 +200,9 @@ template <typename P>
 __device__ __forceinline__ void encode_half_rec(const VertexRec& v, const GridParams& grid, int net,
                                                 const uint32_t* __restrict__ table, int h, float ref,
                                                 const float* st, typename P::vec8* B, typename P::vec8* Bl,
-                                                bool aid = false) {
+                                                bool aid = false, int set = NRRS_NETS_BENCH) {
     encode_grid_half<P>(v, grid, table, h, B);
-    encode_tail_half<P>(v, net, h, ref, st, B, Bl, aid);
+    encode_tail_half<P>(v, net, h, ref, st, B, Bl, aid, set);
 }
 
 template <typename P>
@@ -214,9 +214,9 @@ __device__ __forceinline__ void encode_half(const EncodeArgs& a, int net, const 
         float st[6];
 #pragma unroll
         for (int k = 0; k < 6; ++k) st[k] = a.stats[(size_t)k * a.stats_stride + i];
-        encode_half_rec<P>(v, a.grid, net, table, h, ref, st, B, Bl);
+        encode_half_rec<P>(v, a.grid, net, table, h, ref, st, B, Bl, false, a.net_set);
     } else {
-        encode_half_rec<P>(v, a.grid, net, table, h, ref, nullptr, B, Bl, a.aid != 0);
+        encode_half_rec<P>(v, a.grid, net, table, h, ref, nullptr, B, Bl, a.aid != 0, a.net_set);
     }
 }
 
@@ -248,9 +248,9 @@ __device__ __forceinline__ void encode_half_pre(const EncodeArgs& a, int net, ui
         float st[6];
 #pragma unroll
         for (int k = 0; k < 6; ++k) st[k] = a.stats[(size_t)k * a.stats_stride + i];
-        encode_tail_half<P>(v, net, h, ref, st, B, Bl);
+        encode_tail_half<P>(v, net, h, ref, st, B, Bl, false, a.net_set);
     } else {
-        encode_tail_half<P>(v, net, h, ref, nullptr, B, Bl, a.aid != 0);
+        encode_tail_half<P>(v, net, h, ref, nullptr, B, Bl, a.aid != 0, a.net_set);
     }
 }
 
@@ -258,7 +258,16 @@ __device__ __forceinline__ u32x4 lds_tile(const u32x4* __restrict__ sW, int tile
     return sW[tile * 64 + lane];
 }
 
-template <typename P>
+// Hidden-layer activation of an FP32 accumulator value (Activation).
+template <int ACT>
+__device__ __forceinline__ float act_fwd(float x) {
+    if constexpr (ACT == ACT_LEAKY)
+        return x > 0.0f ? x : x * kLeakySlope;
+    else
+        return fmaxf(x, 0.0f);
+}
+
+template <typename P, int ACT = ACT_RELU>
 __device__ __forceinline__ void relu_to_operands(const f32x16* D, typename P::vec8* H) {
 #pragma unroll
     for (int ot = 0; ot < 2; ++ot) {
@@ -266,7 +275,7 @@ __device__ __forceinline__ void relu_to_operands(const f32x16* D, typename P::ve
         for (int g = 0; g < 2; ++g) {
             float x[8];
 #pragma unroll
-            for (int j = 0; j < 8; ++j) x[j] = fmaxf(D[ot][8 * g + j], 0.0f);
+            for (int j = 0; j < 8; ++j) x[j] = act_fwd<ACT>(D[ot][8 * g + j]);
             H[2 * ot + g] = P::pack(x);
         }
     }
@@ -274,7 +283,7 @@ __device__ __forceinline__ void relu_to_operands(const f32x16* D, typename P::ve
 
 // ReLU of the last hidden layer as a two-term operand: H = lp(x) and L = lp(x - H) (the
 // difference is exact in FP32), so that H + L holds x to about 16 bits.
-template <typename P>
+template <typename P, int ACT = ACT_RELU>
 __device__ __forceinline__ void relu_to_operands_split(const f32x16* D, typename P::vec8* H, typename P::vec8* L) {
 #pragma unroll
     for (int ot = 0; ot < 2; ++ot) {
@@ -282,7 +291,7 @@ __device__ __forceinline__ void relu_to_operands_split(const f32x16* D, typename
         for (int g = 0; g < 2; ++g) {
             float x[8];
 #pragma unroll
-            for (int j = 0; j < 8; ++j) x[j] = fmaxf(D[ot][8 * g + j], 0.0f);
+            for (int j = 0; j < 8; ++j) x[j] = act_fwd<ACT>(D[ot][8 * g + j]);
             pack_split<P>(x, H[2 * ot + g], L[2 * ot + g]);
         }
     }
@@ -297,7 +306,7 @@ __device__ __forceinline__ void relu_to_operands_split(const f32x16* D, typename
 // a square root near zero, where an 8-bit-mantissa product is too coarse). SPLIT runs every
 // hidden layer the same way, and layer 0 too on its SH and per-vertex chunks (Bl: their residual
 // terms), the grid chunks staying single-term.
-template <typename P, int NL, bool SPLIT = false>
+template <typename P, int NL, bool SPLIT = false, int ACT = ACT_RELU>
 __device__ __forceinline__ f32x16 mlp_forward(const u32x4* __restrict__ sW, const typename P::vec8* B,
                                               const typename P::vec8* Bl, int lane) {
     f32x16 D[2];
@@ -319,9 +328,9 @@ __device__ __forceinline__ f32x16 mlp_forward(const u32x4* __restrict__ sW, cons
     for (int layer = 1; layer < NL - 1; ++layer) {
         const int base = hidden_base(layer, SPLIT);
         if constexpr (SPLIT)
-            relu_to_operands_split<P>(D, H, L);
+            relu_to_operands_split<P, ACT>(D, H, L);
         else
-            relu_to_operands<P>(D, H);
+            relu_to_operands<P, ACT>(D, H);
 #pragma unroll
         for (int ot = 0; ot < 2; ++ot) {
             D[ot] = f32x16{};
@@ -336,7 +345,7 @@ __device__ __forceinline__ f32x16 mlp_forward(const u32x4* __restrict__ sW, cons
             for (int kb = 0; kb < 4; ++kb) D[ot] = P::mma(lds_tile(sW, base + ot * 4 + kb, lane), H[kb], D[ot]);
         }
     }
-    relu_to_operands_split<P>(D, H, L);
+    relu_to_operands_split<P, ACT>(D, H, L);
     f32x16 Y = f32x16{};
     const int base = out_base(NL, SPLIT);
 #pragma unroll

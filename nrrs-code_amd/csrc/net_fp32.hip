@@ -33,8 +33,9 @@ __global__ __launch_bounds__(kBlock) void k_grid_indices(EncodeArgs a, uint32_t 
     }
 }
 
-// out = W h (W row-major [OUT][IN] in LDS), optional ReLU.
-template <int IN, int OUT, bool RELU>
+// out = W h (W row-major [OUT][IN] in LDS), optional hidden-layer activation (ACT_RELU /
+// ACT_LEAKY; the output layer is linear).
+template <int IN, int OUT, bool RELU, int ACT = ACT_RELU>
 __device__ __forceinline__ void dense_fp32(const float* __restrict__ sW, const float* hin, float* hout) {
 #pragma unroll
     for (int o = 0; o < OUT; ++o) {
@@ -53,7 +54,12 @@ __device__ __forceinline__ void dense_fp32(const float* __restrict__ sW, const f
             a1 = __fmaf_rn(w1.w, hin[k + 7], a1);
         }
         float acc = a0 + a1;
-        hout[o] = RELU ? fmaxf(acc, 0.0f) : acc;
+        if (!RELU)
+            hout[o] = acc;
+        else if (ACT == ACT_LEAKY)
+            hout[o] = acc > 0.0f ? acc : acc * kLeakySlope;
+        else
+            hout[o] = fmaxf(acc, 0.0f);
     }
 }
 
@@ -77,7 +83,7 @@ constexpr int mlp_floats() {
     return kHidden * kEnc + (NL - 2) * kHidden * kHidden + NOUT * kHidden;
 }
 
-template <int NL, int NOUT, int EPI>
+template <int NL, int NOUT, int EPI, int ACT>
 __global__ __launch_bounds__(kBlock) void k_net_fp32(EncodeArgs a, int net, uint32_t n,
                                                      const float* __restrict__ table,
                                                      const float* __restrict__ W, EpiArgs ep) {
@@ -110,11 +116,11 @@ __global__ __launch_bounds__(kBlock) void k_net_fp32(EncodeArgs a, int net, uint
 #pragma unroll
             for (int k = 0; k < kEnc; ++k) h[k] = 0.0f;
         }
-        dense_fp32<kEnc, kHidden, true>(sW, h, t);
+        dense_fp32<kEnc, kHidden, true, ACT>(sW, h, t);
         const float* w = sW + kHidden * kEnc;
 #pragma unroll
         for (int layer = 1; layer < NL - 1; ++layer) {
-            dense_fp32<kHidden, kHidden, true>(w, t, h);
+            dense_fp32<kHidden, kHidden, true, ACT>(w, t, h);
 #pragma unroll
             for (int k = 0; k < kHidden; ++k) t[k] = h[k];
             w += kHidden * kHidden;
@@ -157,16 +163,16 @@ __global__ __launch_bounds__(kBlock) void k_net_fp32(EncodeArgs a, int net, uint
     block_atomic_add_u64(term, ep.sum);
 }
 
-template <int NL, int NOUT, int EPI>
+template <int NL, int NOUT, int EPI, int ACT = ACT_RELU>
 hipError_t launch_one(const EncodeArgs& a, int net, uint32_t n, const float* table, const float* W,
                       const EpiArgs& ep, hipStream_t s) {
     constexpr size_t smem = sizeof(float) * mlp_floats<NL, NOUT>();
     static std::atomic<bool> attr[kMaxDevices];
-    hipError_t e = set_dyn_lds_once(attr, reinterpret_cast<const void*>(&k_net_fp32<NL, NOUT, EPI>), smem);
+    hipError_t e = set_dyn_lds_once(attr, reinterpret_cast<const void*>(&k_net_fp32<NL, NOUT, EPI, ACT>), smem);
     if (e != hipSuccess) return e;
     dim3 grid((n + kBlock - 1) / kBlock);
     if (grid.x == 0) return hipSuccess;
-    hipLaunchKernelGGL((k_net_fp32<NL, NOUT, EPI>), grid, dim3(kBlock), smem, s, a, net, n, table, W, ep);
+    hipLaunchKernelGGL((k_net_fp32<NL, NOUT, EPI, ACT>), grid, dim3(kBlock), smem, s, a, net, n, table, W, ep);
     return hipGetLastError();
 }
 
@@ -181,7 +187,7 @@ hipError_t launch_encode(const EncodeArgs& a, int net, uint32_t n, const float* 
 
 hipError_t launch_encode_levels_f32(const EncodeArgs& a, const float* table, uint32_t n, float2* out, uint32_t stride,
                                     hipStream_t s) {
-    if (!n) return hipSuccess;
+    if (!n || !a.grid.n_levels) return hipSuccess;  // no grid: nothing to gather
     const uint32_t n_chunks = (n + kChunk - 1) / kChunk;
     hipLaunchKernelGGL(k_encode_levels_f32, dim3(n_chunks * a.grid.n_levels), dim3(kChunk), 0, s, a, table, n,
                        n_chunks, out, stride);
@@ -199,6 +205,18 @@ hipError_t launch_grid_indices(const EncodeArgs& a, uint32_t n, uint32_t* idx, h
 // hipErrorInvalidValue.
 hipError_t launch_net_fp32(const EncodeArgs& a, int net, int n_layers, int n_out, int epi, uint32_t n,
                            const float* table, const float* W, const EpiArgs& ep, hipStream_t s) {
+    if (a.net_set == NRRS_NETS_SHIPPED) {
+        // the shipped nets: 3 hidden LeakyReLU layers, StatNet -> 6, RRSNet -> 1
+        if (n_layers == 4 && n_out == 6) {
+            if (epi == EPI_RAW) return launch_one<4, 6, EPI_RAW, ACT_LEAKY>(a, net, n, table, W, ep, s);
+            if (epi == EPI_ADRRS) return launch_one<4, 6, EPI_ADRRS, ACT_LEAKY>(a, net, n, table, W, ep, s);
+        }
+        if (n_layers == 4 && n_out == 1) {
+            if (epi == EPI_RAW) return launch_one<4, 1, EPI_RAW, ACT_LEAKY>(a, net, n, table, W, ep, s);
+            if (epi == EPI_RRS) return launch_one<4, 1, EPI_RRS, ACT_LEAKY>(a, net, n, table, W, ep, s);
+        }
+        return hipErrorInvalidValue;
+    }
     if (n_layers == 5 && n_out == 6) {
         if (epi == EPI_RAW) return launch_one<5, 6, EPI_RAW>(a, net, n, table, W, ep, s);
         if (epi == EPI_ADRRS) return launch_one<5, 6, EPI_ADRRS>(a, net, n, table, W, ep, s);

@@ -208,9 +208,9 @@ __device__ __forceinline__ void fwd_layer(const typename P::op* __restrict__ Wf,
     }
 }
 
-// ReLU of accumulators into operands (the ReLU outputs double as the mask [h > 0] of the
-// backward, P::mask_by).
-template <typename P>
+// Activation of accumulators into operands. ReLU: its outputs double as the mask [h > 0] of
+// the backward (P::mask_by). LeakyReLU (slope kLeakySlope): x or slope * x, rounded once.
+template <typename P, int ACT>
 __device__ __forceinline__ void relu_ops(const f32x16* D, typename P::op* H) {
 #pragma unroll
     for (int ot = 0; ot < 2; ++ot) {
@@ -219,23 +219,42 @@ __device__ __forceinline__ void relu_ops(const f32x16* D, typename P::op* H) {
             float x[8];
 #pragma unroll
             for (int j = 0; j < 8; ++j) x[j] = D[ot][8 * g + j];
-            H[2 * ot + g] = P::relu_pack(x);
+            if constexpr (ACT == ACT_LEAKY) {
+#pragma unroll
+                for (int j = 0; j < 8; ++j) x[j] = x[j] > 0.0f ? x[j] : x[j] * kLeakySlope;
+                H[2 * ot + g] = P::pack(x);
+            } else {
+                H[2 * ot + g] = P::relu_pack(x);
+            }
         }
+    }
+}
+
+// delta of a hidden layer from its FP32 accumulators x8 and the layer's activations H:
+// x * [h > 0] (ReLU), x * (h > 0 ? 1 : slope) (LeakyReLU, whose output has the sign of its input).
+template <typename P, int ACT>
+__device__ __forceinline__ typename P::op act_mask(float* x8, const typename P::op& H) {
+    if constexpr (ACT == ACT_LEAKY) {
+#pragma unroll
+        for (int j = 0; j < 8; ++j) x8[j] = P::get(H, j) > 0.0f ? x8[j] : x8[j] * kLeakySlope;
+        return P::pack(x8);
+    } else {
+        return P::mask_by(P::pack(x8), H);
     }
 }
 
 // Input of layer `upto` (> from): forward through layers from..upto-1, starting from X, the
 // input of layer `from` (the encoding when from = 0).
-template <typename P, int NL>
+template <typename P, int NL, int ACT>
 __device__ __forceinline__ void forward_to(const typename P::op* __restrict__ Wf, int from, const typename P::op* X,
                                            int upto, typename P::op* H, int lane) {
     f32x16 D[2];
     fwd_layer<P, NL>(Wf, from, X, D, lane);
     for (int layer = from + 1; layer < upto; ++layer) {
-        relu_ops<P>(D, H);
+        relu_ops<P, ACT>(D, H);
         fwd_layer<P, NL>(Wf, layer, H, D, lane);
     }
-    relu_ops<P>(D, H);
+    relu_ops<P, ACT>(D, H);
 }
 
 // The input of this layer is kept from the forward pass (a checkpoint) for the backward sweep:
@@ -247,7 +266,7 @@ constexpr int kCkpt = NL >= 4 ? 2 : 0;
 
 // BIN: dL/denc of the grid goes to t.gdl for the binned scatter (FP16/BF16); otherwise it is
 // added to the table with atomics. Separate instances keep the unused path's registers out.
-template <typename P, int NL, int NOUT, int LOSS, bool BIN>
+template <typename P, int NL, int NOUT, int LOSS, bool BIN, int ACT>
 __global__ __launch_bounds__(64 * P::kWaves) void k_train(TrainArgs t) {
     static_assert(!(BIN && P::kF32Table), "the binned scatter is for FP16/BF16 tables");
     typedef typename P::op op;
@@ -349,15 +368,15 @@ __global__ __launch_bounds__(64 * P::kWaves) void k_train(TrainArgs t) {
         constexpr int C = kCkpt<NL>;
         op Hc[4];  // input of layer C (the checkpoint)
         if constexpr (C > 0) {
-            forward_to<P, NL>(Wf, 0, E, C, Hc, lane);
+            forward_to<P, NL, ACT>(Wf, 0, E, C, Hc, lane);
             if constexpr (NL - 1 > C) {
-                forward_to<P, NL>(Wf, C, Hc, NL - 1, H, lane);
+                forward_to<P, NL, ACT>(Wf, C, Hc, NL - 1, H, lane);
             } else {
 #pragma unroll
                 for (int kb = 0; kb < 4; ++kb) H[kb] = Hc[kb];
             }
         } else {
-            forward_to<P, NL>(Wf, 0, E, NL - 1, H, lane);
+            forward_to<P, NL, ACT>(Wf, 0, E, NL - 1, H, lane);
         }
         f32x16 Y = f32x16{};
         {
@@ -494,12 +513,12 @@ __global__ __launch_bounds__(64 * P::kWaves) void k_train(TrainArgs t) {
             // (1) layer input h_l and its ReLU mask
             if (!last && layer > 0) {
                 if (C > 0 && layer > C) {
-                    forward_to<P, NL>(Wf, C, Hc, layer, H, lane);
+                    forward_to<P, NL, ACT>(Wf, C, Hc, layer, H, lane);
                 } else if (C > 0 && layer == C) {
 #pragma unroll
                     for (int kb = 0; kb < 4; ++kb) H[kb] = Hc[kb];
                 } else {
-                    forward_to<P, NL>(Wf, 0, E, layer, H, lane);
+                    forward_to<P, NL, ACT>(Wf, 0, E, layer, H, lane);
                 }
             }
             // (2) stage delta_l rows [out neuron][vertex] and h_l rows [in neuron][vertex]
@@ -576,7 +595,7 @@ __global__ __launch_bounds__(64 * P::kWaves) void k_train(TrainArgs t) {
                         float x8[8];
 #pragma unroll
                         for (int j = 0; j < 8; ++j) x8[j] = D[ot][8 * g + j];
-                        Dl[2 * ot + g] = P::mask_by(P::pack(x8), H[2 * ot + g]);
+                        Dl[2 * ot + g] = act_mask<P, ACT>(x8, H[2 * ot + g]);
                     }
             } else {
                 // dL/denc for the 32 grid channels: rows c = 8(r/4) + 4h + r%4
@@ -779,27 +798,28 @@ __global__ void k_after_adam(AdamArgs a, const float* __restrict__ W, int nl, in
     }
 }
 
-template <typename P, int NL, int NOUT, int LOSS, bool BIN>
+template <typename P, int NL, int NOUT, int LOSS, bool BIN, int ACT>
 hipError_t launch_train_one_bin(const TrainArgs& t, uint32_t blocks, hipStream_t s) {
     constexpr size_t smem =
         sizeof(Stage<P>) + (P::kWaves == 8 ? (size_t)(10 + (NL - 2) * 8 + 4 + 4 + (NL - 2) * 8 + 2) * 64 *
                                                  sizeof(typename P::op)
                                            : 0);
     static std::atomic<bool> attr[kMaxDevices];
-    hipError_t e = set_dyn_lds_once(attr, reinterpret_cast<const void*>(&k_train<P, NL, NOUT, LOSS, BIN>), smem);
+    hipError_t e =
+        set_dyn_lds_once(attr, reinterpret_cast<const void*>(&k_train<P, NL, NOUT, LOSS, BIN, ACT>), smem);
     if (e != hipSuccess) return e;
-    hipLaunchKernelGGL((k_train<P, NL, NOUT, LOSS, BIN>), dim3(blocks), dim3(Geo<P>::kThreads), smem, s, t);
+    hipLaunchKernelGGL((k_train<P, NL, NOUT, LOSS, BIN, ACT>), dim3(blocks), dim3(Geo<P>::kThreads), smem, s, t);
     return hipGetLastError();
 }
 
-template <typename P, int NL, int NOUT, int LOSS>
+template <typename P, int NL, int NOUT, int LOSS, int ACT = ACT_RELU>
 hipError_t launch_train_one(const TrainArgs& t, uint32_t blocks, hipStream_t s) {
     if constexpr (P::kF32Table) {
         if (t.gdl) return hipErrorInvalidValue;
-        return launch_train_one_bin<P, NL, NOUT, LOSS, false>(t, blocks, s);
+        return launch_train_one_bin<P, NL, NOUT, LOSS, false, ACT>(t, blocks, s);
     } else {
-        return t.gdl ? launch_train_one_bin<P, NL, NOUT, LOSS, true>(t, blocks, s)
-                     : launch_train_one_bin<P, NL, NOUT, LOSS, false>(t, blocks, s);
+        return t.gdl ? launch_train_one_bin<P, NL, NOUT, LOSS, true, ACT>(t, blocks, s)
+                     : launch_train_one_bin<P, NL, NOUT, LOSS, false, ACT>(t, blocks, s);
     }
 }
 
@@ -813,6 +833,20 @@ namespace {
 
 template <typename P>
 hipError_t dispatch_train(const TrainArgs& t, int nl, int n_out, int loss, uint32_t blocks, hipStream_t s) {
+    if (t.a.net_set == NRRS_NETS_SHIPPED) {
+        // the shipped nets: 3 hidden LeakyReLU layers, StatNet -> 6, RRSNet -> 1
+        if (nl == 4 && n_out == 6) {
+            if (loss == NRRS_LOSS_EXTERNAL)
+                return launch_train_one<P, 4, 6, NRRS_LOSS_EXTERNAL, ACT_LEAKY>(t, blocks, s);
+            if (loss == NRRS_LOSS_LL2) return launch_train_one<P, 4, 6, NRRS_LOSS_LL2, ACT_LEAKY>(t, blocks, s);
+        }
+        if (nl == 4 && n_out == 1) {
+            if (loss == NRRS_LOSS_EXTERNAL)
+                return launch_train_one<P, 4, 1, NRRS_LOSS_EXTERNAL, ACT_LEAKY>(t, blocks, s);
+            if (loss == NRRS_LOSS_RRS) return launch_train_one<P, 4, 1, NRRS_LOSS_RRS, ACT_LEAKY>(t, blocks, s);
+        }
+        return hipErrorInvalidValue;
+    }
     if (nl == 5 && n_out == 6) {
         if (loss == NRRS_LOSS_EXTERNAL) return launch_train_one<P, 5, 6, NRRS_LOSS_EXTERNAL>(t, blocks, s);
         if (loss == NRRS_LOSS_LL2) return launch_train_one<P, 5, 6, NRRS_LOSS_LL2>(t, blocks, s);

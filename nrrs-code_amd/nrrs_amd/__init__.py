@@ -16,6 +16,7 @@ OK, ERR_ARG, ERR_SHAPE, ERR_OOM, ERR_OVERFLOW, ERR_NONFINITE, ERR_HIP, ERR_UNSUP
 FP32, FP16, BF16 = 0, 1, 2
 STATNET, RRSNET, STATNET2 = 0, 1, 2  # STATNET2: the second StatNet of AllSelf (ll2_two_head 3)
 Q_RRSNET, Q_RRSNET_CHAINED, Q_STATNET_ADRRS = 0, 1, 2
+NETS_BENCH, NETS_SHIPPED = 0, 1  # nrrs_config.net_set
 ENC_WIDTH = 80
 
 
@@ -72,6 +73,8 @@ class Config(C.Structure):
         ("fp32_level_major", C.c_uint32),
         ("ll2_two_head", C.c_int32),
         ("rrs_aid", C.c_int32),
+        ("net_set", C.c_int32),
+        ("ema_decay", C.c_float),
     ]
 
 
@@ -199,6 +202,7 @@ _SIGS = {
     "nrrs_status_string": (C.c_char_p, [C.c_int]),
     "nrrs_struct_size": (C.c_size_t, [C.c_int32]),
     "nrrs_default_config": (C.c_int, [C.POINTER(Config)]),
+    "nrrs_shipped_config": (C.c_int, [C.POINTER(Config)]),
     "nrrs_create": (C.c_int, [C.POINTER(Config), C.POINTER(_P)]),
     "nrrs_destroy": (C.c_int, [_P]),
     "nrrs_grid_layout": (C.c_int, [C.POINTER(GridConfig), _P, _P, _P, _P, _P, C.POINTER(C.c_uint64)]),
@@ -282,6 +286,17 @@ def check(code, what=""):
 def default_config(**overrides):
     cfg = Config()
     check(load().nrrs_default_config(C.byref(cfg)), "default_config")
+    return _override(cfg, overrides)
+
+
+def shipped_config(**overrides):
+    """The reference's shipped configuration (nrrs_shipped_config), with overrides."""
+    cfg = Config()
+    check(load().nrrs_shipped_config(C.byref(cfg)), "shipped_config")
+    return _override(cfg, overrides)
+
+
+def _override(cfg, overrides):
     for k, v in overrides.items():
         if k in ("gamma", "aabb_min", "aabb_max"):
             arr = getattr(cfg, k)

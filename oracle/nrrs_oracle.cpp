@@ -116,6 +116,51 @@ float sigma_f(float x) {
 double bc_d(double x) { return 2.0 * (std::sqrt(x) - 1.0); }
 
 
+// The shipped RRSNet without the AID input has no grid: network.cpp:503-507 builds it with
+// create_network (no encoding) on the 16 channels of NRRSNetworkInput1 (train.h:39-56).
+bool net_has_grid(const nrrs_config* cfg, int net) {
+    return !(net == NRRS_RRSNET && cfg->net_set == NRRS_NETS_SHIPPED && !cfg->rrs_aid);
+}
+
+// Hidden activation: ReLU, or tiny-cuda-nn's LeakyReLU (slope 0.01) for the shipped nets
+// (2net-simple.json:72, 240).
+double act_d(const nrrs_config* cfg, double x) {
+    if (cfg->net_set == NRRS_NETS_SHIPPED) return x > 0.0 ? x : 0.01 * x;
+    return x > 0.0 ? x : 0.0;
+}
+// Its slope, read from the activation's output (which has the sign of its input).
+double act_slope_d(const nrrs_config* cfg, double h) {
+    if (h > 0.0) return 1.0;
+    return cfg->net_set == NRRS_NETS_SHIPPED ? 0.01 : 0.0;
+}
+
+// worldToLatLong (KiRaRay; called at src/render/nrrs/device.cu:102): azimuth about +y measured
+// from -z, u = atan2(x, -z) / 2pi + 1/2, and polar angle from +y, v = acos(y) / pi.
+void lat_long_d(double x, double y, double z, double* u, double* v) {
+    const double r = std::sqrt(x * x + y * y + z * z);
+    x /= r, y /= r, z /= r;
+    *u = std::atan2(x, -z) / (2.0 * M_PI) + 0.5;
+    *v = std::acos(std::min(1.0, std::max(-1.0, y))) / M_PI;
+}
+
+// OneBlob (tiny-cuda-nn "OneBlob", Mueller et al. 2019): bin k of n holds the mass that a
+// biweight kernel 15/16 (1 - t^2)^2, t = (s - x) n (radius one bin), wrapped periodically onto
+// [0, 1), puts on [k/n, (k+1)/n). F is the kernel's CDF in t, clamped outside [-1, 1].
+double biweight_cdf(double t) {
+    t = std::min(1.0, std::max(-1.0, t));
+    return 0.5 + 15.0 / 16.0 * (t - 2.0 * t * t * t / 3.0 + t * t * t * t * t / 5.0);
+}
+void one_blob_d(double x, int n, double* out) {
+    for (int k = 0; k < n; ++k) {
+        double m = 0;
+        for (int w = -1; w <= 1; ++w) {
+            const double c = x + w;  // a periodic copy of the centre
+            m += biweight_cdf(((double)(k + 1) / n - c) * n) - biweight_cdf(((double)k / n - c) * n);
+        }
+        out[k] = m;
+    }
+}
+
 void mlp_dims(const nrrs_config* cfg, int net, int* n_layers, int* n_out) {
     const nrrs_mlp_config& m = net == NRRS_RRSNET ? cfg->rrsnet : cfg->statnet;
     *n_layers = (int)m.n_hidden_layers + 1;
@@ -151,6 +196,30 @@ std::vector<std::vector<char>> connections(const nrrs_config* cfg, int net) {
             for (int r = 0; r < o; ++r)
                 if (r / 3 != (net == NRRS_STATNET ? 0 : 1))
                     for (int k = 0; k < in; ++k) conn[layer][(size_t)r * in + k] = 0;
+        if (cfg->net_set == NRRS_NETS_SHIPPED) {
+            // 32-wide hidden layers; layer 0 reads the channels of the net's shipped input
+            // (the encoding layout of encode_impl)
+            for (int r = 0; r < o; ++r)
+                for (int k = 0; k < in; ++k) {
+                    bool live = (layer == nl - 1 || r < 32);
+                    if (layer > 0) {
+                        live = live && k < 32;
+                    } else {
+                        const int gch = net_has_grid(cfg, net) ? 2 * (int)cfg->grid.n_levels : 0;
+                        bool in_ch;
+                        if (k < gch)
+                            in_ch = true;
+                        else if (net != NRRS_RRSNET)
+                            in_ch = k >= 32 && k < 48;  // OneBlob(u, v) 8 + OneBlob(rough) 8
+                        else if (!cfg->rrs_aid)
+                            in_ch = k >= 64;  // NRRSNetworkInput1, 16 channels
+                        else
+                            in_ch = (k >= 32 && k < 44) || (k >= 65 && k < 69);
+                        live = live && in_ch;
+                    }
+                    if (!live) conn[layer][(size_t)r * in + k] = 0;
+                }
+        }
         in = o;
     }
     return conn;
@@ -211,6 +280,8 @@ void ora_grid_indices(const nrrs_config* cfg, uint32_t n, const nrrs_vertices* v
 }
 
 double ora_activation_sigma(double x) { return sigma_d(x); }
+void ora_lat_long(double x, double y, double z, double* uv) { lat_long_d(x, y, z, uv, uv + 1); }
+void ora_one_blob(double x, int32_t bins, double* out) { one_blob_d(x, bins, out); }
 double ora_box_cox(double x) { return bc_d(x); }
 
 double ora_weight_window(double q, double size) {
@@ -243,7 +314,7 @@ void ora_sh4(double x, double y, double z, double* o) {
 }
 
 uint64_t ora_param_count(const nrrs_config* cfg, int32_t net, uint64_t* n_grid, uint64_t* n_mlp) {
-    uint64_t ng = ora_grid_layout(&cfg->grid, nullptr, nullptr, nullptr, nullptr, nullptr);
+    uint64_t ng = net_has_grid(cfg, net) ? ora_grid_layout(&cfg->grid, nullptr, nullptr, nullptr, nullptr, nullptr) : 0;
     int nl, no;
     mlp_dims(cfg, net, &nl, &no);
     uint64_t nm = (uint64_t)ORA_HIDDEN * ORA_ENC_WIDTH +
@@ -273,7 +344,8 @@ void encode_impl(const nrrs_config* cfg, int32_t net, uint32_t n, const nrrs_ver
         std::fill(e, e + ORA_ENC_WIDTH, 0.0);
         float x[3];
         for (int k = 0; k < 3; ++k) x[k] = norm_coord(v->pos[k * S + i], c[k], d[k]);
-        for (uint32_t l = 0; l < g.n_levels; ++l) {
+        const uint32_t n_levels = net_has_grid(cfg, net) ? g.n_levels : 0;
+        for (uint32_t l = 0; l < n_levels; ++l) {
             uint32_t cell[3];
             double fr[3];
             for (int k = 0; k < 3; ++k) {
@@ -300,6 +372,44 @@ void encode_impl(const nrrs_config* cfg, int32_t net, uint32_t n, const nrrs_ver
         // nrrs_generate_training_data_from_0to1_aid, train.cu:114-153): position, direction, roughness,
         // reference mean and throughput only; the normal, albedo and statistics channels stay zero
         const bool aid = net == NRRS_RRSNET && cfg->rrs_aid != 0;
+        if (cfg->net_set == NRRS_NETS_SHIPPED) {
+            // shipped inputs (2net-simple.json:29-68, 210-291)
+            const double wr = 1.0 - std::exp(-(double)v->rough[i]);  // warp_roughness, train.h:95
+            uint32_t px = v->pixel[i];
+            if (net == NRRS_RRSNET && !aid) {
+                // NRRSNetworkInput1 (train.h:39-56): BC(max(L,0)), BC(max(L2,0)) (with train_sigma
+                // the training input takes activationSigma of it, train.cu:196-200), BC(thp),
+                // roughness, BC(ref mean), then the padding {1, 0, 0, 0, 0}
+                const double ref = ((double)film[px] + (double)film[cfg->n_pixels + px] +
+                                    (double)film[2 * (size_t)cfg->n_pixels + px]) / 3.0;
+                for (int k = 0; k < 3; ++k) {
+                    const double L = stats ? std::max(stats[(size_t)k * n + i], 0.0) : 0.0;
+                    const double l2 = stats ? std::max(stats[(size_t)(3 + k) * n + i], 0.0) : 0.0;
+                    e[69 + k] = bc_d(L);
+                    e[72 + k] = bc_d(cfg->train_sigma && training ? sigma_d(l2) : l2);
+                    e[65 + k] = bc_d((double)v->thp[k * S + i]);
+                }
+                e[64] = wr;
+                e[68] = bc_d(ref);
+                e[75] = 1.0;
+                continue;
+            }
+            // NRRSNetworkInput0 (train.h:22-29) / NRRSNetworkInput1AID (train.h:58-68): the
+            // lat-long direction through OneBlob (4 bins per coordinate), the roughness through
+            // OneBlob (StatNet 8 bins, AID 4), the AID input's Box-Cox reference mean and thp
+            double uu, vv;
+            lat_long_d(v->dir[i], v->dir[S + i], v->dir[2 * S + i], &uu, &vv);
+            one_blob_d(uu, 4, e + 32);
+            one_blob_d(vv, 4, e + 36);
+            one_blob_d(wr, aid ? 4 : 8, e + 40);
+            if (aid) {
+                const double ref = ((double)film[px] + (double)film[cfg->n_pixels + px] +
+                                    (double)film[2 * (size_t)cfg->n_pixels + px]) / 3.0;
+                for (int k = 0; k < 3; ++k) e[65 + k] = bc_d((double)v->thp[k * S + i]);
+                e[68] = bc_d(ref);
+            }
+            continue;
+        }
         ora_sh4(v->dir[i], v->dir[S + i], v->dir[2 * S + i], e + 32);
         if (!aid) ora_sh4(v->normal[i], v->normal[S + i], v->normal[2 * S + i], e + 48);
         // warp_roughness (src/render/nrrs/train.h:95)
@@ -370,7 +480,7 @@ void ora_net_forward(const nrrs_config* cfg, int32_t net, uint32_t n, const nrrs
                 double acc = 0;
                 for (int k = 0; k < in; ++k)
                     if (conn[layer][(size_t)r * in + k]) acc += w[(size_t)r * in + k] * h[k];
-                t[r] = layer == nl - 1 ? acc : std::max(acc, 0.0);
+                t[r] = layer == nl - 1 ? acc : act_d(cfg, acc);
             }
             w += (size_t)o * in;
             h.assign(t.begin(), t.begin() + o);
@@ -686,7 +796,7 @@ double ora_train_grads(const nrrs_config* cfg, int32_t net, uint32_t n, const nr
                     for (int k = 0; k < in; ++k)
                         if (conn[layer][(size_t)r * in + k])
                             acc += w[(size_t)r * in + k] * act[layer][(size_t)i * in + k];
-                    out[(size_t)i * o + r] = layer == nl - 1 ? acc : std::max(acc, 0.0);
+                    out[(size_t)i * o + r] = layer == nl - 1 ? acc : act_d(cfg, acc);
                 }
             if (layer < nl - 1) {
                 act[layer + 1] = out;
@@ -750,7 +860,13 @@ double ora_train_grads(const nrrs_config* cfg, int32_t net, uint32_t n, const nr
             }
             if (layer > 0)
                 for (int k = 0; k < in; ++k)
-                    if (!(a[(size_t)i * in + k] > 0.0)) prev[(size_t)i * in + k] = 0.0;
+                {
+                    const double sl = act_slope_d(cfg, a[(size_t)i * in + k]);
+                    if (sl == 0.0)
+                        prev[(size_t)i * in + k] = 0.0;
+                    else
+                        prev[(size_t)i * in + k] *= sl;
+                }
         }
         delta.swap(prev);
         o = in;
@@ -764,7 +880,7 @@ double ora_train_grads(const nrrs_config* cfg, int32_t net, uint32_t n, const nr
         for (uint32_t i = 0; i < n; ++i) {
             float x[3];
             for (int k = 0; k < 3; ++k) x[k] = norm_coord(v->pos[k * v->stride + i], c[k], d[k]);
-            for (uint32_t l = 0; l < g.n_levels; ++l) {
+            for (uint32_t l = 0; l < (net_has_grid(cfg, net) ? g.n_levels : 0u); ++l) {
                 uint32_t cell[3];
                 double fr[3];
                 for (int k = 0; k < 3; ++k) {

@@ -66,6 +66,9 @@ double ora_activation_sigma(double x);
 double ora_box_cox(double x);
 double ora_weight_window(double q, double size);
 void ora_sh4(double x, double y, double z, double* out16);
+/* Shipped-net input maps: worldToLatLong of a direction (uv[2]) and the OneBlob bins of x. */
+void ora_lat_long(double x, double y, double z, double* uv);
+void ora_one_blob(double x, int32_t bins, double* out);
 
 /* enc: [n][80]; params: [grid | mlp] of that net (FP64); stats: [6][n] or NULL. ora_encode
  * is the inference input, ora_encode_train the input of a training step (they differ in the

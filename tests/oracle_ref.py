@@ -26,6 +26,8 @@ _SIGS = {
     "ora_box_cox": (C.c_double, [C.c_double]),
     "ora_weight_window": (C.c_double, [C.c_double, C.c_double]),
     "ora_sh4": (None, [C.c_double, C.c_double, C.c_double, _P]),
+    "ora_lat_long": (None, [C.c_double, C.c_double, C.c_double, _P]),
+    "ora_one_blob": (None, [C.c_double, C.c_int32, _P]),
     "ora_encode": (None, [C.POINTER(na.Config), C.c_int32, C.c_uint32, C.POINTER(na.Vertices), _P, _P, _P, _P]),
     "ora_encode_train": (None, [C.POINTER(na.Config), C.c_int32, C.c_uint32, C.POINTER(na.Vertices), _P, _P, _P, _P]),
     "ora_net_forward": (None, [C.POINTER(na.Config), C.c_int32, C.c_uint32, C.POINTER(na.Vertices), _P, _P, _P, _P]),
@@ -104,6 +106,18 @@ def synth_vertices(n, seed, n_pixels):
     load().ora_synth_vertices(n, seed, n_pixels, C.byref(o))
     d["stid"] = None
     return d
+
+
+def lat_long(d):
+    uv = np.zeros(2, np.float64)
+    load().ora_lat_long(float(d[0]), float(d[1]), float(d[2]), uv.ctypes.data)
+    return uv
+
+
+def one_blob(x, bins):
+    out = np.zeros(bins, np.float64)
+    load().ora_one_blob(float(x), bins, out.ctypes.data)
+    return out
 
 
 def param_count(cfg, net):
