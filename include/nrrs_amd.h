@@ -242,8 +242,14 @@ nrrs_status nrrs_grid_layout(const nrrs_grid_config* g, uint32_t* res, uint32_t*
 nrrs_status nrrs_param_count(nrrs_ctx* ctx, int32_t net, uint64_t* n_grid, uint64_t* n_mlp);
 /* Host copy of [grid | W0 | W1 | ... ] (row-major W_l[out][in]). Synchronous. */
 nrrs_status nrrs_params_get(nrrs_ctx* ctx, int32_t net, float* host_dst);
-/* Upload, refresh low-precision shadows. Synchronous. */
+/* Upload, refresh low-precision shadows; with Ema (ema_decay > 0) the moving average restarts
+ * at these parameters. Synchronous. */
 nrrs_status nrrs_params_set(nrrs_ctx* ctx, int32_t net, const float* host_src);
+/* Host copy of the parameters the inference entry points read, in the layout of
+ * nrrs_params_get: the Ema moving average when ema_decay > 0, else the trained parameters
+ * (tiny-cuda-nn's Trainer infers with its optimizer's averaged weights; 2net-simple.json:16-18).
+ * Synchronous. */
+nrrs_status nrrs_params_get_inference(nrrs_ctx* ctx, int32_t net, float* host_dst);
 
 /* ---------------- inference (network.cpp:297-349) ---------------- */
 

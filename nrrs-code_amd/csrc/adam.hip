@@ -41,12 +41,22 @@ __global__ void k_check_finite(const float* __restrict__ g, uint64_t n, const ui
 
 struct AdamStep {
     float inv_s, c1, c2;
+    float e_keep, e_take;  // Ema weights of the old average and of the new parameters
 };
 
 __device__ __forceinline__ AdamStep adam_step(const AdamArgs& a) {
     const uint32_t t = a.state[0] + 1;
-    return AdamStep{1.0f / *a.loss_scale, 1.0f - powf(a.beta1, (float)t), 1.0f - powf(a.beta2, (float)t)};
+    AdamStep k{1.0f / *a.loss_scale, 1.0f - powf(a.beta1, (float)t), 1.0f - powf(a.beta2, (float)t), 0.0f, 1.0f};
+    if (a.ema) {
+        // debiased moving average (the bias correction of Adam's moments applied to the weights)
+        const float d_old = 1.0f - powf(a.ema_decay, (float)(t - 1)), d_new = 1.0f - powf(a.ema_decay, (float)t);
+        k.e_keep = a.ema_decay * d_old / d_new;
+        k.e_take = (1.0f - a.ema_decay) / d_new;
+    }
+    return k;
 }
+
+__device__ __forceinline__ float ema_one(const AdamStep& k, float e, float p) { return k.e_keep * e + k.e_take * p; }
 
 // One parameter: gradient g (loss-scaled), master p and moments m, v updated in place.
 __device__ __forceinline__ void adam_one(const AdamArgs& a, const AdamStep& k, float g, float& p, float& m, float& v) {
@@ -77,6 +87,11 @@ __device__ __forceinline__ void adam_scalar(const AdamArgs& a, const AdamStep& k
     a.v[i] = v;
     a.params[i] = p;
     if (a.grid_lp && i < a.n_grid) a.grid_lp[i] = lp_bits(a, p);
+    if (a.ema) {
+        const float e = ema_one(k, a.ema[i], p);
+        a.ema[i] = e;
+        if (a.ema_lp && i < a.n_grid) a.ema_lp[i] = lp_bits(a, e);
+    }
 }
 
 // Any n_grid: one parameter per thread and iteration. The 16-level grids with base resolutions
@@ -124,6 +139,14 @@ __global__ void k_adam(AdamArgs a) {
         if (a.grid_lp && i < a.n_grid)
             reinterpret_cast<uint2*>(a.grid_lp)[c] = make_uint2(lp_bits(a, p.x) | (uint32_t)lp_bits(a, p.y) << 16,
                                                                 lp_bits(a, p.z) | (uint32_t)lp_bits(a, p.w) << 16);
+        if (a.ema) {
+            float4 e = reinterpret_cast<const float4*>(a.ema)[c];
+            e = make_float4(ema_one(k, e.x, p.x), ema_one(k, e.y, p.y), ema_one(k, e.z, p.z), ema_one(k, e.w, p.w));
+            reinterpret_cast<float4*>(a.ema)[c] = e;
+            if (a.ema_lp && i < a.n_grid)
+                reinterpret_cast<uint2*>(a.ema_lp)[c] = make_uint2(lp_bits(a, e.x) | (uint32_t)lp_bits(a, e.y) << 16,
+                                                                   lp_bits(a, e.z) | (uint32_t)lp_bits(a, e.w) << 16);
+        }
     }
     for (uint64_t i = 4 * n4 + (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < a.n; i += stride)
         adam_scalar(a, k, skip, i);

@@ -292,61 +292,87 @@ bool vertices_ok(const nrrs_vertices* v, uint32_t n) {
 nrrs_status refresh_pair(nrrs_ctx* c, hipStream_t s) {
     if (!c->grid_pair || !c->net[NRRS_STATNET].grid_lp || !c->net[NRRS_RRSNET].grid_lp) return NRRS_OK;
     OpScope t(c, "pack_pair", s);
-    return hip_status(launch_pack_pair(c->net[NRRS_STATNET].grid_lp, c->net[NRRS_RRSNET].grid_lp, c->grid_pair,
-                                       c->net[NRRS_STATNET].n_grid / 2, s));
+    NRRS_HIP(launch_pack_pair(c->net[NRRS_STATNET].grid_lp, c->net[NRRS_RRSNET].grid_lp, c->grid_pair,
+                              c->net[NRRS_STATNET].n_grid / 2, s));
+    if (!c->grid_pair_ema) return NRRS_OK;
+    return hip_status(launch_pack_pair(c->net[NRRS_STATNET].grid_lp_ema, c->net[NRRS_RRSNET].grid_lp_ema,
+                                       c->grid_pair_ema, c->net[NRRS_STATNET].n_grid / 2, s));
 }
 
-// Low-precision shadows (grid table and packed MFMA tiles) from the FP32 masters.
+// Low-precision shadows (grid table and packed MFMA tiles) from the FP32 masters, and from their
+// moving average when the context keeps one.
 nrrs_status refresh_lp(nrrs_ctx* c, int net, hipStream_t s) {
     if (c->cfg.precision == NRRS_FP32) return NRRS_OK;
     NetState& ns = c->net[net];
     NRRS_HIP(launch_pack_lp(c->cfg.precision, ns.grid_f32, ns.n_grid, ns.mlp_f32, ns.n_layers, ns.n_out,
                             ns.grid_lp, ns.mlp_lp, s));
+    if (ns.ema)
+        NRRS_HIP(launch_pack_lp(c->cfg.precision, ns.ema, ns.n_grid, ns.ema + ns.n_grid, ns.n_layers, ns.n_out,
+                                ns.grid_lp_ema, ns.mlp_lp_ema, s));
     return refresh_pair(c, s);
+}
+
+// The parameters one net reads: for inference, the moving average of the Ema optimizer when the
+// context keeps one (tiny-cuda-nn's Trainer infers with the optimizer's averaged weights,
+// 2net-simple.json:16-18), else the masters; for training (`train_side`), the masters.
+struct NetView {
+    const float* grid_f32;
+    const float* mlp_f32;
+    const uint16_t* grid_lp;
+    const uint16_t* mlp_lp;
+    const uint32_t* pair;  // the paired table of the same parameters, or null
+};
+NetView view_of(const nrrs_ctx* c, int net, bool train_side = false) {
+    const NetState& ns = c->net[net];
+    if (ns.ema && !train_side)
+        return NetView{ns.ema, ns.ema + ns.n_grid, ns.grid_lp_ema, ns.mlp_lp_ema, c->grid_pair_ema};
+    return NetView{ns.grid_f32, ns.mlp_f32, ns.grid_lp, ns.mlp_lp, c->grid_pair};
 }
 
 // One fused encode + MLP launch of `net` in the context's precision.
 hipError_t run_net(nrrs_ctx* c, int net, const EncodeArgs& a_in, int epi, uint32_t n, const EpiArgs& ep,
-                   hipStream_t s) {
+                   hipStream_t s, bool train_side = false) {
     const NetState& ns = c->net[net];
+    const NetView w = view_of(c, net, train_side);
     EncodeArgs a = a_in;
     a.grid = ns.grid;  // the net's own grid (none for the shipped RRSNet)
     OpScope t(c, op_label(net, "infer_statnet", "infer_rrsnet", "infer_statnet2"), s);
     const int k = kind_of(net);
     if (c->cfg.precision == NRRS_FP32) {
         if (!c->grid_enc_f32 || n > c->grid_enc_cap)
-            return launch_net_fp32(a, k, ns.n_layers, ns.n_out, epi, n, ns.grid_f32, ns.mlp_f32, ep, s);
+            return launch_net_fp32(a, k, ns.n_layers, ns.n_out, epi, n, w.grid_f32, w.mlp_f32, ep, s);
         // level-major grid gathers first, then the network kernel reads the encoding
         EncodeArgs ae = a;
         ae.grid_enc_f32 = c->grid_enc_f32;
         ae.grid_enc_stride = c->grid_enc_cap;
-        hipError_t e = launch_encode_levels_f32(a, ns.grid_f32, n, c->grid_enc_f32, c->grid_enc_cap, s);
+        hipError_t e = launch_encode_levels_f32(a, w.grid_f32, n, c->grid_enc_f32, c->grid_enc_cap, s);
         if (e != hipSuccess) return e;
-        return launch_net_fp32(ae, k, ns.n_layers, ns.n_out, epi, n, ns.grid_f32, ns.mlp_f32, ep, s);
+        return launch_net_fp32(ae, k, ns.n_layers, ns.n_out, epi, n, w.grid_f32, w.mlp_f32, ep, s);
     }
     if (!c->grid_enc || n > c->grid_enc_cap)
-        return launch_net_mfma(c->cfg.precision, a, k, ns.n_layers, ns.n_out, epi, n, ns.grid_lp, ns.mlp_lp, ep, s);
+        return launch_net_mfma(c->cfg.precision, a, k, ns.n_layers, ns.n_out, epi, n, w.grid_lp, w.mlp_lp, ep, s);
     // level-major grid gathers first, then the MLP kernel reads the encoding
     EncodeArgs ae = a;
     ae.grid_enc = c->grid_enc;
     ae.grid_enc_stride = c->grid_enc_cap;
-    hipError_t e = launch_encode_levels(c->cfg.precision, ae, ns.grid_lp, n, c->grid_enc, c->grid_enc_cap, s);
+    hipError_t e = launch_encode_levels(c->cfg.precision, ae, w.grid_lp, n, c->grid_enc, c->grid_enc_cap, s);
     if (e != hipSuccess) return e;
-    return launch_net_mfma(c->cfg.precision, ae, k, ns.n_layers, ns.n_out, epi, n, ns.grid_lp, ns.mlp_lp, ep, s);
+    return launch_net_mfma(c->cfg.precision, ae, k, ns.n_layers, ns.n_out, epi, n, w.grid_lp, w.mlp_lp, ep, s);
 }
 
 // StatNet statistics [6][stride] of a batch; AllSelf runs both its nets, each writing its head's
 // outputs.
-hipError_t run_statnets(nrrs_ctx* c, const EncodeArgs& a, float* out, uint32_t stride, uint32_t n, hipStream_t s) {
+hipError_t run_statnets(nrrs_ctx* c, const EncodeArgs& a, float* out, uint32_t stride, uint32_t n, hipStream_t s,
+                        bool train_side = false) {
     EpiArgs ep{};
     ep.out = out;
     ep.out_stride = stride;
-    if (!all_self(c->cfg)) return run_net(c, NRRS_STATNET, a, EPI_RAW, n, ep, s);
+    if (!all_self(c->cfg)) return run_net(c, NRRS_STATNET, a, EPI_RAW, n, ep, s, train_side);
     ep.head = 0;
-    const hipError_t e = run_net(c, NRRS_STATNET, a, EPI_RAW, n, ep, s);
+    const hipError_t e = run_net(c, NRRS_STATNET, a, EPI_RAW, n, ep, s, train_side);
     if (e != hipSuccess) return e;
     ep.head = 1;
-    return run_net(c, NRRS_STATNET2, a, EPI_RAW, n, ep, s);
+    return run_net(c, NRRS_STATNET2, a, EPI_RAW, n, ep, s, train_side);
 }
 
 // Whether a batch of n vertices takes the paired encode pass (both nets in one gather).
@@ -710,17 +736,24 @@ nrrs_status run_adam(nrrs_ctx* c, int net, hipStream_t s) {
     a.dynamic_scale = cfg.precision != NRRS_FP32;
     a.growth_interval = cfg.loss_scale_growth_interval ? cfg.loss_scale_growth_interval : 1000;
     a.scan_on_flag = ns.binned;  // the apply kernels report non-finite table-gradient entries
+    a.ema = ns.ema;               // Ema: the moving average and its grid shadow follow every taken step
+    a.ema_lp = ns.grid_lp_ema;
+    a.ema_decay = cfg.ema_decay;
+    const float* W_ema = ns.ema ? ns.ema + ns.n_grid : nullptr;
     if (cfg.precision != NRRS_FP32 && ns.train_ready) {
-        // the update, then the state and both MLP tile layouts in one launch (the grid shadow is
+        // the update, then the state and the MLP tile layouts in one launch (the grid shadows are
         // refreshed inside the Adam kernel)
         NRRS_HIP(launch_adam(a, s, false));
-        return hip_status(launch_after_adam(cfg.precision, a, ns.mlp_f32, ns.n_layers, ns.n_out, ns.mlp_lp, ns.w_fwd,
-                                            ns.w_bwd, s));
+        return hip_status(launch_after_adam(cfg.precision, a, ns.mlp_f32, W_ema, ns.n_layers, ns.n_out, ns.mlp_lp,
+                                            ns.mlp_lp_ema, ns.w_fwd, ns.w_bwd, s));
     }
     NRRS_HIP(launch_adam(a, s));
-    // MLP shadows (the grid shadow is refreshed inside the Adam kernel)
-    if (cfg.precision != NRRS_FP32)
+    // MLP shadows (the grid shadows are refreshed inside the Adam kernel)
+    if (cfg.precision != NRRS_FP32) {
         NRRS_HIP(launch_pack_lp(cfg.precision, nullptr, 0, ns.mlp_f32, ns.n_layers, ns.n_out, nullptr, ns.mlp_lp, s));
+        if (W_ema)
+            NRRS_HIP(launch_pack_lp(cfg.precision, nullptr, 0, W_ema, ns.n_layers, ns.n_out, nullptr, ns.mlp_lp_ema, s));
+    }
     return refresh_train(c, net, s);
 }
 
@@ -751,11 +784,12 @@ nrrs_status frame_params(nrrs_ctx* c, const nrrs_frame_args* a, FrameParams* f) 
     f->grid = c->grid;
     f->frame = c->frame;
     f->d = c->dec;
-    f->table_stat = reinterpret_cast<const uint32_t*>(c->net[NRRS_STATNET].grid_lp);
-    f->table_rrs = reinterpret_cast<const uint32_t*>(c->net[NRRS_RRSNET].grid_lp);
-    f->table_pair = reinterpret_cast<const uint2*>(c->grid_pair);
-    f->w_stat = reinterpret_cast<const u32x4_t*>(c->net[NRRS_STATNET].mlp_lp);
-    f->w_rrs = reinterpret_cast<const u32x4_t*>(c->net[NRRS_RRSNET].mlp_lp);
+    const NetView ws = view_of(c, NRRS_STATNET), wr = view_of(c, NRRS_RRSNET);
+    f->table_stat = reinterpret_cast<const uint32_t*>(ws.grid_lp);
+    f->table_rrs = reinterpret_cast<const uint32_t*>(wr.grid_lp);
+    f->table_pair = reinterpret_cast<const uint2*>(ws.pair);
+    f->w_stat = reinterpret_cast<const u32x4_t*>(ws.mlp_lp);
+    f->w_rrs = reinterpret_cast<const u32x4_t*>(wr.mlp_lp);
     f->film = a->film_dev;
     f->n_pixels = c->cfg.n_pixels;
     f->n_primary = a->n_primary;
@@ -796,6 +830,9 @@ void free_ctx(nrrs_ctx* c) {
         hipFree(ns.params);
         hipFree(ns.grid_lp);
         hipFree(ns.mlp_lp);
+        hipFree(ns.ema);
+        hipFree(ns.grid_lp_ema);
+        hipFree(ns.mlp_lp_ema);
         hipFree(ns.grad);
         hipFree(ns.grad_lp);
         hipFree(ns.dense_rep);
@@ -840,6 +877,7 @@ void free_ctx(nrrs_ctx* c) {
     hipFree(c->grid_enc2);
     hipFree(c->grid_enc_f32);
     hipFree(c->grid_pair);
+    hipFree(c->grid_pair_ema);
     hipFree(c->q_scratch);
     hipFree(c->s_scratch);
     delete c;
@@ -976,7 +1014,6 @@ nrrs_status nrrs_create(const nrrs_config* cfg, nrrs_ctx** out) {
     *out = nullptr;
     if (cfg->net_set != NRRS_NETS_BENCH && cfg->net_set != NRRS_NETS_SHIPPED) return NRRS_ERR_ARG;
     if (!(cfg->ema_decay >= 0.0f && cfg->ema_decay < 1.0f)) return NRRS_ERR_ARG;
-    if (cfg->ema_decay != 0.0f) return NRRS_ERR_UNSUPPORTED;
     if (!valid_shapes(*cfg)) return NRRS_ERR_UNSUPPORTED;
     if (cfg->train_sigma != 0 && cfg->train_sigma != 1) return NRRS_ERR_ARG;
     if (cfg->rrs_aid != 0 && cfg->rrs_aid != 1) return NRRS_ERR_ARG;
@@ -1015,8 +1052,10 @@ nrrs_status nrrs_create(const nrrs_config* cfg, nrrs_ctx** out) {
 
     nrrs_status st = NRRS_OK;
     // paired table: one 32-bit word per entry and net, n_grid / 2 entries (both nets with a grid)
-    if (cfg->precision != NRRS_FP32 && cfg->paired_tables && has_grid(*cfg, NRRS_RRSNET))
+    if (cfg->precision != NRRS_FP32 && cfg->paired_tables && has_grid(*cfg, NRRS_RRSNET)) {
         st = dmalloc(&c->grid_pair, n_grid);
+        if (st == NRRS_OK && cfg->ema_decay > 0.0f) st = dmalloc(&c->grid_pair_ema, n_grid);
+    }
     std::vector<float> host;
     for (int net = 0; net < n_nets(*cfg) && st == NRRS_OK; ++net) {
         const nrrs_mlp_config& m = mlp_of(*cfg, net);
@@ -1039,10 +1078,19 @@ nrrs_status nrrs_create(const nrrs_config* cfg, nrrs_ctx** out) {
             if ((st = dmalloc(&ns.grid_lp, ns.n_grid)) != NRRS_OK) break;
             if ((st = dmalloc(&ns.mlp_lp, mfma_packed_halves(ns.n_layers))) != NRRS_OK) break;
         }
+        if (cfg->ema_decay > 0.0f) {  // the moving average starts at the initial parameters
+            if ((st = dmalloc(&ns.ema, ns.n_grid + ns.n_mlp)) != NRRS_OK) break;
+            if (cfg->precision != NRRS_FP32) {
+                if ((st = dmalloc(&ns.grid_lp_ema, ns.n_grid)) != NRRS_OK) break;
+                if ((st = dmalloc(&ns.mlp_lp_ema, mfma_packed_halves(ns.n_layers))) != NRRS_OK) break;
+            }
+        }
         init_params(*cfg, net, ns.n_grid, host);
         st = hip_status(hipMemcpy(ns.grid_f32, host.data(), ns.n_grid * 4, hipMemcpyHostToDevice));
         if (st == NRRS_OK)
             st = hip_status(hipMemcpy(ns.mlp_f32, host.data() + ns.n_grid, ns.n_mlp * 4, hipMemcpyHostToDevice));
+        if (st == NRRS_OK && ns.ema)
+            st = hip_status(hipMemcpy(ns.ema, host.data(), (ns.n_grid + ns.n_mlp) * 4, hipMemcpyHostToDevice));
         if (st == NRRS_OK) st = refresh_lp(c, net, nullptr);
         if (st == NRRS_OK) st = hip_status(hipDeviceSynchronize());
     }
@@ -1122,9 +1170,21 @@ nrrs_status nrrs_params_set(nrrs_ctx* ctx, int32_t net, const float* host_src) {
     } else {
         NRRS_HIP(hipMemcpy(ns.mlp_f32, host_src + ns.n_grid, ns.n_mlp * 4, hipMemcpyHostToDevice));
     }
+    // setting the parameters restarts the moving average from them
+    if (ns.ema)
+        NRRS_HIP(hipMemcpy(ns.ema, ns.params, (ns.n_grid + ns.n_mlp) * 4, hipMemcpyDeviceToDevice));
     NRRS_TRY(refresh_lp(ctx, net, nullptr));
     NRRS_TRY(refresh_train(ctx, net, nullptr));
     return hip_status(hipDeviceSynchronize());
+}
+
+nrrs_status nrrs_params_get_inference(nrrs_ctx* ctx, int32_t net, float* host_dst) {
+    if (!ctx || !host_dst || !net_ok(ctx, net)) return NRRS_ERR_ARG;
+    const NetState& ns = ctx->net[net];
+    NRRS_HIP(hipDeviceSynchronize());
+    const float* src = ns.ema ? ns.ema : ns.params;
+    NRRS_HIP(hipMemcpy(host_dst, src, (ns.n_grid + ns.n_mlp) * 4, hipMemcpyDeviceToHost));
+    return NRRS_OK;
 }
 
 nrrs_status nrrs_encode(nrrs_ctx* ctx, int32_t net, uint32_t n, const nrrs_vertices* v,
@@ -1133,7 +1193,7 @@ nrrs_status nrrs_encode(nrrs_ctx* ctx, int32_t net, uint32_t n, const nrrs_verti
     if (net == NRRS_RRSNET && !film_dev) return NRRS_ERR_ARG;
     EncodeArgs a = encode_args(ctx, *v, film_dev, net == NRRS_RRSNET ? stats_dev : nullptr, n);
     a.grid = ctx->net[net].grid;
-    return hip_status(launch_encode(a, kind_of(net), n, ctx->net[net].grid_f32, enc_dev, (hipStream_t)stream));
+    return hip_status(launch_encode(a, kind_of(net), n, view_of(ctx, net).grid_f32, enc_dev, (hipStream_t)stream));
 }
 
 nrrs_status nrrs_grid_indices(nrrs_ctx* ctx, uint32_t n, const nrrs_vertices* v, uint32_t* idx_dev,
@@ -1197,20 +1257,18 @@ nrrs_status nrrs_rrs_infer_decide(nrrs_ctx* ctx, const nrrs_decide_args* a, void
         EncodeArgs ea = encode_args(ctx, a->v, a->film_dev, nullptr, 0);
         ea.grid_enc = ctx->grid_enc;
         ea.grid_enc_stride = ctx->grid_enc_cap;
+        const NetView ws = view_of(ctx, NRRS_STATNET), wr = view_of(ctx, NRRS_RRSNET);
         if (paired_pass(ctx, a->n)) {
             OpScope t(ctx, "encode_pair", s);
-            NRRS_HIP(launch_encode_levels_dual(cfg.precision, ea, ctx->grid_pair, a->n, ctx->grid_enc, ctx->grid_enc2,
+            NRRS_HIP(launch_encode_levels_dual(cfg.precision, ea, ws.pair, a->n, ctx->grid_enc, ctx->grid_enc2,
                                                ctx->grid_enc_cap, s));
         } else {
             OpScope t(ctx, "encode_two", s);
-            NRRS_HIP(launch_encode_levels(cfg.precision, ea, ctx->net[NRRS_STATNET].grid_lp, a->n, ctx->grid_enc,
-                                          ctx->grid_enc_cap, s));
-            NRRS_HIP(launch_encode_levels(cfg.precision, ea, ctx->net[NRRS_RRSNET].grid_lp, a->n, ctx->grid_enc2,
-                                          ctx->grid_enc_cap, s));
+            NRRS_HIP(launch_encode_levels(cfg.precision, ea, ws.grid_lp, a->n, ctx->grid_enc, ctx->grid_enc_cap, s));
+            NRRS_HIP(launch_encode_levels(cfg.precision, ea, wr.grid_lp, a->n, ctx->grid_enc2, ctx->grid_enc_cap, s));
         }
         OpScope t(ctx, "mlp_chained", s);
-        NRRS_HIP(launch_net_chained(cfg.precision, ea, ctx->grid_enc2, a->n, ctx->net[NRRS_STATNET].mlp_lp,
-                                    ctx->net[NRRS_RRSNET].mlp_lp, ep, s));
+        NRRS_HIP(launch_net_chained(cfg.precision, ea, ctx->grid_enc2, a->n, ws.mlp_lp, wr.mlp_lp, ep, s));
     } else if (a->q_mode == NRRS_Q_RRSNET || a->q_mode == NRRS_Q_RRSNET_CHAINED) {
         const float* stats = nullptr;
         if (a->q_mode == NRRS_Q_RRSNET_CHAINED) {  // FP32, two-headed StatNet, or a batch beyond the encoding buffers
@@ -1467,9 +1525,9 @@ nrrs_status nrrs_train_step(nrrs_ctx* ctx, uint32_t net_mask, uint32_t n, const 
         }
     } join{ctx, s, forked};
     if (net_mask & 2u) {
-        if (!(net_mask & 1u)) {
+        if (!(net_mask & 1u)) {  // the statistics of the StatNet being trained, as its training forward gives
             EncodeArgs ea = encode_args(ctx, *v, nullptr, nullptr, 0);
-            NRRS_HIP(run_statnets(ctx, ea, ctx->train_stats, ms, n, s));
+            NRRS_HIP(run_statnets(ctx, ea, ctx->train_stats, ms, n, s, true));
         }
         NetState& ns = ctx->net[NRRS_RRSNET];
         NRRS_TRY(run_backward(ctx, NRRS_RRSNET, n, *v, film_dev, ctx->train_stats, ms, NRRS_LOSS_RRS, nullptr,

@@ -125,6 +125,11 @@ struct NetState {
     float* mlp_f32 = nullptr;        // = params + n_grid: MLP weights, row-major W_l[out][in]
     uint16_t* grid_lp = nullptr;     // FP16/BF16 shadow of the grid
     uint16_t* mlp_lp = nullptr;      // FP16/BF16 shadow of the MLP (MFMA tile order)
+    // Ema optimizer (nrrs_config.ema_decay > 0): debiased moving average of the masters [grid | mlp],
+    // the parameters every inference entry point reads, and its FP16/BF16 shadows
+    float* ema = nullptr;
+    uint16_t* grid_lp_ema = nullptr;
+    uint16_t* mlp_lp_ema = nullptr;
     // training state (allocated by nrrs_create when max_train > 0)
     bool train_ready = false;
     float* grad = nullptr;           // [grid | mlp]
@@ -180,6 +185,7 @@ struct nrrs_ctx {
     uint32_t* grid_enc2 = nullptr;          // RRSNet's encoding of a paired pass, same shape
     uint32_t grid_enc_cap = 0;
     uint32_t* grid_pair = nullptr;          // paired table: [entries][2 nets] FP16/BF16 words, or null
+    uint32_t* grid_pair_ema = nullptr;      // the same of the moving averages (Ema contexts), or null
     float2* grid_enc_f32 = nullptr;         // FP32 level-major encoding [levels][grid_enc_cap], or null
     nrrs::NodeBuf nodes;                    // path nodes of the training paths
     void* nodes_mem = nullptr;

@@ -377,14 +377,16 @@ hipError_t launch_pack_lp(int precision, const float* grid, uint64_t n_grid, con
                           int n_out, uint16_t* grid_lp, uint16_t* W_lp, hipStream_t s) {
     if (precision == NRRS_BF16) {
         const int split = split_of<BF16>(n_out);
-        hipLaunchKernelGGL(k_pack_mlp<BF16>, dim3(tiles_of(n_layers, split)), dim3(64), 0, s, W, n_layers, n_out, split,
-                           W_lp);
+        if (W)
+            hipLaunchKernelGGL(k_pack_mlp<BF16>, dim3(tiles_of(n_layers, split)), dim3(64), 0, s, W, n_layers, n_out,
+                               split, W_lp);
         if (grid)
             hipLaunchKernelGGL(k_to_lp<BF16>, dim3(2048), dim3(256), 0, s, grid, grid_lp, n_grid);
     } else if (precision == NRRS_FP16) {
         const int split = split_of<F16>(n_out);
-        hipLaunchKernelGGL(k_pack_mlp<F16>, dim3(tiles_of(n_layers, split)), dim3(64), 0, s, W, n_layers, n_out, split,
-                           W_lp);
+        if (W)
+            hipLaunchKernelGGL(k_pack_mlp<F16>, dim3(tiles_of(n_layers, split)), dim3(64), 0, s, W, n_layers, n_out,
+                               split, W_lp);
         if (grid)
             hipLaunchKernelGGL(k_to_lp<F16>, dim3(2048), dim3(256), 0, s, grid, grid_lp, n_grid);
     } else {

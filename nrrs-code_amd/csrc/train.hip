@@ -782,20 +782,31 @@ __global__ void k_pack_train(const float* __restrict__ W, int nl, int n_out, E* 
 
 // After an optimizer step of an FP16/BF16 net, one wave per workgroup: workgroup 0 updates the
 // step and loss-scale state, the next n_inf the inference tiles of the MLP (P's layout, with
-// the split-precision residual tiles where the net has them), the rest its training tiles (E).
+// the split-precision residual tiles where the net has them), then, with a moving average
+// W_ema, n_inf more of its inference tiles, the rest the training tiles (E) of the masters W.
 // Three dependent small launches become one.
 template <typename P, typename E>
-__global__ void k_after_adam(AdamArgs a, const float* __restrict__ W, int nl, int n_out, int split,
-                             uint16_t* __restrict__ inf, int n_inf, E* __restrict__ fwd, E* __restrict__ bwd, int n_fwd,
-                             int n_bwd) {
-    const int b = (int)blockIdx.x - 1;
+__global__ void k_after_adam(AdamArgs a, const float* __restrict__ W, const float* __restrict__ W_ema, int nl,
+                             int n_out, int split, uint16_t* __restrict__ inf, uint16_t* __restrict__ inf_ema,
+                             int n_inf, E* __restrict__ fwd, E* __restrict__ bwd, int n_fwd, int n_bwd) {
+    int b = (int)blockIdx.x - 1;
     if (b < 0) {
         if (threadIdx.x == 0) adam_finish_state(a);
-    } else if (b < n_inf) {
-        mfma::pack_mlp_tile<P>(W, nl, n_out, split, inf, b, threadIdx.x);
-    } else {
-        pack_train_tile<E>(W, nl, n_out, fwd, bwd, n_fwd, n_bwd, b - n_inf, threadIdx.x);
+        return;
     }
+    if (b < n_inf) {
+        mfma::pack_mlp_tile<P>(W, nl, n_out, split, inf, b, threadIdx.x);
+        return;
+    }
+    b -= n_inf;
+    if (W_ema) {
+        if (b < n_inf) {
+            mfma::pack_mlp_tile<P>(W_ema, nl, n_out, split, inf_ema, b, threadIdx.x);
+            return;
+        }
+        b -= n_inf;
+    }
+    pack_train_tile<E>(W, nl, n_out, fwd, bwd, n_fwd, n_bwd, b, threadIdx.x);
 }
 
 template <typename P, int NL, int NOUT, int LOSS, bool BIN, int ACT>
@@ -1358,17 +1369,19 @@ hipError_t launch_pack_train(int precision, const float* W, int nl, int n_out, v
     return hipGetLastError();
 }
 
-hipError_t launch_after_adam(int precision, const AdamArgs& a, const float* W, int nl, int n_out, uint16_t* mlp_lp,
-                             void* fwd, void* bwd, hipStream_t s) {
+hipError_t launch_after_adam(int precision, const AdamArgs& a, const float* W, const float* W_ema, int nl, int n_out,
+                             uint16_t* mlp_lp, uint16_t* mlp_lp_ema, void* fwd, void* bwd, hipStream_t s) {
     const int nf = train_tiles_fwd(nl), nb = train_tiles_bwd(nl);
     if (precision == NRRS_BF16) {
         const int split = mfma::split_of<mfma::BF16>(n_out), ni = mfma::tiles_of(nl, split);
-        hipLaunchKernelGGL((k_after_adam<mfma::BF16, __bf16>), dim3(1 + ni + nf + nb), dim3(64), 0, s, a, W, nl, n_out,
-                           split, mlp_lp, ni, static_cast<__bf16*>(fwd), static_cast<__bf16*>(bwd), nf, nb);
+        const dim3 grid(1 + ni * (W_ema ? 2 : 1) + nf + nb);
+        hipLaunchKernelGGL((k_after_adam<mfma::BF16, __bf16>), grid, dim3(64), 0, s, a, W, W_ema, nl, n_out, split,
+                           mlp_lp, mlp_lp_ema, ni, static_cast<__bf16*>(fwd), static_cast<__bf16*>(bwd), nf, nb);
     } else if (precision == NRRS_FP16) {
         const int split = mfma::split_of<mfma::F16>(n_out), ni = mfma::tiles_of(nl, split);
-        hipLaunchKernelGGL((k_after_adam<mfma::F16, _Float16>), dim3(1 + ni + nf + nb), dim3(64), 0, s, a, W, nl,
-                           n_out, split, mlp_lp, ni, static_cast<_Float16*>(fwd), static_cast<_Float16*>(bwd), nf, nb);
+        const dim3 grid(1 + ni * (W_ema ? 2 : 1) + nf + nb);
+        hipLaunchKernelGGL((k_after_adam<mfma::F16, _Float16>), grid, dim3(64), 0, s, a, W, W_ema, nl, n_out, split,
+                           mlp_lp, mlp_lp_ema, ni, static_cast<_Float16*>(fwd), static_cast<_Float16*>(bwd), nf, nb);
     } else {
         return hipErrorInvalidValue;
     }

@@ -177,6 +177,11 @@ struct AdamArgs {
     bool scan_on_flag;
     int dynamic_scale;
     uint32_t growth_interval;
+    // Ema: after the update of step t, ema = (decay (1 - decay^(t-1)) ema + (1 - decay) p) /
+    // (1 - decay^t) (a skipped step leaves it), its grid part also written to ema_lp. Null: off.
+    float* ema = nullptr;
+    uint16_t* ema_lp = nullptr;
+    float ema_decay = 0.0f;
 };
 // Loss-scale and step state after an optimizer step (one thread): a step whose gradient had a
 // NaN/Inf is counted as skipped and halves the scale; growth_interval finite steps in a row
@@ -202,9 +207,10 @@ __device__ inline void adam_finish_state(const AdamArgs& a) {
 // adam_finish_state, as launch_after_adam does).
 hipError_t launch_adam(const AdamArgs& a, hipStream_t s, bool finish = true);
 // After launch_adam(..., false) of an FP16/BF16 net: the state update, the inference tiles
-// (mlp_lp) and the training tiles (fwd, bwd) of the updated MLP in one launch.
-hipError_t launch_after_adam(int precision, const AdamArgs& a, const float* W, int nl, int n_out, uint16_t* mlp_lp,
-                             void* fwd, void* bwd, hipStream_t s);
+// (mlp_lp) and the training tiles (fwd, bwd) of the updated MLP W, and, with a moving average
+// W_ema (or null), its inference tiles (mlp_lp_ema), in one launch.
+hipError_t launch_after_adam(int precision, const AdamArgs& a, const float* W, const float* W_ema, int nl, int n_out,
+                             uint16_t* mlp_lp, uint16_t* mlp_lp_ema, void* fwd, void* bwd, hipStream_t s);
 hipError_t launch_fill_f32(float* p, float v, uint64_t n, hipStream_t s);
 // Paired table (entry e = {StatNet word e, RRSNet word e}) from the nets' FP16/BF16 grids,
 // both 16-byte aligned with n_entries words each.

@@ -209,6 +209,7 @@ _SIGS = {
     "nrrs_param_count": (C.c_int, [_P, C.c_int32, C.POINTER(C.c_uint64), C.POINTER(C.c_uint64)]),
     "nrrs_params_get": (C.c_int, [_P, C.c_int32, _P]),
     "nrrs_params_set": (C.c_int, [_P, C.c_int32, _P]),
+    "nrrs_params_get_inference": (C.c_int, [_P, C.c_int32, _P]),
     "nrrs_encode": (C.c_int, [_P, C.c_int32, C.c_uint32, C.POINTER(Vertices), _P, _P, _P, _P]),
     "nrrs_grid_indices": (C.c_int, [_P, C.c_uint32, C.POINTER(Vertices), _P, _P]),
     "nrrs_statnet_infer": (C.c_int, [_P, C.c_uint32, C.POINTER(Vertices), _P, _P]),
@@ -431,6 +432,13 @@ class Context:
         g, m = self.param_count(net)
         out = np.zeros(g + m, np.float32)
         check(self.lib.nrrs_params_get(self.h, net, out.ctypes.data), "params_get")
+        return out
+
+    def params_get_inference(self, net):
+        """The parameters inference reads (the Ema moving average when the context keeps one)."""
+        g, m = self.param_count(net)
+        out = np.zeros(g + m, np.float32)
+        check(self.lib.nrrs_params_get_inference(self.h, net, out.ctypes.data), "params_get_inference")
         return out
 
     def params_set(self, net, arr):

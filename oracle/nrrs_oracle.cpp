@@ -904,6 +904,15 @@ double ora_train_grads(const nrrs_config* cfg, int32_t net, uint32_t n, const nr
     return loss;
 }
 
+// Ema optimizer (2net-simple.json:16-18; tiny-cuda-nn's "Ema", absent from the checkout):
+// after optimizer step t >= 1, the debiased exponential moving average of the parameters,
+// ema_t = sum_k (1 - d) d^(t-k) p_k / (1 - d^t), kept as the recursion
+// ema_t = (d (1 - d^(t-1)) ema_{t-1} + (1 - d) p_t) / (1 - d^t).
+void ora_ema_step(uint64_t n, double* ema, const double* p, double decay, uint32_t t) {
+    const double keep = decay * (1.0 - std::pow(decay, (double)(t - 1))), norm = 1.0 - std::pow(decay, (double)t);
+    for (uint64_t i = 0; i < n; ++i) ema[i] = (keep * ema[i] + (1.0 - decay) * p[i]) / norm;
+}
+
 // Adam (Kingma & Ba 2015) with L2 regularisation added to the gradient; step t >= 1.
 void ora_adam(uint64_t n, double* p, double* m, double* v, const double* g, double lr, double b1, double b2,
               double eps, double l2, uint32_t t) {

@@ -124,6 +124,10 @@ double ora_train_grads(const nrrs_config* cfg, int32_t net, uint32_t n, const nr
 void ora_adam(uint64_t n, double* p, double* m, double* v, const double* g, double lr, double b1, double b2,
               double eps, double l2, uint32_t t);
 
+/* Ema optimizer: the debiased moving average after optimizer step t >= 1 (ema updated in place
+ * from the parameters p of that step). */
+void ora_ema_step(uint64_t n, double* ema, const double* p, double decay, uint32_t t);
+
 /* FP32 multi-threaded CPU step (cpu_fp32.cpp; a timing baseline, checked against the FP64
  * functions above): chained StatNet -> RRSNet inference, q, normalisation, split and
  * compaction of n vertices, then one training step per net (StatNet on target_L with

@@ -204,6 +204,7 @@ _SIGS_TRAIN = {
                                      _P, C.c_int32, _P, _P, _P, _P, _P, _P, C.c_double, C.c_double, _P, _P]),
     "ora_adam": (None, [C.c_uint64, _P, _P, _P, _P, C.c_double, C.c_double, C.c_double, C.c_double,
                         C.c_double, C.c_uint32]),
+    "ora_ema_step": (None, [C.c_uint64, _P, _P, C.c_double, C.c_uint32]),
 }
 
 
@@ -283,6 +284,13 @@ def adam(p, m, v, g, lr, b1, b2, eps, l2, t):
         assert a.dtype == np.float64 and a.flags.c_contiguous
     _load_train().ora_adam(p.size, p.ctypes.data, m.ctypes.data, v.ctypes.data, g.ctypes.data, lr, b1, b2, eps,
                            l2, t)
+
+
+def ema_step(ema, p, decay, t):
+    """Ema moving average after optimizer step t (in place on a float64 array)."""
+    assert ema.dtype == np.float64 and ema.flags.c_contiguous
+    p = np.ascontiguousarray(p, np.float64)
+    _load_train().ora_ema_step(ema.size, ema.ctypes.data, p.ctypes.data, decay, t)
 
 
 def synth_targets(n, seed):

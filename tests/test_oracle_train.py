@@ -160,3 +160,25 @@ def test_adam_constant_gradient_closed_form(ora):
     m, v = np.zeros(1), np.zeros(1)
     ora.adam(p, m, v, np.array([0.0]), 1e-2, 0.9, 0.99, 1e-15, 0.5, 1)
     assert p[0] == pytest.approx(2.0 - 1e-2)
+
+
+def test_ema_is_the_debiased_weighted_average(ora):
+    """The Ema recursion (2net-simple.json:16-18) equals the explicit debiased average
+    sum_k (1 - d) d^(t-k) p_k / (1 - d^t) of the parameter trajectory: its first value is the
+    first step's parameters, a constant trajectory stays constant, and for d = 0.95 over 12
+    random steps it matches the weighted sum (a missing debias or an undamped old average fails)."""
+    rng = np.random.default_rng(0)
+    traj = rng.normal(0, 1, (12, 5))
+    for d in (0.95, 0.5):
+        ema = np.full(5, 123.0)  # the value before the first step does not matter
+        for t, p in enumerate(traj, start=1):
+            ora.ema_step(ema, p, d, t)
+            w = (1.0 - d) * d ** (t - np.arange(1, t + 1))
+            want = (w[:, None] * traj[:t]).sum(axis=0) / (1.0 - d ** t)
+            assert np.allclose(ema, want, rtol=1e-12, atol=1e-12)
+            if t == 1:
+                assert np.allclose(ema, traj[0], rtol=1e-15, atol=1e-15)
+    ema = np.zeros(3)
+    for t in range(1, 6):
+        ora.ema_step(ema, np.array([1.5, -2.0, 0.25]), 0.95, t)
+    assert np.allclose(ema, [1.5, -2.0, 0.25], rtol=1e-14)
