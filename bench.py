@@ -19,6 +19,8 @@ roofline; `roofline` is the entry of the step's dominant kernel, the paired
 level-major grid-encoding pass (k_encode_levels<BF16, true>, two launches per step).
 `matched_error` compares each precision's RRS stage (StatNet statistics, q) and both
 nets' gradients with the FP64 oracle on the first 2^16 vertices of the batch.
+`shipped_nets` times the same step with the reference's shipped nets (nrrs_shipped_config) in
+BF16 and FP32, with the BF16 q error against the FP64 oracle.
 `wavefront_frame` and `online_loop` time the 10-bounce persistent frame launch and the
 256-frame online loop; `online_loop.q_drift_vs_fp64` is the q drift of a reduced online loop
 trained in BF16 against the FP64 oracle trained on the same seed stream.
@@ -84,6 +86,7 @@ def parse():
     ap.add_argument("--no-frame", action="store_true", help="skip the multi-bounce wavefront and online loop")
     ap.add_argument("--no-peaks", action="store_true", help="skip the copy and GEMM measurements of the GPU")
     ap.add_argument("--no-oracle", action="store_true", help="skip the FP64-oracle error report")
+    ap.add_argument("--no-shipped", action="store_true", help="skip the step with the reference's shipped nets")
     ap.add_argument("--pmc-probe", action="store_true",
                     help="only run a 256 MiB device copy and, after 8 warm-up steps, 3 BF16 steps at 2^20 "
                          "(for rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE runs; the profile summaries use the "
@@ -202,13 +205,16 @@ def cpu_baseline(sample):
 
 
 class Path:
-    """One precision: context, inputs and the timed step."""
+    """One precision: context, inputs and the timed step. `shipped`: the reference's shipped nets
+    (nrrs_shipped_config: 8-level grid, 32-wide LeakyReLU nets, OneBlob inputs, grid-less RRSNet,
+    Ema of the weights) instead of the benchmark nets."""
 
-    def __init__(self, precision, torch, na, np, **cfg):
+    def __init__(self, precision, torch, na, np, shipped=False, **cfg):
         self.torch, self.na = torch, na
         n = N_VERTICES
         self.n = n
-        self.ctx = na.Context(precision=precision, n_pixels=N_PIXELS, max_queue=n, max_train=n, **cfg)
+        config = na.shipped_config if shipped else na.default_config
+        self.ctx = na.Context(config(precision=precision, n_pixels=N_PIXELS, max_queue=n, max_train=n, **cfg))
         rng = np.random.default_rng(0)
         for net in (na.STATNET, na.RRSNET):
             p = self.ctx.params_get(net)
@@ -348,6 +354,56 @@ def matched_errors(torch, na, np, precisions, m=1 << 16):
         out[name] = r
         del path, ctx
         torch.cuda.empty_cache()
+    return out
+
+
+def shipped_nets(torch, na, np, steps, warmup, m=1 << 16):
+    """The same step with the reference's shipped nets (nrrs_shipped_config) in BF16 and FP32:
+    chained StatNet -> RRSNet inference (the grid-less RRSNet reads the StatNet statistics) and
+    decision, then the joint training step with the Ema update; q and the StatNet statistics of
+    the BF16 stage against the FP64 oracle on the first m vertices of the batch."""
+    import copy
+
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    import oracle_ref as ora
+
+    out = {}
+    for name, prec in (("bf16", na.BF16), ("fp32", na.FP32)):
+        path = Path(prec, torch, na, np, shipped=True)
+        if name == "bf16":
+            ctx = path.ctx
+            sub = copy.copy(path.vb)
+            sub.n = m
+            hv = ora.HostVertices(sub.to_numpy())
+            film = path.film.cpu().numpy()
+            ps, pr = (ctx.params_get_inference(net).astype(np.float64) for net in (na.STATNET, na.RRSNET))
+            s64 = np.ascontiguousarray(ora.net_forward(ctx.cfg, na.STATNET, hv, film, None, ps).T)
+            raw64 = ora.net_forward(ctx.cfg, na.RRSNET, hv, film, s64, pr)[:, 0]
+            q64 = ora.q_f64(ctx.cfg, na.Q_RRSNET_CHAINED, hv, raw64, film)
+            q = torch.empty(m, device="cuda")
+            surv = torch.empty(8 * m, dtype=torch.int32, device="cuda")
+            ns = torch.zeros(1, dtype=torch.int32, device="cuda")
+            na.check(ctx.rrs_infer_decide(ctx.decide_args(sub, path.film, q, surv, ns, q_mode=na.Q_RRSNET_CHAINED)),
+                     "rrs_infer_decide")
+            stats = ctx.statnet_infer(sub).cpu().numpy().astype(np.float64)
+            rq = np.abs(q.cpu().numpy().astype(np.float64) - q64) / q64
+            err = {"vertices": m, "q_rel_err_max": float(rq.max()), "q_rel_err_mean": float(rq.mean()),
+                   "stats_err_max_rel_to_range": float(np.max(np.abs(stats - s64)) / np.max(np.abs(s64)))}
+        r = path.run(steps if name == "bf16" else max(2, steps // 2), warmup)
+        out[name] = {k: (round(v, 4) if isinstance(v, float) else v) for k, v in r.items()}
+        if name == "bf16":
+            out[name]["vs_fp64_oracle"] = err
+            out[name]["ops_ms"] = {k: round(v["ms_per_step"], 4) for k, v in path.op_table(3).items()}
+        del path
+        torch.cuda.empty_cache()
+    out["config"] = {"grid": "8 levels x 2 features, T=2^19, base 16, scale 2.0",
+                     "mlp": "StatNet 32 inputs (grid 16 + OneBlob 16) -> 32x3 LeakyReLU -> 6; RRSNet 16 inputs "
+                            "(NRRSNetworkInput1, no grid) -> 32x3 LeakyReLU -> 1",
+                     "optimizer": "Ema(0.95) of Adam (lr 5e-3 / 5e-4, l2 3e-6)", "vertices": N_VERTICES}
+    out["bf16_over_fp32"] = {"rrs_queries": round(out["bf16"]["rrs_queries_per_s"] / out["fp32"]["rrs_queries_per_s"], 3),
+                             "train_samples": round(out["bf16"]["train_samples_per_s"] /
+                                                    out["fp32"]["train_samples_per_s"], 3),
+                             "step": round(out["bf16"]["step_queries_per_s"] / out["fp32"]["step_queries_per_s"], 3)}
     return out
 
 
@@ -601,6 +657,9 @@ def main():
                 "rrs_queries": round(bf["rrs_queries_per_s"] / results[ref]["rrs_queries_per_s"], 3),
                 "train_samples": round(bf["train_samples_per_s"] / results[ref]["train_samples_per_s"], 3),
                 "step": round(bf["step_queries_per_s"] / results[ref]["step_queries_per_s"], 3)}
+    if not args.no_shipped:
+        out["shipped_nets"] = shipped_nets(torch, na, np, args.steps, args.warmup)
+        print(f"[bench] shipped nets: {json.dumps(out['shipped_nets'])}", file=sys.stderr, flush=True)
     if not args.no_peaks:
         pk = measured_peaks(torch)
         out["measured_peaks"] = pk
