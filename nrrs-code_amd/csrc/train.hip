@@ -153,10 +153,24 @@ struct Geo {
     static constexpr int kStageLd = kVert + 8;  // LDS row stride (elements), breaks bank aliasing
 };
 
+// FP16/BF16 operands stage vertex-major ([vertex][neuron], neurons contiguous), so that a lane
+// writes the neurons of its vertex with 8- and 16-byte stores, and the dW MFMAs read their
+// operands with the transposing LDS load ds_read_b64_tr_b16 (stage_tr). FP32 operands stage
+// neuron-major ([neuron][vertex]) and read 8 consecutive vertices of a row (stage_op).
 template <typename P>
+constexpr bool kTrStage = sizeof(typename P::elem) == 2;
+constexpr int kTrPitchD = 72;   // elements per vertex row of the delta stage (64 neurons + pad)
+constexpr int kTrPitchH = 104;  // of the input stage (80 channels, zeros up to 96, + pad)
+
+template <typename P, bool TR = kTrStage<P>>
 struct Stage {
     typename P::elem d[kStageRowsD * Geo<P>::kStageLd];
     typename P::elem h[kStageRowsH * Geo<P>::kStageLd];
+};
+template <typename P>
+struct Stage<P, true> {
+    typename P::elem d[Geo<P>::kVert * kTrPitchD];
+    typename P::elem h[Geo<P>::kVert * kTrPitchH];
 };
 
 // Number of 32x32 dW tiles of layer l (out rows x in cols).
@@ -187,6 +201,42 @@ __device__ __forceinline__ typename P::op stage_op(const typename P::elem* s, in
 #pragma unroll
     for (int j = 0; j < 8; ++j) v[j] = p[j];
     return v;
+}
+
+// LDS operand of the vertex-major stage: lane l gets neuron n0 + l % 32 of vertices
+// v0 + 8 (l / 32) + 0..7 (the A/B operand layout of v_mfma_f32_32x32x16, k = vertex), in two
+// ds_read_b64_tr_b16. That load treats each group of 16 lanes as a unit: lane i of the group
+// receives element i % 4 of the 4-element (8-byte) loads of lanes 4j + i / 4, j = 0..3 (checked
+// on the device with an exact-integer probe). So lane p loads the four neurons
+// n0 + 16 (group % 2) + 4 (p % 4) .. +3 of vertex v0 + 8 h + 4r + p / 4, and lane i ends with
+// neuron n0 + 16 (group % 2) + i of the four vertices v0 + 8 h + 4r .. +3: elements 4r..4r+3.
+typedef short i16x4 __attribute__((ext_vector_type(4)));
+typedef __attribute__((address_space(3))) i16x4 lds_i16x4;
+template <typename P>
+__device__ __forceinline__ typename P::op stage_tr(const typename P::elem* s, int pitch, int n0, int v0, int lane) {
+    const int n = n0 + 16 * ((lane >> 4) & 1) + 4 * (lane & 3);
+    const int v = v0 + 8 * (lane >> 5) + ((lane & 15) >> 2);
+    const i16x4 r0 = __builtin_amdgcn_ds_read_tr16_b64_v4i16((lds_i16x4*)(s + v * pitch + n));
+    const i16x4 r1 = __builtin_amdgcn_ds_read_tr16_b64_v4i16((lds_i16x4*)(s + (v + 4) * pitch + n));
+    typedef short i16x8 __attribute__((ext_vector_type(8)));
+    const i16x8 r = {r0[0], r0[1], r0[2], r0[3], r1[0], r1[1], r1[2], r1[3]};
+    return __builtin_bit_cast(typename P::op, r);
+}
+
+// Stores of one operand of 8 16-bit values into row `row` (pitch elements) of a vertex-major
+// stage: elements 0-3 at column c0, 4-7 at column c1 (8-byte stores), or all 8 at c0 (c1 < 0,
+// one 16-byte store).
+template <typename P>
+__device__ __forceinline__ void stage_put(typename P::elem* s, int pitch, int row, const typename P::op& v, int c0,
+                                          int c1) {
+    const u32x4 u = __builtin_bit_cast(u32x4, v);
+    typename P::elem* r = s + row * pitch;
+    if (c1 < 0) {
+        *reinterpret_cast<u32x4*>(r + c0) = u;
+    } else {
+        *reinterpret_cast<uint2*>(r + c0) = make_uint2(u[0], u[1]);
+        *reinterpret_cast<uint2*>(r + c1) = make_uint2(u[2], u[3]);
+    }
 }
 
 // Forward of hidden layer `layer` (1..NL-2) or layer 0 from operands X (5 chunks
@@ -300,8 +350,12 @@ __global__ __launch_bounds__(64 * P::kWaves) void k_train(TrainArgs t) {
     for (int s = 0; s < kSlots; ++s) acc[s] = f32x16{};
     if (lane == 0) loss_part[wave] = 0.0;  // by the wave that accumulates into it
 
-    // zero the padded input rows once (layer 0 has 80 of 96 rows)
-    for (int k = threadIdx.x; k < 16 * kStageLd; k += kThreads) st.h[80 * kStageLd + k] = (typename P::elem)0.0f;
+    // zero the padded input channels once (layer 0 has 80 of 96)
+    if constexpr (kTrStage<P>) {
+        for (int k = threadIdx.x; k < 16 * kVert; k += kThreads) st.h[(k >> 4) * kTrPitchH + 80 + (k & 15)] = (typename P::elem)0.0f;
+    } else {
+        for (int k = threadIdx.x; k < 16 * kStageLd; k += kThreads) st.h[80 * kStageLd + k] = (typename P::elem)0.0f;
+    }
     __syncthreads();  // weight tiles in LDS before the first forward
 
     const uint32_t n_iter = (t.n + kVert - 1) / kVert;
@@ -523,31 +577,51 @@ __global__ __launch_bounds__(64 * P::kWaves) void k_train(TrainArgs t) {
             }
             // (2) stage delta_l rows [out neuron][vertex] and h_l rows [in neuron][vertex]
             __syncthreads();  // previous layer's dW reads are done
-            if (last) {
+            if constexpr (kTrStage<P>) {
+                // vertex-major: operand element j of chunk kb is neuron 16kb + 8(j/4) + 4h + j%4
+                // (two runs of 4), of the encoding channel 16kb + 8h + j (one run of 8)
+                if (last) {
+                    if (h == 0) stage_put<P>(st.d, kTrPitchD, vcol, Dl[0], 0, -1);  // outputs 0-7
+                } else {
 #pragma unroll
-                for (int j = 0; j < 8; ++j)
-                    if (h == 0 && j < NOUT) st.d[j * kStageLd + vcol] = (typename P::elem)P::get(Dl[0], j);
+                    for (int kb = 0; kb < 4; ++kb)
+                        stage_put<P>(st.d, kTrPitchD, vcol, Dl[kb], 16 * kb + 4 * h, 16 * kb + 8 + 4 * h);
+                }
+                if (layer == 0) {
+#pragma unroll
+                    for (int kb = 0; kb < 5; ++kb) stage_put<P>(st.h, kTrPitchH, vcol, E[kb], 16 * kb + 8 * h, -1);
+                } else {
+#pragma unroll
+                    for (int kb = 0; kb < 4; ++kb)
+                        stage_put<P>(st.h, kTrPitchH, vcol, H[kb], 16 * kb + 4 * h, 16 * kb + 8 + 4 * h);
+                }
             } else {
-#pragma unroll
-                for (int kb = 0; kb < 4; ++kb)
-#pragma unroll
-                    for (int j = 0; j < 8; ++j)
-                        st.d[(16 * kb + 8 * (j >> 2) + 4 * h + (j & 3)) * kStageLd + vcol] =
-                            (typename P::elem)P::get(Dl[kb], j);
-            }
-            if (layer == 0) {
-#pragma unroll
-                for (int kb = 0; kb < 5; ++kb)
+                if (last) {
 #pragma unroll
                     for (int j = 0; j < 8; ++j)
-                        st.h[(kb * 16 + 8 * h + j) * kStageLd + vcol] = (typename P::elem)P::get(E[kb], j);
-            } else {
+                        if (h == 0 && j < NOUT) st.d[j * kStageLd + vcol] = (typename P::elem)P::get(Dl[0], j);
+                } else {
 #pragma unroll
-                for (int kb = 0; kb < 4; ++kb)
+                    for (int kb = 0; kb < 4; ++kb)
 #pragma unroll
-                    for (int j = 0; j < 8; ++j)
-                        st.h[(16 * kb + 8 * (j >> 2) + 4 * h + (j & 3)) * kStageLd + vcol] =
-                            (typename P::elem)P::get(H[kb], j);
+                        for (int j = 0; j < 8; ++j)
+                            st.d[(16 * kb + 8 * (j >> 2) + 4 * h + (j & 3)) * kStageLd + vcol] =
+                                (typename P::elem)P::get(Dl[kb], j);
+                }
+                if (layer == 0) {
+#pragma unroll
+                    for (int kb = 0; kb < 5; ++kb)
+#pragma unroll
+                        for (int j = 0; j < 8; ++j)
+                            st.h[(kb * 16 + 8 * h + j) * kStageLd + vcol] = (typename P::elem)P::get(E[kb], j);
+                } else {
+#pragma unroll
+                    for (int kb = 0; kb < 4; ++kb)
+#pragma unroll
+                        for (int j = 0; j < 8; ++j)
+                            st.h[(16 * kb + 8 * (j >> 2) + 4 * h + (j & 3)) * kStageLd + vcol] =
+                                (typename P::elem)P::get(H[kb], j);
+                }
             }
             __syncthreads();
             // (3) dW tiles of this layer owned by this wave
@@ -564,8 +638,14 @@ __global__ __launch_bounds__(64 * P::kWaves) void k_train(TrainArgs t) {
                         if (s2 == slot) {
 #pragma unroll
                             for (int s = 0; s < kVert / 16; ++s) {
-                                op A = stage_op<P>(st.d, to * 32 + col, 16 * s + 8 * h);
-                                op B = stage_op<P>(st.h, ti * 32 + col, 16 * s + 8 * h);
+                                op A, B;
+                                if constexpr (kTrStage<P>) {
+                                    A = stage_tr<P>(st.d, kTrPitchD, to * 32, 16 * s, lane);
+                                    B = stage_tr<P>(st.h, kTrPitchH, ti * 32, 16 * s, lane);
+                                } else {
+                                    A = stage_op<P>(st.d, to * 32 + col, 16 * s + 8 * h);
+                                    B = stage_op<P>(st.h, ti * 32 + col, 16 * s + 8 * h);
+                                }
                                 acc[s2] = P::mma16(A, B, acc[s2]);
                             }
                         }
@@ -629,6 +709,7 @@ __global__ __launch_bounds__(64 * P::kWaves) void k_train(TrainArgs t) {
                     //  - FP32 table: lane parity = feature, both features of a corner are one 8-byte
                     //    entry.
                     __syncthreads();
+                    static_assert(sizeof(st.d) >= 32 * kVert * sizeof(float), "grid-gradient stage");
                     float* gst = reinterpret_cast<float*>(st.d);  // [32][kVert]
 #pragma unroll
                     for (int r = 0; r < 16; ++r) gst[(8 * (r >> 2) + 4 * h + (r & 3)) * kVert + vcol] = G[r];
