@@ -157,10 +157,17 @@ struct Geo {
 // writes the neurons of its vertex with 8- and 16-byte stores, and the dW MFMAs read their
 // operands with the transposing LDS load ds_read_b64_tr_b16 (stage_tr). FP32 operands stage
 // neuron-major ([neuron][vertex]) and read 8 consecutive vertices of a row (stage_op).
+// Rows of the vertex-major stages: 64 neurons (channels 0-63 of the layer-0 input) padded to 68
+// elements; channels 64-95 of the layer-0 input (80-95 zero) in a second stage padded to 36. With
+// rows of 34 and 18 dwords, the 8-byte stores of 32 consecutive rows (one per lane) start on 32
+// different even banks of a 64-bank LDS, and rows 8 apart start 16 banks apart, so the four rows
+// that one group of 16 lanes reads with a transposing load (stage_tr) are taken 8 vertices apart:
+// k-step s of the dW contraction covers vertices 32 (s / 2) + 4 (s % 2) + 8j + 2h + r (lane half
+// h, load r, element j).
 template <typename P>
 constexpr bool kTrStage = sizeof(typename P::elem) == 2;
-constexpr int kTrPitchD = 72;   // elements per vertex row of the delta stage (64 neurons + pad)
-constexpr int kTrPitchH = 104;  // of the input stage (80 channels, zeros up to 96, + pad)
+constexpr int kTrRow = 68, kTrRow2 = 36;  // elements per row of the main and the second stage
+__host__ __device__ constexpr int tr_kstep_row(int s) { return 32 * (s >> 1) + 4 * (s & 1); }
 
 template <typename P, bool TR = kTrStage<P>>
 struct Stage {
@@ -169,8 +176,9 @@ struct Stage {
 };
 template <typename P>
 struct Stage<P, true> {
-    typename P::elem d[Geo<P>::kVert * kTrPitchD];
-    typename P::elem h[Geo<P>::kVert * kTrPitchH];
+    typename P::elem d[Geo<P>::kVert * kTrRow];
+    typename P::elem h[Geo<P>::kVert * kTrRow];
+    typename P::elem h2[Geo<P>::kVert * kTrRow2];
 };
 
 // Number of 32x32 dW tiles of layer l (out rows x in cols).
@@ -203,40 +211,41 @@ __device__ __forceinline__ typename P::op stage_op(const typename P::elem* s, in
     return v;
 }
 
-// LDS operand of the vertex-major stage: lane l gets neuron n0 + l % 32 of vertices
-// v0 + 8 (l / 32) + 0..7 (the A/B operand layout of v_mfma_f32_32x32x16, k = vertex), in two
-// ds_read_b64_tr_b16. That load treats each group of 16 lanes as a unit: lane i of the group
-// receives element i % 4 of the 4-element (8-byte) loads of lanes 4j + i / 4, j = 0..3 (checked
-// on the device with an exact-integer probe). So lane p loads the four neurons
-// n0 + 16 (group % 2) + 4 (p % 4) .. +3 of vertex v0 + 8 h + 4r + p / 4, and lane i ends with
-// neuron n0 + 16 (group % 2) + i of the four vertices v0 + 8 h + 4r .. +3: elements 4r..4r+3.
+// LDS operand of the vertex-major stages: lane l gets neuron n0 + l % 32 of the 8 vertices of
+// its lane half h = l / 32 in one k-step (the A/B operand layout of v_mfma_f32_32x32x16 with the
+// k-step's vertices as k), in two ds_read_b64_tr_b16. That load treats each group of 16 lanes as
+// a unit: lane i of the group receives element i % 4 of the 4-element (8-byte) loads of lanes
+// 4j + i / 4, j = 0..3 (checked on the device with an exact-integer probe). So lane p of the group
+// loads the four neurons n0 + 16 (group % 2) + 4 (p % 4) .. +3 of the vertex of element j = p / 4,
+// and lane i ends with neuron n0 + 16 (group % 2) + i of the vertices of elements 0..3 (load r
+// gives operand elements 4r..4r+3). `s` points at the k-step's first row and the tile's first
+// neuron; off[r] is the lane's offset (tr_lane_offsets).
 typedef short i16x4 __attribute__((ext_vector_type(4)));
 typedef __attribute__((address_space(3))) i16x4 lds_i16x4;
 template <typename P>
-__device__ __forceinline__ typename P::op stage_tr(const typename P::elem* s, int pitch, int n0, int v0, int lane) {
-    const int n = n0 + 16 * ((lane >> 4) & 1) + 4 * (lane & 3);
-    const int v = v0 + 8 * (lane >> 5) + ((lane & 15) >> 2);
-    const i16x4 r0 = __builtin_amdgcn_ds_read_tr16_b64_v4i16((lds_i16x4*)(s + v * pitch + n));
-    const i16x4 r1 = __builtin_amdgcn_ds_read_tr16_b64_v4i16((lds_i16x4*)(s + (v + 4) * pitch + n));
+__device__ __forceinline__ typename P::op stage_tr(const typename P::elem* s, int off0, int off1) {
+    const i16x4 r0 = __builtin_amdgcn_ds_read_tr16_b64_v4i16((lds_i16x4*)(s + off0));
+    const i16x4 r1 = __builtin_amdgcn_ds_read_tr16_b64_v4i16((lds_i16x4*)(s + off1));
     typedef short i16x8 __attribute__((ext_vector_type(8)));
     const i16x8 r = {r0[0], r0[1], r0[2], r0[3], r1[0], r1[1], r1[2], r1[3]};
     return __builtin_bit_cast(typename P::op, r);
 }
+// The lane's offsets for loads r = 0, 1 in a stage of `len` elements per row: the row of element
+// j = (l % 16) / 4 of load r in lane half h (8j + 2h + r after the k-step's first row) and the
+// lane's four neurons 16 (group % 2) + 4 (l % 4).
+__device__ __forceinline__ int tr_lane_offset(int lane, int r, int len) {
+    return (8 * ((lane & 15) >> 2) + 2 * (lane >> 5) + r) * len + 16 * ((lane >> 4) & 1) + 4 * (lane & 3);
+}
 
-// Stores of one operand of 8 16-bit values into row `row` (pitch elements) of a vertex-major
-// stage: elements 0-3 at column c0, 4-7 at column c1 (8-byte stores), or all 8 at c0 (c1 < 0,
-// one 16-byte store).
+// Stores of one operand of 8 16-bit values into row `row` of a vertex-major stage (len elements
+// per row): elements 0-3 at column c0, 4-7 at column c1 (8-byte stores).
 template <typename P>
-__device__ __forceinline__ void stage_put(typename P::elem* s, int pitch, int row, const typename P::op& v, int c0,
+__device__ __forceinline__ void stage_put(typename P::elem* s, int len, int row, const typename P::op& v, int c0,
                                           int c1) {
-    const u32x4 u = __builtin_bit_cast(u32x4, v);
-    typename P::elem* r = s + row * pitch;
-    if (c1 < 0) {
-        *reinterpret_cast<u32x4*>(r + c0) = u;
-    } else {
-        *reinterpret_cast<uint2*>(r + c0) = make_uint2(u[0], u[1]);
-        *reinterpret_cast<uint2*>(r + c1) = make_uint2(u[2], u[3]);
-    }
+    const u32x4 w = __builtin_bit_cast(u32x4, v);
+    typename P::elem* r = s + row * len;
+    *reinterpret_cast<uint2*>(r + c0) = make_uint2(w[0], w[1]);
+    *reinterpret_cast<uint2*>(r + c1) = make_uint2(w[2], w[3]);
 }
 
 // Forward of hidden layer `layer` (1..NL-2) or layer 0 from operands X (5 chunks
@@ -330,6 +339,7 @@ __global__ __launch_bounds__(64 * P::kWaves) void k_train(TrainArgs t) {
     // scalar branches
     const int lane = threadIdx.x & 63, h = lane >> 5, col = lane & 31;
     const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+
     // FP16/BF16: the packed forward and W^T tiles live in LDS behind the stage (they are read
     // by every layer of every iteration); FP32: from global memory (cache)
     constexpr int kTilesF = 10 + (NL - 2) * 8 + 4, kTilesB = 4 + (NL - 2) * 8 + 2;
@@ -352,7 +362,7 @@ __global__ __launch_bounds__(64 * P::kWaves) void k_train(TrainArgs t) {
 
     // zero the padded input channels once (layer 0 has 80 of 96)
     if constexpr (kTrStage<P>) {
-        for (int k = threadIdx.x; k < 16 * kVert; k += kThreads) st.h[(k >> 4) * kTrPitchH + 80 + (k & 15)] = (typename P::elem)0.0f;
+        for (int k = threadIdx.x; k < kTrRow2 * kVert; k += kThreads) st.h2[k] = (typename P::elem)0.0f;
     } else {
         for (int k = threadIdx.x; k < 16 * kStageLd; k += kThreads) st.h[80 * kStageLd + k] = (typename P::elem)0.0f;
     }
@@ -579,21 +589,22 @@ __global__ __launch_bounds__(64 * P::kWaves) void k_train(TrainArgs t) {
             __syncthreads();  // previous layer's dW reads are done
             if constexpr (kTrStage<P>) {
                 // vertex-major: operand element j of chunk kb is neuron 16kb + 8(j/4) + 4h + j%4
-                // (two runs of 4), of the encoding channel 16kb + 8h + j (one run of 8)
+                // (columns 16kb + 4h and 16kb + 8 + 4h), of the encoding channel 16kb + 8h + j
+                // (columns 16kb + 8h and 16kb + 8h + 4; chunk 4 in the second stage)
                 if (last) {
-                    if (h == 0) stage_put<P>(st.d, kTrPitchD, vcol, Dl[0], 0, -1);  // outputs 0-7
+                    if (h == 0) stage_put<P>(st.d, kTrRow, vcol, Dl[0], 0, 4);  // outputs 0-7
                 } else {
 #pragma unroll
-                    for (int kb = 0; kb < 4; ++kb)
-                        stage_put<P>(st.d, kTrPitchD, vcol, Dl[kb], 16 * kb + 4 * h, 16 * kb + 8 + 4 * h);
+                    for (int kb = 0; kb < 4; ++kb) stage_put<P>(st.d, kTrRow, vcol, Dl[kb], 16 * kb + 4 * h, 16 * kb + 8 + 4 * h);
                 }
                 if (layer == 0) {
 #pragma unroll
-                    for (int kb = 0; kb < 5; ++kb) stage_put<P>(st.h, kTrPitchH, vcol, E[kb], 16 * kb + 8 * h, -1);
+                    for (int kb = 0; kb < 4; ++kb)
+                        stage_put<P>(st.h, kTrRow, vcol, E[kb], 16 * kb + 8 * h, 16 * kb + 8 * h + 4);
+                    stage_put<P>(st.h2, kTrRow2, vcol, E[4], 8 * h, 8 * h + 4);
                 } else {
 #pragma unroll
-                    for (int kb = 0; kb < 4; ++kb)
-                        stage_put<P>(st.h, kTrPitchH, vcol, H[kb], 16 * kb + 4 * h, 16 * kb + 8 + 4 * h);
+                    for (int kb = 0; kb < 4; ++kb) stage_put<P>(st.h, kTrRow, vcol, H[kb], 16 * kb + 4 * h, 16 * kb + 8 + 4 * h);
                 }
             } else {
                 if (last) {
@@ -631,6 +642,13 @@ __global__ __launch_bounds__(64 * P::kWaves) void k_train(TrainArgs t) {
                 for (int g = dw_first(base, wave, kWaves); g < base + n_tiles; g += kWaves) {
                     const int tt = g - base, slot = g / kWaves;
                     const int to = tt / n_ti, ti = tt % n_ti;
+                    // offsets of the transposing loads (FP16/BF16 stages), recomputed per tile
+                    // (cheaper than keeping four registers through the whole kernel)
+                    int ln = lane;
+                    asm volatile("" : "+v"(ln));
+                    [[maybe_unused]] const int tro[2] = {tr_lane_offset(ln, 0, kTrRow), tr_lane_offset(ln, 1, kTrRow)};
+                    [[maybe_unused]] const int tro2[2] = {tr_lane_offset(ln, 0, kTrRow2),
+                                                          tr_lane_offset(ln, 1, kTrRow2)};
                     // the MFMAs accumulate straight into the tile's register slot; the slot is
                     // wave-uniform, so its choice is a scalar branch
 #pragma unroll
@@ -640,8 +658,10 @@ __global__ __launch_bounds__(64 * P::kWaves) void k_train(TrainArgs t) {
                             for (int s = 0; s < kVert / 16; ++s) {
                                 op A, B;
                                 if constexpr (kTrStage<P>) {
-                                    A = stage_tr<P>(st.d, kTrPitchD, to * 32, 16 * s, lane);
-                                    B = stage_tr<P>(st.h, kTrPitchH, ti * 32, 16 * s, lane);
+                                    const int row = tr_kstep_row(s);
+                                    A = stage_tr<P>(st.d + row * kTrRow + 32 * to, tro[0], tro[1]);
+                                    B = ti < 2 ? stage_tr<P>(st.h + row * kTrRow + 32 * ti, tro[0], tro[1])
+                                               : stage_tr<P>(st.h2 + row * kTrRow2, tro2[0], tro2[1]);
                                 } else {
                                     A = stage_op<P>(st.d, to * 32 + col, 16 * s + 8 * h);
                                     B = stage_op<P>(st.h, ti * 32 + col, 16 * s + 8 * h);
