@@ -175,6 +175,17 @@ bool shipped_live(const nrrs_config& c, int net, int layer, int n_layers, int ro
     return (col >= 32 && col < 44) || (col >= 65 && col < 69);
 }
 
+// Layer-0 input chunks (16 channels each, bit c for channels 16c..16c+15) that hold an input of
+// the net: all five for the benchmark nets, those with a live column for the shipped ones.
+uint32_t l0_chunks(const nrrs_config& c, int net) {
+    if (c.net_set != NRRS_NETS_SHIPPED) return 0x1fu;
+    const int nl = (int)(net == NRRS_RRSNET ? c.rrsnet : c.statnet).n_hidden_layers + 1;
+    uint32_t m = 0;
+    for (int col = 0; col < kEnc; ++col)
+        if (shipped_live(c, net, 0, nl, 0, col)) m |= 1u << (col / 16);
+    return m;
+}
+
 // AllSelf (ll2_two_head = 3): the StatNet of L (NRRS_STATNET) and the StatNet of L^2
 // (NRRS_STATNET2), each with its own table, optimizer and loss (twn_AllSelf.cpp).
 bool all_self(const nrrs_config& c) { return c.ll2_two_head == 3; }
@@ -336,6 +347,7 @@ hipError_t run_net(nrrs_ctx* c, int net, const EncodeArgs& a_in, int epi, uint32
     const NetView w = view_of(c, net, train_side);
     EncodeArgs a = a_in;
     a.grid = ns.grid;  // the net's own grid (none for the shipped RRSNet)
+    a.l0_chunks = l0_chunks(c->cfg, net);
     OpScope t(c, op_label(net, "infer_statnet", "infer_rrsnet", "infer_statnet2"), s);
     const int k = kind_of(net);
     if (c->cfg.precision == NRRS_FP32) {
@@ -582,6 +594,7 @@ nrrs_status run_backward(nrrs_ctx* c, int net, uint32_t n, const nrrs_vertices& 
     TrainArgs t{};
     t.a = encode_args(c, v, film, stats, stats_stride);
     t.a.grid = ns.grid;
+    t.a.l0_chunks = l0_chunks(cfg, net);
     // the RRSNet training input takes activationSigma(max(L2, 0)) when StatNet models sigma
     // (train.cu:196-200); inference keeps max(L2, 0)
     t.a.sigma_l2 = cfg.train_sigma && net == NRRS_RRSNET;

@@ -74,6 +74,9 @@ struct EncodeArgs {
     int aid = 0;
     // network set of the non-grid channels (NRRS_NETS_BENCH / NRRS_NETS_SHIPPED)
     int net_set = 0;
+    // layer-0 K chunks (input channels 16c..16c+15, bit c) that hold an input of the net; the
+    // 32-wide shipped nets' kernels multiply only these
+    uint32_t l0_chunks = 0x1fu;
 };
 
 struct DecideParams {

@@ -267,10 +267,18 @@ __device__ __forceinline__ float act_fwd(float x) {
         return fmaxf(x, 0.0f);
 }
 
-template <typename P, int ACT = ACT_RELU>
+// Hidden width W: 64, or 32 for the shipped nets, whose 64-wide storage holds zeros outside the
+// 32-wide net: their kernels compute output tile 0 and K chunks 0-1 of the hidden layers only,
+// and of layer 0 the K chunks set in a mask (the chunks that hold the net's inputs).
+template <int W>
+constexpr int kOutTiles = W / 32;
+template <int W>
+constexpr int kHidChunks = W / 16;
+
+template <typename P, int ACT = ACT_RELU, int W = 64>
 __device__ __forceinline__ void relu_to_operands(const f32x16* D, typename P::vec8* H) {
 #pragma unroll
-    for (int ot = 0; ot < 2; ++ot) {
+    for (int ot = 0; ot < kOutTiles<W>; ++ot) {
 #pragma unroll
         for (int g = 0; g < 2; ++g) {
             float x[8];
@@ -283,10 +291,10 @@ __device__ __forceinline__ void relu_to_operands(const f32x16* D, typename P::ve
 
 // ReLU of the last hidden layer as a two-term operand: H = lp(x) and L = lp(x - H) (the
 // difference is exact in FP32), so that H + L holds x to about 16 bits.
-template <typename P, int ACT = ACT_RELU>
+template <typename P, int ACT = ACT_RELU, int W = 64>
 __device__ __forceinline__ void relu_to_operands_split(const f32x16* D, typename P::vec8* H, typename P::vec8* L) {
 #pragma unroll
-    for (int ot = 0; ot < 2; ++ot) {
+    for (int ot = 0; ot < kOutTiles<W>; ++ot) {
 #pragma unroll
         for (int g = 0; g < 2; ++g) {
             float x[8];
@@ -305,55 +313,59 @@ __device__ __forceinline__ void relu_to_operands_split(const f32x16* D, typename
 // is about as accurate as an FP32 product of the FP32 activations (the outputs reach q through
 // a square root near zero, where an 8-bit-mantissa product is too coarse). SPLIT runs every
 // hidden layer the same way, and layer 0 too on its SH and per-vertex chunks (Bl: their residual
-// terms), the grid chunks staying single-term.
-template <typename P, int NL, bool SPLIT = false, int ACT = ACT_RELU>
+// terms), the grid chunks staying single-term. W, l0: the hidden width and the live layer-0
+// K chunks (above).
+template <typename P, int NL, bool SPLIT = false, int ACT = ACT_RELU, int W = 64>
 __device__ __forceinline__ f32x16 mlp_forward(const u32x4* __restrict__ sW, const typename P::vec8* B,
-                                              const typename P::vec8* Bl, int lane) {
+                                              const typename P::vec8* Bl, int lane, uint32_t l0 = 0x1fu) {
+    constexpr int kOT = kOutTiles<W>, kKB = kHidChunks<W>;
     f32x16 D[2];
 #pragma unroll
-    for (int ot = 0; ot < 2; ++ot) {
+    for (int ot = 0; ot < kOT; ++ot) {
         D[ot] = f32x16{};
         if constexpr (SPLIT) {
 #pragma unroll
             for (int kb = 2; kb < 5; ++kb) {
+                if (W == 32 && !((l0 >> kb) & 1u)) continue;
                 D[ot] = P::mma(lds_tile(sW, 10 + ot * 3 + (kb - 2), lane), B[kb], D[ot]);
                 D[ot] = P::mma(lds_tile(sW, ot * 5 + kb, lane), Bl[kb - 2], D[ot]);
             }
         }
 #pragma unroll
-        for (int kb = 0; kb < 5; ++kb) D[ot] = P::mma(lds_tile(sW, ot * 5 + kb, lane), B[kb], D[ot]);
+        for (int kb = 0; kb < 5; ++kb)
+            if (W == 64 || ((l0 >> kb) & 1u)) D[ot] = P::mma(lds_tile(sW, ot * 5 + kb, lane), B[kb], D[ot]);
     }
     typename P::vec8 H[4], L[4];
 #pragma unroll
     for (int layer = 1; layer < NL - 1; ++layer) {
         const int base = hidden_base(layer, SPLIT);
         if constexpr (SPLIT)
-            relu_to_operands_split<P, ACT>(D, H, L);
+            relu_to_operands_split<P, ACT, W>(D, H, L);
         else
-            relu_to_operands<P, ACT>(D, H);
+            relu_to_operands<P, ACT, W>(D, H);
 #pragma unroll
-        for (int ot = 0; ot < 2; ++ot) {
+        for (int ot = 0; ot < kOT; ++ot) {
             D[ot] = f32x16{};
             if constexpr (SPLIT) {
 #pragma unroll
-                for (int kb = 0; kb < 4; ++kb) {
+                for (int kb = 0; kb < kKB; ++kb) {
                     D[ot] = P::mma(lds_tile(sW, base + 8 + ot * 4 + kb, lane), H[kb], D[ot]);
                     D[ot] = P::mma(lds_tile(sW, base + ot * 4 + kb, lane), L[kb], D[ot]);
                 }
             }
 #pragma unroll
-            for (int kb = 0; kb < 4; ++kb) D[ot] = P::mma(lds_tile(sW, base + ot * 4 + kb, lane), H[kb], D[ot]);
+            for (int kb = 0; kb < kKB; ++kb) D[ot] = P::mma(lds_tile(sW, base + ot * 4 + kb, lane), H[kb], D[ot]);
         }
     }
-    relu_to_operands_split<P, ACT>(D, H, L);
+    relu_to_operands_split<P, ACT, W>(D, H, L);
     f32x16 Y = f32x16{};
     const int base = out_base(NL, SPLIT);
 #pragma unroll
-    for (int kb = 0; kb < 4; ++kb) Y = P::mma(lds_tile(sW, base + 4 + kb, lane), H[kb], Y);
+    for (int kb = 0; kb < kKB; ++kb) Y = P::mma(lds_tile(sW, base + 4 + kb, lane), H[kb], Y);
 #pragma unroll
-    for (int kb = 0; kb < 4; ++kb) Y = P::mma(lds_tile(sW, base + kb, lane), L[kb], Y);
+    for (int kb = 0; kb < kKB; ++kb) Y = P::mma(lds_tile(sW, base + kb, lane), L[kb], Y);
 #pragma unroll
-    for (int kb = 0; kb < 4; ++kb) Y = P::mma(lds_tile(sW, base + kb, lane), H[kb], Y);
+    for (int kb = 0; kb < kKB; ++kb) Y = P::mma(lds_tile(sW, base + kb, lane), H[kb], Y);
     return Y;
 }
 

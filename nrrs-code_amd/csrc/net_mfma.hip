@@ -103,7 +103,7 @@ __global__ __launch_bounds__(kRunVert) void k_encode_levels(EncodeArgs a, const 
 // Four waves per SIMD (at most 128 registers): the weight tiles are read from LDS in every
 // iteration of the persistent loop rather than kept in registers, which would take 120-152
 // of them and leave one or two waves per SIMD to hide the latency of the loads.
-template <typename P, int NL, int NOUT, int EPI, int ACT>
+template <typename P, int NL, int NOUT, int EPI, int ACT, int W>
 __global__ __launch_bounds__(kBlockT, 4) void k_net_mfma(EncodeArgs a, int net, uint32_t n,
                                                       const uint32_t* __restrict__ table,
                                                       const u32x4* __restrict__ Wp, EpiArgs ep) {
@@ -133,7 +133,7 @@ __global__ __launch_bounds__(kBlockT, 4) void k_net_mfma(EncodeArgs a, int net, 
 #pragma unroll
             for (int kb = 0; kb < 3; ++kb) Bl[kb] = typename P::vec8{};
         }
-        f32x16 Y = mlp_forward<P, NL, kSplit, ACT>(sW, B, Bl, lane);
+        f32x16 Y = mlp_forward<P, NL, kSplit, ACT, W>(sW, B, Bl, lane, a.l0_chunks);
 
         // output c of vertex i sits in lane half c / 4, register c % 4
         if (EPI == EPI_RAW) {
@@ -283,18 +283,19 @@ hipError_t launch_chained(const EncodeArgs& a, const uint32_t* enc_rrs, uint32_t
     return hipGetLastError();
 }
 
-template <typename P, int NL, int NOUT, int EPI, int ACT = ACT_RELU>
+template <typename P, int NL, int NOUT, int EPI, int ACT = ACT_RELU, int W = 64>
 hipError_t launch_one(const EncodeArgs& a, int net, uint32_t n, const uint16_t* table, const uint16_t* Wp,
                       const EpiArgs& ep, hipStream_t s) {
     constexpr size_t smem = (size_t)tiles_of(NL, split_of<P>(NOUT)) * kTileHalves * 2;
     static std::atomic<bool> attr[kMaxDevices];
-    hipError_t e = set_dyn_lds_once(attr, reinterpret_cast<const void*>(&k_net_mfma<P, NL, NOUT, EPI, ACT>), smem);
+    hipError_t e =
+        set_dyn_lds_once(attr, reinterpret_cast<const void*>(&k_net_mfma<P, NL, NOUT, EPI, ACT, W>), smem);
     if (e != hipSuccess) return e;
     const uint32_t n_cb = (n + 31) / 32;
     uint32_t blocks = (n_cb + kWaves - 1) / kWaves;
     blocks = blocks > 4096 ? 4096 : blocks;
     if (!blocks) return hipSuccess;
-    hipLaunchKernelGGL((k_net_mfma<P, NL, NOUT, EPI, ACT>), dim3(blocks), dim3(kBlockT), smem, s, a, net, n,
+    hipLaunchKernelGGL((k_net_mfma<P, NL, NOUT, EPI, ACT, W>), dim3(blocks), dim3(kBlockT), smem, s, a, net, n,
                        reinterpret_cast<const uint32_t*>(table), reinterpret_cast<const u32x4*>(Wp), ep);
     return hipGetLastError();
 }
@@ -303,14 +304,14 @@ template <typename P>
 hipError_t dispatch(const EncodeArgs& a, int net, int n_layers, int n_out, int epi, uint32_t n,
                     const uint16_t* table, const uint16_t* Wp, const EpiArgs& ep, hipStream_t s) {
     if (a.net_set == NRRS_NETS_SHIPPED) {
-        // the shipped nets: 3 hidden LeakyReLU layers, StatNet -> 6, RRSNet -> 1
+        // the shipped nets: 3 hidden 32-wide LeakyReLU layers, StatNet -> 6, RRSNet -> 1
         if (n_layers == 4 && n_out == 6) {
-            if (epi == EPI_RAW) return launch_one<P, 4, 6, EPI_RAW, ACT_LEAKY>(a, net, n, table, Wp, ep, s);
-            if (epi == EPI_ADRRS) return launch_one<P, 4, 6, EPI_ADRRS, ACT_LEAKY>(a, net, n, table, Wp, ep, s);
+            if (epi == EPI_RAW) return launch_one<P, 4, 6, EPI_RAW, ACT_LEAKY, 32>(a, net, n, table, Wp, ep, s);
+            if (epi == EPI_ADRRS) return launch_one<P, 4, 6, EPI_ADRRS, ACT_LEAKY, 32>(a, net, n, table, Wp, ep, s);
         }
         if (n_layers == 4 && n_out == 1) {
-            if (epi == EPI_RAW) return launch_one<P, 4, 1, EPI_RAW, ACT_LEAKY>(a, net, n, table, Wp, ep, s);
-            if (epi == EPI_RRS) return launch_one<P, 4, 1, EPI_RRS, ACT_LEAKY>(a, net, n, table, Wp, ep, s);
+            if (epi == EPI_RAW) return launch_one<P, 4, 1, EPI_RAW, ACT_LEAKY, 32>(a, net, n, table, Wp, ep, s);
+            if (epi == EPI_RRS) return launch_one<P, 4, 1, EPI_RRS, ACT_LEAKY, 32>(a, net, n, table, Wp, ep, s);
         }
         return hipErrorInvalidValue;
     }

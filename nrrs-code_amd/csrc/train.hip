@@ -27,6 +27,9 @@
 namespace nrrs {
 namespace {
 
+using mfma::kHidChunks;
+using mfma::kOutTiles;
+
 typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
 typedef _Float16 f16x8 __attribute__((ext_vector_type(8)));
 typedef float f32x8 __attribute__((ext_vector_type(8)));
@@ -249,30 +252,33 @@ __device__ __forceinline__ void stage_put(typename P::elem* s, int len, int row,
 }
 
 // Forward of hidden layer `layer` (1..NL-2) or layer 0 from operands X (5 chunks
-// for layer 0, 4 otherwise) into fp32 accumulators D[2].
-template <typename P, int NL>
+// for layer 0, 4 otherwise) into fp32 accumulators D[2] (D[0] only for hidden width W = 32,
+// which takes of layer 0 the K chunks set in l0).
+template <typename P, int NL, int W = 64>
 __device__ __forceinline__ void fwd_layer(const typename P::op* __restrict__ Wf, int layer, const typename P::op* X,
-                                          f32x16* D, int lane) {
+                                          f32x16* D, int lane, uint32_t l0 = 0x1fu) {
 #pragma unroll
-    for (int ot = 0; ot < 2; ++ot) {
+    for (int ot = 0; ot < kOutTiles<W>; ++ot) {
         D[ot] = f32x16{};
         if (layer == 0) {
 #pragma unroll
-            for (int kb = 0; kb < 5; ++kb) D[ot] = P::mma16(load_op<P>(Wf, ot * 5 + kb, lane), X[kb], D[ot]);
+            for (int kb = 0; kb < 5; ++kb)
+                if (W == 64 || ((l0 >> kb) & 1u)) D[ot] = P::mma16(load_op<P>(Wf, ot * 5 + kb, lane), X[kb], D[ot]);
         } else {
             const int base = 10 + (layer - 1) * 8;
 #pragma unroll
-            for (int kb = 0; kb < 4; ++kb) D[ot] = P::mma16(load_op<P>(Wf, base + ot * 4 + kb, lane), X[kb], D[ot]);
+            for (int kb = 0; kb < kHidChunks<W>; ++kb)
+                D[ot] = P::mma16(load_op<P>(Wf, base + ot * 4 + kb, lane), X[kb], D[ot]);
         }
     }
 }
 
 // Activation of accumulators into operands. ReLU: its outputs double as the mask [h > 0] of
 // the backward (P::mask_by). LeakyReLU (slope kLeakySlope): x or slope * x, rounded once.
-template <typename P, int ACT>
+template <typename P, int ACT, int W = 64>
 __device__ __forceinline__ void relu_ops(const f32x16* D, typename P::op* H) {
 #pragma unroll
-    for (int ot = 0; ot < 2; ++ot) {
+    for (int ot = 0; ot < kOutTiles<W>; ++ot) {
 #pragma unroll
         for (int g = 0; g < 2; ++g) {
             float x[8];
@@ -304,16 +310,16 @@ __device__ __forceinline__ typename P::op act_mask(float* x8, const typename P::
 
 // Input of layer `upto` (> from): forward through layers from..upto-1, starting from X, the
 // input of layer `from` (the encoding when from = 0).
-template <typename P, int NL, int ACT>
+template <typename P, int NL, int ACT, int W = 64>
 __device__ __forceinline__ void forward_to(const typename P::op* __restrict__ Wf, int from, const typename P::op* X,
-                                           int upto, typename P::op* H, int lane) {
+                                           int upto, typename P::op* H, int lane, uint32_t l0 = 0x1fu) {
     f32x16 D[2];
-    fwd_layer<P, NL>(Wf, from, X, D, lane);
+    fwd_layer<P, NL, W>(Wf, from, X, D, lane, l0);
     for (int layer = from + 1; layer < upto; ++layer) {
-        relu_ops<P, ACT>(D, H);
-        fwd_layer<P, NL>(Wf, layer, H, D, lane);
+        relu_ops<P, ACT, W>(D, H);
+        fwd_layer<P, NL, W>(Wf, layer, H, D, lane, l0);
     }
-    relu_ops<P, ACT>(D, H);
+    relu_ops<P, ACT, W>(D, H);
 }
 
 // The input of this layer is kept from the forward pass (a checkpoint) for the backward sweep:
@@ -325,9 +331,12 @@ constexpr int kCkpt = NL >= 4 ? 2 : 0;
 
 // BIN: dL/denc of the grid goes to t.gdl for the binned scatter (FP16/BF16); otherwise it is
 // added to the table with atomics. Separate instances keep the unused path's registers out.
-template <typename P, int NL, int NOUT, int LOSS, bool BIN, int ACT>
+// W: hidden width (64, or 32 for the shipped nets: only the live tiles are computed).
+template <typename P, int NL, int NOUT, int LOSS, bool BIN, int ACT, int W>
 __global__ __launch_bounds__(64 * P::kWaves) void k_train(TrainArgs t) {
     static_assert(!(BIN && P::kF32Table), "the binned scatter is for FP16/BF16 tables");
+    static_assert(W == 64 || W == 32, "hidden width");
+    const uint32_t l0 = t.a.l0_chunks;  // layer-0 input chunks that hold the net's inputs
     typedef typename P::op op;
     constexpr int kWaves = Geo<P>::kWaves, kThreads = Geo<P>::kThreads, kVert = Geo<P>::kVert;
     constexpr int kStageLd = Geo<P>::kStageLd;
@@ -432,21 +441,21 @@ __global__ __launch_bounds__(64 * P::kWaves) void k_train(TrainArgs t) {
         constexpr int C = kCkpt<NL>;
         op Hc[4];  // input of layer C (the checkpoint)
         if constexpr (C > 0) {
-            forward_to<P, NL, ACT>(Wf, 0, E, C, Hc, lane);
+            forward_to<P, NL, ACT, W>(Wf, 0, E, C, Hc, lane, l0);
             if constexpr (NL - 1 > C) {
-                forward_to<P, NL, ACT>(Wf, C, Hc, NL - 1, H, lane);
+                forward_to<P, NL, ACT, W>(Wf, C, Hc, NL - 1, H, lane, l0);
             } else {
 #pragma unroll
                 for (int kb = 0; kb < 4; ++kb) H[kb] = Hc[kb];
             }
         } else {
-            forward_to<P, NL, ACT>(Wf, 0, E, NL - 1, H, lane);
+            forward_to<P, NL, ACT, W>(Wf, 0, E, NL - 1, H, lane, l0);
         }
         f32x16 Y = f32x16{};
         {
             const int base = 10 + (NL - 2) * 8;
 #pragma unroll
-            for (int kb = 0; kb < 4; ++kb) Y = P::mma16(load_op<P>(Wf, base + kb, lane), H[kb], Y);
+            for (int kb = 0; kb < kHidChunks<W>; ++kb) Y = P::mma16(load_op<P>(Wf, base + kb, lane), H[kb], Y);
         }
         // output c sits in lane half c/4, register c%4
         float dyr[4] = {0.0f, 0.0f, 0.0f, 0.0f};
@@ -577,12 +586,12 @@ __global__ __launch_bounds__(64 * P::kWaves) void k_train(TrainArgs t) {
             // (1) layer input h_l and its ReLU mask
             if (!last && layer > 0) {
                 if (C > 0 && layer > C) {
-                    forward_to<P, NL, ACT>(Wf, C, Hc, layer, H, lane);
+                    forward_to<P, NL, ACT, W>(Wf, C, Hc, layer, H, lane, l0);
                 } else if (C > 0 && layer == C) {
 #pragma unroll
                     for (int kb = 0; kb < 4; ++kb) H[kb] = Hc[kb];
                 } else {
-                    forward_to<P, NL, ACT>(Wf, 0, E, layer, H, lane);
+                    forward_to<P, NL, ACT, W>(Wf, 0, E, layer, H, lane, l0);
                 }
             }
             // (2) stage delta_l rows [out neuron][vertex] and h_l rows [in neuron][vertex]
@@ -595,7 +604,8 @@ __global__ __launch_bounds__(64 * P::kWaves) void k_train(TrainArgs t) {
                     if (h == 0) stage_put<P>(st.d, kTrRow, vcol, Dl[0], 0, 4);  // outputs 0-7
                 } else {
 #pragma unroll
-                    for (int kb = 0; kb < 4; ++kb) stage_put<P>(st.d, kTrRow, vcol, Dl[kb], 16 * kb + 4 * h, 16 * kb + 8 + 4 * h);
+                    for (int kb = 0; kb < kHidChunks<W>; ++kb)
+                        stage_put<P>(st.d, kTrRow, vcol, Dl[kb], 16 * kb + 4 * h, 16 * kb + 8 + 4 * h);
                 }
                 if (layer == 0) {
 #pragma unroll
@@ -604,7 +614,8 @@ __global__ __launch_bounds__(64 * P::kWaves) void k_train(TrainArgs t) {
                     stage_put<P>(st.h2, kTrRow2, vcol, E[4], 8 * h, 8 * h + 4);
                 } else {
 #pragma unroll
-                    for (int kb = 0; kb < 4; ++kb) stage_put<P>(st.h, kTrRow, vcol, H[kb], 16 * kb + 4 * h, 16 * kb + 8 + 4 * h);
+                    for (int kb = 0; kb < kHidChunks<W>; ++kb)
+                        stage_put<P>(st.h, kTrRow, vcol, H[kb], 16 * kb + 4 * h, 16 * kb + 8 + 4 * h);
                 }
             } else {
                 if (last) {
@@ -613,7 +624,7 @@ __global__ __launch_bounds__(64 * P::kWaves) void k_train(TrainArgs t) {
                         if (h == 0 && j < NOUT) st.d[j * kStageLd + vcol] = (typename P::elem)P::get(Dl[0], j);
                 } else {
 #pragma unroll
-                    for (int kb = 0; kb < 4; ++kb)
+                    for (int kb = 0; kb < kHidChunks<W>; ++kb)
 #pragma unroll
                         for (int j = 0; j < 8; ++j)
                             st.d[(16 * kb + 8 * (j >> 2) + 4 * h + (j & 3)) * kStageLd + vcol] =
@@ -627,7 +638,7 @@ __global__ __launch_bounds__(64 * P::kWaves) void k_train(TrainArgs t) {
                             st.h[(kb * 16 + 8 * h + j) * kStageLd + vcol] = (typename P::elem)P::get(E[kb], j);
                 } else {
 #pragma unroll
-                    for (int kb = 0; kb < 4; ++kb)
+                    for (int kb = 0; kb < kHidChunks<W>; ++kb)
 #pragma unroll
                         for (int j = 0; j < 8; ++j)
                             st.h[(16 * kb + 8 * (j >> 2) + 4 * h + (j & 3)) * kStageLd + vcol] =
@@ -642,6 +653,9 @@ __global__ __launch_bounds__(64 * P::kWaves) void k_train(TrainArgs t) {
                 for (int g = dw_first(base, wave, kWaves); g < base + n_tiles; g += kWaves) {
                     const int tt = g - base, slot = g / kWaves;
                     const int to = tt / n_ti, ti = tt % n_ti;
+                    // W = 32: the tiles outside the net (and of layer 0 those of input chunks
+                    // without inputs) are zero and stay so
+                    if (W == 32 && !(to == 0 && (layer == 0 ? ((l0 >> (2 * ti)) & 3u) != 0 : ti == 0))) continue;
                     // offsets of the transposing loads (FP16/BF16 stages), recomputed per tile
                     // (cheaper than keeping four registers through the whole kernel)
                     int ln = lane;
@@ -676,20 +690,20 @@ __global__ __launch_bounds__(64 * P::kWaves) void k_train(TrainArgs t) {
             if (layer > 0) {
                 f32x16 D[2];
 #pragma unroll
-                for (int ot = 0; ot < 2; ++ot) {
+                for (int ot = 0; ot < kOutTiles<W>; ++ot) {
                     D[ot] = f32x16{};
                     if (last) {
                         D[ot] = P::mma16(load_op<P>(Wb, t.bwd_last_base + ot, lane), Dl[0], D[ot]);
                     } else {
                         const int base = t.bwd_hidden_base + (layer - 1) * 8;
 #pragma unroll
-                        for (int kb = 0; kb < 4; ++kb)
+                        for (int kb = 0; kb < kHidChunks<W>; ++kb)
                             D[ot] = P::mma16(load_op<P>(Wb, base + ot * 4 + kb, lane), Dl[kb], D[ot]);
                     }
                 }
                 // delta_{l-1} = D * [h_l > 0]
 #pragma unroll
-                for (int ot = 0; ot < 2; ++ot)
+                for (int ot = 0; ot < kOutTiles<W>; ++ot)
 #pragma unroll
                     for (int g = 0; g < 2; ++g) {
                         float x8[8];
@@ -701,7 +715,7 @@ __global__ __launch_bounds__(64 * P::kWaves) void k_train(TrainArgs t) {
                 // dL/denc for the 32 grid channels: rows c = 8(r/4) + 4h + r%4
                 f32x16 G = f32x16{};
 #pragma unroll
-                for (int kb = 0; kb < 4; ++kb) G = P::mma16(load_op<P>(Wb, kb, lane), Dl[kb], G);
+                for (int kb = 0; kb < kHidChunks<W>; ++kb) G = P::mma16(load_op<P>(Wb, kb, lane), Dl[kb], G);
                 if constexpr (BIN) {
                     // Binned scatter (FP16/BF16): the level pairs of dL/denc go to HBM as
                     // [level][vertex] FP16 pairs (the precision of the records they become) and
@@ -910,7 +924,7 @@ __global__ void k_after_adam(AdamArgs a, const float* __restrict__ W, const floa
     pack_train_tile<E>(W, nl, n_out, fwd, bwd, n_fwd, n_bwd, b, threadIdx.x);
 }
 
-template <typename P, int NL, int NOUT, int LOSS, bool BIN, int ACT>
+template <typename P, int NL, int NOUT, int LOSS, bool BIN, int ACT, int W>
 hipError_t launch_train_one_bin(const TrainArgs& t, uint32_t blocks, hipStream_t s) {
     constexpr size_t smem =
         sizeof(Stage<P>) + (P::kWaves == 8 ? (size_t)(10 + (NL - 2) * 8 + 4 + 4 + (NL - 2) * 8 + 2) * 64 *
@@ -918,20 +932,20 @@ hipError_t launch_train_one_bin(const TrainArgs& t, uint32_t blocks, hipStream_t
                                            : 0);
     static std::atomic<bool> attr[kMaxDevices];
     hipError_t e =
-        set_dyn_lds_once(attr, reinterpret_cast<const void*>(&k_train<P, NL, NOUT, LOSS, BIN, ACT>), smem);
+        set_dyn_lds_once(attr, reinterpret_cast<const void*>(&k_train<P, NL, NOUT, LOSS, BIN, ACT, W>), smem);
     if (e != hipSuccess) return e;
-    hipLaunchKernelGGL((k_train<P, NL, NOUT, LOSS, BIN, ACT>), dim3(blocks), dim3(Geo<P>::kThreads), smem, s, t);
+    hipLaunchKernelGGL((k_train<P, NL, NOUT, LOSS, BIN, ACT, W>), dim3(blocks), dim3(Geo<P>::kThreads), smem, s, t);
     return hipGetLastError();
 }
 
-template <typename P, int NL, int NOUT, int LOSS, int ACT = ACT_RELU>
+template <typename P, int NL, int NOUT, int LOSS, int ACT = ACT_RELU, int W = 64>
 hipError_t launch_train_one(const TrainArgs& t, uint32_t blocks, hipStream_t s) {
     if constexpr (P::kF32Table) {
         if (t.gdl) return hipErrorInvalidValue;
-        return launch_train_one_bin<P, NL, NOUT, LOSS, false, ACT>(t, blocks, s);
+        return launch_train_one_bin<P, NL, NOUT, LOSS, false, ACT, W>(t, blocks, s);
     } else {
-        return t.gdl ? launch_train_one_bin<P, NL, NOUT, LOSS, true, ACT>(t, blocks, s)
-                     : launch_train_one_bin<P, NL, NOUT, LOSS, false, ACT>(t, blocks, s);
+        return t.gdl ? launch_train_one_bin<P, NL, NOUT, LOSS, true, ACT, W>(t, blocks, s)
+                     : launch_train_one_bin<P, NL, NOUT, LOSS, false, ACT, W>(t, blocks, s);
     }
 }
 
@@ -946,16 +960,16 @@ namespace {
 template <typename P>
 hipError_t dispatch_train(const TrainArgs& t, int nl, int n_out, int loss, uint32_t blocks, hipStream_t s) {
     if (t.a.net_set == NRRS_NETS_SHIPPED) {
-        // the shipped nets: 3 hidden LeakyReLU layers, StatNet -> 6, RRSNet -> 1
+        // the shipped nets: 3 hidden 32-wide LeakyReLU layers, StatNet -> 6, RRSNet -> 1
         if (nl == 4 && n_out == 6) {
             if (loss == NRRS_LOSS_EXTERNAL)
-                return launch_train_one<P, 4, 6, NRRS_LOSS_EXTERNAL, ACT_LEAKY>(t, blocks, s);
-            if (loss == NRRS_LOSS_LL2) return launch_train_one<P, 4, 6, NRRS_LOSS_LL2, ACT_LEAKY>(t, blocks, s);
+                return launch_train_one<P, 4, 6, NRRS_LOSS_EXTERNAL, ACT_LEAKY, 32>(t, blocks, s);
+            if (loss == NRRS_LOSS_LL2) return launch_train_one<P, 4, 6, NRRS_LOSS_LL2, ACT_LEAKY, 32>(t, blocks, s);
         }
         if (nl == 4 && n_out == 1) {
             if (loss == NRRS_LOSS_EXTERNAL)
-                return launch_train_one<P, 4, 1, NRRS_LOSS_EXTERNAL, ACT_LEAKY>(t, blocks, s);
-            if (loss == NRRS_LOSS_RRS) return launch_train_one<P, 4, 1, NRRS_LOSS_RRS, ACT_LEAKY>(t, blocks, s);
+                return launch_train_one<P, 4, 1, NRRS_LOSS_EXTERNAL, ACT_LEAKY, 32>(t, blocks, s);
+            if (loss == NRRS_LOSS_RRS) return launch_train_one<P, 4, 1, NRRS_LOSS_RRS, ACT_LEAKY, 32>(t, blocks, s);
         }
         return hipErrorInvalidValue;
     }
