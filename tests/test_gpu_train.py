@@ -98,6 +98,87 @@ def test_backward_external_matches_fp64(torch_cuda, ora, prec, net):
         assert n_grid < GRID_NORM_TOL[prec]
 
 
+# Entry-wise table-gradient error of FP16/BF16 when no ReLU mask can flip, relative to the
+# largest entry (ENTRY_TOL) and per entry among the entries above a twentieth of the largest
+# (BIG_ENTRY_TOL). A term of an entry's gradient carries the rounding of 16-bit operands in each
+# layer of the backward (2^-9 relative for BF16, 2^-12 for FP16, four layers deep) and of the
+# FP16 record value.
+ENTRY_TOL = {na.FP16: 4e-3, na.BF16: 3e-2}
+BIG_ENTRY_TOL = {na.FP16: 8e-3, na.BF16: 6e-2}
+MARGIN = 0.05  # smallest hidden pre-activation the weights below leave
+
+
+def _flip_free_weights(rng, n_hidden, n_out, const):
+    """MLP weights (the flat MLP part of a benchmark net's parameters) whose hidden
+    pre-activations stay positive: layer 0 puts 2.0 on the constant input channel and at most
+    0.01 on the others (80 inputs of magnitude below a few units move it by well under 1.0);
+    hidden layers have positive weights in [0.005, 0.03] on positive inputs; the output layer is
+    arbitrary."""
+    k, h = na.ENC_WIDTH, 64
+    w0 = rng.uniform(-0.01, 0.01, (h, k))
+    w0[:, const] = 2.0
+    mats = [w0] + [rng.uniform(0.005, 0.03, (h, h)) for _ in range(n_hidden - 1)]
+    mats.append(rng.uniform(-0.3, 0.3, (n_out, h)))
+    return mats
+
+
+@pytest.mark.parametrize("prec", [na.FP16, na.BF16])
+@pytest.mark.parametrize("net", [na.STATNET, na.RRSNET])
+def test_table_gradient_entrywise_without_mask_flips(torch_cuda, ora, prec, net):
+    """Entry-wise table gradient of the FP16/BF16 backward against the FP64 oracle. The weights
+    keep every hidden pre-activation of the batch at least MARGIN above zero (checked on the
+    FP64 forward, whose layer layout is checked against the oracle's outputs), so the
+    low-precision forward cannot differ from the oracle's in any ReLU mask and each entry's
+    error is the rounding of the backward alone, bounded entry by entry."""
+    torch = torch_cuda
+    ctx, vb, film_np, film = _setup(torch, prec)
+    mcfg = ctx.cfg.statnet if net == na.STATNET else ctx.cfg.rrsnet
+    n_out = int(mcfg.n_outputs)
+    mats = _flip_free_weights(np.random.default_rng(21), int(mcfg.n_hidden_layers), n_out,
+                              71 if net == na.STATNET else 75)  # the constant input channel
+    p = ctx.params_get(net)
+    g, m = ctx.param_count(net)
+    flat = np.concatenate([w.ravel() for w in mats]).astype(np.float32)
+    assert flat.size == m
+    p[g:] = flat
+    ctx.params_set(net, p)
+    p64 = ctx.params_get(net).astype(np.float64)
+    rng = np.random.default_rng(3)
+    stats_np = np.abs(rng.normal(0.5, 0.5, (6, N))).astype(np.float32) if net == na.RRSNET else None
+    stats64 = None if stats_np is None else stats_np.astype(np.float64)
+    stats = None if stats_np is None else torch.from_numpy(stats_np).cuda()
+    hv = ora.HostVertices(vb.to_numpy())
+    # FP64 forward with the hidden pre-activations, from the oracle's network input
+    x = ora.encode(ctx.cfg, net, hv, film_np, stats64, p64)
+    w64 = [w.astype(np.float32).astype(np.float64) for w in mats]
+    z_min = np.inf
+    for w in w64[:-1]:
+        z = x @ w.T
+        z_min = min(z_min, float(z.min()))
+        x = np.maximum(z, 0.0)
+    y_np = x @ w64[-1].T
+    y64 = ora.net_forward(ctx.cfg, net, hv, film_np, stats64, p64)
+    assert np.max(np.abs(y_np - y64)) <= 1e-9 * max(1.0, np.max(np.abs(y64)))
+    assert z_min >= MARGIN, z_min
+    dy = (1e-3 * y64.T + rng.normal(0, 1e-5, (n_out, N))).astype(np.float32)
+    ctx.grads_zero(net)
+    na.check(ctx.backward(net, vb, na.LOSS_EXTERNAL, film=film, stats=stats, dy=torch.from_numpy(dy).cuda(),
+                          loss_scale=SCALE[prec]), "backward")
+    g_dev = ctx.grads_get(net)[:g] / SCALE[prec]
+    _, g_ref, _ = ora.train_grads(ctx.cfg, net, hv, p64, 0, film=film_np, stats=stats64,
+                                  dy_ext=dy.astype(np.float64))
+    g_ref = g_ref[:g]
+    top = np.max(np.abs(g_ref))
+    err = np.abs(g_dev - g_ref)
+    big = np.abs(g_ref) >= top / 20
+    e_max = float(err.max() / top)
+    e_big = float(np.max(err[big] / np.abs(g_ref[big])))
+    print(f"prec {prec} net {net}: min pre-activation {z_min:.3f}; table gradient error {e_max:.2e} of the "
+          f"largest entry, {e_big:.2e} per entry over {int(big.sum())} entries above 1/20 of it")
+    assert e_max < ENTRY_TOL[prec]
+    assert e_big < BIG_ENTRY_TOL[prec]
+
+
 @pytest.mark.parametrize("case", ["table_2p20", "z_plane"])
 def test_low_precision_table_gradient_fallbacks(torch_cuda, ora, case):
     """BF16 StatNet backward through the two table-gradient paths that the default
