@@ -91,6 +91,8 @@ def parse():
                     help="only run a 256 MiB device copy and, after 8 warm-up steps, 3 BF16 steps at 2^20 "
                          "(for rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE runs; the profile summaries use the "
                          "dispatches of the last 3 steps)")
+    ap.add_argument("--probe-shipped", action="store_true",
+                    help="with --pmc-probe: the steps use the reference's shipped nets (nrrs_shipped_config)")
     return ap.parse_args()
 
 
@@ -555,7 +557,7 @@ def main():
         src = torch.ones(64 << 20, device="cuda")  # 256 MiB
         dst = torch.empty_like(src)
         dst.copy_(src)
-        path = Path(na.BF16, torch, na, np)
+        path = Path(na.BF16, torch, na, np, shipped=args.probe_shipped)
         for k in range(8 + 3):  # warm-up (clocks, caches), then 3 measured steps
             path.infer(k)
             path.train()
