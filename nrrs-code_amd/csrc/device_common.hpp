@@ -227,27 +227,25 @@ __device__ __forceinline__ void lat_long(const float* d, float* uv) {
 // OneBlob encoding of x in [0,1] over `bins` bins (tiny-cuda-nn's "OneBlob", 2net-simple.json:
 // 53-66): bin k holds the mass that a biweight kernel K(u) = 15/16 (1 - u^2)^2 of radius one bin
 // width, centred at x and wrapped periodically onto [0,1), puts on [k / bins, (k+1) / bins).
-// With C the kernel's CDF, C(u) = 1/2 + 15/16 u (1 - 2u^2/3 + u^4/5) on [-1,1],
-// bin k = sum over s in {-1,0,1} of C(bins ((k+1)/bins - x - s)) - C(bins (k/bins - x - s)).
-// The bins sum to 1.
+// With C the kernel's CDF, C(u) = 1/2 + 15/16 u (1 - 2u^2/3 + u^4/5) on [-1,1] (0 below, 1
+// above), bin k = sum over s in {-1,0,1} of C(k + 1 - t - s bins) - C(k - t - s bins), t = x bins.
+// The kernel covers [t - 1, t + 1], so with m = floor(t) and f = t - m only three bins (mod bins)
+// receive mass: m - 1 takes C(-f), m takes C(1 - f) - C(-f), m + 1 takes 1 - C(1 - f) (two
+// evaluations of C instead of 3 (bins + 1)). The bins sum to 1.
 __device__ __forceinline__ float quartic_cdf(float u) {
     u = fminf(fmaxf(u, -1.0f), 1.0f);
     const float u2 = u * u;
     return 0.5f + 0.9375f * u * (1.0f - u2 * (2.0f / 3.0f) + u2 * u2 * 0.2f);
 }
-__device__ __forceinline__ float one_blob_cdf(float b, float x, float bins) {
-    return quartic_cdf((b - x) * bins) + quartic_cdf((b - x - 1.0f) * bins) + quartic_cdf((b - x + 1.0f) * bins);
-}
 template <int BINS>
 __device__ __forceinline__ void one_blob(float x, float* o) {
-    const float fb = (float)BINS;
-    float left = one_blob_cdf(0.0f, x, fb);
+    static_assert(BINS >= 4 && (BINS & (BINS - 1)) == 0, "three distinct bins, index wrap by mask");
+    const float t = x * (float)BINS, fm = floorf(t), f = t - fm;
+    const int m = (int)fm & (BINS - 1), ml = (m + BINS - 1) & (BINS - 1), mr = (m + 1) & (BINS - 1);
+    const float c0 = quartic_cdf(-f), c1 = quartic_cdf(1.0f - f);
+    const float w_l = c0, w_m = c1 - c0, w_r = 1.0f - c1;
 #pragma unroll
-    for (int k = 0; k < BINS; ++k) {
-        const float right = one_blob_cdf((float)(k + 1) / fb, x, fb);
-        o[k] = right - left;
-        left = right;
-    }
+    for (int k = 0; k < BINS; ++k) o[k] = k == ml ? w_l : (k == m ? w_m : (k == mr ? w_r : 0.0f));
 }
 
 // ---------------------------------------------------------------- SH degree 4
