@@ -415,6 +415,8 @@ nrrs_status ensure_train(nrrs_ctx* c, int net) {
     NRRS_TRY(dmalloc(reinterpret_cast<uint8_t**>(&ns.w_fwd), (size_t)train_tiles_fwd(ns.n_layers) * 64 * ob));
     NRRS_TRY(dmalloc(reinterpret_cast<uint8_t**>(&ns.w_bwd), (size_t)train_tiles_bwd(ns.n_layers) * 64 * ob));
     NRRS_TRY(dmalloc(&ns.dw_partial, (size_t)c->max_train_blocks * ns.n_mlp));
+    // the narrow kernels write only the tiles of their 32-wide nets
+    NRRS_HIP(hipMemset(ns.dw_partial, 0, (size_t)c->max_train_blocks * ns.n_mlp * sizeof(float)));
     {
         // dense levels come first: their entries end where the first hashed level starts
         const GridParams& gp = ns.grid;
@@ -655,7 +657,7 @@ nrrs_status run_backward(nrrs_ctx* c, int net, uint32_t n, const nrrs_vertices& 
     t.dw_partial = ns.dw_partial;
     t.n_mlp = (uint32_t)ns.n_mlp;
     t.loss_partial = kind == NRRS_LOSS_EXTERNAL ? nullptr : ns.loss_partial;
-    const uint32_t vpi = train_vertices_per_iter(cfg.precision);
+    const uint32_t vpi = train_vertices_per_iter(cfg.precision, cfg.net_set);
     uint32_t iters = (n + vpi - 1) / vpi;
     uint32_t blocks = iters < c->max_train_blocks ? iters : c->max_train_blocks;
     if (!blocks) return NRRS_OK;
@@ -1047,8 +1049,10 @@ nrrs_status nrrs_create(const nrrs_config* cfg, nrrs_ctx** out) {
     c->cfg = *cfg;
     c->device = dev;
     // the training kernel keeps one workgroup per CU resident (its LDS stage and weight tiles):
-    // one persistent workgroup per CU, so the dW partials are one per CU as well
+    // one persistent workgroup per CU, so the dW partials are one per CU as well; the narrow
+    // kernels of the shipped FP16/BF16 nets keep two
     if (prop.multiProcessorCount > 0) c->max_train_blocks = (uint32_t)std::min(prop.multiProcessorCount, 1024);
+    if (cfg->net_set == NRRS_NETS_SHIPPED && cfg->precision != NRRS_FP32) c->max_train_blocks *= 2;
     uint64_t n_grid = 0;
     if (!make_grid(cfg->grid, &c->grid, nullptr, nullptr, nullptr, nullptr, nullptr, &n_grid)) {
         delete c;

@@ -645,8 +645,11 @@ __device__ __forceinline__ void encode_tail_rec(const VertexRec& v, int net, flo
     }
 }
 
-// Non-grid channels of vertex i of a SoA batch.
+// Non-grid channels of vertex i of a SoA batch. SET: the network set when the caller knows it at
+// compile time (the loads and channels of the other set are then dropped), else -1 (a.net_set).
+template <int SET = -1>
 __device__ __forceinline__ void encode_tail(const EncodeArgs& a, int net, uint32_t i, float* e) {
+    const int set = SET >= 0 ? SET : a.net_set;
     VertexRec v = vertex_from_soa(a, i);
     const float ref = net == NRRS_RRSNET ? film_mean(a.film, a.n_pixels, v.pixel) : 0.0f;
     // two calls rather than a selected pointer, which would put st in memory
@@ -654,9 +657,9 @@ __device__ __forceinline__ void encode_tail(const EncodeArgs& a, int net, uint32
         float st[6];
 #pragma unroll
         for (int k = 0; k < 6; ++k) st[k] = a.stats[(size_t)k * a.stats_stride + i];
-        encode_tail_rec(v, net, ref, st, e, a.sigma_l2 != 0, false, a.net_set);
+        encode_tail_rec(v, net, ref, st, e, a.sigma_l2 != 0, false, set);
     } else {
-        encode_tail_rec(v, net, ref, nullptr, e, false, a.aid != 0, a.net_set);
+        encode_tail_rec(v, net, ref, nullptr, e, false, a.aid != 0, set);
     }
 }
 

@@ -148,9 +148,19 @@ constexpr uint32_t kDenseReplicas = kDenseGradReplicas;
 constexpr int kStageRowsD = 64;        // delta rows
 constexpr int kStageRowsH = 96;        // input rows (80 padded to 3 x 32)
 
+// FP16/BF16 operands stage vertex-major (below); the kernels of 32-wide nets on them are the
+// narrow kernels.
 template <typename P>
+constexpr bool kTrStage = sizeof(typename P::elem) == 2;
+template <typename P, int W>
+constexpr bool kNarrow = W == 32 && kTrStage<P>;
+
+// Workgroup shape. The narrow kernels run workgroups of 4 waves, two per CU: each needs 35 KB of
+// LDS (their weight tiles are read from global memory, where they stay in L1), and the two
+// workgroups' independent barriers let one hide the latency of the other's global loads.
+template <typename P, int W = 64>
 struct Geo {
-    static constexpr int kWaves = P::kWaves;
+    static constexpr int kWaves = kNarrow<P, W> ? 4 : P::kWaves;
     static constexpr int kThreads = 64 * kWaves;
     static constexpr int kVert = 32 * kWaves;   // vertices per workgroup iteration
     static constexpr int kStageLd = kVert + 8;  // LDS row stride (elements), breaks bank aliasing
@@ -167,8 +177,6 @@ struct Geo {
 // that one group of 16 lanes reads with a transposing load (stage_tr) are taken 8 vertices apart:
 // k-step s of the dW contraction covers vertices 32 (s / 2) + 4 (s % 2) + 8j + 2h + r (lane half
 // h, load r, element j).
-template <typename P>
-constexpr bool kTrStage = sizeof(typename P::elem) == 2;
 constexpr int kTrRow = 68, kTrRow2 = 36;  // elements per row of the main and the second stage
 __host__ __device__ constexpr int tr_kstep_row(int s) { return 32 * (s >> 1) + 4 * (s & 1); }
 
@@ -183,6 +191,22 @@ struct Stage<P, true> {
     typename P::elem h[Geo<P>::kVert * kTrRow];
     typename P::elem h2[Geo<P>::kVert * kTrRow2];
 };
+// Narrow kernels: the delta stage holds 32 neurons in rows of kTrRow2 elements.
+template <typename P>
+struct Stage32 {
+    typename P::elem d[Geo<P, 32>::kVert * kTrRow2];
+    typename P::elem h[Geo<P, 32>::kVert * kTrRow];
+    typename P::elem h2[Geo<P, 32>::kVert * kTrRow2];
+};
+template <typename P, int W>
+using StageOf = std::conditional_t<kNarrow<P, W>, Stage32<P>, Stage<P>>;
+// weight tiles in LDS: the 8-wave FP16/BF16 kernels of the 64-wide nets
+template <typename P, int W>
+constexpr bool kTilesInLds = P::kWaves == 8 && W == 64;
+// dW tiles of the narrow kernels: tile u = ti for input tile ti of layer 0 (0-2), u = l + 2 for
+// the one tile of layer l > 0; tile u on wave u % waves, in register slot u / waves.
+template <int NL>
+constexpr int kNarrowTiles = NL + 2;
 
 // Number of 32x32 dW tiles of layer l (out rows x in cols).
 __host__ __device__ constexpr int dw_tiles(int l, int nl) {
@@ -331,17 +355,24 @@ constexpr int kCkpt = NL >= 4 ? 2 : 0;
 
 // BIN: dL/denc of the grid goes to t.gdl for the binned scatter (FP16/BF16); otherwise it is
 // added to the table with atomics. Separate instances keep the unused path's registers out.
-// W: hidden width (64, or 32 for the shipped nets: only the live tiles are computed).
+// W: hidden width (64, or 32 for the shipped nets: only the live tiles are computed; their
+// encoding is that of the shipped network set).
 template <typename P, int NL, int NOUT, int LOSS, bool BIN, int ACT, int W>
-__global__ __launch_bounds__(64 * P::kWaves) void k_train(TrainArgs t) {
+__global__ __launch_bounds__((64 * Geo<P, W>::kWaves)) __attribute__((amdgpu_waves_per_eu(kNarrow<P, W> ? 2 : 1)))
+void k_train(TrainArgs t) {
     static_assert(!(BIN && P::kF32Table), "the binned scatter is for FP16/BF16 tables");
     static_assert(W == 64 || W == 32, "hidden width");
     const uint32_t l0 = t.a.l0_chunks;  // layer-0 input chunks that hold the net's inputs
     typedef typename P::op op;
-    constexpr int kWaves = Geo<P>::kWaves, kThreads = Geo<P>::kThreads, kVert = Geo<P>::kVert;
+    constexpr int kWaves = Geo<P, W>::kWaves, kThreads = Geo<P, W>::kThreads, kVert = Geo<P, W>::kVert;
     constexpr int kStageLd = Geo<P>::kStageLd;
+    constexpr bool kN = kNarrow<P, W>;
+    constexpr int kDRow = kN ? kTrRow2 : kTrRow;  // elements per row of the FP16/BF16 delta stage
+    constexpr int kNet = NOUT == 1 ? NRRS_RRSNET : NRRS_STATNET;
+    constexpr int kSet = W == 32 ? NRRS_NETS_SHIPPED : NRRS_NETS_BENCH;
     extern __shared__ __align__(16) unsigned char smem_raw[];
-    Stage<P>& st = *reinterpret_cast<Stage<P>*>(smem_raw);
+    typedef StageOf<P, W> St;
+    St& st = *reinterpret_cast<St*>(smem_raw);
     __shared__ double loss_part[kWaves];  // loss per wave (FP64, summed in a fixed order)
 
     // the wave index in a scalar register: tile ownership and slot choices below are then
@@ -352,18 +383,18 @@ __global__ __launch_bounds__(64 * P::kWaves) void k_train(TrainArgs t) {
     // FP16/BF16: the packed forward and W^T tiles live in LDS behind the stage (they are read
     // by every layer of every iteration); FP32: from global memory (cache)
     constexpr int kTilesF = 10 + (NL - 2) * 8 + 4, kTilesB = 4 + (NL - 2) * 8 + 2;
-    op* sW = reinterpret_cast<op*>(smem_raw + sizeof(Stage<P>));
-    if constexpr (kWaves == 8) {
+    op* sW = reinterpret_cast<op*>(smem_raw + sizeof(St));
+    if constexpr (kTilesInLds<P, W>) {
         const op* gF = reinterpret_cast<const op*>(t.w_fwd);
         const op* gB = reinterpret_cast<const op*>(t.w_bwd);
         for (int k = threadIdx.x; k < (kTilesF + kTilesB) * 64; k += kThreads)
             sW[k] = k < kTilesF * 64 ? gF[k] : gB[k - kTilesF * 64];
     }
-    const op* Wf0 = kWaves == 8 ? sW : reinterpret_cast<const op*>(t.w_fwd);
-    const op* Wb0 = kWaves == 8 ? sW + kTilesF * 64 : reinterpret_cast<const op*>(t.w_bwd);
+    const op* Wf0 = kTilesInLds<P, W> ? sW : reinterpret_cast<const op*>(t.w_fwd);
+    const op* Wb0 = kTilesInLds<P, W> ? sW + kTilesF * 64 : reinterpret_cast<const op*>(t.w_bwd);
     const float S = *t.loss_scale;
     const float ebar = t.error_sum_dev ? fmaxf(*t.error_sum_dev / t.n_pixels_f, 1e-4f) : t.ebar;
-    constexpr int kSlots = dw_slots(NL, kWaves);
+    constexpr int kSlots = kN ? (kNarrowTiles<NL> + kWaves - 1) / kWaves : dw_slots(NL, kWaves);
     f32x16 acc[kSlots];
 #pragma unroll
     for (int s = 0; s < kSlots; ++s) acc[s] = f32x16{};
@@ -421,7 +452,7 @@ __global__ __launch_bounds__(64 * P::kWaves) void k_train(TrainArgs t) {
                     }
                 }
                 float e[kEnc];
-                encode_tail(t.a, t.net, i, e);
+                encode_tail<kSet>(t.a, kNet, i, e);
 #pragma unroll
                 for (int kb = 2; kb < 5; ++kb) {
 #pragma unroll
@@ -601,11 +632,11 @@ __global__ __launch_bounds__(64 * P::kWaves) void k_train(TrainArgs t) {
                 // (columns 16kb + 4h and 16kb + 8 + 4h), of the encoding channel 16kb + 8h + j
                 // (columns 16kb + 8h and 16kb + 8h + 4; chunk 4 in the second stage)
                 if (last) {
-                    if (h == 0) stage_put<P>(st.d, kTrRow, vcol, Dl[0], 0, 4);  // outputs 0-7
+                    if (h == 0) stage_put<P>(st.d, kDRow, vcol, Dl[0], 0, 4);  // outputs 0-7
                 } else {
 #pragma unroll
                     for (int kb = 0; kb < kHidChunks<W>; ++kb)
-                        stage_put<P>(st.d, kTrRow, vcol, Dl[kb], 16 * kb + 4 * h, 16 * kb + 8 + 4 * h);
+                        stage_put<P>(st.d, kDRow, vcol, Dl[kb], 16 * kb + 4 * h, 16 * kb + 8 + 4 * h);
                 }
                 if (layer == 0) {
 #pragma unroll
@@ -647,7 +678,31 @@ __global__ __launch_bounds__(64 * P::kWaves) void k_train(TrainArgs t) {
             }
             __syncthreads();
             // (3) dW tiles of this layer owned by this wave
-            {
+            if constexpr (kN) {
+                // narrow: this layer's tiles u owned by this wave (layer 0: those of input
+                // chunks that hold inputs)
+                for (int u = layer == 0 ? 0 : layer + 2; u < (layer == 0 ? 3 : layer + 3); ++u) {
+                    const int ti = layer == 0 ? u : 0;
+                    if (u % kWaves != wave || (layer == 0 && ((l0 >> (2 * ti)) & 3u) == 0)) continue;
+                    int ln = lane;
+                    asm volatile("" : "+v"(ln));
+                    const int tro[2] = {tr_lane_offset(ln, 0, kTrRow), tr_lane_offset(ln, 1, kTrRow)};
+                    const int tro2[2] = {tr_lane_offset(ln, 0, kTrRow2), tr_lane_offset(ln, 1, kTrRow2)};
+#pragma unroll
+                    for (int s2 = 0; s2 < kSlots; ++s2) {
+                        if (s2 == u / kWaves) {
+#pragma unroll
+                            for (int s = 0; s < kVert / 16; ++s) {
+                                const int row = tr_kstep_row(s);
+                                const op A = stage_tr<P>(st.d + row * kTrRow2, tro2[0], tro2[1]);
+                                const op B = ti < 2 ? stage_tr<P>(st.h + row * kTrRow + 32 * ti, tro[0], tro[1])
+                                                    : stage_tr<P>(st.h2 + row * kTrRow2, tro2[0], tro2[1]);
+                                acc[s2] = P::mma16(A, B, acc[s2]);
+                            }
+                        }
+                    }
+                }
+            } else {
                 const int n_ti = layer == 0 ? 3 : 2;
                 const int base = dw_tile_base(layer, NL), n_tiles = dw_tiles(layer, NL);
                 for (int g = dw_first(base, wave, kWaves); g < base + n_tiles; g += kWaves) {
@@ -673,7 +728,7 @@ __global__ __launch_bounds__(64 * P::kWaves) void k_train(TrainArgs t) {
                                 op A, B;
                                 if constexpr (kTrStage<P>) {
                                     const int row = tr_kstep_row(s);
-                                    A = stage_tr<P>(st.d + row * kTrRow + 32 * to, tro[0], tro[1]);
+                                    A = stage_tr<P>(st.d + row * kDRow + 32 * to, tro[0], tro[1]);
                                     B = ti < 2 ? stage_tr<P>(st.h + row * kTrRow + 32 * ti, tro[0], tro[1])
                                                : stage_tr<P>(st.h2 + row * kTrRow2, tro2[0], tro2[1]);
                                 } else {
@@ -743,8 +798,10 @@ __global__ __launch_bounds__(64 * P::kWaves) void k_train(TrainArgs t) {
                     //  - FP32 table: lane parity = feature, both features of a corner are one 8-byte
                     //    entry.
                     __syncthreads();
-                    static_assert(sizeof(st.d) >= 32 * kVert * sizeof(float), "grid-gradient stage");
-                    float* gst = reinterpret_cast<float*>(st.d);  // [32][kVert]
+                    // [32][kVert] in the delta stage, or in the input stage of the narrow kernels
+                    static_assert((kN ? sizeof(st.h) : sizeof(st.d)) >= 32 * kVert * sizeof(float),
+                                  "grid-gradient stage");
+                    float* gst = reinterpret_cast<float*>(kN ? (void*)st.h : (void*)st.d);
 #pragma unroll
                     for (int r = 0; r < 16; ++r) gst[(8 * (r >> 2) + 4 * h + (r & 3)) * kVert + vcol] = G[r];
                     // the values a lane reads were written by its own wave
@@ -795,8 +852,28 @@ __global__ __launch_bounds__(64 * P::kWaves) void k_train(TrainArgs t) {
 
     // ---- per-workgroup dW partials (rows < out, cols < in) and the loss
     float* part = t.dw_partial + (size_t)blockIdx.x * t.n_mlp;
+    if constexpr (kN) {
+        // the wave's narrow tiles; partials outside them stay zero (cleared once at allocation)
+        for (int u = wave; u < kNarrowTiles<NL>; u += kWaves) {
+            const int layer = u < 3 ? 0 : u - 2, ti = u < 3 ? u : 0;
+            const int in = layer == 0 ? kEnc : kHidden;
+            const int out = layer == NL - 1 ? NOUT : kHidden;
+            const int woff = layer == 0 ? 0 : kHidden * kEnc + (layer - 1) * kHidden * kHidden;
+            f32x16 a = f32x16{};
 #pragma unroll
-    for (int layer = 0; layer < NL; ++layer) {
+            for (int s2 = 0; s2 < kSlots; ++s2)
+                if (s2 == u / kWaves) a = acc[s2];
+#pragma unroll
+            for (int r = 0; r < 16; ++r) {
+                const int row = 8 * (r >> 2) + 4 * h + (r & 3);
+                const int cc = ti * 32 + col;
+                if (row < out && cc < in)
+                    part[woff + row * in + cc] = head_masked(t.head_first, layer, NL, row, cc) ? 0.0f : a[r];
+            }
+        }
+    }
+#pragma unroll
+    for (int layer = 0; layer < (kN ? 0 : NL); ++layer) {
         const int n_ti = layer == 0 ? 3 : 2;
         const int n_tiles = dw_tiles(layer, NL);
         const int in = layer == 0 ? kEnc : kHidden;
@@ -927,14 +1004,15 @@ __global__ void k_after_adam(AdamArgs a, const float* __restrict__ W, const floa
 template <typename P, int NL, int NOUT, int LOSS, bool BIN, int ACT, int W>
 hipError_t launch_train_one_bin(const TrainArgs& t, uint32_t blocks, hipStream_t s) {
     constexpr size_t smem =
-        sizeof(Stage<P>) + (P::kWaves == 8 ? (size_t)(10 + (NL - 2) * 8 + 4 + 4 + (NL - 2) * 8 + 2) * 64 *
-                                                 sizeof(typename P::op)
-                                           : 0);
+        sizeof(StageOf<P, W>) + (kTilesInLds<P, W> ? (size_t)(10 + (NL - 2) * 8 + 4 + 4 + (NL - 2) * 8 + 2) * 64 *
+                                                         sizeof(typename P::op)
+                                                   : 0);
     static std::atomic<bool> attr[kMaxDevices];
     hipError_t e =
         set_dyn_lds_once(attr, reinterpret_cast<const void*>(&k_train<P, NL, NOUT, LOSS, BIN, ACT, W>), smem);
     if (e != hipSuccess) return e;
-    hipLaunchKernelGGL((k_train<P, NL, NOUT, LOSS, BIN, ACT, W>), dim3(blocks), dim3(Geo<P>::kThreads), smem, s, t);
+    hipLaunchKernelGGL((k_train<P, NL, NOUT, LOSS, BIN, ACT, W>), dim3(blocks), dim3(Geo<P, W>::kThreads), smem, s,
+                       t);
     return hipGetLastError();
 }
 
@@ -951,8 +1029,9 @@ hipError_t launch_train_one(const TrainArgs& t, uint32_t blocks, hipStream_t s) 
 
 }  // namespace
 
-uint32_t train_vertices_per_iter(int precision) {
-    return precision == NRRS_FP32 ? Geo<TF32>::kVert : Geo<TBF16>::kVert;
+uint32_t train_vertices_per_iter(int precision, int net_set) {
+    if (precision == NRRS_FP32) return Geo<TF32>::kVert;
+    return net_set == NRRS_NETS_SHIPPED ? Geo<TBF16, 32>::kVert : Geo<TBF16>::kVert;
 }
 
 namespace {

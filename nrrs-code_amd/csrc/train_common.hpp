@@ -147,7 +147,7 @@ hipError_t launch_bin_records(const BinArgs& b, hipStream_t s);
 // each, n_chunks chunks of the other segments, then their fixed-point sums.
 hipError_t launch_apply_bins(const ApplyArgs& a, uint32_t n_direct, uint32_t n_chunks, hipStream_t s);
 // Vertices one training workgroup processes per iteration.
-uint32_t train_vertices_per_iter(int precision);
+uint32_t train_vertices_per_iter(int precision, int net_set);
 hipError_t launch_reduce_dense(float* rep_f32, void* rep_h, uint32_t n_dense, float* grad, void* grad_h,
                                hipStream_t s);
 hipError_t launch_reduce_partials(const float* part, uint32_t n_blocks, uint32_t n_mlp, float* grad, hipStream_t s);
