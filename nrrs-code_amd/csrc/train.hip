@@ -467,6 +467,43 @@ void k_train(TrainArgs t) {
             }
         }
 
+        // ---- loss inputs, loaded here so that their latency overlaps the encoding's loads and the
+        // forward (loaded where they are used, behind the branches of the loss, each load was a
+        // round trip of its own)
+        float lin[4] = {0.0f, 0.0f, 0.0f, 0.0f};  // dL/dy (EXTERNAL) or target L (LL2) of outputs 4h + r
+        float rq_qt = -1.0f, rq_ref = 0.0f, rq_e = 0.0f, rq_w = 0.0f;  // NRRS_RRS, lane half 0
+        float rq_th[3] = {0.0f, 0.0f, 0.0f}, rq_mu[3] = {0.0f, 0.0f, 0.0f}, rq_s2[3] = {0.0f, 0.0f, 0.0f};
+        if (live) {
+            if constexpr (LOSS == NRRS_LOSS_EXTERNAL) {
+#pragma unroll
+                for (int r = 0; r < 4; ++r)
+                    if (4 * h + r < NOUT) lin[r] = t.dy_ext[(size_t)(4 * h + r) * t.n + i];
+            } else if constexpr (LOSS == NRRS_LOSS_LL2) {
+#pragma unroll
+                for (int r = 0; r < 4; ++r)
+                    if (4 * h + r < NOUT) lin[r] = t.target_L[(size_t)((4 * h + r) % 3) * t.a.v.stride + i];
+            } else if constexpr (LOSS == NRRS_LOSS_RRS) {
+                if (h == 0 && t.rrs_step >= 3) {
+                    const uint32_t S3 = t.a.v.stride;
+                    rq_qt = t.q_target[i];
+                    rq_e = t.error[i];
+                    rq_w = t.sample_weight[i];
+#pragma unroll
+                    for (int c = 0; c < 3; ++c) {
+                        rq_th[c] = t.a.v.thp[c * S3 + i];
+                        rq_mu[c] = t.a.stats[(size_t)c * t.a.stats_stride + i];
+                        rq_s2[c] = t.a.stats[(size_t)(3 + c) * t.a.stats_stride + i];
+                    }
+                    if (t.ref_mean) {
+                        rq_ref = t.ref_mean[i];
+                    } else {
+                        const uint32_t px = t.a.v.pixel[i], Pn = t.a.n_pixels;
+                        rq_ref = (t.a.film[px] + t.a.film[Pn + px] + t.a.film[2 * Pn + px]) * (1.0f / 3.0f);
+                    }
+                }
+            }
+        }
+
         // ---- forward to the output, loss, dL/dy
         op H[4];
         constexpr int C = kCkpt<NL>;
@@ -501,17 +538,14 @@ void k_train(TrainArgs t) {
         if (live) {
             if (LOSS == NRRS_LOSS_EXTERNAL) {
 #pragma unroll
-                for (int r = 0; r < 4; ++r) {
-                    const int c = 4 * h + r;
-                    if (c < NOUT) dyr[r] = t.dy_ext[(size_t)c * t.n + i];
-                }
+                for (int r = 0; r < 4; ++r) dyr[r] = lin[r];
             } else if (LOSS == NRRS_LOSS_LL2) {
                 const int active = t.ll2_step == 0 ? 3 : 6;
 #pragma unroll
                 for (int r = 0; r < 4; ++r) {
                     const int c = 4 * h + r;
                     if (c < NOUT && c < active && (t.head < 0 || c / 3 == t.head)) {
-                        float tg = fmaxf(t.target_L[(size_t)(c % 3) * t.a.v.stride + i], 0.0f);
+                        float tg = fmaxf(lin[r], 0.0f);
                         if (t.clamp_on) tg = fminf(tg, t.clamp_max);
                         if (c >= 3) tg = tg * tg;
                         const float y = Y[r];
@@ -536,25 +570,18 @@ void k_train(TrainArgs t) {
                 }
             } else if (LOSS == NRRS_LOSS_RRS) {
                 if (h == 0 && t.rrs_step >= 3) {
-                    const float qt = t.q_target[i];
+                    const float qt = rq_qt;
                     if (qt >= 0.0f) {
-                        const uint32_t S3 = t.a.v.stride;
-                        float ref;
-                        if (t.ref_mean) {
-                            ref = t.ref_mean[i];
-                        } else {
-                            const uint32_t px = t.a.v.pixel[i], Pn = t.a.n_pixels;
-                            ref = (t.a.film[px] + t.a.film[Pn + px] + t.a.film[2 * Pn + px]) * (1.0f / 3.0f);
-                        }
+                        const float ref = rq_ref;
                         float m2 = 0.0f;
 #pragma unroll
                         for (int c = 0; c < 3; ++c) {
-                            const float th = t.a.v.thp[c * S3 + i];
-                            const float s2 = t.a.stats[(size_t)(3 + c) * t.a.stats_stride + i];
+                            const float th = rq_th[c];
+                            const float s2 = rq_s2[c];
                             float l2;
                             if (t.train_sigma) {  // E[L^2] = sigma^2 + max(L, 0)^2
                                 const float sg = activation_sigma(s2);
-                                const float mu = fmaxf(t.a.stats[(size_t)c * t.a.stats_stride + i], 0.0f);
+                                const float mu = fmaxf(rq_mu[c], 0.0f);
                                 l2 = sg * sg + mu * mu;
                             } else {
                                 l2 = fmaxf(s2, 0.0f);
@@ -563,11 +590,11 @@ void k_train(TrainArgs t) {
                         }
                         float vv = m2 * (1.0f / 3.0f) / (ref * ref + 0.01f);
                         if (t.clamp_on) vv = fminf(vv, t.clamp_max);
-                        const float e = fmaxf(t.error[i], 1e-4f);
+                        const float e = fmaxf(rq_e, 1e-4f);
                         const float y = Y[0];
                         const float q = activation_sigma(y);
                         const float qe = q + 0.01f;
-                        const float w = t.sample_weight[i];
+                        const float w = rq_w;
                         const float rq = 1.0f / qe, vq = vv * rq;  // v / (q + 0.01), one division
                         const float f = t.gamma[0] * (vq + q * ebar) + t.gamma[1] * (vq + q * e) +
                                         t.gamma[2] * q + t.gamma[3] * (q - qt) * (q - qt);
@@ -766,7 +793,7 @@ void k_train(TrainArgs t) {
                         for (int j = 0; j < 8; ++j) x8[j] = D[ot][8 * g + j];
                         Dl[2 * ot + g] = act_mask<P, ACT>(x8, H[2 * ot + g]);
                     }
-            } else {
+            } else if (t.a.grid.n_levels) {  // (a net without a grid has no table gradient)
                 // dL/denc for the 32 grid channels: rows c = 8(r/4) + 4h + r%4
                 f32x16 G = f32x16{};
 #pragma unroll
