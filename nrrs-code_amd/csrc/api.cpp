@@ -1267,6 +1267,23 @@ nrrs_status nrrs_rrs_infer_decide(nrrs_ctx* ctx, const nrrs_decide_args* a, void
         EncodeArgs ea = encode_args(ctx, a->v, a->film_dev, nullptr, 0);
         NRRS_HIP(run_net(ctx, NRRS_STATNET, ea, EPI_ADRRS, a->n, ep, s));
     } else if (a->q_mode == NRRS_Q_RRSNET_CHAINED && ctx->cfg.precision != NRRS_FP32 && !ctx->cfg.ll2_two_head &&
+               ctx->cfg.net_set == NRRS_NETS_SHIPPED && ctx->grid_enc && a->n <= ctx->grid_enc_cap) {
+        // shipped nets: StatNet's level-major grid encoding, then StatNet -> RRSNet (no grid) -> q
+        // in one launch
+        const nrrs_config& cfg = ctx->cfg;
+        EncodeArgs ea = encode_args(ctx, a->v, a->film_dev, nullptr, 0);
+        ea.grid = ctx->net[NRRS_STATNET].grid;
+        ea.l0_chunks = l0_chunks(cfg, NRRS_STATNET);
+        ea.grid_enc = ctx->grid_enc;
+        ea.grid_enc_stride = ctx->grid_enc_cap;
+        const NetView ws = view_of(ctx, NRRS_STATNET), wr = view_of(ctx, NRRS_RRSNET);
+        {
+            OpScope t(ctx, "encode_statnet", s);
+            NRRS_HIP(launch_encode_levels(cfg.precision, ea, ws.grid_lp, a->n, ctx->grid_enc, ctx->grid_enc_cap, s));
+        }
+        OpScope t(ctx, "mlp_chained", s);
+        NRRS_HIP(launch_net_chained(cfg.precision, ea, nullptr, a->n, ws.mlp_lp, wr.mlp_lp, ep, s));
+    } else if (a->q_mode == NRRS_Q_RRSNET_CHAINED && ctx->cfg.precision != NRRS_FP32 && !ctx->cfg.ll2_two_head &&
                ctx->cfg.net_set == NRRS_NETS_BENCH && ctx->grid_enc && ctx->grid_enc2 && a->n <= ctx->grid_enc_cap) {
         // both nets' level-major grid encodings (one pass over the paired table, else a pass per
         // net), then StatNet -> RRSNet -> q in one launch

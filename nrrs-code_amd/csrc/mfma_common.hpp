@@ -237,10 +237,12 @@ __device__ __forceinline__ void grid_half_from_levels(const uint32_t* __restrict
     }
 }
 
-// encode_half with the grid channels read from the level-major encoding.
-template <typename P>
+// encode_half with the grid channels read from the level-major encoding. SET: the network set
+// when the caller knows it at compile time, else -1 (a.net_set).
+template <typename P, int SET = -1>
 __device__ __forceinline__ void encode_half_pre(const EncodeArgs& a, int net, uint32_t i, int h, typename P::vec8* B,
                                                 typename P::vec8* Bl = nullptr) {
+    const int set = SET >= 0 ? SET : a.net_set;
     VertexRec v = vertex_from_soa(a, i);
     const float ref = net == NRRS_RRSNET ? film_mean(a.film, a.n_pixels, v.pixel) : 0.0f;
     grid_half_from_levels<P>(a.grid_enc, a.grid_enc_stride, a.grid.n_levels, i, h, B);
@@ -248,9 +250,9 @@ __device__ __forceinline__ void encode_half_pre(const EncodeArgs& a, int net, ui
         float st[6];
 #pragma unroll
         for (int k = 0; k < 6; ++k) st[k] = a.stats[(size_t)k * a.stats_stride + i];
-        encode_tail_half<P>(v, net, h, ref, st, B, Bl, false, a.net_set);
+        encode_tail_half<P>(v, net, h, ref, st, B, Bl, false, set);
     } else {
-        encode_tail_half<P>(v, net, h, ref, nullptr, B, Bl, a.aid != 0, a.net_set);
+        encode_tail_half<P>(v, net, h, ref, nullptr, B, Bl, a.aid != 0, set);
     }
 }
 

@@ -176,25 +176,35 @@ __global__ __launch_bounds__(kBlockT, 4) void k_net_mfma(EncodeArgs a, int net, 
 // Chained StatNet -> RRSNet inference in one launch: the StatNet statistics of a vertex stay in
 // registers (exchanged between the two lane halves) and feed the RRSNet input directly, which
 // saves writing and re-reading them and a second pass over the vertex records. The grid
-// channels of both nets come from level-major encodings (a.grid_enc: StatNet, enc_rrs: RRSNet).
-// Both nets' weight tiles sit in LDS: FP16 76 KB, 8-wave workgroups, two per CU; BF16 (split
-// StatNet) 106 KB, one 16-wave workgroup per CU. Four waves per SIMD either way (at most 128
-// registers). Epilogue as k_net_mfma's EPI_RRS: q and its exact sum term.
-template <typename P>
+// channels come from level-major encodings (a.grid_enc: StatNet, enc_rrs: RRSNet; the shipped
+// RRSNet has no grid). Both nets' weight tiles sit in LDS: FP16 76 KB, 8-wave workgroups, two
+// per CU; BF16 (split StatNet) 106 KB, one 16-wave workgroup per CU. Four waves per SIMD either
+// way (at most 128 registers). Epilogue as k_net_mfma's EPI_RRS: q and its exact sum term.
+// SET: the network set (NRRS_NETS_BENCH: StatNet 80-64x4-6, RRSNet 80-64x3-1, ReLU;
+// NRRS_NETS_SHIPPED: 32-wide LeakyReLU nets, StatNet 3 hidden layers, RRSNet on
+// NRRSNetworkInput1 without a grid).
+template <typename P, int SET = NRRS_NETS_BENCH>
 struct Chain {
+    static constexpr bool kShipped = SET == NRRS_NETS_SHIPPED;
+    static constexpr int kLayersS = kShipped ? 4 : 5;
+    static constexpr int kWidth = kShipped ? 32 : 64;
+    static constexpr int kAct = kShipped ? ACT_LEAKY : ACT_RELU;
+    static constexpr uint32_t kL0Rrs = kShipped ? 0x10u : 0x1fu;  // RRSNet's layer-0 input chunks
     static constexpr int kWaves = P::kSplitStat ? 16 : 8;
     static constexpr int kThreads = 64 * kWaves;
     static constexpr int kPerCu = P::kSplitStat ? 1 : 2;
-    static constexpr int kTilesS = tiles_of(5, split_of<P>(6)), kTilesR = tiles_of(4);
+    static constexpr int kTilesS = tiles_of(kLayersS, split_of<P>(6)), kTilesR = tiles_of(4);
 };
 
-template <typename P>
-__global__ __launch_bounds__(Chain<P>::kThreads) void k_net_chained(EncodeArgs a, const uint32_t* __restrict__ enc_rrs,
-                                                                   uint32_t n, const u32x4* __restrict__ Ws,
-                                                                   const u32x4* __restrict__ Wr, EpiArgs ep) {
+template <typename P, int SET>
+__global__ __launch_bounds__((Chain<P, SET>::kThreads)) void k_net_chained(EncodeArgs a,
+                                                                        const uint32_t* __restrict__ enc_rrs,
+                                                                        uint32_t n, const u32x4* __restrict__ Ws,
+                                                                        const u32x4* __restrict__ Wr, EpiArgs ep) {
+    typedef Chain<P, SET> Ch;
     extern __shared__ u32x4 sW[];
-    constexpr int kChainWaves = Chain<P>::kWaves, kChainT = Chain<P>::kThreads;
-    constexpr int kTilesS = Chain<P>::kTilesS, kTilesR = Chain<P>::kTilesR;
+    constexpr int kChainWaves = Ch::kWaves, kChainT = Ch::kThreads;
+    constexpr int kTilesS = Ch::kTilesS, kTilesR = Ch::kTilesR;
     constexpr bool kSplit = split_of<P>(6);
     u32x4* sWs = sW;
     u32x4* sWr = sW + kTilesS * 64;
@@ -212,16 +222,17 @@ __global__ __launch_bounds__(Chain<P>::kThreads) void k_net_chained(EncodeArgs a
         const bool live = i < n;
         typename P::vec8 B[5], Bl[3];
         if (live) {
-            encode_half_pre<P>(a, NRRS_STATNET, i, h, B, kSplit ? Bl : nullptr);
+            encode_half_pre<P, SET>(a, NRRS_STATNET, i, h, B, kSplit ? Bl : nullptr);
         } else {
 #pragma unroll
             for (int kb = 0; kb < 5; ++kb) B[kb] = typename P::vec8{};
 #pragma unroll
             for (int kb = 0; kb < 3; ++kb) Bl[kb] = typename P::vec8{};
         }
-        // K chunks 2 and 3 (SH-4 of the direction and the normal) are the same in both nets' inputs
+        // benchmark nets: K chunks 2 and 3 (SH-4 of the direction and the normal) are the same in
+        // both nets' inputs
         const typename P::vec8 sh_dir = B[2], sh_nrm = B[3];
-        const f32x16 Ys = mlp_forward<P, 5, kSplit>(sWs, B, Bl, lane);
+        const f32x16 Ys = mlp_forward<P, Ch::kLayersS, kSplit, Ch::kAct, Ch::kWidth>(sWs, B, Bl, lane, a.l0_chunks);
         // statistics of the lane's vertex: outputs 0..3 in half 0, 4..5 in half 1
         float st[6];
         {
@@ -237,15 +248,21 @@ __global__ __launch_bounds__(Chain<P>::kThreads) void k_net_chained(EncodeArgs a
         if (live) {
             const VertexRec v = vertex_from_soa(a, i);
             const float ref = film_mean(a.film, a.n_pixels, v.pixel);
-            grid_half_from_levels<P>(enc_rrs, a.grid_enc_stride, a.grid.n_levels, i, h, B);
-            encode_tail_half<P>(v, NRRS_RRSNET, h, ref, st, B);
-            B[2] = sh_dir;  // (the recomputed chunks are dead and not evaluated)
-            B[3] = sh_nrm;
+            if constexpr (Ch::kShipped) {
+                B[0] = B[1] = typename P::vec8{};
+            } else {
+                grid_half_from_levels<P>(enc_rrs, a.grid_enc_stride, a.grid.n_levels, i, h, B);
+            }
+            encode_tail_half<P>(v, NRRS_RRSNET, h, ref, st, B, nullptr, false, SET);
+            if constexpr (!Ch::kShipped) {
+                B[2] = sh_dir;  // (the recomputed chunks are dead and not evaluated)
+                B[3] = sh_nrm;
+            }
         } else {
 #pragma unroll
             for (int kb = 0; kb < 5; ++kb) B[kb] = typename P::vec8{};
         }
-        const f32x16 Yr = mlp_forward<P, 4>(sWr, B, nullptr, lane);
+        const f32x16 Yr = mlp_forward<P, 4, false, Ch::kAct, Ch::kWidth>(sWr, B, nullptr, lane, Ch::kL0Rrs);
         if (live && h == 0) {
             const float q = rrs_window_clamp(ep.d, activation_sigma(Yr[0]));
             ep.q[i] = q;
@@ -255,13 +272,13 @@ __global__ __launch_bounds__(Chain<P>::kThreads) void k_net_chained(EncodeArgs a
     block_atomic_add_u64(term, ep.sum);
 }
 
-template <typename P>
+template <typename P, int SET>
 hipError_t launch_chained(const EncodeArgs& a, const uint32_t* enc_rrs, uint32_t n, const uint16_t* Ws,
                           const uint16_t* Wr, const EpiArgs& ep, hipStream_t s) {
-    constexpr size_t smem = (size_t)(Chain<P>::kTilesS + Chain<P>::kTilesR) * kTileHalves * 2;
-    constexpr int kChainWaves = Chain<P>::kWaves;
+    constexpr size_t smem = (size_t)(Chain<P, SET>::kTilesS + Chain<P, SET>::kTilesR) * kTileHalves * 2;
+    constexpr int kChainWaves = Chain<P, SET>::kWaves;
     static std::atomic<bool> attr[kMaxDevices];
-    hipError_t e = set_dyn_lds_once(attr, reinterpret_cast<const void*>(&k_net_chained<P>), smem);
+    hipError_t e = set_dyn_lds_once(attr, reinterpret_cast<const void*>(&k_net_chained<P, SET>), smem);
     if (e != hipSuccess) return e;
     // persistent: the resident workgroups only, each loading both nets' tiles once
     static std::atomic<uint32_t> resident_of[kMaxDevices];
@@ -271,14 +288,14 @@ hipError_t launch_chained(const EncodeArgs& a, const uint32_t* enc_rrs, uint32_t
     if (!resident) {
         int cus = 0;
         if ((e = hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev)) != hipSuccess) return e;
-        resident = (uint32_t)Chain<P>::kPerCu * (uint32_t)(cus > 0 ? cus : 1);
+        resident = (uint32_t)Chain<P, SET>::kPerCu * (uint32_t)(cus > 0 ? cus : 1);
         if (dev >= 0 && dev < kMaxDevices) resident_of[dev].store(resident, std::memory_order_relaxed);
     }
     const uint32_t n_cb = (n + 31) / 32;
     uint32_t blocks = (n_cb + kChainWaves - 1) / kChainWaves;
     blocks = blocks > resident ? resident : blocks;
     if (!blocks) return hipSuccess;
-    hipLaunchKernelGGL(k_net_chained<P>, dim3(blocks), dim3(Chain<P>::kThreads), smem, s, a, enc_rrs, n,
+    hipLaunchKernelGGL((k_net_chained<P, SET>), dim3(blocks), dim3(Chain<P, SET>::kThreads), smem, s, a, enc_rrs, n,
                        reinterpret_cast<const u32x4*>(Ws), reinterpret_cast<const u32x4*>(Wr), ep);
     return hipGetLastError();
 }
@@ -398,9 +415,16 @@ hipError_t launch_pack_lp(int precision, const float* grid, uint64_t n_grid, con
 
 hipError_t launch_net_chained(int precision, const EncodeArgs& a, const uint32_t* enc_rrs, uint32_t n,
                               const uint16_t* Ws, const uint16_t* Wr, const EpiArgs& ep, hipStream_t s) {
-    if (!a.grid_enc || !enc_rrs || a.stats) return hipErrorInvalidValue;
-    if (precision == NRRS_BF16) return launch_chained<BF16>(a, enc_rrs, n, Ws, Wr, ep, s);
-    if (precision == NRRS_FP16) return launch_chained<F16>(a, enc_rrs, n, Ws, Wr, ep, s);
+    if (a.stats || a.aid) return hipErrorInvalidValue;
+    if (a.net_set == NRRS_NETS_SHIPPED) {  // the shipped RRSNet has no grid
+        if (!a.grid_enc && a.grid.n_levels) return hipErrorInvalidValue;
+        if (precision == NRRS_BF16) return launch_chained<BF16, NRRS_NETS_SHIPPED>(a, nullptr, n, Ws, Wr, ep, s);
+        if (precision == NRRS_FP16) return launch_chained<F16, NRRS_NETS_SHIPPED>(a, nullptr, n, Ws, Wr, ep, s);
+        return hipErrorInvalidValue;
+    }
+    if (!a.grid_enc || !enc_rrs) return hipErrorInvalidValue;
+    if (precision == NRRS_BF16) return launch_chained<BF16, NRRS_NETS_BENCH>(a, enc_rrs, n, Ws, Wr, ep, s);
+    if (precision == NRRS_FP16) return launch_chained<F16, NRRS_NETS_BENCH>(a, enc_rrs, n, Ws, Wr, ep, s);
     return hipErrorInvalidValue;
 }
 
