@@ -60,8 +60,13 @@ def test_shipped_encode_matches_oracle(torch_cuda, ora, aid):
 
 
 @pytest.mark.parametrize("prec", PRECISIONS)
-def test_shipped_rrs_stage_matches_fp64(torch_cuda, ora, prec):
+@pytest.mark.parametrize("n", [N, 3001])
+def test_shipped_rrs_stage_matches_fp64(torch_cuda, ora, prec, n):
+    """The chained RRS stage (FP16/BF16: one kernel from StatNet to q; FP32: a StatNet kernel,
+    then RRSNet on its statistics) and the ADRRS form against the FP64 oracle, at a full and a
+    ragged batch (3001: no multiple of the 32-vertex blocks)."""
     torch = torch_cuda
+    N = n
     ctx = na.Context(_cfg(prec, n_pixels=N, max_queue=N, max_train=N))
     ps = random_params(ctx, na.STATNET, 11)
     pr = random_params(ctx, na.RRSNET, 12)
