@@ -213,7 +213,7 @@ struct nrrs_ctx {
     // training scratch
     float* train_stats = nullptr;           // [6][max_train] StatNet outputs of the batch
     float* scale_tmp = nullptr;             // device scalar for nrrs_backward's loss scale
-    uint32_t max_train_blocks = 256;  // training workgroups: one per CU (set from the device)
+    uint32_t max_train_blocks = 256;  // training workgroups: one per CU, two for the narrow kernels (set at create)
     // helper stream (default priority) on which a joint training step runs the StatNet update
     // beside the RRSNet backward; joined to the caller's stream before the step returns
     hipStream_t aux_stream = nullptr;
