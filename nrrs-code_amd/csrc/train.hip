@@ -1364,8 +1364,10 @@ __device__ __forceinline__ void apply_feature(unsigned long long* acc, uint32_t 
         ((unsigned long long)(m | (norm << 10)) << (ex - norm)) & (0ull - (unsigned long long)finite);
     const unsigned long long t = (mag * u) >> 16, x0 = mag - t;  // mag * u < 2^57
     const bool neg = (h & 0x8000u) != 0u;
-    atomicAdd(&acc[e], neg ? 0ull - x0 : x0);
-    atomicAdd(&acc[partner], neg ? 0ull - t : t);
+    // zero parts (the partner of a single-corner record, zero values) skip their atomic: the
+    // lanes drop out of the LDS instruction, whose cost grows with its active lanes
+    if (x0) atomicAdd(&acc[e], neg ? 0ull - x0 : x0);
+    if (t) atomicAdd(&acc[partner], neg ? 0ull - t : t);
 }
 
 // One record (key, value) into the LDS sums of its segment. Key: entry within the segment
