@@ -416,25 +416,64 @@ void k_train(TrainArgs t) {
         const bool live = i < t.n;
         const int vcol = wave * 32 + col;  // vertex column in the stage
 
-        // ---- encoding (B layout, natural channel order): lane half h holds channels
-        // 16kb + 8h + j, so it gathers only levels 8kb + 4h + {0..3} of the grid
+        // ---- the iteration's inputs. Every load is issued here, before any is used: the film
+        // gather through the pixel index is the only dependent one (loaded where the encoding and
+        // the loss use them, behind their branches and arithmetic, the groups of loads had been
+        // round trips of their own).
+        VertexRec v{};
+        float st6[6] = {0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f};  // RRSNet: StatNet statistics
+        float lin[4] = {0.0f, 0.0f, 0.0f, 0.0f};  // dL/dy (EXTERNAL) or target L (LL2) of outputs 4h + r
+        float rq_qt = -1.0f, rq_e = 0.0f, rq_w = 0.0f, rq_refm = 0.0f;  // NRRS_RRS
         op E[5];
+        if (live) {
+            if constexpr (!P::kF32Table) {
+                // FP16/BF16 grid channels: the level-major encoding computed before this kernel;
+                // lane half h holds channels 16kb + 8h + j, levels 8kb + 4h + {0..3}
+#pragma unroll
+                for (int kb = 0; kb < 2; ++kb) {
+                    u32x4 u;
+#pragma unroll
+                    for (int ll = 0; ll < 4; ++ll) {
+                        const uint32_t l = 8 * kb + 4 * h + ll;
+                        u[ll] = l < t.a.grid.n_levels ? t.a.grid_enc[(size_t)l * t.a.grid_enc_stride + i] : 0u;
+                    }
+                    E[kb] = __builtin_bit_cast(op, u);
+                }
+            }
+            v = vertex_from_soa(t.a, i);
+            if constexpr (kNet == NRRS_RRSNET) {
+                if (t.a.stats) {
+#pragma unroll
+                    for (int k = 0; k < 6; ++k) st6[k] = t.a.stats[(size_t)k * t.a.stats_stride + i];
+                }
+            }
+            if constexpr (LOSS == NRRS_LOSS_EXTERNAL) {
+#pragma unroll
+                for (int r = 0; r < 4; ++r)
+                    if (4 * h + r < NOUT) lin[r] = t.dy_ext[(size_t)(4 * h + r) * t.n + i];
+            } else if constexpr (LOSS == NRRS_LOSS_LL2) {
+#pragma unroll
+                for (int r = 0; r < 4; ++r)
+                    if (4 * h + r < NOUT) lin[r] = t.target_L[(size_t)((4 * h + r) % 3) * t.a.v.stride + i];
+            } else if constexpr (LOSS == NRRS_LOSS_RRS) {
+                if (t.rrs_step >= 3) {
+                    rq_qt = t.q_target[i];
+                    rq_e = t.error[i];
+                    rq_w = t.sample_weight[i];
+                    if (t.ref_mean) rq_refm = t.ref_mean[i];
+                }
+            }
+        }
+        // mean of the pixel estimate: RRSNet input channel and the NRRS_RRS loss's reference
+        const float pix_ref = live && kNet == NRRS_RRSNET ? film_mean(t.a.film, t.a.n_pixels, v.pixel) : 0.0f;
+        const float rq_ref = t.ref_mean ? rq_refm : pix_ref;
+
+        // ---- encoding (B layout, natural channel order): lane half h holds channels 16kb + 8h + j
         {
             float e8[8];
             if (live) {
-                if constexpr (!P::kF32Table) {
-                    // FP16/BF16: the level-major encoding computed before this kernel
-#pragma unroll
-                    for (int kb = 0; kb < 2; ++kb) {
-                        u32x4 u;
-#pragma unroll
-                        for (int ll = 0; ll < 4; ++ll) {
-                            const uint32_t l = 8 * kb + 4 * h + ll;
-                            u[ll] = l < t.a.grid.n_levels ? t.a.grid_enc[(size_t)l * t.a.grid_enc_stride + i] : 0u;
-                        }
-                        E[kb] = __builtin_bit_cast(op, u);
-                    }
-                } else {
+                if constexpr (P::kF32Table) {
+                    // FP32: the grid gathered here, each lane half only its 8 levels
                     float x[3];
                     load_xn(t.a, i, x);
 #pragma unroll
@@ -451,8 +490,12 @@ void k_train(TrainArgs t) {
                         E[kb] = P::pack(e8);
                     }
                 }
+                // the non-grid channels (as encode_tail, from the values loaded above)
                 float e[kEnc];
-                encode_tail<kSet>(t.a, kNet, i, e);
+                if (kNet == NRRS_RRSNET && t.a.stats && !t.a.aid)
+                    encode_tail_rec(v, kNet, pix_ref, st6, e, t.a.sigma_l2 != 0, false, kSet);
+                else
+                    encode_tail_rec(v, kNet, pix_ref, nullptr, e, false, t.a.aid != 0, kSet);
 #pragma unroll
                 for (int kb = 2; kb < 5; ++kb) {
 #pragma unroll
@@ -464,43 +507,6 @@ void k_train(TrainArgs t) {
                 for (int j = 0; j < 8; ++j) e8[j] = 0.0f;
 #pragma unroll
                 for (int kb = 0; kb < 5; ++kb) E[kb] = P::pack(e8);
-            }
-        }
-
-        // ---- loss inputs, loaded here so that their latency overlaps the encoding's loads and the
-        // forward (loaded where they are used, behind the branches of the loss, each load was a
-        // round trip of its own)
-        float lin[4] = {0.0f, 0.0f, 0.0f, 0.0f};  // dL/dy (EXTERNAL) or target L (LL2) of outputs 4h + r
-        float rq_qt = -1.0f, rq_ref = 0.0f, rq_e = 0.0f, rq_w = 0.0f;  // NRRS_RRS, lane half 0
-        float rq_th[3] = {0.0f, 0.0f, 0.0f}, rq_mu[3] = {0.0f, 0.0f, 0.0f}, rq_s2[3] = {0.0f, 0.0f, 0.0f};
-        if (live) {
-            if constexpr (LOSS == NRRS_LOSS_EXTERNAL) {
-#pragma unroll
-                for (int r = 0; r < 4; ++r)
-                    if (4 * h + r < NOUT) lin[r] = t.dy_ext[(size_t)(4 * h + r) * t.n + i];
-            } else if constexpr (LOSS == NRRS_LOSS_LL2) {
-#pragma unroll
-                for (int r = 0; r < 4; ++r)
-                    if (4 * h + r < NOUT) lin[r] = t.target_L[(size_t)((4 * h + r) % 3) * t.a.v.stride + i];
-            } else if constexpr (LOSS == NRRS_LOSS_RRS) {
-                if (h == 0 && t.rrs_step >= 3) {
-                    const uint32_t S3 = t.a.v.stride;
-                    rq_qt = t.q_target[i];
-                    rq_e = t.error[i];
-                    rq_w = t.sample_weight[i];
-#pragma unroll
-                    for (int c = 0; c < 3; ++c) {
-                        rq_th[c] = t.a.v.thp[c * S3 + i];
-                        rq_mu[c] = t.a.stats[(size_t)c * t.a.stats_stride + i];
-                        rq_s2[c] = t.a.stats[(size_t)(3 + c) * t.a.stats_stride + i];
-                    }
-                    if (t.ref_mean) {
-                        rq_ref = t.ref_mean[i];
-                    } else {
-                        const uint32_t px = t.a.v.pixel[i], Pn = t.a.n_pixels;
-                        rq_ref = (t.a.film[px] + t.a.film[Pn + px] + t.a.film[2 * Pn + px]) * (1.0f / 3.0f);
-                    }
-                }
             }
         }
 
@@ -576,12 +582,12 @@ void k_train(TrainArgs t) {
                         float m2 = 0.0f;
 #pragma unroll
                         for (int c = 0; c < 3; ++c) {
-                            const float th = rq_th[c];
-                            const float s2 = rq_s2[c];
+                            const float th = v.thp[c];
+                            const float s2 = st6[3 + c];
                             float l2;
                             if (t.train_sigma) {  // E[L^2] = sigma^2 + max(L, 0)^2
                                 const float sg = activation_sigma(s2);
-                                const float mu = fmaxf(rq_mu[c], 0.0f);
+                                const float mu = fmaxf(st6[c], 0.0f);
                                 l2 = sg * sg + mu * mu;
                             } else {
                                 l2 = fmaxf(s2, 0.0f);
