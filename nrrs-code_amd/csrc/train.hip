@@ -1343,10 +1343,10 @@ __global__ __launch_bounds__(kBinVert) void k_bin_levels(BinArgs b, uint32_t n_c
 // Sums of the binned records. Every FP16 value is an integer multiple of 2^-24 below 2^16, so
 // the records are summed exactly as 64-bit fixed point with integer LDS atomics (LDS float
 // atomics are an order of magnitude slower); the sums do not depend on the record order.
-// k_apply_bins: one workgroup per segment of a hashed level adds its sum to the table
-// gradient. k_apply_chunks: segments of the dense levels, which receive many records per
-// entry, are cut into chunks whose sums are added to a 64-bit accumulator with coalesced
-// integer atomics; k_apply_finish adds the accumulator to the table gradient. A non-finite
+// k_apply: one workgroup per segment of a hashed level adds its sum to the table gradient
+// (apply_segment); segments of the dense levels, which receive many records per entry, are cut
+// into chunks whose sums are added to a 64-bit accumulator with coalesced integer atomics
+// (apply_chunk); k_apply_finish adds the accumulator to the table gradient. A non-finite
 // record turns a table-gradient entry into NaN, so the finite check of the optimiser step
 // sees it. Counts and accumulators are cleared for the next launch.
 constexpr int kApplyThreads = 1024;
@@ -1424,10 +1424,11 @@ __device__ __forceinline__ h2v add_fixed(h2v o, long long s0, long long s1) {
                (_Float16)((float)o.y + (float)((double)s1 * 0x1p-24))};
 }
 
-__global__ __launch_bounds__(kApplyThreads) void k_apply_bins(ApplyArgs a) {
-    extern __shared__ unsigned long long acc_s[];  // [2][kSegEntries], units of 2^-24
-    __shared__ uint32_t bad_s;
-    const uint32_t seg = a.direct[blockIdx.x];
+// One hashed-level segment (`direct` entry b) by one workgroup. acc_s: [2][kSegEntries], units
+// of 2^-24.
+__device__ __forceinline__ void apply_segment(const ApplyArgs& a, uint32_t b, unsigned long long* acc_s,
+                                              uint32_t& bad_s) {
+    const uint32_t seg = a.direct[b];
     const SegInfo sg = a.segs[seg];
     for (uint32_t k = threadIdx.x; k < 2 * kSegEntries; k += kApplyThreads) acc_s[k] = 0ull;
     if (threadIdx.x == 0) bad_s = 0u;
@@ -1456,9 +1457,9 @@ __global__ __launch_bounds__(kApplyThreads) void k_apply_bins(ApplyArgs a) {
     }
 }
 
-__global__ __launch_bounds__(kApplyThreads) void k_apply_chunks(ApplyArgs a) {
-    extern __shared__ unsigned long long acc_s[];
-    const uint2 ch = a.chunks[blockIdx.x];  // {segment, first record}
+// One chunk of a dense-level segment (`chunks` entry b) by one workgroup.
+__device__ __forceinline__ void apply_chunk(const ApplyArgs& a, uint32_t b, unsigned long long* acc_s) {
+    const uint2 ch = a.chunks[b];  // {segment, first record}
     const SegInfo sg = a.segs[ch.x];
     const uint32_t n = min(a.seg_count[ch.x], sg.cap);
     if (ch.y >= n) return;  // whole workgroup
@@ -1474,6 +1475,18 @@ __global__ __launch_bounds__(kApplyThreads) void k_apply_chunks(ApplyArgs a) {
         const unsigned long long v = acc_s[(k & 1u) * kSegEntries + (k >> 1)];
         if (v) atomicAdd(acc + k, v);
     }
+}
+
+// Both kinds in one launch: workgroups [0, n_direct) take the hashed segments, the rest the
+// dense-level chunks, so that the chunks fill the CUs the last segments leave idle (and the
+// launch gap between two kernels goes away).
+__global__ __launch_bounds__(kApplyThreads) void k_apply(ApplyArgs a, uint32_t n_direct) {
+    extern __shared__ unsigned long long acc_s[];
+    __shared__ uint32_t bad_s;
+    if (blockIdx.x < n_direct)
+        apply_segment(a, blockIdx.x, acc_s, bad_s);
+    else
+        apply_chunk(a, blockIdx.x - n_direct, acc_s);
 }
 
 __global__ void k_apply_finish(ApplyArgs a) {
@@ -1626,13 +1639,12 @@ hipError_t launch_train(int precision, const TrainArgs& t, int nl, int n_out, in
 
 hipError_t launch_apply_bins(const ApplyArgs& a, uint32_t n_direct, uint32_t n_chunks, hipStream_t s) {
     constexpr size_t smem = 2 * kSegEntries * sizeof(unsigned long long);
-    static std::atomic<bool> attr_bins[kMaxDevices], attr_chunks[kMaxDevices];
-    hipError_t e = set_dyn_lds_once(attr_bins, reinterpret_cast<const void*>(&k_apply_bins), smem);
-    if (e == hipSuccess) e = set_dyn_lds_once(attr_chunks, reinterpret_cast<const void*>(&k_apply_chunks), smem);
+    static std::atomic<bool> attr[kMaxDevices];
+    hipError_t e = set_dyn_lds_once(attr, reinterpret_cast<const void*>(&k_apply), smem);
     if (e != hipSuccess) return e;
-    if (n_direct) hipLaunchKernelGGL(k_apply_bins, dim3(n_direct), dim3(kApplyThreads), smem, s, a);
+    if (n_direct + n_chunks)
+        hipLaunchKernelGGL(k_apply, dim3(n_direct + n_chunks), dim3(kApplyThreads), smem, s, a, n_direct);
     if (n_chunks) {
-        hipLaunchKernelGGL(k_apply_chunks, dim3(n_chunks), dim3(kApplyThreads), smem, s, a);
         const uint32_t blocks = std::min<uint32_t>((a.n_acc + 255) / 256, 2048);
         hipLaunchKernelGGL(k_apply_finish, dim3(blocks), dim3(256), 0, s, a);
     }
