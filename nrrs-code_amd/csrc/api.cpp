@@ -560,12 +560,16 @@ BinArgs bin_args(nrrs_ctx* c, int net, uint32_t n, const EncodeArgs& a) {
 }
 
 // Sums of one net's binned records (keys `key`, its own values) into its table gradient.
-hipError_t launch_apply(nrrs_ctx* c, int net, const uint32_t* key, hipStream_t s) {
+// `counts`: the per-segment record counts to read (default: the net's own); `keep`: leave them
+// for a second apply of the same records (the joint step's RRSNet reads StatNet's).
+hipError_t launch_apply(nrrs_ctx* c, int net, const uint32_t* key, hipStream_t s, uint32_t* counts = nullptr,
+                        bool keep = false) {
     NetState& ns = c->net[net];
     ApplyArgs ap{};
     ap.key = key;
     ap.val = ns.rec_val;
-    ap.seg_count = ns.seg_count;
+    ap.seg_count = counts ? counts : ns.seg_count;
+    ap.keep_counts = keep;
     ap.segs = static_cast<const SegInfo*>(ns.segs);
     ap.direct = ns.direct;
     ap.chunks = ns.chunks;
@@ -1466,9 +1470,6 @@ nrrs_status nrrs_train_step(nrrs_ctx* ctx, uint32_t net_mask, uint32_t n, const 
             {
                 OpScope t(ctx, "bin_scan", s);
                 NRRS_HIP(launch_bin_scan(bn, s));
-                // RRSNet's records take StatNet's slots: its apply reads the same per-segment counts
-                NRRS_HIP(hipMemcpyAsync(s1.seg_count, s0.seg_count, (size_t)s0.n_segs * 4, hipMemcpyDeviceToDevice,
-                                        s));
             }
             bn.gdl2 = s1.gdl;
             bn.rec_val2 = s1.rec_val;
@@ -1479,8 +1480,10 @@ nrrs_status nrrs_train_step(nrrs_ctx* ctx, uint32_t net_mask, uint32_t n, const 
                 NRRS_HIP(launch_bin_records(bn, s));
             }
             {
+                // RRSNet's records take StatNet's slots: its apply reads StatNet's per-segment
+                // counts after this one, which leaves them
                 OpScope t(ctx, "apply_statnet", s);
-                NRRS_HIP(launch_apply(ctx, NRRS_STATNET, s0.rec_key, s));
+                NRRS_HIP(launch_apply(ctx, NRRS_STATNET, s0.rec_key, s, nullptr, true));
             }
             OpScope t(ctx, "reduce_statnet", s);
             NRRS_HIP(launch_reduce_partials(s0.dw_partial, b0, (uint32_t)s0.n_mlp, s0.grad + s0.n_grid, s));
@@ -1504,7 +1507,7 @@ nrrs_status nrrs_train_step(nrrs_ctx* ctx, uint32_t net_mask, uint32_t n, const 
             hipError_t e;
             {
                 OpScope t(ctx, "apply_rrsnet", s);
-                e = launch_apply(ctx, NRRS_RRSNET, s0.rec_key, s);
+                e = launch_apply(ctx, NRRS_RRSNET, s0.rec_key, s, s0.seg_count);
             }
             if (e == hipSuccess) {
                 OpScope t(ctx, "reduce_rrsnet", s);

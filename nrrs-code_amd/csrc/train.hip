@@ -1453,7 +1453,7 @@ __device__ __forceinline__ void apply_segment(const ApplyArgs& a, uint32_t b, un
             g[0] = h2v{(_Float16)__builtin_nanf(""), (_Float16)__builtin_nanf("")};
             if (a.nonfinite) atomicOr(a.nonfinite, 1u);
         }
-        a.seg_count[seg] = 0;
+        if (!a.keep_counts) a.seg_count[seg] = 0;
     }
 }
 
@@ -1503,7 +1503,8 @@ __global__ void k_apply_finish(ApplyArgs a) {
     }
     if (a.nonfinite && __any(inf) && (threadIdx.x & 63) == 0) atomicOr(a.nonfinite, 1u);
     if (blockIdx.x == 0) {
-        for (uint32_t k = threadIdx.x; k < a.n_chunked; k += blockDim.x) a.seg_count[a.chunked[k]] = 0;
+        if (!a.keep_counts)
+            for (uint32_t k = threadIdx.x; k < a.n_chunked; k += blockDim.x) a.seg_count[a.chunked[k]] = 0;
         if (threadIdx.x == 0 && *a.bad) {
             a.grad[0] = h2v{(_Float16)__builtin_nanf(""), (_Float16)__builtin_nanf("")};
             if (a.nonfinite) atomicOr(a.nonfinite, 1u);

@@ -51,6 +51,7 @@ struct ApplyArgs {
     _Float16 __attribute__((ext_vector_type(2)))* grad;  // FP16 pair table gradient
     uint32_t* nonfinite;             // the optimizer's non-finite-gradient flag: set when a sum or a
                                      // gradient entry written is not finite
+    bool keep_counts = false;        // leave seg_count for a second apply of the same records
 };
 
 struct TrainArgs {
