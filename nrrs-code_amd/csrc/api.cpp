@@ -1259,7 +1259,12 @@ nrrs_status nrrs_rrs_infer_decide(nrrs_ctx* ctx, const nrrs_decide_args* a, void
     if (a->q_mode == NRRS_Q_RRSNET && ctx->cfg.net_set == NRRS_NETS_SHIPPED && !ctx->cfg.rrs_aid)
         return NRRS_ERR_UNSUPPORTED;
     hipStream_t s = (hipStream_t)stream;
-    {
+    const bool chain = a->q_mode == NRRS_Q_RRSNET_CHAINED && ctx->cfg.precision != NRRS_FP32 &&
+                       !ctx->cfg.ll2_two_head && ctx->grid_enc && a->n <= ctx->grid_enc_cap;
+    const bool ship_chain = chain && ctx->cfg.net_set == NRRS_NETS_SHIPPED;
+    const bool bench_chain = chain && ctx->cfg.net_set == NRRS_NETS_BENCH && ctx->grid_enc2;
+    // the chained paths' encode pass clears the sum; the others a launch of its own
+    if (!((ship_chain || bench_chain) && a->n)) {
         OpScope t(ctx, "zero_sum", s);
         NRRS_HIP(launch_zero_u64(ctx->sum_dev, s));
     }
@@ -1270,8 +1275,7 @@ nrrs_status nrrs_rrs_infer_decide(nrrs_ctx* ctx, const nrrs_decide_args* a, void
     if (a->q_mode == NRRS_Q_STATNET_ADRRS) {
         EncodeArgs ea = encode_args(ctx, a->v, a->film_dev, nullptr, 0);
         NRRS_HIP(run_net(ctx, NRRS_STATNET, ea, EPI_ADRRS, a->n, ep, s));
-    } else if (a->q_mode == NRRS_Q_RRSNET_CHAINED && ctx->cfg.precision != NRRS_FP32 && !ctx->cfg.ll2_two_head &&
-               ctx->cfg.net_set == NRRS_NETS_SHIPPED && ctx->grid_enc && a->n <= ctx->grid_enc_cap) {
+    } else if (ship_chain) {
         // shipped nets: StatNet's level-major grid encoding, then StatNet -> RRSNet (no grid) -> q
         // in one launch
         const nrrs_config& cfg = ctx->cfg;
@@ -1280,6 +1284,7 @@ nrrs_status nrrs_rrs_infer_decide(nrrs_ctx* ctx, const nrrs_decide_args* a, void
         ea.l0_chunks = l0_chunks(cfg, NRRS_STATNET);
         ea.grid_enc = ctx->grid_enc;
         ea.grid_enc_stride = ctx->grid_enc_cap;
+        ea.zero_u64 = ctx->sum_dev;
         const NetView ws = view_of(ctx, NRRS_STATNET), wr = view_of(ctx, NRRS_RRSNET);
         {
             OpScope t(ctx, "encode_statnet", s);
@@ -1287,14 +1292,14 @@ nrrs_status nrrs_rrs_infer_decide(nrrs_ctx* ctx, const nrrs_decide_args* a, void
         }
         OpScope t(ctx, "mlp_chained", s);
         NRRS_HIP(launch_net_chained(cfg.precision, ea, nullptr, a->n, ws.mlp_lp, wr.mlp_lp, ep, s));
-    } else if (a->q_mode == NRRS_Q_RRSNET_CHAINED && ctx->cfg.precision != NRRS_FP32 && !ctx->cfg.ll2_two_head &&
-               ctx->cfg.net_set == NRRS_NETS_BENCH && ctx->grid_enc && ctx->grid_enc2 && a->n <= ctx->grid_enc_cap) {
+    } else if (bench_chain) {
         // both nets' level-major grid encodings (one pass over the paired table, else a pass per
         // net), then StatNet -> RRSNet -> q in one launch
         const nrrs_config& cfg = ctx->cfg;
         EncodeArgs ea = encode_args(ctx, a->v, a->film_dev, nullptr, 0);
         ea.grid_enc = ctx->grid_enc;
         ea.grid_enc_stride = ctx->grid_enc_cap;
+        ea.zero_u64 = ctx->sum_dev;
         const NetView ws = view_of(ctx, NRRS_STATNET), wr = view_of(ctx, NRRS_RRSNET);
         if (paired_pass(ctx, a->n)) {
             OpScope t(ctx, "encode_pair", s);

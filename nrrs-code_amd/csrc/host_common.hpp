@@ -77,6 +77,9 @@ struct EncodeArgs {
     // layer-0 K chunks (input channels 16c..16c+15, bit c) that hold an input of the net; the
     // 32-wide shipped nets' kernels multiply only these
     uint32_t l0_chunks = 0x1fu;
+    // k_encode_levels only: one thread sets this word to 0 (the RRS stage's sum, which the net
+    // kernel launched next accumulates), in place of a launch of its own
+    unsigned long long* zero_u64 = nullptr;
 };
 
 struct DecideParams {

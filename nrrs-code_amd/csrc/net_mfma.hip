@@ -41,6 +41,7 @@ __global__ __launch_bounds__(kRunVert) void k_encode_levels(EncodeArgs a, const 
                                                             uint32_t* __restrict__ out2, uint32_t stride,
                                                             uint32_t* __restrict__ run_counts,
                                                             uint32_t* __restrict__ run_counts2) {
+    if (a.zero_u64 && blockIdx.x == 0 && threadIdx.x == 0) *a.zero_u64 = 0ull;
     const uint32_t l = blockIdx.x / n_chunks, chunk = blockIdx.x % n_chunks;
     if (l >= a.grid.n_levels) return;  // whole workgroup
     const uint32_t xb = threadIdx.x & 1u, k = threadIdx.x >> 1;
